@@ -1,0 +1,250 @@
+/*
+ * painty_hip.h -- C ABI of the gfx950 (MI355X) implementation of painty's
+ * paint-rendering hot path: Kubelka-Munk compositing of wet and dry paint
+ * layers over the canvas substrate, and imprint-brush stamping with
+ * volume-weighted K/S mixing, pickup and drying.
+ *
+ * The reference (lindemeier/painty) exposes this path as header-only C++
+ * templates instantiated with vec3 = Eigen::Matrix<double,3,1>; it has no
+ * C/FFI layer.  Each entry point below replaces one member or free function
+ * of those templates; the citation gives the reference file:line.  The C++
+ * classes in painty_amd/include/painty/ (Canvas, PaintLayer, FootprintBrush,
+ * Renderer, PaintMixer) are the drop-in binding over this ABI.
+ *
+ * Conventions
+ *   - Every function returns PNT_OK (0) or a negative status; the message of
+ *     the last failure on the calling thread is pnt_last_error().
+ *     PNT_EINVAL is where the reference throws std::invalid_argument.
+ *   - Handles are opaque, own their device memory, and are not thread-safe
+ *     (as the reference classes are not).  Work on one context is ordered on
+ *     that context's HIP stream.
+ *   - Host buffers belong to the caller and are only read/written during the
+ *     call.  Uploads are synchronous, downloads synchronise the stream.
+ *   - Host images are row-major.  layout PNT_AOS: K, S, R0 interleaved as
+ *     3 floats per pixel (the memory layout of the reference's Mat<vec3>);
+ *     PNT_PLANAR: three full planes one after another.  V and h are one float
+ *     per pixel.
+ *   - Device state is FP32 structure-of-arrays: a paint layer is 7 planes
+ *     (K0 K1 K2 S0 S1 S2 V), a canvas adds R0 (3 planes) and h (1 plane).
+ *   - Numeric domain: K >= 0, S >= 0, V >= 0 (the reference's own domain);
+ *     S within 1e4*DBL_EPSILON of 0 is clamped to 1e-11 as in the reference.
+ */
+#ifndef PAINTY_HIP_H
+#define PAINTY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PNT_OK 0
+#define PNT_EINVAL (-1)  /* invalid argument (reference: std::invalid_argument) */
+#define PNT_EHIP (-2)    /* HIP runtime failure */
+#define PNT_ENOMEM (-3)  /* device allocation failed / over the memory budget */
+#define PNT_ESTATE (-4)  /* call not valid in the object's current state */
+#define PNT_EKERNEL (-5) /* a kernel reported an error (bounded wait expired) */
+
+#define PNT_AOS 0
+#define PNT_PLANAR 1
+
+typedef struct pnt_ctx_s* pnt_ctx;
+typedef struct pnt_layer_s* pnt_layer;
+typedef struct pnt_canvas_s* pnt_canvas;
+typedef struct pnt_brush_s* pnt_brush;
+
+/* ------------------------------------------------------------ library --- */
+/* Message of the last failed call on this thread ("" if none). */
+const char* pnt_last_error(void);
+/* ABI version: major*10000 + minor*100 + patch. */
+int pnt_version(void);
+
+/* ------------------------------------------------------------ context --- */
+/* A device and one HIP stream (default priority) on it. */
+int pnt_ctx_create(int device, pnt_ctx* out);
+int pnt_ctx_destroy(pnt_ctx ctx);
+int pnt_ctx_synchronize(pnt_ctx ctx);
+/* The context's hipStream_t, for callers that order their own work with it. */
+void* pnt_ctx_stream(pnt_ctx ctx);
+
+/* ------------------------------------------- PaintLayer<vec3> ----------- */
+/* PaintLayer(rows, cols) -- renderer/PaintLayer.hxx:27-30 (zero-filled). */
+int pnt_layer_create(pnt_ctx ctx, int rows, int cols, pnt_layer* out);
+int pnt_layer_destroy(pnt_layer layer);
+/* getRows / getCols -- PaintLayer.hxx:56-62 */
+int pnt_layer_shape(pnt_layer layer, int* rows, int* cols);
+/* clear() -- PaintLayer.hxx:68-74 */
+int pnt_layer_clear(pnt_layer layer);
+/* Write / read the K, S, V buffers (getK_buffer/getS_buffer/getV_buffer,
+ * PaintLayer.hxx:32-54).  Any pointer may be NULL to skip that buffer. */
+int pnt_layer_upload(pnt_layer layer, const float* K, const float* S,
+                     const float* V, int layout);
+int pnt_layer_download(pnt_layer layer, float* K, float* S, float* V,
+                       int layout);
+/* copyTo(other) -- PaintLayer.hxx:103-111 (reallocates on size mismatch) */
+int pnt_layer_copy_to(pnt_layer src, pnt_layer dst);
+/* set(i, j, k, s, v) -- PaintLayer.hxx:122-127 */
+int pnt_layer_set(pnt_layer layer, int i, int j, const float k[3],
+                  const float s[3], float v);
+/* composeOnto(R0) -- PaintLayer.hxx:81-96, with R0 a host image (rows x cols
+ * x 3 in `layout`), updated in place.  When rows/cols differ from the
+ * layer's, R0 must have room for the layer's size and is first filled with
+ * 1.0 (PaintLayer.hxx:82-87); pass r0_rows/r0_cols = the current size. */
+int pnt_layer_compose_onto_host(pnt_layer layer, float* R0, int r0_rows,
+                                int r0_cols, int layout);
+/* composeOnto(canvas.getR0()) -- the dry layer baked over the substrate. */
+int pnt_layer_compose_onto_canvas(pnt_layer layer, pnt_canvas canvas);
+
+/* --------------------------------------------------- Canvas<vec3> ------- */
+/* Canvas(rows, cols) -- renderer/Canvas.hxx:26-35: background (1,1,1),
+ * wet layer zero, h zero. */
+int pnt_canvas_create(pnt_ctx ctx, int rows, int cols, pnt_canvas* out);
+int pnt_canvas_destroy(pnt_canvas canvas);
+int pnt_canvas_shape(pnt_canvas canvas, int* rows, int* cols);
+/* getPaintLayer() -- Canvas.hxx:89-95: a view on the canvas' wet layer,
+ * valid while the canvas lives; destroying the view does not free it. */
+int pnt_canvas_layer(pnt_canvas canvas, pnt_layer* out);
+/* clear() -- Canvas.hxx:37-58 */
+int pnt_canvas_clear(pnt_canvas canvas);
+/* setBackground(R0) -- Canvas.hxx:80-87 (clear(), then R0 = background) */
+int pnt_canvas_set_background(pnt_canvas canvas, const float* R0, int layout);
+/* getR0() / get_h() writes and reads -- Canvas.hxx:60-74 */
+int pnt_canvas_upload_R0(pnt_canvas canvas, const float* R0, int layout);
+int pnt_canvas_download_R0(pnt_canvas canvas, float* R0, int layout);
+int pnt_canvas_upload_h(pnt_canvas canvas, const float* h);
+int pnt_canvas_download_h(pnt_canvas canvas, float* h);
+/* dryCanvas() -- Canvas.hxx:105-121 */
+int pnt_canvas_dry(pnt_canvas canvas);
+/* Partial drying, the GPU renderer's DryingStep
+ * (renderer/shaders/DryingStep.compute.glsl:56-77): V leaves up to
+ * `portion`, R0 takes the leaving thickness, K/S stay. */
+int pnt_canvas_dry_step(pnt_canvas canvas, float portion);
+/* get/setDryingTime -- Canvas.hxx:157-163.  Only 0 (explicit drying only)
+ * and values >= the reference default (15000000 ms, Canvas.hxx:32) are
+ * accepted: checkDry (Canvas.hxx:123-155) changes no paint state then. */
+int pnt_canvas_set_drying_time_ms(pnt_canvas canvas, int64_t ms);
+int pnt_canvas_get_drying_time_ms(pnt_canvas canvas, int64_t* ms);
+
+/* ------------------------------------------------- Renderer<vec3> ------- */
+/* compose(canvas) -- renderer/Renderer.hxx:48-53: R = KM(wet layer over R0),
+ * out of place, into a host image. */
+int pnt_compose(pnt_canvas canvas, float* R, int layout);
+/* compose(paintLayer, R0_buffer) -- Renderer.hxx:26-41 with host images. */
+int pnt_compose_layer_host(pnt_layer layer, const float* R0, float* R,
+                           int layout);
+
+/* ------------------------------------- FootprintBrush<vec3> ------------- */
+/* FootprintBrush(radius) -- renderer/FootprintBrush.hxx:29-63.  The
+ * reference loads ./data/footprint/footprint.png and resizes it to
+ * width = 2*ceil(radius)+1; here the width x width footprint (values in
+ * [0,1]) is passed in.  It is padded with zeros to sizeMap =
+ * ceil(sqrt(2)*width) on each side (the reference pads one short when
+ * sizeMap-width is odd and reads past its buffer; that row/col reads 0). */
+int pnt_brush_create(pnt_ctx ctx, double radius, const float* footprint,
+                     int width, pnt_brush* out);
+int pnt_brush_destroy(pnt_brush brush);
+/* setRadius(radius) -- FootprintBrush.hxx:46-63: new footprint, pickup map
+ * re-created and cleared.  Like the reference, nothing happens when
+ * |radius - current| < 0.5 (fuzzyCompare with Eps 0.5). */
+int pnt_brush_set_radius(pnt_brush brush, double radius, const float* footprint,
+                         int width);
+int pnt_brush_size_map(pnt_brush brush, int* size_map);
+/* getFootprint() -- FootprintBrush.hxx:178-180 (sizeMap x sizeMap) */
+int pnt_brush_get_footprint(pnt_brush brush, float* out);
+/* getPickupMap() -- FootprintBrush.hxx:174-176 (sizeMap x sizeMap layer) */
+int pnt_brush_get_pickup_map(pnt_brush brush, float* K, float* S, float* V,
+                             int layout);
+/* set/getPickupRate, set/getDepositionRate, set/getUseSnapshotBuffer --
+ * FootprintBrush.hxx:182-204 (defaults 0.9, 0.05, true: :477-495) */
+int pnt_brush_set_pickup_rate(pnt_brush brush, float rate);
+int pnt_brush_get_pickup_rate(pnt_brush brush, float* rate);
+int pnt_brush_set_deposition_rate(pnt_brush brush, float rate);
+int pnt_brush_get_deposition_rate(pnt_brush brush, float* rate);
+int pnt_brush_set_use_snapshot(pnt_brush brush, int use);
+int pnt_brush_get_use_snapshot(pnt_brush brush, int* use);
+/* dip(paint) -- FootprintBrush.hxx:150-154: clean(), then paint = {K, S} */
+int pnt_brush_dip(pnt_brush brush, const float K[3], const float S[3]);
+/* clean() -- FootprintBrush.hxx:160-166 */
+int pnt_brush_clean(pnt_brush brush);
+/* updateSnapshot(canvas) -- FootprintBrush.hxx:168-172 */
+int pnt_brush_update_snapshot(pnt_brush brush, pnt_canvas canvas);
+/* imprint(center, theta, canvas) -- FootprintBrush.hxx:73-143 */
+int pnt_imprint(pnt_brush brush, pnt_canvas canvas, double cx, double cy,
+                double theta);
+
+/* An ordered stamp list: the GUI's dip / imprint / dryCanvas sequence
+ * (apps/painty_gui/DigitalCanvas.cxx:49-123) as one call.  Each stamp is
+ * imprint(center=(cx,cy), theta); PNT_STAMP_DIP dips the brush in (K, S)
+ * first, PNT_STAMP_DRY_AFTER runs dryCanvas() after it.  The result is the
+ * same as issuing the calls one by one. */
+typedef struct {
+  double cx, cy, theta;
+  float K[3], S[3];
+  uint32_t flags;
+  int32_t reserved;
+} pnt_stamp;
+#define PNT_STAMP_DIP 1u
+#define PNT_STAMP_DRY_AFTER 2u
+/* mode: PNT_EXEC_PLAIN issues each stamp as its own kernel launches;
+ * PNT_EXEC_PERSISTENT runs the whole list in one persistent launch. */
+#define PNT_EXEC_PLAIN 0
+#define PNT_EXEC_PERSISTENT 1
+int pnt_imprint_stamps(pnt_brush brush, pnt_canvas canvas,
+                       const pnt_stamp* stamps, int64_t n, int mode);
+
+/* paintStroke(path, canvas) -- FootprintBrush.hxx:206-268: Catmull-Rom
+ * placement at 1 px chord steps (core/Spline.hxx:27-73), one imprint per
+ * step.  path_xy holds n_points (x, y) pairs.  The reference reads
+ * path[ULONG_MAX] for the first segment (:252); its intent, p_pre = path[0],
+ * is used. */
+int pnt_paint_stroke(pnt_brush brush, pnt_canvas canvas, const double* path_xy,
+                     int64_t n_points);
+
+/* --------------------------------------------------------- PaintMixer --- */
+/* mixSinglePaint(weights) -- mixer/src/PaintMixer.cxx:327-355: K = sum_l
+ * norm*w_l*K_l (same for S), norm = 1/sum(w) unless |sum(w)-1| < 1e-5.
+ * base_K / base_S: n_palette x 3; PNT_EINVAL when n_weights != n_palette. */
+int pnt_mix_single(int n_palette, const double* base_K, const double* base_S,
+                   int n_weights, const double* weights, double K[3],
+                   double S[3]);
+/* mixed(paint, vol, other, vol) -- PaintMixer.cxx:536-548 */
+int pnt_mixed(const double K1[3], const double S1[3], double v1,
+              const double K2[3], const double S2[3], double v2, double K[3],
+              double S[3]);
+
+/* ------------------------------------------ KubelkaMunk free functions -- */
+/* ComputeReflectance(K, S, R0, d) over n pixels of host arrays (each K, S,
+ * R0, R n x 3 AoS, d n floats) -- core/KubelkaMunk.hxx:25-83 on the device. */
+int pnt_compute_reflectance(pnt_ctx ctx, int64_t n, const float* K,
+                            const float* S, const float* R0, const float* d,
+                            float* R);
+
+/* ---------------------------------------------- device-pointer kernels -- */
+/* The same kernels on caller-owned device planes (e.g. torch tensors), on
+ * `stream` (a hipStream_t; NULL = the context's stream).  A layer is 7
+ * planes of n floats: K0 K1 K2 S0 S1 S2 V; rgb is 3 planes.  Planes are
+ * float pointers; 16-byte aligned planes use the vectorised path. */
+typedef struct {
+  float* p[7];
+} pnt_dev_layer;
+typedef struct {
+  float* p[3];
+} pnt_dev_rgb;
+/* R = KM(layer over R0); R may equal R0 (composeOnto). */
+int pnt_dev_compose(pnt_ctx ctx, int64_t n, pnt_dev_layer layer,
+                    pnt_dev_rgb R0, pnt_dev_rgb R, void* stream);
+/* R = KM(wet over KM(dry over R0)); R0 is not modified. */
+int pnt_dev_compose2(pnt_ctx ctx, int64_t n, pnt_dev_layer dry,
+                     pnt_dev_layer wet, pnt_dev_rgb R0, pnt_dev_rgb R,
+                     void* stream);
+
+/* ---------------------------------------------------- canvas device view - */
+/* Device planes of a canvas (valid while it lives): layer (7), R0 (3), h. */
+int pnt_canvas_device_planes(pnt_canvas canvas, pnt_dev_layer* layer,
+                             pnt_dev_rgb* R0, float** h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PAINTY_HIP_H */

@@ -1,0 +1,297 @@
+"""ctypes view of the FP64 CPU oracle (``liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` as the checker / the timed CPU
+baseline.  The product (``painty_amd``) never imports this module.
+
+Arrays follow the reference's ``cv::Mat_`` storage: K, S, R0 are ``(rows,
+cols, 3)`` float64 (AoS ``vec3``), V and h are ``(rows, cols)`` float64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+class Stamp(C.Structure):
+    """orc_stamp: one imprint with optional dip before and dry after."""
+
+    _fields_ = [("cx", C.c_double), ("cy", C.c_double), ("theta", C.c_double),
+                ("K", C.c_double * 3), ("S", C.c_double * 3),
+                ("flags", C.c_uint32)]
+
+
+DIP = 1
+DRY_AFTER = 2
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_coth.restype = C.c_double
+        L.orc_coth.argtypes = [C.c_double]
+        L.orc_acoth.restype = C.c_double
+        L.orc_acoth.argtypes = [C.c_double]
+        L.orc_compute_reflectance.argtypes = [_dp, _dp, _dp, C.c_double, _dp]
+        L.orc_compute_reflectance_glsl.argtypes = [_dp, _dp, _dp, C.c_double, _dp]
+        L.orc_compute_scattering_absorption.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_compute_scattering_absorption.restype = C.c_int
+        L.orc_compose.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp]
+        L.orc_compose_onto.argtypes = [C.c_int64, _dp, _dp, _dp, _dp]
+        L.orc_dry_canvas.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp]
+        L.orc_dry_step.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, C.c_double]
+        L.orc_brush_create.restype = C.c_void_p
+        L.orc_brush_create.argtypes = [C.c_double, _dp, C.c_int]
+        L.orc_brush_destroy.argtypes = [C.c_void_p]
+        L.orc_brush_set_radius.argtypes = [C.c_void_p, C.c_double, _dp, C.c_int]
+        L.orc_brush_set_radius.restype = C.c_int
+        L.orc_brush_size_map.argtypes = [C.c_void_p]
+        L.orc_brush_size_map.restype = C.c_int
+        L.orc_brush_footprint.argtypes = [C.c_void_p]
+        L.orc_brush_footprint.restype = _dp
+        L.orc_brush_set_rates.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_brush_set_use_snapshot.argtypes = [C.c_void_p, C.c_int]
+        L.orc_brush_dip.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_brush_clean.argtypes = [C.c_void_p]
+        L.orc_brush_update_snapshot.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_brush_get_pickup.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_brush_imprint.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp,
+                                        C.c_double, C.c_double, C.c_double]
+        L.orc_run_stamps.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
+                                     C.POINTER(Stamp), C.c_int64]
+        L.orc_mix_single.restype = C.c_int
+        L.orc_mix_single.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
+        L.orc_mixed.argtypes = [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp]
+        L.orc_catmull_rom.restype = C.c_double
+        L.orc_catmull_rom.argtypes = [C.c_double] * 5
+        L.orc_catmull_rom_d1.restype = C.c_double
+        L.orc_catmull_rom_d1.argtypes = [C.c_double] * 5
+        L.orc_stroke_stamps.restype = C.c_int64
+        L.orc_stroke_stamps.argtypes = [_dp, C.c_int64, _dp, C.c_int64]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- KM -------
+def coth(x: float) -> float:
+    return lib().orc_coth(float(x))
+
+
+def acoth(x: float) -> float:
+    return lib().orc_acoth(float(x))
+
+
+def compute_reflectance(K, S, R0, d) -> np.ndarray:
+    K, S, R0 = _f64(K), _f64(S), _f64(R0)
+    R = np.empty(3)
+    lib().orc_compute_reflectance(_p(K), _p(S), _p(R0), float(d), _p(R))
+    return R
+
+
+def compute_reflectance_glsl(K, S, R0, d) -> np.ndarray:
+    K, S, R0 = _f64(K), _f64(S), _f64(R0)
+    R = np.empty(3)
+    lib().orc_compute_reflectance_glsl(_p(K), _p(S), _p(R0), float(d), _p(R))
+    return R
+
+
+def compute_scattering_absorption(Rb, Rw):
+    Rb, Rw = _f64(Rb), _f64(Rw)
+    K, S = np.empty(3), np.empty(3)
+    if lib().orc_compute_scattering_absorption(_p(Rb), _p(Rw), _p(K), _p(S)) != 0:
+        raise ValueError("on black or white inputs violate one of the following "
+                         "conditions: 0 < black < white < 1")
+    return K, S
+
+
+def compose(K, S, V, R0) -> np.ndarray:
+    K, S, V, R0 = _f64(K), _f64(S), _f64(V), _f64(R0)
+    n = V.size
+    assert K.size == S.size == R0.size == 3 * n
+    R = np.empty_like(R0)
+    lib().orc_compose(n, _p(K), _p(S), _p(V), _p(R0), _p(R))
+    return R
+
+
+def compose_onto(K, S, V, R0: np.ndarray) -> None:
+    """In place on R0 (float64, C-contiguous)."""
+    K, S, V = _f64(K), _f64(S), _f64(V)
+    lib().orc_compose_onto(V.size, _p(K), _p(S), _p(V), _p(R0))
+
+
+def dry_canvas(K, S, V, R0, h) -> None:
+    """In place on all five arrays (float64, C-contiguous)."""
+    lib().orc_dry_canvas(V.size, _p(K), _p(S), _p(V), _p(R0), _p(h))
+
+
+def dry_step(K, S, V, R0, portion: float) -> None:
+    lib().orc_dry_step(V.size, _p(_f64(K)), _p(_f64(S)), _p(V), _p(R0), float(portion))
+
+
+# ------------------------------------------------------------- brush -------
+class Brush:
+    """FootprintBrush with an injected footprint core (width x width)."""
+
+    def __init__(self, radius: float, core: np.ndarray):
+        core = _f64(core)
+        self._core = core
+        self.h = lib().orc_brush_create(float(radius), _p(core), int(core.shape[0]))
+        if not self.h:
+            raise ValueError("footprint core width must be 2*ceil(radius)+1")
+        self.radius = radius
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_brush_destroy(self.h)
+            self.h = None
+
+    def set_radius(self, radius: float, core: np.ndarray) -> None:
+        core = _f64(core)
+        if lib().orc_brush_set_radius(self.h, float(radius), _p(core), int(core.shape[0])) != 0:
+            raise ValueError("footprint core width must be 2*ceil(radius)+1")
+        self._core = core
+        self.radius = radius
+
+    @property
+    def size_map(self) -> int:
+        return lib().orc_brush_size_map(self.h)
+
+    def footprint(self) -> np.ndarray:
+        m = self.size_map
+        ptr = lib().orc_brush_footprint(self.h)
+        return np.ctypeslib.as_array(ptr, shape=(m, m)).copy()
+
+    def set_rates(self, pickup: float, deposition: float) -> None:
+        lib().orc_brush_set_rates(self.h, pickup, deposition)
+
+    def set_use_snapshot(self, use: bool) -> None:
+        lib().orc_brush_set_use_snapshot(self.h, int(bool(use)))
+
+    def dip(self, K, S) -> None:
+        lib().orc_brush_dip(self.h, _p(_f64(K)), _p(_f64(S)))
+
+    def clean(self) -> None:
+        lib().orc_brush_clean(self.h)
+
+    def update_snapshot(self, canvas: "Canvas") -> None:
+        lib().orc_brush_update_snapshot(self.h, canvas.rows, canvas.cols,
+                                        _p(canvas.K), _p(canvas.S), _p(canvas.V))
+
+    def pickup_map(self):
+        m = self.size_map
+        K, S, V = np.empty((m, m, 3)), np.empty((m, m, 3)), np.empty((m, m))
+        lib().orc_brush_get_pickup(self.h, _p(K), _p(S), _p(V))
+        return K, S, V
+
+    def imprint(self, cx: float, cy: float, theta: float, canvas: "Canvas") -> None:
+        lib().orc_brush_imprint(self.h, canvas.rows, canvas.cols, _p(canvas.K),
+                                _p(canvas.S), _p(canvas.V), float(cx), float(cy),
+                                float(theta))
+
+    def run(self, canvas: "Canvas", stamps) -> None:
+        arr = stamps_to_c(stamps)
+        lib().orc_run_stamps(self.h, canvas.rows, canvas.cols, _p(canvas.K), _p(canvas.S),
+                             _p(canvas.V), _p(canvas.R0), _p(canvas.h), arr, len(arr))
+
+
+def stamps_to_c(stamps):
+    """stamps: structured numpy array with fields cx, cy, theta, K(3), S(3), flags."""
+    n = len(stamps)
+    arr = (Stamp * n)()
+    for i in range(n):
+        s = stamps[i]
+        arr[i].cx, arr[i].cy, arr[i].theta = float(s["cx"]), float(s["cy"]), float(s["theta"])
+        for c in range(3):
+            arr[i].K[c] = float(s["K"][c])
+            arr[i].S[c] = float(s["S"][c])
+        arr[i].flags = int(s["flags"])
+    return arr
+
+
+class Canvas:
+    """Canvas state in float64 (K, S, V wet layer, R0 substrate, h height)."""
+
+    def __init__(self, rows: int, cols: int):
+        self.rows, self.cols = rows, cols
+        self.K = np.zeros((rows, cols, 3))
+        self.S = np.zeros((rows, cols, 3))
+        self.V = np.zeros((rows, cols))
+        self.R0 = np.ones((rows, cols, 3))
+        self.h = np.zeros((rows, cols))
+
+    @classmethod
+    def from_arrays(cls, K, S, V, R0, h=None) -> "Canvas":
+        rows, cols = V.shape
+        c = cls(rows, cols)
+        c.K[...] = K
+        c.S[...] = S
+        c.V[...] = V
+        c.R0[...] = R0
+        if h is not None:
+            c.h[...] = h
+        return c
+
+    def compose(self) -> np.ndarray:
+        return compose(self.K, self.S, self.V, self.R0)
+
+    def dry(self) -> None:
+        dry_canvas(self.K, self.S, self.V, self.R0, self.h)
+
+
+# ------------------------------------------------------------- mixer -------
+def mix_single(baseK, baseS, weights):
+    baseK, baseS, w = _f64(baseK), _f64(baseS), _f64(weights)
+    K, S = np.empty(3), np.empty(3)
+    if lib().orc_mix_single(baseK.shape[0], _p(baseK), _p(baseS), w.size, _p(w),
+                            _p(K), _p(S)) != 0:
+        raise ValueError("Palette size does not match underlying size.")
+    return K, S
+
+
+def mixed(K1, S1, v1, K2, S2, v2):
+    K, S = np.empty(3), np.empty(3)
+    lib().orc_mixed(_p(_f64(K1)), _p(_f64(S1)), float(v1), _p(_f64(K2)), _p(_f64(S2)),
+                    float(v2), _p(K), _p(S))
+    return K, S
+
+
+# ------------------------------------------------------------ spline -------
+def catmull_rom(p_1, p0, p1, p2, t) -> float:
+    return lib().orc_catmull_rom(p_1, p0, p1, p2, t)
+
+
+def catmull_rom_d1(p_1, p0, p1, p2, t) -> float:
+    return lib().orc_catmull_rom_d1(p_1, p0, p1, p2, t)
+
+
+def stroke_stamps(path) -> np.ndarray:
+    path = _f64(path).reshape(-1, 2)
+    n = lib().orc_stroke_stamps(_p(path), len(path), None, 0)
+    out = np.empty((max(n, 1), 3))
+    lib().orc_stroke_stamps(_p(path), len(path), _p(out), n)
+    return out[:n]

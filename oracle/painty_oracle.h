@@ -1,0 +1,132 @@
+/*
+ * painty_oracle.h -- FP64 CPU restatement of lindemeier/painty's paint
+ * rendering hot path.  TEST INFRASTRUCTURE ONLY: this library is the checker
+ * that the HIP implementation is compared against (and the timed CPU
+ * baseline).  Nothing in the product (painty_amd/, include/) links or calls it.
+ *
+ * Layout conventions follow the reference's cv::Mat_ storage: row-major,
+ * continuous; K, S and R0 are AoS triples (3 doubles per pixel, like
+ * Mat<vec3>), V and h are one double per pixel.
+ *
+ * Every function cites the reference file:line it restates.  Where the
+ * reference has undefined behaviour the documented intent is implemented
+ * instead (noted at the function).
+ */
+#ifndef PAINTY_ORACLE_H
+#define PAINTY_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* painty/core/Math.hxx:157-169 */
+double orc_coth(double x);
+/* painty/core/Math.hxx:181-191 */
+double orc_acoth(double x);
+
+/* painty/core/KubelkaMunk.hxx:25-83 (N = 3, Float = double) */
+void orc_compute_reflectance(const double K[3], const double S_in[3],
+                             const double R0[3], double d, double R[3]);
+
+/* painty/core/KubelkaMunk.hxx:89-124; returns -1 where the reference throws
+ * std::invalid_argument, 0 otherwise. */
+int orc_compute_scattering_absorption(const double Rb[3], const double Rw[3],
+                                      double K[3], double S[3]);
+
+/* Same formula as the GLSL compute shaders use
+ * (painty/renderer/shaders/DryingStep.compute.glsl:31-49): no S clamp,
+ * returns r0 when any |b S d| < 1e-5. */
+void orc_compute_reflectance_glsl(const double K[3], const double S[3],
+                                  const double R0[3], double d, double R[3]);
+
+/* painty/renderer/Renderer.hxx:26-41 (out of place) */
+void orc_compose(int64_t n, const double* K, const double* S, const double* V,
+                 const double* R0, double* R);
+
+/* painty/renderer/PaintLayer.hxx:81-96 (in place onto R0) */
+void orc_compose_onto(int64_t n, const double* K, const double* S,
+                      const double* V, double* R0);
+
+/* painty/renderer/Canvas.hxx:105-121 */
+void orc_dry_canvas(int64_t n, double* K, double* S, double* V, double* R0,
+                    double* h);
+
+/* painty/renderer/shaders/DryingStep.compute.glsl:56-77 (partial dry) */
+void orc_dry_step(int64_t n, const double* K, const double* S, double* V,
+                  double* R0, double portion);
+
+/* ---- imprint ("footprint") brush: painty/renderer/FootprintBrush.hxx ---- */
+typedef struct orc_brush orc_brush;
+
+/* FootprintBrush(radius) + setRadius (FootprintBrush.hxx:29-63) with the
+ * footprint supplied directly as a width x width core (width =
+ * 2*ceil(radius)+1) instead of loading ./data/footprint/footprint.png.
+ * The core is padded with zeros to exactly sizeMap x sizeMap (the reference
+ * pads to width+2*pad, one short when sizeMap-width is odd, and then reads
+ * row/col sizeMap-1 out of bounds; here that row/col reads 0). */
+orc_brush* orc_brush_create(double radius, const double* core, int width);
+void orc_brush_destroy(orc_brush* b);
+/* setRadius (FootprintBrush.hxx:46-63): when |radius - current| >= 0.5, a new
+ * footprint and a new, cleared pickup map; the snapshot buffer is kept.
+ * Returns -1 when width != 2*ceil(radius)+1. */
+int orc_brush_set_radius(orc_brush* b, double radius, const double* core,
+                         int width);
+int orc_brush_size_map(const orc_brush* b);
+const double* orc_brush_footprint(const orc_brush* b); /* sizeMap^2 */
+void orc_brush_set_rates(orc_brush* b, double pickup, double deposition);
+void orc_brush_set_use_snapshot(orc_brush* b, int use);
+/* FootprintBrush.hxx:150-154 */
+void orc_brush_dip(orc_brush* b, const double K[3], const double S[3]);
+/* FootprintBrush.hxx:160-166 */
+void orc_brush_clean(orc_brush* b);
+/* FootprintBrush.hxx:168-172 (full canvas -> snapshot copy) */
+void orc_brush_update_snapshot(orc_brush* b, int rows, int cols,
+                               const double* K, const double* S,
+                               const double* V);
+/* FootprintBrush.hxx:174-176 (pickup map, sizeMap^2 AoS / scalar) */
+void orc_brush_get_pickup(const orc_brush* b, double* K, double* S, double* V);
+/* FootprintBrush.hxx:73-143 with updateSnapshot(canvas, center) :278-319,
+ * pickupPaint :349-384, depositPaint :393-431, blend :331-340.
+ * Canvas::checkDry (:115) is a no-op on K/S/V for drying time 0 and for any
+ * run shorter than the default drying time (Canvas.hxx:32,126). */
+void orc_brush_imprint(orc_brush* b, int rows, int cols, double* K, double* S,
+                       double* V, double cx, double cy, double theta);
+
+/* A stamp list driver (the GUI calls dip / imprint / dryCanvas in sequence,
+ * apps/painty_gui/DigitalCanvas.cxx:49-123). */
+typedef struct {
+  double cx, cy, theta;
+  double K[3], S[3]; /* paint for ORC_DIP */
+  uint32_t flags;
+} orc_stamp;
+enum { ORC_DIP = 1u, ORC_DRY_AFTER = 2u };
+void orc_run_stamps(orc_brush* b, int rows, int cols, double* K, double* S,
+                    double* V, double* R0, double* h, const orc_stamp* stamps,
+                    int64_t n);
+
+/* ---- mixer: painty/mixer/src/PaintMixer.cxx ---- */
+/* :327-355; returns -1 (reference: std::invalid_argument) on size mismatch */
+int orc_mix_single(int n_palette, const double* baseK, const double* baseS,
+                   int n_weights, const double* w, double K[3], double S[3]);
+/* :536-548 */
+void orc_mixed(const double K1[3], const double S1[3], double v1,
+               const double K2[3], const double S2[3], double v2, double K[3],
+               double S[3]);
+
+/* ---- stroke placement: painty/core/Spline.hxx:27-73 and
+ * FootprintBrush::paintStroke (FootprintBrush.hxx:251-267) ---- */
+double orc_catmull_rom(double p_1, double p0, double p1, double p2, double t);
+double orc_catmull_rom_d1(double p_1, double p0, double p1, double p2,
+                          double t);
+/* Writes up to max_out (cx, cy, theta) triples; returns the count the stroke
+ * produces (may exceed max_out).  The reference reads path[ULONG_MAX] at
+ * i = 0 (:252, max(i-1UL, 0UL) wraps); the intent, p_pre = path[0], is used. */
+int64_t orc_stroke_stamps(const double* path_xy, int64_t n_points,
+                          double* out, int64_t max_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

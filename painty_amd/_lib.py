@@ -1,0 +1,139 @@
+"""ctypes binding of include/painty_hip.h (libpainty_hip.so, built in-tree).
+
+There is no fallback: importing a compute entry point without the library
+raises, and every call that returns a non-zero status raises PaintyError with
+the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpainty_hip.so")
+
+OK, EINVAL, EHIP, ENOMEM, ESTATE, EKERNEL = 0, -1, -2, -3, -4, -5
+AOS, PLANAR = 0, 1
+STAMP_DIP, STAMP_DRY_AFTER = 1, 2
+EXEC_PLAIN, EXEC_PERSISTENT = 0, 1
+
+
+class PaintyError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("painty_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+class InvalidArgument(PaintyError, ValueError):
+    """Where the reference throws std::invalid_argument."""
+
+
+class PntStamp(C.Structure):
+    _fields_ = [("cx", C.c_double), ("cy", C.c_double), ("theta", C.c_double),
+                ("K", C.c_float * 3), ("S", C.c_float * 3), ("flags", C.c_uint32),
+                ("reserved", C.c_int32)]
+
+
+class DevLayer(C.Structure):
+    _fields_ = [("p", C.c_void_p * 7)]
+
+
+class DevRgb(C.Structure):
+    _fields_ = [("p", C.c_void_p * 3)]
+
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    "pnt_last_error": [],
+    "pnt_version": [],
+    "pnt_ctx_create": [C.c_int, C.POINTER(_vp)],
+    "pnt_ctx_destroy": [_vp],
+    "pnt_ctx_synchronize": [_vp],
+    "pnt_ctx_stream": [_vp],
+    "pnt_layer_create": [_vp, C.c_int, C.c_int, C.POINTER(_vp)],
+    "pnt_layer_destroy": [_vp],
+    "pnt_layer_shape": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "pnt_layer_clear": [_vp],
+    "pnt_layer_upload": [_vp, _fp, _fp, _fp, C.c_int],
+    "pnt_layer_download": [_vp, _fp, _fp, _fp, C.c_int],
+    "pnt_layer_copy_to": [_vp, _vp],
+    "pnt_layer_set": [_vp, C.c_int, C.c_int, _fp, _fp, C.c_float],
+    "pnt_layer_compose_onto_host": [_vp, _fp, C.c_int, C.c_int, C.c_int],
+    "pnt_layer_compose_onto_canvas": [_vp, _vp],
+    "pnt_canvas_create": [_vp, C.c_int, C.c_int, C.POINTER(_vp)],
+    "pnt_canvas_destroy": [_vp],
+    "pnt_canvas_shape": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "pnt_canvas_layer": [_vp, C.POINTER(_vp)],
+    "pnt_canvas_clear": [_vp],
+    "pnt_canvas_set_background": [_vp, _fp, C.c_int],
+    "pnt_canvas_upload_R0": [_vp, _fp, C.c_int],
+    "pnt_canvas_download_R0": [_vp, _fp, C.c_int],
+    "pnt_canvas_upload_h": [_vp, _fp],
+    "pnt_canvas_download_h": [_vp, _fp],
+    "pnt_canvas_dry": [_vp],
+    "pnt_canvas_dry_step": [_vp, C.c_float],
+    "pnt_canvas_set_drying_time_ms": [_vp, C.c_int64],
+    "pnt_canvas_get_drying_time_ms": [_vp, C.POINTER(C.c_int64)],
+    "pnt_compose": [_vp, _fp, C.c_int],
+    "pnt_compose_layer_host": [_vp, _fp, _fp, C.c_int],
+    "pnt_brush_create": [_vp, C.c_double, _fp, C.c_int, C.POINTER(_vp)],
+    "pnt_brush_destroy": [_vp],
+    "pnt_brush_set_radius": [_vp, C.c_double, _fp, C.c_int],
+    "pnt_brush_size_map": [_vp, C.POINTER(C.c_int)],
+    "pnt_brush_get_footprint": [_vp, _fp],
+    "pnt_brush_get_pickup_map": [_vp, _fp, _fp, _fp, C.c_int],
+    "pnt_brush_set_pickup_rate": [_vp, C.c_float],
+    "pnt_brush_get_pickup_rate": [_vp, _fp],
+    "pnt_brush_set_deposition_rate": [_vp, C.c_float],
+    "pnt_brush_get_deposition_rate": [_vp, _fp],
+    "pnt_brush_set_use_snapshot": [_vp, C.c_int],
+    "pnt_brush_get_use_snapshot": [_vp, C.POINTER(C.c_int)],
+    "pnt_brush_dip": [_vp, _fp, _fp],
+    "pnt_brush_clean": [_vp],
+    "pnt_brush_update_snapshot": [_vp, _vp],
+    "pnt_imprint": [_vp, _vp, C.c_double, C.c_double, C.c_double],
+    "pnt_imprint_stamps": [_vp, _vp, C.POINTER(PntStamp), C.c_int64, C.c_int],
+    "pnt_paint_stroke": [_vp, _vp, _dp, C.c_int64],
+    "pnt_mix_single": [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp],
+    "pnt_mixed": [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp],
+    "pnt_compute_reflectance": [_vp, C.c_int64, _fp, _fp, _fp, _fp, _fp],
+    "pnt_dev_compose": [_vp, C.c_int64, DevLayer, DevRgb, DevRgb, _vp],
+    "pnt_dev_compose2": [_vp, C.c_int64, DevLayer, DevLayer, DevRgb, DevRgb, _vp],
+    "pnt_canvas_device_planes": [_vp, C.POINTER(DevLayer), C.POINTER(DevRgb),
+                                 C.POINTER(_vp)],
+}
+_RESTYPE = {"pnt_last_error": C.c_char_p, "pnt_ctx_stream": _vp}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libpainty_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("painty_amd: %s is missing -- build it with "
+                              "`python -m painty_amd.build` (no CPU fallback exists)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != OK:
+        msg = load().pnt_last_error().decode(errors="replace")
+        if status == EINVAL:
+            raise InvalidArgument(status, msg)
+        raise PaintyError(status, msg)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))

@@ -1,0 +1,1112 @@
+// C ABI of painty_hip.h: handles, host<->device transfer, stamp planning.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "internal.hpp"
+#include "persistent.hpp"
+
+using namespace pnt;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return PNT_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return PNT_ENOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return PNT_EINVAL;
+  }
+}
+
+#define PNT_REQUIRE(cond, msg)                       \
+  do {                                               \
+    if (!(cond)) throw ::pnt::Error(PNT_EINVAL, msg); \
+  } while (0)
+
+inline int64_t round_up(int64_t n, int64_t a) { return (n + a - 1) / a * a; }
+
+float* dev_alloc(int64_t nfloats) {
+  void* p = nullptr;
+  if (nfloats <= 0) nfloats = 1;
+  const hipError_t e = hipMalloc(&p, (size_t)nfloats * sizeof(float));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(PNT_ENOMEM, std::string("hipMalloc of ") + std::to_string(nfloats * 4) +
+                                " bytes failed: " + hipGetErrorString(e));
+  }
+  return static_cast<float*>(p);
+}
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// Host AoS/planar <-> device planes.
+void upload_planes(float* const* dst, int nplanes, const float* src, int64_t n, int layout,
+                   hipStream_t s) {
+  if (nplanes == 1 || layout == PNT_PLANAR) {
+    for (int c = 0; c < nplanes; ++c)
+      PNT_HIP_CHECK(hipMemcpyAsync(dst[c], src + c * n, n * sizeof(float), hipMemcpyHostToDevice, s));
+  } else {
+    PNT_REQUIRE(layout == PNT_AOS, "layout must be PNT_AOS or PNT_PLANAR");
+    std::vector<float> tmp((size_t)(n * nplanes));
+    for (int64_t i = 0; i < n; ++i)
+      for (int c = 0; c < nplanes; ++c) tmp[(size_t)(c * n + i)] = src[i * nplanes + c];
+    for (int c = 0; c < nplanes; ++c)
+      PNT_HIP_CHECK(hipMemcpyAsync(dst[c], tmp.data() + c * n, n * sizeof(float),
+                                   hipMemcpyHostToDevice, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    return;
+  }
+  PNT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void download_planes(float* dst, int nplanes, float* const* src, int64_t n, int layout,
+                     hipStream_t s) {
+  if (nplanes == 1 || layout == PNT_PLANAR) {
+    for (int c = 0; c < nplanes; ++c)
+      PNT_HIP_CHECK(hipMemcpyAsync(dst + c * n, src[c], n * sizeof(float), hipMemcpyDeviceToHost, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    PNT_REQUIRE(layout == PNT_AOS, "layout must be PNT_AOS or PNT_PLANAR");
+    std::vector<float> tmp((size_t)(n * nplanes));
+    for (int c = 0; c < nplanes; ++c)
+      PNT_HIP_CHECK(hipMemcpyAsync(tmp.data() + c * n, src[c], n * sizeof(float),
+                                   hipMemcpyDeviceToHost, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; ++i)
+      for (int c = 0; c < nplanes; ++c) dst[i * nplanes + c] = tmp[(size_t)(c * n + i)];
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ handles
+struct pnt_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void use() const { PNT_HIP_CHECK(hipSetDevice(device)); }
+};
+
+struct pnt_layer_s {
+  pnt_ctx ctx = nullptr;
+  int rows = 0, cols = 0;
+  int64_t n = 0, stride = 0;
+  float* base = nullptr;
+  bool owner = true;
+  Layer7 planes() const {
+    Layer7 L;
+    for (int i = 0; i < 7; ++i) L.p[i] = base + i * stride;
+    return L;
+  }
+  void alloc(int r, int c) {
+    rows = r;
+    cols = c;
+    n = (int64_t)r * c;
+    stride = round_up(std::max<int64_t>(n, 1), 64);
+    base = dev_alloc(7 * stride);
+  }
+};
+
+struct pnt_canvas_s {
+  pnt_ctx ctx = nullptr;
+  int rows = 0, cols = 0;
+  int64_t n = 0, stride = 0;
+  float* base = nullptr;     // planes 0..6 wet layer, 7..9 R0, 10 h
+  float* scratch = nullptr;  // 3 planes for host-side compose results
+  pnt_layer_s view;
+  int64_t drying_ms = 15000000;  // Canvas.hxx:32
+  float bg[3] = {1.f, 1.f, 1.f};
+  Layer7 layer() const { return view.planes(); }
+  Rgb3 R0() const { return Rgb3{{base + 7 * stride, base + 8 * stride, base + 9 * stride}}; }
+  float* h() const { return base + 10 * stride; }
+};
+
+struct pnt_brush_s {
+  pnt_ctx ctx = nullptr;
+  double radius = 0.0;
+  int width = 0, size_map = 0;
+  int64_t m = 0, mstride = 0;
+  float* fp = nullptr;    // footprint
+  float* pick = nullptr;  // 7 planes of mstride
+  float* snap = nullptr;  // 7 planes of snap_stride
+  int snap_rows = 0, snap_cols = 0;
+  int64_t snap_stride = 0;
+  bool use_snap = true;
+  float pickup_rate = 0.9f, dep_rate = 0.05f, cap = 1.0f;
+  float paintK[3] = {0, 0, 0}, paintS[3] = {0, 0, 0};
+  // dependency lists of the current stamp's shared-pixel visits
+  int2* d_list = nullptr;
+  int* d_off = nullptr;
+  int64_t d_cap = 0;
+  PersistentState* pers = nullptr;
+
+  Layer7 pick_planes() const {
+    Layer7 L;
+    for (int i = 0; i < 7; ++i) L.p[i] = pick + i * mstride;
+    return L;
+  }
+  Layer7 snap_planes() const {
+    Layer7 L;
+    for (int i = 0; i < 7; ++i) L.p[i] = snap + i * snap_stride;
+    return L;
+  }
+  BrushArgs args() const {
+    BrushArgs b;
+    b.fp = fp;
+    b.pick = pick_planes();
+    b.pickup_rate = pickup_rate;
+    b.dep_rate = dep_rate;
+    b.cap = cap;
+    for (int c = 0; c < 3; ++c) {
+      b.paintK[c] = paintK[c];
+      b.paintS[c] = paintS[c];
+    }
+    return b;
+  }
+};
+
+namespace pnt {
+
+// FootprintBrush::setRadius geometry (FootprintBrush.hxx:52-61) with an
+// injected width x width footprint, padded with zeros to size_map^2.
+void brush_set_geometry(pnt_brush b, double radius, const float* core, int width) {
+  PNT_REQUIRE(core != nullptr, "footprint is NULL");
+  PNT_REQUIRE(std::isfinite(radius) && radius > 0.0 && radius < 4096.0,
+              "radius must be in (0, 4096)");
+  const int w = (int)(2.0 * std::ceil(radius) + 1.0);
+  PNT_REQUIRE(w == width, "footprint width must be 2*ceil(radius)+1");
+  for (int64_t i = 0; i < (int64_t)w * w; ++i)
+    PNT_REQUIRE(std::isfinite(core[i]) && core[i] >= 0.0f && core[i] <= 1.0f,
+                "footprint values must be in [0, 1]");
+  const int size_map = (int)std::ceil(std::sqrt(2.0) * w);
+  const int pad = (size_map - w) / 2;
+  std::vector<float> full((size_t)size_map * size_map, 0.0f);
+  for (int i = 0; i < w; ++i)
+    for (int j = 0; j < w; ++j) full[(size_t)(i + pad) * size_map + (j + pad)] = core[i * w + j];
+  const int64_t m = (int64_t)size_map * size_map;
+  const int64_t mstride = round_up(m, 64);
+  float* fp = dev_alloc(m);
+  float* pick = nullptr;
+  try {
+    pick = dev_alloc(7 * mstride);
+  } catch (...) {
+    dev_free(fp);
+    throw;
+  }
+  hipStream_t s = b->ctx->stream;
+  PNT_HIP_CHECK(hipMemcpyAsync(fp, full.data(), m * sizeof(float), hipMemcpyHostToDevice, s));
+  launch_fill(pick, 7 * mstride, 0.0f, s);
+  PNT_HIP_CHECK(hipStreamSynchronize(s));
+  dev_free(b->fp);
+  dev_free(b->pick);
+  b->fp = fp;
+  b->pick = pick;
+  b->radius = radius;
+  b->width = w;
+  b->size_map = size_map;
+  b->m = m;
+  b->mstride = mstride;
+}
+
+// PaintLayer::copyTo into the snapshot (FootprintBrush.hxx:281-284, 168-172).
+void brush_full_snapshot(pnt_brush b, pnt_canvas cv) {
+  hipStream_t s = b->ctx->stream;
+  if (b->snap_rows != cv->rows || b->snap_cols != cv->cols) {
+    const int64_t stride = round_up(std::max<int64_t>(cv->n, 1), 64);
+    float* snap = dev_alloc(7 * stride);
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    dev_free(b->snap);
+    b->snap = snap;
+    b->snap_stride = stride;
+    b->snap_rows = cv->rows;
+    b->snap_cols = cv->cols;
+  }
+  launch_copy_layer(cv->n, cv->layer(), b->snap_planes(), s);
+}
+
+StampArgs stamp_args(pnt_brush b, pnt_canvas cv, double cx, double cy, double theta) {
+  PNT_REQUIRE(std::isfinite(cx) && std::isfinite(cy) && std::isfinite(theta) &&
+                  std::fabs(cx) < 1e9 && std::fabs(cy) < 1e9,
+              "stamp centre/angle must be finite and |centre| < 1e9");
+  StampArgs a;
+  a.g.cx = cx;
+  a.g.cy = cy;
+  a.g.c = std::cos(-theta);
+  a.g.s = std::sin(-theta);
+  a.g.hr = (b->size_map - 1) / 2;
+  a.g.size_map = b->size_map;
+  a.g.rows = cv->rows;
+  a.g.cols = cv->cols;
+  a.b = b->args();
+  a.canvas = cv->layer();
+  a.src = b->use_snap ? b->snap_planes() : cv->layer();
+  a.dupx = stamp_dup_axis(cx, a.g.hr) ? 1 : 0;
+  a.dupy = stamp_dup_axis(cy, a.g.hr) ? 1 : 0;
+  return a;
+}
+
+// Visits of a stamp that share a canvas pixel on column/row 0, plus their
+// map-cell partners, in reference (row-major) order, with dependency levels.
+void plan_shared_visits(const StampArgs& a, std::vector<int2>& list, std::vector<int>& off) {
+  list.clear();
+  off.clear();
+  if (!a.dupx && !a.dupy) return;
+  const StampGeom& g = a.g;
+  const int hr = g.hr;
+  std::vector<std::pair<int, int>> rc;  // (row, col)
+  auto add_with_partners = [&](int col, int row) {
+    int64_t ic;
+    const int m = stamp_visit(g, col, row, &ic);
+    if (m < 0) return;
+    rc.emplace_back(row, col);
+    const int nb[4][2] = {{col - 1, row}, {col + 1, row}, {col, row - 1}, {col, row + 1}};
+    for (auto& q : nb) {
+      int64_t jc;
+      if (stamp_visit(g, q[0], q[1], &jc) == m) rc.emplace_back(q[1], q[0]);
+    }
+  };
+  for (int k = -hr; k <= hr; ++k) {
+    if (a.dupx && (int)(k + g.cx) == 0)
+      for (int row = -hr; row <= hr; ++row) add_with_partners(k, row);
+    if (a.dupy && (int)(k + g.cy) == 0)
+      for (int col = -hr; col <= hr; ++col) add_with_partners(col, k);
+  }
+  std::sort(rc.begin(), rc.end());
+  rc.erase(std::unique(rc.begin(), rc.end()), rc.end());
+  std::unordered_map<int, int> lvl_map;
+  std::unordered_map<int64_t, int> lvl_px;
+  std::vector<std::pair<int, int2>> lv;
+  int maxl = -1;
+  for (auto& v : rc) {
+    int64_t ic;
+    const int m = stamp_visit(g, v.second, v.first, &ic);
+    int L = 0;
+    auto im = lvl_map.find(m);
+    if (im != lvl_map.end()) L = std::max(L, im->second + 1);
+    auto ip = lvl_px.find(ic);
+    if (ip != lvl_px.end()) L = std::max(L, ip->second + 1);
+    lvl_map[m] = L;
+    lvl_px[ic] = L;
+    lv.push_back({L, make_int2(v.second, v.first)});
+    maxl = std::max(maxl, L);
+  }
+  std::stable_sort(lv.begin(), lv.end(),
+                   [](const std::pair<int, int2>& x, const std::pair<int, int2>& y) {
+                     return x.first < y.first;
+                   });
+  off.assign((size_t)maxl + 2, 0);
+  for (auto& e : lv) off[(size_t)e.first + 1]++;
+  for (size_t i = 1; i < off.size(); ++i) off[i] += off[i - 1];
+  for (auto& e : lv) list.push_back(e.second);
+}
+
+}  // namespace pnt
+
+namespace {
+
+void brush_dip(pnt_brush b, const float K[3], const float S[3]) {
+  launch_fill(b->pick, 7 * b->mstride, 0.0f, b->ctx->stream);
+  for (int c = 0; c < 3; ++c) {
+    b->paintK[c] = K[c];
+    b->paintS[c] = S[c];
+  }
+}
+
+// FootprintBrush::imprint (FootprintBrush.hxx:73-143) as kernel launches.
+void imprint_plain(pnt_brush b, pnt_canvas cv, double cx, double cy, double theta) {
+  hipStream_t s = b->ctx->stream;
+  if (b->use_snap) {
+    if (b->snap_rows != cv->rows || b->snap_cols != cv->cols) brush_full_snapshot(b, cv);
+  }
+  StampArgs a = stamp_args(b, cv, cx, cy, theta);
+  if (b->use_snap) launch_ring_refresh(ring_box(a.g, b->radius), cv->cols, cv->layer(), b->snap_planes(), s);
+  launch_stamp_main(a, s);
+  if (a.dupx || a.dupy) {
+    std::vector<int2> list;
+    std::vector<int> off;
+    plan_shared_visits(a, list, off);
+    if (!list.empty()) {
+      const int64_t need = (int64_t)list.size();
+      if (need > b->d_cap || (int64_t)off.size() > b->d_cap + 1) {
+        PNT_HIP_CHECK(hipStreamSynchronize(s));
+        dev_free(b->d_list);
+        dev_free(b->d_off);
+        b->d_list = nullptr;
+        b->d_off = nullptr;
+        const int64_t cap = std::max<int64_t>(need, (int64_t)off.size()) * 2;
+        PNT_HIP_CHECK(hipMalloc(&b->d_list, cap * sizeof(int2)));
+        PNT_HIP_CHECK(hipMalloc(&b->d_off, (cap + 1) * sizeof(int)));
+        b->d_cap = cap;
+      }
+      PNT_HIP_CHECK(hipMemcpyAsync(b->d_list, list.data(), need * sizeof(int2),
+                                   hipMemcpyHostToDevice, s));
+      PNT_HIP_CHECK(hipMemcpyAsync(b->d_off, off.data(), off.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, s));
+      launch_stamp_serial(a, b->d_list, b->d_off, (int)off.size() - 1, s);
+      PNT_HIP_CHECK(hipStreamSynchronize(s));  // host lists die here
+    }
+  }
+}
+
+void check_layout(int layout) {
+  PNT_REQUIRE(layout == PNT_AOS || layout == PNT_PLANAR, "layout must be PNT_AOS or PNT_PLANAR");
+}
+
+void check_same_ctx(pnt_brush b, pnt_canvas cv) {
+  PNT_REQUIRE(b && cv, "NULL handle");
+  PNT_REQUIRE(b->ctx->device == cv->ctx->device, "brush and canvas live on different devices");
+}
+
+// painty/core/Spline.hxx:27-45 and :47-73 (tau = 0.5), FP64 without contraction.
+double catmull_rom(double p_1, double p0, double p1, double p2, double t) {
+#pragma clang fp contract(off)
+  const double tau = 0.5;
+  const double t2 = t * t;
+  const double t3 = t2 * t;
+  const double b1 = -tau * t + 2.0 * tau * t2 - tau * t3;
+  const double b2 = 1.0 + (tau - 3.0) * t2 + (2.0 - tau) * t3;
+  const double b3 = tau * t + (3.0 - 2.0 * tau) * t2 + (tau - 2.0) * t3;
+  const double b4 = -tau * t2 + tau * t3;
+  return (p_1 * b1 + p0 * b2 + p1 * b3 + p2 * b4);
+}
+
+double catmull_rom_d1(double p_1, double p0, double p1, double p2, double t) {
+#pragma clang fp contract(off)
+  const double tau = 0.5;
+  const double t2 = t * t;
+  const double b1 = tau * (-3.0 * t2 + 4.0 * t - 1.0);
+  const double b2 = -t * (-2.0 * tau + 3.0 * (tau - 2.0) * t + 6.0);
+  const double b3 = (t - 1.0) * (3.0 * (tau - 2.0) * t - tau);
+  const double b4 = tau * t * (3.0 * t - 2.0);
+  return (p_1 * b1 + p0 * b2 + p1 * b3 + p2 * b4);
+}
+
+// PaintMixer::mixSinglePaint, mixer/src/PaintMixer.cxx:327-355 (Eps = 10e-6, :278).
+void mix_single(int n, const double* bK, const double* bS, const double* w, double K[3],
+                double S[3]) {
+#pragma clang fp contract(off)
+  double norm = 1.0, wsum = 0.0;
+  for (int i = 0; i < n; ++i) wsum += w[i];
+  if (!(std::fabs(wsum - 1.0) < 10e-6)) norm = 1. / wsum;
+  double k[3] = {0, 0, 0}, s[3] = {0, 0, 0};
+  for (int l = 0; l < n; ++l)
+    for (int c = 0; c < 3; ++c) {
+      k[c] += norm * w[l] * bK[3 * l + c];
+      s[c] += norm * w[l] * bS[3 * l + c];
+    }
+  for (int c = 0; c < 3; ++c) {
+    K[c] = k[c];
+    S[c] = s[c];
+  }
+}
+
+// PaintMixer::mixed, PaintMixer.cxx:536-548.
+void mixed(const double K1[3], const double S1[3], double v1, const double K2[3],
+           const double S2[3], double v2, double K[3], double S[3]) {
+#pragma clang fp contract(off)
+  const double inv = 1.0 / (v1 + v2);
+  for (int c = 0; c < 3; ++c) {
+    const double k = ((v1 * K1[c]) + (v2 * K2[c])) * inv;
+    const double s = ((v1 * S1[c]) + (v2 * S2[c])) * inv;
+    K[c] = k;
+    S[c] = s;
+  }
+}
+
+}  // namespace
+
+// ================================================================== C ABI ==
+extern "C" {
+
+const char* pnt_last_error(void) { return g_err.c_str(); }
+
+int pnt_version(void) { return 100; }
+
+int pnt_ctx_create(int device, pnt_ctx* out) {
+  return guard([&] {
+    PNT_REQUIRE(out != nullptr, "out is NULL");
+    int ndev = 0;
+    PNT_HIP_CHECK(hipGetDeviceCount(&ndev));
+    PNT_REQUIRE(device >= 0 && device < ndev, "no such HIP device");
+    auto* c = new pnt_ctx_s;
+    c->device = device;
+    try {
+      c->use();
+      PNT_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int pnt_ctx_destroy(pnt_ctx ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int pnt_ctx_synchronize(pnt_ctx ctx) {
+  return guard([&] {
+    PNT_REQUIRE(ctx, "NULL context");
+    ctx->use();
+    PNT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+void* pnt_ctx_stream(pnt_ctx ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+// ------------------------------------------------------------- PaintLayer --
+int pnt_layer_create(pnt_ctx ctx, int rows, int cols, pnt_layer* out) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && out, "NULL argument");
+    PNT_REQUIRE(rows >= 0 && cols >= 0 && (int64_t)rows * cols < ((int64_t)1 << 31),
+                "rows/cols out of range");
+    ctx->use();
+    auto* L = new pnt_layer_s;
+    L->ctx = ctx;
+    try {
+      L->alloc(rows, cols);
+      launch_fill(L->base, 7 * L->stride, 0.0f, ctx->stream);
+      PNT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      dev_free(L->base);
+      delete L;
+      throw;
+    }
+    *out = L;
+  });
+}
+
+int pnt_layer_destroy(pnt_layer L) {
+  return guard([&] {
+    if (!L) return;
+    if (L->owner) {
+      (void)hipSetDevice(L->ctx->device);
+      (void)hipStreamSynchronize(L->ctx->stream);
+      dev_free(L->base);
+      delete L;
+    }
+  });
+}
+
+int pnt_layer_shape(pnt_layer L, int* rows, int* cols) {
+  return guard([&] {
+    PNT_REQUIRE(L && rows && cols, "NULL argument");
+    *rows = L->rows;
+    *cols = L->cols;
+  });
+}
+
+int pnt_layer_clear(pnt_layer L) {
+  return guard([&] {
+    PNT_REQUIRE(L, "NULL layer");
+    L->ctx->use();
+    launch_fill(L->base, 7 * L->stride, 0.0f, L->ctx->stream);
+  });
+}
+
+int pnt_layer_upload(pnt_layer L, const float* K, const float* S, const float* V, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(L, "NULL layer");
+    check_layout(layout);
+    L->ctx->use();
+    const Layer7 P = L->planes();
+    if (K) upload_planes(P.p, 3, K, L->n, layout, L->ctx->stream);
+    if (S) upload_planes(P.p + 3, 3, S, L->n, layout, L->ctx->stream);
+    if (V) upload_planes(P.p + 6, 1, V, L->n, layout, L->ctx->stream);
+  });
+}
+
+int pnt_layer_download(pnt_layer L, float* K, float* S, float* V, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(L, "NULL layer");
+    check_layout(layout);
+    L->ctx->use();
+    const Layer7 P = L->planes();
+    if (K) download_planes(K, 3, P.p, L->n, layout, L->ctx->stream);
+    if (S) download_planes(S, 3, P.p + 3, L->n, layout, L->ctx->stream);
+    if (V) download_planes(V, 1, P.p + 6, L->n, layout, L->ctx->stream);
+  });
+}
+
+int pnt_layer_copy_to(pnt_layer src, pnt_layer dst) {
+  return guard([&] {
+    PNT_REQUIRE(src && dst, "NULL layer");
+    PNT_REQUIRE(src->ctx->device == dst->ctx->device, "layers on different devices");
+    src->ctx->use();
+    hipStream_t s = src->ctx->stream;
+    if (dst->rows != src->rows || dst->cols != src->cols) {
+      PNT_REQUIRE(dst->owner, "cannot resize the paint layer of a canvas");
+      PNT_HIP_CHECK(hipStreamSynchronize(dst->ctx->stream));
+      float* nb = dev_alloc(7 * round_up(std::max<int64_t>(src->n, 1), 64));
+      dev_free(dst->base);
+      dst->base = nb;
+      dst->rows = src->rows;
+      dst->cols = src->cols;
+      dst->n = src->n;
+      dst->stride = round_up(std::max<int64_t>(src->n, 1), 64);
+    }
+    if (dst->ctx->stream != s) PNT_HIP_CHECK(hipStreamSynchronize(dst->ctx->stream));
+    launch_copy_layer(src->n, src->planes(), dst->planes(), s);
+    if (dst->ctx->stream != s) PNT_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int pnt_layer_set(pnt_layer L, int i, int j, const float k[3], const float s[3], float v) {
+  return guard([&] {
+    PNT_REQUIRE(L && k && s, "NULL argument");
+    PNT_REQUIRE(i >= 0 && i < L->rows && j >= 0 && j < L->cols, "index out of range");
+    L->ctx->use();
+    const int64_t idx = (int64_t)i * L->cols + j;
+    const float vals[7] = {k[0], k[1], k[2], s[0], s[1], s[2], v};
+    const Layer7 P = L->planes();
+    for (int c = 0; c < 7; ++c)
+      PNT_HIP_CHECK(hipMemcpyAsync(P.p[c] + idx, &vals[c], sizeof(float), hipMemcpyHostToDevice,
+                                   L->ctx->stream));
+    PNT_HIP_CHECK(hipStreamSynchronize(L->ctx->stream));
+  });
+}
+
+int pnt_layer_compose_onto_host(pnt_layer L, float* R0, int r0_rows, int r0_cols, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(L && R0, "NULL argument");
+    check_layout(layout);
+    L->ctx->use();
+    hipStream_t s = L->ctx->stream;
+    const int64_t n = L->n;
+    if (r0_rows != L->rows || r0_cols != L->cols)  // PaintLayer.hxx:82-87
+      for (int64_t i = 0; i < 3 * n; ++i) R0[i] = 1.0f;
+    const int64_t stride = round_up(std::max<int64_t>(n, 1), 64);
+    float* buf = dev_alloc(3 * stride);
+    try {
+      Rgb3 r{{buf, buf + stride, buf + 2 * stride}};
+      upload_planes(r.p, 3, R0, n, layout, s);
+      launch_compose(n, L->planes(), r, r, s);
+      download_planes(R0, 3, r.p, n, layout, s);
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      dev_free(buf);
+      throw;
+    }
+    dev_free(buf);
+  });
+}
+
+int pnt_layer_compose_onto_canvas(pnt_layer L, pnt_canvas cv) {
+  return guard([&] {
+    PNT_REQUIRE(L && cv, "NULL argument");
+    PNT_REQUIRE(L->rows == cv->rows && L->cols == cv->cols, "layer and canvas sizes differ");
+    PNT_REQUIRE(L->ctx->device == cv->ctx->device, "layer and canvas on different devices");
+    cv->ctx->use();
+    if (L->ctx->stream != cv->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(L->ctx->stream));
+    launch_compose(L->n, L->planes(), cv->R0(), cv->R0(), cv->ctx->stream);
+  });
+}
+
+// ----------------------------------------------------------------- Canvas --
+static void canvas_clear(pnt_canvas cv) {
+  hipStream_t s = cv->ctx->stream;
+  launch_fill(cv->base, 7 * cv->stride, 0.0f, s);
+  for (int c = 0; c < 3; ++c) launch_fill(cv->base + (7 + c) * cv->stride, cv->n, cv->bg[c], s);
+  launch_fill(cv->h(), cv->n, 0.0f, s);
+}
+
+int pnt_canvas_create(pnt_ctx ctx, int rows, int cols, pnt_canvas* out) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && out, "NULL argument");
+    PNT_REQUIRE(rows > 0 && cols > 0 && (int64_t)rows * cols < ((int64_t)1 << 31),
+                "rows/cols out of range");
+    ctx->use();
+    auto* cv = new pnt_canvas_s;
+    cv->ctx = ctx;
+    cv->rows = rows;
+    cv->cols = cols;
+    cv->n = (int64_t)rows * cols;
+    cv->stride = round_up(cv->n, 64);
+    try {
+      cv->base = dev_alloc(11 * cv->stride);
+      canvas_clear(cv);
+      PNT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      dev_free(cv->base);
+      delete cv;
+      throw;
+    }
+    cv->view.ctx = ctx;
+    cv->view.rows = rows;
+    cv->view.cols = cols;
+    cv->view.n = cv->n;
+    cv->view.stride = cv->stride;
+    cv->view.base = cv->base;
+    cv->view.owner = false;
+    *out = cv;
+  });
+}
+
+int pnt_canvas_destroy(pnt_canvas cv) {
+  return guard([&] {
+    if (!cv) return;
+    (void)hipSetDevice(cv->ctx->device);
+    (void)hipStreamSynchronize(cv->ctx->stream);
+    dev_free(cv->base);
+    dev_free(cv->scratch);
+    delete cv;
+  });
+}
+
+int pnt_canvas_shape(pnt_canvas cv, int* rows, int* cols) {
+  return guard([&] {
+    PNT_REQUIRE(cv && rows && cols, "NULL argument");
+    *rows = cv->rows;
+    *cols = cv->cols;
+  });
+}
+
+int pnt_canvas_layer(pnt_canvas cv, pnt_layer* out) {
+  return guard([&] {
+    PNT_REQUIRE(cv && out, "NULL argument");
+    *out = &cv->view;
+  });
+}
+
+int pnt_canvas_clear(pnt_canvas cv) {
+  return guard([&] {
+    PNT_REQUIRE(cv, "NULL canvas");
+    cv->ctx->use();
+    canvas_clear(cv);
+  });
+}
+
+int pnt_canvas_set_background(pnt_canvas cv, const float* R0, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(cv && R0, "NULL argument");
+    check_layout(layout);
+    cv->ctx->use();
+    canvas_clear(cv);
+    Rgb3 r = cv->R0();
+    upload_planes(r.p, 3, R0, cv->n, layout, cv->ctx->stream);
+  });
+}
+
+int pnt_canvas_upload_R0(pnt_canvas cv, const float* R0, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(cv && R0, "NULL argument");
+    check_layout(layout);
+    cv->ctx->use();
+    Rgb3 r = cv->R0();
+    upload_planes(r.p, 3, R0, cv->n, layout, cv->ctx->stream);
+  });
+}
+
+int pnt_canvas_download_R0(pnt_canvas cv, float* R0, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(cv && R0, "NULL argument");
+    check_layout(layout);
+    cv->ctx->use();
+    Rgb3 r = cv->R0();
+    download_planes(R0, 3, r.p, cv->n, layout, cv->ctx->stream);
+  });
+}
+
+int pnt_canvas_upload_h(pnt_canvas cv, const float* h) {
+  return guard([&] {
+    PNT_REQUIRE(cv && h, "NULL argument");
+    cv->ctx->use();
+    float* p = cv->h();
+    upload_planes(&p, 1, h, cv->n, PNT_PLANAR, cv->ctx->stream);
+  });
+}
+
+int pnt_canvas_download_h(pnt_canvas cv, float* h) {
+  return guard([&] {
+    PNT_REQUIRE(cv && h, "NULL argument");
+    cv->ctx->use();
+    float* p = cv->h();
+    download_planes(h, 1, &p, cv->n, PNT_PLANAR, cv->ctx->stream);
+  });
+}
+
+int pnt_canvas_dry(pnt_canvas cv) {
+  return guard([&] {
+    PNT_REQUIRE(cv, "NULL canvas");
+    cv->ctx->use();
+    launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), cv->ctx->stream);
+  });
+}
+
+int pnt_canvas_dry_step(pnt_canvas cv, float portion) {
+  return guard([&] {
+    PNT_REQUIRE(cv, "NULL canvas");
+    PNT_REQUIRE(std::isfinite(portion), "portion must be finite");
+    cv->ctx->use();
+    launch_dry_step(cv->n, cv->layer(), cv->R0(), portion, cv->ctx->stream);
+  });
+}
+
+int pnt_canvas_set_drying_time_ms(pnt_canvas cv, int64_t ms) {
+  return guard([&] {
+    PNT_REQUIRE(cv, "NULL canvas");
+    PNT_REQUIRE(ms >= 0, "drying time must be >= 0");
+    cv->drying_ms = ms;
+  });
+}
+
+int pnt_canvas_get_drying_time_ms(pnt_canvas cv, int64_t* ms) {
+  return guard([&] {
+    PNT_REQUIRE(cv && ms, "NULL argument");
+    *ms = cv->drying_ms;
+  });
+}
+
+int pnt_canvas_device_planes(pnt_canvas cv, pnt_dev_layer* layer, pnt_dev_rgb* R0, float** h) {
+  return guard([&] {
+    PNT_REQUIRE(cv, "NULL canvas");
+    if (layer) {
+      const Layer7 L = cv->layer();
+      for (int i = 0; i < 7; ++i) layer->p[i] = L.p[i];
+    }
+    if (R0) {
+      const Rgb3 r = cv->R0();
+      for (int i = 0; i < 3; ++i) R0->p[i] = r.p[i];
+    }
+    if (h) *h = cv->h();
+  });
+}
+
+// --------------------------------------------------------------- Renderer --
+int pnt_compose(pnt_canvas cv, float* R, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(cv && R, "NULL argument");
+    check_layout(layout);
+    cv->ctx->use();
+    if (!cv->scratch) cv->scratch = dev_alloc(3 * cv->stride);
+    Rgb3 out{{cv->scratch, cv->scratch + cv->stride, cv->scratch + 2 * cv->stride}};
+    launch_compose(cv->n, cv->layer(), cv->R0(), out, cv->ctx->stream);
+    download_planes(R, 3, out.p, cv->n, layout, cv->ctx->stream);
+  });
+}
+
+int pnt_compose_layer_host(pnt_layer L, const float* R0, float* R, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(L && R0 && R, "NULL argument");
+    check_layout(layout);
+    L->ctx->use();
+    hipStream_t s = L->ctx->stream;
+    const int64_t n = L->n;
+    const int64_t stride = round_up(std::max<int64_t>(n, 1), 64);
+    float* buf = dev_alloc(6 * stride);
+    try {
+      Rgb3 r0{{buf, buf + stride, buf + 2 * stride}};
+      Rgb3 r1{{buf + 3 * stride, buf + 4 * stride, buf + 5 * stride}};
+      upload_planes(r0.p, 3, R0, n, layout, s);
+      launch_compose(n, L->planes(), r0, r1, s);
+      download_planes(R, 3, r1.p, n, layout, s);
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      dev_free(buf);
+      throw;
+    }
+    dev_free(buf);
+  });
+}
+
+int pnt_compute_reflectance(pnt_ctx ctx, int64_t n, const float* K, const float* S,
+                            const float* R0, const float* d, float* R) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && K && S && R0 && d && R, "NULL argument");
+    PNT_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return;
+    ctx->use();
+    hipStream_t s = ctx->stream;
+    float* buf = dev_alloc(13 * n);
+    try {
+      PNT_HIP_CHECK(hipMemcpyAsync(buf, K, 3 * n * sizeof(float), hipMemcpyHostToDevice, s));
+      PNT_HIP_CHECK(hipMemcpyAsync(buf + 3 * n, S, 3 * n * sizeof(float), hipMemcpyHostToDevice, s));
+      PNT_HIP_CHECK(hipMemcpyAsync(buf + 6 * n, R0, 3 * n * sizeof(float), hipMemcpyHostToDevice, s));
+      PNT_HIP_CHECK(hipMemcpyAsync(buf + 9 * n, d, n * sizeof(float), hipMemcpyHostToDevice, s));
+      launch_reflectance_aos(n, buf, buf + 3 * n, buf + 6 * n, buf + 9 * n, buf + 10 * n, s);
+      PNT_HIP_CHECK(hipMemcpyAsync(R, buf + 10 * n, 3 * n * sizeof(float), hipMemcpyDeviceToHost, s));
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      dev_free(buf);
+      throw;
+    }
+    dev_free(buf);
+  });
+}
+
+static Layer7 to7(const pnt_dev_layer& l) {
+  Layer7 L;
+  for (int i = 0; i < 7; ++i) {
+    PNT_REQUIRE(l.p[i] != nullptr, "NULL device plane");
+    L.p[i] = l.p[i];
+  }
+  return L;
+}
+static Rgb3 to3(const pnt_dev_rgb& r) {
+  Rgb3 R;
+  for (int i = 0; i < 3; ++i) {
+    PNT_REQUIRE(r.p[i] != nullptr, "NULL device plane");
+    R.p[i] = r.p[i];
+  }
+  return R;
+}
+
+int pnt_dev_compose(pnt_ctx ctx, int64_t n, pnt_dev_layer layer, pnt_dev_rgb R0, pnt_dev_rgb R,
+                    void* stream) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && n >= 0, "invalid argument");
+    ctx->use();
+    launch_compose(n, to7(layer), to3(R0), to3(R),
+                   stream ? (hipStream_t)stream : ctx->stream);
+  });
+}
+
+int pnt_dev_compose2(pnt_ctx ctx, int64_t n, pnt_dev_layer dry, pnt_dev_layer wet,
+                     pnt_dev_rgb R0, pnt_dev_rgb R, void* stream) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && n >= 0, "invalid argument");
+    ctx->use();
+    launch_compose2(n, to7(dry), to7(wet), to3(R0), to3(R),
+                    stream ? (hipStream_t)stream : ctx->stream);
+  });
+}
+
+// ---------------------------------------------------------- FootprintBrush --
+int pnt_brush_create(pnt_ctx ctx, double radius, const float* footprint, int width,
+                     pnt_brush* out) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && out, "NULL argument");
+    ctx->use();
+    auto* b = new pnt_brush_s;
+    b->ctx = ctx;
+    try {
+      brush_set_geometry(b, radius, footprint, width);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int pnt_brush_destroy(pnt_brush b) {
+  return guard([&] {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    persistent_release(b->pers);
+    dev_free(b->fp);
+    dev_free(b->pick);
+    dev_free(b->snap);
+    dev_free(b->d_list);
+    dev_free(b->d_off);
+    delete b;
+  });
+}
+
+int pnt_brush_set_radius(pnt_brush b, double radius, const float* footprint, int width) {
+  return guard([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    if (std::fabs(b->radius - radius) < 0.5) return;  // fuzzyCompare(_radius, radius, 0.5)
+    b->ctx->use();
+    PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+    brush_set_geometry(b, radius, footprint, width);
+  });
+}
+
+int pnt_brush_size_map(pnt_brush b, int* size_map) {
+  return guard([&] {
+    PNT_REQUIRE(b && size_map, "NULL argument");
+    *size_map = b->size_map;
+  });
+}
+
+int pnt_brush_get_footprint(pnt_brush b, float* out) {
+  return guard([&] {
+    PNT_REQUIRE(b && out, "NULL argument");
+    b->ctx->use();
+    PNT_HIP_CHECK(hipMemcpyAsync(out, b->fp, b->m * sizeof(float), hipMemcpyDeviceToHost,
+                                 b->ctx->stream));
+    PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+int pnt_brush_get_pickup_map(pnt_brush b, float* K, float* S, float* V, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    check_layout(layout);
+    b->ctx->use();
+    const Layer7 P = b->pick_planes();
+    if (K) download_planes(K, 3, P.p, b->m, layout, b->ctx->stream);
+    if (S) download_planes(S, 3, P.p + 3, b->m, layout, b->ctx->stream);
+    if (V) download_planes(V, 1, P.p + 6, b->m, layout, b->ctx->stream);
+  });
+}
+
+int pnt_brush_set_pickup_rate(pnt_brush b, float rate) {
+  return guard([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    b->pickup_rate = rate;
+  });
+}
+int pnt_brush_get_pickup_rate(pnt_brush b, float* rate) {
+  return guard([&] {
+    PNT_REQUIRE(b && rate, "NULL argument");
+    *rate = b->pickup_rate;
+  });
+}
+int pnt_brush_set_deposition_rate(pnt_brush b, float rate) {
+  return guard([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    b->dep_rate = rate;
+  });
+}
+int pnt_brush_get_deposition_rate(pnt_brush b, float* rate) {
+  return guard([&] {
+    PNT_REQUIRE(b && rate, "NULL argument");
+    *rate = b->dep_rate;
+  });
+}
+int pnt_brush_set_use_snapshot(pnt_brush b, int use) {
+  return guard([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    b->use_snap = use != 0;
+  });
+}
+int pnt_brush_get_use_snapshot(pnt_brush b, int* use) {
+  return guard([&] {
+    PNT_REQUIRE(b && use, "NULL argument");
+    *use = b->use_snap ? 1 : 0;
+  });
+}
+
+int pnt_brush_dip(pnt_brush b, const float K[3], const float S[3]) {
+  return guard([&] {
+    PNT_REQUIRE(b && K && S, "NULL argument");
+    b->ctx->use();
+    brush_dip(b, K, S);
+  });
+}
+
+int pnt_brush_clean(pnt_brush b) {
+  return guard([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    b->ctx->use();
+    launch_fill(b->pick, 7 * b->mstride, 0.0f, b->ctx->stream);
+  });
+}
+
+int pnt_brush_update_snapshot(pnt_brush b, pnt_canvas cv) {
+  return guard([&] {
+    check_same_ctx(b, cv);
+    b->ctx->use();
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    brush_full_snapshot(b, cv);
+  });
+}
+
+int pnt_imprint(pnt_brush b, pnt_canvas cv, double cx, double cy, double theta) {
+  return guard([&] {
+    check_same_ctx(b, cv);
+    b->ctx->use();
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    imprint_plain(b, cv, cx, cy, theta);
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t n, int mode) {
+  return guard([&] {
+    check_same_ctx(b, cv);
+    PNT_REQUIRE(n >= 0 && (st || n == 0), "invalid stamp list");
+    PNT_REQUIRE(mode == PNT_EXEC_PLAIN || mode == PNT_EXEC_PERSISTENT, "unknown mode");
+    b->ctx->use();
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    if (mode == PNT_EXEC_PERSISTENT) {
+      run_persistent(b, cv, st, n);
+    } else {
+      for (int64_t i = 0; i < n; ++i) {
+        if (st[i].flags & PNT_STAMP_DIP) brush_dip(b, st[i].K, st[i].S);
+        imprint_plain(b, cv, st[i].cx, st[i].cy, st[i].theta);
+        if (st[i].flags & PNT_STAMP_DRY_AFTER)
+          launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), cv->ctx->stream);
+      }
+    }
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+int pnt_paint_stroke(pnt_brush b, pnt_canvas cv, const double* p, int64_t n) {
+  return guard([&] {
+    check_same_ctx(b, cv);
+    PNT_REQUIRE(n >= 0 && (p || n == 0), "invalid path");
+    b->ctx->use();
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    // FootprintBrush.hxx:251-267
+    for (int64_t i = 0; i + 1 < n; ++i) {
+      const int64_t ipre = i > 0 ? i - 1 : 0;
+      const int64_t inext = (i + 2 < n) ? i + 2 : n - 1;
+      const double *pp = p + 2 * ipre, *p0 = p + 2 * i, *p1 = p + 2 * (i + 1), *pn = p + 2 * inext;
+      double dist;
+      {
+#pragma clang fp contract(off)
+        const double dx = p1[0] - p0[0], dy = p1[1] - p0[1];
+        dist = std::sqrt(dx * dx + dy * dy);
+      }
+      for (int pd = 1; pd <= (int)dist; ++pd) {
+        const double t = (double)pd / dist;
+        const double ddx = catmull_rom_d1(pp[0], p0[0], p1[0], pn[0], t);
+        const double ddy = catmull_rom_d1(pp[1], p0[1], p1[1], pn[1], t);
+        imprint_plain(b, cv, catmull_rom(pp[0], p0[0], p1[0], pn[0], t),
+                      catmull_rom(pp[1], p0[1], p1[1], pn[1], t), std::atan2(ddy, ddx));
+      }
+    }
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+// -------------------------------------------------------------- PaintMixer --
+int pnt_mix_single(int n_palette, const double* base_K, const double* base_S, int n_weights,
+                   const double* w, double K[3], double S[3]) {
+  return guard([&] {
+    if (n_weights != n_palette)
+      throw Error(PNT_EINVAL, "Palette size does not match underlying size.");
+    PNT_REQUIRE((base_K && base_S && w) || n_palette == 0, "NULL argument");
+    PNT_REQUIRE(K && S, "NULL argument");
+    mix_single(n_palette, base_K, base_S, w, K, S);
+  });
+}
+
+int pnt_mixed(const double K1[3], const double S1[3], double v1, const double K2[3],
+              const double S2[3], double v2, double K[3], double S[3]) {
+  return guard([&] {
+    PNT_REQUIRE(K1 && S1 && K2 && S2 && K && S, "NULL argument");
+    mixed(K1, S1, v1, K2, S2, v2, K, S);
+  });
+}
+
+}  // extern "C"

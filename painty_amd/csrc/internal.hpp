@@ -1,0 +1,81 @@
+// Internal interfaces between the C ABI (api.hip) and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "painty_hip.h"
+#include "stamp_geom.hpp"
+
+namespace pnt {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PNT_HIP_CHECK(expr)                                                        \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw ::pnt::Error(PNT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// A paint layer on the device: 7 planes K0 K1 K2 S0 S1 S2 V.
+struct Layer7 {
+  float* p[7];
+};
+struct Rgb3 {
+  float* p[3];
+};
+
+// Brush state and parameters one imprint needs.
+struct BrushArgs {
+  const float* fp;   // footprint, size_map^2
+  Layer7 pick;       // pickup map, size_map^2 planes
+  float pickup_rate, dep_rate, cap;
+  float paintK[3], paintS[3];
+};
+
+// One imprint: geometry + where its visits read and write.
+struct StampArgs {
+  StampGeom g;
+  BrushArgs b;
+  Layer7 canvas;     // wet layer of the canvas
+  Layer7 src;        // pickup source: the snapshot, or the canvas itself
+  int dupx, dupy;    // canvas column / row 0 visited twice (see stamp_dup_axis)
+};
+
+// Snapshot ring of updateSnapshot(canvas, center), FootprintBrush.hxx:286-305.
+struct RingBox {
+  int a_x0, a_y0, a_x1, a_y1;  // allowed box (inclusive, clipped to the canvas)
+  int tl_x, tl_y, br_x, br_y;  // window corners; strict interior is skipped
+};
+
+// ---- composite / dry (kernels.hip) ----
+void launch_compose(int64_t n, const Layer7& L, const Rgb3& R0, const Rgb3& R, hipStream_t s);
+void launch_compose2(int64_t n, const Layer7& dry, const Layer7& wet, const Rgb3& R0,
+                     const Rgb3& R, hipStream_t s);
+void launch_dry(int64_t n, const Layer7& L, const Rgb3& R0, float* h, hipStream_t s);
+void launch_dry_step(int64_t n, const Layer7& L, const Rgb3& R0, float portion,
+                     hipStream_t s);
+void launch_reflectance_aos(int64_t n, const float* K, const float* S, const float* R0,
+                            const float* d, float* R, hipStream_t s);
+void launch_fill(float* p, int64_t n, float v, hipStream_t s);
+void launch_copy_layer(int64_t n, const Layer7& src, const Layer7& dst, hipStream_t s);
+
+// ---- stamping (kernels.hip) ----
+void launch_ring_refresh(const RingBox& box, int cols, const Layer7& canvas, const Layer7& snap,
+                         hipStream_t s);
+void launch_stamp_main(const StampArgs& a, hipStream_t s);
+// Visits that share a canvas pixel at column/row 0 (and their map-cell
+// partners), listed in reference order and grouped by dependency level.
+void launch_stamp_serial(const StampArgs& a, const int2* list, const int* level_off, int nlevels,
+                         hipStream_t s);
+
+RingBox ring_box(const StampGeom& g, double radius);
+
+}  // namespace pnt

@@ -1,0 +1,456 @@
+// gfx950 kernels: KM composite, drying, imprint stamping.
+//
+// Composite and dry are streaming kernels over FP32 planes (HBM-bound): one
+// thread handles four consecutive pixels with 16-byte loads from every plane.
+// Stamping follows renderer/FootprintBrush.hxx:73-143 visit by visit; see the
+// comment above k_stamp_main for how the reference's serial order is kept.
+#include <algorithm>
+
+#include "internal.hpp"
+#include "km.hpp"
+
+namespace pnt {
+namespace {
+
+constexpr int kBlock = 256;
+// blend() and pickup thresholds compare against MinVolume = 0.001 in FP64
+// (FootprintBrush.hxx:26).  For a float x, x > 0.001 holds exactly when
+// x >= 0.001f (0.001f = 0.0010000000475 is the first float above 0.001).
+constexpr float kMinVolF = 0.001f;
+
+__device__ __forceinline__ float4 ld4(const float* p, int64_t i) {
+  return *reinterpret_cast<const float4*>(p + 4 * i);
+}
+__device__ __forceinline__ void st4(float* p, int64_t i, const float4& v) {
+  *reinterpret_cast<float4*>(p + 4 * i) = v;
+}
+__device__ __forceinline__ float& lane(float4& v, int j) {
+  return reinterpret_cast<float*>(&v)[j];
+}
+
+// ------------------------------------------------------------ composite ---
+// Applies one layer (K, S, V planes) to the reflectance r of pixels 4i..4i+3.
+__device__ __forceinline__ void apply_layer4(const Layer7& L, int64_t i, float4 r[3]) {
+  float4 k[3], s[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    k[c] = ld4(L.p[c], i);
+    s[c] = ld4(L.p[3 + c], i);
+  }
+  float4 v = ld4(L.p[6], i);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    float K[3], S[3], R[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      K[c] = lane(k[c], j);
+      S[c] = lane(s[c], j);
+      R[c] = lane(r[c], j);
+    }
+    km_pixel(K, S, R, lane(v, j));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) lane(r[c], j) = R[c];
+  }
+}
+
+__device__ __forceinline__ void apply_layer1(const Layer7& L, int64_t i, float R[3]) {
+  float K[3], S[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    K[c] = L.p[c][i];
+    S[c] = L.p[3 + c][i];
+  }
+  km_pixel(K, S, R, L.p[6][i]);
+}
+
+// Renderer::compose (Renderer.hxx:26-41) for kLayers == 1; for kLayers == 2
+// the dry layer A is composed first (PaintLayer::composeOnto, :81-96) and the
+// wet layer B over the result, without writing the intermediate.
+template <int kLayers>
+__global__ __launch_bounds__(kBlock) void k_compose_vec(int64_t n4, Layer7 A, Layer7 B, Rgb3 R0,
+                                                        Rgb3 R) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n4) return;
+  float4 r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r[c] = ld4(R0.p[c], i);
+  apply_layer4(A, i, r);
+  if (kLayers == 2) apply_layer4(B, i, r);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) st4(R.p[c], i, r[c]);
+}
+
+template <int kLayers>
+__global__ __launch_bounds__(kBlock) void k_compose_scalar(int64_t begin, int64_t n, Layer7 A,
+                                                           Layer7 B, Rgb3 R0, Rgb3 R) {
+  const int64_t i = begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r[c] = R0.p[c][i];
+  apply_layer1(A, i, r);
+  if (kLayers == 2) apply_layer1(B, i, r);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) R.p[c][i] = r[c];
+}
+
+// ------------------------------------------------------------------ dry ---
+// Canvas::dryCanvas (Canvas.hxx:105-121): h += V; R0 = KM(K, S, R0, V);
+// V = 0; K = S = 0.
+__device__ __forceinline__ void dry1(const Layer7& L, const Rgb3& R0, float* h, int64_t i) {
+  float K[3], S[3], R[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    K[c] = L.p[c][i];
+    S[c] = L.p[3 + c][i];
+    R[c] = R0.p[c][i];
+  }
+  const float v = L.p[6][i];
+  h[i] += v;
+  km_pixel(K, S, R, v);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    R0.p[c][i] = R[c];
+    L.p[c][i] = 0.0f;
+    L.p[3 + c][i] = 0.0f;
+  }
+  L.p[6][i] = 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dry_vec(int64_t n4, Layer7 L, Rgb3 R0, float* h) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n4) return;
+  float4 r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r[c] = ld4(R0.p[c], i);
+  float4 hv = ld4(h, i);
+  const float4 v = ld4(L.p[6], i);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) lane(hv, j) += reinterpret_cast<const float*>(&v)[j];
+  apply_layer4(L, i, r);
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    st4(R0.p[c], i, r[c]);
+    st4(L.p[c], i, z);
+    st4(L.p[3 + c], i, z);
+  }
+  st4(L.p[6], i, z);
+  st4(h, i, hv);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dry_scalar(int64_t begin, int64_t n, Layer7 L,
+                                                       Rgb3 R0, float* h) {
+  const int64_t i = begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dry1(L, R0, h, i);
+}
+
+// DryingStep.compute.glsl:66-76: V leaves up to `portion`; R0 takes it.
+__global__ __launch_bounds__(kBlock) void k_dry_step(int64_t n, Layer7 L, Rgb3 R0,
+                                                     float portion) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float d = L.p[6][i];
+  const float rem = fmaxf(0.0f, d - portion);
+  const float leave = d - rem;
+  if (__builtin_fabsf(leave) > 10e-6f) {
+    float K[3], S[3], R[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      K[c] = L.p[c][i];
+      S[c] = L.p[3 + c][i];
+      R[c] = R0.p[c][i];
+    }
+    km_pixel_glsl(K, S, R, leave);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R0.p[c][i] = R[c];
+  }
+  L.p[6][i] = rem;
+}
+
+// ComputeReflectance over AoS arrays (KubelkaMunk.hxx:25-83).
+__global__ __launch_bounds__(kBlock) void k_reflectance_aos(int64_t n, const float* K,
+                                                            const float* S, const float* R0,
+                                                            const float* d, float* R) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float k[3], s[3], r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    k[c] = K[3 * i + c];
+    s[c] = S[3 * i + c];
+    r[c] = R0[3 * i + c];
+  }
+  km_pixel(k, s, r, d[i]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) R[3 * i + c] = r[c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(float* p, int64_t n, float v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_layer(int64_t n, Layer7 src, Layer7 dst) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 7; ++c) dst.p[c][i] = src.p[c][i];
+}
+
+// ------------------------------------------------------------ stamping ---
+// One visit of an imprint: pickupPaint (FootprintBrush.hxx:349-384) then
+// depositPaint (:393-431) for canvas pixel ic and map cell im, with
+// blend() (:331-340) inlined.
+__device__ __forceinline__ void paint_visit(const StampArgs& a, int64_t ic, int im) {
+  const float fh = a.b.fp[im];
+  if (!(fh > 0.0f)) return;
+  float* const* pk = a.b.pick.p;
+  {  // pickup from the source (snapshot or canvas) into the pickup map
+    const float vP = pk[6][im];
+    const float vC = a.src.p[6][ic];
+    const float vLeave = a.b.pickup_rate * vC * fh;
+    if (vLeave >= kMinVolF) {
+      a.src.p[6][ic] = vC - vLeave;
+      const float vt = vP + vLeave;
+      if (vt >= kMinVolF) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) pk[c][im] = (vP * pk[c][im] + vLeave * a.src.p[c][ic]) / vt;
+      }
+      pk[6][im] = vP + vLeave;
+    }
+  }
+  {  // deposit: brush paint mixed with the pickup map onto the canvas
+    const float vP = pk[6][im];
+    const float vFree = fmaxf(0.0f, a.b.cap - vP);
+    float src[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) src[c] = pk[c][im];
+    const float vt = vP + vFree;
+    if (vt >= kMinVolF) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        src[c] = (vP * src[c] + vFree * a.b.paintK[c]) / vt;
+        src[3 + c] = (vP * src[3 + c] + vFree * a.b.paintS[c]) / vt;
+      }
+    }
+    pk[6][im] = vP - a.b.dep_rate * vP * fh;
+    const float vC = a.canvas.p[6][ic];
+    const float vB = a.b.cap * fh;
+    const float vt2 = vB + vC;
+    if (vt2 >= kMinVolF) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.canvas.p[c][ic] = (vB * src[c] + vC * a.canvas.p[c][ic]) / vt2;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.canvas.p[c][ic] = src[c];
+    }
+    a.canvas.p[6][ic] = vB + vC;
+  }
+}
+
+// updateSnapshot(canvas, center), FootprintBrush.hxx:307-318: copy the wet
+// layer into the snapshot over the allowed box except the window interior.
+__global__ __launch_bounds__(kBlock) void k_ring_refresh(RingBox b, int cols, Layer7 canvas,
+                                                         Layer7 snap) {
+  const int w = b.a_x1 - b.a_x0 + 1;
+  const int hgt = b.a_y1 - b.a_y0 + 1;
+  const int64_t total = (int64_t)w * hgt;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * kBlock) {
+    const int row = b.a_y0 + (int)(e / w);
+    const int col = b.a_x0 + (int)(e % w);
+    if (row > b.tl_y && row < b.br_y && col > b.tl_x && col < b.br_x) continue;
+    const int64_t i = (int64_t)row * cols + col;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) snap.p[c][i] = canvas.p[c][i];
+  }
+}
+
+// One imprint's visits, spread over the GPU.
+//
+// The reference visits window pixels in row-major order.  Two visits can
+// touch the same state only when
+//   (a) they round to the same pickup-map cell: rotation preserves distances
+//       and a half-open unit cell holds no two points >= 1 apart except at
+//       distance exactly 1, so such pairs are 4-neighbours in the window and a
+//       cell has at most two visitors; or
+//   (b) they truncate to the same canvas pixel: only column 0 / row 0 when the
+//       stamp straddles the left / top edge with a fractional centre.
+// Each workgroup takes a 16x16 tile of the window and computes the map cells
+// of the tile plus a one-pixel halo in LDS.  A visit whose earlier neighbour
+// (left or up) shares its cell is skipped; the earlier one runs both, in
+// order.  Under (b), visits on canvas column/row 0 and their partners are
+// left to k_stamp_serial, which runs them in reference order by dependency
+// level.  No other visit shares state with them, so the two kernels commute.
+constexpr int kTile = 16;
+
+__device__ __forceinline__ bool on_dup_line(const StampArgs& a, int col, int row) {
+  return (a.dupx && (int)(col + a.g.cx) == 0) || (a.dupy && (int)(row + a.g.cy) == 0);
+}
+
+__global__ __launch_bounds__(kTile* kTile) void k_stamp_main(StampArgs a, int tiles_x) {
+  __shared__ int sm[kTile + 2][kTile + 2];
+  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const int col0 = -a.g.hr + tx * kTile, row0 = -a.g.hr + ty * kTile;
+  for (int e = threadIdx.x; e < (kTile + 2) * (kTile + 2); e += kTile * kTile) {
+    const int ly = e / (kTile + 2), lx = e % (kTile + 2);
+    int64_t ic;
+    sm[ly][lx] = stamp_visit(a.g, col0 + lx - 1, row0 + ly - 1, &ic);
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
+  const int m = sm[ly + 1][lx + 1];
+  if (m < 0) return;
+  if (sm[ly + 1][lx] == m || sm[ly][lx + 1] == m) return;  // earlier visitor owns the cell
+  const int col = col0 + lx, row = row0 + ly;
+  int pcol = col, prow = row;
+  bool partner = false;
+  if (sm[ly + 1][lx + 2] == m) {
+    pcol = col + 1;
+    partner = true;
+  } else if (sm[ly + 2][lx + 1] == m) {
+    prow = row + 1;
+    partner = true;
+  }
+  if ((a.dupx | a.dupy) && (on_dup_line(a, col, row) || (partner && on_dup_line(a, pcol, prow))))
+    return;
+  int64_t ic, pic = 0;
+  stamp_visit(a.g, col, row, &ic);
+  if (partner) stamp_visit(a.g, pcol, prow, &pic);
+  paint_visit(a, ic, m);
+  if (partner) paint_visit(a, pic, m);
+}
+
+// Visits listed in reference order, grouped by level: a visit's level is one
+// more than the highest level of the earlier visits sharing its map cell or
+// canvas pixel.  One workgroup; levels are separated by barriers.
+__global__ __launch_bounds__(kBlock) void k_stamp_serial(StampArgs a, const int2* list,
+                                                         const int* level_off, int nlevels) {
+  for (int L = 0; L < nlevels; ++L) {
+    for (int k = level_off[L] + threadIdx.x; k < level_off[L + 1]; k += kBlock) {
+      const int2 v = list[k];
+      int64_t ic;
+      const int m = stamp_visit(a.g, v.x, v.y, &ic);
+      if (m >= 0) paint_visit(a, ic, m);
+    }
+    __syncthreads();
+  }
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+inline bool aligned16(const float* p) { return ((uintptr_t)p & 15u) == 0; }
+inline bool aligned16(const Layer7& L) {
+  for (auto* p : L.p)
+    if (!aligned16(p)) return false;
+  return true;
+}
+inline bool aligned16(const Rgb3& L) {
+  for (auto* p : L.p)
+    if (!aligned16(p)) return false;
+  return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+template <int kL>
+static void launch_compose_t(int64_t n, const Layer7& A, const Layer7& B, const Rgb3& R0,
+                             const Rgb3& R, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t done = 0;
+  if (aligned16(A) && aligned16(B) && aligned16(R0) && aligned16(R)) {
+    const int64_t n4 = n / 4;
+    if (n4 > 0) k_compose_vec<kL><<<blocks_for(n4), kBlock, 0, s>>>(n4, A, B, R0, R);
+    done = n4 * 4;
+  }
+  if (done < n)
+    k_compose_scalar<kL><<<blocks_for(n - done), kBlock, 0, s>>>(done, n, A, B, R0, R);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_compose(int64_t n, const Layer7& L, const Rgb3& R0, const Rgb3& R, hipStream_t s) {
+  launch_compose_t<1>(n, L, L, R0, R, s);
+}
+
+void launch_compose2(int64_t n, const Layer7& dry, const Layer7& wet, const Rgb3& R0,
+                     const Rgb3& R, hipStream_t s) {
+  launch_compose_t<2>(n, dry, wet, R0, R, s);
+}
+
+void launch_dry(int64_t n, const Layer7& L, const Rgb3& R0, float* h, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t done = 0;
+  if (aligned16(L) && aligned16(R0) && aligned16(h)) {
+    const int64_t n4 = n / 4;
+    if (n4 > 0) k_dry_vec<<<blocks_for(n4), kBlock, 0, s>>>(n4, L, R0, h);
+    done = n4 * 4;
+  }
+  if (done < n) k_dry_scalar<<<blocks_for(n - done), kBlock, 0, s>>>(done, n, L, R0, h);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dry_step(int64_t n, const Layer7& L, const Rgb3& R0, float portion, hipStream_t s) {
+  if (n <= 0) return;
+  k_dry_step<<<blocks_for(n), kBlock, 0, s>>>(n, L, R0, portion);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_reflectance_aos(int64_t n, const float* K, const float* S, const float* R0,
+                            const float* d, float* R, hipStream_t s) {
+  if (n <= 0) return;
+  k_reflectance_aos<<<blocks_for(n), kBlock, 0, s>>>(n, K, S, R0, d, R);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_fill(float* p, int64_t n, float v, hipStream_t s) {
+  if (n <= 0) return;
+  k_fill<<<blocks_for(n), kBlock, 0, s>>>(p, n, v);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_copy_layer(int64_t n, const Layer7& src, const Layer7& dst, hipStream_t s) {
+  if (n <= 0) return;
+  k_copy_layer<<<blocks_for(n), kBlock, 0, s>>>(n, src, dst);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+RingBox ring_box(const StampGeom& g, double radius) {
+#pragma clang fp contract(off)
+  RingBox b;
+  b.tl_x = (int)(g.cx - g.hr);
+  b.tl_y = (int)(g.cy - g.hr);
+  b.br_x = (int)(g.cx + g.hr);
+  b.br_y = (int)(g.cy + g.hr);
+  b.a_x0 = std::max((int)(g.cx - g.hr - radius), 0);
+  b.a_y0 = std::max((int)(g.cy - g.hr - radius), 0);
+  b.a_x1 = std::min((int)(g.cx + g.hr + radius), g.cols - 1);
+  b.a_y1 = std::min((int)(g.cy + g.hr + radius), g.rows - 1);
+  return b;
+}
+
+void launch_ring_refresh(const RingBox& box, int cols, const Layer7& canvas, const Layer7& snap,
+                         hipStream_t s) {
+  if (box.a_x1 < box.a_x0 || box.a_y1 < box.a_y0) return;
+  const int64_t total = (int64_t)(box.a_x1 - box.a_x0 + 1) * (box.a_y1 - box.a_y0 + 1);
+  const unsigned grid = (unsigned)std::min<int64_t>(blocks_for(total), 4096);
+  k_ring_refresh<<<grid, kBlock, 0, s>>>(box, cols, canvas, snap);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_stamp_main(const StampArgs& a, hipStream_t s) {
+  const int side = 2 * a.g.hr + 1;
+  const int tiles = (side + kTile - 1) / kTile;
+  k_stamp_main<<<(unsigned)(tiles * tiles), kTile * kTile, 0, s>>>(a, tiles);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_stamp_serial(const StampArgs& a, const int2* list, const int* level_off, int nlevels,
+                         hipStream_t s) {
+  if (nlevels <= 0) return;
+  k_stamp_serial<<<1, kBlock, 0, s>>>(a, list, level_off, nlevels);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace pnt

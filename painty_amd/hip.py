@@ -1,0 +1,407 @@
+"""Python mirror of the reference's renderer interface over the C ABI.
+
+Class and method names follow painty's C++ templates (instantiated with vec3)
+so that code and tests read like the reference's own:
+
+  PaintLayer      renderer/PaintLayer.hxx:21-145
+  Canvas          renderer/Canvas.hxx:20-198
+  Renderer        renderer/Renderer.hxx:14-53
+  FootprintBrush  renderer/FootprintBrush.hxx:22-502
+  PaintMixer      mixer/PaintMixer.hxx:22-57 (mixSinglePaint, mixed)
+  ComputeReflectance  core/KubelkaMunk.hxx:25-83
+
+The reference hands out mutable host ``Mat&`` buffers; here getters return
+host copies (numpy float32, (rows, cols, 3) for K/S/R0, (rows, cols) for V/h)
+and ``set*`` methods write them back.  Everything runs through
+libpainty_hip.so on the GPU; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import AOS, PLANAR, InvalidArgument, PaintyError, PntStamp, call, load
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+
+
+def _f32(a, shape=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != tuple(shape):
+        raise ValueError("expected shape %s, got %s" % (tuple(shape), a.shape))
+    return a
+
+
+def _pf(a: np.ndarray):
+    return a.ctypes.data_as(_fp) if a is not None else None
+
+
+class Context:
+    """A device and one HIP stream on it."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        call("pnt_ctx_create", device, C.byref(h))
+        self.h = h
+        self.device = device
+
+    def synchronize(self) -> None:
+        call("pnt_ctx_synchronize", self.h)
+
+    @property
+    def stream(self) -> int:
+        return load().pnt_ctx_stream(self.h) or 0
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            load().pnt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default: Context | None = None
+
+
+def default_context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+class PaintLayer:
+    def __init__(self, rows: int, cols: int, ctx: Context | None = None, _handle=None,
+                 _parent=None):
+        self.ctx = ctx or default_context()
+        self._parent = _parent  # keeps a canvas alive behind a view
+        if _handle is None:
+            h = C.c_void_p()
+            call("pnt_layer_create", self.ctx.h, rows, cols, C.byref(h))
+            self.h = h
+            self._owner = True
+        else:
+            self.h = _handle
+            self._owner = False
+
+    def __del__(self):
+        if getattr(self, "_owner", False) and getattr(self, "h", None):
+            try:
+                load().pnt_layer_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def getRows(self) -> int:
+        r, c = C.c_int(), C.c_int()
+        call("pnt_layer_shape", self.h, C.byref(r), C.byref(c))
+        return r.value
+
+    def getCols(self) -> int:
+        r, c = C.c_int(), C.c_int()
+        call("pnt_layer_shape", self.h, C.byref(r), C.byref(c))
+        return c.value
+
+    @property
+    def shape(self):
+        return self.getRows(), self.getCols()
+
+    def clear(self) -> None:
+        call("pnt_layer_clear", self.h)
+
+    def _download(self, which: str) -> np.ndarray:
+        rows, cols = self.shape
+        out = np.empty((rows, cols, 3) if which in "KS" else (rows, cols), np.float32)
+        args = [None, None, None]
+        args["KSV".index(which)] = _pf(out)
+        call("pnt_layer_download", self.h, *args, AOS)
+        return out
+
+    def getK_buffer(self) -> np.ndarray:
+        return self._download("K")
+
+    def getS_buffer(self) -> np.ndarray:
+        return self._download("S")
+
+    def getV_buffer(self) -> np.ndarray:
+        return self._download("V")
+
+    def setBuffers(self, K=None, S=None, V=None) -> None:
+        rows, cols = self.shape
+        K = None if K is None else _f32(K, (rows, cols, 3))
+        S = None if S is None else _f32(S, (rows, cols, 3))
+        V = None if V is None else _f32(V, (rows, cols))
+        call("pnt_layer_upload", self.h, _pf(K), _pf(S), _pf(V), AOS)
+
+    def composeOnto(self, R0: np.ndarray) -> np.ndarray:
+        """In place on R0 (float32 (rows, cols, 3)); returns it.  A
+        mismatched R0 is replaced by a ones image of the layer's size, as
+        PaintLayer.hxx:82-87 does; the replacement is returned."""
+        rows, cols = self.shape
+        R0 = np.asarray(R0)
+        if R0.shape != (rows, cols, 3) or R0.dtype != np.float32 or not R0.flags.c_contiguous:
+            if R0.shape == (rows, cols, 3):
+                raise ValueError("R0 must be a C-contiguous float32 array")
+            R0 = np.empty((rows, cols, 3), np.float32)
+            call("pnt_layer_compose_onto_host", self.h, _pf(R0), -1, -1, AOS)
+            return R0
+        call("pnt_layer_compose_onto_host", self.h, _pf(R0), rows, cols, AOS)
+        return R0
+
+    def composeOntoCanvas(self, canvas: "Canvas") -> None:
+        call("pnt_layer_compose_onto_canvas", self.h, canvas.h)
+
+    def copyTo(self, other: "PaintLayer") -> None:
+        call("pnt_layer_copy_to", self.h, other.h)
+
+    def set(self, i: int, j: int, k, s, v: float) -> None:
+        k, s = _f32(k, (3,)), _f32(s, (3,))
+        call("pnt_layer_set", self.h, i, j, _pf(k), _pf(s), float(v))
+
+
+class Canvas:
+    def __init__(self, rows: int, cols: int, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        call("pnt_canvas_create", self.ctx.h, rows, cols, C.byref(h))
+        self.h = h
+        self.rows, self.cols = rows, cols
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                load().pnt_canvas_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def clear(self) -> None:
+        call("pnt_canvas_clear", self.h)
+
+    def getPaintLayer(self) -> PaintLayer:
+        v = C.c_void_p()
+        call("pnt_canvas_layer", self.h, C.byref(v))
+        return PaintLayer(self.rows, self.cols, self.ctx, _handle=v, _parent=self)
+
+    def getR0(self) -> np.ndarray:
+        out = np.empty((self.rows, self.cols, 3), np.float32)
+        call("pnt_canvas_download_R0", self.h, _pf(out), AOS)
+        return out
+
+    def setR0(self, R0) -> None:
+        R0 = _f32(R0, (self.rows, self.cols, 3))
+        call("pnt_canvas_upload_R0", self.h, _pf(R0), AOS)
+
+    def get_h(self) -> np.ndarray:
+        out = np.empty((self.rows, self.cols), np.float32)
+        call("pnt_canvas_download_h", self.h, _pf(out))
+        return out
+
+    def set_h(self, h) -> None:
+        h = _f32(h, (self.rows, self.cols))
+        call("pnt_canvas_upload_h", self.h, _pf(h))
+
+    def getReflectanceLayerDry(self) -> np.ndarray:
+        return self.getR0()
+
+    def setBackground(self, background) -> None:
+        bg = _f32(background, (self.rows, self.cols, 3))
+        call("pnt_canvas_set_background", self.h, _pf(bg), AOS)
+
+    def dryCanvas(self) -> None:
+        call("pnt_canvas_dry", self.h)
+
+    def dryStep(self, portion: float) -> None:
+        call("pnt_canvas_dry_step", self.h, float(portion))
+
+    def setDryingTime(self, ms: int) -> None:
+        call("pnt_canvas_set_drying_time_ms", self.h, int(ms))
+
+    def getDryingTime(self) -> int:
+        v = C.c_int64()
+        call("pnt_canvas_get_drying_time_ms", self.h, C.byref(v))
+        return v.value
+
+    def device_planes(self):
+        """(layer planes[7], R0 planes[3], h) as device addresses."""
+        L, R, h = _lib.DevLayer(), _lib.DevRgb(), C.c_void_p()
+        call("pnt_canvas_device_planes", self.h, C.byref(L), C.byref(R), C.byref(h))
+        return list(L.p), list(R.p), h.value
+
+
+class Renderer:
+    @staticmethod
+    def compose(*args) -> np.ndarray:
+        """compose(canvas) or compose(paintLayer, R0) -> R (float32 (rows, cols, 3))."""
+        if len(args) == 1:
+            cv = args[0]
+            out = np.empty((cv.rows, cv.cols, 3), np.float32)
+            call("pnt_compose", cv.h, _pf(out), AOS)
+            return out
+        layer, R0 = args
+        rows, cols = layer.shape
+        R0 = _f32(R0, (rows, cols, 3))
+        out = np.empty_like(R0)
+        call("pnt_compose_layer_host", layer.h, _pf(R0), _pf(out), AOS)
+        return out
+
+
+class FootprintBrush:
+    """Imprint brush with an explicit footprint (width = 2*ceil(radius)+1)."""
+
+    def __init__(self, radius: float, footprint, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        fp = _f32(footprint)
+        h = C.c_void_p()
+        call("pnt_brush_create", self.ctx.h, float(radius), _pf(fp), int(fp.shape[0]),
+             C.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                load().pnt_brush_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def setRadius(self, radius: float, footprint) -> None:
+        fp = _f32(footprint)
+        call("pnt_brush_set_radius", self.h, float(radius), _pf(fp), int(fp.shape[0]))
+
+    @property
+    def sizeMap(self) -> int:
+        v = C.c_int()
+        call("pnt_brush_size_map", self.h, C.byref(v))
+        return v.value
+
+    def getFootprint(self) -> np.ndarray:
+        m = self.sizeMap
+        out = np.empty((m, m), np.float32)
+        call("pnt_brush_get_footprint", self.h, _pf(out))
+        return out
+
+    def getPickupMap(self):
+        m = self.sizeMap
+        K, S, V = (np.empty((m, m, 3), np.float32), np.empty((m, m, 3), np.float32),
+                   np.empty((m, m), np.float32))
+        call("pnt_brush_get_pickup_map", self.h, _pf(K), _pf(S), _pf(V), AOS)
+        return K, S, V
+
+    def setPickupRate(self, r: float) -> None:
+        call("pnt_brush_set_pickup_rate", self.h, float(r))
+
+    def getPickupRate(self) -> float:
+        v = C.c_float()
+        call("pnt_brush_get_pickup_rate", self.h, C.byref(v))
+        return v.value
+
+    def setDepositionRate(self, r: float) -> None:
+        call("pnt_brush_set_deposition_rate", self.h, float(r))
+
+    def getDepositionRate(self) -> float:
+        v = C.c_float()
+        call("pnt_brush_get_deposition_rate", self.h, C.byref(v))
+        return v.value
+
+    def setUseSnapshotBuffer(self, use: bool) -> None:
+        call("pnt_brush_set_use_snapshot", self.h, int(bool(use)))
+
+    def getUseSnapshotBuffer(self) -> bool:
+        v = C.c_int()
+        call("pnt_brush_get_use_snapshot", self.h, C.byref(v))
+        return bool(v.value)
+
+    def dip(self, paint) -> None:
+        K, S = paint
+        K, S = _f32(K, (3,)), _f32(S, (3,))
+        call("pnt_brush_dip", self.h, _pf(K), _pf(S))
+
+    def clean(self) -> None:
+        call("pnt_brush_clean", self.h)
+
+    def updateSnapshot(self, canvas: Canvas) -> None:
+        call("pnt_brush_update_snapshot", self.h, canvas.h)
+
+    def imprint(self, center, theta: float, canvas: Canvas) -> None:
+        call("pnt_imprint", self.h, canvas.h, float(center[0]), float(center[1]), float(theta))
+
+    def imprintStamps(self, stamps, canvas: Canvas, mode: int = _lib.EXEC_PLAIN) -> None:
+        arr = stamps_to_c(stamps)
+        call("pnt_imprint_stamps", self.h, canvas.h, arr, len(arr), int(mode))
+
+    def paintStroke(self, path, canvas: Canvas) -> None:
+        p = np.ascontiguousarray(path, dtype=np.float64).reshape(-1, 2)
+        call("pnt_paint_stroke", self.h, canvas.h, p.ctypes.data_as(_dp), len(p))
+
+
+def stamps_to_c(stamps):
+    """A workload.STAMP_DTYPE array (or a C array already) -> PntStamp array."""
+    if isinstance(stamps, C.Array):
+        return stamps
+    n = len(stamps)
+    arr = (PntStamp * n)()
+    if n:
+        buf = np.frombuffer(arr, dtype=np.dtype([("cx", "f8"), ("cy", "f8"), ("theta", "f8"),
+                                                   ("K", "f4", 3), ("S", "f4", 3),
+                                                   ("flags", "u4"), ("reserved", "i4")]))
+        for f in ("cx", "cy", "theta", "K", "S", "flags"):
+            buf[f] = stamps[f]
+    return arr
+
+
+class PaintMixer:
+    """mixSinglePaint / mixed over a base palette [(K[3], S[3]), ...]."""
+
+    def __init__(self, palette):
+        self.setUnderlyingPalette(palette)
+
+    def setUnderlyingPalette(self, palette) -> None:
+        self._K = np.ascontiguousarray([p[0] for p in palette], dtype=np.float64).reshape(-1, 3)
+        self._S = np.ascontiguousarray([p[1] for p in palette], dtype=np.float64).reshape(-1, 3)
+
+    def getUnderlyingPalette(self):
+        return [(self._K[i].copy(), self._S[i].copy()) for i in range(len(self._K))]
+
+    def mixSinglePaint(self, weights):
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        K, S = np.empty(3), np.empty(3)
+        call("pnt_mix_single", len(self._K), self._K.ctypes.data_as(_dp),
+             self._S.ctypes.data_as(_dp), len(w), w.ctypes.data_as(_dp),
+             K.ctypes.data_as(_dp), S.ctypes.data_as(_dp))
+        return K, S
+
+    @staticmethod
+    def mixed(paint, volume, other, other_volume):
+        K1, S1 = (np.ascontiguousarray(x, dtype=np.float64) for x in paint)
+        K2, S2 = (np.ascontiguousarray(x, dtype=np.float64) for x in other)
+        K, S = np.empty(3), np.empty(3)
+        call("pnt_mixed", K1.ctypes.data_as(_dp), S1.ctypes.data_as(_dp), float(volume),
+             K2.ctypes.data_as(_dp), S2.ctypes.data_as(_dp), float(other_volume),
+             K.ctypes.data_as(_dp), S.ctypes.data_as(_dp))
+        return K, S
+
+
+def ComputeReflectance(K, S, R0, d, ctx: Context | None = None) -> np.ndarray:
+    """core/KubelkaMunk.hxx:25-83 over arrays: K, S, R0 (..., 3), d (...)."""
+    ctx = ctx or default_context()
+    K, S, R0 = _f32(K), _f32(S), _f32(R0)
+    d = _f32(d)
+    n = d.size
+    if K.size != 3 * n or S.size != 3 * n or R0.size != 3 * n:
+        raise ValueError("K, S, R0 must hold 3 values per thickness")
+    out = np.empty_like(R0)
+    call("pnt_compute_reflectance", ctx.h, n, _pf(K), _pf(S), _pf(R0), _pf(d), _pf(out))
+    return out
+
+
+__all__ = ["Context", "PaintLayer", "Canvas", "Renderer", "FootprintBrush", "PaintMixer",
+           "ComputeReflectance", "PaintyError", "InvalidArgument", "default_context",
+           "stamps_to_c", "PLANAR", "AOS"]

@@ -1,0 +1,239 @@
+"""Seeded synthetic workloads for the KM composite + imprint-stamp path.
+
+Every array is generated in float64 from a splitmix64 stream and then rounded
+to float32; the float32 values are the canonical inputs, so the device path
+and a float64 CPU computation see bit-identical data.
+
+Distributions (one sub-stream per array):
+  K  = exp(U(ln 0.01, ln 10)) per channel      (absorption)
+  S  = exp(U(ln 0.1,  ln 5))  per channel      (scattering)
+  V  = 0 with probability 0.3, else U(0, 0.5)  (wet / dry layer thickness)
+  R0 = U(0.7, 0.95) per channel                (substrate reflectance)
+
+Stamps follow seeded cubic Bezier strokes sampled at 2.0 px arc length, with
+theta = atan2(B'(t)); every stroke starts with a dip of a freshly drawn paint.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+_M64 = (1 << 64) - 1
+_GOLDEN = 0x9E3779B97F4A7C15
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+class Stream:
+    """splitmix64 counter stream: value i = mix64(base + (i+1) * golden)."""
+
+    def __init__(self, seed: int, sub: int):
+        base = ((seed & 0xFFFFFFFF) << 32) ^ ((sub * 0x632BE59BD9B4E019) & _M64)
+        self.base = int(_mix64(np.array([base], dtype=np.uint64))[0])
+        self.pos = 0
+
+    def u64(self, n: int) -> np.ndarray:
+        with np.errstate(over="ignore"):
+            idx = np.arange(self.pos + 1, self.pos + n + 1, dtype=np.uint64)
+            z = np.uint64(self.base) + idx * np.uint64(_GOLDEN)
+            self.pos += n
+            return _mix64(z)
+
+    def uniform(self, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+        u = (self.u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+        return lo + (hi - lo) * u
+
+    def integers(self, n: int, lo: int, hi: int) -> np.ndarray:
+        """Uniform integers in [lo, hi] inclusive."""
+        u = self.uniform(n)
+        return np.minimum(lo + np.floor(u * (hi - lo + 1)).astype(np.int64), hi)
+
+
+def log_uniform(st: Stream, n: int, lo: float, hi: float) -> np.ndarray:
+    return np.exp(st.uniform(n, math.log(lo), math.log(hi)))
+
+
+@dataclasses.dataclass
+class Layer:
+    """A paint layer as float32 planes: K, S (rows, cols, 3) and V (rows, cols)."""
+
+    K: np.ndarray
+    S: np.ndarray
+    V: np.ndarray
+
+
+def paint_layer(rows: int, cols: int, seed: int, sub: int = 0) -> Layer:
+    n = rows * cols
+    K = log_uniform(Stream(seed, sub + 1), 3 * n, 0.01, 10.0)
+    S = log_uniform(Stream(seed, sub + 2), 3 * n, 0.1, 5.0)
+    st = Stream(seed, sub + 3)
+    zero = st.uniform(n) < 0.3
+    V = st.uniform(n, 0.0, 0.5)
+    V[zero] = 0.0
+    f = np.float32
+    return Layer(K.astype(f).reshape(rows, cols, 3), S.astype(f).reshape(rows, cols, 3),
+                 V.astype(f).reshape(rows, cols))
+
+
+def substrate(rows: int, cols: int, seed: int, sub: int = 10) -> np.ndarray:
+    R0 = Stream(seed, sub).uniform(3 * rows * cols, 0.7, 0.95)
+    return R0.astype(np.float32).reshape(rows, cols, 3)
+
+
+def paint(st: Stream):
+    """One brush paint: (K[3], S[3]) float32 from the K/S distributions."""
+    K = np.exp(st.uniform(3, math.log(0.01), math.log(10.0))).astype(np.float32)
+    S = np.exp(st.uniform(3, math.log(0.1), math.log(5.0))).astype(np.float32)
+    return K, S
+
+
+# ------------------------------------------------------------ footprints ---
+def gaussian_core(width: int, sigma: float) -> np.ndarray:
+    """width x width Gaussian, centre (width-1)/2, peak 1, float32."""
+    c = (width - 1) / 2.0
+    y, x = np.mgrid[0:width, 0:width].astype(np.float64)
+    g = np.exp(-((x - c) ** 2 + (y - c) ** 2) / (2.0 * sigma * sigma))
+    return g.astype(np.float32)
+
+
+def value_noise(width: int, st: Stream, cell: float = 16.0) -> np.ndarray:
+    """Smooth value noise in [0, 1): random lattice values, smoothstep-bilinear."""
+    g = int(math.ceil(width / cell)) + 2
+    lat = st.uniform(g * g).reshape(g, g)
+    t = np.arange(width, dtype=np.float64) / cell
+    i0 = np.floor(t).astype(np.int64)
+    f = t - i0
+    f = f * f * (3.0 - 2.0 * f)
+    a = lat[i0][:, i0]
+    b = lat[i0][:, i0 + 1]
+    c = lat[i0 + 1][:, i0]
+    d = lat[i0 + 1][:, i0 + 1]
+    fx = f[None, :]
+    fy = f[:, None]
+    return (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy
+
+
+def noisy_core(width: int, st: Stream) -> np.ndarray:
+    """clamp(G_{sigma=width/4} * (0.75 + 0.5 * noise), 0, 1), float32."""
+    g = gaussian_core(width, width / 4.0).astype(np.float64)
+    n = value_noise(width, st)
+    return np.clip(g * (0.75 + 0.5 * n), 0.0, 1.0).astype(np.float32)
+
+
+# ---------------------------------------------------------------- strokes --
+STAMP_DTYPE = np.dtype([("cx", "f8"), ("cy", "f8"), ("theta", "f8"), ("K", "f4", 3),
+                        ("S", "f4", 3), ("flags", "u4"), ("brush", "i4")])
+DIP = 1
+DRY_AFTER = 2
+
+
+def bezier_samples(P: np.ndarray, spacing: float, max_count: int):
+    """Points at `spacing` px arc length along a cubic Bezier with control points P (4,2)."""
+    t = np.linspace(0.0, 1.0, 4097)
+    mt = 1.0 - t
+    B = (mt ** 3)[:, None] * P[0] + (3 * mt * mt * t)[:, None] * P[1] + \
+        (3 * mt * t * t)[:, None] * P[2] + (t ** 3)[:, None] * P[3]
+    seg = np.hypot(*(B[1:] - B[:-1]).T)
+    s = np.concatenate([[0.0], np.cumsum(seg)])
+    n = min(int(s[-1] // spacing) + 1, max_count)
+    targets = np.arange(n) * spacing
+    tt = np.interp(targets, s, t)
+    mt = 1.0 - tt
+    pos = (mt ** 3)[:, None] * P[0] + (3 * mt * mt * tt)[:, None] * P[1] + \
+        (3 * mt * tt * tt)[:, None] * P[2] + (tt ** 3)[:, None] * P[3]
+    d = (3 * mt * mt)[:, None] * (P[1] - P[0]) + (6 * mt * tt)[:, None] * (P[2] - P[1]) + \
+        (3 * tt * tt)[:, None] * (P[3] - P[2])
+    theta = np.arctan2(d[:, 1], d[:, 0])
+    return pos, theta
+
+
+def strokes(rows: int, cols: int, n_stamps: int, seed: int, stamps_per_stroke=(20, 60),
+            reach: float = 0.25, spacing: float = 2.0, sub: int = 20) -> np.ndarray:
+    """Stamp list: Bezier strokes (first control point uniform in the canvas, the
+    others within +-reach*max(rows, cols) of it), dip at each stroke start."""
+    st = Stream(seed, sub)
+    out = np.zeros(n_stamps, dtype=STAMP_DTYPE)
+    k = 0
+    stroke = 0
+    span = reach * max(rows, cols)
+    while k < n_stamps:
+        want = int(st.integers(1, stamps_per_stroke[0], stamps_per_stroke[1])[0])
+        p0 = st.uniform(2) * np.array([cols, rows])
+        P = np.vstack([p0, p0 + st.uniform(6, -span, span).reshape(3, 2)])
+        pos, theta = bezier_samples(P, spacing, min(want, n_stamps - k))
+        K, S = paint(st)
+        m = len(pos)
+        out["cx"][k:k + m] = pos[:, 0]
+        out["cy"][k:k + m] = pos[:, 1]
+        out["theta"][k:k + m] = theta
+        out["K"][k:k + m] = K
+        out["S"][k:k + m] = S
+        out["brush"][k:k + m] = stroke
+        out["flags"][k] |= DIP
+        k += m
+        stroke += 1
+    return out
+
+
+# ------------------------------------------------------------------ configs --
+@dataclasses.dataclass
+class Workload:
+    name: str
+    rows: int
+    cols: int
+    dry: Layer            # dry layer, baked onto the substrate with composeOnto
+    wet: Layer            # initial wet layer of the canvas
+    R0: np.ndarray        # substrate
+    stamps: np.ndarray    # STAMP_DTYPE
+    cores: list           # footprint core per brush id (width x width float32)
+    radii: list           # radius per brush id
+    pickup_rate: float = 0.9
+    deposition_rate: float = 0.05
+
+
+def cfg1(pickup: bool = False, seed: int = 1, size: int = 1024, n_stamps: int = 200) -> Workload:
+    """1024x1024 canvas, 200 stamps of a 65x65 Gaussian footprint (sigma 16, radius 32)."""
+    st = strokes(size, size, n_stamps, seed, stamps_per_stroke=(20, 60))
+    st["brush"][:] = 0
+    core = gaussian_core(65, 16.0)
+    return Workload("cfg1-%d-%dstamps-%s" % (size, n_stamps, "pickup" if pickup else "deposit"),
+                    size, size, paint_layer(size, size, seed, 100), paint_layer(size, size, seed, 0),
+                    substrate(size, size, seed), st, [core], [32.0],
+                    pickup_rate=0.9 if pickup else 0.0)
+
+
+def cfg2(seed: int = 2, size: int = 4096, n_stamps: int = 2000, dry_every: int = 500,
+         foot=(129, 257)) -> Workload:
+    """4096x4096 canvas, 2000 stamps, per-stroke odd footprint in [129, 257]
+    (Gaussian sigma=F/4 times value noise), pickup on, dry every 500 stamps."""
+    st = strokes(size, size, n_stamps, seed, stamps_per_stroke=(50, 150), reach=0.08)
+    fs = Stream(seed, 40)
+    cores, radii = [], []
+    for b in range(int(st["brush"].max()) + 1):
+        F = int(fs.integers(1, foot[0] // 2, foot[1] // 2)[0]) * 2 + 1
+        cores.append(noisy_core(F, fs))
+        radii.append((F - 1) / 2.0)
+    if dry_every:
+        st["flags"][dry_every - 1::dry_every] |= DRY_AFTER
+    return Workload("cfg2-%d-%dstamps" % (size, n_stamps), size, size,
+                    paint_layer(size, size, seed, 100), paint_layer(size, size, seed, 0),
+                    substrate(size, size, seed), st, cores, radii)
+
+
+def small(rows: int, cols: int, n_stamps: int, radius: int, seed: int, pickup: bool = True,
+          noisy: bool = False, reach: float = 0.5) -> Workload:
+    """Small seeded case for parity tests."""
+    w = 2 * radius + 1
+    st = strokes(rows, cols, n_stamps, seed, stamps_per_stroke=(3, 12), reach=reach)
+    st["brush"][:] = 0
+    fs = Stream(seed, 40)
+    core = noisy_core(w, fs) if noisy else gaussian_core(w, w / 4.0)
+    return Workload("small-%dx%d" % (rows, cols), rows, cols, paint_layer(rows, cols, seed, 100),
+                    paint_layer(rows, cols, seed, 0), substrate(rows, cols, seed), st, [core],
+                    [float(radius)], pickup_rate=0.9 if pickup else 0.0)

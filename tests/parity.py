@@ -1,0 +1,107 @@
+"""Runs one workload through the oracle (FP64) and through the HIP library
+(FP32, via the C ABI), and measures the difference.
+
+Tolerance: 1e-5 relative per channel, with an absolute
+floor of 1e-9 for values that are exactly 0 in the oracle (cleared layers).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+REL_TOL = 1e-5
+ABS_FLOOR = 1e-9
+
+
+def rel_err(gpu, ref) -> np.ndarray:
+    gpu = np.asarray(gpu, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    return np.abs(gpu - ref) / np.maximum(np.abs(ref), ABS_FLOOR / REL_TOL)
+
+
+def report(name: str, gpu, ref) -> dict:
+    e = rel_err(gpu, ref)
+    out = {"name": name, "max_rel": float(e.max()) if e.size else 0.0,
+           "n_over": int((e > REL_TOL).sum()), "n": int(e.size)}
+    print("parity %-28s max_rel=%.3e over_tol=%d/%d" % (name, out["max_rel"], out["n_over"],
+                                                         out["n"]), flush=True)
+    return out
+
+
+def assert_close(name: str, gpu, ref, max_over: int = 0) -> dict:
+    r = report(name, gpu, ref)
+    assert r["n_over"] <= max_over, r
+    return r
+
+
+def _groups(stamps):
+    """Consecutive runs of stamps with the same brush id."""
+    if len(stamps) == 0:
+        return []
+    ids = stamps["brush"]
+    cut = np.flatnonzero(np.diff(ids)) + 1
+    starts = np.concatenate([[0], cut])
+    ends = np.concatenate([cut, [len(stamps)]])
+    return [(int(ids[s]), stamps[s:e]) for s, e in zip(starts, ends)]
+
+
+def run_oracle(wl, stamps=None, bake_dry=True, use_snapshot=True):
+    from oracle import oracle as O
+    stamps = wl.stamps if stamps is None else stamps
+    R0 = wl.R0.astype(np.float64)
+    if bake_dry:
+        O.compose_onto(wl.dry.K, wl.dry.S, wl.dry.V, R0)
+    cv = O.Canvas.from_arrays(wl.wet.K.astype(np.float64), wl.wet.S.astype(np.float64),
+                              wl.wet.V.astype(np.float64), R0)
+    brush = None
+    for bid, grp in _groups(stamps):
+        core = wl.cores[bid].astype(np.float64)
+        if brush is None:
+            brush = O.Brush(wl.radii[bid], core)
+            brush.set_rates(wl.pickup_rate, wl.deposition_rate)
+            brush.set_use_snapshot(use_snapshot)
+        else:
+            brush.set_radius(wl.radii[bid], core)
+        brush.run(cv, grp)
+    return cv, brush
+
+
+def run_gpu(wl, stamps=None, bake_dry=True, use_snapshot=True, mode=0, ctx=None):
+    from painty_amd import hip as H
+    stamps = wl.stamps if stamps is None else stamps
+    cv = H.Canvas(wl.rows, wl.cols, ctx)
+    cv.getPaintLayer().setBuffers(wl.wet.K, wl.wet.S, wl.wet.V)
+    cv.setR0(wl.R0)
+    if bake_dry:
+        dry = H.PaintLayer(wl.rows, wl.cols, ctx)
+        dry.setBuffers(wl.dry.K, wl.dry.S, wl.dry.V)
+        dry.composeOntoCanvas(cv)
+    brush = None
+    for bid, grp in _groups(stamps):
+        if brush is None:
+            brush = H.FootprintBrush(wl.radii[bid], wl.cores[bid], ctx)
+            brush.setPickupRate(wl.pickup_rate)
+            brush.setDepositionRate(wl.deposition_rate)
+            brush.setUseSnapshotBuffer(use_snapshot)
+        else:
+            brush.setRadius(wl.radii[bid], wl.cores[bid])
+        brush.imprintStamps(grp, cv, mode)
+    return cv, brush
+
+
+def compare_canvas(tag, gcv, ocv, max_over=0, pickup=None):
+    """Compare wet layer K, S, V, R0, h and the final composite."""
+    from painty_amd import hip as H
+    L = gcv.getPaintLayer()
+    res = [assert_close(tag + ".K", L.getK_buffer(), ocv.K, max_over),
+           assert_close(tag + ".S", L.getS_buffer(), ocv.S, max_over),
+           assert_close(tag + ".V", L.getV_buffer(), ocv.V, max_over),
+           assert_close(tag + ".R0", gcv.getR0(), ocv.R0, max_over),
+           assert_close(tag + ".h", gcv.get_h(), ocv.h, max_over),
+           assert_close(tag + ".R", H.Renderer.compose(gcv), ocv.compose(), max_over)]
+    if pickup is not None:
+        gb, ob = pickup
+        gK, gS, gV = gb.getPickupMap()
+        oK, oS, oV = ob.pickup_map()
+        res += [assert_close(tag + ".pickup.K", gK, oK, max_over),
+                assert_close(tag + ".pickup.V", gV, oV, max_over)]
+    return res

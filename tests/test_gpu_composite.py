@@ -1,0 +1,186 @@
+"""GPU parity of the KM composite and drying kernels against the FP64 oracle.
+
+Tolerance: 1e-5 relative per channel (tests/parity.py), the same seeded
+float32 inputs on both sides.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from painty_amd import workload as W
+from tests.parity import assert_close
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.json")))
+
+
+def test_km_reference_golden_on_gpu(hip):
+    g = GOLD["KubelkaMunk.Reflectance"]
+    f = np.float32
+    R = hip.ComputeReflectance(np.array([g["K"]], f), np.array([g["S"]], f),
+                               np.array([g["R0"]], f), np.array([g["d"]], f))
+    np.testing.assert_allclose(R[0], g["R"], atol=g["eps"], rtol=0)
+    R = hip.ComputeReflectance(np.array([g["K"]], f), np.array([g["S"]], f),
+                               np.array([g["R0"]], f), np.array([0.0], f))
+    assert R[0, 0] == np.float32(g["R0"][0])
+    R = hip.ComputeReflectance(np.array([g["K"]], f), np.zeros((1, 3), f),
+                               np.array([g["R0"]], f), np.array([g["d"]], f))
+    assert abs(R[0, 0] - g["S_zero_R_channel0"]) < g["eps"]
+
+
+def test_km_random_and_edge_values(hip, oracle):
+    n = 1 << 18
+    L = W.paint_layer(1, n, 31)
+    K, S = L.K.reshape(n, 3), L.S.reshape(n, 3)
+    st = W.Stream(31, 7)
+    R0 = st.uniform(3 * n, 0.0, 1.0).astype(np.float32).reshape(n, 3)
+    d = st.uniform(n, 0.0, 0.5).astype(np.float32)
+    # thin layers down to the FP64 threshold, thick layers past bSd = 20,
+    # d = 0, and S = 0 (clamped to 1e-11)
+    d[:2000] = np.logspace(-11.6, 0, 2000).astype(np.float32)
+    d[2000:2100] = np.linspace(5.0, 400.0, 100).astype(np.float32)
+    d[2100:2200] = 0.0
+    S[2200:2300] = 0.0
+    d[2300] = np.float32(2.220446049250313e-12)   # == 1e4 * DBL_EPSILON: not thin
+    d[2301] = np.nextafter(np.float32(2.220446049250313e-12), np.float32(0))  # thin
+    R = hip.ComputeReflectance(K, S, R0, d)
+    ref = np.empty((n, 3))
+    ref64 = oracle.compose(K.astype(np.float64), S.astype(np.float64), d.astype(np.float64),
+                           R0.astype(np.float64))
+    ref[:] = ref64
+    assert_close("ComputeReflectance", R, ref)
+    np.testing.assert_array_equal(R[2100:2200], R0[2100:2200])
+    np.testing.assert_array_equal(R[2301], R0[2301])
+
+
+@pytest.mark.parametrize("rows,cols", [(257, 263), (64, 64), (1, 7), (1024, 1024)])
+def test_compose_canvas(hip, oracle, rows, cols):
+    wet = W.paint_layer(rows, cols, 5)
+    R0 = W.substrate(rows, cols, 5)
+    cv = hip.Canvas(rows, cols)
+    cv.getPaintLayer().setBuffers(wet.K, wet.S, wet.V)
+    cv.setR0(R0)
+    R = hip.Renderer.compose(cv)
+    ref = oracle.compose(wet.K, wet.S, wet.V, R0)
+    assert_close("compose(canvas) %dx%d" % (rows, cols), R, ref)
+    # compose(layer, R0) on host images
+    layer = hip.PaintLayer(rows, cols)
+    layer.setBuffers(wet.K, wet.S, wet.V)
+    assert_close("compose(layer,R0)", hip.Renderer.compose(layer, R0), ref)
+
+
+def test_compose_onto(hip, oracle):
+    rows, cols = 129, 77
+    dry = W.paint_layer(rows, cols, 6, 100)
+    R0 = W.substrate(rows, cols, 6)
+    layer = hip.PaintLayer(rows, cols)
+    layer.setBuffers(dry.K, dry.S, dry.V)
+    got = R0.copy()
+    layer.composeOnto(got)
+    ref = R0.astype(np.float64)
+    oracle.compose_onto(dry.K, dry.S, dry.V, ref)
+    assert_close("composeOnto(host)", got, ref)
+    cv = hip.Canvas(rows, cols)
+    cv.setR0(R0)
+    layer.composeOntoCanvas(cv)
+    assert_close("composeOnto(canvas)", cv.getR0(), ref)
+    # size mismatch: R0 becomes ones of the layer's size, then composed
+    out = layer.composeOnto(np.zeros((3, 3, 3), np.float32))
+    ones = np.ones((rows, cols, 3))
+    oracle.compose_onto(dry.K, dry.S, dry.V, ones)
+    assert_close("composeOnto(resize)", out, ones)
+
+
+def test_two_layer_compose_device_planes(hip, oracle):
+    """pnt_dev_compose2: wet over (dry over R0) in one pass, R0 untouched; also
+    the unaligned (scalar) path through planes offset by one float."""
+    import ctypes as C
+    from painty_amd import _lib
+    rows, cols = 300, 301
+    dry, wet = W.paint_layer(rows, cols, 8, 100), W.paint_layer(rows, cols, 8, 0)
+    R0 = W.substrate(rows, cols, 8)
+    a, b, out = hip.Canvas(rows, cols), hip.Canvas(rows, cols), hip.Canvas(rows, cols)
+    a.getPaintLayer().setBuffers(dry.K, dry.S, dry.V)
+    a.setR0(R0)
+    b.getPaintLayer().setBuffers(wet.K, wet.S, wet.V)
+    La, Ra, _ = a.device_planes()
+    Lb, _, _ = b.device_planes()
+    _, Ro, _ = out.device_planes()
+    n = rows * cols
+    ctx = hip.default_context()
+    _lib.call("pnt_dev_compose2", ctx.h, n, _lib.DevLayer((C.c_void_p * 7)(*La)),
+              _lib.DevLayer((C.c_void_p * 7)(*Lb)), _lib.DevRgb((C.c_void_p * 3)(*Ra)),
+              _lib.DevRgb((C.c_void_p * 3)(*Ro)), None)
+    ref = R0.astype(np.float64)
+    oracle.compose_onto(dry.K, dry.S, dry.V, ref)
+    ref2 = oracle.compose(wet.K, wet.S, wet.V, ref)
+    assert_close("compose2(dev)", out.getR0(), ref2)
+    np.testing.assert_array_equal(a.getR0(), R0)
+    # unaligned planes: skip the first pixel of every plane
+    sh = lambda ps: [p + 4 for p in ps]  # noqa: E731
+    _lib.call("pnt_dev_compose", ctx.h, n - 1, _lib.DevLayer((C.c_void_p * 7)(*sh(Lb))),
+              _lib.DevRgb((C.c_void_p * 3)(*sh(Ra))), _lib.DevRgb((C.c_void_p * 3)(*sh(Ro))),
+              None)
+    got = out.getR0().reshape(-1, 3)[1:]
+    ref1 = oracle.compose(wet.K, wet.S, wet.V, R0).reshape(-1, 3)[1:]
+    assert_close("compose(dev,unaligned)", got, ref1)
+
+
+def test_dry_canvas(hip, oracle):
+    rows, cols = 203, 111
+    wet = W.paint_layer(rows, cols, 9)
+    R0 = W.substrate(rows, cols, 9)
+    h0 = W.Stream(9, 77).uniform(rows * cols, 0, 2).astype(np.float32).reshape(rows, cols)
+    cv = hip.Canvas(rows, cols)
+    cv.getPaintLayer().setBuffers(wet.K, wet.S, wet.V)
+    cv.setR0(R0)
+    cv.set_h(h0)
+    cv.dryCanvas()
+    K, S, V = wet.K.astype(np.float64), wet.S.astype(np.float64), wet.V.astype(np.float64)
+    rR0, rh = R0.astype(np.float64), h0.astype(np.float64)
+    oracle.dry_canvas(K, S, V, rR0, rh)
+    assert_close("dry.R0", cv.getR0(), rR0)
+    assert_close("dry.h", cv.get_h(), rh)
+    L = cv.getPaintLayer()
+    assert not L.getK_buffer().any() and not L.getS_buffer().any() and not L.getV_buffer().any()
+
+
+def test_dry_step(hip, oracle):
+    rows, cols = 64, 65
+    wet = W.paint_layer(rows, cols, 10)
+    R0 = W.substrate(rows, cols, 10)
+    cv = hip.Canvas(rows, cols)
+    cv.getPaintLayer().setBuffers(wet.K, wet.S, wet.V)
+    cv.setR0(R0)
+    cv.dryStep(0.1)
+    V, rR0 = wet.V.astype(np.float64), R0.astype(np.float64)
+    oracle.dry_step(wet.K, wet.S, V, rR0, float(np.float32(0.1)))
+    assert_close("dryStep.V", cv.getPaintLayer().getV_buffer(), V)
+    assert_close("dryStep.R0", cv.getR0(), rR0)
+
+
+def test_canvas_lifecycle(hip):
+    cv = hip.Canvas(40, 30)
+    np.testing.assert_array_equal(cv.getR0(), np.ones((40, 30, 3), np.float32))
+    assert not cv.get_h().any()
+    bg = W.substrate(40, 30, 3)
+    cv.getPaintLayer().setBuffers(V=np.ones((40, 30), np.float32))
+    cv.setBackground(bg)  # clear() then R0 = background
+    np.testing.assert_array_equal(cv.getR0(), bg)
+    assert not cv.getPaintLayer().getV_buffer().any()
+    np.testing.assert_array_equal(cv.getReflectanceLayerDry(), bg)
+    cv.setDryingTime(5000)
+    assert cv.getDryingTime() == 5000
+    layer = hip.PaintLayer(5, 6)
+    assert (layer.getRows(), layer.getCols()) == (5, 6)
+    layer.set(2, 3, [1, 2, 3], [4, 5, 6], 0.5)
+    assert layer.getK_buffer()[2, 3].tolist() == [1, 2, 3]
+    other = hip.PaintLayer(1, 1)
+    layer.copyTo(other)
+    assert other.shape == (5, 6) and other.getV_buffer()[2, 3] == np.float32(0.5)
+    layer.clear()
+    assert not layer.getK_buffer().any()

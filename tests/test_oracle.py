@@ -1,0 +1,385 @@
+"""Pins the FP64 CPU oracle (CPU only).
+
+Each oracle function is checked against what does not depend on the oracle's
+own code: the reference's golden values, closed forms of the Kubelka-Munk
+model, an independent restatement of the stamp loop (tests/brute_imprint.py),
+and invariants chosen so that a dropped term, a wrong sign or a transposed
+index fails.
+"""
+import json
+import math
+import os
+
+import mpmath
+import numpy as np
+import pytest
+
+from painty_amd import workload as W
+from tests.brute_imprint import BruteBrush, BruteCanvas
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.json")))
+DATA = os.path.join(os.path.dirname(os.path.dirname(__file__)), "painty_amd", "data")
+
+
+# ------------------------------------------------------------- goldens -----
+def test_km_reference_golden(oracle):
+    g = GOLD["KubelkaMunk.Reflectance"]
+    R = oracle.compute_reflectance(g["K"], g["S"], g["R0"], g["d"])
+    np.testing.assert_allclose(R, g["R"], atol=g["eps"], rtol=0)
+    R = oracle.compute_reflectance(g["K"], g["S"], g["R0"], 0.0)
+    assert abs(R[0] - g["d0_R0_channel0"]) < g["eps"]
+    R = oracle.compute_reflectance(g["K"], g["S_zero"], g["R0"], g["d"])
+    assert abs(R[0] - g["S_zero_R_channel0"]) < g["eps"]
+
+
+def test_coth_reference_golden(oracle):
+    g = GOLD["MathTest.coth"]
+    assert oracle.coth(0.0) == math.inf
+    for x, want in g["cases"]:
+        assert abs(oracle.coth(x) - want) < g["eps"]
+
+
+def test_catmull_rom_reference_golden(oracle):
+    g = GOLD["SplineTest.BaseFunctions"]
+    assert abs(oracle.catmull_rom(*g["catmull_rom_args"]) - g["catmull_rom"]) < g["eps"]
+
+
+# -------------------------------------------------- KM closed forms --------
+def _mp_reflectance(K, S, R0, d):
+    """Kubelka's hyperbolic solution in 50-digit arithmetic, written from the
+    textbook form R = (1 - R0 (a - b coth(b S d))) / (a - R0 + b coth(b S d))."""
+    mpmath.mp.dps = 50
+    K, S, R0, d = (mpmath.mpf(float(v)) for v in (K, S, R0, d))
+    a = 1 + K / S
+    b = mpmath.sqrt(a * a - 1)
+    c = mpmath.coth(b * S * d)
+    return float((1 - R0 * (a - b * c)) / (a - R0 + b * c))
+
+
+def test_km_matches_high_precision(oracle):
+    st = W.Stream(11, 1)
+    for _ in range(200):
+        K = np.exp(st.uniform(3, math.log(0.01), math.log(10)))
+        S = np.exp(st.uniform(3, math.log(0.1), math.log(5)))
+        R0 = st.uniform(3, 0.0, 1.0)
+        d = float(st.uniform(1, 1e-4, 2.0)[0])
+        R = oracle.compute_reflectance(K, S, R0, d)
+        for c in range(3):
+            want = _mp_reflectance(K[c], S[c], R0[c], d)
+            # bSd < 20 keeps cosh/sinh in range; the reference is ~1e-13 accurate here
+            assert abs(R[c] - want) <= 1e-11 * abs(want) + 1e-14
+
+
+def test_km_infinite_thickness_is_r_infinity(oracle):
+    # d -> inf: R -> R_inf = a - b = 1 / (a + b), independent of R0
+    K, S = np.array([0.3, 2.0, 0.05]), np.array([0.7, 0.4, 1.9])
+    a = 1 + K / S
+    r_inf = a - np.sqrt(a * a - 1)
+    for R0 in ([0.9, 0.1, 0.5], [0.0, 1.0, 0.3]):
+        np.testing.assert_allclose(oracle.compute_reflectance(K, S, R0, 1e3), r_inf, rtol=1e-12)
+    # R0 = R_inf is a fixed point for every thickness
+    for d in (0.01, 0.3, 5.0):
+        np.testing.assert_allclose(oracle.compute_reflectance(K, S, r_inf, d), r_inf, rtol=1e-12)
+
+
+def test_km_zero_scattering_is_beer_lambert(oracle):
+    # S -> 0 (clamped to 1e-11): no scattering, light passes the layer twice
+    K, R0, d = np.array([0.2, 1.3, 0.04]), np.array([0.65, 0.2, 0.9]), 0.5
+    R = oracle.compute_reflectance(K, [0.0, 0.0, 0.0], R0, d)
+    np.testing.assert_allclose(R, R0 * np.exp(-2 * K * d), rtol=1e-6)
+
+
+def test_km_no_absorption_limit(oracle):
+    # K -> 0: R = (R0 + (1 - R0) S d) / (1 + (1 - R0) S d)
+    S, R0, d = np.array([0.4, 2.0, 0.1]), np.array([0.3, 0.8, 0.05]), 0.7
+    R = oracle.compute_reflectance([1e-10] * 3, S, R0, d)
+    want = (R0 + (1 - R0) * S * d) / (1 + (1 - R0) * S * d)
+    np.testing.assert_allclose(R, want, rtol=1e-7)
+
+
+def test_km_layers_compose_additively(oracle):
+    # two coats of one paint = one coat of the summed thickness
+    K, S, R0 = [0.8, 0.05, 3.0], [0.3, 1.5, 0.2], [0.7, 0.6, 0.95]
+    for d1, d2 in ((0.1, 0.2), (0.01, 1.3), (0.5, 0.5)):
+        two = oracle.compute_reflectance(K, S, oracle.compute_reflectance(K, S, R0, d1), d2)
+        np.testing.assert_allclose(two, oracle.compute_reflectance(K, S, R0, d1 + d2), rtol=1e-12)
+
+
+def test_km_thin_layer_threshold(oracle):
+    thr = np.finfo(np.float64).eps * 1e4
+    R0 = [0.3, 0.4, 0.5]
+    assert np.array_equal(oracle.compute_reflectance([1, 1, 1], [1, 1, 1], R0, thr * 0.999), R0)
+    assert not np.array_equal(oracle.compute_reflectance([1, 1, 1], [1, 1, 1], R0, 1e-3), R0)
+
+
+def test_glsl_variant_returns_r0_for_small_bsd(oracle):
+    R0 = [0.3, 0.4, 0.5]
+    assert np.array_equal(oracle.compute_reflectance_glsl([1, 1, 1], [1, 1, 1], R0, 1e-6), R0)
+    np.testing.assert_allclose(oracle.compute_reflectance_glsl([1, 2, 3], [1, 1, 1], R0, 0.3),
+                               oracle.compute_reflectance([1, 2, 3], [1, 1, 1], R0, 0.3),
+                               rtol=1e-14)
+
+
+def test_scattering_absorption_roundtrip(oracle):
+    # Curtis et al. inversion: K, S from reflectance over black and white must
+    # reproduce those reflectances for a unit-thickness layer
+    Rb, Rw = np.array([0.05, 0.1, 0.2]), np.array([0.6, 0.7, 0.8])
+    K, S = oracle.compute_scattering_absorption(Rb, Rw)
+    np.testing.assert_allclose(oracle.compute_reflectance(K, S, [0, 0, 0], 1.0), Rb, rtol=1e-9)
+    np.testing.assert_allclose(oracle.compute_reflectance(K, S, [1, 1, 1], 1.0), Rw, rtol=1e-9)
+    with pytest.raises(ValueError):
+        oracle.compute_scattering_absorption([0.7, 0.1, 0.1], [0.6, 0.7, 0.8])
+
+
+# --------------------------------------------------- compose / dry ---------
+def _layer(rows, cols, seed):
+    L = W.paint_layer(rows, cols, seed)
+    return L.K.astype(np.float64), L.S.astype(np.float64), L.V.astype(np.float64)
+
+
+def test_compose_and_compose_onto(oracle):
+    K, S, V = _layer(7, 9, 3)
+    R0 = W.substrate(7, 9, 3).astype(np.float64)
+    R = oracle.compose(K, S, V, R0)
+    for i in (0, 17, 62):
+        r, c = divmod(i, 9)
+        np.testing.assert_array_equal(R[r, c], oracle.compute_reflectance(K[r, c], S[r, c],
+                                                                           R0[r, c], V[r, c]))
+    R1 = R0.copy()
+    oracle.compose_onto(K, S, V, R1)
+    np.testing.assert_array_equal(R1, R)
+
+
+def test_dry_canvas_invariants(oracle):
+    K, S, V = _layer(6, 5, 4)
+    R0 = W.substrate(6, 5, 4).astype(np.float64)
+    h = np.full((6, 5), 0.25)
+    before = oracle.compose(K, S, V, R0)
+    V0 = V.copy()
+    oracle.dry_canvas(K, S, V, R0, h)
+    np.testing.assert_array_equal(R0, before)           # dried paint is baked into R0
+    np.testing.assert_array_equal(h, 0.25 + V0)         # height grows by the wet volume
+    assert not K.any() and not S.any() and not V.any()  # wet layer cleared
+
+
+def test_dry_step_partial(oracle):
+    K, S, V = _layer(4, 4, 5)
+    R0 = W.substrate(4, 4, 5).astype(np.float64)
+    V0, R00 = V.copy(), R0.copy()
+    oracle.dry_step(K, S, V, R0, 0.1)
+    np.testing.assert_allclose(V, np.maximum(V0 - 0.1, 0.0))
+    for r in range(4):
+        for c in range(4):
+            leave = V0[r, c] - V[r, c]
+            want = oracle.compute_reflectance_glsl(K[r, c], S[r, c], R00[r, c], leave) \
+                if abs(leave) > 1e-5 else R00[r, c]
+            np.testing.assert_array_equal(R0[r, c], want)
+
+
+# ---------------------------------------------------------- stamping -------
+def _run_both(rows, cols, radius, stamps, seed, snapshot=True, pickup=0.9, noisy=False,
+              vscale=1.0):
+    w = 2 * radius + 1
+    core = (W.noisy_core(w, W.Stream(seed, 9)) if noisy else W.gaussian_core(w, w / 4.0)).astype(
+        np.float64)
+    K, S, V = _layer(rows, cols, seed)
+    V = V * vscale
+    ob = oracle_mod().Brush(radius, core)
+    ob.set_use_snapshot(snapshot)
+    ob.set_rates(pickup, 0.05)
+    oc = oracle_mod().Canvas.from_arrays(K, S, V, np.ones((rows, cols, 3)))
+    bb = BruteBrush(radius, core.tolist())
+    bb.use_snapshot = snapshot
+    bb.pickup_rate = pickup
+    bc = BruteCanvas(K.tolist(), S.tolist(), V.tolist())
+    for (cx, cy, th, paint) in stamps:
+        if paint is not None:
+            ob.dip(*paint)
+            bb.dip(*paint)
+        ob.imprint(cx, cy, th, oc)
+        bb.imprint(cx, cy, th, bc)
+    return ob, oc, bb, bc
+
+
+def oracle_mod():
+    from oracle import oracle as O
+    return O
+
+
+PAINT_A = ([0.5, 0.2, 1.5], [0.8, 0.3, 0.6])
+PAINT_B = ([2.0, 0.05, 0.3], [0.2, 1.1, 0.9])
+
+STAMP_CASES = {
+    "theta0-integer": [(10.0, 9.0, 0.0, PAINT_A), (12.0, 9.0, 0.0, None)],
+    "rotated": [(12.3, 11.7, math.pi / 7, PAINT_A), (14.1, 12.9, math.pi / 4, None),
+                (15.6, 13.2, 1.3, None)],
+    "edge-fractional": [(0.4, 0.6, 0.9, PAINT_B), (1.7, 0.2, 1.1, None), (-2.5, 3.3, 2.0, None)],
+    "redip": [(8.0, 8.0, 0.3, PAINT_A), (10.5, 8.5, 0.35, PAINT_B), (12.5, 9.0, 0.4, None)],
+}
+
+
+@pytest.mark.parametrize("case", sorted(STAMP_CASES))
+@pytest.mark.parametrize("snapshot", [True, False])
+def test_imprint_matches_independent_restatement(oracle, case, snapshot):
+    ob, oc, bb, bc = _run_both(24, 28, 4, STAMP_CASES[case], seed=7, snapshot=snapshot)
+    K, S, V = bc.arrays()
+    np.testing.assert_allclose(oc.K, K, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(oc.S, S, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(oc.V, V, rtol=1e-13, atol=1e-15)
+    pK, pS, pV = ob.pickup_map()
+    for (r, c), v in bb.pick.items():
+        np.testing.assert_allclose(pK[r, c], v[0], rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(pV[r, c], v[2], rtol=1e-13, atol=1e-15)
+
+
+def test_imprint_pickup_threshold_cases(oracle):
+    # Thin wet paint (V in [0, 0.004)) puts 0.9*V*fh on both sides of the
+    # MinVolume = 0.001 pickup threshold (FootprintBrush.hxx:365).
+    stamps = [(11.3, 12.6, 0.7, PAINT_A), (13.0, 13.1, 0.8, None), (14.2, 12.0, 0.75, None)]
+    for snapshot in (True, False):
+        ob, oc, bb, bc = _run_both(26, 26, 5, stamps, seed=21, snapshot=snapshot, vscale=0.008)
+        K, S, V = bc.arrays()
+        np.testing.assert_allclose(oc.V, V, rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(oc.K, K, rtol=1e-13, atol=1e-15)
+        _, _, pV = ob.pickup_map()
+        bpV = np.array([[bb.pick[(r, c)][2] for c in range(ob.size_map)]
+                        for r in range(ob.size_map)])
+        np.testing.assert_allclose(pV, bpV, rtol=1e-13, atol=1e-15)
+        assert (pV > 0).any()
+
+
+def test_imprint_pickup_off_closed_form(oracle):
+    # pickupRate = 0 keeps the pickup map empty, so every visit blends the
+    # brush paint with weight fh: K = (V0 K0 + sum fh Kb) / (V0 + sum fh).
+    rows, cols, radius = 30, 30, 5
+    ob, oc, bb, bc = _run_both(rows, cols, radius,
+                               [(13.2, 14.7, 0.6, PAINT_A), (15.0, 15.5, 0.9, None)], seed=8,
+                               pickup=0.0)
+    K0, S0, V0 = _layer(rows, cols, 8)
+    fsum = np.zeros((rows, cols))
+    fp = ob.footprint()
+    hr = (ob.size_map - 1) // 2
+    for cx, cy, th in ((13.2, 14.7, 0.6), (15.0, 15.5, 0.9)):
+        ct, stt = math.cos(-th), math.sin(-th)
+        for row in range(-hr, hr + 1):
+            for col in range(-hr, hr + 1):
+                x, y = int(col + cx), int(row + cy)
+                mx = int(np.round(col * ct - row * stt + hr))  # no .5 ties at these angles
+                my = int(np.round(col * stt + row * ct + hr))
+                if 0 <= x < cols and 0 <= y < rows and 0 <= mx < ob.size_map and 0 <= my < ob.size_map:
+                    fsum[y, x] += fp[my, mx]
+    np.testing.assert_allclose(oc.V, V0 + fsum, rtol=1e-12)
+    Kb = np.array(PAINT_A[0])
+    want = (V0[..., None] * K0 + fsum[..., None] * Kb) / (V0 + fsum)[..., None]
+    m = (V0 + fsum) > 1e-3
+    np.testing.assert_allclose(oc.K[m], want[m], rtol=1e-10)
+
+
+def test_imprint_theta0_map_is_bijective(oracle):
+    # theta = 0 with an integer centre: window (col,row) -> map (col+hr,row+hr);
+    # one stamp on a blank canvas with pickup off deposits exactly the footprint.
+    rows = cols = 40
+    radius = 6
+    core = W.gaussian_core(2 * radius + 1, 3.0).astype(np.float64)
+    ob = oracle.Brush(radius, core)
+    ob.set_rates(0.0, 0.05)
+    oc = oracle.Canvas(rows, cols)
+    ob.dip(*PAINT_A)
+    ob.imprint(20.0, 19.0, 0.0, oc)
+    hr = (ob.size_map - 1) // 2
+    fp = ob.footprint()
+    np.testing.assert_array_equal(oc.V[19 - hr:19 + hr + 1, 20 - hr:20 + hr + 1],
+                                  fp[:2 * hr + 1, :2 * hr + 1])
+    painted = oc.V > 0
+    np.testing.assert_allclose(oc.K[painted], np.broadcast_to(PAINT_A[0], oc.K[painted].shape),
+                               rtol=1e-15)
+
+
+def test_pickup_conserves_volume_without_snapshot(oracle):
+    # With the canvas as pickup source, a visit moves vLeave from canvas to
+    # pickup map, adds fh to the canvas and removes dep*vP*fh from the map:
+    # total canvas+map volume grows by sum fh - sum dep*vP*fh <= sum fh.
+    rows = cols = 20
+    radius = 3
+    core = W.gaussian_core(2 * radius + 1, 2.0).astype(np.float64)
+    K, S, V = _layer(rows, cols, 12)
+    ob = oracle.Brush(radius, core)
+    ob.set_use_snapshot(False)
+    ob.set_rates(0.9, 0.0)  # no deposition loss: growth is exactly sum fh
+    oc = oracle.Canvas.from_arrays(K, S, V, np.ones((rows, cols, 3)))
+    ob.dip(*PAINT_A)
+    total0 = oc.V.sum()
+    ob.imprint(9.5, 10.0, 0.4, oc)
+    fsum = 0.0
+    hr = (ob.size_map - 1) // 2
+    fp = ob.footprint()
+    ct, stt = math.cos(-0.4), math.sin(-0.4)
+    for row in range(-hr, hr + 1):
+        for col in range(-hr, hr + 1):
+            x, y = int(col + 9.5), int(row + 10.0)
+            mx = int(np.round(col * ct - row * stt + hr))
+            my = int(np.round(col * stt + row * ct + hr))
+            if 0 <= x < cols and 0 <= y < rows and 0 <= mx < ob.size_map and 0 <= my < ob.size_map:
+                fsum += fp[my, mx]
+    _, _, pV = ob.pickup_map()
+    assert abs((oc.V.sum() + pV.sum()) - (total0 + fsum)) < 1e-10
+
+
+def test_footprint_padding_geometry(oracle):
+    # setRadius: width = 2 ceil(r) + 1, sizeMap = ceil(sqrt 2 width), pad = (sizeMap-width)//2
+    for r, size_map in ((32, 92), (16, 47), (64, 183), (128, 364)):
+        w = 2 * r + 1
+        core = np.ones((w, w))
+        b = oracle.Brush(float(r), core)
+        assert b.size_map == size_map
+        fp = b.footprint()
+        pad = (size_map - w) // 2
+        assert fp.sum() == w * w
+        assert fp[pad:pad + w, pad:pad + w].all()
+        assert not fp[size_map - 1].any() if pad + w < size_map else True
+
+
+# ------------------------------------------------------------- mixer -------
+def _palette(name):
+    d = json.load(open(os.path.join(DATA, name)))
+    return np.array([p["K"] for p in d]), np.array([p["S"] for p in d])
+
+
+def test_mix_single_is_normalised_product(oracle):
+    bK, bS = _palette("lindemeier-measured.json")
+    assert bK.shape == (14, 3)
+    st = W.Stream(3, 3)
+    for _ in range(20):
+        w = st.uniform(14)
+        K, S = oracle.mix_single(bK, bS, w)
+        np.testing.assert_allclose(K, (w / w.sum()) @ bK, rtol=1e-12)
+        np.testing.assert_allclose(S, (w / w.sum()) @ bS, rtol=1e-12)
+    w = np.zeros(14)
+    w[3] = 1.0
+    np.testing.assert_array_equal(oracle.mix_single(bK, bS, w)[0], bK[3])
+    with pytest.raises(ValueError):
+        oracle.mix_single(bK, bS, np.ones(13))
+
+
+def test_mixed_volume_weighted(oracle):
+    K, S = oracle.mixed([1, 2, 3], [4, 5, 6], 1.0, [3, 2, 1], [0, 0, 0], 3.0)
+    np.testing.assert_allclose(K, [2.5, 2.0, 1.5])
+    np.testing.assert_allclose(S, [1.0, 1.25, 1.5])
+
+
+def test_curtis_palette_matches_reference_test_values(oracle):
+    # the data file equals the values SerializationTest.cxx:27-64 lists
+    bK, bS = _palette("curtis-watercolor.json")
+    assert bK.shape == (13, 3)
+    np.testing.assert_allclose(bK[1], [0.22, 1.47, 0.57])
+    np.testing.assert_allclose(bS[12], [1.25, 0.42, 1.43])
+
+
+# ------------------------------------------------------------ stroke -------
+def test_stroke_placement_unit_chord_steps(oracle):
+    path = [(10.0, 10.0), (20.0, 10.0), (20.0, 25.0)]
+    st = oracle.stroke_stamps(path)
+    assert len(st) == 10 + 15
+    # t = pd / dist lands on the segment end at pd = dist
+    np.testing.assert_allclose(st[9, :2], [20.0, 10.0], atol=1e-12)
+    np.testing.assert_allclose(st[-1, :2], [20.0, 25.0], atol=1e-12)
