@@ -1,0 +1,264 @@
+"""Benchmark: KM composite + imprint stamping on one MI355X.
+
+Workload (one step): the large single-canvas configuration -- a 4096x4096
+FP32 canvas (K log-uniform [0.01,10], S log-uniform [0.1,5], thickness
+U(0,0.5) with 30% zeros, substrate U(0.7,0.95)), 2000 imprint stamps along
+seeded Bezier strokes at 2 px spacing with a per-stroke footprint of 129-257
+px (Gaussian x value noise), pickup on, dryCanvas() after every 500th stamp,
+then a full composite of the canvas.  All inputs are resident in HBM before
+the timed region.  Synthetic data; see painty_amd/workload.py.
+
+Prints one JSON line.  `value` is stamps/s of the whole job (stamps + drying
++ composite); the composite alone is reported as Gpixel/s beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "KM composite Gpixel/s + stamps/s; p50/p99 served latency vs plain GPU path"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes (FP32 SoA minimum traffic)
+BYTES_COMPOSE1 = 52   # K3 S3 V R0_3 in, R_3 out
+BYTES_DRY = 88        # 11 planes read, 11 written
+BYTES_VISIT = 148     # canvas K,S,V rw 56 + snapshot K,S,V r + V w 32 + pickup rw 56 + footprint 4
+BYTES_RING = 56       # snapshot refresh: 7 floats read + 7 written
+
+
+class HipEvents:
+    """hipEvent timing on a given stream (the stream the kernels run on)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b) -> float:
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def groups(stamps):
+    ids = stamps["brush"]
+    cut = np.flatnonzero(np.diff(ids)) + 1
+    starts = np.concatenate([[0], cut])
+    ends = np.concatenate([cut, [len(stamps)]])
+    return [(int(ids[s]), stamps[s:e]) for s, e in zip(starts, ends)]
+
+
+def cpu_baseline(wl, n_stamps: int, comp_rows: int):
+    """FP64 oracle (single thread, the reference's loops) on a bounded sample:
+    the first n_stamps of the job on the full canvas, and a composite of the
+    first comp_rows rows."""
+    from oracle import oracle as O
+    R0 = wl.R0.astype(np.float64)
+    O.compose_onto(wl.dry.K, wl.dry.S, wl.dry.V, R0)
+    cv = O.Canvas.from_arrays(wl.wet.K.astype(np.float64), wl.wet.S.astype(np.float64),
+                              wl.wet.V.astype(np.float64), R0)
+    sample = wl.stamps[:n_stamps].copy()
+    sample["flags"] &= ~np.uint32(2)  # no dry inside the stamp sample
+    brush = None
+    t0 = time.perf_counter()
+    for bid, grp in groups(sample):
+        if brush is None:
+            brush = O.Brush(wl.radii[bid], wl.cores[bid].astype(np.float64))
+            brush.set_rates(wl.pickup_rate, wl.deposition_rate)
+        else:
+            brush.set_radius(wl.radii[bid], wl.cores[bid].astype(np.float64))
+        brush.run(cv, grp)
+    t_st = time.perf_counter() - t0
+    n = comp_rows * wl.cols
+    t0 = time.perf_counter()
+    O.compose(cv.K[:comp_rows], cv.S[:comp_rows], cv.V[:comp_rows], cv.R0[:comp_rows])
+    t_c = time.perf_counter() - t0
+    return {"stamps_s": n_stamps / t_st, "composite_gpixel_s": n / t_c / 1e9,
+            "t_stamps_s": t_st, "t_composite_s": t_c}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["plain", "persistent"], default="plain")
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--stamps", type=int, default=2000)
+    ap.add_argument("--cpu-stamps", type=int, default=300)
+    ap.add_argument("--cpu-rows", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(canvases would shard one per GPU with no exchange)", file=sys.stderr)
+        sys.exit(2)
+
+    from painty_amd import _lib
+    from painty_amd import hip as H
+    from painty_amd import workload as W
+
+    t0 = time.perf_counter()
+    wl = W.cfg2(size=args.size, n_stamps=args.stamps)
+    log("bench: workload %s generated in %.1fs (%d strokes)" % (wl.name, time.perf_counter() - t0,
+                                                               len(wl.cores)))
+    ctx = H.default_context()
+    stream = ctx.stream
+    ev = HipEvents()
+    cv = H.Canvas(wl.rows, wl.cols)
+    cv.getPaintLayer().setBuffers(wl.wet.K, wl.wet.S, wl.wet.V)
+    cv.setR0(wl.R0)
+    dry = H.PaintLayer(wl.rows, wl.cols)
+    dry.setBuffers(wl.dry.K, wl.dry.S, wl.dry.V)
+    dry.composeOntoCanvas(cv)
+    del dry
+    out = H.Canvas(wl.rows, wl.cols)  # its R0 planes receive the composite
+    Lc, Rc, _ = cv.device_planes()
+    _, Ro, _ = out.device_planes()
+    dev_layer = _lib.DevLayer((C.c_void_p * 7)(*Lc))
+    dev_r0 = _lib.DevRgb((C.c_void_p * 3)(*Rc))
+    dev_out = _lib.DevRgb((C.c_void_p * 3)(*Ro))
+    n_px = wl.rows * wl.cols
+    mode = _lib.EXEC_PERSISTENT if args.mode == "persistent" else _lib.EXEC_PLAIN
+
+    # segments: one brush geometry, no dryCanvas inside (it runs between them)
+    segs = []
+    for bid, g in groups(wl.stamps):
+        cut = np.flatnonzero(g["flags"] & W.DRY_AFTER) + 1
+        for part in np.split(g, cut):
+            if len(part):
+                p2 = part.copy()
+                p2["flags"] &= ~np.uint32(W.DRY_AFTER)
+                segs.append((bid, H.stamps_to_c(p2), bool(part["flags"][-1] & W.DRY_AFTER)))
+    brush = H.FootprintBrush(wl.radii[segs[0][0]], wl.cores[segs[0][0]])
+    brush.setPickupRate(wl.pickup_rate)
+    brush.setDepositionRate(wl.deposition_rate)
+
+    # algorithmic traffic of the stamps (exact host replay of the index math)
+    visits = ring = 0
+    for bid, arr, _ in segs:
+        brush.setRadius(wl.radii[bid], wl.cores[bid])
+        a, b = C.c_int64(), C.c_int64()
+        _lib.call("pnt_brush_stamp_traffic", brush.h, wl.rows, wl.cols, arr, len(arr),
+                  C.byref(a), C.byref(b))
+        visits += a.value
+        ring += b.value
+    n_dry = sum(1 for s_ in segs if s_[2])
+    stamp_bytes = BYTES_VISIT * visits + BYTES_RING * ring
+    log("bench: %d segments, %d active visits, %d ring pixels, %d dries per step" %
+        (len(segs), visits, ring, n_dry))
+    seg_events = [(ev.event(), ev.event()) for _ in segs]
+
+    def step():
+        for k, (bid, arr, dry_after) in enumerate(segs):
+            brush.setRadius(wl.radii[bid], wl.cores[bid])
+            ev.record(seg_events[k][0], stream)
+            _lib.call("pnt_imprint_stamps", brush.h, cv.h, arr, len(arr), mode)
+            ev.record(seg_events[k][1], stream)
+            if dry_after:
+                _lib.call("pnt_canvas_dry", cv.h)
+        _lib.call("pnt_dev_compose", ctx.h, n_px, dev_layer, dev_r0, dev_out, None)
+
+    def stamp_ms():
+        return sum(ev.elapsed_ms(a, b) for a, b in seg_events)
+
+    for i in range(args.warmup):
+        step()
+        ctx.synchronize()
+        log("bench: warmup step %d done" % i)
+    e0, e1 = ev.event(), ev.event()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ev.record(e0, stream)
+    st_ms = 0.0
+    for i in range(args.steps):
+        step()
+        if i + 1 < args.steps:
+            ctx.synchronize()  # read this step's stamp events before they are reused
+        st_ms += stamp_ms()
+    ev.record(e1, stream)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev.elapsed_ms(e0, e1)
+    ms_per_step = wall * 1e3 / args.steps
+    st_ms /= args.steps
+    log("bench: %d steps, %.2f ms/step wall, %.2f ms/step stream, stamp launches %.2f ms/step" %
+        (args.steps, ms_per_step, gpu_ms / args.steps, st_ms))
+
+    # composite kernel alone (the KM composite metric), live HIP events
+    reps = 20
+    a, b = ev.event(), ev.event()
+    ev.record(a, stream)
+    for _ in range(reps):
+        _lib.call("pnt_dev_compose", ctx.h, n_px, dev_layer, dev_r0, dev_out, None)
+    ev.record(b, stream)
+    comp_ms = ev.elapsed_ms(a, b) / reps
+    comp_gpx = n_px / (comp_ms * 1e-3) / 1e9
+    comp_gbs = BYTES_COMPOSE1 * n_px / (comp_ms * 1e-3) / 1e9
+    log("bench: composite %.3f ms, %.1f Gpixel/s, %.0f GB/s" % (comp_ms, comp_gpx, comp_gbs))
+
+    stamps_s = args.stamps / (ms_per_step * 1e-3)
+    stamp_gbs = stamp_bytes / (st_ms * 1e-3) / 1e9
+    kernel = "k_persistent" if mode == _lib.EXEC_PERSISTENT else "k_stamp_main+k_ring_refresh"
+    result = {
+        "metric": METRIC, "value": round(stamps_s, 1), "unit": "stamps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "large single canvas %dx%d, %d stamps (footprint 129-257 px, "
+                               "pickup on, dry every 500) + composite" % (wl.rows, wl.cols,
+                                                                          args.stamps),
+                   "mode": args.mode},
+        "stamps_per_s_kernel": round(args.stamps / (st_ms * 1e-3), 1),
+        "composite_gpixel_s": round(comp_gpx, 2),
+        "composite_ms": round(comp_ms, 4),
+        "roofline": {"bound": "hbm", "kernel": kernel,
+                     "achieved": round(stamp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(stamp_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_step": int(stamp_bytes),
+                     "launches_per_step": len(segs),
+                     "avg_launch_ms": round(st_ms / len(segs), 4)},
+        "roofline_composite": {"bound": "hbm", "kernel": "k_compose_vec<1>",
+                               "achieved": round(comp_gbs, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(comp_gbs / HBM_PEAK_GBS, 4)},
+    }
+    if not args.no_cpu:
+        log("bench: cpu baseline (oracle, 1 thread): %d stamps + %d-row composite" %
+            (args.cpu_stamps, args.cpu_rows))
+        cb = cpu_baseline(wl, args.cpu_stamps, args.cpu_rows)
+        result["cpu_baseline"] = {
+            "value": round(cb["stamps_s"], 3), "unit": "stamps/s", "cores": 1, "kind": "port",
+            "composite_gpixel_s": round(cb["composite_gpixel_s"], 5),
+            "sample": "first %d stamps of the job on the full canvas (%.1fs) + composite of "
+                      "%dx%d (%.1fs), FP64 oracle" % (args.cpu_stamps, cb["t_stamps_s"],
+                                                      args.cpu_rows, wl.cols,
+                                                      cb["t_composite_s"])}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

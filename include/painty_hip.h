@@ -201,6 +201,15 @@ int pnt_imprint_stamps(pnt_brush brush, pnt_canvas canvas,
 int pnt_paint_stroke(pnt_brush brush, pnt_canvas canvas, const double* path_xy,
                      int64_t n_points);
 
+/* Measurement helper (no reference counterpart): replays the index math of
+ * `n` stamps on a rows x cols canvas with this brush's footprint and counts
+ * the active visits (in canvas, in map, footprint > 0) and the in-canvas
+ * snapshot-ring pixels the stamps refresh (0 when the snapshot is off).
+ * Host only; the brush and canvas state are not touched. */
+int pnt_brush_stamp_traffic(pnt_brush brush, int rows, int cols,
+                            const pnt_stamp* stamps, int64_t n,
+                            int64_t* active_visits, int64_t* ring_pixels);
+
 /* --------------------------------------------------------- PaintMixer --- */
 /* mixSinglePaint(weights) -- mixer/src/PaintMixer.cxx:327-355: K = sum_l
  * norm*w_l*K_l (same for S), norm = 1/sum(w) unless |sum(w)-1| < 1e-5.

@@ -98,6 +98,8 @@ SIGNATURES = {
     "pnt_imprint": [_vp, _vp, C.c_double, C.c_double, C.c_double],
     "pnt_imprint_stamps": [_vp, _vp, C.POINTER(PntStamp), C.c_int64, C.c_int],
     "pnt_paint_stroke": [_vp, _vp, _dp, C.c_int64],
+    "pnt_brush_stamp_traffic": [_vp, C.c_int, C.c_int, C.POINTER(PntStamp), C.c_int64,
+                                C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "pnt_mix_single": [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp],
     "pnt_mixed": [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp],
     "pnt_compute_reflectance": [_vp, C.c_int64, _fp, _fp, _fp, _fp, _fp],

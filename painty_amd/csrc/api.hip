@@ -6,6 +6,7 @@
 #include <utility>
 #include <vector>
 
+#include "handles.hpp"
 #include "internal.hpp"
 #include "persistent.hpp"
 
@@ -31,29 +32,6 @@ int guard(F&& f) {
     g_err = e.what();
     return PNT_EINVAL;
   }
-}
-
-#define PNT_REQUIRE(cond, msg)                       \
-  do {                                               \
-    if (!(cond)) throw ::pnt::Error(PNT_EINVAL, msg); \
-  } while (0)
-
-inline int64_t round_up(int64_t n, int64_t a) { return (n + a - 1) / a * a; }
-
-float* dev_alloc(int64_t nfloats) {
-  void* p = nullptr;
-  if (nfloats <= 0) nfloats = 1;
-  const hipError_t e = hipMalloc(&p, (size_t)nfloats * sizeof(float));
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    throw Error(PNT_ENOMEM, std::string("hipMalloc of ") + std::to_string(nfloats * 4) +
-                                " bytes failed: " + hipGetErrorString(e));
-  }
-  return static_cast<float*>(p);
-}
-
-void dev_free(void* p) {
-  if (p) (void)hipFree(p);
 }
 
 // Host AoS/planar <-> device planes.
@@ -96,92 +74,24 @@ void download_planes(float* dst, int nplanes, float* const* src, int64_t n, int 
 
 }  // namespace
 
-// ------------------------------------------------------------------ handles
-struct pnt_ctx_s {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  void use() const { PNT_HIP_CHECK(hipSetDevice(device)); }
-};
-
-struct pnt_layer_s {
-  pnt_ctx ctx = nullptr;
-  int rows = 0, cols = 0;
-  int64_t n = 0, stride = 0;
-  float* base = nullptr;
-  bool owner = true;
-  Layer7 planes() const {
-    Layer7 L;
-    for (int i = 0; i < 7; ++i) L.p[i] = base + i * stride;
-    return L;
-  }
-  void alloc(int r, int c) {
-    rows = r;
-    cols = c;
-    n = (int64_t)r * c;
-    stride = round_up(std::max<int64_t>(n, 1), 64);
-    base = dev_alloc(7 * stride);
-  }
-};
-
-struct pnt_canvas_s {
-  pnt_ctx ctx = nullptr;
-  int rows = 0, cols = 0;
-  int64_t n = 0, stride = 0;
-  float* base = nullptr;     // planes 0..6 wet layer, 7..9 R0, 10 h
-  float* scratch = nullptr;  // 3 planes for host-side compose results
-  pnt_layer_s view;
-  int64_t drying_ms = 15000000;  // Canvas.hxx:32
-  float bg[3] = {1.f, 1.f, 1.f};
-  Layer7 layer() const { return view.planes(); }
-  Rgb3 R0() const { return Rgb3{{base + 7 * stride, base + 8 * stride, base + 9 * stride}}; }
-  float* h() const { return base + 10 * stride; }
-};
-
-struct pnt_brush_s {
-  pnt_ctx ctx = nullptr;
-  double radius = 0.0;
-  int width = 0, size_map = 0;
-  int64_t m = 0, mstride = 0;
-  float* fp = nullptr;    // footprint
-  float* pick = nullptr;  // 7 planes of mstride
-  float* snap = nullptr;  // 7 planes of snap_stride
-  int snap_rows = 0, snap_cols = 0;
-  int64_t snap_stride = 0;
-  bool use_snap = true;
-  float pickup_rate = 0.9f, dep_rate = 0.05f, cap = 1.0f;
-  float paintK[3] = {0, 0, 0}, paintS[3] = {0, 0, 0};
-  // dependency lists of the current stamp's shared-pixel visits
-  int2* d_list = nullptr;
-  int* d_off = nullptr;
-  int64_t d_cap = 0;
-  PersistentState* pers = nullptr;
-
-  Layer7 pick_planes() const {
-    Layer7 L;
-    for (int i = 0; i < 7; ++i) L.p[i] = pick + i * mstride;
-    return L;
-  }
-  Layer7 snap_planes() const {
-    Layer7 L;
-    for (int i = 0; i < 7; ++i) L.p[i] = snap + i * snap_stride;
-    return L;
-  }
-  BrushArgs args() const {
-    BrushArgs b;
-    b.fp = fp;
-    b.pick = pick_planes();
-    b.pickup_rate = pickup_rate;
-    b.dep_rate = dep_rate;
-    b.cap = cap;
-    for (int c = 0; c < 3; ++c) {
-      b.paintK[c] = paintK[c];
-      b.paintS[c] = paintS[c];
-    }
-    return b;
-  }
-};
-
 namespace pnt {
+
+float* dev_alloc(int64_t nfloats) {
+  void* p = nullptr;
+  if (nfloats <= 0) nfloats = 1;
+  const hipError_t e = hipMalloc(&p, (size_t)nfloats * sizeof(float));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(PNT_ENOMEM, std::string("hipMalloc of ") + std::to_string(nfloats * 4) +
+                                " bytes failed: " + hipGetErrorString(e));
+  }
+  return static_cast<float*>(p);
+}
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
 
 // FootprintBrush::setRadius geometry (FootprintBrush.hxx:52-61) with an
 // injected width x width footprint, padded with zeros to size_map^2.
@@ -216,6 +126,7 @@ void brush_set_geometry(pnt_brush b, double radius, const float* core, int width
   dev_free(b->fp);
   dev_free(b->pick);
   b->fp = fp;
+  b->fp_host.swap(full);
   b->pick = pick;
   b->radius = radius;
   b->width = w;
@@ -238,6 +149,13 @@ void brush_full_snapshot(pnt_brush b, pnt_canvas cv) {
     b->snap_cols = cv->cols;
   }
   launch_copy_layer(cv->n, cv->layer(), b->snap_planes(), s);
+  if (b->dirty && b->dirty_n == cv->n) {
+    launch_fill_u8(b->dirty, cv->n, 0, s);
+    b->dirty_canvas = cv;
+    b->dirty_epoch = cv->epoch;
+  } else {
+    b->dirty_canvas = nullptr;
+  }
 }
 
 StampArgs stamp_args(pnt_brush b, pnt_canvas cv, double cx, double cy, double theta) {
@@ -331,6 +249,8 @@ void brush_dip(pnt_brush b, const float K[3], const float S[3]) {
 // FootprintBrush::imprint (FootprintBrush.hxx:73-143) as kernel launches.
 void imprint_plain(pnt_brush b, pnt_canvas cv, double cx, double cy, double theta) {
   hipStream_t s = b->ctx->stream;
+  cv->epoch++;
+  b->dirty_canvas = nullptr;
   if (b->use_snap) {
     if (b->snap_rows != cv->rows || b->snap_cols != cv->cols) brush_full_snapshot(b, cv);
   }
@@ -449,7 +369,10 @@ int pnt_ctx_create(int device, pnt_ctx* out) {
     try {
       c->use();
       PNT_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      PNT_HIP_CHECK(hipHostMalloc((void**)&c->kernel_error, sizeof(unsigned), hipHostMallocMapped));
+      *c->kernel_error = 0;
     } catch (...) {
+      if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
       throw;
     }
@@ -463,6 +386,7 @@ int pnt_ctx_destroy(pnt_ctx ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->kernel_error) (void)hipHostFree(ctx->kernel_error);
     delete ctx;
   });
 }
@@ -472,6 +396,11 @@ int pnt_ctx_synchronize(pnt_ctx ctx) {
     PNT_REQUIRE(ctx, "NULL context");
     ctx->use();
     PNT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (*(volatile unsigned*)ctx->kernel_error) {
+      *ctx->kernel_error = 0;
+      throw Error(PNT_EKERNEL, "a persistent stamp kernel aborted (grid barrier wait expired); "
+                               "canvas and brush state are undefined");
+    }
   });
 }
 
@@ -523,6 +452,7 @@ int pnt_layer_clear(pnt_layer L) {
   return guard([&] {
     PNT_REQUIRE(L, "NULL layer");
     L->ctx->use();
+    L->touched();
     launch_fill(L->base, 7 * L->stride, 0.0f, L->ctx->stream);
   });
 }
@@ -532,6 +462,7 @@ int pnt_layer_upload(pnt_layer L, const float* K, const float* S, const float* V
     PNT_REQUIRE(L, "NULL layer");
     check_layout(layout);
     L->ctx->use();
+    L->touched();
     const Layer7 P = L->planes();
     if (K) upload_planes(P.p, 3, K, L->n, layout, L->ctx->stream);
     if (S) upload_planes(P.p + 3, 3, S, L->n, layout, L->ctx->stream);
@@ -556,6 +487,7 @@ int pnt_layer_copy_to(pnt_layer src, pnt_layer dst) {
     PNT_REQUIRE(src && dst, "NULL layer");
     PNT_REQUIRE(src->ctx->device == dst->ctx->device, "layers on different devices");
     src->ctx->use();
+    dst->touched();
     hipStream_t s = src->ctx->stream;
     if (dst->rows != src->rows || dst->cols != src->cols) {
       PNT_REQUIRE(dst->owner, "cannot resize the paint layer of a canvas");
@@ -579,6 +511,7 @@ int pnt_layer_set(pnt_layer L, int i, int j, const float k[3], const float s[3],
     PNT_REQUIRE(L && k && s, "NULL argument");
     PNT_REQUIRE(i >= 0 && i < L->rows && j >= 0 && j < L->cols, "index out of range");
     L->ctx->use();
+    L->touched();
     const int64_t idx = (int64_t)i * L->cols + j;
     const float vals[7] = {k[0], k[1], k[2], s[0], s[1], s[2], v};
     const Layer7 P = L->planes();
@@ -628,6 +561,7 @@ int pnt_layer_compose_onto_canvas(pnt_layer L, pnt_canvas cv) {
 // ----------------------------------------------------------------- Canvas --
 static void canvas_clear(pnt_canvas cv) {
   hipStream_t s = cv->ctx->stream;
+  cv->epoch++;
   launch_fill(cv->base, 7 * cv->stride, 0.0f, s);
   for (int c = 0; c < 3; ++c) launch_fill(cv->base + (7 + c) * cv->stride, cv->n, cv->bg[c], s);
   launch_fill(cv->h(), cv->n, 0.0f, s);
@@ -661,6 +595,7 @@ int pnt_canvas_create(pnt_ctx ctx, int rows, int cols, pnt_canvas* out) {
     cv->view.stride = cv->stride;
     cv->view.base = cv->base;
     cv->view.owner = false;
+    cv->view.parent = cv;
     *out = cv;
   });
 }
@@ -752,6 +687,7 @@ int pnt_canvas_dry(pnt_canvas cv) {
   return guard([&] {
     PNT_REQUIRE(cv, "NULL canvas");
     cv->ctx->use();
+    cv->epoch++;
     launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), cv->ctx->stream);
   });
 }
@@ -761,6 +697,7 @@ int pnt_canvas_dry_step(pnt_canvas cv, float portion) {
     PNT_REQUIRE(cv, "NULL canvas");
     PNT_REQUIRE(std::isfinite(portion), "portion must be finite");
     cv->ctx->use();
+    cv->epoch++;
     launch_dry_step(cv->n, cv->layer(), cv->R0(), portion, cv->ctx->stream);
   });
 }
@@ -783,6 +720,7 @@ int pnt_canvas_get_drying_time_ms(pnt_canvas cv, int64_t* ms) {
 int pnt_canvas_device_planes(pnt_canvas cv, pnt_dev_layer* layer, pnt_dev_rgb* R0, float** h) {
   return guard([&] {
     PNT_REQUIRE(cv, "NULL canvas");
+    cv->epoch++;  // the caller may write the planes
     if (layer) {
       const Layer7 L = cv->layer();
       for (int i = 0; i < 7; ++i) layer->p[i] = L.p[i];
@@ -924,6 +862,7 @@ int pnt_brush_destroy(pnt_brush b) {
     dev_free(b->snap);
     dev_free(b->d_list);
     dev_free(b->d_off);
+    dev_free(b->dirty);
     delete b;
   });
 }
@@ -1052,11 +991,52 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
       for (int64_t i = 0; i < n; ++i) {
         if (st[i].flags & PNT_STAMP_DIP) brush_dip(b, st[i].K, st[i].S);
         imprint_plain(b, cv, st[i].cx, st[i].cy, st[i].theta);
-        if (st[i].flags & PNT_STAMP_DRY_AFTER)
+        if (st[i].flags & PNT_STAMP_DRY_AFTER) {
+          cv->epoch++;
           launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), cv->ctx->stream);
+        }
       }
     }
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+int pnt_brush_stamp_traffic(pnt_brush b, int rows, int cols, const pnt_stamp* st, int64_t n,
+                            int64_t* visits, int64_t* ring) {
+  return guard([&] {
+    PNT_REQUIRE(b && visits && ring && (st || n == 0), "NULL argument");
+    PNT_REQUIRE(rows > 0 && cols > 0, "rows/cols must be positive");
+    int64_t nv = 0, nr = 0;
+    StampGeom g;
+    g.hr = (b->size_map - 1) / 2;
+    g.size_map = b->size_map;
+    g.rows = rows;
+    g.cols = cols;
+    for (int64_t k = 0; k < n; ++k) {
+      g.cx = st[k].cx;
+      g.cy = st[k].cy;
+      g.c = std::cos(-st[k].theta);
+      g.s = std::sin(-st[k].theta);
+      for (int row = -g.hr; row <= g.hr; ++row)
+        for (int col = -g.hr; col <= g.hr; ++col) {
+          int64_t ic;
+          const int m = stamp_visit(g, col, row, &ic);
+          if (m >= 0 && b->fp_host[(size_t)m] > 0.0f) ++nv;
+        }
+      if (b->use_snap) {
+        const RingBox box = ring_box(g, b->radius);
+        if (box.a_x1 >= box.a_x0 && box.a_y1 >= box.a_y0) {
+          const int64_t all = (int64_t)(box.a_x1 - box.a_x0 + 1) * (box.a_y1 - box.a_y0 + 1);
+          const int ix0 = std::max(box.tl_x + 1, box.a_x0), ix1 = std::min(box.br_x - 1, box.a_x1);
+          const int iy0 = std::max(box.tl_y + 1, box.a_y0), iy1 = std::min(box.br_y - 1, box.a_y1);
+          const int64_t inner =
+              (ix1 >= ix0 && iy1 >= iy0) ? (int64_t)(ix1 - ix0 + 1) * (iy1 - iy0 + 1) : 0;
+          nr += all - inner;
+        }
+      }
+    }
+    *visits = nv;
+    *ring = nr;
   });
 }
 

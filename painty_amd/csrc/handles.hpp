@@ -1,0 +1,136 @@
+// Definitions of the opaque handles of painty_hip.h (shared by api.hip and
+// persistent.hip).
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "internal.hpp"
+
+#define PNT_REQUIRE(cond, msg)                        \
+  do {                                                \
+    if (!(cond)) throw ::pnt::Error(PNT_EINVAL, msg); \
+  } while (0)
+
+namespace pnt {
+inline int64_t round_up(int64_t n, int64_t a) { return (n + a - 1) / a * a; }
+float* dev_alloc(int64_t nfloats);
+void dev_free(void* p);
+struct PersistentState;
+}  // namespace pnt
+
+struct pnt_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // Pinned host word that persistent kernels raise when a bounded wait
+  // expires; checked by pnt_ctx_synchronize and at the next persistent run.
+  unsigned* kernel_error = nullptr;
+  void use() const { PNT_HIP_CHECK(hipSetDevice(device)); }
+};
+
+struct pnt_canvas_s;
+
+struct pnt_layer_s {
+  pnt_ctx ctx = nullptr;
+  int rows = 0, cols = 0;
+  int64_t n = 0, stride = 0;
+  float* base = nullptr;
+  bool owner = true;
+  pnt_canvas_s* parent = nullptr;  // the canvas whose wet layer this views
+  pnt::Layer7 planes() const {
+    pnt::Layer7 L;
+    for (int i = 0; i < 7; ++i) L.p[i] = base + i * stride;
+    return L;
+  }
+  void alloc(int r, int c) {
+    rows = r;
+    cols = c;
+    n = (int64_t)r * c;
+    stride = pnt::round_up(n > 0 ? n : 1, 64);
+    base = pnt::dev_alloc(7 * stride);
+  }
+  void touched();  // the wet layer changed outside a brush's tracking
+};
+
+struct pnt_canvas_s {
+  pnt_ctx ctx = nullptr;
+  int rows = 0, cols = 0;
+  int64_t n = 0, stride = 0;
+  float* base = nullptr;     // planes 0..6 wet layer, 7..9 R0, 10 h
+  float* scratch = nullptr;  // 3 planes for host-side compose results
+  pnt_layer_s view;
+  int64_t drying_ms = 15000000;  // Canvas.hxx:32
+  float bg[3] = {1.f, 1.f, 1.f};
+  // Bumped by every change of the wet layer; a brush's snapshot dirty bytes
+  // are valid only for the (canvas, epoch) they were last brought up to.
+  uint64_t epoch = 1;
+  pnt::Layer7 layer() const { return view.planes(); }
+  pnt::Rgb3 R0() const {
+    return pnt::Rgb3{{base + 7 * stride, base + 8 * stride, base + 9 * stride}};
+  }
+  float* h() const { return base + 10 * stride; }
+};
+
+inline void pnt_layer_s::touched() {
+  if (parent) parent->epoch++;
+}
+
+struct pnt_brush_s {
+  pnt_ctx ctx = nullptr;
+  double radius = 0.0;
+  int width = 0, size_map = 0;
+  int64_t m = 0, mstride = 0;
+  float* fp = nullptr;         // footprint, size_map^2
+  std::vector<float> fp_host;  // same footprint on the host (traffic replay)
+  float* pick = nullptr;       // pickup map: 7 planes of mstride
+  float* snap = nullptr;       // snapshot: 7 planes of snap_stride
+  int snap_rows = 0, snap_cols = 0;
+  int64_t snap_stride = 0;
+  bool use_snap = true;
+  float pickup_rate = 0.9f, dep_rate = 0.05f, cap = 1.0f;
+  float paintK[3] = {0, 0, 0}, paintS[3] = {0, 0, 0};
+  // dependency lists of the current stamp's shared-pixel visits (plain path)
+  int2* d_list = nullptr;
+  int* d_off = nullptr;
+  int64_t d_cap = 0;
+  // snapshot dirty bytes (persistent path): dirty[i] == 0 means snapshot
+  // pixel i equals canvas pixel i of (dirty_canvas, dirty_epoch)
+  uint8_t* dirty = nullptr;
+  int64_t dirty_n = 0;
+  const pnt_canvas_s* dirty_canvas = nullptr;
+  uint64_t dirty_epoch = 0;
+  pnt::PersistentState* pers = nullptr;
+
+  pnt::Layer7 pick_planes() const {
+    pnt::Layer7 L;
+    for (int i = 0; i < 7; ++i) L.p[i] = pick + i * mstride;
+    return L;
+  }
+  pnt::Layer7 snap_planes() const {
+    pnt::Layer7 L;
+    for (int i = 0; i < 7; ++i) L.p[i] = snap + i * snap_stride;
+    return L;
+  }
+  pnt::BrushArgs args() const {
+    pnt::BrushArgs b;
+    b.fp = fp;
+    b.pick = pick_planes();
+    b.pickup_rate = pickup_rate;
+    b.dep_rate = dep_rate;
+    b.cap = cap;
+    for (int c = 0; c < 3; ++c) {
+      b.paintK[c] = paintK[c];
+      b.paintS[c] = paintS[c];
+    }
+    return b;
+  }
+};
+
+namespace pnt {
+void brush_full_snapshot(pnt_brush b, pnt_canvas cv);
+StampArgs stamp_args(pnt_brush b, pnt_canvas cv, double cx, double cy, double theta);
+void plan_shared_visits(const StampArgs& a, std::vector<int2>& list, std::vector<int>& off);
+void launch_dry_dirty(int64_t n, const Layer7& L, const Rgb3& R0, float* h, uint8_t* dirty,
+                      hipStream_t s);
+}  // namespace pnt

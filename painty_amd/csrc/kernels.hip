@@ -8,15 +8,12 @@
 
 #include "internal.hpp"
 #include "km.hpp"
+#include "visit.hpp"
 
 namespace pnt {
 namespace {
 
 constexpr int kBlock = 256;
-// blend() and pickup thresholds compare against MinVolume = 0.001 in FP64
-// (FootprintBrush.hxx:26).  For a float x, x > 0.001 holds exactly when
-// x >= 0.001f (0.001f = 0.0010000000475 is the first float above 0.001).
-constexpr float kMinVolF = 0.001f;
 
 __device__ __forceinline__ float4 ld4(const float* p, int64_t i) {
   return *reinterpret_cast<const float4*>(p + 4 * i);
@@ -199,54 +196,25 @@ __global__ __launch_bounds__(kBlock) void k_copy_layer(int64_t n, Layer7 src, La
 }
 
 // ------------------------------------------------------------ stamping ---
-// One visit of an imprint: pickupPaint (FootprintBrush.hxx:349-384) then
-// depositPaint (:393-431) for canvas pixel ic and map cell im, with
-// blend() (:331-340) inlined.
+// One visit of an imprint for canvas pixel ic and map cell im: the shared
+// visit arithmetic (visit.hpp) on the pickup map in global memory.
+struct GlobalVisit {
+  const StampArgs& a;
+  int64_t ic;
+  int im;
+  __device__ float pick(int c) const { return a.b.pick.p[c][im]; }
+  __device__ void set_pick(int c, float v) const { a.b.pick.p[c][im] = v; }
+  __device__ float src(int c) const { return a.src.p[c][ic]; }
+  __device__ void set_src_v(float v) const { a.src.p[6][ic] = v; }
+  __device__ float canvas(int c) const { return a.canvas.p[c][ic]; }
+  __device__ void set_canvas(int c, float v) const { a.canvas.p[c][ic] = v; }
+};
+
 __device__ __forceinline__ void paint_visit(const StampArgs& a, int64_t ic, int im) {
   const float fh = a.b.fp[im];
   if (!(fh > 0.0f)) return;
-  float* const* pk = a.b.pick.p;
-  {  // pickup from the source (snapshot or canvas) into the pickup map
-    const float vP = pk[6][im];
-    const float vC = a.src.p[6][ic];
-    const float vLeave = a.b.pickup_rate * vC * fh;
-    if (vLeave >= kMinVolF) {
-      a.src.p[6][ic] = vC - vLeave;
-      const float vt = vP + vLeave;
-      if (vt >= kMinVolF) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) pk[c][im] = (vP * pk[c][im] + vLeave * a.src.p[c][ic]) / vt;
-      }
-      pk[6][im] = vP + vLeave;
-    }
-  }
-  {  // deposit: brush paint mixed with the pickup map onto the canvas
-    const float vP = pk[6][im];
-    const float vFree = fmaxf(0.0f, a.b.cap - vP);
-    float src[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) src[c] = pk[c][im];
-    const float vt = vP + vFree;
-    if (vt >= kMinVolF) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        src[c] = (vP * src[c] + vFree * a.b.paintK[c]) / vt;
-        src[3 + c] = (vP * src[3 + c] + vFree * a.b.paintS[c]) / vt;
-      }
-    }
-    pk[6][im] = vP - a.b.dep_rate * vP * fh;
-    const float vC = a.canvas.p[6][ic];
-    const float vB = a.b.cap * fh;
-    const float vt2 = vB + vC;
-    if (vt2 >= kMinVolF) {
-#pragma unroll
-      for (int c = 0; c < 6; ++c) a.canvas.p[c][ic] = (vB * src[c] + vC * a.canvas.p[c][ic]) / vt2;
-    } else {
-#pragma unroll
-      for (int c = 0; c < 6; ++c) a.canvas.p[c][ic] = src[c];
-    }
-    a.canvas.p[6][ic] = vB + vC;
-  }
+  GlobalVisit acc{a, ic, im};
+  paint_visit_math(acc, fh, a.b.pickup_rate, a.b.dep_rate, a.b.cap, a.b.paintK, a.b.paintS);
 }
 
 // updateSnapshot(canvas, center), FootprintBrush.hxx:307-318: copy the wet
