@@ -193,6 +193,26 @@ typedef struct {
 int pnt_imprint_stamps(pnt_brush brush, pnt_canvas canvas,
                        const pnt_stamp* stamps, int64_t n, int mode);
 
+/* Many independent (canvas, brush, stamp list) jobs at once, each then
+ * composited (Renderer::compose, Renderer.hxx:48-53) into `out` -- three
+ * 16-byte-aligned device planes of rows*cols floats, or out[0] == NULL for
+ * no composite.  Canvases and brushes must be distinct across the jobs.
+ * PNT_BATCH_ONE_LAUNCH runs every job and composite in one cooperative
+ * launch (jobs packed onto groups of workgroups by cost, composites taken
+ * by whichever workgroups are free); PNT_BATCH_PLAIN issues each job's
+ * stamps and composite as separate launches, one job after another.  Each
+ * job's result equals pnt_imprint_stamps + pnt_compose on its own. */
+typedef struct {
+  pnt_canvas canvas;
+  pnt_brush brush;
+  const pnt_stamp* stamps;
+  int64_t n_stamps;
+  float* out[3];
+} pnt_batch_job;
+#define PNT_BATCH_ONE_LAUNCH 0
+#define PNT_BATCH_PLAIN 1
+int pnt_batch_run(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int mode);
+
 /* paintStroke(path, canvas) -- FootprintBrush.hxx:206-268: Catmull-Rom
  * placement at 1 px chord steps (core/Spline.hxx:27-73), one imprint per
  * step.  path_xy holds n_points (x, y) pairs.  The reference reads

@@ -16,6 +16,7 @@ OK, EINVAL, EHIP, ENOMEM, ESTATE, EKERNEL = 0, -1, -2, -3, -4, -5
 AOS, PLANAR = 0, 1
 STAMP_DIP, STAMP_DRY_AFTER = 1, 2
 EXEC_PLAIN, EXEC_PERSISTENT = 0, 1
+BATCH_ONE_LAUNCH, BATCH_PLAIN = 0, 1
 
 
 class PaintyError(RuntimeError):
@@ -32,6 +33,11 @@ class PntStamp(C.Structure):
     _fields_ = [("cx", C.c_double), ("cy", C.c_double), ("theta", C.c_double),
                 ("K", C.c_float * 3), ("S", C.c_float * 3), ("flags", C.c_uint32),
                 ("reserved", C.c_int32)]
+
+
+class BatchJob(C.Structure):
+    _fields_ = [("canvas", C.c_void_p), ("brush", C.c_void_p), ("stamps", C.POINTER(PntStamp)),
+                ("n_stamps", C.c_int64), ("out", C.c_void_p * 3)]
 
 
 class DevLayer(C.Structure):
@@ -98,6 +104,7 @@ SIGNATURES = {
     "pnt_imprint": [_vp, _vp, C.c_double, C.c_double, C.c_double],
     "pnt_imprint_stamps": [_vp, _vp, C.POINTER(PntStamp), C.c_int64, C.c_int],
     "pnt_paint_stroke": [_vp, _vp, _dp, C.c_int64],
+    "pnt_batch_run": [_vp, C.POINTER(BatchJob), C.c_int, C.c_int],
     "pnt_brush_stamp_traffic": [_vp, C.c_int, C.c_int, C.POINTER(PntStamp), C.c_int64,
                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "pnt_mix_single": [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp],

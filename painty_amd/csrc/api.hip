@@ -385,6 +385,7 @@ int pnt_ctx_destroy(pnt_ctx ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    batch_release(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->kernel_error) (void)hipHostFree(ctx->kernel_error);
     delete ctx;
@@ -1037,6 +1038,59 @@ int pnt_brush_stamp_traffic(pnt_brush b, int rows, int cols, const pnt_stamp* st
     }
     *visits = nv;
     *ring = nr;
+  });
+}
+
+int pnt_batch_run(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int mode) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && (jobs || n_jobs == 0) && n_jobs >= 0, "invalid batch");
+    PNT_REQUIRE(mode == PNT_BATCH_ONE_LAUNCH || mode == PNT_BATCH_PLAIN, "unknown batch mode");
+    if (n_jobs == 0) return;
+    ctx->use();
+    std::vector<const void*> seen;
+    for (int j = 0; j < n_jobs; ++j) {
+      const pnt_batch_job& J = jobs[j];
+      PNT_REQUIRE(J.canvas && J.brush, "batch job without canvas or brush");
+      PNT_REQUIRE(J.canvas->ctx->device == ctx->device && J.brush->ctx->device == ctx->device,
+                  "batch job on another device");
+      PNT_REQUIRE(J.n_stamps >= 0 && (J.stamps || J.n_stamps == 0), "invalid stamp list");
+      if (J.out[0])
+        for (int c = 0; c < 3; ++c)
+          PNT_REQUIRE(J.out[c] && ((uintptr_t)J.out[c] & 15u) == 0,
+                      "composite planes must be 16-byte aligned device pointers");
+      seen.push_back(J.canvas);
+      seen.push_back(J.brush);
+    }
+    std::sort(seen.begin(), seen.end());
+    PNT_REQUIRE(std::adjacent_find(seen.begin(), seen.end()) == seen.end(),
+                "canvases and brushes must be distinct across the jobs of a batch");
+    for (int j = 0; j < n_jobs; ++j)  // other streams' work on these objects comes first
+      if (jobs[j].canvas->ctx->stream != ctx->stream || jobs[j].brush->ctx->stream != ctx->stream) {
+        PNT_HIP_CHECK(hipStreamSynchronize(jobs[j].canvas->ctx->stream));
+        PNT_HIP_CHECK(hipStreamSynchronize(jobs[j].brush->ctx->stream));
+      }
+    if (mode == PNT_BATCH_ONE_LAUNCH) {
+      run_batch_one_launch(ctx, jobs, n_jobs);
+      return;
+    }
+    for (int j = 0; j < n_jobs; ++j) {  // plain: the job's launches on its brush's stream
+      const pnt_batch_job& J = jobs[j];
+      pnt_brush b = J.brush;
+      pnt_canvas cv = J.canvas;
+      hipStream_t s = b->ctx->stream;
+      for (int64_t i = 0; i < J.n_stamps; ++i) {
+        const pnt_stamp& st = J.stamps[i];
+        if (st.flags & PNT_STAMP_DIP) brush_dip(b, st.K, st.S);
+        imprint_plain(b, cv, st.cx, st.cy, st.theta);
+        if (st.flags & PNT_STAMP_DRY_AFTER) {
+          cv->epoch++;
+          launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), s);
+        }
+      }
+      if (J.out[0])
+        launch_compose(cv->n, cv->layer(), cv->R0(), Rgb3{{J.out[0], J.out[1], J.out[2]}}, s);
+      if (s != ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(s));
+    }
   });
 }
 

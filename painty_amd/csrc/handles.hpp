@@ -133,4 +133,7 @@ StampArgs stamp_args(pnt_brush b, pnt_canvas cv, double cx, double cy, double th
 void plan_shared_visits(const StampArgs& a, std::vector<int2>& list, std::vector<int>& off);
 void launch_dry_dirty(int64_t n, const Layer7& L, const Rgb3& R0, float* h, uint8_t* dirty,
                       hipStream_t s);
+// Snapshot of brush b ready for stamping canvas cv (allocated, copied when
+// the size changed) and its dirty bytes valid for cv.
+void brush_prepare_snapshot(pnt_brush b, pnt_canvas cv, hipStream_t s);
 }  // namespace pnt

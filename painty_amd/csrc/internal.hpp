@@ -66,6 +66,7 @@ void launch_reflectance_aos(int64_t n, const float* K, const float* S, const flo
                             const float* d, float* R, hipStream_t s);
 void launch_fill(float* p, int64_t n, float v, hipStream_t s);
 void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
+void launch_fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t s);
 void launch_copy_layer(int64_t n, const Layer7& src, const Layer7& dst, hipStream_t s);
 
 // ---- stamping (kernels.hip) ----

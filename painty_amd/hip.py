@@ -357,6 +357,27 @@ def stamps_to_c(stamps):
     return arr
 
 
+def batch_run(jobs, mode: int = _lib.BATCH_ONE_LAUNCH, ctx: Context | None = None) -> None:
+    """pnt_batch_run over [(canvas, brush, stamps, out_canvas_or_None), ...]:
+    the composite of job k lands in out_canvas's R0 planes."""
+    ctx = ctx or default_context()
+    n = len(jobs)
+    arr = (_lib.BatchJob * n)()
+    keep = []
+    for k, (cv, br, stamps, out) in enumerate(jobs):
+        st = stamps_to_c(stamps)
+        keep.append(st)
+        arr[k].canvas = cv.h.value if isinstance(cv.h, C.c_void_p) else cv.h
+        arr[k].brush = br.h.value if isinstance(br.h, C.c_void_p) else br.h
+        arr[k].stamps = C.cast(st, C.POINTER(PntStamp)) if len(st) else None
+        arr[k].n_stamps = len(st)
+        if out is not None:
+            _, R, _ = out.device_planes()
+            for c in range(3):
+                arr[k].out[c] = R[c]
+    call("pnt_batch_run", ctx.h, arr, n, int(mode))
+
+
 class PaintMixer:
     """mixSinglePaint / mixed over a base palette [(K[3], S[3]), ...]."""
 

@@ -226,6 +226,62 @@ def cfg2(seed: int = 2, size: int = 4096, n_stamps: int = 2000, dry_every: int =
                     substrate(size, size, seed), st, cores, radii)
 
 
+@dataclasses.dataclass
+class BatchWorkload:
+    name: str
+    jobs: list  # Workload per canvas (one brush size each)
+
+
+class _Pool:
+    """A fixed pool of seeded pixels (wet K, S, V and substrate R0) that many
+    large canvases draw from at seeded offsets, so a batch of hundreds of
+    megapixels needs no per-pixel generation of its own."""
+
+    def __init__(self, seed: int, n: int = 1 << 22):
+        L = paint_layer(1, n, seed, 300)
+        self.K, self.S, self.V = L.K.reshape(n, 3), L.S.reshape(n, 3), L.V.reshape(n)
+        self.R0 = substrate(1, n, seed, 310).reshape(n, 3)
+        self.n = n
+
+    def layer(self, rows: int, cols: int, offset: int) -> tuple:
+        idx = (np.arange(rows * cols, dtype=np.int64) + offset) % self.n
+        return (Layer(self.K[idx].reshape(rows, cols, 3), self.S[idx].reshape(rows, cols, 3),
+                      self.V[idx].reshape(rows, cols)), self.R0[idx].reshape(rows, cols, 3))
+
+
+def cfg3(n_canvas: int = 256, seed: int = 3, sides=(256, 2048), stamps=(50, 500),
+         foot=(32, 128), pool: bool = True) -> BatchWorkload:
+    """Batched: n_canvas canvases with independent sides uniform in
+    [sides], stamps uniform in [stamps], one footprint per canvas with
+    diameter uniform in [foot] (radius (d-1)//2 rounded up to a 2r+1 grid,
+    Gaussian sigma = width/4), pickup on.  With pool=True the pixel data
+    comes from a shared seeded pool (see _Pool)."""
+    st = Stream(seed, 50)
+    P = _Pool(seed) if pool else None
+    jobs = []
+    for j in range(n_canvas):
+        rows, cols = (int(x) for x in st.integers(2, sides[0], sides[1]))
+        n_st = int(st.integers(1, stamps[0], stamps[1])[0])
+        diam = int(st.integers(1, foot[0], foot[1])[0])
+        radius = max(1, diam // 2)
+        width = 2 * radius + 1
+        if P is not None:
+            off = int(st.integers(1, 0, P.n - 1)[0])
+            wet, R0 = P.layer(rows, cols, off)
+            dry = Layer(np.zeros((rows, cols, 3), np.float32),
+                        np.zeros((rows, cols, 3), np.float32), np.zeros((rows, cols), np.float32))
+        else:
+            wet, R0 = paint_layer(rows, cols, seed * 1000 + j), substrate(rows, cols,
+                                                                         seed * 1000 + j)
+            dry = paint_layer(rows, cols, seed * 1000 + j, 100)
+        sts = strokes(rows, cols, n_st, seed * 1000 + j, stamps_per_stroke=(20, 80),
+                      reach=0.15)
+        sts["brush"][:] = 0
+        jobs.append(Workload("cfg3-%d-%dx%d" % (j, rows, cols), rows, cols, dry, wet, R0, sts,
+                             [gaussian_core(width, width / 4.0)], [float(radius)]))
+    return BatchWorkload("cfg3-%d" % n_canvas, jobs)
+
+
 def small(rows: int, cols: int, n_stamps: int, radius: int, seed: int, pickup: bool = True,
           noisy: bool = False, reach: float = 0.5) -> Workload:
     """Small seeded case for parity tests."""

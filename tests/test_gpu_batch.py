@@ -1,0 +1,114 @@
+"""pnt_batch_run: many (canvas, brush, stamps) jobs in one launch, each job
+checked against the FP64 oracle running that job alone (1e-5 relative per
+channel): stamped wet layer, R0/h after drying, pickup map and composite."""
+import numpy as np
+import pytest
+
+from painty_amd import workload as W
+from tests.parity import assert_close, compare_canvas, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(hip, wl, use_snapshot=True):
+    cv = hip.Canvas(wl.rows, wl.cols)
+    cv.getPaintLayer().setBuffers(wl.wet.K, wl.wet.S, wl.wet.V)
+    cv.setR0(wl.R0)
+    dry = hip.PaintLayer(wl.rows, wl.cols)
+    dry.setBuffers(wl.dry.K, wl.dry.S, wl.dry.V)
+    dry.composeOntoCanvas(cv)
+    br = hip.FootprintBrush(wl.radii[0], wl.cores[0])
+    br.setPickupRate(wl.pickup_rate)
+    br.setDepositionRate(wl.deposition_rate)
+    br.setUseSnapshotBuffer(use_snapshot)
+    out = hip.Canvas(wl.rows, wl.cols)
+    return cv, br, out
+
+
+def _mixed_jobs(seed):
+    """Canvases of assorted sizes and brushes: pickup on/off, snapshot
+    on/off, re-dips, a dry in the middle, strokes near the edges."""
+    rng = W.Stream(seed, 77)
+    jobs = []
+    for k in range(14):
+        rows, cols = (int(x) for x in rng.integers(2, 24, 200))
+        radius = int(rng.integers(1, 2, 40)[0])
+        n = int(rng.integers(1, 0, 30)[0])
+        wl = W.small(rows, cols, n, radius, seed * 100 + k, pickup=bool(k % 3), noisy=bool(k % 2),
+                     reach=0.4)
+        if n > 4 and k % 4 == 1:
+            wl.stamps["flags"][n // 2] |= W.DRY_AFTER
+        jobs.append((wl, k % 5 != 2))
+    return jobs
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_batch_matches_oracle_per_job(hip, oracle, mode, seed):
+    jobs = _mixed_jobs(seed)
+    gpu = []
+    for wl, snap in jobs:
+        cv, br, out = _setup(hip, wl, snap)
+        gpu.append((cv, br, wl.stamps, out))
+    hip.batch_run(gpu, mode=mode)
+    for k, ((wl, snap), (cv, br, _, out)) in enumerate(zip(jobs, gpu)):
+        ocv, ob = run_oracle(wl, use_snapshot=snap)
+        tag = "batch%d/s%d/job%d(%dx%d,r%g,n%d)" % (mode, seed, k, wl.rows, wl.cols, wl.radii[0],
+                                                   len(wl.stamps))
+        compare_canvas(tag, cv, ocv, pickup=(br, ob) if ob is not None else None)
+        assert_close(tag + ".out", out.getR0(), ocv.compose())
+
+
+def test_batch_big_and_small_jobs(hip, oracle):
+    """A few large jobs among small ones, so the large ones get groups of
+    several workgroups (group barriers) and the small ones share."""
+    jobs = []
+    for k in range(6):
+        big = k < 2
+        wl = W.small(300 if big else 60, 280 if big else 70, 60 if big else 8, 40 if big else 6,
+                     40 + k, pickup=True, noisy=True, reach=0.3)
+        jobs.append(wl)
+    gpu = []
+    for wl in jobs:
+        cv, br, out = _setup(hip, wl)
+        gpu.append((cv, br, wl.stamps, out))
+    hip.batch_run(gpu, mode=0)
+    for k, (wl, (cv, br, _, out)) in enumerate(zip(jobs, gpu)):
+        ocv, ob = run_oracle(wl)
+        compare_canvas("bigsmall/job%d" % k, cv, ocv, pickup=(br, ob))
+        assert_close("bigsmall/job%d.out" % k, out.getR0(), ocv.compose())
+
+
+def test_batch_cfg3_reduced(hip, oracle):
+    """The batched configuration at reduced size: 24 canvases 128..384 px,
+    10..60 stamps, footprints 32..128 px, pixels from the shared pool."""
+    bw = W.cfg3(n_canvas=24, seed=5, sides=(128, 384), stamps=(10, 60), foot=(32, 128))
+    gpu = []
+    for wl in bw.jobs:
+        cv, br, out = _setup(hip, wl)
+        gpu.append((cv, br, wl.stamps, out))
+    hip.batch_run(gpu, mode=0)
+    for k, (wl, (cv, br, _, out)) in enumerate(zip(bw.jobs, gpu)):
+        ocv, ob = run_oracle(wl)
+        compare_canvas("cfg3r/job%d" % k, cv, ocv, pickup=(br, ob))
+        assert_close("cfg3r/job%d.out" % k, out.getR0(), ocv.compose())
+
+
+def test_batch_then_single_calls_continue_state(hip, oracle):
+    """After a batch, the same brush and canvas continue with ordinary calls
+    (snapshot, dirty bytes and paint carried over)."""
+    wl = W.small(90, 100, 30, 7, 61, pickup=True, noisy=True, reach=0.3)
+    first, second = wl.stamps[:18], wl.stamps[18:]
+    cv, br, out = _setup(hip, wl)
+    hip.batch_run([(cv, br, first, out)], mode=0)
+    br.imprintStamps(second, cv, 1)
+    ocv, ob = run_oracle(wl)
+    compare_canvas("batch+persistent", cv, ocv, pickup=(br, ob))
+
+
+def test_batch_rejects_shared_brush(hip):
+    wl = W.small(40, 40, 5, 4, 3)
+    cv1, br, out = _setup(hip, wl)
+    cv2 = hip.Canvas(40, 40)
+    with pytest.raises(hip.InvalidArgument):
+        hip.batch_run([(cv1, br, wl.stamps, None), (cv2, br, wl.stamps, None)])
