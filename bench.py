@@ -101,6 +101,85 @@ def cpu_baseline(wl, n_stamps: int, comp_rows: int):
             "t_stamps_s": t_st, "t_composite_s": t_c}
 
 
+def bench_batched(args):
+    """Batched configuration: N canvases (sides U{256..2048}, 50..500 stamps,
+    footprints 32..128 px, pickup on) stamped and composited in one launch,
+    against the same jobs issued one after another as plain launches."""
+    from painty_amd import _lib
+    from painty_amd import hip as H
+    from painty_amd import workload as W
+    t0 = time.perf_counter()
+    bw = W.cfg3(n_canvas=args.canvases)
+    log("bench: %s generated in %.1fs" % (bw.name, time.perf_counter() - t0))
+    ctx = H.default_context()
+    stream = ctx.stream
+    ev = HipEvents()
+    jobs, visits, ring, n_px, n_st = [], 0, 0, 0, 0
+    t0 = time.perf_counter()
+    for wl in bw.jobs:
+        cv = H.Canvas(wl.rows, wl.cols)
+        cv.getPaintLayer().setBuffers(wl.wet.K, wl.wet.S, wl.wet.V)
+        cv.setR0(wl.R0)
+        br = H.FootprintBrush(wl.radii[0], wl.cores[0])
+        out = H.Canvas(wl.rows, wl.cols)
+        arr = H.stamps_to_c(wl.stamps)
+        a, b = C.c_int64(), C.c_int64()
+        _lib.call("pnt_brush_stamp_traffic", br.h, wl.rows, wl.cols, arr, len(arr),
+                  C.byref(a), C.byref(b))
+        visits += a.value
+        ring += b.value
+        n_px += wl.rows * wl.cols
+        n_st += len(arr)
+        jobs.append((cv, br, arr, out))
+        wl.wet = wl.dry = wl.R0 = None  # host copies no longer needed
+    log("bench: uploaded %d canvases, %.1f Mpx, %d stamps in %.1fs" %
+        (len(jobs), n_px / 1e6, n_st, time.perf_counter() - t0))
+    alg_bytes = BYTES_VISIT * visits + BYTES_RING * ring + BYTES_COMPOSE1 * n_px
+
+    def run(mode):
+        H.batch_run(jobs, mode=mode)
+
+    for _ in range(args.warmup):
+        run(_lib.BATCH_ONE_LAUNCH)
+        ctx.synchronize()
+    a, b = ev.event(), ev.event()
+    t0 = time.perf_counter()
+    ev.record(a, stream)
+    for _ in range(args.steps):
+        run(_lib.BATCH_ONE_LAUNCH)
+    ev.record(b, stream)
+    ctx.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / args.steps
+    gpu = ev.elapsed_ms(a, b) / args.steps
+    log("bench: one launch %.2f ms/step wall, %.2f ms/step stream" % (wall, gpu))
+    a2, b2 = ev.event(), ev.event()
+    t0 = time.perf_counter()
+    ev.record(a2, stream)
+    for _ in range(args.plain_steps):
+        run(_lib.BATCH_PLAIN)
+    ev.record(b2, stream)
+    ctx.synchronize()
+    plain = (time.perf_counter() - t0) * 1e3 / args.plain_steps
+    log("bench: plain %.2f ms/step" % plain)
+    gbs = alg_bytes / (gpu * 1e-3) / 1e9
+    result = {
+        "metric": METRIC, "value": round(n_st / (wall * 1e-3), 1), "unit": "stamps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched: %d canvases (sides 256..2048), %d stamps "
+                               "(footprint 32-128 px, pickup on) + composites, one launch"
+                               % (len(jobs), n_st)},
+        "composite_gpixel_s_in_batch": round(n_px / (wall * 1e-3) / 1e9, 2),
+        "plain_sequential_ms_per_step": round(plain, 2),
+        "speedup_vs_plain": round(plain / wall, 2),
+        "roofline": {"bound": "hbm", "kernel": "k_batch", "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes_per_step": int(alg_bytes)},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,11 +191,16 @@ def main():
     ap.add_argument("--cpu-stamps", type=int, default=300)
     ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", choices=["single", "batched"], default="single")
+    ap.add_argument("--canvases", type=int, default=256)
+    ap.add_argument("--plain-steps", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               "(canvases would shard one per GPU with no exchange)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "batched":
+        return bench_batched(args)
 
     from painty_amd import _lib
     from painty_amd import hip as H
