@@ -180,18 +180,45 @@ def bench_batched(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_copy(args):
+    """Calibration: PaintLayer::copyTo of a size x size layer (k_copy_layer,
+    7 FP32 planes read and written).  Gives the achievable copy bandwidth and,
+    under a counter profiler, the FETCH_SIZE / WRITE_SIZE that a kernel of
+    known traffic reports."""
+    from painty_amd import _lib
+    from painty_amd import hip as H
+    n = args.size * args.size
+    a = H.PaintLayer(args.size, args.size)
+    b = H.PaintLayer(args.size, args.size)
+    ctx = H.default_context()
+    ev = HipEvents()
+    reps = max(args.steps, 1)
+    a.copyTo(b)
+    ctx.synchronize()
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0, ctx.stream)
+    for _ in range(reps):
+        _lib.call("pnt_layer_copy_to", a.h, b.h)
+    ev.record(e1, ctx.stream)
+    ms = ev.elapsed_ms(e0, e1) / reps
+    nbytes = 28 * n
+    print(json.dumps({"calibration": "k_copy_layer", "pixels": n, "bytes_read": nbytes,
+                      "bytes_written": nbytes, "ms": round(ms, 4),
+                      "gb_s": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["plain", "persistent"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "persistent", "fused"], default="fused")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--stamps", type=int, default=2000)
     ap.add_argument("--cpu-stamps", type=int, default=300)
     ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["single", "batched"], default="single")
+    ap.add_argument("--config", choices=["single", "batched", "copy"], default="single")
     ap.add_argument("--canvases", type=int, default=256)
     ap.add_argument("--plain-steps", type=int, default=1)
     args = ap.parse_args()
@@ -201,6 +228,8 @@ def main():
         sys.exit(2)
     if args.config == "batched":
         return bench_batched(args)
+    if args.config == "copy":
+        return bench_copy(args)
 
     from painty_amd import _lib
     from painty_amd import hip as H
@@ -227,7 +256,8 @@ def main():
     dev_r0 = _lib.DevRgb((C.c_void_p * 3)(*Rc))
     dev_out = _lib.DevRgb((C.c_void_p * 3)(*Ro))
     n_px = wl.rows * wl.cols
-    mode = _lib.EXEC_PERSISTENT if args.mode == "persistent" else _lib.EXEC_PLAIN
+    mode = {"plain": _lib.EXEC_PLAIN, "persistent": _lib.EXEC_PERSISTENT,
+            "fused": _lib.EXEC_FUSED}[args.mode]
 
     # segments: one brush geometry, no dryCanvas inside (it runs between them)
     segs = []
@@ -307,7 +337,8 @@ def main():
 
     stamps_s = args.stamps / (ms_per_step * 1e-3)
     stamp_gbs = stamp_bytes / (st_ms * 1e-3) / 1e9
-    kernel = "k_persistent" if mode == _lib.EXEC_PERSISTENT else "k_stamp_main+k_ring_refresh"
+    kernel = {"plain": "k_stamp_main+k_ring_refresh", "persistent": "k_persistent",
+              "fused": "k_stamp_fused<16>"}[args.mode]
     result = {
         "metric": METRIC, "value": round(stamps_s, 1), "unit": "stamps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),

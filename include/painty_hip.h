@@ -120,9 +120,12 @@ int pnt_canvas_dry(pnt_canvas canvas);
  * (renderer/shaders/DryingStep.compute.glsl:56-77): V leaves up to
  * `portion`, R0 takes the leaving thickness, K/S stay. */
 int pnt_canvas_dry_step(pnt_canvas canvas, float portion);
-/* get/setDryingTime -- Canvas.hxx:157-163.  Only 0 (explicit drying only)
- * and values >= the reference default (15000000 ms, Canvas.hxx:32) are
- * accepted: checkDry (Canvas.hxx:123-155) changes no paint state then. */
+/* get/setDryingTime -- Canvas.hxx:157-163.  Any value >= 0 is stored (0,
+ * the default here, means explicit drying only).  Stamping is deterministic:
+ * the wall-clock checkDry (Canvas.hxx:123-155) is not run, so only
+ * pnt_canvas_dry / PNT_STAMP_DRY_AFTER / pnt_canvas_dry_step dry paint.
+ * With the reference's default of 15000000 ms (Canvas.hxx:32) its checkDry
+ * changes no paint state either within that time. */
 int pnt_canvas_set_drying_time_ms(pnt_canvas canvas, int64_t ms);
 int pnt_canvas_get_drying_time_ms(pnt_canvas canvas, int64_t* ms);
 
@@ -186,10 +189,14 @@ typedef struct {
 } pnt_stamp;
 #define PNT_STAMP_DIP 1u
 #define PNT_STAMP_DRY_AFTER 2u
-/* mode: PNT_EXEC_PLAIN issues each stamp as its own kernel launches;
- * PNT_EXEC_PERSISTENT runs the whole list in one persistent launch. */
+/* mode: PNT_EXEC_PLAIN issues each stamp as its own kernel launches
+ * (snapshot ring copy, then the visits); PNT_EXEC_PERSISTENT runs the whole
+ * list in one persistent launch; PNT_EXEC_FUSED plans the list once and
+ * issues one launch per stamp in which the snapshot refresh (dirty pixels
+ * only) and the visits run side by side. */
 #define PNT_EXEC_PLAIN 0
 #define PNT_EXEC_PERSISTENT 1
+#define PNT_EXEC_FUSED 2
 int pnt_imprint_stamps(pnt_brush brush, pnt_canvas canvas,
                        const pnt_stamp* stamps, int64_t n, int mode);
 

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpainty_hip.so")
 OK, EINVAL, EHIP, ENOMEM, ESTATE, EKERNEL = 0, -1, -2, -3, -4, -5
 AOS, PLANAR = 0, 1
 STAMP_DIP, STAMP_DRY_AFTER = 1, 2
-EXEC_PLAIN, EXEC_PERSISTENT = 0, 1
+EXEC_PLAIN, EXEC_PERSISTENT, EXEC_FUSED = 0, 1, 2
 BATCH_ONE_LAUNCH, BATCH_PLAIN = 0, 1
 
 

@@ -983,11 +983,14 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
   return guard([&] {
     check_same_ctx(b, cv);
     PNT_REQUIRE(n >= 0 && (st || n == 0), "invalid stamp list");
-    PNT_REQUIRE(mode == PNT_EXEC_PLAIN || mode == PNT_EXEC_PERSISTENT, "unknown mode");
+    PNT_REQUIRE(mode == PNT_EXEC_PLAIN || mode == PNT_EXEC_PERSISTENT || mode == PNT_EXEC_FUSED,
+                "unknown mode");
     b->ctx->use();
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
     if (mode == PNT_EXEC_PERSISTENT) {
       run_persistent(b, cv, st, n);
+    } else if (mode == PNT_EXEC_FUSED) {
+      run_fused(b, cv, st, n);
     } else {
       for (int64_t i = 0; i < n; ++i) {
         if (st[i].flags & PNT_STAMP_DIP) brush_dip(b, st[i].K, st[i].S);

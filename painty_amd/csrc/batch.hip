@@ -41,8 +41,6 @@ namespace {
 
 constexpr int kWT = 32;                  // window tile side
 constexpr int kBT = kWT * kWT;           // threads per block
-constexpr int kHalo = kWT + 2;           // tile plus one-pixel halo
-constexpr int kU = 1;                    // window tiles per block iteration
 constexpr int64_t kComposeItem = 1 << 18;  // pixels per composite work item
 
 struct BJob {
@@ -106,114 +104,8 @@ __device__ __forceinline__ bool wait_done(unsigned* flag, unsigned* host_err, un
   return true;
 }
 
-// Ring refresh of one stamp over the group's threads: 16 dirty bytes per
-// load; a set byte is a pixel whose snapshot differs from the canvas.
-__device__ __forceinline__ void ring_scan(const BJob& J, const StampGeom& g, const PStamp& st,
-                                          bool pull, int tid, int nthreads) {
-  const RingBox& b = st.box;
-  if (b.a_x1 < b.a_x0 || b.a_y1 < b.a_y0) return;
-  const int rows = b.a_y1 - b.a_y0 + 1;
-  // 16-byte chunks covering [row*cols + a_x0, row*cols + a_x1] of each row
-  // (the dirty buffer is padded so the last chunk stays inside it)
-  const int span = b.a_x1 - b.a_x0 + 1;
-  const int per_row = (span + 15) / 16 + 1;  // +1: misaligned start
-  const int total = per_row * rows;
-  constexpr int kInFlight = 2;  // chunk loads issued before any is examined
-  // chunk e: row a_y0 + e / per_row, bytes [base, base + 16) with
-  // base = (lo & ~15) + 16 * (e % per_row), lo = row*cols + a_x0
-  auto chunk_base = [&](int e, int64_t* lo) {
-    const int row = b.a_y0 + e / per_row;
-    *lo = (int64_t)row * J.cols + b.a_x0;
-    return (*lo & ~(int64_t)15) + 16 * (int64_t)(e % per_row);
-  };
-  for (int e0 = tid; e0 < total; e0 += kInFlight * nthreads) {
-    uint4 d[kInFlight];
-#pragma unroll
-    for (int u = 0; u < kInFlight; ++u) {
-      const int e = e0 + u * nthreads;
-      d[u] = make_uint4(0u, 0u, 0u, 0u);
-      if (e >= total) continue;
-      int64_t lo;
-      const int64_t base = chunk_base(e, &lo);
-      if (base >= lo + span) continue;
-      const int row = b.a_y0 + e / per_row;
-      if (row > b.tl_y && row < b.br_y) {
-        const int x0 = (int)(base - (int64_t)row * J.cols);
-        if (x0 > b.tl_x && x0 + 15 < b.br_x) continue;  // chunk inside the window interior
-      }
-      d[u] = *reinterpret_cast<const uint4*>(J.dirty + base);
-    }
-#pragma unroll
-    for (int u = 0; u < kInFlight; ++u) {
-      if ((d[u].x | d[u].y | d[u].z | d[u].w) == 0u) continue;
-      const int e = e0 + u * nthreads;
-      const int row = b.a_y0 + e / per_row;
-      int64_t lo;
-      const int64_t base = chunk_base(e, &lo);
-      const unsigned words[4] = {d[u].x, d[u].y, d[u].z, d[u].w};
-      const bool interior_row = row > b.tl_y && row < b.br_y;
-      for (int q = 0; q < 16; ++q) {
-        if (!((words[q >> 2] >> (8 * (q & 3))) & 0xffu)) continue;
-        const int64_t i = base + q;
-        if (i < lo || i >= lo + span) continue;
-        const int col = (int)(i - (int64_t)row * J.cols);
-        if (interior_row && col > b.tl_x && col < b.br_x) continue;
-        if (pull && ring_pixel_visited(g, b, J.fp, col, row)) continue;
-#pragma unroll
-        for (int c = 0; c < 7; ++c) J.snap.p[c][i] = J.canvas.p[c][i];
-        J.dirty[i] = 0;
-      }
-    }
-  }
-}
-
-// Pickup map cell held in registers between its load and its store.
-struct RegPick {
-  float v[7];
-  __device__ float pick(int c) const { return v[c]; }
-  __device__ void set_pick(int c, float x) { v[c] = x; }
-};
-
-__device__ __forceinline__ int64_t canvas_index(const StampGeom& g, int col, int row) {
-#pragma clang fp contract(off)
-  return (int64_t)(int)(row + g.cy) * g.cols + (int)(col + g.cx);
-}
-
-// Loads everything one visit reads: canvas and pickup-source pixel.
-__device__ __forceinline__ void load_pixel(const VisitTarget& T, int64_t ic, bool pull,
-                                           CachedVisit<RegPick>& acc) {
-#pragma unroll
-  for (int c = 0; c < 7; ++c) acc.cv[c] = T.canvas->p[c][ic];
-  if (T.use_snap && !pull) {
-#pragma unroll
-    for (int c = 0; c < 7; ++c) acc.sv[c] = T.src->p[c][ic];
-  } else {
-#pragma unroll
-    for (int c = 0; c < 7; ++c) acc.sv[c] = acc.cv[c];
-  }
-  acc.src_moved = false;
-}
-
-// Runs one visit on loaded values and stores the canvas / snapshot side.
-__device__ __forceinline__ void run_pixel(const VisitTarget& T, int64_t ic, bool pull, float fh,
-                                          const float pk[3], const float ps[3],
-                                          CachedVisit<RegPick>& acc) {
-  paint_visit_math(acc, fh, T.pickup_rate, T.dep_rate, T.cap, pk, ps);
-#pragma unroll
-  for (int c = 0; c < 7; ++c) T.canvas->p[c][ic] = acc.cv[c];
-  if (T.use_snap) {
-    if (pull) {
-#pragma unroll
-      for (int c = 0; c < 7; ++c) T.src->p[c][ic] = acc.sv[c];
-    } else if (acc.src_moved) {
-      T.src->p[6][ic] = acc.sv[6];
-    }
-    T.dirty[ic] = 1;
-  }
-}
-
 __global__ __launch_bounds__(kBT) void k_batch(BArgs A) {
-  __shared__ int sm[kU][kHalo][kHalo];
+  __shared__ int sm[kWT + 2][kWT + 2];
   __shared__ int s_item;
   __shared__ int s_ok;
   const int t = threadIdx.x;
@@ -269,86 +161,15 @@ __global__ __launch_bounds__(kBT) void k_batch(BArgs A) {
         }
       }
       if (J.use_snap) {
-        ring_scan(J, g, st, pull, gtid, gthreads);
+        ring_scan<1>(J.canvas, J.snap, J.dirty, J.fp, g, st.box, pull, gtid, gthreads);
         if (dup && !group_sync(bar, nb, rank, A.host_err)) {
           alive = false;
           break;
         }
       }
-      // ---- visits: kU 32x32 window tiles per block iteration.  Every value
-      // a visit reads (footprint, pickup cell, canvas and source pixel) is
-      // loaded in one batch for all kU pixels of a thread, then the visits
-      // run; a partner's pixel is loaded in a second, smaller batch.
-      const int lx = t % kWT, ly = t / kWT;
-      for (int base = rank * kU; base < ntiles; base += (int)nb * kU) {
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const int tile = base + u;
-          if (tile >= ntiles) break;
-          const int col0 = -J.hr + (tile % tiles_x) * kWT;
-          const int row0 = -J.hr + (tile / tiles_x) * kWT;
-          for (int e = t; e < kHalo * kHalo; e += kBT) {
-            const int hy = e / kHalo, hx = e % kHalo;
-            int64_t ic;
-            sm[u][hy][hx] = stamp_visit(g, col0 + hx - 1, row0 + hy - 1, &ic);
-          }
-        }
-        __syncthreads();
-        CachedVisit<RegPick> acc[kU];
-        int mmv[kU], pdir[kU];
-        int64_t icv[kU];
-        float fhv[kU];
-        bool act[kU], pullv[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const int tile = base + u;
-          act[u] = false;
-          pdir[u] = 0;
-          if (tile >= ntiles) continue;
-          const int mm = sm[u][ly + 1][lx + 1];
-          mmv[u] = mm;
-          if (mm < 0 || sm[u][ly + 1][lx] == mm || sm[u][ly][lx + 1] == mm) continue;
-          const int col = -J.hr + (tile % tiles_x) * kWT + lx;
-          const int row = -J.hr + (tile / tiles_x) * kWT + ly;
-          pdir[u] = sm[u][ly + 1][lx + 2] == mm ? 1 : (sm[u][ly + 2][lx + 1] == mm ? 2 : 0);
-          if (dup && (on_dup_line(st, col, row) ||
-                      (pdir[u] && on_dup_line(st, col + (pdir[u] == 1), row + (pdir[u] == 2)))))
-            continue;
-          act[u] = true;
-          icv[u] = canvas_index(g, col, row);
-          pullv[u] = pull && on_window_border(st.box, (int)(icv[u] % J.cols),
-                                              (int)(icv[u] / J.cols));
-          fhv[u] = J.fp[mm];
-#pragma unroll
-          for (int c = 0; c < 7; ++c) acc[u].v[c] = J.pick.p[c][mm];
-          acc[u].alias = !J.use_snap;
-          load_pixel(T, icv[u], pullv[u], acc[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          act[u] = act[u] && fhv[u] > 0.0f;
-          if (act[u]) run_pixel(T, icv[u], pullv[u], fhv[u], pk, ps, acc[u]);
-        }
-        // partners (the second visitor of the same map cell, right or below)
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          if (!act[u]) continue;
-          if (pdir[u]) {
-            const int tile = base + u;
-            const int col = -J.hr + (tile % tiles_x) * kWT + lx + (pdir[u] == 1);
-            const int row = -J.hr + (tile / tiles_x) * kWT + ly + (pdir[u] == 2);
-            const int64_t pic = canvas_index(g, col, row);
-            const bool ppull =
-                pull && on_window_border(st.box, (int)(pic % J.cols), (int)(pic / J.cols));
-            acc[u].alias = !J.use_snap;
-            load_pixel(T, pic, ppull, acc[u]);  // pickup cell stays in acc[u].v
-            run_pixel(T, pic, ppull, fhv[u], pk, ps, acc[u]);
-          }
-#pragma unroll
-          for (int c = 0; c < 7; ++c) J.pick.p[c][mmv[u]] = acc[u].v[c];
-        }
-        __syncthreads();
-      }
+      // ---- visits: 32x32 window tiles, round-robin over the group's blocks
+      for (int tile = rank; tile < ntiles; tile += (int)nb)
+        window_tile<kWT>(T, J.pick.p, J.fp, g, st, dup, pull, tile, tiles_x, sm, pk, ps);
       // ---- visits sharing canvas column/row 0, level by level
       for (int L = 0; L < st.dl_nlev; ++L) {
         if (!group_sync(bar, nb, rank, A.host_err)) {

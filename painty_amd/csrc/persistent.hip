@@ -54,14 +54,6 @@ struct PArgs {
   unsigned* host_err; // pinned host word raised on abort
 };
 
-__device__ __forceinline__ int tile_of_block(int b, int ntiles) {
-  // blocks go round-robin to the 8 XCDs; give each XCD a contiguous run of
-  // tiles so one XCD works on one band of the map (and of the window)
-  const int x = b & 7, q = b >> 3;
-  const int base = ntiles >> 3, rem = ntiles & 7;
-  return x * base + (x < rem ? x : rem) + q;
-}
-
 template <int kT>
 struct Pick {  // LDS pickup state of this block's map tile
   float K[3][kT * kT];
@@ -413,46 +405,27 @@ static void ensure_dev(T*& p, int64_t& cap, int64_t need, hipStream_t s) {
   cap = c;
 }
 
-void run_persistent(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
-  hipStream_t s = b->ctx->stream;
-  if (*(volatile unsigned*)b->ctx->kernel_error)
-    throw Error(PNT_EKERNEL, "an earlier persistent stamp kernel aborted");
-  if (n == 0) return;
+static PersistentState& plan_state(pnt_brush_s* b) {
   if (!b->pers) {
     auto* P = new PersistentState;
     b->pers = P;
     PNT_HIP_CHECK(hipMalloc((void**)&P->d_bar, kBarWords * sizeof(unsigned)));
     PNT_HIP_CHECK(hipEventCreateWithFlags(&P->uploaded, hipEventDisableTiming));
-    int dev = 0, coop = 0, ncu = 0, per_cu = 0;
-    PNT_HIP_CHECK(hipGetDevice(&dev));
-    PNT_HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-    if (!coop) throw Error(PNT_ESTATE, "device has no cooperative launch");
-    PNT_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k_persistent<16>), 256, 0));
-    P->max_blocks16 = ncu * per_cu;
-    PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k_persistent<32>), 1024, 0));
-    P->max_blocks32 = ncu * per_cu;
   }
-  PersistentState& P = *b->pers;
-  // map tile per workgroup: 16x16 cells for small brushes, 32x32 (1024
-  // threads) for large ones, where fewer workgroups meet at each barrier
-  int kT = b->size_map >= 128 ? 32 : 16;
-  if (const char* e = std::getenv("PAINTY_PERSIST_TILE")) kT = std::atoi(e) == 32 ? 32 : 16;
-  const int tiles_x = (b->size_map + kT - 1) / kT;
-  const int ntiles = tiles_x * tiles_x;
-  if (ntiles > (kT == 32 ? P.max_blocks32 : P.max_blocks16))
-    throw Error(PNT_ESTATE, "brush map of " + std::to_string(b->size_map) +
-                                " cells per side needs more co-resident workgroups than the "
-                                "device holds; use PNT_EXEC_PLAIN");
-  // ---- plan every stamp on the host (FP64 geometry, shared-pixel levels)
+  return *b->pers;
+}
+
+// Plans stamps [0, n) on the host, readies the snapshot and dirty bytes,
+// and uploads the plan through pinned staging.  The host copy of the plan
+// stays in the brush's state until the next call.
+PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
+  hipStream_t s = b->ctx->stream;
+  PersistentState& P = plan_state(b);
   P.h_st.clear();
   P.h_dl.clear();
   P.h_off.clear();
   plan_stamps(b, cv, st, n, P.h_st, P.h_dl, P.h_off);
   brush_prepare_snapshot(b, cv, s);
-  // ---- upload the plan through pinned staging
   const size_t bst = P.h_st.size() * sizeof(PStamp);
   const size_t bdl = P.h_dl.size() * sizeof(int2);
   const size_t boff = P.h_off.size() * sizeof(int);
@@ -468,7 +441,7 @@ void run_persistent(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
   std::memcpy(P.stage, P.h_st.data(), bst);
   if (bdl) std::memcpy(P.stage + bst, P.h_dl.data(), bdl);
   if (boff) std::memcpy(P.stage + bst + bdl, P.h_off.data(), boff);
-  ensure_dev(P.d_st, P.st_cap, n, s);
+  ensure_dev(P.d_st, P.st_cap, std::max<int64_t>(n, 1), s);
   ensure_dev(P.d_dl, P.dl_cap, std::max<int64_t>((int64_t)P.h_dl.size(), 1), s);
   ensure_dev(P.d_off, P.off_cap, std::max<int64_t>((int64_t)P.h_off.size(), 1), s);
   PNT_HIP_CHECK(hipMemcpyAsync(P.d_st, P.stage, bst, hipMemcpyHostToDevice, s));
@@ -477,6 +450,39 @@ void run_persistent(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
   if (boff)
     PNT_HIP_CHECK(hipMemcpyAsync(P.d_off, P.stage + bst + bdl, boff, hipMemcpyHostToDevice, s));
   PNT_HIP_CHECK(hipEventRecord(P.uploaded, s));
+  return PlanDev{P.d_st, P.d_dl, P.d_off, P.h_st.data()};
+}
+
+void run_persistent(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
+  hipStream_t s = b->ctx->stream;
+  if (*(volatile unsigned*)b->ctx->kernel_error)
+    throw Error(PNT_EKERNEL, "an earlier persistent stamp kernel aborted");
+  if (n == 0) return;
+  PersistentState& P = plan_state(b);
+  if (!P.max_blocks16) {
+    int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+    PNT_HIP_CHECK(hipGetDevice(&dev));
+    PNT_HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (!coop) throw Error(PNT_ESTATE, "device has no cooperative launch");
+    PNT_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_persistent<16>), 256, 0));
+    P.max_blocks16 = ncu * per_cu;
+    PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_persistent<32>), 1024, 0));
+    P.max_blocks32 = ncu * per_cu;
+  }
+  // map tile per workgroup: 16x16 cells for small brushes, 32x32 (1024
+  // threads) for large ones, where fewer workgroups meet at each barrier
+  int kT = b->size_map >= 128 ? 32 : 16;
+  if (const char* e = std::getenv("PAINTY_PERSIST_TILE")) kT = std::atoi(e) == 32 ? 32 : 16;
+  const int tiles_x = (b->size_map + kT - 1) / kT;
+  const int ntiles = tiles_x * tiles_x;
+  if (ntiles > (kT == 32 ? P.max_blocks32 : P.max_blocks16))
+    throw Error(PNT_ESTATE, "brush map of " + std::to_string(b->size_map) +
+                                " cells per side needs more co-resident workgroups than the "
+                                "device holds; use PNT_EXEC_PLAIN");
+  upload_plan(b, cv, st, n);
 
   PArgs A;
   A.hr = (b->size_map - 1) / 2;

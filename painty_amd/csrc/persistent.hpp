@@ -1,6 +1,9 @@
-// Whole stamp lists in one persistent launch (PNT_EXEC_PERSISTENT).
+// Stamp lists planned on the host and run from a device-side plan:
+// PNT_EXEC_PERSISTENT (persistent.hip), PNT_EXEC_FUSED (fused.hip) and
+// batches (batch.hip).
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "painty_hip.h"
@@ -11,7 +14,19 @@ struct pnt_canvas_s;
 namespace pnt {
 
 struct PersistentState;
+struct PStamp;
 
+// A stamp list's plan on the device, and its host copy (valid until the
+// brush plans again).
+struct PlanDev {
+  const PStamp* st;
+  const int2* dl;
+  const int* dl_off;
+  const PStamp* host;
+};
+PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
+
+void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 void run_persistent(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 void persistent_release(PersistentState*& p);
 

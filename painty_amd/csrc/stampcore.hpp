@@ -171,6 +171,15 @@ __device__ __forceinline__ void cached_visit(const VisitTarget& T, const PickAcc
   }
 }
 
+// Tile of block b when blocks [0, ntiles) take one tile each: blocks go
+// round-robin to the 8 XCDs, so each XCD gets a contiguous run of tiles and
+// works on one band of the map (and of the window) in its own L2.
+__device__ __forceinline__ int tile_of_block(int b, int ntiles) {
+  const int x = b & 7, q = b >> 3;
+  const int base = ntiles >> 3, rem = ntiles & 7;
+  return x * base + (x < rem ? x : rem) + q;
+}
+
 // ----------------------------------------------------------- ring helpers -
 __device__ __forceinline__ bool on_dup_line(const PStamp& st, int col, int row) {
   return ((st.flags & kStDupX) && (int)(col + st.cx) == 0) ||
@@ -206,6 +215,197 @@ __device__ __forceinline__ bool ring_pixel_visited(const StampGeom& g, const Rin
 
 __device__ __forceinline__ bool on_window_border(const RingBox& b, int x, int y) {
   return x == b.tl_x || x == b.br_x || y == b.tl_y || y == b.br_y;
+}
+
+// Snapshot ring refresh of one stamp (updateSnapshot(canvas, center),
+// FootprintBrush.hxx:307-318) over `nthreads` threads.  The allowed box is
+// cut into 4-pixel groups at 4-aligned linear indices; a thread reads a
+// group's four dirty bytes (a set byte: snapshot differs from canvas), and
+// for the set pixels outside the window interior copies canvas -> snapshot
+// and clears the byte.  With `pull`, window-border pixels that a visit of
+// this stamp refreshes itself are left to that visit.  Each thread takes
+// kG groups per pass with all their loads in flight together (the dirty
+// strip behind a moving window is contiguous, so a pixel-serial copy would
+// wait on one round trip per pixel).  Plane and dirty buffers are padded
+// past the last pixel to a multiple of 16, so a group never leaves them.
+template <int kG>
+__device__ __forceinline__ void ring_scan(const Layer7& canvas, const Layer7& snap, uint8_t* dirty,
+                                          const float* fp, const StampGeom& g, const RingBox& b,
+                                          bool pull, int tid, int nthreads) {
+  if (b.a_x1 < b.a_x0 || b.a_y1 < b.a_y0) return;
+  const int rows = b.a_y1 - b.a_y0 + 1;
+  const int span = b.a_x1 - b.a_x0 + 1;
+  const int per_row = (span + 3) / 4 + 1;  // +1: misaligned start
+  const int total = per_row * rows;
+  for (int e0 = tid; e0 < total; e0 += kG * nthreads) {
+    unsigned mask[kG];
+    int64_t base[kG];
+#pragma unroll
+    for (int u = 0; u < kG; ++u) {
+      mask[u] = 0u;
+      base[u] = 0;
+      const int e = e0 + u * nthreads;
+      if (e >= total) continue;
+      // group e: row a_y0 + e / per_row, pixels [base, base + 4) with
+      // base = (lo & ~3) + 4 * (e % per_row), lo = row*cols + a_x0
+      const int row = b.a_y0 + e / per_row;
+      const int64_t rowi = (int64_t)row * g.cols;
+      const int64_t lo = rowi + b.a_x0;
+      const int64_t bs = (lo & ~(int64_t)3) + 4 * (int64_t)(e % per_row);
+      if (bs >= lo + span) continue;
+      const bool interior_row = row > b.tl_y && row < b.br_y;
+      const int x0 = (int)(bs - rowi);
+      if (interior_row && x0 > b.tl_x && x0 + 3 < b.br_x) continue;  // inside the window interior
+      const unsigned d = *reinterpret_cast<const unsigned*>(dirty + bs);
+      if (!d) continue;
+      unsigned m = 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (!((d >> (8 * q)) & 0xffu)) continue;
+        const int64_t i = bs + q;
+        if (i < lo || i >= lo + span) continue;
+        const int col = x0 + q;
+        if (interior_row && col > b.tl_x && col < b.br_x) continue;
+        if (pull && ring_pixel_visited(g, b, fp, col, row)) continue;
+        m |= 1u << q;
+      }
+      mask[u] = m;
+      base[u] = bs;
+    }
+    float4 v[kG][7];
+#pragma unroll
+    for (int u = 0; u < kG; ++u) {
+      if (!mask[u]) continue;
+#pragma unroll
+      for (int c = 0; c < 7; ++c) v[u][c] = reinterpret_cast<const float4*>(canvas.p[c])[base[u] >> 2];
+    }
+#pragma unroll
+    for (int u = 0; u < kG; ++u) {
+      if (!mask[u]) continue;
+      if (mask[u] == 0xfu) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) reinterpret_cast<float4*>(snap.p[c])[base[u] >> 2] = v[u][c];
+        *reinterpret_cast<unsigned*>(dirty + base[u]) = 0u;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (!((mask[u] >> q) & 1u)) continue;
+#pragma unroll
+          for (int c = 0; c < 7; ++c) snap.p[c][base[u] + q] = reinterpret_cast<const float*>(&v[u][c])[q];
+          dirty[base[u] + q] = 0;
+        }
+      }
+    }
+  }
+}
+
+// Pickup map cell held in registers between its load and its store.
+struct RegPick {
+  float v[7];
+  __device__ float pick(int c) const { return v[c]; }
+  __device__ void set_pick(int c, float x) { v[c] = x; }
+};
+
+__device__ __forceinline__ int64_t canvas_index(const StampGeom& g, int col, int row) {
+#pragma clang fp contract(off)
+  return (int64_t)(int)(row + g.cy) * g.cols + (int)(col + g.cx);
+}
+
+// Loads everything one visit reads: canvas and pickup-source pixel.
+__device__ __forceinline__ void load_pixel(const VisitTarget& T, int64_t ic, bool pull,
+                                           CachedVisit<RegPick>& acc) {
+#pragma unroll
+  for (int c = 0; c < 7; ++c) acc.cv[c] = T.canvas->p[c][ic];
+  if (T.use_snap && !pull) {
+#pragma unroll
+    for (int c = 0; c < 7; ++c) acc.sv[c] = T.src->p[c][ic];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 7; ++c) acc.sv[c] = acc.cv[c];
+  }
+  acc.src_moved = false;
+}
+
+// Runs one visit on loaded values and stores the canvas / snapshot side.
+__device__ __forceinline__ void run_pixel(const VisitTarget& T, int64_t ic, bool pull, float fh,
+                                          const float pk[3], const float ps[3],
+                                          CachedVisit<RegPick>& acc) {
+  paint_visit_math(acc, fh, T.pickup_rate, T.dep_rate, T.cap, pk, ps);
+#pragma unroll
+  for (int c = 0; c < 7; ++c) T.canvas->p[c][ic] = acc.cv[c];
+  if (T.use_snap) {
+    if (pull) {
+#pragma unroll
+      for (int c = 0; c < 7; ++c) T.src->p[c][ic] = acc.sv[c];
+    } else if (acc.src_moved) {
+      T.src->p[6][ic] = acc.sv[6];
+    }
+    T.dirty[ic] = 1;
+  }
+}
+
+// The visits of one kT x kT tile of a stamp's window, by one workgroup of
+// kT*kT threads (all of which must call it).  The map cells of the tile and
+// a one-pixel halo go to LDS (`sm`); a visit whose left or upper neighbour
+// shares its cell is left to that neighbour, which runs both in row-major
+// order (a cell has at most two visitors, and they are 4-neighbours).
+// Visits on a duplicated canvas column/row 0 (and their partners) are left
+// out; they run afterwards by dependency level.  Each visit's pickup cell,
+// canvas pixel and source pixel are loaded in one batch.
+template <int kT>
+__device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* pick,
+                                            const float* fp, const StampGeom& g,
+                                            const PStamp& st, bool dup, bool pull, int tile,
+                                            int tiles_x, int (*sm)[kT + 2], const float pk[3],
+                                            const float ps[3]) {
+  const int t = threadIdx.x;
+  const int lx = t % kT, ly = t / kT;
+  const int col0 = -g.hr + (tile % tiles_x) * kT;
+  const int row0 = -g.hr + (tile / tiles_x) * kT;
+  for (int e = t; e < (kT + 2) * (kT + 2); e += kT * kT) {
+    const int hy = e / (kT + 2), hx = e % (kT + 2);
+    int64_t ic;
+    sm[hy][hx] = stamp_visit(g, col0 + hx - 1, row0 + hy - 1, &ic);
+  }
+  __syncthreads();
+  const int mm = sm[ly + 1][lx + 1];
+  bool act = false;
+  int pdir = 0;
+  int64_t ic = 0;
+  bool pv = false;
+  float fh = 0.0f;
+  CachedVisit<RegPick> acc;
+  if (mm >= 0 && sm[ly + 1][lx] != mm && sm[ly][lx + 1] != mm) {
+    const int col = col0 + lx, row = row0 + ly;
+    pdir = sm[ly + 1][lx + 2] == mm ? 1 : (sm[ly + 2][lx + 1] == mm ? 2 : 0);
+    if (!(dup && (on_dup_line(st, col, row) ||
+                  (pdir && on_dup_line(st, col + (pdir == 1), row + (pdir == 2)))))) {
+      act = true;
+      ic = canvas_index(g, col, row);
+      pv = pull && on_window_border(st.box, (int)(ic % g.cols), (int)(ic / g.cols));
+      fh = fp[mm];
+#pragma unroll
+      for (int c = 0; c < 7; ++c) acc.v[c] = pick[c][mm];
+      acc.alias = !T.use_snap;
+      load_pixel(T, ic, pv, acc);
+    }
+  }
+  act = act && fh > 0.0f;
+  if (act) {
+    run_pixel(T, ic, pv, fh, pk, ps, acc);
+    if (pdir) {  // the second visitor of the cell, right or below
+      const int col = col0 + lx + (pdir == 1);
+      const int row = row0 + ly + (pdir == 2);
+      const int64_t pic = canvas_index(g, col, row);
+      const bool ppull = pull && on_window_border(st.box, (int)(pic % g.cols), (int)(pic / g.cols));
+      acc.alias = !T.use_snap;
+      load_pixel(T, pic, ppull, acc);  // the pickup cell stays in acc.v
+      run_pixel(T, pic, ppull, fh, pk, ps, acc);
+    }
+#pragma unroll
+    for (int c = 0; c < 7; ++c) pick[c][mm] = acc.v[c];
+  }
+  __syncthreads();  // sm is rewritten by the block's next tile
 }
 
 // ------------------------------------------------------------- host side --

@@ -1,0 +1,7 @@
+// Everything of the drop-in interface.
+#pragma once
+
+#include "painty/core/KubelkaMunk.hxx"
+#include "painty/mixer/PaintMixer.hxx"
+#include "painty/renderer/FootprintBrush.hxx"
+#include "painty/renderer/Renderer.hxx"

@@ -1,0 +1,203 @@
+// Drives the C++ drop-in classes (painty_amd/include/painty) the way the
+// reference's own code drives its templates: Canvas / PaintLayer buffers
+// written through their getters, FootprintBrush dip / imprint / dryCanvas,
+// Renderer::compose, PaintMixer.  tests/test_gpu_dropin.py writes the
+// inputs, runs this program on the GPU and checks its outputs against the
+// FP64 oracle.
+//
+//   test_dropin km <in.txt> <out.txt>       ComputeReflectance per case
+//   test_dropin stamp <in.bin> <out.bin>    one canvas, one brush, a stamp list
+//   test_dropin api                         mixer and error behaviour
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "painty/painty.hxx"
+
+using painty::Canvas;
+using painty::FootprintBrush;
+using painty::Mat;
+using painty::PaintLayer;
+using painty::Renderer;
+using painty::vec2;
+using painty::vec3;
+
+namespace {
+
+int km(const char* in, const char* out) {
+  std::ifstream f(in);
+  std::ofstream o(out);
+  int n = 0;
+  f >> n;
+  for (int i = 0; i < n; ++i) {
+    vec3 K, S, R0;
+    double d;
+    f >> K[0] >> K[1] >> K[2] >> S[0] >> S[1] >> S[2] >> R0[0] >> R0[1] >> R0[2] >> d;
+    const vec3 R = painty::ComputeReflectance(K, S, R0, d);
+    o.precision(9);
+    o << R[0] << " " << R[1] << " " << R[2] << "\n";
+  }
+  return 0;
+}
+
+template <class T>
+void read(std::ifstream& f, T* p, size_t n) {
+  f.read(reinterpret_cast<char*>(p), sizeof(T) * n);
+  if (!f) throw std::runtime_error("short input");
+}
+template <class T>
+void write(std::ofstream& f, const T* p, size_t n) {
+  f.write(reinterpret_cast<const char*>(p), sizeof(T) * n);
+}
+
+struct Stamp {
+  double cx, cy, theta;
+  float K[3], S[3];
+  uint32_t flags;
+  int32_t pad;
+};
+
+int stamp(const char* in, const char* out) {
+  std::ifstream f(in, std::ios::binary);
+  int32_t hdr[5];
+  read(f, hdr, 5);
+  const int rows = hdr[0], cols = hdr[1], width = hdr[2], nst = hdr[3], snap = hdr[4];
+  float rates[3];
+  read(f, rates, 3);
+  const size_t n = (size_t)rows * cols;
+  std::vector<float> K(3 * n), S(3 * n), V(n), R0(3 * n), fp((size_t)width * width);
+  read(f, K.data(), 3 * n);
+  read(f, S.data(), 3 * n);
+  read(f, V.data(), n);
+  read(f, R0.data(), 3 * n);
+  read(f, fp.data(), fp.size());
+  std::vector<Stamp> st(nst);
+  read(f, st.data(), st.size());
+
+  Canvas<vec3> canvas(rows, cols);
+  {  // the reference's way of loading a canvas: write through the getters
+    PaintLayer<vec3>& L = canvas.getPaintLayer();
+    Mat<vec3>& Kb = L.getK_buffer();
+    Mat<vec3>& Sb = L.getS_buffer();
+    Mat<double>& Vb = L.getV_buffer();
+    Mat<vec3>& R0b = canvas.getR0();
+    for (size_t i = 0; i < n; ++i) {
+      for (int c = 0; c < 3; ++c) {
+        Kb((int64_t)i)[c] = K[3 * i + c];
+        Sb((int64_t)i)[c] = S[3 * i + c];
+        R0b((int64_t)i)[c] = R0[3 * i + c];
+      }
+      Vb((int64_t)i) = V[i];
+    }
+  }
+  Mat<double> core(width, width);
+  for (size_t i = 0; i < fp.size(); ++i) core((int64_t)i) = fp[i];
+  FootprintBrush<vec3> brush(rates[2], core);
+  brush.setPickupRate(rates[0]);
+  brush.setDepositionRate(rates[1]);
+  brush.setUseSnapshotBuffer(snap != 0);
+  for (const Stamp& s : st) {
+    if (s.flags & PNT_STAMP_DIP)
+      brush.dip({vec3(s.K[0], s.K[1], s.K[2]), vec3(s.S[0], s.S[1], s.S[2])});
+    brush.imprint(vec2(s.cx, s.cy), s.theta, canvas);
+    if (s.flags & PNT_STAMP_DRY_AFTER) canvas.dryCanvas();
+  }
+  const Mat<vec3> R = Renderer<vec3>().compose(canvas);
+
+  std::ofstream o(out, std::ios::binary);
+  const Canvas<vec3>& cc = canvas;
+  const PaintLayer<vec3>& L = cc.getPaintLayer();
+  auto put3 = [&](const Mat<vec3>& m) {
+    for (const vec3& v : m) write(o, v.data(), 3);
+  };
+  put3(L.getK_buffer());
+  put3(L.getS_buffer());
+  write(o, L.getV_buffer().data(), n);
+  put3(cc.getR0());
+  write(o, cc.get_h().data(), n);
+  put3(R);
+  const PaintLayer<vec3>& P = brush.getPickupMap();
+  put3(P.getK_buffer());
+  put3(P.getS_buffer());
+  write(o, P.getV_buffer().data(), P.getV_buffer().total());
+  return 0;
+}
+
+#define EXPECT(cond)                                                     \
+  do {                                                                   \
+    if (!(cond)) {                                                       \
+      std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+      return 1;                                                          \
+    }                                                                    \
+  } while (0)
+
+template <class F>
+bool throws_invalid(F f) {
+  try {
+    f();
+  } catch (const std::invalid_argument&) {
+    return true;
+  }
+  return false;
+}
+
+int api() {
+  // PaintMixer (PaintMixer.cxx:327-355, 536-548)
+  painty::Palette pal(3);
+  for (int l = 0; l < 3; ++l)
+    for (int c = 0; c < 3; ++c) {
+      pal[l].K[c] = 0.1 * (l + 1) + 0.01 * c;
+      pal[l].S[c] = 0.5 * (l + 1) + 0.02 * c;
+    }
+  painty::PaintMixer mixer(pal);
+  const auto p1 = mixer.mixSinglePaint({0.0, 1.0, 0.0});
+  for (int c = 0; c < 3; ++c) EXPECT(p1.K[c] == pal[1].K[c] && p1.S[c] == pal[1].S[c]);
+  const auto p2 = mixer.mixSinglePaint({2.0, 2.0, 0.0});  // normalised: mean of 0 and 1
+  for (int c = 0; c < 3; ++c) EXPECT(std::abs(p2.K[c] - 0.5 * (pal[0].K[c] + pal[1].K[c])) < 1e-12);
+  EXPECT(throws_invalid([&] { mixer.mixSinglePaint({1.0, 0.0}); }));
+  const auto m = mixer.mixed(pal[0], 1.0, pal[2], 3.0);
+  for (int c = 0; c < 3; ++c) EXPECT(std::abs(m.K[c] - 0.25 * pal[0].K[c] - 0.75 * pal[2].K[c]) < 1e-12);
+  // Canvas / PaintLayer lifecycle (CanvasTest, PaintLayerTest)
+  Canvas<vec3> cv(8, 6);
+  EXPECT(cv.getR0().rows == 8 && cv.getR0().cols == 6);
+  EXPECT(cv.getR0()(3, 2)[1] == 1.0);  // background (1,1,1)
+  cv.getPaintLayer().set(2, 3, vec3(0.5, 0.6, 0.7), vec3(1.0, 1.1, 1.2), 0.25);
+  EXPECT(std::abs(cv.getPaintLayer().getV_buffer()(2, 3) - 0.25) < 1e-7);
+  cv.dryCanvas();
+  EXPECT(std::abs(cv.get_h()(2, 3) - 0.25) < 1e-7);
+  EXPECT(cv.getPaintLayer().getV_buffer()(2, 3) == 0.0);
+  PaintLayer<vec3> other(2, 2);
+  cv.getPaintLayer().copyTo(other);
+  EXPECT(other.getRows() == 8 && other.getCols() == 6);
+  Mat<vec3> r0(1, 1);
+  other.composeOnto(r0);  // reallocated to the layer's size, filled with 1
+  EXPECT(r0.rows == 8 && r0.cols == 6);
+  // errors map to the reference's exception types
+  EXPECT(throws_invalid([] { Canvas<vec3> bad(-1, 4); }));
+  EXPECT(throws_invalid([] { FootprintBrush<vec3> b(3.0, Mat<double>(6, 6, 0.5)); }));
+  FootprintBrush<vec3> b(3.0);  // default footprint source: width 7
+  EXPECT(b.getFootprint().rows == 10);  // sizeMap = ceil(sqrt(2) * 7)
+  EXPECT(std::abs(b.getPickupRate() - 0.9) < 1e-7 && b.getUseSnapshotBuffer());
+  std::printf("api ok\n");
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  try {
+    const std::string mode = argc > 1 ? argv[1] : "";
+    if (mode == "km" && argc == 4) return km(argv[2], argv[3]);
+    if (mode == "stamp" && argc == 4) return stamp(argv[2], argv[3]);
+    if (mode == "api") return api();
+    std::fprintf(stderr, "usage: test_dropin km|stamp <in> <out> | api\n");
+    return 2;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "test_dropin: %s\n", e.what());
+    return 1;
+  }
+}

@@ -84,3 +84,21 @@ def test_fp32_thresholds_match_fp64_comparisons():
     mv = np.float32(0.001)                         # FootprintBrush.hxx:26, x > 0.001
     for x in (mv, np.nextafter(mv, np.float32(0)), np.nextafter(mv, np.float32(1))):
         assert (x >= mv) == (float(x) > 0.001)
+
+
+def test_cpp_dropin_headers_compile():
+    """The C++ drop-in classes compile cleanly against the C ABI header."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I", os.path.join(ROOT, "include"),
+                        "-I", os.path.join(ROOT, "painty_amd", "include"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_cpp_dropin_program_links():
+    """The drop-in test program is built and starts (usage path, no device call)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+    assert os.path.exists(exe), "build with make -C tests/cpp"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

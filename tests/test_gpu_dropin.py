@@ -1,0 +1,96 @@
+"""The C++ drop-in classes (painty_amd/include/painty: Canvas, PaintLayer,
+FootprintBrush, Renderer, PaintMixer, ComputeReflectance) driven by a C++
+program the way the reference's code drives its templates, checked against
+the reference's own known answers and the FP64 oracle (1e-5 relative per
+channel)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from painty_amd import workload as W
+from tests.parity import assert_close, run_oracle
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+STAMP = np.dtype([("cx", "<f8"), ("cy", "<f8"), ("theta", "<f8"), ("K", "<f4", 3),
+                  ("S", "<f4", 3), ("flags", "<u4"), ("pad", "<i4")])
+
+
+def _run(*args):
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/test_dropin is not built (make -C tests/cpp)")
+    print("run:", " ".join(args), flush=True)
+    r = subprocess.run([BIN] + list(args), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_km_reference_goldens_through_cpp(tmp_path, oracle):
+    """KubelkaMunkTest.cxx:14-31 values through painty::ComputeReflectance."""
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_goldens.json")))
+    km = g["KubelkaMunk.Reflectance"]
+    cases = [(km["K"], km["S"], km["R0"], km["d"]), (km["K"], km["S"], km["R0"], 0.0),
+             (km["K"], km["S_zero"], km["R0"], km["d"])]
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        cases.append((np.exp(rng.uniform(np.log(0.01), np.log(10), 3)),
+                      np.exp(rng.uniform(np.log(0.1), np.log(5), 3)),
+                      rng.uniform(0.7, 0.95, 3), rng.uniform(0, 0.5)))
+    with open(tmp_path / "in.txt", "w") as f:
+        f.write("%d\n" % len(cases))
+        for K, S, R0, d in cases:
+            f.write(" ".join("%.17g" % x for x in list(K) + list(S) + list(R0) + [d]) + "\n")
+    _run("km", str(tmp_path / "in.txt"), str(tmp_path / "out.txt"))
+    R = np.loadtxt(tmp_path / "out.txt").reshape(-1, 3)
+    # the reference's own assertions (eps 1e-5)
+    assert np.all(np.abs(R[0] - np.array(km["R"])) < km["eps"])
+    assert abs(R[1][0] - km["d0_R0_channel0"]) < km["eps"]
+    assert abs(R[2][0] - km["S_zero_R_channel0"]) < km["eps"]
+    ref = np.array([oracle.compute_reflectance(np.array(K, float), np.array(S, float),
+                                               np.array(R0, float), float(d))
+                    for K, S, R0, d in cases])
+    assert_close("cpp.km", R, ref)
+
+
+@pytest.mark.parametrize("snapshot", [True, False])
+def test_stamp_scenario_through_cpp(tmp_path, oracle, snapshot):
+    wl = W.small(60, 70, 30, 6, 21, pickup=True, noisy=True, reach=0.4)
+    st = wl.stamps.copy()
+    st["flags"][12] |= W.DRY_AFTER
+    n = wl.rows * wl.cols
+    rec = np.zeros(len(st), STAMP)
+    for f in ("cx", "cy", "theta", "K", "S", "flags"):
+        rec[f] = st[f]
+    core = wl.cores[0].astype(np.float32)
+    with open(tmp_path / "in.bin", "wb") as f:
+        np.array([wl.rows, wl.cols, core.shape[0], len(st), int(snapshot)], "<i4").tofile(f)
+        np.array([wl.pickup_rate, wl.deposition_rate, wl.radii[0]], "<f4").tofile(f)
+        for a in (wl.wet.K, wl.wet.S, wl.wet.V, wl.R0, core):
+            np.ascontiguousarray(a, dtype="<f4").tofile(f)
+        rec.tofile(f)
+    _run("stamp", str(tmp_path / "in.bin"), str(tmp_path / "out.bin"))
+    out = np.fromfile(tmp_path / "out.bin", "<f8")
+    ocv, ob = run_oracle(wl, st, bake_dry=False, use_snapshot=snapshot)
+    m = ob.size_map * ob.size_map
+    sizes = [3 * n, 3 * n, n, 3 * n, n, 3 * n, 3 * m, 3 * m, m]
+    assert out.size == sum(sizes)
+    parts = np.split(out, np.cumsum(sizes)[:-1])
+    K, S, V, R0, h, R, pK, pS, pV = parts
+    tag = "cpp.stamp/snap%d" % snapshot
+    assert_close(tag + ".K", K.reshape(wl.rows, wl.cols, 3), ocv.K)
+    assert_close(tag + ".S", S.reshape(wl.rows, wl.cols, 3), ocv.S)
+    assert_close(tag + ".V", V.reshape(wl.rows, wl.cols), ocv.V)
+    assert_close(tag + ".R0", R0.reshape(wl.rows, wl.cols, 3), ocv.R0)
+    assert_close(tag + ".h", h.reshape(wl.rows, wl.cols), ocv.h)
+    assert_close(tag + ".R", R.reshape(wl.rows, wl.cols, 3), ocv.compose())
+    oK, oS, oV = ob.pickup_map()
+    assert_close(tag + ".pickup.K", pK.reshape(oK.shape), oK)
+    assert_close(tag + ".pickup.V", pV.reshape(oV.shape), oV)
+
+
+def test_mixer_lifecycle_and_errors_through_cpp():
+    assert "api ok" in _run("api")

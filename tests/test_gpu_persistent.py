@@ -1,7 +1,9 @@
-"""The persistent one-launch stamp path (PNT_EXEC_PERSISTENT) against the
-FP64 oracle, and the bookkeeping that lets it skip no-op snapshot copies:
-every way the canvas or the snapshot can change behind its back must make it
-copy again.  Tolerance 1e-5 relative per channel."""
+"""The planned stamp paths -- one persistent launch per list
+(PNT_EXEC_PERSISTENT) and one fused launch per stamp (PNT_EXEC_FUSED) --
+against the FP64 oracle, and the bookkeeping that lets them skip no-op
+snapshot copies: every way the canvas or the snapshot can change behind
+their back must make them copy again.  Tolerance 1e-5 relative per
+channel."""
 import math
 
 import numpy as np
@@ -14,40 +16,46 @@ from tests.test_gpu_stamp import HAND, PA, PB, _stamp_array
 pytestmark = pytest.mark.gpu
 PERSIST = 1
 PLAIN = 0
+FUSED = 2
+
+
+@pytest.fixture(params=[PERSIST, FUSED], ids=["persistent", "fused"])
+def lmode(request):
+    return request.param
 
 
 @pytest.mark.parametrize("case", sorted(HAND))
 @pytest.mark.parametrize("snapshot", [True, False])
 @pytest.mark.parametrize("radius", [4, 9, 20])
-def test_hand_cases(hip, oracle, case, snapshot, radius):
+def test_hand_cases(hip, oracle, case, snapshot, radius, lmode):
     wl = W.small(28, 24, 0, radius, 7)
     st = _stamp_array(HAND[case])
     ocv, ob = run_oracle(wl, st, use_snapshot=snapshot)
-    gcv, gb = run_gpu(wl, st, use_snapshot=snapshot, mode=PERSIST)
-    compare_canvas("P/%s/snap%d/r%d" % (case, snapshot, radius), gcv, ocv, pickup=(gb, ob))
+    gcv, gb = run_gpu(wl, st, use_snapshot=snapshot, mode=lmode)
+    compare_canvas("%s/" % lmode + "%s/snap%d/r%d" % (case, snapshot, radius), gcv, ocv, pickup=(gb, ob))
 
 
 @pytest.mark.parametrize("seed", [1, 2])
 @pytest.mark.parametrize("radius,noisy", [(3, False), (8, True), (16, False), (40, True)])
-def test_seeded_strokes(hip, oracle, seed, radius, noisy):
+def test_seeded_strokes(hip, oracle, seed, radius, noisy, lmode):
     wl = W.small(96, 112, 40, radius, seed, pickup=True, noisy=noisy)
     ocv, ob = run_oracle(wl)
-    gcv, gb = run_gpu(wl, mode=PERSIST)
-    compare_canvas("P/strokes/s%d/r%d" % (seed, radius), gcv, ocv, pickup=(gb, ob))
+    gcv, gb = run_gpu(wl, mode=lmode)
+    compare_canvas("%s/" % lmode + "strokes/s%d/r%d" % (seed, radius), gcv, ocv, pickup=(gb, ob))
 
 
-def test_cfg1_pickup(hip, oracle):
+def test_cfg1_pickup(hip, oracle, lmode):
     wl = W.cfg1(pickup=True)
     ocv, ob = run_oracle(wl)
-    gcv, gb = run_gpu(wl, mode=PERSIST)
-    compare_canvas("P/cfg1", gcv, ocv, pickup=(gb, ob))
+    gcv, gb = run_gpu(wl, mode=lmode)
+    compare_canvas("%s/" % lmode + "cfg1", gcv, ocv, pickup=(gb, ob))
 
 
-def test_multi_brush_with_drying(hip, oracle):
+def test_multi_brush_with_drying(hip, oracle, lmode):
     wl = W.cfg2(seed=4, size=512, n_stamps=150, dry_every=50, foot=(41, 129))
     ocv, ob = run_oracle(wl)
-    gcv, gb = run_gpu(wl, mode=PERSIST)
-    compare_canvas("P/cfg2-small", gcv, ocv, pickup=(gb, ob))
+    gcv, gb = run_gpu(wl, mode=lmode)
+    compare_canvas("%s/" % lmode + "cfg2-small", gcv, ocv, pickup=(gb, ob))
 
 
 def _pair(rows, cols, radius, seed):
@@ -87,38 +95,38 @@ def _run(ob, gb, ocv, gcv, st, mode):
     gb.imprintStamps(st, gcv, mode)
 
 
-def test_mode_mixing_and_external_writes(hip, oracle):
+def test_mode_mixing_and_external_writes(hip, oracle, lmode):
     wl, core, ob, gb, ocv, gcv = _pair(60, 70, 6, 3)
-    _run(ob, gb, ocv, gcv, _stroke(10, 12, 8), PERSIST)
+    _run(ob, gb, ocv, gcv, _stroke(10, 12, 8), lmode)
     _run(ob, gb, ocv, gcv, _stroke(24, 20, 6, paint=PB), PLAIN)      # plain in between
-    _run(ob, gb, ocv, gcv, _stroke(30, 25, 8, paint=None), PERSIST)
+    _run(ob, gb, ocv, gcv, _stroke(30, 25, 8, paint=None), lmode)
     # external write of the wet layer (host upload) between persistent runs
     K = gcv.getPaintLayer().getK_buffer()
     K[20:40, 20:40] *= 0.5
     gcv.getPaintLayer().setBuffers(K=K)
     ocv.K[...] = K.astype(np.float64)
-    _run(ob, gb, ocv, gcv, _stroke(25, 22, 10, dx=-1.1, paint=None), PERSIST)
+    _run(ob, gb, ocv, gcv, _stroke(25, 22, 10, dx=-1.1, paint=None), lmode)
     # dryCanvas through the API, then more stamps
     ocv.dry()
     gcv.dryCanvas()
-    _run(ob, gb, ocv, gcv, _stroke(15, 30, 10, paint=PB), PERSIST)
-    compare_canvas("P/mixing", gcv, ocv, pickup=(gb, ob))
+    _run(ob, gb, ocv, gcv, _stroke(15, 30, 10, paint=PB), lmode)
+    compare_canvas("%s/" % lmode + "mixing", gcv, ocv, pickup=(gb, ob))
 
 
-def test_two_brushes_one_canvas(hip, oracle):
+def test_two_brushes_one_canvas(hip, oracle, lmode):
     wl, core, ob1, gb1, ocv, gcv = _pair(64, 64, 5, 4)
     ob2 = oracle.Brush(9.0, W.gaussian_core(19, 4.0).astype(np.float64))
     from painty_amd import hip as H
     gb2 = H.FootprintBrush(9.0, W.gaussian_core(19, 4.0))
     for k in range(3):
         _run(ob1, gb1, ocv, gcv, _stroke(10 + 5 * k, 10 + 3 * k, 7, paint=PA if k == 0 else None),
-             PERSIST)
+             lmode)
         _run(ob2, gb2, ocv, gcv, _stroke(30 - 4 * k, 12 + 5 * k, 7, dx=-1.3,
-                                         paint=PB if k == 0 else None), PERSIST)
-    compare_canvas("P/two-brushes", gcv, ocv, pickup=(gb1, ob1))
+                                         paint=PB if k == 0 else None), lmode)
+    compare_canvas("%s/" % lmode + "two-brushes", gcv, ocv, pickup=(gb1, ob1))
 
 
-def test_brush_on_two_canvases_same_size(hip, oracle):
+def test_brush_on_two_canvases_same_size(hip, oracle, lmode):
     # the snapshot is not re-copied for a canvas of the same size (the
     # reference keeps the stale snapshot, FootprintBrush.hxx:281-284)
     wl, core, ob, gb, ocv, gcv = _pair(50, 50, 5, 5)
@@ -130,19 +138,19 @@ def test_brush_on_two_canvases_same_size(hip, oracle):
     gcv2 = H.Canvas(50, 50)
     gcv2.getPaintLayer().setBuffers(wl2.wet.K, wl2.wet.S, wl2.wet.V)
     gcv2.setR0(wl2.R0)
-    _run(ob, gb, ocv, gcv, _stroke(10, 10, 9), PERSIST)
-    _run(ob, gb, ocv2, gcv2, _stroke(12, 11, 9, paint=None), PERSIST)
-    _run(ob, gb, ocv, gcv, _stroke(20, 15, 9, paint=None), PERSIST)
-    compare_canvas("P/canvas1", gcv, ocv, pickup=(gb, ob))
-    compare_canvas("P/canvas2", gcv2, ocv2)
+    _run(ob, gb, ocv, gcv, _stroke(10, 10, 9), lmode)
+    _run(ob, gb, ocv2, gcv2, _stroke(12, 11, 9, paint=None), lmode)
+    _run(ob, gb, ocv, gcv, _stroke(20, 15, 9, paint=None), lmode)
+    compare_canvas("%s/" % lmode + "canvas1", gcv, ocv, pickup=(gb, ob))
+    compare_canvas("%s/" % lmode + "canvas2", gcv2, ocv2)
 
 
-def test_persistent_equals_plain_bitwise_on_one_stroke(hip):
-    """Both GPU paths run the same FP32 visit code; with one stroke of stamps
+def test_planned_paths_equal_plain_bitwise_on_one_stroke(hip, lmode):
+    """All GPU paths run the same FP32 visit code; with one stroke of stamps
     (no shared pixels) they agree to the last bit."""
     wl = W.small(80, 80, 30, 7, 9, pickup=True, noisy=True, reach=0.2)
     a, _ = run_gpu(wl, mode=PLAIN)
-    b, _ = run_gpu(wl, mode=PERSIST)
+    b, _ = run_gpu(wl, mode=lmode)
     for get in ("getK_buffer", "getS_buffer", "getV_buffer"):
         x = getattr(a.getPaintLayer(), get)()
         y = getattr(b.getPaintLayer(), get)()
