@@ -16,6 +16,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -180,6 +181,183 @@ def bench_batched(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_diff(args):
+    """Differentiable configuration: 2048^2 canvas, 500 stamps (65 px
+    Gaussian footprint, pickup on), 64 paints x 14 pigments; one step =
+    forward (stamps + composite, checkpoints) + L2 loss gradient + backward
+    (recompute + adjoints) for K, S, V, R0 and W."""
+    import torch
+
+    from painty_amd import diff as D
+    from painty_amd import hip as H
+    from painty_amd import workload as W
+    t0 = time.perf_counter()
+    dw = W.cfg4(size=args.size if args.size != 4096 else 2048, n_stamps=500)
+    wl = dw.wl
+    log("bench: %s generated in %.1fs" % (wl.name, time.perf_counter() - t0))
+    dev = "cuda:0"
+    pl = lambda a: torch.from_numpy(np.ascontiguousarray(a.transpose(2, 0, 1) if a.ndim == 3 else a)).to(dev)
+    K, S, V, R0 = pl(wl.wet.K), pl(wl.wet.S), pl(wl.wet.V), pl(wl.R0)
+    T = pl(dw.target)
+    br = H.FootprintBrush(wl.radii[0], wl.cores[0])
+    br.setPickupRate(wl.pickup_rate)
+    run = D.GradRunner(wl.rows, wl.cols, br, wl.stamps, dw.paint_of, dw.palK, dw.palS,
+                       dw.W.shape[0], seg_len=args.seg_len)
+
+    def step():
+        R = run.forward(K, S, V, R0, dw.W)
+        gR = 2.0 * (R - T)
+        torch.cuda.synchronize()
+        return run.backward(gR)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    fw = bw = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        R = run.forward(K, S, V, R0, dw.W)
+        gR = 2.0 * (R - T)
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        run.backward(gR)
+        torch.cuda.synchronize()
+        fw += b - a
+        bw += time.perf_counter() - b
+    wall = (time.perf_counter() - t0) * 1e3 / args.steps
+    n_st = len(wl.stamps)
+    result = {
+        "metric": METRIC, "value": round(n_st / (wall * 1e-3), 1), "unit": "stamps/s (fwd+bwd)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "differentiable: %dx%d canvas, %d stamps (65 px footprint, pickup "
+                               "on), 64 paints x 14 pigments, L2 loss; gradients of K, S, V, R0, W"
+                               % (wl.rows, wl.cols, n_st), "segment": args.seg_len},
+        "forward_ms": round(fw * 1e3 / args.steps, 3),
+        "backward_ms": round(bw * 1e3 / args.steps, 3),
+    }
+    print(json.dumps(result), flush=True)
+
+
+def served_requests(n, seed, side=(512, 4096), stamps=(10, 1000), foot=(32, 128),
+                    follow=0.3, recent=32):
+    """The served configuration's request stream: square canvases with side
+    log-uniform in `side`, stamps U{stamps}, footprint diameter U{foot},
+    `follow` of the requests going to one of the `recent` last new canvases."""
+    from painty_amd import workload as W
+    rs = W.Stream(seed, 95)
+    pool = [W.strokes(1000, 1000, stamps[1], seed * 1000 + k, stamps_per_stroke=(20, 80),
+                      reach=0.15) for k in range(16)]
+    cores = {}
+    reqs, created = [], []
+    for i in range(n):
+        if created and rs.uniform(1)[0] < follow:
+            cid, s_ = created[int(rs.integers(1, max(0, len(created) - recent), len(created) - 1)[0])]
+        else:
+            s_ = int(round(math.exp(rs.uniform(1, math.log(side[0]), math.log(side[1]))[0])))
+            cid = len(created)
+            created.append((cid, s_))
+        n_st = int(rs.integers(1, stamps[0], stamps[1])[0])
+        st = pool[i % len(pool)][:n_st].copy()
+        st["cx"] *= s_ / 1000.0
+        st["cy"] *= s_ / 1000.0
+        st["flags"][0] |= W.DIP
+        d = int(rs.integers(1, foot[0], foot[1])[0])
+        r = max(1, d // 2)
+        if r not in cores:
+            cores[r] = W.gaussian_core(2 * r + 1, (2 * r + 1) / 4.0)
+        reqs.append(dict(canvas_id=cid, rows=s_, cols=s_, stamps=st, radius=float(r),
+                         footprint=cores[r], background=(0.85, 0.85, 0.85)))
+    return reqs
+
+
+def bench_served(args):
+    """Served configuration: open-loop Poisson arrivals at each rate; request =
+    stamps + composite on a resident (or new) canvas; p50/p99 latency and
+    requests/s, against the plain per-request path (each request alone:
+    canvas created, PNT_EXEC_PLAIN stamps, composite, synchronised)."""
+    import threading
+
+    from painty_amd import _lib
+    from painty_amd import hip as H
+    from painty_amd import serve as SV
+    rates = [int(x) for x in args.rates.split(",")]
+    srv = SV.Server(n_slots=args.slots, hbm_budget_bytes=int(args.hbm_gb * (1 << 30)))
+    per_rate = {}
+    seed = 5
+    for rate in rates:
+        n = max(50, int(rate * args.seconds))
+        reqs = served_requests(n, seed)
+        seed += 1
+        offset = 1_000_000 * seed  # fresh canvas ids per rate
+        arr = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n))
+        tickets = [None] * n
+        lat = [None] * n
+        err = [0]
+
+        def waiter():
+            for i in range(n):
+                while tickets[i] is None:
+                    time.sleep(0.0002)
+                try:
+                    lat[i] = srv.wait(tickets[i])
+                except _lib.PaintyError:
+                    err[0] += 1
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for i, q in enumerate(reqs):
+            while time.perf_counter() - t0 < arr[i]:
+                time.sleep(0.0001)
+            q = dict(q, canvas_id=q["canvas_id"] + offset)
+            tickets[i] = srv.submit(**q)
+        th.join()
+        el = time.perf_counter() - t0
+        ok = np.array([x for x in lat if x is not None])
+        per_rate[rate] = {"requests": n, "errors": err[0], "req_s": round(len(ok) / el, 1),
+                          "p50_ms": round(float(np.percentile(ok, 50)), 2),
+                          "p99_ms": round(float(np.percentile(ok, 99)), 2)}
+        log("bench: served %d req/s offered: %s" % (rate, per_rate[rate]))
+    st = srv.stats()
+    srv.close()
+    # plain per-request path on a sample of the same stream
+    sample = served_requests(args.plain_requests, 99)
+    ctx = H.default_context()
+    t0 = time.perf_counter()
+    lat_plain = []
+    for q in sample:
+        a = time.perf_counter()
+        cv = H.Canvas(q["rows"], q["cols"])
+        cv.setR0(np.full((q["rows"], q["cols"], 3), 0.85, np.float32))
+        br = H.FootprintBrush(q["radius"], q["footprint"])
+        br.imprintStamps(q["stamps"], cv, _lib.EXEC_PLAIN)
+        H.Renderer.compose(cv)
+        ctx.synchronize()
+        lat_plain.append((time.perf_counter() - a) * 1e3)
+        del cv, br
+    plain_rps = len(sample) / (time.perf_counter() - t0)
+    top = per_rate[rates[-1]]
+    result = {
+        "metric": METRIC, "value": top["req_s"], "unit": "req/s", "n_gpus": 1,
+        "steps": len(rates), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "served: Poisson arrivals, square canvases 512..4096 "
+                               "(log-uniform side), 10..1000 stamps (footprint 32-128), "
+                               "composite per request, 30% follow-ups to resident canvases",
+                   "slots": args.slots, "hbm_budget_gb": args.hbm_gb},
+        "per_rate": per_rate,
+        "plain_per_request": {"req_s": round(plain_rps, 2),
+                              "p50_ms": round(float(np.percentile(lat_plain, 50)), 2),
+                              "p99_ms": round(float(np.percentile(lat_plain, 99)), 2),
+                              "sample": len(sample)},
+        "speedup_vs_plain_at_top_rate": round(top["req_s"] / plain_rps, 1),
+        "resident_at_end": st,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def bench_copy(args):
     """Calibration: PaintLayer::copyTo of a size x size layer (k_copy_layer,
     7 FP32 planes read and written).  Gives the achievable copy bandwidth and,
@@ -218,7 +396,14 @@ def main():
     ap.add_argument("--cpu-stamps", type=int, default=300)
     ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["single", "batched", "copy"], default="single")
+    ap.add_argument("--config", choices=["single", "batched", "copy", "diff", "served"],
+                    default="single")
+    ap.add_argument("--seg-len", type=int, default=32)
+    ap.add_argument("--rates", default="200,500,1000,2000")
+    ap.add_argument("--seconds", type=float, default=3.0)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--hbm-gb", type=float, default=128.0)
+    ap.add_argument("--plain-requests", type=int, default=40)
     ap.add_argument("--canvases", type=int, default=256)
     ap.add_argument("--plain-steps", type=int, default=1)
     args = ap.parse_args()
@@ -230,6 +415,10 @@ def main():
         return bench_batched(args)
     if args.config == "copy":
         return bench_copy(args)
+    if args.config == "diff":
+        return bench_diff(args)
+    if args.config == "served":
+        return bench_served(args)
 
     from painty_amd import _lib
     from painty_amd import hip as H

@@ -120,12 +120,12 @@ int pnt_canvas_dry(pnt_canvas canvas);
  * (renderer/shaders/DryingStep.compute.glsl:56-77): V leaves up to
  * `portion`, R0 takes the leaving thickness, K/S stay. */
 int pnt_canvas_dry_step(pnt_canvas canvas, float portion);
-/* get/setDryingTime -- Canvas.hxx:157-163.  Any value >= 0 is stored (0,
- * the default here, means explicit drying only).  Stamping is deterministic:
- * the wall-clock checkDry (Canvas.hxx:123-155) is not run, so only
- * pnt_canvas_dry / PNT_STAMP_DRY_AFTER / pnt_canvas_dry_step dry paint.
- * With the reference's default of 15000000 ms (Canvas.hxx:32) its checkDry
- * changes no paint state either within that time. */
+/* get/setDryingTime -- Canvas.hxx:157-163.  Any value >= 0 is stored; the
+ * default is the reference's 15000000 ms (Canvas.hxx:32).  Stamping is
+ * deterministic: the wall-clock checkDry (Canvas.hxx:123-155) is not run,
+ * so only pnt_canvas_dry / PNT_STAMP_DRY_AFTER / pnt_canvas_dry_step dry
+ * paint.  (With that default the reference's checkDry changes no paint
+ * state within a run shorter than 4.17 h either.) */
 int pnt_canvas_set_drying_time_ms(pnt_canvas canvas, int64_t ms);
 int pnt_canvas_get_drying_time_ms(pnt_canvas canvas, int64_t* ms);
 
@@ -274,6 +274,73 @@ int pnt_dev_compose(pnt_ctx ctx, int64_t n, pnt_dev_layer layer,
 int pnt_dev_compose2(pnt_ctx ctx, int64_t n, pnt_dev_layer dry,
                      pnt_dev_layer wet, pnt_dev_rgb R0, pnt_dev_rgb R,
                      void* stream);
+
+/* ------------------------------------- differentiable composite+stamp -- */
+/* No reference counterpart: the reference has no gradients.  A stamp list
+ * (PNT_STAMP_DIP on stamp 0; no PNT_STAMP_DRY_AFTER) run with `brush` on a
+ * rows x cols canvas, each dip's paint K/S = mixSinglePaint(W[paint_of[i]])
+ * (PaintMixer.cxx:327-355) over an n_pig palette (palK/palS n_pig x 3), then
+ * R = Renderer::compose (Renderer.hxx:26-41).  pnt_grad_backward gives the
+ * gradient of <gR, R> w.r.t. the initial wet layer (K0..S2, V planes), the
+ * substrate R0 and W (n_paint x n_pig, row-major), by recomputing the stamp
+ * state from checkpoints taken every seg_len stamps (and at every dip).
+ * Device planes are caller-owned, n = rows*cols floats each; the brush's
+ * pickup map and snapshot are used as scratch by both calls. */
+typedef struct pnt_grad_s* pnt_grad;
+int pnt_grad_create(pnt_ctx ctx, int rows, int cols, pnt_brush brush,
+                    const pnt_stamp* stamps, int64_t n, const int32_t* paint_of,
+                    int n_pig, const double* palK, const double* palS,
+                    int n_paint, int seg_len, pnt_grad* out);
+int pnt_grad_destroy(pnt_grad g);
+/* R = compose(stamps(wet0 over R0)); W: n_paint x n_pig host doubles. */
+int pnt_grad_forward(pnt_grad g, pnt_dev_layer wet0, pnt_dev_rgb R0,
+                     const double* W, pnt_dev_rgb R);
+/* After pnt_grad_forward: gradients of <gR, R>.  gW: host n_paint x n_pig. */
+int pnt_grad_backward(pnt_grad g, pnt_dev_rgb gR, pnt_dev_layer gwet,
+                      pnt_dev_rgb gR0, double* gW);
+
+/* ------------------------------------------------------------- server -- */
+/* No reference counterpart (the reference paints on one canvas in one GUI
+ * thread).  A request server for stroke + composite work on canvases kept
+ * resident on one device: n_slots worker slots (a host thread, a HIP stream
+ * and a brush each) run admitted requests concurrently; requests on one
+ * canvas run in submission order; a request whose canvas is not resident is
+ * admitted when it fits hbm_budget_bytes (canvas planes + composite planes,
+ * 56 B/px), evicting idle canvases least recently used first; an evicted
+ * canvas' state is dropped and later requests on its id fail with
+ * PNT_ESTATE.  Each request
+ * runs as a fresh FootprintBrush (FootprintBrush.hxx:29-63): its stamps
+ * (PNT_EXEC_FUSED), then Renderer::compose (Renderer.hxx:48-53) into the
+ * canvas' resident composite planes, copied to `out` when it is not NULL. */
+typedef struct pnt_server_s* pnt_server;
+typedef struct {
+  int64_t canvas_id;       /* resident canvas key; a new id creates a canvas */
+  int rows, cols;          /* its size (must match an existing canvas) */
+  float background[3];     /* substrate R0 of a new canvas */
+  double radius;           /* brush radius and its width x width footprint */
+  const float* footprint;  /* (copied at submit) */
+  int width;
+  float pickup_rate, deposition_rate;
+  int use_snapshot;
+  const pnt_stamp* stamps; /* stamp list (copied at submit); PNT_STAMP_DIP sets paint */
+  int64_t n_stamps;
+  float* out;              /* host composite, 3 planes of rows*cols, or NULL */
+} pnt_request;
+int pnt_server_create(int device, int n_slots, int64_t hbm_budget_bytes,
+                      pnt_server* out);
+/* Queues a request; returns at once with a ticket. */
+int pnt_server_submit(pnt_server server, const pnt_request* request,
+                      int64_t* ticket);
+/* Blocks until the ticket's request has run; its latency (submit to done,
+ * ms) and its status. */
+int pnt_server_wait(pnt_server server, int64_t ticket, double* latency_ms);
+/* A resident canvas (no request running on it) and its composite planes. */
+int pnt_server_canvas(pnt_server server, int64_t canvas_id, pnt_canvas* canvas,
+                      pnt_dev_rgb* composite);
+int pnt_server_stats(pnt_server server, int64_t* resident_bytes,
+                     int64_t* resident_canvases, int64_t* completed);
+/* Finishes the queued work, then releases everything. */
+int pnt_server_destroy(pnt_server server);
 
 /* ---------------------------------------------------- canvas device view - */
 /* Device planes of a canvas (valid while it lives): layer (7), R0 (3), h. */

@@ -84,6 +84,13 @@ def lib():
         L.orc_catmull_rom_d1.argtypes = [C.c_double] * 5
         L.orc_stroke_stamps.restype = C.c_int64
         L.orc_stroke_stamps.argtypes = [_dp, C.c_int64, _dp, C.c_int64]
+        _ip = C.POINTER(C.c_int)
+        L.orc_grad_run.restype = C.c_int
+        L.orc_grad_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                                   C.POINTER(Stamp), C.c_int64, _ip, C.c_int, _dp, _dp, C.c_int,
+                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_compute_reflectance_adj.argtypes = [_dp, _dp, _dp, C.c_double, _dp, _dp, _dp,
+                                                  _dp, _dp]
         _lib = L
     return _lib
 
@@ -278,6 +285,41 @@ def mixed(K1, S1, v1, K2, S2, v2):
     lib().orc_mixed(_p(_f64(K1)), _p(_f64(S1)), float(v1), _p(_f64(K2)), _p(_f64(S2)),
                     float(v2), _p(K), _p(S))
     return K, S
+
+
+# --------------------------------------------------------- gradient -------
+def compute_reflectance_adj(K, S, R0, d, gR):
+    """Adjoint of ComputeReflectance for one pixel: (gK, gS, gd, gR0)."""
+    K, S, R0, gR = _f64(K), _f64(S), _f64(R0), _f64(gR)
+    gK, gS, gR0, gd = np.empty(3), np.empty(3), np.empty(3), C.c_double()
+    lib().orc_compute_reflectance_adj(_p(K), _p(S), _p(R0), float(d), _p(gR), _p(gK), _p(gS),
+                                      C.cast(C.pointer(gd), _dp), _p(gR0))
+    return gK, gS, gd.value, gR0
+
+
+def grad_run(radius, core, rates, use_snapshot, K, S, V, R0, stamps, paint_of, palK, palS, W,
+             gR):
+    """Forward composite+stamp with dip paints mixSinglePaint(W[paint_of[i]])
+    and the gradient of <gR, R>.  Returns (R, gK, gS, gV, gR0, gW) and the
+    final wet layer (K, S, V)."""
+    b = Brush(radius, core)
+    b.set_rates(*rates)
+    b.set_use_snapshot(use_snapshot)
+    K, S, V, R0, gR = (np.array(a, dtype=np.float64, order="C") for a in (K, S, V, R0, gR))
+    palK, palS, W = _f64(palK), _f64(palS), _f64(W)
+    rows, cols = V.shape
+    pi = np.ascontiguousarray(paint_of, dtype=np.int32)
+    R, gK, gS, gR0 = (np.empty((rows, cols, 3)) for _ in range(4))
+    gV = np.empty((rows, cols))
+    gW = np.empty(W.shape)
+    arr = stamps_to_c(stamps)
+    rc = lib().orc_grad_run(b.h, rows, cols, _p(K), _p(S), _p(V), _p(R0), arr, len(arr),
+                            pi.ctypes.data_as(C.POINTER(C.c_int)), palK.shape[0], _p(palK),
+                            _p(palS), W.shape[0], _p(W), _p(gR), _p(R), _p(gK), _p(gS), _p(gV),
+                            _p(gR0), _p(gW))
+    if rc != 0:
+        raise ValueError("unsupported stamp list for the gradient run")
+    return (R, gK, gS, gV, gR0, gW), (K, S, V)
 
 
 # ------------------------------------------------------------ spline -------

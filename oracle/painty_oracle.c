@@ -521,3 +521,462 @@ int64_t orc_stroke_stamps(const double* p, int64_t n, double* out,
   }
   return k;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Gradient of composite + stamp chain, FP64 reverse mode.
+ *
+ * Forward: the stamps of orc_run_stamps (dips take their paint from
+ * orc_mix_single of a weight row), then Renderer::compose.  Backward: the
+ * adjoint of each operation in reverse order, from per-stamp tapes (the
+ * window of canvas and snapshot as the visits see them, and the pickup map
+ * before the stamp).  Visits inside a stamp are replayed in the reference's
+ * row-major order from the tape, so each visit's adjoint sees its exact
+ * inputs.  Branches (volume thresholds, max(0, .), coth's x > 20 clamp, the
+ * d and S thresholds of ComputeReflectance) are differentiated as the
+ * branch taken. */
+
+/* One visit (pickupPaint then depositPaint, FootprintBrush.hxx:349-431) on
+ * explicit state: P = pickup cell, C = canvas pixel, X = pickup source pixel
+ * (the snapshot; the canvas pixel itself when `alias`).  Same operations,
+ * in the same order, as pickup_paint / deposit_paint above. */
+typedef struct {
+  double P[7], C[7], X[7]; /* K0 K1 K2 S0 S1 S2 V */
+} orc_vstate;
+
+static void visit_fwd(orc_vstate* s, double fh, double pr, double dr,
+                      double cap, const double B[6], int alias) {
+  double* P = s->P;
+  double* C = s->C;
+  double* X = alias ? s->C : s->X;
+  const double vP = P[6], vC = X[6];
+  const double vLeave = pr * vC * fh;
+  if (vLeave > kMinVolume) {
+    X[6] = vC - vLeave;
+    double k[3], ss[3];
+    blend3(vP, P, vLeave, X, k);
+    blend3(vP, P + 3, vLeave, X + 3, ss);
+    for (int c = 0; c < 3; c++) {
+      P[c] = k[c];
+      P[3 + c] = ss[c];
+    }
+    P[6] = vP + vLeave;
+  }
+  const double u = P[6];
+  const double vFree = (cap - u > 0.0) ? cap - u : 0.0;
+  double ks[3], sk[3];
+  blend3(u, P, vFree, B, ks);
+  blend3(u, P + 3, vFree, B + 3, sk);
+  P[6] = u - dr * u * fh;
+  const double vCc = C[6], vB = cap * fh;
+  double k2[3], s2[3];
+  blend3(vB, ks, vCc, C, k2);
+  blend3(vB, sk, vCc, C + 3, s2);
+  for (int c = 0; c < 3; c++) {
+    C[c] = k2[c];
+    C[3 + c] = s2[c];
+  }
+  C[6] = vB + vCc;
+}
+
+/* Adjoint of visit_fwd at input state `pre`.  On entry g holds the adjoints
+ * of the outputs (P, C and, unless alias, X); on return the adjoints of the
+ * inputs.  gB accumulates the adjoint of the brush paint (K0..S2). */
+static void visit_adj(const orc_vstate* pre, orc_vstate* g, double gB[6],
+                      double fh, double pr, double dr, double cap,
+                      const double B[6], int alias) {
+  const double* P = pre->P;
+  const double* C = pre->C;
+  const double* X = alias ? pre->C : pre->X;
+  /* forward intermediates */
+  const double vP = P[6], vC = X[6];
+  const double vLeave = pr * vC * fh;
+  const int a = vLeave > kMinVolume;
+  double P1[7];
+  for (int c = 0; c < 7; c++) P1[c] = P[c];
+  double X6n = vC, vt1 = 0.0;
+  int bb = 0;
+  if (a) {
+    X6n = vC - vLeave;
+    vt1 = vP + vLeave;
+    bb = vt1 > kMinVolume;
+    if (bb)
+      for (int c = 0; c < 6; c++) P1[c] = (vP * P[c] + vLeave * X[c]) / vt1;
+    P1[6] = vP + vLeave;
+  }
+  const double u = P1[6];
+  const int f = cap - u > 0.0;
+  const double vFree = f ? cap - u : 0.0;
+  const double vt2 = u + vFree;
+  const int d = vt2 > kMinVolume;
+  double src[6];
+  for (int c = 0; c < 6; c++) src[c] = d ? (u * P1[c] + vFree * B[c]) / vt2 : P1[c];
+  double Cd[7]; /* the canvas pixel as depositPaint reads it */
+  for (int c = 0; c < 6; c++) Cd[c] = C[c];
+  Cd[6] = alias ? X6n : C[6];
+  const double vB = cap * fh, vCc = Cd[6], vt3 = vB + vCc;
+  const int e = vt3 > kMinVolume;
+  double C1[6];
+  for (int c = 0; c < 6; c++) C1[c] = e ? (vB * src[c] + vCc * Cd[c]) / vt3 : src[c];
+
+  /* depositPaint */
+  double gCd[7], gsrc[6];
+  double gvCc = g->C[6]; /* C.V = vB + vCc */
+  for (int c = 0; c < 6; c++) {
+    if (e) {
+      gsrc[c] = g->C[c] * vB / vt3;
+      gCd[c] = g->C[c] * vCc / vt3;
+      gvCc += g->C[c] * (Cd[c] - C1[c]) / vt3;
+    } else {
+      gsrc[c] = g->C[c];
+      gCd[c] = 0.0;
+    }
+  }
+  gCd[6] = gvCc;
+  double gP1[7];
+  double gu = g->P[6] * (1.0 - dr * fh); /* P.V = u - dr*u*fh */
+  if (d) {
+    double gvFree = 0.0;
+    for (int c = 0; c < 6; c++) {
+      gP1[c] = g->P[c] + gsrc[c] * u / vt2;
+      gB[c] += gsrc[c] * vFree / vt2;
+      gu += gsrc[c] * (P1[c] - src[c]) / vt2;
+      gvFree += gsrc[c] * (B[c] - src[c]) / vt2;
+    }
+    if (f) gu -= gvFree;
+  } else {
+    for (int c = 0; c < 6; c++) gP1[c] = g->P[c] + gsrc[c];
+  }
+  gP1[6] = gu;
+  /* the source pixel after pickupPaint */
+  double gXo[7];
+  for (int c = 0; c < 7; c++) gXo[c] = alias ? gCd[c] : g->X[c];
+  /* pickupPaint */
+  double gPin[7], gXin[7];
+  for (int c = 0; c < 7; c++) gXin[c] = gXo[c];
+  if (a) {
+    double gvP = gP1[6], gvLeave = gP1[6] - gXo[6], gvC = gXo[6];
+    for (int c = 0; c < 6; c++) {
+      if (bb) {
+        gPin[c] = gP1[c] * vP / vt1;
+        gXin[c] += gP1[c] * vLeave / vt1;
+        gvP += gP1[c] * (P[c] - P1[c]) / vt1;
+        gvLeave += gP1[c] * (X[c] - P1[c]) / vt1;
+      } else {
+        gPin[c] = gP1[c];
+      }
+    }
+    gvC += gvLeave * pr * fh;
+    gPin[6] = gvP;
+    gXin[6] = gvC;
+  } else {
+    for (int c = 0; c < 7; c++) gPin[c] = gP1[c];
+  }
+  for (int c = 0; c < 7; c++) g->P[c] = gPin[c];
+  if (alias) {
+    for (int c = 0; c < 7; c++) g->C[c] = gXin[c];
+  } else {
+    for (int c = 0; c < 7; c++) {
+      g->C[c] = gCd[c];
+      g->X[c] = gXin[c];
+    }
+  }
+}
+
+/* Adjoint of ComputeReflectance (KubelkaMunk.hxx:25-83) for one pixel:
+ * gK, gS, gR0 set, *gd set. */
+void orc_compute_reflectance_adj(const double K[3], const double S_in[3],
+                                 const double R0[3], double d,
+                                 const double gR[3], double gK[3],
+                                 double gS[3], double* gd, double gR0[3]) {
+  const double thr = DBL_EPSILON * 10000.0;
+  *gd = 0.0;
+  if (fabs(d) < thr) {
+    for (int i = 0; i < 3; i++) {
+      gR0[i] = gR[i];
+      gK[i] = gS[i] = 0.0;
+    }
+    return;
+  }
+  for (int i = 0; i < 3; i++) {
+    const int clamp = !(fabs(S_in[i]) > thr);
+    const double S = clamp ? 0.00000000001 : S_in[i];
+    const double a = 1.0 + K[i] / S;
+    const double v = a * a - 1.0;
+    const double b = (v < 0.0) ? 0.0 : sqrt(v);
+    const double x = b * S * d;
+    const double ct = orc_coth(x);
+    double csch2 = 0.0; /* -d coth / dx; 0 where coth is clamped to 1 */
+    if (x <= 20.0 && fabs(x) > 0.0) {
+      const double sh = sinh(x);
+      csch2 = 1.0 / (sh * sh);
+    }
+    const double q = b * ct;
+    const double N = 1.0 - R0[i] * (a - q);
+    const double D = a - R0[i] + q;
+    const double dR_dR0 = (N - (a - q) * D) / (D * D);
+    const double dR_da = (-R0[i] * D - N) / (D * D);
+    const double dR_dq = (R0[i] * D - N) / (D * D);
+    const double dq_db = ct - x * csch2;
+    const double dq_dS = -b * b * d * csch2;
+    const double dq_dd = -b * b * S * csch2;
+    const double db_da = (v > 0.0) ? a / b : 0.0;
+    const double dRa = dR_da + dR_dq * dq_db * db_da;
+    gK[i] = gR[i] * dRa / S;
+    gS[i] = clamp ? 0.0 : gR[i] * (dRa * (-K[i] / (S * S)) + dR_dq * dq_dS);
+    *gd += gR[i] * dR_dq * dq_dd;
+    gR0[i] = gR[i] * dR_dR0;
+  }
+}
+
+/* Window box of a stamp: canvas pixels its visits can touch. */
+static void window_box(const orc_brush* b, int rows, int cols, double cx,
+                       double cy, int* x0, int* y0, int* x1, int* y1) {
+  const int hr = (b->size_map - 1) / 2;
+  int a = (int)(-hr + cx), c = (int)(hr + cx), e = (int)(-hr + cy), f = (int)(hr + cy);
+  if (a < 0) a = 0;
+  if (e < 0) e = 0;
+  if (c > cols - 1) c = cols - 1;
+  if (f > rows - 1) f = rows - 1;
+  *x0 = a;
+  *x1 = c;
+  *y0 = e;
+  *y1 = f;
+}
+
+typedef struct {
+  int x0, y0, x1, y1; /* window box (empty when x1 < x0 or y1 < y0) */
+  double* C;          /* box pixels x 7, canvas as the visits find it */
+  double* X;          /* box pixels x 7, snapshot as the visits find it */
+  double* P;          /* size_map^2 x 7, pickup map before the stamp */
+  double B[6];        /* brush paint */
+} orc_tape;
+
+static void get7(const double* K, const double* S, const double* V, int64_t i,
+                 double* o) {
+  for (int c = 0; c < 3; c++) {
+    o[c] = K[3 * i + c];
+    o[3 + c] = S[3 * i + c];
+  }
+  o[6] = V[i];
+}
+
+int orc_grad_run(orc_brush* b, int rows, int cols, double* K, double* S,
+                 double* V, const double* R0, const orc_stamp* st, int64_t n,
+                 const int* paint_of, int n_pig, const double* palK,
+                 const double* palS, int n_paint, const double* W,
+                 const double* gR, double* R, double* gK, double* gS,
+                 double* gV, double* gR0, double* gW) {
+  const int64_t npx = (int64_t)rows * cols;
+  const int alias = !b->use_snapshot;
+  if (n > 0 && !(st[0].flags & ORC_DIP)) return -1; /* paint must come from a dip */
+  for (int64_t i = 0; i < n; i++) {
+    if (st[i].flags & ORC_DRY_AFTER) return -1;
+    if ((st[i].flags & ORC_DIP) && (paint_of[i] < 0 || paint_of[i] >= n_paint)) return -1;
+  }
+  if (b->snap_rows == rows && b->snap_cols == cols) return -1; /* needs a fresh snapshot */
+  const int sm = b->size_map;
+  const int64_t m = (int64_t)sm * sm;
+  orc_tape* tape = (orc_tape*)calloc((size_t)(n > 0 ? n : 1), sizeof(orc_tape));
+  double paint[6] = {0, 0, 0, 0, 0, 0};
+  /* ---- forward, taping each stamp */
+  for (int64_t i = 0; i < n; i++) {
+    if (st[i].flags & ORC_DIP) {
+      double k[3], s[3];
+      orc_mix_single(n_pig, palK, palS, n_pig, W + (int64_t)paint_of[i] * n_pig, k, s);
+      orc_brush_dip(b, k, s);
+      for (int c = 0; c < 3; c++) {
+        paint[c] = k[c];
+        paint[3 + c] = s[c];
+      }
+    }
+    orc_tape* t = tape + i;
+    for (int c = 0; c < 6; c++) t->B[c] = paint[c];
+    t->P = (double*)malloc(sizeof(double) * 7 * (size_t)m);
+    for (int64_t q = 0; q < m; q++) get7(b->pK, b->pS, b->pV, q, t->P + 7 * q);
+    if (!alias) update_snapshot_ring(b, rows, cols, K, S, V, st[i].cx, st[i].cy);
+    window_box(b, rows, cols, st[i].cx, st[i].cy, &t->x0, &t->y0, &t->x1, &t->y1);
+    const int bw = t->x1 - t->x0 + 1, bh = t->y1 - t->y0 + 1;
+    if (bw > 0 && bh > 0) {
+      t->C = (double*)malloc(sizeof(double) * 7 * (size_t)bw * bh);
+      t->X = (double*)malloc(sizeof(double) * 7 * (size_t)bw * bh);
+      for (int y = t->y0; y <= t->y1; y++)
+        for (int x = t->x0; x <= t->x1; x++) {
+          const int64_t ic = (int64_t)y * cols + x;
+          const int64_t k = (int64_t)(y - t->y0) * bw + (x - t->x0);
+          get7(K, S, V, ic, t->C + 7 * k);
+          if (!alias) get7(b->sK, b->sS, b->sV, ic, t->X + 7 * k);
+        }
+    }
+    /* the visits (the snapshot ring was refreshed above) */
+    const int hr = (sm - 1) / 2;
+    const double cosT = cos(-st[i].theta), sinT = sin(-st[i].theta);
+    for (int row = -hr; row <= hr; row++)
+      for (int col = -hr; col <= hr; col++) {
+        const int x = (int)(col + st[i].cx), y = (int)(row + st[i].cy);
+        const double rc = col * cosT - row * sinT, rr = col * sinT + row * cosT;
+        const int mx = (int)round(rc + hr), my = (int)round(rr + hr);
+        if (y < 0 || x < 0 || x >= cols || y >= rows) continue;
+        if (my < 0 || mx < 0 || mx >= sm || my >= sm) continue;
+        const int64_t ic = (int64_t)y * cols + x, im = (int64_t)my * sm + mx;
+        if (alias) {
+          pickup_paint(b, ic, im, K, S, V);
+        } else {
+          pickup_paint(b, ic, im, b->sK, b->sS, b->sV);
+        }
+        deposit_paint(b, ic, im, K, S, V);
+      }
+  }
+  /* ---- composite and its adjoint */
+  for (int64_t i = 0; i < npx; i++) {
+    orc_compute_reflectance(K + 3 * i, S + 3 * i, R0 + 3 * i, V[i], R + 3 * i);
+    double gd;
+    orc_compute_reflectance_adj(K + 3 * i, S + 3 * i, R0 + 3 * i, V[i], gR + 3 * i,
+                                gK + 3 * i, gS + 3 * i, &gd, gR0 + 3 * i);
+    gV[i] = gd;
+  }
+  double* gsK = (double*)calloc((size_t)npx * 3, sizeof(double));
+  double* gsS = (double*)calloc((size_t)npx * 3, sizeof(double));
+  double* gsV = (double*)calloc((size_t)npx, sizeof(double));
+  double* gP = (double*)calloc((size_t)m * 7, sizeof(double));
+  for (int64_t q = 0; q < (int64_t)n_paint * n_pig; q++) gW[q] = 0.0;
+  double gpaint[6] = {0, 0, 0, 0, 0, 0};
+  /* per-visit replay buffers */
+  const int side = sm;
+  orc_vstate* pre = (orc_vstate*)malloc(sizeof(orc_vstate) * (size_t)side * side);
+  int64_t* vic = (int64_t*)malloc(sizeof(int64_t) * (size_t)side * side);
+  int64_t* vim = (int64_t*)malloc(sizeof(int64_t) * (size_t)side * side);
+  double* Pm = (double*)malloc(sizeof(double) * 7 * (size_t)m);
+  /* ---- backward over the stamps */
+  for (int64_t i = n - 1; i >= 0; i--) {
+    const orc_tape* t = tape + i;
+    const int bw = t->x1 - t->x0 + 1, bh = t->y1 - t->y0 + 1;
+    const int64_t nb = (bw > 0 && bh > 0) ? (int64_t)bw * bh : 0;
+    double* Cw = (double*)malloc(sizeof(double) * 7 * (size_t)(nb ? nb : 1));
+    double* Xw = (double*)malloc(sizeof(double) * 7 * (size_t)(nb ? nb : 1));
+    if (nb) {
+      memcpy(Cw, t->C, sizeof(double) * 7 * (size_t)nb);
+      if (!alias) memcpy(Xw, t->X, sizeof(double) * 7 * (size_t)nb);
+    }
+    memcpy(Pm, t->P, sizeof(double) * 7 * (size_t)m);
+    /* replay the visits in reference order, recording each one's input */
+    int64_t nv = 0;
+    const int hr = (sm - 1) / 2;
+    const double cosT = cos(-st[i].theta), sinT = sin(-st[i].theta);
+    for (int row = -hr; row <= hr; row++)
+      for (int col = -hr; col <= hr; col++) {
+        const int x = (int)(col + st[i].cx), y = (int)(row + st[i].cy);
+        const double rc = col * cosT - row * sinT, rr = col * sinT + row * cosT;
+        const int mx = (int)round(rc + hr), my = (int)round(rr + hr);
+        if (y < 0 || x < 0 || x >= cols || y >= rows) continue;
+        if (my < 0 || mx < 0 || mx >= sm || my >= sm) continue;
+        const int64_t im = (int64_t)my * sm + mx;
+        const double fh = b->footprint[im];
+        if (!(fh > 0.0)) continue;
+        const int64_t k = (int64_t)(y - t->y0) * bw + (x - t->x0);
+        orc_vstate s;
+        memset(&s, 0, sizeof(s));
+        memcpy(s.P, Pm + 7 * im, sizeof(s.P));
+        memcpy(s.C, Cw + 7 * k, sizeof(s.C));
+        if (!alias) memcpy(s.X, Xw + 7 * k, sizeof(s.X));
+        pre[nv] = s;
+        vic[nv] = (int64_t)y * cols + x;
+        vim[nv] = im;
+        visit_fwd(&s, fh, b->pickup_rate, b->deposition_rate, b->cap, t->B, alias);
+        memcpy(Pm + 7 * im, s.P, sizeof(s.P));
+        memcpy(Cw + 7 * k, s.C, sizeof(s.C));
+        if (!alias) memcpy(Xw + 7 * k, s.X, sizeof(s.X));
+        nv++;
+      }
+    /* adjoints in reverse order */
+    for (int64_t v = nv - 1; v >= 0; v--) {
+      const int64_t ic = vic[v], im = vim[v];
+      orc_vstate g;
+      memset(&g, 0, sizeof(g));
+      memcpy(g.P, gP + 7 * im, sizeof(g.P));
+      get7(gK, gS, gV, ic, g.C);
+      if (!alias) get7(gsK, gsS, gsV, ic, g.X);
+      visit_adj(pre + v, &g, gpaint, b->footprint[im], b->pickup_rate,
+                b->deposition_rate, b->cap, t->B, alias);
+      memcpy(gP + 7 * im, g.P, sizeof(g.P));
+      for (int c = 0; c < 3; c++) {
+        gK[3 * ic + c] = g.C[c];
+        gS[3 * ic + c] = g.C[3 + c];
+        if (!alias) {
+          gsK[3 * ic + c] = g.X[c];
+          gsS[3 * ic + c] = g.X[3 + c];
+        }
+      }
+      gV[ic] = g.C[6];
+      if (!alias) gsV[ic] = g.X[6];
+    }
+    free(Cw);
+    free(Xw);
+    if (!alias) {
+      /* the ring refresh copied canvas -> snapshot (FootprintBrush.hxx:307-318) */
+      const int tl_x = (int)(st[i].cx - hr), tl_y = (int)(st[i].cy - hr);
+      const int br_x = (int)(st[i].cx + hr), br_y = (int)(st[i].cy + hr);
+      int ax0 = (int)(st[i].cx - hr - b->radius), ay0 = (int)(st[i].cy - hr - b->radius);
+      int ax1 = (int)(st[i].cx + hr + b->radius), ay1 = (int)(st[i].cy + hr + b->radius);
+      if (ax0 < 0) ax0 = 0;
+      if (ay0 < 0) ay0 = 0;
+      if (ax1 > cols - 1) ax1 = cols - 1;
+      if (ay1 > rows - 1) ay1 = rows - 1;
+      for (int y = ay0; y <= ay1; y++)
+        for (int x = ax0; x <= ax1; x++) {
+          if (y > tl_y && y < br_y && x > tl_x && x < br_x) continue;
+          const int64_t p = (int64_t)y * cols + x;
+          for (int c = 0; c < 3; c++) {
+            gK[3 * p + c] += gsK[3 * p + c];
+            gS[3 * p + c] += gsS[3 * p + c];
+            gsK[3 * p + c] = gsS[3 * p + c] = 0.0;
+          }
+          gV[p] += gsV[p];
+          gsV[p] = 0.0;
+        }
+      if (i == 0) { /* the first imprint copied the whole canvas (:281-284) */
+        for (int64_t p = 0; p < npx; p++) {
+          for (int c = 0; c < 3; c++) {
+            gK[3 * p + c] += gsK[3 * p + c];
+            gS[3 * p + c] += gsS[3 * p + c];
+          }
+          gV[p] += gsV[p];
+        }
+      }
+    }
+    if (st[i].flags & ORC_DIP) {
+      /* clean() zeroed the pickup map: nothing before flows through it */
+      memset(gP, 0, sizeof(double) * 7 * (size_t)m);
+      /* paint = mixSinglePaint(W[p]) (PaintMixer.cxx:327-355) */
+      const double* w = W + (int64_t)paint_of[i] * n_pig;
+      double wsum = 0.0;
+      for (int l = 0; l < n_pig; l++) wsum += w[l];
+      const int unit = fabs(wsum - 1.0) < 10e-6;
+      const double norm = unit ? 1.0 : 1.0 / wsum;
+      for (int l = 0; l < n_pig; l++) {
+        double acc = 0.0;
+        for (int c = 0; c < 3; c++) {
+          const double dK = unit ? palK[3 * l + c] : norm * (palK[3 * l + c] - t->B[c]);
+          const double dS = unit ? palS[3 * l + c] : norm * (palS[3 * l + c] - t->B[3 + c]);
+          acc += gpaint[c] * dK + gpaint[3 + c] * dS;
+        }
+        gW[(int64_t)paint_of[i] * n_pig + l] += acc;
+      }
+      for (int c = 0; c < 6; c++) gpaint[c] = 0.0;
+    }
+  }
+  for (int64_t i = 0; i < n; i++) {
+    free(tape[i].C);
+    free(tape[i].X);
+    free(tape[i].P);
+  }
+  free(tape);
+  free(pre);
+  free(vic);
+  free(vim);
+  free(Pm);
+  free(gsK);
+  free(gsS);
+  free(gsV);
+  free(gP);
+  return 0;
+}

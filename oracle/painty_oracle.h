@@ -106,6 +106,26 @@ void orc_run_stamps(orc_brush* b, int rows, int cols, double* K, double* S,
                     double* V, double* R0, double* h, const orc_stamp* stamps,
                     int64_t n);
 
+/* ---- gradient of composite + stamping (FP64 reverse mode) ----
+ * Forward: stamps as orc_run_stamps (no ORC_DRY_AFTER; stamp 0 must dip),
+ * each dip's paint = orc_mix_single(palette, W[paint_of[i]]), then
+ * R = Renderer::compose of the final wet layer over R0 (Renderer.hxx:26-41).
+ * Backward: gK, gS, gV (initial wet layer), gR0 and gW (n_paint x n_pig)
+ * of the loss whose gradient w.r.t. R is gR.  K, S, V are updated in place
+ * to the final state.  The brush must be fresh (no snapshot yet).  Returns
+ * -1 for an unsupported stamp list. */
+int orc_grad_run(orc_brush* b, int rows, int cols, double* K, double* S,
+                 double* V, const double* R0, const orc_stamp* st, int64_t n,
+                 const int* paint_of, int n_pig, const double* palK,
+                 const double* palS, int n_paint, const double* W,
+                 const double* gR, double* R, double* gK, double* gS,
+                 double* gV, double* gR0, double* gW);
+/* Adjoint of ComputeReflectance (KubelkaMunk.hxx:25-83) for one pixel. */
+void orc_compute_reflectance_adj(const double K[3], const double S_in[3],
+                                 const double R0[3], double d,
+                                 const double gR[3], double gK[3],
+                                 double gS[3], double* gd, double gR0[3]);
+
 /* ---- mixer: painty/mixer/src/PaintMixer.cxx ---- */
 /* :327-355; returns -1 (reference: std::invalid_argument) on size mismatch */
 int orc_mix_single(int n_palette, const double* baseK, const double* baseS,

@@ -40,6 +40,15 @@ class BatchJob(C.Structure):
                 ("n_stamps", C.c_int64), ("out", C.c_void_p * 3)]
 
 
+class PntRequest(C.Structure):
+    _fields_ = [("canvas_id", C.c_int64), ("rows", C.c_int), ("cols", C.c_int),
+                ("background", C.c_float * 3), ("radius", C.c_double),
+                ("footprint", C.POINTER(C.c_float)), ("width", C.c_int),
+                ("pickup_rate", C.c_float), ("deposition_rate", C.c_float),
+                ("use_snapshot", C.c_int), ("stamps", C.POINTER(PntStamp)),
+                ("n_stamps", C.c_int64), ("out", C.POINTER(C.c_float))]
+
+
 class DevLayer(C.Structure):
     _fields_ = [("p", C.c_void_p * 7)]
 
@@ -114,6 +123,19 @@ SIGNATURES = {
     "pnt_dev_compose2": [_vp, C.c_int64, DevLayer, DevLayer, DevRgb, DevRgb, _vp],
     "pnt_canvas_device_planes": [_vp, C.POINTER(DevLayer), C.POINTER(DevRgb),
                                  C.POINTER(_vp)],
+    "pnt_grad_create": [_vp, C.c_int, C.c_int, _vp, C.POINTER(PntStamp), C.c_int64,
+                        C.POINTER(C.c_int32), C.c_int, _dp, _dp, C.c_int, C.c_int,
+                        C.POINTER(_vp)],
+    "pnt_grad_destroy": [_vp],
+    "pnt_server_create": [C.c_int, C.c_int, C.c_int64, C.POINTER(_vp)],
+    "pnt_server_submit": [_vp, C.POINTER(PntRequest), C.POINTER(C.c_int64)],
+    "pnt_server_wait": [_vp, C.c_int64, C.POINTER(C.c_double)],
+    "pnt_server_canvas": [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(DevRgb)],
+    "pnt_server_stats": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                         C.POINTER(C.c_int64)],
+    "pnt_server_destroy": [_vp],
+    "pnt_grad_forward": [_vp, DevLayer, DevRgb, _dp, DevRgb],
+    "pnt_grad_backward": [_vp, DevRgb, DevLayer, DevRgb, _dp],
 }
 _RESTYPE = {"pnt_last_error": C.c_char_p, "pnt_ctx_stream": _vp}
 

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -14,22 +15,31 @@ using namespace pnt;
 
 namespace {
 
-thread_local std::string g_err;
+// Message of the last failed call on this thread.  A plain character array:
+// it has no destructor, so calls made while the process exits (a caller's
+// static objects releasing their handles run after thread-local objects are
+// destroyed) still find valid storage.
+thread_local char g_err[1024];
+
+void set_err(const char* m) {
+  std::strncpy(g_err, m, sizeof(g_err) - 1);
+  g_err[sizeof(g_err) - 1] = '\0';
+}
 
 template <class F>
 int guard(F&& f) {
   try {
     f();
-    g_err.clear();
+    g_err[0] = '\0';
     return PNT_OK;
   } catch (const Error& e) {
-    g_err = e.what();
+    set_err(e.what());
     return e.code;
   } catch (const std::bad_alloc&) {
-    g_err = "host allocation failed";
+    set_err("host allocation failed");
     return PNT_ENOMEM;
   } catch (const std::exception& e) {
-    g_err = e.what();
+    set_err(e.what());
     return PNT_EINVAL;
   }
 }
@@ -76,6 +86,8 @@ void download_planes(float* dst, int nplanes, float* const* src, int64_t n, int 
 
 namespace pnt {
 
+int guard_call(const std::function<void()>& f) { return guard(f); }
+
 float* dev_alloc(int64_t nfloats) {
   void* p = nullptr;
   if (nfloats <= 0) nfloats = 1;
@@ -111,20 +123,29 @@ void brush_set_geometry(pnt_brush b, double radius, const float* core, int width
     for (int j = 0; j < w; ++j) full[(size_t)(i + pad) * size_map + (j + pad)] = core[i * w + j];
   const int64_t m = (int64_t)size_map * size_map;
   const int64_t mstride = round_up(m, 64);
-  float* fp = dev_alloc(m);
-  float* pick = nullptr;
-  try {
-    pick = dev_alloc(7 * mstride);
-  } catch (...) {
-    dev_free(fp);
-    throw;
-  }
   hipStream_t s = b->ctx->stream;
+  // buffers keep their capacity: a smaller brush reuses them (no free, which
+  // would wait for the whole device)
+  float* fp = b->fp;
+  float* pick = b->pick;
+  if (m > b->fp_cap || 7 * mstride > b->pick_cap) {
+    fp = dev_alloc(m);
+    try {
+      pick = dev_alloc(7 * mstride);
+    } catch (...) {
+      dev_free(fp);
+      throw;
+    }
+  }
   PNT_HIP_CHECK(hipMemcpyAsync(fp, full.data(), m * sizeof(float), hipMemcpyHostToDevice, s));
   launch_fill(pick, 7 * mstride, 0.0f, s);
   PNT_HIP_CHECK(hipStreamSynchronize(s));
-  dev_free(b->fp);
-  dev_free(b->pick);
+  if (fp != b->fp) {
+    dev_free(b->fp);
+    dev_free(b->pick);
+    b->fp_cap = m;
+    b->pick_cap = 7 * mstride;
+  }
   b->fp = fp;
   b->fp_host.swap(full);
   b->pick = pick;
@@ -140,16 +161,20 @@ void brush_full_snapshot(pnt_brush b, pnt_canvas cv) {
   hipStream_t s = b->ctx->stream;
   if (b->snap_rows != cv->rows || b->snap_cols != cv->cols) {
     const int64_t stride = round_up(std::max<int64_t>(cv->n, 1), 64);
-    float* snap = dev_alloc(7 * stride);
-    PNT_HIP_CHECK(hipStreamSynchronize(s));
-    dev_free(b->snap);
-    b->snap = snap;
+    if (7 * stride > b->snap_cap) {  // grow; a smaller canvas reuses the buffer
+      float* snap = dev_alloc(7 * stride);
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+      dev_free(b->snap);
+      b->snap = snap;
+      b->snap_cap = 7 * stride;
+    }
     b->snap_stride = stride;
     b->snap_rows = cv->rows;
     b->snap_cols = cv->cols;
   }
   launch_copy_layer(cv->n, cv->layer(), b->snap_planes(), s);
-  if (b->dirty && b->dirty_n == cv->n) {
+  if (b->dirty && round_up(cv->n, 16) + 16 <= b->dirty_cap) {
+    b->dirty_n = cv->n;
     launch_fill_u8(b->dirty, cv->n, 0, s);
     b->dirty_canvas = cv;
     b->dirty_epoch = cv->epoch;
@@ -354,7 +379,7 @@ void mixed(const double K1[3], const double S1[3], double v1, const double K2[3]
 // ================================================================== C ABI ==
 extern "C" {
 
-const char* pnt_last_error(void) { return g_err.c_str(); }
+const char* pnt_last_error(void) { return g_err; }
 
 int pnt_version(void) { return 100; }
 
@@ -568,6 +593,39 @@ static void canvas_clear(pnt_canvas cv) {
   launch_fill(cv->h(), cv->n, 0.0f, s);
 }
 
+}  // extern "C"
+
+namespace pnt {
+// A canvas on a caller-provided buffer of at least 11*round_up(rows*cols, 64)
+// floats, which pnt_canvas_destroy leaves alone.
+pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base) {
+  PNT_REQUIRE(ctx && base, "NULL argument");
+  PNT_REQUIRE(rows > 0 && cols > 0 && (int64_t)rows * cols < ((int64_t)1 << 31),
+              "rows/cols out of range");
+  ctx->use();
+  auto* cv = new pnt_canvas_s;
+  cv->ctx = ctx;
+  cv->rows = rows;
+  cv->cols = cols;
+  cv->n = (int64_t)rows * cols;
+  cv->stride = round_up(cv->n, 64);
+  cv->base = base;
+  cv->owns_base = false;
+  cv->view.ctx = ctx;
+  cv->view.rows = rows;
+  cv->view.cols = cols;
+  cv->view.n = cv->n;
+  cv->view.stride = cv->stride;
+  cv->view.base = cv->base;
+  cv->view.owner = false;
+  cv->view.parent = cv;
+  canvas_clear(cv);
+  return cv;
+}
+}  // namespace pnt
+
+extern "C" {
+
 int pnt_canvas_create(pnt_ctx ctx, int rows, int cols, pnt_canvas* out) {
   return guard([&] {
     PNT_REQUIRE(ctx && out, "NULL argument");
@@ -606,7 +664,7 @@ int pnt_canvas_destroy(pnt_canvas cv) {
     if (!cv) return;
     (void)hipSetDevice(cv->ctx->device);
     (void)hipStreamSynchronize(cv->ctx->stream);
-    dev_free(cv->base);
+    if (cv->owns_base) dev_free(cv->base);
     dev_free(cv->scratch);
     delete cv;
   });

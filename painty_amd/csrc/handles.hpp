@@ -58,6 +58,7 @@ struct pnt_canvas_s {
   int rows = 0, cols = 0;
   int64_t n = 0, stride = 0;
   float* base = nullptr;     // planes 0..6 wet layer, 7..9 R0, 10 h
+  bool owns_base = true;     // false: base belongs to the creator (canvas_over)
   float* scratch = nullptr;  // 3 planes for host-side compose results
   pnt_layer_s view;
   int64_t drying_ms = 15000000;  // Canvas.hxx:32
@@ -82,11 +83,12 @@ struct pnt_brush_s {
   int width = 0, size_map = 0;
   int64_t m = 0, mstride = 0;
   float* fp = nullptr;         // footprint, size_map^2
+  int64_t fp_cap = 0, pick_cap = 0;  // floats allocated at fp / pick
   std::vector<float> fp_host;  // same footprint on the host (traffic replay)
   float* pick = nullptr;       // pickup map: 7 planes of mstride
   float* snap = nullptr;       // snapshot: 7 planes of snap_stride
   int snap_rows = 0, snap_cols = 0;
-  int64_t snap_stride = 0;
+  int64_t snap_stride = 0, snap_cap = 0;  // floats allocated at snap
   bool use_snap = true;
   float pickup_rate = 0.9f, dep_rate = 0.05f, cap = 1.0f;
   float paintK[3] = {0, 0, 0}, paintS[3] = {0, 0, 0};
@@ -97,7 +99,7 @@ struct pnt_brush_s {
   // snapshot dirty bytes (persistent path): dirty[i] == 0 means snapshot
   // pixel i equals canvas pixel i of (dirty_canvas, dirty_epoch)
   uint8_t* dirty = nullptr;
-  int64_t dirty_n = 0;
+  int64_t dirty_n = 0, dirty_cap = 0;  // pixels covered / bytes allocated
   const pnt_canvas_s* dirty_canvas = nullptr;
   uint64_t dirty_epoch = 0;
   pnt::PersistentState* pers = nullptr;
@@ -136,4 +138,10 @@ void launch_dry_dirty(int64_t n, const Layer7& L, const Rgb3& R0, float* h, uint
 // Snapshot of brush b ready for stamping canvas cv (allocated, copied when
 // the size changed) and its dirty bytes valid for cv.
 void brush_prepare_snapshot(pnt_brush b, pnt_canvas cv, hipStream_t s);
+// FootprintBrush::setRadius geometry with an injected width x width footprint
+// (api.hip): new footprint and a cleared pickup map.
+void brush_set_geometry(pnt_brush b, double radius, const float* core, int width);
+// A canvas (cleared, background 1) on a caller buffer of at least
+// 11*round_up(rows*cols, 64) floats that pnt_canvas_destroy does not free.
+pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base);
 }  // namespace pnt

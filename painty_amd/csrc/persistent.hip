@@ -348,11 +348,15 @@ void plan_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t n,
 void brush_prepare_snapshot(pnt_brush b, pnt_canvas cv, hipStream_t s) {
   if (!b->use_snap) return;
   if (b->dirty_n != cv->n || !b->dirty) {
-    PNT_HIP_CHECK(hipStreamSynchronize(s));
-    dev_free(b->dirty);
-    b->dirty = nullptr;
-    // padded: the batch kernel scans the bytes 16 at a time
-    PNT_HIP_CHECK(hipMalloc((void**)&b->dirty, (size_t)(round_up(cv->n, 16) + 16)));
+    // padded: the ring scan reads the bytes in aligned groups
+    const int64_t need = round_up(cv->n, 16) + 16;
+    if (need > b->dirty_cap) {
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+      dev_free(b->dirty);
+      b->dirty = nullptr;
+      PNT_HIP_CHECK(hipMalloc((void**)&b->dirty, (size_t)need));
+      b->dirty_cap = need;
+    }
     b->dirty_n = cv->n;
     b->dirty_canvas = nullptr;
   }
@@ -450,7 +454,7 @@ PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
   if (boff)
     PNT_HIP_CHECK(hipMemcpyAsync(P.d_off, P.stage + bst + bdl, boff, hipMemcpyHostToDevice, s));
   PNT_HIP_CHECK(hipEventRecord(P.uploaded, s));
-  return PlanDev{P.d_st, P.d_dl, P.d_off, P.h_st.data()};
+  return PlanDev{P.d_st, P.d_dl, P.d_off, P.h_st.data(), P.h_off.data()};
 }
 
 void run_persistent(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {

@@ -23,6 +23,7 @@ struct PlanDev {
   const int2* dl;
   const int* dl_off;
   const PStamp* host;
+  const int* host_off;  // host copy of dl_off
 };
 PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 

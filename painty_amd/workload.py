@@ -293,3 +293,54 @@ def small(rows: int, cols: int, n_stamps: int, radius: int, seed: int, pickup: b
     return Workload("small-%dx%d" % (rows, cols), rows, cols, paint_layer(rows, cols, seed, 100),
                     paint_layer(rows, cols, seed, 0), substrate(rows, cols, seed), st, [core],
                     [float(radius)], pickup_rate=0.9 if pickup else 0.0)
+
+
+@dataclasses.dataclass
+class DiffWorkload:
+    """A stamp workload with mixed paints and a loss target."""
+    wl: Workload
+    palK: np.ndarray      # n_pig x 3
+    palS: np.ndarray
+    W: np.ndarray         # n_paint x n_pig mixing weights (rows on the simplex)
+    paint_of: np.ndarray  # per stamp: paint index at dips, -1 elsewhere
+    target: np.ndarray    # rows x cols x 3
+
+
+def palette(name: str = "lindemeier-measured") -> tuple:
+    """A palette from painty_amd/data (Serialization.cxx:26-53's [{K:[3], S:[3]}])."""
+    import json
+    import os
+    d = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                                    name + ".json")))
+    return (np.array([p["K"] for p in d], np.float64), np.array([p["S"] for p in d], np.float64))
+
+
+def dirichlet_rows(n: int, k: int, st: Stream) -> np.ndarray:
+    """n rows of Dirichlet(1, ..., 1) weights (normalised exponential draws)."""
+    e = -np.log(np.maximum(st.uniform(n * k), 1e-300)).reshape(n, k)
+    return e / e.sum(axis=1, keepdims=True)
+
+
+def cfg4(seed: int = 4, size: int = 2048, n_stamps: int = 500, n_paint: int = 64,
+         radius: int = 32, pickup: bool = True) -> DiffWorkload:
+    """Differentiable configuration: size x size canvas, n_stamps stamps of a
+    (2r+1)^2 Gaussian footprint (sigma = width/4) along seeded strokes, each
+    stroke's paint = mixSinglePaint(W[p]) over the 14-pigment measured
+    palette, W (n_paint x 14) Dirichlet(1); target = seeded smooth value
+    noise in [0.2, 0.9]."""
+    st = strokes(size, size, n_stamps, seed, stamps_per_stroke=(20, 60), reach=0.15)
+    st["brush"][:] = 0
+    width = 2 * radius + 1
+    wl = Workload("cfg4-%d-%dstamps" % (size, n_stamps), size, size,
+                  paint_layer(size, size, seed, 100), paint_layer(size, size, seed, 0),
+                  substrate(size, size, seed), st, [gaussian_core(width, width / 4.0)],
+                  [float(radius)], pickup_rate=0.9 if pickup else 0.0)
+    palK, palS = palette()
+    ws = Stream(seed, 60)
+    W = dirichlet_rows(n_paint, len(palK), ws)
+    dips = np.flatnonzero(st["flags"] & DIP)
+    paint_of = np.full(len(st), -1, np.int32)
+    paint_of[dips] = ws.integers(len(dips), 0, n_paint - 1).astype(np.int32)
+    tn = Stream(seed, 61)
+    target = np.stack([0.2 + 0.7 * value_noise(size, tn, cell=128.0) for _ in range(3)], axis=-1)
+    return DiffWorkload(wl, palK, palS, W, paint_of, target.astype(np.float32))

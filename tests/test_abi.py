@@ -102,3 +102,12 @@ def test_cpp_dropin_program_links():
     assert os.path.exists(exe), "build with make -C tests/cpp"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_calls_during_process_exit_are_safe():
+    """A caller's static destructor calling the library after thread-local
+    objects are gone (process exit) must not write to freed memory."""
+    exe = os.path.join(ROOT, "tests", "cpp", "exit_order")
+    assert os.path.exists(exe), "build with make -C tests/cpp"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "exit ok" in r.stdout, (r.returncode, r.stdout, r.stderr)

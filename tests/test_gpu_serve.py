@@ -1,0 +1,121 @@
+"""The request server (pnt_server_*): requests run concurrently on several
+worker slots give the same canvases, bit for bit, as the same requests run
+one at a time directly through the library (each request = a fresh
+FootprintBrush, its stamps as PNT_EXEC_FUSED, then Renderer::compose); small
+requests match the FP64 oracle (1e-5 relative); evicted canvases fail
+loudly."""
+import numpy as np
+import pytest
+
+from painty_amd import _lib
+from painty_amd import hip as H
+from painty_amd import serve as SV
+from painty_amd import workload as W
+from tests.parity import assert_close
+
+pytestmark = pytest.mark.gpu
+
+
+def make_requests(seed, n_canvas=6, n_req=18, sides=(48, 160), max_stamps=40):
+    rs = W.Stream(seed, 91)
+    canvases = {}
+    for cid in range(n_canvas):
+        rows, cols = (int(x) for x in rs.integers(2, sides[0], sides[1]))
+        bg = rs.uniform(3, 0.7, 0.95).astype(np.float32)
+        canvases[cid] = (rows, cols, bg)
+    reqs = []
+    for k in range(n_req):
+        cid = int(rs.integers(1, 0, n_canvas - 1)[0])
+        rows, cols, bg = canvases[cid]
+        radius = int(rs.integers(1, 2, 10)[0])
+        n = int(rs.integers(1, 1, max_stamps)[0])
+        st = W.strokes(rows, cols, n, seed * 100 + k, stamps_per_stroke=(3, 15), reach=0.4)
+        core = W.noisy_core(2 * radius + 1, W.Stream(seed * 100 + k, 41))
+        reqs.append(dict(canvas_id=cid, rows=rows, cols=cols, background=bg, stamps=st,
+                         radius=float(radius), footprint=core,
+                         use_snapshot=bool(k % 4 != 3)))
+    return canvases, reqs
+
+
+def run_alone(canvases, reqs):
+    """The same requests, one at a time, straight through the library."""
+    state = {}
+    for q in reqs:
+        cid = q["canvas_id"]
+        if cid not in state:
+            rows, cols, bg = canvases[cid]
+            cv = H.Canvas(rows, cols)
+            cv.setR0(np.broadcast_to(bg, (rows, cols, 3)).copy())
+            state[cid] = cv
+        cv = state[cid]
+        br = H.FootprintBrush(q["radius"], q["footprint"])
+        br.setUseSnapshotBuffer(q["use_snapshot"])
+        br.imprintStamps(q["stamps"], cv, _lib.EXEC_FUSED)
+    out = {}
+    for cid, cv in state.items():
+        L = cv.getPaintLayer()
+        R = H.Renderer.compose(cv)
+        out[cid] = (L.getK_buffer(), L.getS_buffer(), L.getV_buffer(), cv.getR0(), R)
+    return out
+
+
+@pytest.mark.parametrize("slots", [1, 4])
+def test_mixed_equals_alone(hip, slots):
+    canvases, reqs = make_requests(7)
+    alone = run_alone(canvases, reqs)
+    srv = SV.Server(n_slots=slots, hbm_budget_bytes=1 << 30)
+    tickets = [srv.submit(**q) for q in reqs]  # all in flight together
+    lat = [srv.wait(t) for t in tickets]
+    assert all(l > 0 for l in lat)
+    last = {}
+    for q in reqs:
+        last[q["canvas_id"]] = q
+    for cid in last:
+        K, S, V, R0, R = srv.canvas_state(cid)
+        aK, aS, aV, aR0, aR = alone[cid]
+        for name, x, y in (("K", K, aK), ("S", S, aS), ("V", V, aV), ("R0", R0, aR0)):
+            assert np.array_equal(x, y), (cid, name)
+        # the composite of the canvas' last request
+        assert np.array_equal(R.transpose(1, 2, 0), aR), cid
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["resident_canvases"] == len(last)
+    srv.close()
+
+
+def test_served_matches_oracle(hip, oracle):
+    canvases, reqs = make_requests(3, n_canvas=2, n_req=5, sides=(40, 70), max_stamps=25)
+    srv = SV.Server(n_slots=3, hbm_budget_bytes=1 << 30)
+    for t in [srv.submit(**q) for q in reqs]:
+        srv.wait(t)
+    for cid, (rows, cols, bg) in canvases.items():
+        ocv = oracle.Canvas.from_arrays(np.zeros((rows, cols, 3)), np.zeros((rows, cols, 3)),
+                                        np.zeros((rows, cols)),
+                                        np.broadcast_to(bg.astype(np.float64), (rows, cols, 3)).copy())
+        for q in reqs:
+            if q["canvas_id"] != cid:
+                continue
+            b = oracle.Brush(q["radius"], q["footprint"].astype(np.float64))
+            b.set_use_snapshot(q["use_snapshot"])
+            b.run(ocv, q["stamps"])
+        if not any(q["canvas_id"] == cid for q in reqs):
+            continue
+        K, S, V, R0, R = srv.canvas_state(cid)
+        tag = "serve/c%d" % cid
+        assert_close(tag + ".K", K, ocv.K)
+        assert_close(tag + ".V", V, ocv.V)
+        assert_close(tag + ".R", R.transpose(1, 2, 0), ocv.compose())
+    srv.close()
+
+
+def test_eviction_drops_state_and_fails_loudly(hip):
+    core = W.gaussian_core(9, 2.25)
+    st = W.strokes(64, 64, 10, 5, stamps_per_stroke=(5, 10), reach=0.3)
+    one = 56 * 64 * 64  # canvas + composite planes of a 64x64 canvas
+    srv = SV.Server(n_slots=2, hbm_budget_bytes=2 * one + 1024)
+    for cid in (1, 2, 3):  # the third evicts the least recently used (1)
+        srv.wait(srv.submit(cid, 64, 64, st, 4.0, core))
+    assert srv.stats()["resident_canvases"] == 2
+    with pytest.raises(_lib.PaintyError):
+        srv.wait(srv.submit(1, 64, 64, st, 4.0, core))
+    srv.wait(srv.submit(3, 64, 64, st, 4.0, core))  # a resident one still works
+    srv.close()
