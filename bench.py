@@ -143,16 +143,41 @@ def bench_batched(args):
     for _ in range(args.warmup):
         run(_lib.BATCH_ONE_LAUNCH)
         ctx.synchronize()
-    a, b = ev.event(), ev.event()
-    t0 = time.perf_counter()
-    ev.record(a, stream)
-    for _ in range(args.steps):
-        run(_lib.BATCH_ONE_LAUNCH)
-    ev.record(b, stream)
-    ctx.synchronize()
-    wall = (time.perf_counter() - t0) * 1e3 / args.steps
-    gpu = ev.elapsed_ms(a, b) / args.steps
-    log("bench: one launch %.2f ms/step wall, %.2f ms/step stream" % (wall, gpu))
+    tuned = None
+    if args.batch_tile:
+        H.batch_set_config(args.batch_tile, args.batch_group)
+        tuned = {"tile": args.batch_tile, "max_group": args.batch_group, "ms": None}
+    elif not args.no_tune:
+        t0 = time.perf_counter()
+        (tile, group), tms = H.batch_tune(jobs)
+        tuned = {"tile": tile, "max_group": group, "ms": round(tms, 3)}
+        log("bench: tuned tile %d, max_group %d (%.2f ms) in %.1fs" %
+            (tile, group, tms, time.perf_counter() - t0))
+
+    def timed(step):
+        a, b = ev.event(), ev.event()
+        t0 = time.perf_counter()
+        ev.record(a, stream)
+        for _ in range(args.steps):
+            step()
+        ev.record(b, stream)
+        ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, ev.elapsed_ms(a, b) / args.steps
+
+    eager_wall, eager_gpu = timed(lambda: run(_lib.BATCH_ONE_LAUNCH))
+    log("bench: one launch %.2f ms/step wall, %.2f ms/step stream" % (eager_wall, eager_gpu))
+    wall, gpu, path = eager_wall, eager_gpu, "eager one-launch"
+    graph_ms = None
+    if not args.no_graph:
+        g = H.BatchGraph(jobs)
+        g.launch()
+        ctx.synchronize()
+        gw, gg = timed(g.launch)
+        g.close()
+        graph_ms = round(gw, 3)
+        log("bench: graph replay %.2f ms/step wall, %.2f ms/step stream" % (gw, gg))
+        if gw < wall:
+            wall, gpu, path = gw, gg, "captured graph replay"
     a2, b2 = ev.event(), ev.event()
     t0 = time.perf_counter()
     ev.record(a2, stream)
@@ -160,7 +185,7 @@ def bench_batched(args):
         run(_lib.BATCH_PLAIN)
     ev.record(b2, stream)
     ctx.synchronize()
-    plain = (time.perf_counter() - t0) * 1e3 / args.plain_steps
+    plain = (time.perf_counter() - t0) * 1e3 / max(1, args.plain_steps)
     log("bench: plain %.2f ms/step" % plain)
     gbs = alg_bytes / (gpu * 1e-3) / 1e9
     result = {
@@ -172,6 +197,8 @@ def bench_batched(args):
                                "(footprint 32-128 px, pickup on) + composites, one launch"
                                % (len(jobs), n_st)},
         "composite_gpixel_s_in_batch": round(n_px / (wall * 1e-3) / 1e9, 2),
+        "path": path, "eager_ms_per_step": round(eager_wall, 3), "graph_ms_per_step": graph_ms,
+        "tuned": tuned,
         "plain_sequential_ms_per_step": round(plain, 2),
         "speedup_vs_plain": round(plain / wall, 2),
         "roofline": {"bound": "hbm", "kernel": "k_batch", "achieved": round(gbs, 1),
@@ -390,7 +417,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["plain", "persistent", "fused"], default="fused")
+    ap.add_argument("--mode", choices=["plain", "persistent", "fused", "chained"], default="fused")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--stamps", type=int, default=2000)
     ap.add_argument("--cpu-stamps", type=int, default=300)
@@ -406,6 +433,11 @@ def main():
     ap.add_argument("--plain-requests", type=int, default=40)
     ap.add_argument("--canvases", type=int, default=256)
     ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-tune", action="store_true", help="batched: keep the default launch shape")
+    ap.add_argument("--no-graph", action="store_true", help="batched: skip the captured graph")
+    ap.add_argument("--batch-tile", type=int, default=0, help="batched: fixed tile (16 or 32)")
+    ap.add_argument("--batch-group", type=int, default=16,
+                    help="batched: workgroups per job with --batch-tile")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -446,7 +478,7 @@ def main():
     dev_out = _lib.DevRgb((C.c_void_p * 3)(*Ro))
     n_px = wl.rows * wl.cols
     mode = {"plain": _lib.EXEC_PLAIN, "persistent": _lib.EXEC_PERSISTENT,
-            "fused": _lib.EXEC_FUSED}[args.mode]
+            "fused": _lib.EXEC_FUSED, "chained": _lib.EXEC_CHAINED}[args.mode]
 
     # segments: one brush geometry, no dryCanvas inside (it runs between them)
     segs = []
@@ -527,7 +559,7 @@ def main():
     stamps_s = args.stamps / (ms_per_step * 1e-3)
     stamp_gbs = stamp_bytes / (st_ms * 1e-3) / 1e9
     kernel = {"plain": "k_stamp_main+k_ring_refresh", "persistent": "k_persistent",
-              "fused": "k_stamp_fused<16>"}[args.mode]
+              "fused": "k_stamp_fused<16>", "chained": "k_stamp_chain<16>"}[args.mode]
     result = {
         "metric": METRIC, "value": round(stamps_s, 1), "unit": "stamps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),

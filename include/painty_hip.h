@@ -193,10 +193,13 @@ typedef struct {
  * (snapshot ring copy, then the visits); PNT_EXEC_PERSISTENT runs the whole
  * list in one persistent launch; PNT_EXEC_FUSED plans the list once and
  * issues one launch per stamp in which the snapshot refresh (dirty pixels
- * only) and the visits run side by side. */
+ * only) and the visits run side by side; PNT_EXEC_CHAINED runs the same
+ * work for the whole list in one launch, each stamp's blocks starting when
+ * the previous stamp's blocks signal completion through device counters. */
 #define PNT_EXEC_PLAIN 0
 #define PNT_EXEC_PERSISTENT 1
 #define PNT_EXEC_FUSED 2
+#define PNT_EXEC_CHAINED 3
 int pnt_imprint_stamps(pnt_brush brush, pnt_canvas canvas,
                        const pnt_stamp* stamps, int64_t n, int mode);
 
@@ -219,6 +222,36 @@ typedef struct {
 #define PNT_BATCH_ONE_LAUNCH 0
 #define PNT_BATCH_PLAIN 1
 int pnt_batch_run(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int mode);
+
+/* Shape of a one-launch batch (no reference counterpart): window tiles of
+ * tile x tile pixels (16 or 32; one thread each) and at most max_group
+ * workgroups on one job's stamps.  A context starts with {32, 16}. */
+typedef struct {
+  int tile;
+  int max_group;
+  int reserved[2];
+} pnt_batch_config;
+int pnt_batch_set_config(pnt_ctx ctx, const pnt_batch_config* cfg);
+int pnt_batch_get_config(pnt_ctx ctx, pnt_batch_config* cfg);
+/* Autotuner: runs the batch under each candidate configuration, timing it
+ * on the context's stream, restores every canvas and brush of the batch
+ * after each run, and keeps the fastest configuration for the context.
+ * The jobs' state is unchanged on return. */
+int pnt_batch_tune(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs,
+                   pnt_batch_config* best, double* best_ms);
+/* The batch planned once and captured as a HIP graph (plan copy, counter
+ * clear, k_batch); each launch replays it on the jobs' current state -- the
+ * same result as pnt_batch_run(PNT_BATCH_ONE_LAUNCH) with the same jobs.
+ * Every stamp list must start with PNT_STAMP_DIP; a canvas or brush changed
+ * outside the graph since its last launch makes the next launch fail
+ * (PNT_EINVAL).  The captured kernel is a plain launch of at most the
+ * co-resident workgroup count: run it on an otherwise idle device (its
+ * waits are bounded, so a violation reports PNT_EKERNEL, it cannot hang). */
+typedef struct pnt_batch_graph_s* pnt_batch_graph;
+int pnt_batch_capture(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs,
+                      pnt_batch_graph* out);
+int pnt_batch_graph_launch(pnt_batch_graph graph);
+int pnt_batch_graph_destroy(pnt_batch_graph graph);
 
 /* paintStroke(path, canvas) -- FootprintBrush.hxx:206-268: Catmull-Rom
  * placement at 1 px chord steps (core/Spline.hxx:27-73), one imprint per

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpainty_hip.so")
 OK, EINVAL, EHIP, ENOMEM, ESTATE, EKERNEL = 0, -1, -2, -3, -4, -5
 AOS, PLANAR = 0, 1
 STAMP_DIP, STAMP_DRY_AFTER = 1, 2
-EXEC_PLAIN, EXEC_PERSISTENT, EXEC_FUSED = 0, 1, 2
+EXEC_PLAIN, EXEC_PERSISTENT, EXEC_FUSED, EXEC_CHAINED = 0, 1, 2, 3
 BATCH_ONE_LAUNCH, BATCH_PLAIN = 0, 1
 
 
@@ -38,6 +38,10 @@ class PntStamp(C.Structure):
 class BatchJob(C.Structure):
     _fields_ = [("canvas", C.c_void_p), ("brush", C.c_void_p), ("stamps", C.POINTER(PntStamp)),
                 ("n_stamps", C.c_int64), ("out", C.c_void_p * 3)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("tile", C.c_int), ("max_group", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class PntRequest(C.Structure):
@@ -114,6 +118,12 @@ SIGNATURES = {
     "pnt_imprint_stamps": [_vp, _vp, C.POINTER(PntStamp), C.c_int64, C.c_int],
     "pnt_paint_stroke": [_vp, _vp, _dp, C.c_int64],
     "pnt_batch_run": [_vp, C.POINTER(BatchJob), C.c_int, C.c_int],
+    "pnt_batch_set_config": [_vp, C.POINTER(BatchConfig)],
+    "pnt_batch_get_config": [_vp, C.POINTER(BatchConfig)],
+    "pnt_batch_tune": [_vp, C.POINTER(BatchJob), C.c_int, C.POINTER(BatchConfig), _dp],
+    "pnt_batch_capture": [_vp, C.POINTER(BatchJob), C.c_int, C.POINTER(_vp)],
+    "pnt_batch_graph_launch": [_vp],
+    "pnt_batch_graph_destroy": [_vp],
     "pnt_brush_stamp_traffic": [_vp, C.c_int, C.c_int, C.POINTER(PntStamp), C.c_int64,
                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "pnt_mix_single": [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp],

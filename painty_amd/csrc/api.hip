@@ -137,9 +137,17 @@ void brush_set_geometry(pnt_brush b, double radius, const float* core, int width
       throw;
     }
   }
-  PNT_HIP_CHECK(hipMemcpyAsync(fp, full.data(), m * sizeof(float), hipMemcpyHostToDevice, s));
-  launch_fill(pick, 7 * mstride, 0.0f, s);
-  PNT_HIP_CHECK(hipStreamSynchronize(s));
+  try {
+    PNT_HIP_CHECK(hipMemcpyAsync(fp, full.data(), m * sizeof(float), hipMemcpyHostToDevice, s));
+    launch_fill(pick, 7 * mstride, 0.0f, s);
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    if (fp != b->fp) {  // the new buffers are not the brush's yet
+      dev_free(fp);
+      dev_free(pick);
+    }
+    throw;
+  }
   if (fp != b->fp) {
     dev_free(b->fp);
     dev_free(b->pick);
@@ -383,6 +391,24 @@ const char* pnt_last_error(void) { return g_err; }
 
 int pnt_version(void) { return 100; }
 
+}  // extern "C"
+
+namespace pnt {
+void ctx_retain(pnt_ctx ctx) { ctx->refs.fetch_add(1, std::memory_order_relaxed); }
+
+void ctx_release(pnt_ctx ctx) {
+  if (ctx->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  batch_release(ctx);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->kernel_error) (void)hipHostFree(ctx->kernel_error);
+  delete ctx;
+}
+}  // namespace pnt
+
+extern "C" {
+
 int pnt_ctx_create(int device, pnt_ctx* out) {
   return guard([&] {
     PNT_REQUIRE(out != nullptr, "out is NULL");
@@ -407,13 +433,7 @@ int pnt_ctx_create(int device, pnt_ctx* out) {
 
 int pnt_ctx_destroy(pnt_ctx ctx) {
   return guard([&] {
-    if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    batch_release(ctx);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    if (ctx->kernel_error) (void)hipHostFree(ctx->kernel_error);
-    delete ctx;
+    if (ctx) ctx_release(ctx);
   });
 }
 
@@ -450,6 +470,7 @@ int pnt_layer_create(pnt_ctx ctx, int rows, int cols, pnt_layer* out) {
       delete L;
       throw;
     }
+    ctx_retain(ctx);
     *out = L;
   });
 }
@@ -458,10 +479,12 @@ int pnt_layer_destroy(pnt_layer L) {
   return guard([&] {
     if (!L) return;
     if (L->owner) {
-      (void)hipSetDevice(L->ctx->device);
-      (void)hipStreamSynchronize(L->ctx->stream);
+      pnt_ctx ctx = L->ctx;
+      (void)hipSetDevice(ctx->device);
+      (void)hipStreamSynchronize(ctx->stream);
       dev_free(L->base);
       delete L;
+      ctx_release(ctx);
     }
   });
 }
@@ -619,7 +642,13 @@ pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base) {
   cv->view.base = cv->base;
   cv->view.owner = false;
   cv->view.parent = cv;
-  canvas_clear(cv);
+  try {
+    canvas_clear(cv);
+  } catch (...) {
+    delete cv;
+    throw;
+  }
+  ctx_retain(ctx);
   return cv;
 }
 }  // namespace pnt
@@ -655,6 +684,7 @@ int pnt_canvas_create(pnt_ctx ctx, int rows, int cols, pnt_canvas* out) {
     cv->view.base = cv->base;
     cv->view.owner = false;
     cv->view.parent = cv;
+    ctx_retain(ctx);
     *out = cv;
   });
 }
@@ -662,11 +692,13 @@ int pnt_canvas_create(pnt_ctx ctx, int rows, int cols, pnt_canvas* out) {
 int pnt_canvas_destroy(pnt_canvas cv) {
   return guard([&] {
     if (!cv) return;
-    (void)hipSetDevice(cv->ctx->device);
-    (void)hipStreamSynchronize(cv->ctx->stream);
+    pnt_ctx ctx = cv->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     if (cv->owns_base) dev_free(cv->base);
     dev_free(cv->scratch);
     delete cv;
+    ctx_release(ctx);
   });
 }
 
@@ -906,6 +938,7 @@ int pnt_brush_create(pnt_ctx ctx, double radius, const float* footprint, int wid
       delete b;
       throw;
     }
+    ctx_retain(ctx);
     *out = b;
   });
 }
@@ -916,13 +949,16 @@ int pnt_brush_destroy(pnt_brush b) {
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     persistent_release(b->pers);
+    chain_release(b->chain);
     dev_free(b->fp);
     dev_free(b->pick);
     dev_free(b->snap);
     dev_free(b->d_list);
     dev_free(b->d_off);
     dev_free(b->dirty);
+    pnt_ctx ctx = b->ctx;
     delete b;
+    ctx_release(ctx);
   });
 }
 
@@ -1041,7 +1077,8 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
   return guard([&] {
     check_same_ctx(b, cv);
     PNT_REQUIRE(n >= 0 && (st || n == 0), "invalid stamp list");
-    PNT_REQUIRE(mode == PNT_EXEC_PLAIN || mode == PNT_EXEC_PERSISTENT || mode == PNT_EXEC_FUSED,
+    PNT_REQUIRE(mode == PNT_EXEC_PLAIN || mode == PNT_EXEC_PERSISTENT || mode == PNT_EXEC_FUSED ||
+                    mode == PNT_EXEC_CHAINED,
                 "unknown mode");
     b->ctx->use();
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
@@ -1049,6 +1086,8 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
       run_persistent(b, cv, st, n);
     } else if (mode == PNT_EXEC_FUSED) {
       run_fused(b, cv, st, n);
+    } else if (mode == PNT_EXEC_CHAINED) {
+      run_chained(b, cv, st, n);
     } else {
       for (int64_t i = 0; i < n; ++i) {
         if (st[i].flags & PNT_STAMP_DIP) brush_dip(b, st[i].K, st[i].S);
@@ -1102,36 +1141,47 @@ int pnt_brush_stamp_traffic(pnt_brush b, int rows, int cols, const pnt_stamp* st
   });
 }
 
+// The checks every batch entry point makes on its jobs.
+static void validate_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs) {
+  PNT_REQUIRE(ctx && (jobs || n_jobs == 0) && n_jobs >= 0, "invalid batch");
+  ctx->use();
+  std::vector<const void*> seen;
+  for (int j = 0; j < n_jobs; ++j) {
+    const pnt_batch_job& J = jobs[j];
+    PNT_REQUIRE(J.canvas && J.brush, "batch job without canvas or brush");
+    PNT_REQUIRE(J.canvas->ctx->device == ctx->device && J.brush->ctx->device == ctx->device,
+                "batch job on another device");
+    PNT_REQUIRE(J.n_stamps >= 0 && (J.stamps || J.n_stamps == 0), "invalid stamp list");
+    if (J.out[0])
+      for (int c = 0; c < 3; ++c)
+        PNT_REQUIRE(J.out[c] && ((uintptr_t)J.out[c] & 15u) == 0,
+                    "composite planes must be 16-byte aligned device pointers");
+    seen.push_back(J.canvas);
+    seen.push_back(J.brush);
+  }
+  std::sort(seen.begin(), seen.end());
+  PNT_REQUIRE(std::adjacent_find(seen.begin(), seen.end()) == seen.end(),
+              "canvases and brushes must be distinct across the jobs of a batch");
+  for (int j = 0; j < n_jobs; ++j)  // other streams' work on these objects comes first
+    if (jobs[j].canvas->ctx->stream != ctx->stream || jobs[j].brush->ctx->stream != ctx->stream) {
+      PNT_HIP_CHECK(hipStreamSynchronize(jobs[j].canvas->ctx->stream));
+      PNT_HIP_CHECK(hipStreamSynchronize(jobs[j].brush->ctx->stream));
+    }
+}
+
+static void check_config(const pnt_batch_config* c) {
+  PNT_REQUIRE(c, "NULL configuration");
+  PNT_REQUIRE(c->tile == 16 || c->tile == 32, "tile must be 16 or 32");
+  PNT_REQUIRE(c->max_group >= 1 && c->max_group <= 256, "max_group must be in [1, 256]");
+}
+
 int pnt_batch_run(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int mode) {
   return guard([&] {
-    PNT_REQUIRE(ctx && (jobs || n_jobs == 0) && n_jobs >= 0, "invalid batch");
     PNT_REQUIRE(mode == PNT_BATCH_ONE_LAUNCH || mode == PNT_BATCH_PLAIN, "unknown batch mode");
+    validate_batch(ctx, jobs, n_jobs);
     if (n_jobs == 0) return;
-    ctx->use();
-    std::vector<const void*> seen;
-    for (int j = 0; j < n_jobs; ++j) {
-      const pnt_batch_job& J = jobs[j];
-      PNT_REQUIRE(J.canvas && J.brush, "batch job without canvas or brush");
-      PNT_REQUIRE(J.canvas->ctx->device == ctx->device && J.brush->ctx->device == ctx->device,
-                  "batch job on another device");
-      PNT_REQUIRE(J.n_stamps >= 0 && (J.stamps || J.n_stamps == 0), "invalid stamp list");
-      if (J.out[0])
-        for (int c = 0; c < 3; ++c)
-          PNT_REQUIRE(J.out[c] && ((uintptr_t)J.out[c] & 15u) == 0,
-                      "composite planes must be 16-byte aligned device pointers");
-      seen.push_back(J.canvas);
-      seen.push_back(J.brush);
-    }
-    std::sort(seen.begin(), seen.end());
-    PNT_REQUIRE(std::adjacent_find(seen.begin(), seen.end()) == seen.end(),
-                "canvases and brushes must be distinct across the jobs of a batch");
-    for (int j = 0; j < n_jobs; ++j)  // other streams' work on these objects comes first
-      if (jobs[j].canvas->ctx->stream != ctx->stream || jobs[j].brush->ctx->stream != ctx->stream) {
-        PNT_HIP_CHECK(hipStreamSynchronize(jobs[j].canvas->ctx->stream));
-        PNT_HIP_CHECK(hipStreamSynchronize(jobs[j].brush->ctx->stream));
-      }
     if (mode == PNT_BATCH_ONE_LAUNCH) {
-      run_batch_one_launch(ctx, jobs, n_jobs);
+      run_batch_one_launch(ctx, jobs, n_jobs, nullptr);
       return;
     }
     for (int j = 0; j < n_jobs; ++j) {  // plain: the job's launches on its brush's stream
@@ -1152,6 +1202,56 @@ int pnt_batch_run(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int mode) 
         launch_compose(cv->n, cv->layer(), cv->R0(), Rgb3{{J.out[0], J.out[1], J.out[2]}}, s);
       if (s != ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(s));
     }
+  });
+}
+
+int pnt_batch_tune(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, pnt_batch_config* best,
+                   double* best_ms) {
+  return guard([&] {
+    validate_batch(ctx, jobs, n_jobs);
+    PNT_REQUIRE(n_jobs > 0, "empty batch");
+    tune_batch(ctx, jobs, n_jobs, best, best_ms);
+  });
+}
+
+int pnt_batch_set_config(pnt_ctx ctx, const pnt_batch_config* cfg) {
+  return guard([&] {
+    PNT_REQUIRE(ctx, "NULL context");
+    check_config(cfg);
+    ctx->use();
+    pnt_batch_config c = *cfg;
+    set_batch_config(ctx, c);
+  });
+}
+
+int pnt_batch_get_config(pnt_ctx ctx, pnt_batch_config* cfg) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && cfg, "NULL argument");
+    ctx->use();
+    *cfg = get_batch_config(ctx);
+  });
+}
+
+int pnt_batch_capture(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, pnt_batch_graph* out) {
+  return guard([&] {
+    PNT_REQUIRE(out, "NULL argument");
+    validate_batch(ctx, jobs, n_jobs);
+    PNT_REQUIRE(n_jobs > 0, "empty batch");
+    *out = reinterpret_cast<pnt_batch_graph>(capture_batch(ctx, jobs, n_jobs));
+  });
+}
+
+int pnt_batch_graph_launch(pnt_batch_graph g) {
+  return guard([&] {
+    PNT_REQUIRE(g, "NULL graph");
+    launch_batch_graph(reinterpret_cast<BatchGraph*>(g));
+  });
+}
+
+int pnt_batch_graph_destroy(pnt_batch_graph g) {
+  return guard([&] {
+    BatchGraph* G = reinterpret_cast<BatchGraph*>(g);
+    release_batch_graph(G);
   });
 }
 

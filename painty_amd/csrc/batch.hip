@@ -31,6 +31,7 @@
 #include "handles.hpp"
 #include "internal.hpp"
 #include "km.hpp"
+#include "persistent.hpp"
 #include "stampcore.hpp"
 
 namespace pnt {
@@ -39,8 +40,6 @@ enum : uint32_t { kStDry = 8u };  // dryCanvas() after this stamp
 
 namespace {
 
-constexpr int kWT = 32;                  // window tile side
-constexpr int kBT = kWT * kWT;           // threads per block
 constexpr int64_t kComposeItem = 1 << 18;  // pixels per composite work item
 
 struct BJob {
@@ -104,7 +103,11 @@ __device__ __forceinline__ bool wait_done(unsigned* flag, unsigned* host_err, un
   return true;
 }
 
-__global__ __launch_bounds__(kBT) void k_batch(BArgs A) {
+// One launch of a batch plan; kWT x kWT window tiles, one thread per tile
+// pixel.
+template <int kWT>
+__global__ __launch_bounds__(kWT* kWT) void k_batch(BArgs A) {
+  constexpr int kBT = kWT * kWT;
   __shared__ int sm[kWT + 2][kWT + 2];
   __shared__ int s_item;
   __shared__ int s_ok;
@@ -294,7 +297,8 @@ __global__ __launch_bounds__(kBT) void k_batch(BArgs A) {
 
 // ------------------------------------------------------------------- host --
 struct BatchState {
-  int max_blocks = 0;
+  int max_blocks16 = 0, max_blocks32 = 0;
+  pnt_batch_config cfg{32, 16, {0, 0}};  // the configuration runs use (pnt_batch_tune sets it)
   char* stage = nullptr;
   int64_t stage_cap = 0;
   char* dev = nullptr;
@@ -305,9 +309,9 @@ struct BatchState {
 static BatchState* batch_state(pnt_ctx ctx);
 
 // cost of one stamp for a group of g blocks, in tile-iteration units
-static double stamp_cost(int hr, int g) {
+static double stamp_cost(int hr, int g, int tile) {
   const int side = 2 * hr + 1;
-  const int tiles = ((side + kWT - 1) / kWT) * ((side + kWT - 1) / kWT);
+  const int tiles = ((side + tile - 1) / tile) * ((side + tile - 1) / tile);
   const double iters = std::ceil((double)tiles / g);
   return iters + (g > 1 ? 3.0 : 0.0) + 1.0;  // + group barrier, + ring scan / bookkeeping
 }
@@ -321,17 +325,18 @@ struct Placement {
 
 // Packs jobs into groups of blocks (within XCD lanes) to balance the
 // makespan: each job starts with one block; blocks are then added to the
-// job whose finish time is latest while that lowers it; jobs beyond the
-// block count share blocks, longest first onto the least loaded block.
+// job whose finish time is latest while that lowers it (up to max_group
+// blocks a job); jobs beyond the block count share blocks, longest first
+// onto the least loaded block.
 static Placement place_jobs(const std::vector<int>& hr, const std::vector<int64_t>& nst,
-                            int max_blocks) {
+                            int max_blocks, int max_group, int tile) {
   const int nj = (int)hr.size();
   Placement P;
   const int lanes = 8;
   int slots = std::max(lanes, (max_blocks / lanes) * lanes);
   std::vector<int> order(nj);
   std::iota(order.begin(), order.end(), 0);
-  auto cost = [&](int j, int g) { return (double)nst[j] * stamp_cost(hr[j], g); };
+  auto cost = [&](int j, int g) { return (double)nst[j] * stamp_cost(hr[j], g, tile); };
   std::sort(order.begin(), order.end(), [&](int a, int b) { return cost(a, 1) > cost(b, 1); });
   std::vector<int> gsize;                 // blocks per group
   std::vector<std::vector<int>> gjobs;    // jobs per group
@@ -347,7 +352,7 @@ static Placement place_jobs(const std::vector<int>& hr, const std::vector<int64_
       int worst = (int)(std::max_element(gload.begin(), gload.end()) - gload.begin());
       const int j = gjobs[worst][0];
       const int g = gsize[worst];
-      if (g >= 16) break;
+      if (g >= max_group) break;
       const double c1 = cost(j, g + 1);
       if (c1 >= gload[worst] * 0.97) break;
       gsize[worst] = g + 1;
@@ -409,12 +414,21 @@ static Placement place_jobs(const std::vector<int>& hr, const std::vector<int64_
   return P;
 }
 
-void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n) {
+// A batch, planned: the packed device image of jobs, groups, stamps, shared-
+// pixel lists and composite items (counters follow it on the device).
+struct BatchPlan {
+  std::vector<char> blob;
+  int64_t o_jobs, o_groups, o_bg, o_jl, o_st, o_dl, o_off, o_items, o_counters;
+  int64_t n_counters, total;
+  int njobs = 0, n_items = 0, nblocks = 0, tile = 32;
+};
+
+// Plans a batch; readies each brush's snapshot and dirty bytes on the
+// context's stream first (that work is done here, once).
+static BatchPlan plan_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
+                            const pnt_batch_config& cfg) {
   hipStream_t s = ctx->stream;
-  if (*(volatile unsigned*)ctx->kernel_error)
-    throw Error(PNT_EKERNEL, "an earlier persistent stamp kernel aborted");
   BatchState& B = *batch_state(ctx);
-  // ---- plan
   std::vector<PStamp> h_st;
   std::vector<int2> h_dl;
   std::vector<int> h_off;
@@ -458,7 +472,9 @@ void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n) {
     hr[j] = J.hr;
     nst[j] = jobs[j].n_stamps;
   }
-  const Placement PL = place_jobs(hr, nst, B.max_blocks);
+  const int tile = cfg.tile == 16 ? 16 : 32;
+  const int max_blocks = tile == 16 ? B.max_blocks16 : B.max_blocks32;
+  const Placement PL = place_jobs(hr, nst, max_blocks, std::max(1, cfg.max_group), tile);
   // composite items, jobs that finish stamping early first
   std::vector<int> by_cost(n);
   std::iota(by_cost.begin(), by_cost.end(), 0);
@@ -467,7 +483,7 @@ void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n) {
     double tacc = 0.0;
     for (int k = 0; k < G.njobs; ++k) {
       const int j = PL.job_list[G.job0 + k];
-      tacc += (double)nst[j] * stamp_cost(hr[j], G.nblocks);
+      tacc += (double)nst[j] * stamp_cost(hr[j], G.nblocks, tile);
       finish[j] = tacc;
     }
   }
@@ -478,67 +494,76 @@ void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n) {
     for (int64_t b0 = 0; b0 < npx; b0 += kComposeItem)
       items.push_back(CItem{j, 0, b0, std::min(npx, b0 + kComposeItem)});
   }
-  // ---- one device buffer for the plan, through pinned staging
+  BatchPlan P;
   auto align = [](int64_t x) { return (x + 255) / 256 * 256; };
-  const int64_t o_jobs = 0;
-  const int64_t o_groups = align(o_jobs + (int64_t)h_jobs.size() * sizeof(BJob));
-  const int64_t o_bg = align(o_groups + (int64_t)PL.groups.size() * sizeof(BGroup));
-  const int64_t o_jl = align(o_bg + (int64_t)PL.block_group.size() * sizeof(int));
-  const int64_t o_st = align(o_jl + (int64_t)PL.job_list.size() * sizeof(int));
-  const int64_t o_dl = align(o_st + (int64_t)h_st.size() * sizeof(PStamp));
-  const int64_t o_off = align(o_dl + (int64_t)h_dl.size() * sizeof(int2));
-  const int64_t o_items = align(o_off + (int64_t)h_off.size() * sizeof(int));
-  const int64_t o_counters = align(o_items + (int64_t)items.size() * sizeof(CItem));
-  const int64_t n_counters = 1 + n + (int64_t)PL.groups.size() * kBarWords;
-  const int64_t total = align(o_counters + n_counters * (int64_t)sizeof(unsigned));
-  PNT_HIP_CHECK(hipEventSynchronize(B.uploaded));
-  if (total > B.stage_cap) {
-    if (B.stage) PNT_HIP_CHECK(hipHostFree(B.stage));
-    B.stage = nullptr;
-    B.stage_cap = std::max<int64_t>(total, 2 * B.stage_cap);
-    PNT_HIP_CHECK(hipHostMalloc((void**)&B.stage, (size_t)B.stage_cap));
-  }
-  if (total > B.dev_cap) {
-    PNT_HIP_CHECK(hipStreamSynchronize(s));
-    if (B.dev) PNT_HIP_CHECK(hipFree(B.dev));
-    B.dev = nullptr;
-    B.dev_cap = std::max<int64_t>(total, 2 * B.dev_cap);
-    PNT_HIP_CHECK(hipMalloc((void**)&B.dev, (size_t)B.dev_cap));
-  }
+  P.o_jobs = 0;
+  P.o_groups = align(P.o_jobs + (int64_t)h_jobs.size() * sizeof(BJob));
+  P.o_bg = align(P.o_groups + (int64_t)PL.groups.size() * sizeof(BGroup));
+  P.o_jl = align(P.o_bg + (int64_t)PL.block_group.size() * sizeof(int));
+  P.o_st = align(P.o_jl + (int64_t)PL.job_list.size() * sizeof(int));
+  P.o_dl = align(P.o_st + (int64_t)h_st.size() * sizeof(PStamp));
+  P.o_off = align(P.o_dl + (int64_t)h_dl.size() * sizeof(int2));
+  P.o_items = align(P.o_off + (int64_t)h_off.size() * sizeof(int));
+  P.o_counters = align(P.o_items + (int64_t)items.size() * sizeof(CItem));
+  P.n_counters = 1 + n + (int64_t)PL.groups.size() * kBarWords;
+  P.total = align(P.o_counters + P.n_counters * (int64_t)sizeof(unsigned));
+  P.blob.assign((size_t)P.o_counters, 0);
   auto put = [&](int64_t off, const void* src, size_t bytes) {
-    if (bytes) std::memcpy(B.stage + off, src, bytes);
+    if (bytes) std::memcpy(P.blob.data() + off, src, bytes);
   };
-  put(o_jobs, h_jobs.data(), h_jobs.size() * sizeof(BJob));
-  put(o_groups, PL.groups.data(), PL.groups.size() * sizeof(BGroup));
-  put(o_bg, PL.block_group.data(), PL.block_group.size() * sizeof(int));
-  put(o_jl, PL.job_list.data(), PL.job_list.size() * sizeof(int));
-  put(o_st, h_st.data(), h_st.size() * sizeof(PStamp));
-  put(o_dl, h_dl.data(), h_dl.size() * sizeof(int2));
-  put(o_off, h_off.data(), h_off.size() * sizeof(int));
-  put(o_items, items.data(), items.size() * sizeof(CItem));
-  PNT_HIP_CHECK(hipMemcpyAsync(B.dev, B.stage, (size_t)o_counters, hipMemcpyHostToDevice, s));
-  PNT_HIP_CHECK(hipEventRecord(B.uploaded, s));
-  unsigned* counters = reinterpret_cast<unsigned*>(B.dev + o_counters);
-  launch_fill_u32(counters, n_counters, 0u, s);
+  put(P.o_jobs, h_jobs.data(), h_jobs.size() * sizeof(BJob));
+  put(P.o_groups, PL.groups.data(), PL.groups.size() * sizeof(BGroup));
+  put(P.o_bg, PL.block_group.data(), PL.block_group.size() * sizeof(int));
+  put(P.o_jl, PL.job_list.data(), PL.job_list.size() * sizeof(int));
+  put(P.o_st, h_st.data(), h_st.size() * sizeof(PStamp));
+  put(P.o_dl, h_dl.data(), h_dl.size() * sizeof(int2));
+  put(P.o_off, h_off.data(), h_off.size() * sizeof(int));
+  put(P.o_items, items.data(), items.size() * sizeof(CItem));
+  P.njobs = n;
+  P.n_items = (int)items.size();
+  P.nblocks = PL.nblocks;
+  P.tile = tile;
+  return P;
+}
 
+// Enqueues a planned batch on `s`: plan copy (staging -> dev), counters
+// cleared by a kernel, then k_batch -- cooperative, or as a plain launch of
+// at most the co-resident block count (inside a captured graph).
+static void enqueue_plan(const BatchPlan& P, const char* stage, char* dev, hipStream_t s,
+                         unsigned* host_err, bool cooperative) {
+  PNT_HIP_CHECK(hipMemcpyAsync(dev, stage, (size_t)P.o_counters, hipMemcpyHostToDevice, s));
+  unsigned* counters = reinterpret_cast<unsigned*>(dev + P.o_counters);
+  launch_fill_u32(counters, P.n_counters, 0u, s);
   BArgs A;
-  A.jobs = reinterpret_cast<const BJob*>(B.dev + o_jobs);
-  A.groups = reinterpret_cast<const BGroup*>(B.dev + o_groups);
-  A.block_group = reinterpret_cast<const int*>(B.dev + o_bg);
-  A.job_list = reinterpret_cast<const int*>(B.dev + o_jl);
-  A.st = reinterpret_cast<const PStamp*>(B.dev + o_st);
-  A.dl = reinterpret_cast<const int2*>(B.dev + o_dl);
-  A.dl_off = reinterpret_cast<const int*>(B.dev + o_off);
-  A.items = reinterpret_cast<const CItem*>(B.dev + o_items);
-  A.n_items = (int)items.size();
+  A.jobs = reinterpret_cast<const BJob*>(dev + P.o_jobs);
+  A.groups = reinterpret_cast<const BGroup*>(dev + P.o_groups);
+  A.block_group = reinterpret_cast<const int*>(dev + P.o_bg);
+  A.job_list = reinterpret_cast<const int*>(dev + P.o_jl);
+  A.st = reinterpret_cast<const PStamp*>(dev + P.o_st);
+  A.dl = reinterpret_cast<const int2*>(dev + P.o_dl);
+  A.dl_off = reinterpret_cast<const int*>(dev + P.o_off);
+  A.items = reinterpret_cast<const CItem*>(dev + P.o_items);
+  A.n_items = P.n_items;
   A.item_next = counters;
   A.done = counters + 1;
-  A.bars = counters + 1 + n;
-  PNT_HIP_CHECK(hipHostGetDevicePointer((void**)&A.host_err, ctx->kernel_error, 0));
-  void* kargs[] = {&A};
-  PNT_HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_batch),
-                                           dim3(PL.nblocks), dim3(kBT), kargs, 0, s));
-  // host mirrors of the brush and canvas state
+  A.bars = counters + 1 + P.njobs;
+  A.host_err = host_err;
+  const void* fn = P.tile == 16 ? reinterpret_cast<const void*>(k_batch<16>)
+                                : reinterpret_cast<const void*>(k_batch<32>);
+  const unsigned threads = (unsigned)(P.tile * P.tile);
+  if (cooperative) {
+    void* kargs[] = {&A};
+    PNT_HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(P.nblocks), dim3(threads), kargs, 0, s));
+  } else if (P.tile == 16) {
+    k_batch<16><<<P.nblocks, threads, 0, s>>>(A);
+  } else {
+    k_batch<32><<<P.nblocks, threads, 0, s>>>(A);
+  }
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+// Host mirrors of brush and canvas state after a batch of these jobs.
+static void finish_jobs(const pnt_batch_job* jobs, int n) {
   for (int j = 0; j < n; ++j) {
     pnt_brush b = jobs[j].brush;
     pnt_canvas cv = jobs[j].canvas;
@@ -556,6 +581,276 @@ void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n) {
   }
 }
 
+void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
+                          const pnt_batch_config* cfg) {
+  hipStream_t s = ctx->stream;
+  if (*(volatile unsigned*)ctx->kernel_error)
+    throw Error(PNT_EKERNEL, "an earlier persistent stamp kernel aborted");
+  BatchState& B = *batch_state(ctx);
+  const BatchPlan P = plan_batch(ctx, jobs, n, cfg ? *cfg : B.cfg);
+  PNT_HIP_CHECK(hipEventSynchronize(B.uploaded));
+  if (P.total > B.stage_cap) {
+    if (B.stage) PNT_HIP_CHECK(hipHostFree(B.stage));
+    B.stage = nullptr;
+    B.stage_cap = std::max<int64_t>(P.total, 2 * B.stage_cap);
+    PNT_HIP_CHECK(hipHostMalloc((void**)&B.stage, (size_t)B.stage_cap));
+  }
+  if (P.total > B.dev_cap) {
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    if (B.dev) PNT_HIP_CHECK(hipFree(B.dev));
+    B.dev = nullptr;
+    B.dev_cap = std::max<int64_t>(P.total, 2 * B.dev_cap);
+    PNT_HIP_CHECK(hipMalloc((void**)&B.dev, (size_t)B.dev_cap));
+  }
+  std::memcpy(B.stage, P.blob.data(), P.blob.size());
+  unsigned* host_err = nullptr;
+  PNT_HIP_CHECK(hipHostGetDevicePointer((void**)&host_err, ctx->kernel_error, 0));
+  enqueue_plan(P, B.stage, B.dev, s, host_err, true);
+  PNT_HIP_CHECK(hipEventRecord(B.uploaded, s));
+  finish_jobs(jobs, n);
+}
+
+void set_batch_config(pnt_ctx ctx, const pnt_batch_config& c) { batch_state(ctx)->cfg = c; }
+pnt_batch_config get_batch_config(pnt_ctx ctx) { return batch_state(ctx)->cfg; }
+
+// ----------------------------------------------------------- autotuner --
+// Times candidate (tile, max group) configurations on the batch itself and
+// keeps the fastest for the context.  The state every candidate changes --
+// canvas planes, pickup maps, snapshots, dirty bytes, host mirrors -- is
+// saved first and restored after each candidate, so tuning leaves the jobs
+// as it found them.
+struct SavedJob {
+  float* canvas = nullptr;  // 11 planes
+  float* pick = nullptr;    // 7 planes
+  float* snap = nullptr;    // 7 planes
+  uint8_t* dirty = nullptr;
+  int64_t canvas_n = 0, pick_n = 0, snap_n = 0, dirty_n = 0;
+  float paintK[3], paintS[3];
+  uint64_t epoch = 0, dirty_epoch = 0;
+  const pnt_canvas_s* dirty_canvas = nullptr;
+};
+
+static void save_or_restore(const pnt_batch_job* jobs, int n, std::vector<SavedJob>& sv,
+                            bool save, hipStream_t s) {
+  auto copy = [&](void* dst, const void* src, int64_t bytes) {
+    if (bytes > 0) PNT_HIP_CHECK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, s));
+  };
+  for (int j = 0; j < n; ++j) {
+    pnt_brush b = jobs[j].brush;
+    pnt_canvas cv = jobs[j].canvas;
+    SavedJob& S = sv[j];
+    if (save) {
+      S.canvas_n = 11 * cv->stride;
+      S.pick_n = 7 * b->mstride;
+      S.snap_n = b->snap ? 7 * b->snap_stride : 0;
+      S.dirty_n = b->dirty ? b->dirty_cap : 0;
+      S.canvas = dev_alloc(S.canvas_n);
+      S.pick = dev_alloc(S.pick_n);
+      if (S.snap_n) S.snap = dev_alloc(S.snap_n);
+      if (S.dirty_n) PNT_HIP_CHECK(hipMalloc((void**)&S.dirty, (size_t)S.dirty_n));
+      copy(S.canvas, cv->base, S.canvas_n * 4);
+      copy(S.pick, b->pick, S.pick_n * 4);
+      copy(S.snap, b->snap, S.snap_n * 4);
+      copy(S.dirty, b->dirty, S.dirty_n);
+      for (int c = 0; c < 3; ++c) {
+        S.paintK[c] = b->paintK[c];
+        S.paintS[c] = b->paintS[c];
+      }
+      S.epoch = cv->epoch;
+      S.dirty_epoch = b->dirty_epoch;
+      S.dirty_canvas = b->dirty_canvas;
+    } else {
+      copy(cv->base, S.canvas, S.canvas_n * 4);
+      copy(b->pick, S.pick, S.pick_n * 4);
+      copy(b->snap, S.snap, S.snap_n * 4);
+      copy(b->dirty, S.dirty, S.dirty_n);
+      for (int c = 0; c < 3; ++c) {
+        b->paintK[c] = S.paintK[c];
+        b->paintS[c] = S.paintS[c];
+      }
+      cv->epoch = S.epoch;
+      b->dirty_epoch = S.dirty_epoch;
+      b->dirty_canvas = S.dirty_canvas;
+    }
+  }
+}
+
+void tune_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n, pnt_batch_config* best,
+                double* best_ms) {
+  hipStream_t s = ctx->stream;
+  BatchState& B = *batch_state(ctx);
+  // the snapshot and dirty bytes the runs will use exist before the save
+  for (int j = 0; j < n; ++j) brush_prepare_snapshot(jobs[j].brush, jobs[j].canvas, s);
+  std::vector<SavedJob> sv(n);
+  save_or_restore(jobs, n, sv, true, s);
+  const pnt_batch_config cands[] = {{32, 16, {0, 0}}, {32, 4, {0, 0}}, {32, 1, {0, 0}},
+                                    {16, 64, {0, 0}}, {16, 16, {0, 0}}, {16, 4, {0, 0}}};
+  hipEvent_t e0, e1;
+  PNT_HIP_CHECK(hipEventCreate(&e0));
+  PNT_HIP_CHECK(hipEventCreate(&e1));
+  double best_t = 1e30;
+  pnt_batch_config best_c = B.cfg;
+  try {
+    // untimed first run: grows the plan buffers, warms the code and caches
+    run_batch_one_launch(ctx, jobs, n, &cands[0]);
+    save_or_restore(jobs, n, sv, false, s);
+    for (const pnt_batch_config& c : cands) {
+      PNT_HIP_CHECK(hipEventRecord(e0, s));
+      run_batch_one_launch(ctx, jobs, n, &c);
+      PNT_HIP_CHECK(hipEventRecord(e1, s));
+      PNT_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      PNT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (*(volatile unsigned*)ctx->kernel_error)
+        throw Error(PNT_EKERNEL, "a batch kernel aborted while tuning");
+      if (ms < best_t) {
+        best_t = ms;
+        best_c = c;
+      }
+      save_or_restore(jobs, n, sv, false, s);
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    for (auto& S : sv) {
+      dev_free(S.canvas);
+      dev_free(S.pick);
+      dev_free(S.snap);
+      dev_free(S.dirty);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  PNT_HIP_CHECK(hipStreamSynchronize(s));
+  for (auto& S : sv) {
+    dev_free(S.canvas);
+    dev_free(S.pick);
+    dev_free(S.snap);
+    dev_free(S.dirty);
+  }
+  PNT_HIP_CHECK(hipEventDestroy(e0));
+  PNT_HIP_CHECK(hipEventDestroy(e1));
+  B.cfg = best_c;
+  if (best) *best = best_c;
+  if (best_ms) *best_ms = best_t;
+}
+
+// ------------------------------------------------------- captured graph --
+// What a captured plan baked in about a brush: buffers and parameters.
+struct BrushKey {
+  const float* fp;
+  const float* pick;
+  const float* snap;
+  const uint8_t* dirty;
+  int64_t snap_stride;
+  int size_map;
+  bool use_snap;
+  float pickup_rate, dep_rate, cap;
+  bool operator==(const BrushKey& o) const {
+    return fp == o.fp && pick == o.pick && snap == o.snap && dirty == o.dirty &&
+           snap_stride == o.snap_stride && size_map == o.size_map && use_snap == o.use_snap &&
+           pickup_rate == o.pickup_rate && dep_rate == o.dep_rate && cap == o.cap;
+  }
+};
+
+static BrushKey brush_key(const pnt_brush_s* b) {
+  return BrushKey{b->fp,          b->pick,     b->snap,     b->dirty,        b->snap_stride,
+                  b->size_map,    b->use_snap, b->pickup_rate, b->dep_rate, b->cap};
+}
+
+struct BatchGraph {
+  std::vector<BrushKey> brushes;
+  pnt_ctx ctx = nullptr;
+  bool retained = false;
+  std::vector<pnt_batch_job> jobs;
+  std::vector<std::vector<pnt_stamp>> stamps;
+  std::vector<uint64_t> epochs;  // canvas epochs a launch expects
+  char* stage = nullptr;
+  char* dev = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+// Plans the batch once and captures plan copy + counter clear + k_batch into
+// a graph; each launch replays the same jobs on the canvases' current state.
+// The captured k_batch is a plain launch sized to the co-resident block
+// count (every wait in it is bounded).
+BatchGraph* capture_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n) {
+  for (int j = 0; j < n; ++j)
+    PNT_REQUIRE(jobs[j].n_stamps == 0 || (jobs[j].stamps[0].flags & PNT_STAMP_DIP),
+                "a captured batch needs every stamp list to start with a dip "
+                "(its paint is fixed at capture)");
+  BatchState& B = *batch_state(ctx);
+  auto* G = new BatchGraph;
+  G->ctx = ctx;
+  try {
+    G->jobs.assign(jobs, jobs + n);
+    G->stamps.resize(n);
+    for (int j = 0; j < n; ++j) {  // own copies of the stamp lists
+      G->stamps[j].assign(jobs[j].stamps, jobs[j].stamps + jobs[j].n_stamps);
+      G->jobs[j].stamps = G->stamps[j].data();
+    }
+    hipStream_t s = ctx->stream;
+    const BatchPlan P = plan_batch(ctx, G->jobs.data(), n, B.cfg);
+    PNT_HIP_CHECK(hipStreamSynchronize(s));  // snapshot preparation done before capture
+    PNT_HIP_CHECK(hipHostMalloc((void**)&G->stage, (size_t)P.total));
+    PNT_HIP_CHECK(hipMalloc((void**)&G->dev, (size_t)P.total));
+    std::memcpy(G->stage, P.blob.data(), P.blob.size());
+    unsigned* host_err = nullptr;
+    PNT_HIP_CHECK(hipHostGetDevicePointer((void**)&host_err, ctx->kernel_error, 0));
+    PNT_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_plan(P, G->stage, G->dev, s, host_err, false);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    PNT_HIP_CHECK(hipStreamEndCapture(s, &G->graph));
+    PNT_HIP_CHECK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+    for (int j = 0; j < n; ++j) {
+      G->epochs.push_back(G->jobs[j].canvas->epoch);
+      G->brushes.push_back(brush_key(G->jobs[j].brush));
+    }
+  } catch (...) {
+    release_batch_graph(G);
+    throw;
+  }
+  ctx_retain(ctx);
+  G->retained = true;
+  return G;
+}
+
+void launch_batch_graph(BatchGraph* G) {
+  pnt_ctx ctx = G->ctx;
+  if (*(volatile unsigned*)ctx->kernel_error)
+    throw Error(PNT_EKERNEL, "an earlier persistent stamp kernel aborted");
+  for (size_t j = 0; j < G->jobs.size(); ++j) {
+    const pnt_batch_job& J = G->jobs[j];
+    PNT_REQUIRE(J.canvas->epoch == G->epochs[j] && brush_key(J.brush) == G->brushes[j] &&
+                    (!J.brush->use_snap || (J.brush->dirty_canvas == J.canvas &&
+                                            J.brush->dirty_epoch == J.canvas->epoch)),
+                "a canvas or brush of the captured batch changed outside it; capture again");
+  }
+  PNT_HIP_CHECK(hipGraphLaunch(G->exec, ctx->stream));
+  finish_jobs(G->jobs.data(), (int)G->jobs.size());
+  for (size_t j = 0; j < G->jobs.size(); ++j) G->epochs[j] = G->jobs[j].canvas->epoch;
+}
+
+void release_batch_graph(BatchGraph*& G) {
+  if (!G) return;
+  if (G->ctx) (void)hipStreamSynchronize(G->ctx->stream);
+  if (G->exec) (void)hipGraphExecDestroy(G->exec);
+  if (G->graph) (void)hipGraphDestroy(G->graph);
+  if (G->stage) (void)hipHostFree(G->stage);
+  if (G->dev) (void)hipFree(G->dev);
+  pnt_ctx ctx = G->retained ? G->ctx : nullptr;
+  delete G;
+  G = nullptr;
+  if (ctx) ctx_release(ctx);
+}
+
 // One BatchState per context (the plan buffers of its batches).
 static std::vector<std::pair<pnt_ctx, BatchState*>> g_batch_states;
 
@@ -570,8 +865,11 @@ static BatchState* batch_state(pnt_ctx ctx) {
   if (!coop) throw Error(PNT_ESTATE, "device has no cooperative launch");
   PNT_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void*>(k_batch), kBT, 0));
-  B->max_blocks = ncu * std::max(per_cu, 1);
+      &per_cu, reinterpret_cast<const void*>(k_batch<32>), 1024, 0));
+  B->max_blocks32 = ncu * std::max(per_cu, 1);
+  PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(k_batch<16>), 256, 0));
+  B->max_blocks16 = ncu * std::max(per_cu, 1);
   g_batch_states.push_back({ctx, B});
   return B;
 }

@@ -632,6 +632,7 @@ int pnt_grad_create(pnt_ctx ctx, int rows, int cols, pnt_brush brush, const pnt_
       delete g;
       throw Error(st_create, msg);
     }
+    ctx_retain(ctx);
     *out = g;
   });
 }
@@ -647,7 +648,9 @@ int pnt_grad_destroy(pnt_grad g) {
     (void)hipFree(g->gP);
     (void)hipFree(g->gB);
     (void)hipFree(g->scratch);
+    pnt_ctx ctx = g->ctx;
     delete g;
+    ctx_release(ctx);
   });
 }
 

@@ -4,6 +4,7 @@
 
 #include <stdint.h>
 
+#include <atomic>
 #include <vector>
 
 #include "internal.hpp"
@@ -18,9 +19,15 @@ inline int64_t round_up(int64_t n, int64_t a) { return (n + a - 1) / a * a; }
 float* dev_alloc(int64_t nfloats);
 void dev_free(void* p);
 struct PersistentState;
+struct ChainState;
 }  // namespace pnt
 
 struct pnt_ctx_s {
+  // One reference for the caller's handle plus one per object made on the
+  // context (layers, canvases, brushes, gradient runners, captured batches);
+  // the stream and buffers go with the last one, so objects may be destroyed
+  // after pnt_ctx_destroy in any order.
+  std::atomic<int> refs{1};
   int device = 0;
   hipStream_t stream = nullptr;
   // Pinned host word that persistent kernels raise when a bounded wait
@@ -28,6 +35,11 @@ struct pnt_ctx_s {
   unsigned* kernel_error = nullptr;
   void use() const { PNT_HIP_CHECK(hipSetDevice(device)); }
 };
+
+namespace pnt {
+void ctx_retain(pnt_ctx ctx);
+void ctx_release(pnt_ctx ctx);
+}  // namespace pnt
 
 struct pnt_canvas_s;
 
@@ -103,6 +115,7 @@ struct pnt_brush_s {
   const pnt_canvas_s* dirty_canvas = nullptr;
   uint64_t dirty_epoch = 0;
   pnt::PersistentState* pers = nullptr;
+  pnt::ChainState* chain = nullptr;  // buffers of PNT_EXEC_CHAINED runs
 
   pnt::Layer7 pick_planes() const {
     pnt::Layer7 L;

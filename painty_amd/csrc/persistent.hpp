@@ -28,11 +28,24 @@ struct PlanDev {
 PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 
 void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
+void run_chained(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
+struct ChainState;
+void chain_release(ChainState*& c);
 void run_persistent(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 void persistent_release(PersistentState*& p);
 
-// pnt_batch_run(..., PNT_BATCH_ONE_LAUNCH) (batch.hip)
-void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n);
+// pnt_batch_run(..., PNT_BATCH_ONE_LAUNCH), pnt_batch_tune and captured
+// batches (batch.hip).  cfg NULL: the context's configuration.
+void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
+                          const pnt_batch_config* cfg);
+void tune_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n, pnt_batch_config* best,
+                double* best_ms);
+void set_batch_config(pnt_ctx ctx, const pnt_batch_config& c);
+pnt_batch_config get_batch_config(pnt_ctx ctx);
+struct BatchGraph;
+BatchGraph* capture_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n);
+void launch_batch_graph(BatchGraph* g);
+void release_batch_graph(BatchGraph*& g);
 void batch_release(pnt_ctx ctx);
 
 }  // namespace pnt

@@ -357,10 +357,7 @@ def stamps_to_c(stamps):
     return arr
 
 
-def batch_run(jobs, mode: int = _lib.BATCH_ONE_LAUNCH, ctx: Context | None = None) -> None:
-    """pnt_batch_run over [(canvas, brush, stamps, out_canvas_or_None), ...]:
-    the composite of job k lands in out_canvas's R0 planes."""
-    ctx = ctx or default_context()
+def _batch_jobs(jobs):
     n = len(jobs)
     arr = (_lib.BatchJob * n)()
     keep = []
@@ -375,7 +372,63 @@ def batch_run(jobs, mode: int = _lib.BATCH_ONE_LAUNCH, ctx: Context | None = Non
             _, R, _ = out.device_planes()
             for c in range(3):
                 arr[k].out[c] = R[c]
-    call("pnt_batch_run", ctx.h, arr, n, int(mode))
+    return arr, keep
+
+
+def batch_run(jobs, mode: int = _lib.BATCH_ONE_LAUNCH, ctx: Context | None = None) -> None:
+    """pnt_batch_run over [(canvas, brush, stamps, out_canvas_or_None), ...]:
+    the composite of job k lands in out_canvas's R0 planes."""
+    ctx = ctx or default_context()
+    arr, _keep = _batch_jobs(jobs)
+    call("pnt_batch_run", ctx.h, arr, len(jobs), int(mode))
+
+
+def batch_set_config(tile: int, max_group: int, ctx: Context | None = None) -> None:
+    ctx = ctx or default_context()
+    c = _lib.BatchConfig(tile, max_group)
+    call("pnt_batch_set_config", ctx.h, C.byref(c))
+
+
+def batch_get_config(ctx: Context | None = None):
+    ctx = ctx or default_context()
+    c = _lib.BatchConfig()
+    call("pnt_batch_get_config", ctx.h, C.byref(c))
+    return c.tile, c.max_group
+
+
+def batch_tune(jobs, ctx: Context | None = None):
+    """pnt_batch_tune: returns ((tile, max_group), ms) of the fastest
+    configuration, which the context keeps; the jobs' state is unchanged."""
+    ctx = ctx or default_context()
+    arr, _keep = _batch_jobs(jobs)
+    c, ms = _lib.BatchConfig(), C.c_double()
+    call("pnt_batch_tune", ctx.h, arr, len(jobs), C.byref(c), C.byref(ms))
+    return (c.tile, c.max_group), ms.value
+
+
+class BatchGraph:
+    """pnt_batch_capture: the batch captured once as a HIP graph;
+    ``launch()`` replays it on the jobs' current state."""
+
+    def __init__(self, jobs, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        arr, _keep = _batch_jobs(jobs)
+        h = C.c_void_p()
+        call("pnt_batch_capture", self.ctx.h, arr, len(jobs), C.byref(h))
+        self.h = h
+
+    def launch(self) -> None:
+        call("pnt_batch_graph_launch", self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            call("pnt_batch_graph_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.load().pnt_batch_graph_destroy(self.h)
+            self.h = None
 
 
 class PaintMixer:

@@ -112,3 +112,131 @@ def test_batch_rejects_shared_brush(hip):
     cv2 = hip.Canvas(40, 40)
     with pytest.raises(hip.InvalidArgument):
         hip.batch_run([(cv1, br, wl.stamps, None), (cv2, br, wl.stamps, None)])
+
+
+def _big_small(seed0=40):
+    jobs = []
+    for k in range(6):
+        big = k < 2
+        jobs.append(W.small(300 if big else 60, 280 if big else 70, 60 if big else 8,
+                            40 if big else 6, seed0 + k, pickup=True, noisy=True, reach=0.3))
+    return jobs
+
+
+@pytest.fixture
+def restore_config(hip):
+    saved = hip.batch_get_config()
+    yield
+    hip.batch_set_config(*saved)
+
+
+@pytest.mark.parametrize("tile,max_group", [(16, 64), (16, 4), (16, 1), (32, 1), (32, 4)])
+def test_batch_configs_match_oracle(hip, oracle, restore_config, tile, max_group):
+    """Every launch shape the autotuner may pick (16- or 32-pixel window
+    tiles, one to 64 workgroups per job) gives the oracle's result."""
+    hip.batch_set_config(tile, max_group)
+    assert hip.batch_get_config() == (tile, max_group)
+    jobs = _big_small()
+    gpu = []
+    for wl in jobs:
+        cv, br, out = _setup(hip, wl)
+        gpu.append((cv, br, wl.stamps, out))
+    hip.batch_run(gpu, mode=0)
+    for k, (wl, (cv, br, _, out)) in enumerate(zip(jobs, gpu)):
+        ocv, ob = run_oracle(wl)
+        tag = "cfg%d.%d/job%d" % (tile, max_group, k)
+        compare_canvas(tag, cv, ocv, pickup=(br, ob))
+        assert_close(tag + ".out", out.getR0(), ocv.compose())
+
+
+def test_batch_config_rejects_bad_shapes(hip, restore_config):
+    for tile, g in [(8, 4), (64, 4), (32, 0), (16, 257)]:
+        with pytest.raises(hip.InvalidArgument):
+            hip.batch_set_config(tile, g)
+
+
+def test_batch_tune_keeps_state_and_config(hip, oracle, restore_config):
+    """pnt_batch_tune runs the batch many times but leaves every canvas and
+    brush as it found them; the eager run afterwards uses the tuned shape and
+    still equals the oracle."""
+    jobs = _mixed_jobs(3)[:8]
+    gpu = []
+    for wl, snap in jobs:
+        cv, br, out = _setup(hip, wl, snap)
+        gpu.append((cv, br, wl.stamps, out))
+    (tile, g), ms = hip.batch_tune(gpu)
+    assert tile in (16, 32) and 1 <= g <= 256 and ms > 0
+    assert hip.batch_get_config() == (tile, g)
+    hip.batch_run(gpu, mode=0)
+    for k, ((wl, snap), (cv, br, _, out)) in enumerate(zip(jobs, gpu)):
+        ocv, ob = run_oracle(wl, use_snapshot=snap)
+        compare_canvas("tuned/job%d" % k, cv, ocv, pickup=(br, ob) if ob is not None else None)
+        assert_close("tuned/job%d.out" % k, out.getR0(), ocv.compose())
+
+
+def _layers(cv):
+    lay = cv.getPaintLayer()
+    return [lay.getK_buffer(), lay.getS_buffer(), lay.getV_buffer(), cv.getR0()]
+
+
+def test_batch_graph_replays_equal_eager_and_oracle(hip, oracle):
+    """A captured batch launched twice equals the eager batch run twice (to
+    the bit: same kernel, same plan) and the oracle running each stamp list
+    twice."""
+    jobs = _big_small(70)
+    eager, graph = [], []
+    for wl in jobs:
+        cv, br, out = _setup(hip, wl)
+        eager.append((cv, br, wl.stamps, out))
+        cv, br, out = _setup(hip, wl)
+        graph.append((cv, br, wl.stamps, out))
+    g = hip.BatchGraph(graph)
+    for _ in range(2):
+        hip.batch_run(eager, mode=0)
+        g.launch()
+    g.close()
+    for k, (wl, e, q) in enumerate(zip(jobs, eager, graph)):
+        for a, b in zip(_layers(e[0]) + [e[3].getR0()] + list(e[1].getPickupMap()),
+                        _layers(q[0]) + [q[3].getR0()] + list(q[1].getPickupMap())):
+            assert np.array_equal(a, b), "graph/job%d" % k
+        ocv, ob = run_oracle(wl, np.concatenate([wl.stamps, wl.stamps]))
+        compare_canvas("graph/job%d" % k, q[0], ocv, pickup=(q[1], ob))
+        assert_close("graph/job%d.out" % k, q[3].getR0(), ocv.compose())
+
+
+def test_batch_graph_refuses_stale_state(hip):
+    wl = W.small(50, 60, 12, 5, 81)
+    cv, br, out = _setup(hip, wl)
+    g = hip.BatchGraph([(cv, br, wl.stamps, out)])
+    g.launch()
+    br.imprintStamps(wl.stamps[:3], cv, 0)  # an ordinary call changes the canvas
+    with pytest.raises(hip.InvalidArgument):
+        g.launch()
+    g.close()
+    st = wl.stamps.copy()
+    st["flags"][0] = 0  # no leading dip: paint would depend on the brush at launch time
+    with pytest.raises(hip.InvalidArgument):
+        hip.BatchGraph([(cv, br, st, out)])
+
+
+def test_objects_outlive_their_context(hip, oracle):
+    """pnt_ctx_destroy while canvases, brushes and a captured batch still
+    use the context: they keep working, and destroying them afterwards (in
+    any order) releases the context last."""
+    wl = W.small(48, 52, 14, 5, 91, pickup=True)
+    ctx = hip.Context(0)
+    cv = hip.Canvas(wl.rows, wl.cols, ctx)
+    cv.getPaintLayer().setBuffers(wl.wet.K, wl.wet.S, wl.wet.V)
+    cv.setR0(wl.R0)
+    br = hip.FootprintBrush(wl.radii[0], wl.cores[0], ctx)
+    br.setPickupRate(wl.pickup_rate)
+    br.setDepositionRate(wl.deposition_rate)
+    g = hip.BatchGraph([(cv, br, wl.stamps[:7], None)], ctx)
+    ctx.close()  # the caller's reference goes first
+    g.launch()
+    br.imprintStamps(wl.stamps[7:], cv, 2)
+    ocv, ob = run_oracle(wl, bake_dry=False)
+    compare_canvas("outlive", cv, ocv, pickup=(br, ob))
+    del cv
+    g.close()
+    del br

@@ -1,5 +1,6 @@
 """The planned stamp paths -- one persistent launch per list
-(PNT_EXEC_PERSISTENT) and one fused launch per stamp (PNT_EXEC_FUSED) --
+(PNT_EXEC_PERSISTENT), one fused launch per stamp (PNT_EXEC_FUSED) and one
+counter-chained launch per list (PNT_EXEC_CHAINED) --
 against the FP64 oracle, and the bookkeeping that lets them skip no-op
 snapshot copies: every way the canvas or the snapshot can change behind
 their back must make them copy again.  Tolerance 1e-5 relative per
@@ -17,9 +18,10 @@ pytestmark = pytest.mark.gpu
 PERSIST = 1
 PLAIN = 0
 FUSED = 2
+CHAINED = 3
 
 
-@pytest.fixture(params=[PERSIST, FUSED], ids=["persistent", "fused"])
+@pytest.fixture(params=[PERSIST, FUSED, CHAINED], ids=["persistent", "fused", "chained"])
 def lmode(request):
     return request.param
 
