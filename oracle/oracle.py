@@ -30,6 +30,13 @@ class Stamp(C.Structure):
                 ("flags", C.c_uint32)]
 
 
+class Clock(C.Structure):
+    """orc_clock: time map (microseconds), time point, drying time (ms), R0, h."""
+
+    _fields_ = [("tmap", C.POINTER(C.c_int64)), ("now_us", C.c_int64), ("dry_ms", C.c_int64),
+                ("R0", C.POINTER(C.c_double)), ("h", C.POINTER(C.c_double))]
+
+
 DIP = 1
 DRY_AFTER = 2
 
@@ -75,6 +82,9 @@ def lib():
                                         C.c_double, C.c_double, C.c_double]
         L.orc_run_stamps.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
                                      C.POINTER(Stamp), C.c_int64]
+        L.orc_brush_imprint_timed.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp,
+                                              C.c_double, C.c_double, C.c_double,
+                                              C.POINTER(Clock)]
         L.orc_mix_single.restype = C.c_int
         L.orc_mix_single.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.orc_mixed.argtypes = [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp]
@@ -220,6 +230,17 @@ class Brush:
                                 _p(canvas.S), _p(canvas.V), float(cx), float(cy),
                                 float(theta))
 
+    def imprint_timed(self, cx: float, cy: float, theta: float, canvas: "Canvas",
+                      now_us: int, dry_ms: int) -> None:
+        """imprint with Canvas::checkDry on every visited pixel; canvas.tmap
+        (int64 microseconds) is the time map."""
+        assert canvas.tmap.dtype == np.int64 and canvas.tmap.flags.c_contiguous
+        clk = Clock(canvas.tmap.ctypes.data_as(C.POINTER(C.c_int64)), int(now_us), int(dry_ms),
+                    _p(canvas.R0), _p(canvas.h))
+        lib().orc_brush_imprint_timed(self.h, canvas.rows, canvas.cols, _p(canvas.K),
+                                      _p(canvas.S), _p(canvas.V), float(cx), float(cy),
+                                      float(theta), C.byref(clk))
+
     def run(self, canvas: "Canvas", stamps) -> None:
         arr = stamps_to_c(stamps)
         lib().orc_run_stamps(self.h, canvas.rows, canvas.cols, _p(canvas.K), _p(canvas.S),
@@ -250,6 +271,7 @@ class Canvas:
         self.V = np.zeros((rows, cols))
         self.R0 = np.ones((rows, cols, 3))
         self.h = np.zeros((rows, cols))
+        self.tmap = np.zeros((rows, cols), np.int64)  # time map, microseconds
 
     @classmethod
     def from_arrays(cls, K, S, V, R0, h=None) -> "Canvas":

@@ -392,8 +392,40 @@ static void deposit_paint(orc_brush* b, int64_t ic, int64_t im, double* K,
 }
 
 /* FootprintBrush.hxx:73-143 */
-void orc_brush_imprint(orc_brush* b, int rows, int cols, double* K, double* S,
-                       double* V, double cx, double cy, double theta) {
+/* Canvas::checkDry (Canvas.hxx:123-155) for pixel (x, y) at clock time
+ * now_us; the time map holds microseconds, and the elapsed time is truncated
+ * to whole milliseconds as duration_cast<milliseconds> does. */
+static void check_dry(const orc_clock* c, int cols, int x, int y, double* K,
+                      double* S, double* V) {
+  const int64_t ic = (int64_t)y * cols + x;
+  const double v = V[ic];
+  if (c->dry_ms > 0 && v > 0.001) {
+    const int64_t dur = (c->now_us - c->tmap[ic]) / 1000;
+    if (dur >= c->dry_ms) {
+      c->h[ic] += v;
+      orc_compute_reflectance(K + 3 * ic, S + 3 * ic, c->R0 + 3 * ic, v,
+                              c->R0 + 3 * ic);
+      V[ic] = 0.0;
+      K[3 * ic + 0] = K[3 * ic + 1] = K[3 * ic + 2] = 0.0;
+      S[3 * ic + 0] = S[3 * ic + 1] = S[3 * ic + 2] = 0.0;
+    } else {
+      /* integer quotient, as in the reference (Canvas.hxx:141) */
+      const double rate = (double)(dur / c->dry_ms);
+      if (rate > 0.01) {
+        const double vl = rate * v;
+        c->h[ic] += vl;
+        orc_compute_reflectance(K + 3 * ic, S + 3 * ic, c->R0 + 3 * ic, vl,
+                                c->R0 + 3 * ic);
+        V[ic] = v - vl;
+      }
+    }
+  }
+  c->tmap[ic] = c->now_us;
+}
+
+static void imprint_impl(orc_brush* b, int rows, int cols, double* K, double* S,
+                         double* V, double cx, double cy, double theta,
+                         const orc_clock* clk) {
   const int h = b->size_map; /* footprint rows (padded to sizeMap) */
   const int hr = (h - 1) / 2;
   const int wr = hr;
@@ -416,12 +448,24 @@ void orc_brush_imprint(orc_brush* b, int rows, int cols, double* K, double* S,
       const int my = (int)round(rr + hr);
       if (y < 0 || x < 0 || x >= cols || y >= rows) continue;
       if (my < 0 || mx < 0 || mx >= b->size_map || my >= b->size_map) continue;
+      if (clk) check_dry(clk, cols, x, y, K, S, V); /* FootprintBrush.hxx:115 */
       const int64_t ic = (int64_t)y * cols + x;
       const int64_t im = (int64_t)my * b->size_map + mx;
       pickup_paint(b, ic, im, srcK, srcS, srcV);
       deposit_paint(b, ic, im, K, S, V);
     }
   }
+}
+
+void orc_brush_imprint(orc_brush* b, int rows, int cols, double* K, double* S,
+                       double* V, double cx, double cy, double theta) {
+  imprint_impl(b, rows, cols, K, S, V, cx, cy, theta, NULL);
+}
+
+void orc_brush_imprint_timed(orc_brush* b, int rows, int cols, double* K,
+                             double* S, double* V, double cx, double cy,
+                             double theta, const orc_clock* clk) {
+  imprint_impl(b, rows, cols, K, S, V, cx, cy, theta, clk);
 }
 
 void orc_run_stamps(orc_brush* b, int rows, int cols, double* K, double* S,

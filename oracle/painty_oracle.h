@@ -94,6 +94,22 @@ void orc_brush_get_pickup(const orc_brush* b, double* K, double* S, double* V);
 void orc_brush_imprint(orc_brush* b, int rows, int cols, double* K, double* S,
                        double* V, double cx, double cy, double theta);
 
+/* The same imprint with Canvas::checkDry (Canvas.hxx:123-155) run on every
+ * visited pixel before its pickup and deposit, as FootprintBrush.hxx:115
+ * does.  The clock: tmap (rows*cols time points in microseconds), the
+ * imprint's time point now_us, the drying time in milliseconds, and the
+ * canvas R0 (AoS) and h the drying writes. */
+typedef struct {
+  int64_t* tmap;
+  int64_t now_us;
+  int64_t dry_ms;
+  double* R0;
+  double* h;
+} orc_clock;
+void orc_brush_imprint_timed(orc_brush* b, int rows, int cols, double* K,
+                             double* S, double* V, double cx, double cy,
+                             double theta, const orc_clock* clk);
+
 /* A stamp list driver (the GUI calls dip / imprint / dryCanvas in sequence,
  * apps/painty_gui/DigitalCanvas.cxx:49-123). */
 typedef struct {

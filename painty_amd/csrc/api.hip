@@ -3,7 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
-#include <unordered_map>
+#include <climits>
 #include <utility>
 #include <vector>
 
@@ -213,58 +213,127 @@ StampArgs stamp_args(pnt_brush b, pnt_canvas cv, double cx, double cy, double th
 }
 
 // Visits of a stamp that share a canvas pixel on column/row 0, plus their
-// map-cell partners, in reference (row-major) order, with dependency levels.
+// map-cell partners, in reference (row-major) order, with dependency levels:
+// a visit's level is one more than the highest level of the earlier visits
+// with its map cell or its canvas pixel.
+//
+// Canvas column 0 receives the two window columns k0, k0 + 1 with
+// (int)(k + cx) == 0.  A visit's map-cell partner is a 4-neighbour; for a
+// visit on those columns the partners off them can only be (k0 - 1, row) and
+// (k0 + 2, row), so the visits come from the band of columns k0-1 .. k0+2
+// (rows likewise).  The levels of cells and pixels are kept in flat
+// per-thread arrays (pixels relative to the window's canvas box), cleared
+// entry by entry after use.
 void plan_shared_visits(const StampArgs& a, std::vector<int2>& list, std::vector<int>& off) {
   list.clear();
   off.clear();
   if (!a.dupx && !a.dupy) return;
   const StampGeom& g = a.g;
   const int hr = g.hr;
-  std::vector<std::pair<int, int>> rc;  // (row, col)
-  auto add_with_partners = [&](int col, int row) {
+  const int side = 2 * hr + 1;
+  struct Visit {
+    int row, col, m;
     int64_t ic;
-    const int m = stamp_visit(g, col, row, &ic);
-    if (m < 0) return;
-    rc.emplace_back(row, col);
-    const int nb[4][2] = {{col - 1, row}, {col + 1, row}, {col, row - 1}, {col, row + 1}};
-    for (auto& q : nb) {
-      int64_t jc;
-      if (stamp_visit(g, q[0], q[1], &jc) == m) rc.emplace_back(q[1], q[0]);
-    }
   };
-  for (int k = -hr; k <= hr; ++k) {
-    if (a.dupx && (int)(k + g.cx) == 0)
-      for (int row = -hr; row <= hr; ++row) add_with_partners(k, row);
-    if (a.dupy && (int)(k + g.cy) == 0)
-      for (int col = -hr; col <= hr; ++col) add_with_partners(col, k);
+  thread_local std::vector<Visit> vis, vis2;
+  thread_local std::vector<int> lvl_cell, lvl_px, lvl, band;
+  thread_local std::vector<int64_t> band_ic;
+  vis.clear();
+  // first window offset k in [-hr, hr - 1] with (int)(k + c) == (int)(k + 1 + c) == 0
+  auto first_dup = [&](double c) {
+    const double f = std::floor(-c);
+    for (int d = -1; d <= 1; ++d) {
+      const double kd = f + (double)d;
+      if (kd < (double)-hr || kd > (double)(hr - 1)) continue;
+      const int k = (int)kd;
+      if ((int)(k + c) == 0 && (int)(k + 1 + c) == 0) return k;
+    }
+    return INT32_MIN;
+  };
+  // map cells of the band: 4 lines of `side` entries (line j = k0 - 1 + j)
+  band.resize((size_t)4 * side);
+  band_ic.resize((size_t)4 * side);
+  auto fill_band = [&](int k0, bool columns) {
+    for (int j = 0; j < 4; ++j)
+      for (int t = 0; t < side; ++t) {
+        const int line = k0 - 1 + j, along = t - hr;
+        int64_t ic = 0;
+        band[(size_t)j * side + t] = columns ? stamp_visit(g, line, along, &ic)
+                                             : stamp_visit(g, along, line, &ic);
+        band_ic[(size_t)j * side + t] = ic;
+      }
+  };
+  auto keep = [&](int j, int t) {  // band entry j, t is on the dup lines or a partner
+    const int m = band[(size_t)j * side + t];
+    if (m < 0) return false;
+    if (j == 1 || j == 2) return true;
+    return m == band[(size_t)(j == 0 ? 1 : 2) * side + t];
+  };
+  if (a.dupx) {
+    const int k0 = first_dup(g.cx);
+    fill_band(k0, true);
+    for (int t = 0; t < side; ++t)  // row-major: rows outer, band columns inner
+      for (int j = 0; j < 4; ++j)
+        if (keep(j, t))
+          vis.push_back(Visit{t - hr, k0 - 1 + j, band[(size_t)j * side + t],
+                              band_ic[(size_t)j * side + t]});
   }
-  std::sort(rc.begin(), rc.end());
-  rc.erase(std::unique(rc.begin(), rc.end()), rc.end());
-  std::unordered_map<int, int> lvl_map;
-  std::unordered_map<int64_t, int> lvl_px;
-  std::vector<std::pair<int, int2>> lv;
+  if (a.dupy) {
+    const int k0 = first_dup(g.cy);
+    fill_band(k0, false);
+    vis2.clear();
+    for (int j = 0; j < 4; ++j)
+      for (int t = 0; t < side; ++t)
+        if (keep(j, t))
+          vis2.push_back(Visit{k0 - 1 + j, t - hr, band[(size_t)j * side + t],
+                               band_ic[(size_t)j * side + t]});
+    if (vis.empty()) {
+      vis.swap(vis2);
+    } else {  // both axes: merge the two row-major lists, dropping repeats
+      const size_t n1 = vis.size();
+      vis.insert(vis.end(), vis2.begin(), vis2.end());
+      auto lt = [](const Visit& x, const Visit& y) {
+        return x.row < y.row || (x.row == y.row && x.col < y.col);
+      };
+      std::inplace_merge(vis.begin(), vis.begin() + (std::ptrdiff_t)n1, vis.end(), lt);
+      vis.erase(std::unique(vis.begin(), vis.end(),
+                            [](const Visit& x, const Visit& y) {
+                              return x.row == y.row && x.col == y.col;
+                            }),
+                vis.end());
+    }
+  }
+  // levels; canvas pixels of window offsets lie in [x_lo, x_lo + span)^2
+  const int x_lo = (int)(g.cx - hr) - 1, y_lo = (int)(g.cy - hr) - 1;
+  const int span = 2 * hr + 4;
+  const size_t ncell = (size_t)g.size_map * g.size_map;
+  if (lvl_cell.size() < ncell) lvl_cell.assign(ncell, -1);
+  if (lvl_px.size() < (size_t)span * span) lvl_px.assign((size_t)span * span, -1);
+  auto px_slot = [&](int64_t ic) {
+    return (size_t)((int)(ic / g.cols) - y_lo) * span + (size_t)((int)(ic % g.cols) - x_lo);
+  };
+  lvl.resize(vis.size());
   int maxl = -1;
-  for (auto& v : rc) {
-    int64_t ic;
-    const int m = stamp_visit(g, v.second, v.first, &ic);
-    int L = 0;
-    auto im = lvl_map.find(m);
-    if (im != lvl_map.end()) L = std::max(L, im->second + 1);
-    auto ip = lvl_px.find(ic);
-    if (ip != lvl_px.end()) L = std::max(L, ip->second + 1);
-    lvl_map[m] = L;
-    lvl_px[ic] = L;
-    lv.push_back({L, make_int2(v.second, v.first)});
+  for (size_t i = 0; i < vis.size(); ++i) {
+    const size_t p = px_slot(vis[i].ic);
+    const int L = std::max(lvl_cell[(size_t)vis[i].m], lvl_px[p]) + 1;
+    lvl_cell[(size_t)vis[i].m] = L;
+    lvl_px[p] = L;
+    lvl[i] = L;
     maxl = std::max(maxl, L);
   }
-  std::stable_sort(lv.begin(), lv.end(),
-                   [](const std::pair<int, int2>& x, const std::pair<int, int2>& y) {
-                     return x.first < y.first;
-                   });
+  for (const Visit& v : vis) {  // clear what this stamp set
+    lvl_cell[(size_t)v.m] = -1;
+    lvl_px[px_slot(v.ic)] = -1;
+  }
+  // counting sort by level, keeping reference order within a level
   off.assign((size_t)maxl + 2, 0);
-  for (auto& e : lv) off[(size_t)e.first + 1]++;
+  for (int L : lvl) off[(size_t)L + 1]++;
   for (size_t i = 1; i < off.size(); ++i) off[i] += off[i - 1];
-  for (auto& e : lv) list.push_back(e.second);
+  list.resize(vis.size());
+  std::vector<int> pos(off.begin(), off.end() - 1);
+  for (size_t i = 0; i < vis.size(); ++i)
+    list[(size_t)pos[(size_t)lvl[i]]++] = make_int2(vis[i].col, vis[i].row);
 }
 
 }  // namespace pnt

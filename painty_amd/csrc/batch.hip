@@ -23,9 +23,12 @@
 // semantics by the job's group), so blocks that finish early composite other
 // canvases.  Every wait is bounded.
 #include <algorithm>
+#include <atomic>
+#include <exception>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "handles.hpp"
@@ -423,6 +426,35 @@ struct BatchPlan {
   int njobs = 0, n_items = 0, nblocks = 0, tile = 32;
 };
 
+// Runs f(j) for every job, spread over up to 16 host threads by stamp count
+// (planning is pure host work on each job's own brush and canvas).
+template <class F>
+static void parallel_jobs(const pnt_batch_job* jobs, int n, F&& f) {
+  int64_t total = 0;
+  for (int j = 0; j < n; ++j) total += jobs[j].n_stamps;
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<int64_t>({(int64_t)std::min(hw, 16), (int64_t)n, 1 + total / 2048});
+  if (nt <= 1) {
+    for (int j = 0; j < n; ++j) f(j);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::exception_ptr> errs(nt);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&, t] {
+      try {
+        for (int j; (j = next.fetch_add(1)) < n;) f(j);
+      } catch (...) {
+        errs[t] = std::current_exception();
+        next.store(n);
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (auto& e : errs)
+    if (e) std::rethrow_exception(e);
+}
+
 // Plans a batch; readies each brush's snapshot and dirty bytes on the
 // context's stream first (that work is done here, once).
 static BatchPlan plan_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
@@ -436,13 +468,29 @@ static BatchPlan plan_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
   std::vector<int> hr(n);
   std::vector<int64_t> nst(n);
   std::vector<CItem> items;
+  // each job's stamps planned on its own (in parallel), then concatenated
+  std::vector<std::vector<PStamp>> j_st(n);
+  std::vector<std::vector<int2>> j_dl(n);
+  std::vector<std::vector<int>> j_off(n);
+  parallel_jobs(jobs, n, [&](int j) {
+    plan_stamps(jobs[j].brush, jobs[j].canvas, jobs[j].stamps, jobs[j].n_stamps, j_st[j], j_dl[j],
+                j_off[j]);
+    for (int64_t k = 0; k < jobs[j].n_stamps; ++k)
+      if (jobs[j].stamps[k].flags & PNT_STAMP_DRY_AFTER) j_st[j][k].flags |= kStDry;
+  });
   for (int j = 0; j < n; ++j) {
     pnt_brush b = jobs[j].brush;
     pnt_canvas cv = jobs[j].canvas;
     const int s0 = (int)h_st.size();
-    plan_stamps(b, cv, jobs[j].stamps, jobs[j].n_stamps, h_st, h_dl, h_off);
-    for (int64_t k = 0; k < jobs[j].n_stamps; ++k)
-      if (jobs[j].stamps[k].flags & PNT_STAMP_DRY_AFTER) h_st[s0 + k].flags |= kStDry;
+    const int off0 = (int)h_off.size(), dl0 = (int)h_dl.size();
+    for (PStamp p : j_st[j]) {
+      if (p.dl_nlev > 0) p.dl_lev += off0;
+      h_st.push_back(p);
+    }
+    for (int o : j_off[j]) h_off.push_back(o + dl0);
+    h_dl.insert(h_dl.end(), j_dl[j].begin(), j_dl[j].end());
+    std::vector<PStamp>().swap(j_st[j]);
+    std::vector<int2>().swap(j_dl[j]);
     brush_prepare_snapshot(b, cv, s);
     BJob& J = h_jobs[j];
     J = BJob{};

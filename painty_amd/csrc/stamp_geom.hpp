@@ -45,11 +45,18 @@ __host__ __device__ __forceinline__ int stamp_visit(const StampGeom& g, int col,
 // Canvas column 0 (row 0) receives two window columns (rows) when a stamp
 // straddles the left (top) edge with a fractional centre: (int)(z) == 0 for
 // z in (-1, 1).  Returns true when some col in [-hr, hr-1] has
-// (int)(col + cx) == 0 == (int)(col + 1 + cx).
+// (int)(col + cx) == 0 == (int)(col + 1 + cx).  Both truncate to 0 only for
+// -1 < col + cx < 0, so col lies within 1 of -cx: the three integers around
+// floor(-cx) are the only candidates (|cx| < 1e9 keeps them exact).
 __host__ __device__ __forceinline__ bool stamp_dup_axis(double c, int hr) {
 #pragma clang fp contract(off)
-  for (int k = -hr; k < hr; ++k)
+  const double f = floor(-c);
+  for (int d = -1; d <= 1; ++d) {
+    const double kd = f + (double)d;
+    if (kd < (double)-hr || kd > (double)(hr - 1)) continue;
+    const int k = (int)kd;
     if ((int)(k + c) == 0 && (int)(k + 1 + c) == 0) return true;
+  }
   return false;
 }
 

@@ -54,7 +54,11 @@ class BruteBrush:
         self.snap = {k: [list(v[0]), list(v[1]), v[2]] for k, v in canvas.cells.items()}
         self.snap_shape = (canvas.rows, canvas.cols)
 
-    def imprint(self, cx, cy, theta, canvas):
+    def imprint(self, cx, cy, theta, canvas, clock=None):
+        """clock: (tmap, now_us, dry_ms, R0, h, km) -- dicts keyed (row, col)
+        for the time map (microseconds), substrate and height, and a KM
+        function km(K, S, R0, d) -> R; checkDry then runs on every visited
+        pixel before its pickup (FootprintBrush.hxx:115)."""
         hr = (self.size_map - 1) // 2
         if self.use_snapshot:
             if self.snap_shape != (canvas.rows, canvas.cols):
@@ -83,6 +87,8 @@ class BruteBrush:
                     continue
                 if not (0 <= mx < self.size_map and 0 <= my < self.size_map):
                     continue
+                if clock is not None:
+                    _check_dry(clock, canvas.cells[(y, x)], (y, x))
                 fh = self.fp[(my, mx)]
                 if fh <= 0.0:
                     continue
@@ -105,6 +111,24 @@ class BruteBrush:
                 c[0] = _blend(vb, k_src, c[2], c[0])
                 c[1] = _blend(vb, s_src, c[2], c[1])
                 c[2] = vb + c[2]
+
+
+def _check_dry(clock, cell, key):
+    """Canvas::checkDry (Canvas.hxx:123-155): a pixel whose paint is older
+    than the drying time dries into the substrate; its time becomes now."""
+    tmap, now_us, dry_ms, R0, h, km = clock
+    v = cell[2]
+    if dry_ms > 0 and v > 0.001:
+        elapsed_ms = int((now_us - tmap[key]) / 1000)  # truncation toward zero
+        if elapsed_ms >= dry_ms:
+            h[key] += v
+            R0[key] = list(km(cell[0], cell[1], R0[key], v))
+            cell[0] = [0.0, 0.0, 0.0]
+            cell[1] = [0.0, 0.0, 0.0]
+            cell[2] = 0.0
+        # otherwise the reference's partial branch needs elapsed_ms / dry_ms
+        # (an integer quotient, 0 here) above 0.01: never
+    tmap[key] = now_us
 
 
 def round_half_away(v):

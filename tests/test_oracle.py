@@ -383,3 +383,135 @@ def test_stroke_placement_unit_chord_steps(oracle):
     # t = pd / dist lands on the segment end at pd = dist
     np.testing.assert_allclose(st[9, :2], [20.0, 10.0], atol=1e-12)
     np.testing.assert_allclose(st[-1, :2], [20.0, 25.0], atol=1e-12)
+
+
+# ------------------------------------------------ Canvas::checkDry in imprint
+def _timed_pair(rows, cols, radius, seed, snapshot, age_us, dry_ms, now_us=50_000_000):
+    """Oracle and brute restatement on the same canvas, time map aged by
+    age_us (an array or a scalar), one timed stroke each."""
+    O = oracle_mod()
+    w = 2 * radius + 1
+    core = W.noisy_core(w, W.Stream(seed, 9)).astype(np.float64)
+    K, S, V = _layer(rows, cols, seed)
+    R0 = W.substrate(rows, cols, seed).astype(np.float64)
+    tmap = (now_us - np.broadcast_to(np.asarray(age_us, np.int64), (rows, cols))).copy()
+    oc = O.Canvas.from_arrays(K, S, V, R0)
+    oc.tmap[...] = tmap
+    ob = O.Brush(radius, core)
+    ob.set_use_snapshot(snapshot)
+    ob.set_rates(0.9, 0.05)
+    bc = BruteCanvas(K.tolist(), S.tolist(), V.tolist())
+    bb = BruteBrush(radius, core.tolist())
+    bb.use_snapshot = snapshot
+    bt = {(r, c): int(tmap[r, c]) for r in range(rows) for c in range(cols)}
+    bR0 = {(r, c): list(R0[r, c]) for r in range(rows) for c in range(cols)}
+    bh = {(r, c): 0.0 for r in range(rows) for c in range(cols)}
+    stamps = [(7.3, 8.6, 0.4, PAINT_A), (9.1, 9.4, 0.45, None), (0.6, 0.3, 1.2, None)]
+    for k, (cx, cy, th, paint) in enumerate(stamps):
+        t = now_us + 1000 * k
+        if paint is not None:
+            ob.dip(*paint)
+            bb.dip(*paint)
+        ob.imprint_timed(cx, cy, th, oc, t, dry_ms)
+        clock = (bt, t, dry_ms, bR0, bh, lambda K, S, R, d: O.compute_reflectance(K, S, R, d))
+        bb.imprint(cx, cy, th, bc, clock)
+    return oc, bc, bt, bR0, bh, stamps
+
+
+@pytest.mark.parametrize("snapshot", [True, False])
+@pytest.mark.parametrize("age_ms,dry_ms", [(0, 0), (10, 20), (20, 20), (90, 20), (5000, 1)])
+def test_check_dry_matches_independent_restatement(oracle, snapshot, age_ms, dry_ms):
+    oc, bc, bt, bR0, bh, _ = _timed_pair(20, 22, 4, 11, snapshot, age_ms * 1000, dry_ms)
+    K, S, V = bc.arrays()
+    np.testing.assert_allclose(oc.K, K, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(oc.V, V, rtol=1e-13, atol=1e-15)
+    for (r, c), v in bR0.items():
+        np.testing.assert_allclose(oc.R0[r, c], v, rtol=1e-13, atol=1e-15)
+        assert oc.h[r, c] == pytest.approx(bh[(r, c)], rel=1e-13, abs=1e-15)
+        assert oc.tmap[r, c] == bt[(r, c)]
+
+
+def test_check_dry_inert_before_the_drying_time(oracle):
+    """Elapsed time below the drying time (or drying time 0) changes no paint:
+    the timed imprint equals the plain one; only visited pixels get the new
+    time."""
+    O = oracle_mod()
+    for age_ms, dry_ms in [(5, 8), (500, 0), (10**6, 0)]:  # stamps come 1 ms apart
+        oc, _, _, _, _, stamps = _timed_pair(20, 22, 4, 12, True, age_ms * 1000, dry_ms)
+        K, S, V = _layer(20, 22, 12)
+        R0 = W.substrate(20, 22, 12).astype(np.float64)
+        pc = O.Canvas.from_arrays(K, S, V, R0)
+        w = 9
+        pb = O.Brush(4, W.noisy_core(w, W.Stream(12, 9)).astype(np.float64))
+        pb.set_rates(0.9, 0.05)
+        for cx, cy, th, paint in stamps:
+            if paint is not None:
+                pb.dip(*paint)
+            pb.imprint(cx, cy, th, pc)
+        for a, b in [(oc.K, pc.K), (oc.S, pc.S), (oc.V, pc.V), (oc.R0, pc.R0), (oc.h, pc.h)]:
+            assert np.array_equal(a, b)
+        touched = oc.tmap != 50_000_000 - age_ms * 1000
+        assert touched.any() and not touched.all()
+
+
+def test_check_dry_equals_drying_the_visited_pixels_first(oracle):
+    """Snapshot off: every visited pixel with V > 0.001 older than the drying
+    time dries just before its own visit, which no earlier visit reads, so the
+    result equals drying exactly those pixels first (Canvas::dryCanvas
+    arithmetic) and imprinting without the clock.  Only the left half is old."""
+    O = oracle_mod()
+    rows, cols = 20, 22
+    age = np.zeros((rows, cols), np.int64)
+    age[:, :11] = 30_000  # 30 ms old on the left
+    oc, _, _, _, _, stamps = _timed_pair(rows, cols, 4, 13, False, age, 20)
+    K, S, V = _layer(rows, cols, 13)
+    R0 = W.substrate(rows, cols, 13).astype(np.float64)
+    h = np.zeros((rows, cols))
+    # pixels any stamp visits: the first visit of an old pixel dries it (and
+    # makes it fresh for the later stamps)
+    pb = O.Brush(4, W.noisy_core(9, W.Stream(13, 9)).astype(np.float64))
+    pb.set_use_snapshot(False)
+    pb.set_rates(0.9, 0.05)
+    probe = O.Canvas.from_arrays(K, S, np.zeros_like(V), R0)
+    probe.tmap[...] = 0
+    pb.dip(*PAINT_A)
+    for cx, cy, th, _ in stamps:
+        pb.imprint_timed(cx, cy, th, probe, 1, 0)
+    visited = probe.tmap == 1
+    dry = visited & (age > 0) & (V > 0.001)
+    assert dry.sum() > 10
+    Kd, Sd, Vd, R0d, hd = K.copy(), S.copy(), V.copy(), R0.copy(), h.copy()
+    sub = [a[dry].copy() for a in (Kd, Sd, Vd, R0d, hd)]
+    O.dry_canvas(*sub)
+    for a, v in zip((Kd, Sd, Vd, R0d, hd), sub):
+        a[dry] = v
+    pc = O.Canvas.from_arrays(Kd, Sd, Vd, R0d, hd)
+    pb2 = O.Brush(4, W.noisy_core(9, W.Stream(13, 9)).astype(np.float64))
+    pb2.set_use_snapshot(False)
+    pb2.set_rates(0.9, 0.05)
+    for cx, cy, th, paint in stamps:
+        if paint is not None:
+            pb2.dip(*paint)
+        pb2.imprint(cx, cy, th, pc)
+    for a, b in [(oc.K, pc.K), (oc.S, pc.S), (oc.V, pc.V), (oc.R0, pc.R0), (oc.h, pc.h)]:
+        np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-16)
+
+
+def test_check_dry_volume_threshold(oracle):
+    """Only V > 0.001 dries (Canvas.hxx:126): a pixel at exactly 0.001 keeps
+    its paint, one at 0.0011 dries."""
+    O = oracle_mod()
+    rows, cols = 9, 9
+    K = np.full((rows, cols, 3), 0.5)
+    S = np.full((rows, cols, 3), 0.7)
+    V = np.zeros((rows, cols))
+    V[4, 4], V[4, 5] = 0.001, 0.0011
+    c = O.Canvas.from_arrays(K, S, V, np.full((rows, cols, 3), 0.8))
+    core = np.zeros((3, 3))  # fh = 0: visits with no pickup or deposit
+    b = O.Brush(1, core)
+    b.imprint_timed(4.5, 4.5, 0.0, c, 10_000_000, 5)
+    assert c.V[4, 4] == 0.001 and c.V[4, 5] == 0.0
+    assert c.h[4, 5] == 0.0011 and c.h[4, 4] == 0.0
+    assert np.all(c.K[4, 5] == 0.0) and np.all(c.K[4, 4] == 0.5)
+    R = O.compute_reflectance([0.5] * 3, [0.7] * 3, [0.8] * 3, 0.0011)
+    np.testing.assert_allclose(c.R0[4, 5], R, rtol=1e-15)
