@@ -102,6 +102,24 @@ def cpu_baseline(wl, n_stamps: int, comp_rows: int):
             "t_stamps_s": t_st, "t_composite_s": t_c}
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summaries
+    (profiles/*.md lines `traffic_bytes_per_launch[<kernel>]: N`: FETCH_SIZE
+    and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), or
+    None when no summary covers it."""
+    import glob
+    import re
+    pat = re.compile(r"traffic_bytes_per_launch\[(.+?)\]:\s*(\d+)")
+    for path in sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                              "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = pat.search(line)
+                if m and m.group(1) == kernel:
+                    return int(m.group(2))
+    return None
+
+
 def bench_batched(args):
     """Batched configuration: N canvases (sides U{256..2048}, 50..500 stamps,
     footprints 32..128 px, pickup on) stamped and composited in one launch,
@@ -188,6 +206,7 @@ def bench_batched(args):
     plain = (time.perf_counter() - t0) * 1e3 / max(1, args.plain_steps)
     log("bench: plain %.2f ms/step" % plain)
     gbs = alg_bytes / (gpu * 1e-3) / 1e9
+    batch_kernel = "k_batch<%d>" % H.batch_get_config()[0]
     result = {
         "metric": METRIC, "value": round(n_st / (wall * 1e-3), 1), "unit": "stamps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall, 3),
@@ -201,9 +220,10 @@ def bench_batched(args):
         "tuned": tuned,
         "plain_sequential_ms_per_step": round(plain, 2),
         "speedup_vs_plain": round(plain / wall, 2),
-        "roofline": {"bound": "hbm", "kernel": "k_batch", "achieved": round(gbs, 1),
+        "roofline": {"bound": "hbm", "kernel": batch_kernel, "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes_per_step": int(alg_bytes)},
+                     "traffic": profiled_traffic(batch_kernel),
+                     "algorithmic_bytes_per_step": int(alg_bytes)},
     }
     print(json.dumps(result), flush=True)
 
@@ -574,7 +594,8 @@ def main():
         "composite_ms": round(comp_ms, 4),
         "roofline": {"bound": "hbm", "kernel": kernel,
                      "achieved": round(stamp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(stamp_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(stamp_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(kernel),
                      "algorithmic_bytes_per_step": int(stamp_bytes),
                      "launches_per_step": len(segs),
                      "avg_launch_ms": round(st_ms / len(segs), 4)},

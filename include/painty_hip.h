@@ -121,13 +121,39 @@ int pnt_canvas_dry(pnt_canvas canvas);
  * `portion`, R0 takes the leaving thickness, K/S stay. */
 int pnt_canvas_dry_step(pnt_canvas canvas, float portion);
 /* get/setDryingTime -- Canvas.hxx:157-163.  Any value >= 0 is stored; the
- * default is the reference's 15000000 ms (Canvas.hxx:32).  Stamping is
- * deterministic: the wall-clock checkDry (Canvas.hxx:123-155) is not run,
- * so only pnt_canvas_dry / PNT_STAMP_DRY_AFTER / pnt_canvas_dry_step dry
- * paint.  (With that default the reference's checkDry changes no paint
- * state within a run shorter than 4.17 h either.) */
+ * default is the reference's 15000000 ms (Canvas.hxx:32).  The drying time
+ * acts through the canvas clock below. */
 int pnt_canvas_set_drying_time_ms(pnt_canvas canvas, int64_t ms);
 int pnt_canvas_get_drying_time_ms(pnt_canvas canvas, int64_t* ms);
+/* The clock of Canvas::checkDry (Canvas.hxx:123-155), which the reference's
+ * imprint runs on every visited pixel before its pickup
+ * (FootprintBrush.hxx:86, 115): paint older than the drying time dries into
+ * the substrate as dryCanvas does, and the pixel's time point becomes the
+ * imprint's.
+ *   PNT_CLOCK_OFF (the default): no check -- stamping is deterministic, and
+ *     only pnt_canvas_dry / PNT_STAMP_DRY_AFTER / pnt_canvas_dry_step dry.
+ *   PNT_CLOCK_SYSTEM: each imprint takes system_clock::now(), as the
+ *     reference does.
+ *   PNT_CLOCK_MANUAL: each imprint takes the time set by
+ *     pnt_canvas_set_time_us (reproducible runs and tests).
+ * Switching a clock on sets every pixel's time point to now, as the
+ * reference's construction does (Canvas.hxx:45-50); pnt_canvas_clear and
+ * pnt_canvas_dry do the same (Canvas.hxx:47, 118).  Time points are
+ * microseconds; elapsed time is truncated to whole milliseconds.  With a
+ * clock on, pnt_imprint_stamps runs the plain path whatever mode it is
+ * given, and pnt_batch_* refuse the canvas. */
+#define PNT_CLOCK_OFF 0
+#define PNT_CLOCK_SYSTEM 1
+#define PNT_CLOCK_MANUAL 2
+int pnt_canvas_set_clock(pnt_canvas canvas, int clock);
+int pnt_canvas_get_clock(pnt_canvas canvas, int* clock);
+int pnt_canvas_set_time_us(pnt_canvas canvas, int64_t now_us);
+/* Canvas::checkDry (Canvas.hxx:123-155) of pixel (x, y) at time point now_us,
+ * called directly; needs a clock (the time map). */
+int pnt_canvas_check_dry(pnt_canvas canvas, int x, int y, int64_t now_us);
+/* Canvas::getTimeMap (Canvas.hxx:101-103): rows*cols time points (us). */
+int pnt_canvas_upload_time_map(pnt_canvas canvas, const int64_t* tmap_us);
+int pnt_canvas_download_time_map(pnt_canvas canvas, int64_t* tmap_us);
 
 /* ------------------------------------------------- Renderer<vec3> ------- */
 /* compose(canvas) -- renderer/Renderer.hxx:48-53: R = KM(wet layer over R0),

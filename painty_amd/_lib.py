@@ -16,6 +16,7 @@ OK, EINVAL, EHIP, ENOMEM, ESTATE, EKERNEL = 0, -1, -2, -3, -4, -5
 AOS, PLANAR = 0, 1
 STAMP_DIP, STAMP_DRY_AFTER = 1, 2
 EXEC_PLAIN, EXEC_PERSISTENT, EXEC_FUSED, EXEC_CHAINED = 0, 1, 2, 3
+CLOCK_OFF, CLOCK_SYSTEM, CLOCK_MANUAL = 0, 1, 2
 BATCH_ONE_LAUNCH, BATCH_PLAIN = 0, 1
 
 
@@ -96,6 +97,12 @@ SIGNATURES = {
     "pnt_canvas_dry": [_vp],
     "pnt_canvas_dry_step": [_vp, C.c_float],
     "pnt_canvas_set_drying_time_ms": [_vp, C.c_int64],
+    "pnt_canvas_set_clock": [_vp, C.c_int],
+    "pnt_canvas_get_clock": [_vp, C.POINTER(C.c_int)],
+    "pnt_canvas_set_time_us": [_vp, C.c_int64],
+    "pnt_canvas_check_dry": [_vp, C.c_int, C.c_int, C.c_int64],
+    "pnt_canvas_upload_time_map": [_vp, C.POINTER(C.c_int64)],
+    "pnt_canvas_download_time_map": [_vp, C.POINTER(C.c_int64)],
     "pnt_canvas_get_drying_time_ms": [_vp, C.POINTER(C.c_int64)],
     "pnt_compose": [_vp, _fp, C.c_int],
     "pnt_compose_layer_host": [_vp, _fp, _fp, C.c_int],

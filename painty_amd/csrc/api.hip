@@ -1,5 +1,6 @@
 // C ABI of painty_hip.h: handles, host<->device transfer, stamp planning.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -209,7 +210,19 @@ StampArgs stamp_args(pnt_brush b, pnt_canvas cv, double cx, double cy, double th
   a.src = b->use_snap ? b->snap_planes() : cv->layer();
   a.dupx = stamp_dup_axis(cx, a.g.hr) ? 1 : 0;
   a.dupy = stamp_dup_axis(cy, a.g.hr) ? 1 : 0;
+  a.clk = DryClock{};
   return a;
+}
+
+int64_t canvas_now_us(const pnt_canvas_s* cv) {
+  if (cv->clock == PNT_CLOCK_MANUAL) return cv->manual_us;
+  return std::chrono::duration_cast<std::chrono::microseconds>(
+             std::chrono::system_clock::now().time_since_epoch())
+      .count();
+}
+
+void canvas_stamp_time(pnt_canvas cv) {
+  if (cv->tmap) launch_fill_i64(cv->tmap, cv->n, canvas_now_us(cv), cv->ctx->stream);
 }
 
 // Visits of a stamp that share a canvas pixel on column/row 0, plus their
@@ -357,6 +370,8 @@ void imprint_plain(pnt_brush b, pnt_canvas cv, double cx, double cy, double thet
     if (b->snap_rows != cv->rows || b->snap_cols != cv->cols) brush_full_snapshot(b, cv);
   }
   StampArgs a = stamp_args(b, cv, cx, cy, theta);
+  if (cv->tmap)  // Canvas::checkDry on every visit (FootprintBrush.hxx:86, 115)
+    a.clk = DryClock{cv->tmap, canvas_now_us(cv), cv->drying_ms, cv->R0(), cv->h()};
   if (b->use_snap) launch_ring_refresh(ring_box(a.g, b->radius), cv->cols, cv->layer(), b->snap_planes(), s);
   launch_stamp_main(a, s);
   if (a.dupx || a.dupy) {
@@ -683,6 +698,7 @@ static void canvas_clear(pnt_canvas cv) {
   launch_fill(cv->base, 7 * cv->stride, 0.0f, s);
   for (int c = 0; c < 3; ++c) launch_fill(cv->base + (7 + c) * cv->stride, cv->n, cv->bg[c], s);
   launch_fill(cv->h(), cv->n, 0.0f, s);
+  canvas_stamp_time(cv);  // Canvas.hxx:47-50
 }
 
 }  // extern "C"
@@ -766,6 +782,7 @@ int pnt_canvas_destroy(pnt_canvas cv) {
     (void)hipStreamSynchronize(ctx->stream);
     if (cv->owns_base) dev_free(cv->base);
     dev_free(cv->scratch);
+    dev_free(cv->tmap);
     delete cv;
     ctx_release(ctx);
   });
@@ -849,6 +866,7 @@ int pnt_canvas_dry(pnt_canvas cv) {
     cv->ctx->use();
     cv->epoch++;
     launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), cv->ctx->stream);
+    canvas_stamp_time(cv);  // Canvas.hxx:106, 118
   });
 }
 
@@ -867,6 +885,75 @@ int pnt_canvas_set_drying_time_ms(pnt_canvas cv, int64_t ms) {
     PNT_REQUIRE(cv, "NULL canvas");
     PNT_REQUIRE(ms >= 0, "drying time must be >= 0");
     cv->drying_ms = ms;
+  });
+}
+
+int pnt_canvas_set_clock(pnt_canvas cv, int clock) {
+  return guard([&] {
+    PNT_REQUIRE(cv, "NULL canvas");
+    PNT_REQUIRE(clock == PNT_CLOCK_OFF || clock == PNT_CLOCK_SYSTEM || clock == PNT_CLOCK_MANUAL,
+                "unknown clock");
+    cv->ctx->use();
+    if (clock == PNT_CLOCK_OFF) {
+      if (cv->tmap) {
+        PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+        dev_free(cv->tmap);
+        cv->tmap = nullptr;
+      }
+    } else if (!cv->tmap) {
+      cv->tmap = reinterpret_cast<int64_t*>(dev_alloc(2 * cv->n));
+      cv->clock = clock;
+      canvas_stamp_time(cv);  // every pixel starts at now, as at construction
+    }
+    cv->clock = clock;
+  });
+}
+
+int pnt_canvas_check_dry(pnt_canvas cv, int x, int y, int64_t now_us) {
+  return guard([&] {
+    PNT_REQUIRE(cv, "NULL canvas");
+    PNT_REQUIRE(x >= 0 && y >= 0 && x < cv->cols && y < cv->rows, "pixel outside the canvas");
+    PNT_REQUIRE(cv->tmap, "the canvas has no clock (pnt_canvas_set_clock)");
+    cv->ctx->use();
+    cv->epoch++;
+    const DryClock c{cv->tmap, now_us, cv->drying_ms, cv->R0(), cv->h()};
+    launch_check_dry_pixel(c, cv->layer(), (int64_t)y * cv->cols + x, cv->ctx->stream);
+  });
+}
+
+int pnt_canvas_get_clock(pnt_canvas cv, int* clock) {
+  return guard([&] {
+    PNT_REQUIRE(cv && clock, "NULL argument");
+    *clock = cv->clock;
+  });
+}
+
+int pnt_canvas_set_time_us(pnt_canvas cv, int64_t now_us) {
+  return guard([&] {
+    PNT_REQUIRE(cv, "NULL canvas");
+    cv->manual_us = now_us;
+  });
+}
+
+int pnt_canvas_upload_time_map(pnt_canvas cv, const int64_t* tmap_us) {
+  return guard([&] {
+    PNT_REQUIRE(cv && tmap_us, "NULL argument");
+    PNT_REQUIRE(cv->tmap, "the canvas has no clock (pnt_canvas_set_clock)");
+    cv->ctx->use();
+    PNT_HIP_CHECK(hipMemcpyAsync(cv->tmap, tmap_us, cv->n * sizeof(int64_t),
+                                 hipMemcpyHostToDevice, cv->ctx->stream));
+    PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+  });
+}
+
+int pnt_canvas_download_time_map(pnt_canvas cv, int64_t* tmap_us) {
+  return guard([&] {
+    PNT_REQUIRE(cv && tmap_us, "NULL argument");
+    PNT_REQUIRE(cv->tmap, "the canvas has no clock (pnt_canvas_set_clock)");
+    cv->ctx->use();
+    PNT_HIP_CHECK(hipMemcpyAsync(tmap_us, cv->tmap, cv->n * sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, cv->ctx->stream));
+    PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
   });
 }
 
@@ -1151,6 +1238,7 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
                 "unknown mode");
     b->ctx->use();
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    if (cv->clock != PNT_CLOCK_OFF) mode = PNT_EXEC_PLAIN;  // checkDry runs on the plain path
     if (mode == PNT_EXEC_PERSISTENT) {
       run_persistent(b, cv, st, n);
     } else if (mode == PNT_EXEC_FUSED) {
@@ -1164,6 +1252,7 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
         if (st[i].flags & PNT_STAMP_DRY_AFTER) {
           cv->epoch++;
           launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), cv->ctx->stream);
+          canvas_stamp_time(cv);
         }
       }
     }
@@ -1218,6 +1307,8 @@ static void validate_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs) {
   for (int j = 0; j < n_jobs; ++j) {
     const pnt_batch_job& J = jobs[j];
     PNT_REQUIRE(J.canvas && J.brush, "batch job without canvas or brush");
+    PNT_REQUIRE(J.canvas->clock == PNT_CLOCK_OFF,
+                "a canvas with a drying clock stamps on the plain path, not in a batch");
     PNT_REQUIRE(J.canvas->ctx->device == ctx->device && J.brush->ctx->device == ctx->device,
                 "batch job on another device");
     PNT_REQUIRE(J.n_stamps >= 0 && (J.stamps || J.n_stamps == 0), "invalid stamp list");

@@ -74,6 +74,11 @@ struct pnt_canvas_s {
   float* scratch = nullptr;  // 3 planes for host-side compose results
   pnt_layer_s view;
   int64_t drying_ms = 15000000;  // Canvas.hxx:32
+  // checkDry clock (PNT_CLOCK_*); tmap holds one time point (us) per pixel
+  // while a clock is on
+  int clock = PNT_CLOCK_OFF;
+  int64_t manual_us = 0;
+  int64_t* tmap = nullptr;
   float bg[3] = {1.f, 1.f, 1.f};
   // Bumped by every change of the wet layer; a brush's snapshot dirty bytes
   // are valid only for the (canvas, epoch) they were last brought up to.
@@ -157,4 +162,8 @@ void brush_set_geometry(pnt_brush b, double radius, const float* core, int width
 // A canvas (cleared, background 1) on a caller buffer of at least
 // 11*round_up(rows*cols, 64) floats that pnt_canvas_destroy does not free.
 pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base);
+// The canvas clock's current time point (microseconds).
+int64_t canvas_now_us(const pnt_canvas_s* cv);
+// Every pixel's time point set to now (Canvas::clear, dryCanvas), if a clock is on.
+void canvas_stamp_time(pnt_canvas cv);
 }  // namespace pnt

@@ -40,6 +40,16 @@ struct BrushArgs {
   float paintK[3], paintS[3];
 };
 
+// Canvas::checkDry state of one imprint (Canvas.hxx:123-155); a null tmap
+// switches the check off.
+struct DryClock {
+  int64_t* tmap;    // time map of the canvas, microseconds per pixel
+  int64_t now_us;   // the imprint's time point
+  int64_t dry_ms;   // drying time
+  Rgb3 R0;          // substrate and height the drying writes
+  float* h;
+};
+
 // One imprint: geometry + where its visits read and write.
 struct StampArgs {
   StampGeom g;
@@ -47,6 +57,7 @@ struct StampArgs {
   Layer7 canvas;     // wet layer of the canvas
   Layer7 src;        // pickup source: the snapshot, or the canvas itself
   int dupx, dupy;    // canvas column / row 0 visited twice (see stamp_dup_axis)
+  DryClock clk;
 };
 
 // Snapshot ring of updateSnapshot(canvas, center), FootprintBrush.hxx:286-305.
@@ -67,6 +78,8 @@ void launch_reflectance_aos(int64_t n, const float* K, const float* S, const flo
 void launch_fill(float* p, int64_t n, float v, hipStream_t s);
 void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 void launch_fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t s);
+void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
+void launch_check_dry_pixel(const DryClock& c, const Layer7& canvas, int64_t ic, hipStream_t s);
 void launch_copy_layer(int64_t n, const Layer7& src, const Layer7& dst, hipStream_t s);
 
 // ---- stamping (kernels.hip) ----

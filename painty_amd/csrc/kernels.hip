@@ -188,6 +188,11 @@ __global__ __launch_bounds__(kBlock) void k_fill(float* p, int64_t n, float v) {
   if (i < n) p[i] = v;
 }
 
+__global__ __launch_bounds__(kBlock) void k_fill_i64(int64_t* p, int64_t n, int64_t v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy_layer(int64_t n, Layer7 src, Layer7 dst) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -209,6 +214,44 @@ struct GlobalVisit {
   __device__ float canvas(int c) const { return a.canvas.p[c][ic]; }
   __device__ void set_canvas(int c, float v) const { a.canvas.p[c][ic] = v; }
 };
+
+// Canvas::checkDry (Canvas.hxx:123-155) of the visited pixel, before its
+// pickup and deposit (FootprintBrush.hxx:115): paint older than the drying
+// time dries into the substrate as dryCanvas does; the pixel's time becomes
+// the imprint's.  V > 0.001 in FP64 is V >= 0.001f for a float V (0.001f is
+// the first float above 0.001).  The reference's partial branch needs the
+// integer quotient elapsed / drying time above 0.01, which is 0 whenever
+// elapsed < drying time, so it never runs.
+__device__ __forceinline__ void check_dry(const DryClock& c, const Layer7& canvas, int64_t ic) {
+  const float v = canvas.p[6][ic];
+  if (c.dry_ms > 0 && v >= 0.001f) {
+    const int64_t elapsed_ms = (c.now_us - c.tmap[ic]) / 1000;  // duration_cast truncation
+    if (elapsed_ms >= c.dry_ms) {
+      float K[3], S[3], R[3];
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        K[ch] = canvas.p[ch][ic];
+        S[ch] = canvas.p[3 + ch][ic];
+        R[ch] = c.R0.p[ch][ic];
+      }
+      c.h[ic] += v;
+      km_pixel(K, S, R, v);
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        c.R0.p[ch][ic] = R[ch];
+        canvas.p[ch][ic] = 0.0f;
+        canvas.p[3 + ch][ic] = 0.0f;
+      }
+      canvas.p[6][ic] = 0.0f;
+    }
+  }
+  c.tmap[ic] = c.now_us;
+}
+
+// Canvas::checkDry of one pixel, called directly (Canvas.hxx:123-155).
+__global__ __launch_bounds__(64) void k_check_dry_pixel(DryClock c, Layer7 canvas, int64_t ic) {
+  if (threadIdx.x == 0) check_dry(c, canvas, ic);
+}
 
 __device__ __forceinline__ void paint_visit(const StampArgs& a, int64_t ic, int im) {
   const float fh = a.b.fp[im];
@@ -286,8 +329,12 @@ __global__ __launch_bounds__(kTile* kTile) void k_stamp_main(StampArgs a, int ti
   int64_t ic, pic = 0;
   stamp_visit(a.g, col, row, &ic);
   if (partner) stamp_visit(a.g, pcol, prow, &pic);
+  if (a.clk.tmap) check_dry(a.clk, a.canvas, ic);
   paint_visit(a, ic, m);
-  if (partner) paint_visit(a, pic, m);
+  if (partner) {
+    if (a.clk.tmap) check_dry(a.clk, a.canvas, pic);
+    paint_visit(a, pic, m);
+  }
 }
 
 // Visits listed in reference order, grouped by level: a visit's level is one
@@ -300,7 +347,10 @@ __global__ __launch_bounds__(kBlock) void k_stamp_serial(StampArgs a, const int2
       const int2 v = list[k];
       int64_t ic;
       const int m = stamp_visit(a.g, v.x, v.y, &ic);
-      if (m >= 0) paint_visit(a, ic, m);
+      if (m >= 0) {
+        if (a.clk.tmap) check_dry(a.clk, a.canvas, ic);
+        paint_visit(a, ic, m);
+      }
     }
     __syncthreads();
   }
@@ -375,6 +425,17 @@ void launch_reflectance_aos(int64_t n, const float* K, const float* S, const flo
 void launch_fill(float* p, int64_t n, float v, hipStream_t s) {
   if (n <= 0) return;
   k_fill<<<blocks_for(n), kBlock, 0, s>>>(p, n, v);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_check_dry_pixel(const DryClock& c, const Layer7& canvas, int64_t ic, hipStream_t s) {
+  k_check_dry_pixel<<<1, 64, 0, s>>>(c, canvas, ic);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s) {
+  if (n <= 0) return;
+  k_fill_i64<<<blocks_for(n), kBlock, 0, s>>>(p, n, v);
   PNT_HIP_CHECK(hipGetLastError());
 }
 

@@ -229,6 +229,33 @@ class Canvas:
         call("pnt_canvas_get_drying_time_ms", self.h, C.byref(v))
         return v.value
 
+    def setClock(self, clock: int) -> None:
+        """_lib.CLOCK_OFF / CLOCK_SYSTEM / CLOCK_MANUAL (Canvas::checkDry)."""
+        call("pnt_canvas_set_clock", self.h, int(clock))
+
+    def getClock(self) -> int:
+        v = C.c_int()
+        call("pnt_canvas_get_clock", self.h, C.byref(v))
+        return v.value
+
+    def setTime(self, now_us: int) -> None:
+        """The time point of the next imprints under CLOCK_MANUAL (us)."""
+        call("pnt_canvas_set_time_us", self.h, int(now_us))
+
+    def checkDry(self, x: int, y: int, now_us: int) -> None:
+        """Canvas::checkDry of one pixel at time point now_us (needs a clock)."""
+        call("pnt_canvas_check_dry", self.h, int(x), int(y), int(now_us))
+
+    def getTimeMap(self) -> np.ndarray:
+        t = np.empty((self.rows, self.cols), np.int64)
+        call("pnt_canvas_download_time_map", self.h, t.ctypes.data_as(C.POINTER(C.c_int64)))
+        return t
+
+    def setTimeMap(self, tmap) -> None:
+        t = np.ascontiguousarray(tmap, dtype=np.int64)
+        assert t.shape == (self.rows, self.cols)
+        call("pnt_canvas_upload_time_map", self.h, t.ctypes.data_as(C.POINTER(C.c_int64)))
+
     def device_planes(self):
         """(layer planes[7], R0 planes[3], h) as device addresses."""
         L, R, h = _lib.DevLayer(), _lib.DevRgb(), C.c_void_p()

@@ -8,6 +8,8 @@
 //   test_dropin km <in.txt> <out.txt>       ComputeReflectance per case
 //   test_dropin stamp <in.bin> <out.bin>    one canvas, one brush, a stamp list
 //   test_dropin api                         mixer and error behaviour
+#include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -176,6 +178,23 @@ int api() {
   Mat<vec3> r0(1, 1);
   other.composeOnto(r0);  // reallocated to the layer's size, filled with 1
   EXPECT(r0.rows == 8 && r0.cols == 6);
+  // time map and checkDry (Canvas.hxx:101-103, 123-155): the canvas runs on
+  // the system clock; a pixel older than the drying time dries when checked
+  {
+    Canvas<vec3> tc(4, 5);
+    const auto t0 = std::chrono::system_clock::now();
+    auto& tm = tc.getTimeMap();
+    EXPECT(tm.size() == 20u);
+    EXPECT(tm[7] <= t0 + std::chrono::seconds(1) && tm[7] >= t0 - std::chrono::seconds(60));
+    tc.getPaintLayer().set(1, 2, vec3(0.5, 0.6, 0.7), vec3(1.0, 1.1, 1.2), 0.25);
+    tc.setDryingTime(std::chrono::milliseconds(10));
+    tm[1 * 5 + 2] = t0 - std::chrono::milliseconds(50);  // uploaded by the next call
+    tc.checkDry(2, 1, t0);
+    EXPECT(tc.getPaintLayer().getV_buffer()(1, 2) == 0.0);
+    EXPECT(std::abs(tc.get_h()(1, 2) - 0.25) < 1e-7);
+    EXPECT(tc.getTimeMap()[1 * 5 + 2] == t0 - std::chrono::nanoseconds(
+               t0.time_since_epoch().count() % 1000));  // microsecond time points
+  }
   // errors map to the reference's exception types
   EXPECT(throws_invalid([] { Canvas<vec3> bad(-1, 4); }));
   EXPECT(throws_invalid([] { FootprintBrush<vec3> b(3.0, Mat<double>(6, 6, 0.5)); }));

@@ -1,0 +1,149 @@
+"""Canvas::checkDry inside imprint (Canvas.hxx:123-155, FootprintBrush.hxx:
+86, 115) on the GPU's plain path, against the FP64 oracle's timed imprint:
+wet layer, substrate, height and composite within 1e-5 relative per
+channel, time maps exactly.  The canvas clock is PNT_CLOCK_MANUAL, so every
+imprint's time point is set by the test."""
+import numpy as np
+import pytest
+
+from painty_amd import _lib
+from painty_amd import workload as W
+from tests.parity import compare_canvas
+from tests.test_gpu_stamp import PA, PB
+
+pytestmark = pytest.mark.gpu
+NOW = 50_000_000  # microseconds
+
+# (cx, cy, theta, paint or None): interior stamps, a re-dip, and stamps that
+# straddle the top-left corner with fractional centres (canvas column/row 0
+# visited twice)
+STAMPS = [(11.3, 9.6, 0.4, PA), (13.1, 11.4, 0.45, None), (15.2, 12.9, 0.5, None),
+          (0.6, 0.3, 1.2, PB), (1.7, 2.2, 0.9, None), (20.5, 18.25, 2.0, None),
+          (22.0, 19.0, 0.0, None)]
+
+
+def _pair(hip, oracle, rows, cols, radius, seed, snapshot, age_us, dry_ms):
+    wl = W.small(rows, cols, 0, radius, seed, pickup=True, noisy=True)
+    tmap = (NOW - np.broadcast_to(np.asarray(age_us, np.int64), (rows, cols))).copy()
+    oc = oracle.Canvas.from_arrays(wl.wet.K.astype(np.float64), wl.wet.S.astype(np.float64),
+                                   wl.wet.V.astype(np.float64), wl.R0.astype(np.float64))
+    oc.tmap[...] = tmap
+    ob = oracle.Brush(wl.radii[0], wl.cores[0].astype(np.float64))
+    ob.set_rates(wl.pickup_rate, wl.deposition_rate)
+    ob.set_use_snapshot(snapshot)
+    gc = hip.Canvas(rows, cols)
+    gc.getPaintLayer().setBuffers(wl.wet.K, wl.wet.S, wl.wet.V)
+    gc.setR0(wl.R0)
+    gc.setDryingTime(dry_ms)
+    gc.setClock(_lib.CLOCK_MANUAL)
+    gc.setTimeMap(tmap)
+    gb = hip.FootprintBrush(wl.radii[0], wl.cores[0])
+    gb.setPickupRate(wl.pickup_rate)
+    gb.setDepositionRate(wl.deposition_rate)
+    gb.setUseSnapshotBuffer(snapshot)
+    return oc, ob, gc, gb
+
+
+def _run(oc, ob, gc, gb, stamps, dry_ms, step_us=1000):
+    for k, (cx, cy, th, paint) in enumerate(stamps):
+        t = NOW + step_us * k
+        if paint is not None:
+            ob.dip(*paint)
+            gb.dip(paint)
+        ob.imprint_timed(cx, cy, th, oc, t, dry_ms)
+        gc.setTime(t)
+        gb.imprint((cx, cy), th, gc)
+
+
+@pytest.mark.parametrize("snapshot", [True, False])
+@pytest.mark.parametrize("age_ms,dry_ms", [(0, 0), (10, 20), (25, 20), (90, 20), (5000, 1),
+                                           (3, 2)])
+def test_timed_imprint_matches_oracle(hip, oracle, snapshot, age_ms, dry_ms):
+    oc, ob, gc, gb = _pair(hip, oracle, 30, 34, 5, 21, snapshot, age_ms * 1000, dry_ms)
+    _run(oc, ob, gc, gb, STAMPS, dry_ms)
+    tag = "clock/snap%d/age%d/dry%d" % (snapshot, age_ms, dry_ms)
+    compare_canvas(tag, gc, oc, pickup=(gb, ob))
+    assert np.array_equal(gc.getTimeMap(), oc.tmap), tag + ".tmap"
+
+
+def test_timed_strokes_with_redrying(hip, oracle):
+    """Strokes 1 ms apart with a 2 ms drying time: pixels dry again and
+    again under later stamps of the same stroke; half the canvas starts old."""
+    rows, cols = 96, 112
+    age = np.zeros((rows, cols), np.int64)
+    age[:, : cols // 2] = 40_000
+    oc, ob, gc, gb = _pair(hip, oracle, rows, cols, 8, 5, True, age, 2)
+    wl = W.small(rows, cols, 40, 8, 5, pickup=True, noisy=True)
+    stamps = [(float(s["cx"]), float(s["cy"]), float(s["theta"]),
+               (s["K"].astype(np.float64), s["S"].astype(np.float64))
+               if s["flags"] & W.DIP else None) for s in wl.stamps]
+    _run(oc, ob, gc, gb, stamps, 2)
+    compare_canvas("clock/strokes", gc, oc, pickup=(gb, ob))
+    assert np.array_equal(gc.getTimeMap(), oc.tmap)
+
+
+@pytest.mark.parametrize("mode", [_lib.EXEC_PERSISTENT, _lib.EXEC_FUSED, _lib.EXEC_CHAINED])
+def test_clock_runs_stamp_lists_on_the_plain_path(hip, oracle, mode):
+    """With a clock on, a stamp list in any mode gives the plain path's result
+    (every stamp of the list takes the canvas's current time point)."""
+    results = []
+    for m in (_lib.EXEC_PLAIN, mode):
+        _, _, gc, gb = _pair(hip, oracle, 40, 44, 5, 31, True, 30_000, 20)
+        gc.setTime(NOW)
+        wl = W.small(40, 44, 25, 5, 31, pickup=True, noisy=True)
+        gb.imprintStamps(wl.stamps, gc, m)
+        lay = gc.getPaintLayer()
+        results.append([lay.getK_buffer(), lay.getV_buffer(), gc.getR0(), gc.get_h(),
+                        gc.getTimeMap()])
+    for a, b in zip(*results):
+        assert np.array_equal(a, b)
+
+
+def test_clock_clear_dry_and_system_time(hip, oracle):
+    """Switching the clock on, clear and dryCanvas stamp every pixel with the
+    current time; the system clock reads microseconds since the epoch."""
+    import time
+    gc = hip.Canvas(16, 20)
+    t0 = int(time.time() * 1e6)
+    gc.setClock(_lib.CLOCK_SYSTEM)
+    t1 = int(time.time() * 1e6)
+    tm = gc.getTimeMap()
+    assert tm.min() >= t0 - 2000 and tm.max() <= t1 + 2000 and (tm == tm[0, 0]).all()
+    gc.setClock(_lib.CLOCK_MANUAL)
+    gc.setTime(123_456)
+    gc.dryCanvas()
+    assert (gc.getTimeMap() == 123_456).all()
+    gc.setTime(777)
+    gc.clear()
+    assert (gc.getTimeMap() == 777).all()
+    assert gc.getClock() == _lib.CLOCK_MANUAL
+    gc.setClock(_lib.CLOCK_OFF)
+    with pytest.raises(hip.InvalidArgument):
+        gc.getTimeMap()
+
+
+def test_default_drying_time_changes_no_paint(hip, oracle):
+    """The reference's default drying time (15,000,000 ms) under the system
+    clock: the same paint as with no clock."""
+    results = []
+    for clock in (_lib.CLOCK_OFF, _lib.CLOCK_SYSTEM):
+        wl = W.small(40, 44, 20, 5, 41, pickup=True, noisy=True)
+        gc = hip.Canvas(40, 44)
+        gc.getPaintLayer().setBuffers(wl.wet.K, wl.wet.S, wl.wet.V)
+        gc.setR0(wl.R0)
+        gc.setClock(clock)
+        gb = hip.FootprintBrush(wl.radii[0], wl.cores[0])
+        gb.imprintStamps(wl.stamps, gc, _lib.EXEC_PLAIN)
+        lay = gc.getPaintLayer()
+        results.append([lay.getK_buffer(), lay.getS_buffer(), lay.getV_buffer(), gc.getR0()])
+    for a, b in zip(*results):
+        assert np.array_equal(a, b)
+
+
+def test_batch_refuses_clocked_canvas(hip):
+    wl = W.small(30, 30, 5, 4, 3)
+    gc = hip.Canvas(30, 30)
+    gc.setClock(_lib.CLOCK_MANUAL)
+    gb = hip.FootprintBrush(wl.radii[0], wl.cores[0])
+    with pytest.raises(hip.InvalidArgument):
+        hip.batch_run([(gc, gb, wl.stamps, None)])
