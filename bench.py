@@ -158,14 +158,14 @@ def bench_batched(args):
     def run(mode):
         H.batch_run(jobs, mode=mode)
 
-    for _ in range(args.warmup):
-        run(_lib.BATCH_ONE_LAUNCH)
-        ctx.synchronize()
     tuned = None
     if args.batch_tile:
         H.batch_set_config(args.batch_tile, args.batch_group)
         tuned = {"tile": args.batch_tile, "max_group": args.batch_group, "ms": None}
-    elif not args.no_tune:
+    for _ in range(args.warmup):
+        run(_lib.BATCH_ONE_LAUNCH)
+        ctx.synchronize()
+    if not args.batch_tile and not args.no_tune:
         t0 = time.perf_counter()
         (tile, group), tms = H.batch_tune(jobs)
         tuned = {"tile": tile, "max_group": group, "ms": round(tms, 3)}

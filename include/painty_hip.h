@@ -159,6 +159,21 @@ int pnt_canvas_download_time_map(pnt_canvas canvas, int64_t* tmap_us);
 /* compose(canvas) -- renderer/Renderer.hxx:48-53: R = KM(wet layer over R0),
  * out of place, into a host image. */
 int pnt_compose(pnt_canvas canvas, float* R, int layout);
+/* Renderer::render -- Renderer.hxx:60-156: the composite lit by a point
+ * light at (-200, -1500, -2000) with a Beckmann / Cook-Torrance specular
+ * term over the height field of the wet layer's V (reflect-bordered
+ * neighbours), clamped to [0, 1].  pnt_render downloads it (layout as for
+ * downloads); pnt_dev_render (below) writes caller device planes. */
+int pnt_render(pnt_canvas canvas, float* rgb, int layout);
+/* io::imSave's pixel transform (io/src/ImageIO.cxx:158-183):
+ * ColorConverter::rgb2srgb when to_srgb (core/Color.hxx:197-205), then
+ * scale by 2^bits - 1 (bits 8: uint8_t, 16: uint16_t), round half to even
+ * and saturate, as Mat::convertTo does.  Output: n interleaved RGB triples
+ * (the reference swaps to BGR only for OpenCV's file writer; writing files
+ * is left to the caller).  pnt_encode goes host to host; pnt_dev_encode
+ * (below, with the device-plane entry points) stays on the device. */
+int pnt_encode(pnt_ctx ctx, int64_t n, const float* rgb, int layout, int to_srgb,
+               int bits, void* out);
 /* compose(paintLayer, R0_buffer) -- Renderer.hxx:26-41 with host images. */
 int pnt_compose_layer_host(pnt_layer layer, const float* R0, float* R,
                            int layout);
@@ -326,6 +341,12 @@ typedef struct {
 typedef struct {
   float* p[3];
 } pnt_dev_rgb;
+/* Renderer::render into caller device planes (see pnt_render). */
+int pnt_dev_render(pnt_canvas canvas, pnt_dev_rgb rgb);
+/* The imSave pixel transform (see pnt_encode) from device planes to device
+ * memory, on `stream` (NULL: the context's stream). */
+int pnt_dev_encode(pnt_ctx ctx, int64_t n, pnt_dev_rgb rgb, int to_srgb, int bits,
+                   void* out, void* stream);
 /* R = KM(layer over R0); R may equal R0 (composeOnto). */
 int pnt_dev_compose(pnt_ctx ctx, int64_t n, pnt_dev_layer layer,
                     pnt_dev_rgb R0, pnt_dev_rgb R, void* stream);

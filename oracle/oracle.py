@@ -82,6 +82,11 @@ def lib():
                                         C.c_double, C.c_double, C.c_double]
         L.orc_run_stamps.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
                                      C.POINTER(Stamp), C.c_int64]
+        L.orc_render.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_render_pixel_normal.argtypes = [C.c_double] * 4 + [_dp]
+        L.orc_rgb2srgb.restype = C.c_double
+        L.orc_rgb2srgb.argtypes = [C.c_double]
+        L.orc_encode.argtypes = [C.c_int64, _dp, C.c_int, C.c_int, C.POINTER(C.c_uint16)]
         L.orc_brush_imprint_timed.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp,
                                               C.c_double, C.c_double, C.c_double,
                                               C.POINTER(Clock)]
@@ -359,3 +364,33 @@ def stroke_stamps(path) -> np.ndarray:
     out = np.empty((max(n, 1), 3))
     lib().orc_stroke_stamps(_p(path), len(path), _p(out), n)
     return out[:n]
+
+
+# ------------------------------------------------------------ display -------
+def render(K, S, V, R0) -> np.ndarray:
+    """Renderer::render of the wet layer (K, S (rows, cols, 3), V (rows, cols))
+    over R0 -> (rows, cols, 3) in [0, 1]."""
+    K, S, V, R0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (K, S, V, R0))
+    rows, cols = V.shape
+    out = np.empty((rows, cols, 3))
+    lib().orc_render(rows, cols, _p(K), _p(S), _p(V), _p(R0), _p(out))
+    return out
+
+
+def render_pixel_normal(s01, s21, s10, s12) -> np.ndarray:
+    n = np.empty(3)
+    lib().orc_render_pixel_normal(float(s01), float(s21), float(s10), float(s12), _p(n))
+    return n
+
+
+def rgb2srgb(l: float) -> float:
+    return lib().orc_rgb2srgb(float(l))
+
+
+def encode(rgb, to_srgb: bool = True, bits: int = 8) -> np.ndarray:
+    a = np.ascontiguousarray(rgb, dtype=np.float64)
+    out = np.empty(a.shape, np.uint16)
+    lib().orc_encode(a.size // 3, _p(a), 1 if to_srgb else 0, int(bits),
+                     out.ctypes.data_as(C.POINTER(C.c_uint16)))
+    return out.astype(np.uint8) if bits == 8 else out
+

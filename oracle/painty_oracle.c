@@ -1024,3 +1024,125 @@ int orc_grad_run(orc_brush* b, int rows, int cols, double* K, double* S,
   free(gP);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Renderer::render (renderer/Renderer.hxx:60-156): the composite lit by a
+ * point light with a Beckmann / Cook-Torrance specular term over the height
+ * field of the wet layer's thickness V. */
+
+static const double kPi = 3.141592653589793238462643383279502884; /* Math.hxx:21-22 */
+
+/* cv::borderInterpolate with BORDER_REFLECT (the default of Interpolate,
+ * image/Mat.hxx:69-71) for p in [-1, len]: fedcba|abcdefgh|hgfedcb */
+static int reflect_index(int p, int len) {
+  if (len == 1) return 0;
+  if (p < 0) return -p - 1;
+  if (p >= len) return 2 * len - p - 1;
+  return p;
+}
+
+static void v3_normalized(const double a[3], double out[3]) {
+  /* Eigen's normalized(): a / sqrt(squaredNorm), unchanged when zero */
+  const double z = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+  const double n = z > 0.0 ? sqrt(z) : 1.0;
+  for (int i = 0; i < 3; i++) out[i] = z > 0.0 ? a[i] / n : a[i];
+}
+
+static double v3_dot(const double a[3], const double b[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+static double render_G(double NdotH, double NdotV, double VdotH, double NdotL) {
+  const double G1 = 2.0 * NdotH * NdotV / VdotH;
+  const double G2 = 2.0 * NdotH * NdotL / VdotH;
+  const double g = (G2 < G1) ? G2 : G1; /* std::min(G1, G2) */
+  return (g < 1.0) ? g : 1.0;           /* std::min(1.0, g) */
+}
+
+static double render_beckmann(double NdotH, double m) {
+  const double A = 1.0 / (pow(m, 2.0) + pow(NdotH, 4.0) * kPi);
+  const double B = exp(-pow(tan(acos(NdotH)), 2.0) / pow(m, 2.0));
+  return A * B;
+}
+
+void orc_render_pixel_normal(double s01, double s21, double s10, double s12,
+                             double n[3]) {
+  const double a[3] = {2.0, 0.0, s21 - s01}, b[3] = {0.0, 2.0, s12 - s10};
+  double va[3], vb[3], c[3];
+  v3_normalized(a, va);
+  v3_normalized(b, vb);
+  c[0] = va[1] * vb[2] - va[2] * vb[1];
+  c[1] = va[2] * vb[0] - va[0] * vb[2];
+  c[2] = va[0] * vb[1] - va[1] * vb[0];
+  v3_normalized(c, n);
+  n[2] *= -1.0;
+}
+
+void orc_render(int rows, int cols, const double* K, const double* S,
+                const double* V, const double* R0, double* out) {
+  const double lightPos[3] = {-200.0, -1500.0, -2000.0};
+  const double eyePos[3] = {cols / 2.0, rows / 2.0, -100.0};
+  const double lightPower = 15.0, Ks = 1.0, m = 0.5, s = 0.2;
+  for (int i = 0; i < rows; ++i) {
+    for (int j = 0; j < cols; ++j) {
+      const int64_t p = (int64_t)i * cols + j;
+      double Kd[3];
+      orc_compute_reflectance(K + 3 * p, S + 3 * p, R0 + 3 * p, V[p], Kd);
+      const double s11 = V[p];
+      const double s01 = V[(int64_t)i * cols + reflect_index(j - 1, cols)];
+      const double s21 = V[(int64_t)i * cols + reflect_index(j + 1, cols)];
+      const double s10 = V[(int64_t)reflect_index(i - 1, rows) * cols + j];
+      const double s12 = V[(int64_t)reflect_index(i + 1, rows) * cols + j];
+      double n[3];
+      orc_render_pixel_normal(s01, s21, s10, s12, n);
+      const double pix[3] = {(double)j, (double)i, s11};
+      double ld0[3], ld[3], l[3], v0[3], v[3], h0[3], h[3];
+      for (int k = 0; k < 3; k++) ld0[k] = lightPos[k] - pix[k];
+      v3_normalized(ld0, ld);
+      v3_normalized(ld, l);
+      for (int k = 0; k < 3; k++) v0[k] = eyePos[k] - pix[k];
+      v3_normalized(v0, v);
+      for (int k = 0; k < 3; k++) h0[k] = v[k] + l[k];
+      v3_normalized(h0, h);
+      const double NdotH = fmax(0.0, v3_dot(n, h));
+      const double VdotH = fmax(0.0, v3_dot(v, h));
+      const double NdotV = fmax(0.0, v3_dot(n, v));
+      const double NdotL = fmax(0.0, v3_dot(n, l));
+      double spec[3] = {0.0, 0.0, 0.0};
+      if (NdotL > 0.0 && NdotV > 0.0) {
+        const double bg = render_beckmann(NdotH, m) * render_G(NdotH, NdotV, VdotH, NdotL);
+        const double rf = Ks + (1.0 - Ks) * pow(1.0 - VdotH, 5.0);
+        for (int k = 0; k < 3; k++) spec[k] = (bg * rf) / (NdotL * NdotV);
+      }
+      const double ldn = sqrt(v3_dot(ld, ld));
+      const double beta = lightPower * (1.0 / (4.0 * kPi * pow(ldn, 2.0)));
+      for (int k = 0; k < 3; k++) {
+        const double r = (beta * NdotL) * ((1.0 - s) * Kd[k] + s * spec[k]) +
+                         (Kd[k] * 0.2) * Kd[k];
+        out[3 * p + k] = fmin(fmax(r, 0.0), 1.0);
+      }
+    }
+  }
+}
+
+/* ColorConverter::rgb2srgb (core/Color.hxx:197-205) */
+double orc_rgb2srgb(double l) {
+  if (l <= 0.00313066844250063) return l * 12.92;
+  return 1.055 * pow(l, 1.0 / 2.4) - 0.055;
+}
+
+/* io::imSave's pixel transform (io/src/ImageIO.cxx:158-183): optional
+ * linear -> sRGB, then Mat::convertTo to 8 or 16 bits (scale 2^bits - 1,
+ * cvRound = round half to even, saturate).  Output interleaved RGB (the
+ * reference swaps to BGR only for OpenCV's writer). */
+void orc_encode(int64_t n, const double* rgb, int to_srgb, int bits,
+                uint16_t* out) {
+  const double scale = bits == 16 ? 65535.0 : 255.0;
+  for (int64_t i = 0; i < 3 * n; i++) {
+    const double x = to_srgb ? orc_rgb2srgb(rgb[i]) : rgb[i];
+    double r = nearbyint(x * scale);
+    if (r < 0.0) r = 0.0;
+    if (r > scale) r = scale;
+    out[i] = (uint16_t)r;
+  }
+}

@@ -162,6 +162,22 @@ double orc_catmull_rom_d1(double p_1, double p0, double p1, double p2,
 int64_t orc_stroke_stamps(const double* path_xy, int64_t n_points,
                           double* out, int64_t max_out);
 
+/* ---- display (Renderer::render, Renderer.hxx:60-156; imSave's pixel
+ * transform, io/src/ImageIO.cxx:158-183) ----
+ * orc_render: K, S, R0 AoS (rows*cols*3), V (rows*cols) -> out AoS, clamped
+ * to [0, 1].  orc_render_pixel_normal: the surface normal from the four
+ * reflect-bordered neighbours of a pixel's V (Renderer.hxx:104-121). */
+void orc_render(int rows, int cols, const double* K, const double* S,
+                const double* V, const double* R0, double* out);
+void orc_render_pixel_normal(double s01, double s21, double s10, double s12,
+                             double n[3]);
+/* ColorConverter::rgb2srgb (core/Color.hxx:197-205) */
+double orc_rgb2srgb(double l);
+/* optional rgb2srgb, scale by 2^bits - 1 (bits 8 or 16), round half to even,
+ * saturate; interleaved RGB out */
+void orc_encode(int64_t n, const double* rgb, int to_srgb, int bits,
+                uint16_t* out);
+
 #ifdef __cplusplus
 }
 #endif

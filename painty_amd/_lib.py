@@ -137,6 +137,10 @@ SIGNATURES = {
     "pnt_mixed": [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp],
     "pnt_compute_reflectance": [_vp, C.c_int64, _fp, _fp, _fp, _fp, _fp],
     "pnt_dev_compose": [_vp, C.c_int64, DevLayer, DevRgb, DevRgb, _vp],
+    "pnt_render": [_vp, _fp, C.c_int],
+    "pnt_dev_render": [_vp, DevRgb],
+    "pnt_encode": [_vp, C.c_int64, _fp, C.c_int, C.c_int, C.c_int, _vp],
+    "pnt_dev_encode": [_vp, C.c_int64, DevRgb, C.c_int, C.c_int, _vp, _vp],
     "pnt_dev_compose2": [_vp, C.c_int64, DevLayer, DevLayer, DevRgb, DevRgb, _vp],
     "pnt_canvas_device_planes": [_vp, C.POINTER(DevLayer), C.POINTER(DevRgb),
                                  C.POINTER(_vp)],

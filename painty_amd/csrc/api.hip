@@ -993,6 +993,67 @@ int pnt_compose(pnt_canvas cv, float* R, int layout) {
   });
 }
 
+int pnt_render(pnt_canvas cv, float* R, int layout) {
+  return guard([&] {
+    PNT_REQUIRE(cv && R, "NULL argument");
+    check_layout(layout);
+    cv->ctx->use();
+    if (!cv->scratch) cv->scratch = dev_alloc(3 * cv->stride);
+    Rgb3 out{{cv->scratch, cv->scratch + cv->stride, cv->scratch + 2 * cv->stride}};
+    launch_render(cv->rows, cv->cols, cv->layer(), cv->R0(), out, cv->ctx->stream);
+    download_planes(R, 3, out.p, cv->n, layout, cv->ctx->stream);
+  });
+}
+
+int pnt_dev_render(pnt_canvas cv, pnt_dev_rgb R) {
+  return guard([&] {
+    PNT_REQUIRE(cv && R.p[0] && R.p[1] && R.p[2], "NULL argument");
+    cv->ctx->use();
+    Rgb3 out{{R.p[0], R.p[1], R.p[2]}};
+    launch_render(cv->rows, cv->cols, cv->layer(), cv->R0(), out, cv->ctx->stream);
+  });
+}
+
+int pnt_dev_encode(pnt_ctx ctx, int64_t n, pnt_dev_rgb rgb, int to_srgb, int bits, void* out,
+                   void* stream) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && out && rgb.p[0] && rgb.p[1] && rgb.p[2], "NULL argument");
+    PNT_REQUIRE(n >= 0, "n must be >= 0");
+    PNT_REQUIRE(bits == 8 || bits == 16, "bits must be 8 or 16");
+    ctx->use();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    launch_encode(n, Rgb3{{rgb.p[0], rgb.p[1], rgb.p[2]}}, to_srgb ? 1 : 0, bits, out, s);
+  });
+}
+
+int pnt_encode(pnt_ctx ctx, int64_t n, const float* rgb, int layout, int to_srgb, int bits,
+               void* out) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && rgb && out, "NULL argument");
+    PNT_REQUIRE(n >= 0, "n must be >= 0");
+    PNT_REQUIRE(bits == 8 || bits == 16, "bits must be 8 or 16");
+    check_layout(layout);
+    ctx->use();
+    hipStream_t s = ctx->stream;
+    const int64_t stride = round_up(std::max<int64_t>(n, 1), 64);
+    const size_t out_bytes = (size_t)(3 * n) * (bits == 16 ? 2 : 1);
+    float* buf = dev_alloc(3 * stride + (int64_t)(out_bytes + 3) / 4);
+    try {
+      Rgb3 in{{buf, buf + stride, buf + 2 * stride}};
+      upload_planes(in.p, 3, rgb, n, layout, s);
+      void* dout = buf + 3 * stride;
+      launch_encode(n, in, to_srgb ? 1 : 0, bits, dout, s);
+      PNT_HIP_CHECK(hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, s));
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      dev_free(buf);
+      throw;
+    }
+    dev_free(buf);
+  });
+}
+
 int pnt_compose_layer_host(pnt_layer L, const float* R0, float* R, int layout) {
   return guard([&] {
     PNT_REQUIRE(L && R0 && R, "NULL argument");

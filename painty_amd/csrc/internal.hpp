@@ -79,6 +79,11 @@ void launch_fill(float* p, int64_t n, float v, hipStream_t s);
 void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 void launch_fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t s);
 void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
+
+// ---- display (display.hip) ----
+void launch_render(int rows, int cols, const Layer7& L, const Rgb3& R0, const Rgb3& out,
+                   hipStream_t s);
+void launch_encode(int64_t n, const Rgb3& in, int to_srgb, int bits, void* out, hipStream_t s);
 void launch_check_dry_pixel(const DryClock& c, const Layer7& canvas, int64_t ic, hipStream_t s);
 void launch_copy_layer(int64_t n, const Layer7& src, const Layer7& dst, hipStream_t s);
 

@@ -369,12 +369,18 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
   }
   __syncthreads();
   const int mm = sm[ly + 1][lx + 1];
+  // Tiles of up to 16x16 threads also load the partner visit's pixel in the
+  // first batch (it is another canvas pixel, which no other visit of this
+  // stamp touches), saving a dependent round trip; larger tiles keep the
+  // registers and load it after the first visit.
+  constexpr bool kPre = kT <= 16;
   bool act = false;
   int pdir = 0;
-  int64_t ic = 0;
-  bool pv = false;
+  int64_t ic = 0, pic = 0;
+  bool pv = false, ppull = false;
   float fh = 0.0f;
   CachedVisit<RegPick> acc;
+  float pcv[kPre ? 7 : 1], psv[kPre ? 7 : 1];
   if (mm >= 0 && sm[ly + 1][lx] != mm && sm[ly][lx + 1] != mm) {
     const int col = col0 + lx, row = row0 + ly;
     pdir = sm[ly + 1][lx + 2] == mm ? 1 : (sm[ly + 2][lx + 1] == mm ? 2 : 0);
@@ -388,18 +394,38 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
       for (int c = 0; c < 7; ++c) acc.v[c] = pick[c][mm];
       acc.alias = !T.use_snap;
       load_pixel(T, ic, pv, acc);
+      if (pdir) {  // the second visitor of the cell, right or below
+        pic = canvas_index(g, col + (pdir == 1), row + (pdir == 2));
+        ppull = pull && on_window_border(st.box, (int)(pic % g.cols), (int)(pic / g.cols));
+        if constexpr (kPre) {
+#pragma unroll
+          for (int c = 0; c < 7; ++c) pcv[c] = T.canvas->p[c][pic];
+          if (T.use_snap && !ppull) {
+#pragma unroll
+            for (int c = 0; c < 7; ++c) psv[c] = T.src->p[c][pic];
+          } else {
+#pragma unroll
+            for (int c = 0; c < 7; ++c) psv[c] = pcv[c];
+          }
+        }
+      }
     }
   }
   act = act && fh > 0.0f;
   if (act) {
     run_pixel(T, ic, pv, fh, pk, ps, acc);
-    if (pdir) {  // the second visitor of the cell, right or below
-      const int col = col0 + lx + (pdir == 1);
-      const int row = row0 + ly + (pdir == 2);
-      const int64_t pic = canvas_index(g, col, row);
-      const bool ppull = pull && on_window_border(st.box, (int)(pic % g.cols), (int)(pic / g.cols));
+    if (pdir) {
       acc.alias = !T.use_snap;
-      load_pixel(T, pic, ppull, acc);  // the pickup cell stays in acc.v
+      if constexpr (kPre) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) {
+          acc.cv[c] = pcv[c];
+          acc.sv[c] = psv[c];
+        }
+        acc.src_moved = false;
+      } else {
+        load_pixel(T, pic, ppull, acc);  // the pickup cell stays in acc.v
+      }
       run_pixel(T, pic, ppull, fh, pk, ps, acc);
     }
 #pragma unroll

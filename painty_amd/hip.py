@@ -279,6 +279,26 @@ class Renderer:
         call("pnt_compose_layer_host", layer.h, _pf(R0), _pf(out), AOS)
         return out
 
+    @staticmethod
+    def render(canvas) -> np.ndarray:
+        """Renderer::render(canvas): the lit composite, float32 (rows, cols, 3)."""
+        out = np.empty((canvas.rows, canvas.cols, 3), np.float32)
+        call("pnt_render", canvas.h, _pf(out), AOS)
+        return out
+
+
+def encode(rgb, to_srgb: bool = True, bits: int = 8, ctx: Context | None = None) -> np.ndarray:
+    """imSave's pixel transform: (..., 3) linear float RGB -> uint8 / uint16
+    (optionally through sRGB), interleaved RGB of the same shape."""
+    ctx = ctx or default_context()
+    a = np.ascontiguousarray(rgb, dtype=np.float32)
+    assert a.shape[-1] == 3
+    n = a.size // 3
+    out = np.empty(a.shape, np.uint16 if bits == 16 else np.uint8)
+    call("pnt_encode", ctx.h, n, _pf(a), AOS, 1 if to_srgb else 0, int(bits),
+         out.ctypes.data_as(C.c_void_p))
+    return out
+
 
 class FootprintBrush:
     """Imprint brush with an explicit footprint (width = 2*ceil(radius)+1)."""

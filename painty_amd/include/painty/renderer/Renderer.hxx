@@ -1,6 +1,6 @@
-// Renderer<vector_type>::compose -- drop-in for renderer/Renderer.hxx:26-53
-// (the KM composite of a wet layer over R0) on the device.  The display
-// path render() (Renderer.hxx:60-156) is not part of this library.
+// Renderer<vector_type> -- drop-in for renderer/Renderer.hxx:26-156: the KM
+// composite of a wet layer over R0 (compose) and the lit display image
+// (render), on the device.
 #pragma once
 
 #include "painty/renderer/Canvas.hxx"
@@ -32,6 +32,18 @@ class Renderer final {
     Mat<vector_type> R1(canvas.getRows(), canvas.getCols());
     gpu::from_f32(r, R1.data()->data());
     return R1;
+  }
+
+  // Renderer.hxx:60-156: the composite lit by a point light with a
+  // Beckmann / Cook-Torrance specular term over the wet layer's thickness,
+  // clamped to [0, 1].
+  Mat<vector_type> render(const Canvas<vector_type>& canvas) const {
+    canvas.flush();
+    std::vector<float> r(static_cast<std::size_t>(canvas.getRows()) * canvas.getCols() * 3);
+    gpu::check(pnt_render(canvas.handle(), r.data(), PNT_AOS));
+    Mat<vector_type> rgb(canvas.getRows(), canvas.getCols());
+    gpu::from_f32(r, rgb.data()->data());
+    return rgb;
   }
 };
 

@@ -1,7 +1,7 @@
 // Drives the C++ drop-in classes (painty_amd/include/painty) the way the
 // reference's own code drives its templates: Canvas / PaintLayer buffers
 // written through their getters, FootprintBrush dip / imprint / dryCanvas,
-// Renderer::compose, PaintMixer.  tests/test_gpu_dropin.py writes the
+// Renderer::compose / render, PaintMixer.  tests/test_gpu_dropin.py writes the
 // inputs, runs this program on the GPU and checks its outputs against the
 // FP64 oracle.
 //
@@ -109,6 +109,7 @@ int stamp(const char* in, const char* out) {
     if (s.flags & PNT_STAMP_DRY_AFTER) canvas.dryCanvas();
   }
   const Mat<vec3> R = Renderer<vec3>().compose(canvas);
+  const Mat<vec3> lit = Renderer<vec3>().render(canvas);
 
   std::ofstream o(out, std::ios::binary);
   const Canvas<vec3>& cc = canvas;
@@ -126,6 +127,7 @@ int stamp(const char* in, const char* out) {
   put3(P.getK_buffer());
   put3(P.getS_buffer());
   write(o, P.getV_buffer().data(), P.getV_buffer().total());
+  put3(lit);
   return 0;
 }
 

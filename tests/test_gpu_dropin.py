@@ -76,10 +76,10 @@ def test_stamp_scenario_through_cpp(tmp_path, oracle, snapshot):
     out = np.fromfile(tmp_path / "out.bin", "<f8")
     ocv, ob = run_oracle(wl, st, bake_dry=False, use_snapshot=snapshot)
     m = ob.size_map * ob.size_map
-    sizes = [3 * n, 3 * n, n, 3 * n, n, 3 * n, 3 * m, 3 * m, m]
+    sizes = [3 * n, 3 * n, n, 3 * n, n, 3 * n, 3 * m, 3 * m, m, 3 * n]
     assert out.size == sum(sizes)
     parts = np.split(out, np.cumsum(sizes)[:-1])
-    K, S, V, R0, h, R, pK, pS, pV = parts
+    K, S, V, R0, h, R, pK, pS, pV, lit = parts
     tag = "cpp.stamp/snap%d" % snapshot
     assert_close(tag + ".K", K.reshape(wl.rows, wl.cols, 3), ocv.K)
     assert_close(tag + ".S", S.reshape(wl.rows, wl.cols, 3), ocv.S)
@@ -90,6 +90,8 @@ def test_stamp_scenario_through_cpp(tmp_path, oracle, snapshot):
     oK, oS, oV = ob.pickup_map()
     assert_close(tag + ".pickup.K", pK.reshape(oK.shape), oK)
     assert_close(tag + ".pickup.V", pV.reshape(oV.shape), oV)
+    assert_close(tag + ".render", lit.reshape(wl.rows, wl.cols, 3),
+                 oracle.render(ocv.K, ocv.S, ocv.V, ocv.R0))
 
 
 def test_mixer_lifecycle_and_errors_through_cpp():
