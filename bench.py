@@ -454,6 +454,8 @@ def main():
     ap.add_argument("--canvases", type=int, default=256)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-tune", action="store_true", help="batched: keep the default launch shape")
+    ap.add_argument("--per-stroke", action="store_true",
+                    help="single, fused: one pnt_imprint_stamps call per stroke segment")
     ap.add_argument("--no-graph", action="store_true", help="batched: skip the captured graph")
     ap.add_argument("--batch-tile", type=int, default=0, help="batched: fixed tile (16 or 32)")
     ap.add_argument("--batch-group", type=int, default=16,
@@ -527,8 +529,32 @@ def main():
     log("bench: %d segments, %d active visits, %d ring pixels, %d dries per step" %
         (len(segs), visits, ring, n_dry))
     seg_events = [(ev.event(), ev.event()) for _ in segs]
+    # fused: the whole job as one stroke list (per-stroke footprints; strokes
+    # whose snapshot boxes do not meet run concurrently), dries included
+    one_call = args.mode == "fused" and not args.per_stroke
+    strokes = (_lib.PntStroke * len(segs))()
+    keep = []
+    for k, (bid, arr, dry_after) in enumerate(segs):
+        st2 = (_lib.PntStamp * len(arr))()
+        C.memmove(st2, arr, C.sizeof(arr))
+        if dry_after:
+            st2[len(arr) - 1].flags |= W.DRY_AFTER
+        core = np.ascontiguousarray(wl.cores[bid], dtype=np.float32)
+        keep.append((st2, core))
+        strokes[k].radius = float(wl.radii[bid])
+        strokes[k].footprint = core.ctypes.data_as(C.POINTER(C.c_float))
+        strokes[k].width = int(core.shape[0])
+        strokes[k].stamps = C.cast(st2, C.POINTER(_lib.PntStamp))
+        strokes[k].n_stamps = len(arr)
+    call_events = (ev.event(), ev.event())
 
     def step():
+        if one_call:
+            ev.record(call_events[0], stream)
+            _lib.call("pnt_imprint_strokes", brush.h, cv.h, strokes, len(segs), mode)
+            ev.record(call_events[1], stream)
+            _lib.call("pnt_dev_compose", ctx.h, n_px, dev_layer, dev_r0, dev_out, None)
+            return
         for k, (bid, arr, dry_after) in enumerate(segs):
             brush.setRadius(wl.radii[bid], wl.cores[bid])
             ev.record(seg_events[k][0], stream)
@@ -539,6 +565,8 @@ def main():
         _lib.call("pnt_dev_compose", ctx.h, n_px, dev_layer, dev_r0, dev_out, None)
 
     def stamp_ms():
+        if one_call:  # the dries run inside the call
+            return ev.elapsed_ms(*call_events)
         return sum(ev.elapsed_ms(a, b) for a, b in seg_events)
 
     for i in range(args.warmup):
@@ -588,7 +616,9 @@ def main():
         "config": {"workload": "large single canvas %dx%d, %d stamps (footprint 129-257 px, "
                                "pickup on, dry every 500) + composite" % (wl.rows, wl.cols,
                                                                           args.stamps),
-                   "mode": args.mode},
+                   "mode": args.mode,
+                   "call": "pnt_imprint_strokes (one call, dries inside)" if one_call
+                           else "pnt_imprint_stamps per stroke segment"},
         "stamps_per_s_kernel": round(args.stamps / (st_ms * 1e-3), 1),
         "composite_gpixel_s": round(comp_gpx, 2),
         "composite_ms": round(comp_ms, 4),
@@ -597,8 +627,10 @@ def main():
                      "frac": round(stamp_gbs / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(kernel),
                      "algorithmic_bytes_per_step": int(stamp_bytes),
-                     "launches_per_step": len(segs),
-                     "avg_launch_ms": round(st_ms / len(segs), 4)},
+                     "stamp_time": "stream time of the stamp calls"
+                                   + (" (incl. %d dries)" % n_dry if one_call else ""),
+                     "concurrent_chains": int(os.environ.get("PAINTY_FUSED_CHAINS", "4"))
+                     if one_call else 1},
         "roofline_composite": {"bound": "hbm", "kernel": "k_compose_vec<1>",
                                "achieved": round(comp_gbs, 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(comp_gbs / HBM_PEAK_GBS, 4)},

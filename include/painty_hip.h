@@ -244,6 +244,24 @@ typedef struct {
 int pnt_imprint_stamps(pnt_brush brush, pnt_canvas canvas,
                        const pnt_stamp* stamps, int64_t n, int mode);
 
+/* A stroke list: for each stroke, FootprintBrush::setRadius(radius) with
+ * its width x width footprint (FootprintBrush.hxx:46-63: nothing changes
+ * within 0.5 of the current radius, otherwise a new footprint and a clean
+ * pickup map), then its stamps as pnt_imprint_stamps would run them.  The
+ * brush ends with the last stroke's geometry.  Under PNT_EXEC_FUSED the
+ * strokes whose snapshot boxes do not meet run concurrently (each with its
+ * own pickup map; the result is the same to the bit); other modes run the
+ * strokes one after another. */
+typedef struct {
+  double radius;
+  const float* footprint;
+  int width;
+  const pnt_stamp* stamps;
+  int64_t n_stamps;
+} pnt_stroke;
+int pnt_imprint_strokes(pnt_brush brush, pnt_canvas canvas, const pnt_stroke* strokes,
+                        int64_t n_strokes, int mode);
+
 /* Many independent (canvas, brush, stamp list) jobs at once, each then
  * composited (Renderer::compose, Renderer.hxx:48-53) into `out` -- three
  * 16-byte-aligned device planes of rows*cols floats, or out[0] == NULL for

@@ -36,6 +36,11 @@ class PntStamp(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class PntStroke(C.Structure):
+    _fields_ = [("radius", C.c_double), ("footprint", C.POINTER(C.c_float)), ("width", C.c_int),
+                ("stamps", C.POINTER(PntStamp)), ("n_stamps", C.c_int64)]
+
+
 class BatchJob(C.Structure):
     _fields_ = [("canvas", C.c_void_p), ("brush", C.c_void_p), ("stamps", C.POINTER(PntStamp)),
                 ("n_stamps", C.c_int64), ("out", C.c_void_p * 3)]
@@ -123,6 +128,7 @@ SIGNATURES = {
     "pnt_brush_update_snapshot": [_vp, _vp],
     "pnt_imprint": [_vp, _vp, C.c_double, C.c_double, C.c_double],
     "pnt_imprint_stamps": [_vp, _vp, C.POINTER(PntStamp), C.c_int64, C.c_int],
+    "pnt_imprint_strokes": [_vp, _vp, C.POINTER(PntStroke), C.c_int64, C.c_int],
     "pnt_paint_stroke": [_vp, _vp, _dp, C.c_int64],
     "pnt_batch_run": [_vp, C.POINTER(BatchJob), C.c_int, C.c_int],
     "pnt_batch_set_config": [_vp, C.POINTER(BatchConfig)],

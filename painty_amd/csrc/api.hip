@@ -108,7 +108,7 @@ void dev_free(void* p) {
 
 // FootprintBrush::setRadius geometry (FootprintBrush.hxx:52-61) with an
 // injected width x width footprint, padded with zeros to size_map^2.
-void brush_set_geometry(pnt_brush b, double radius, const float* core, int width) {
+std::vector<float> padded_footprint(double radius, const float* core, int width, int* size_map_out) {
   PNT_REQUIRE(core != nullptr, "footprint is NULL");
   PNT_REQUIRE(std::isfinite(radius) && radius > 0.0 && radius < 4096.0,
               "radius must be in (0, 4096)");
@@ -122,6 +122,14 @@ void brush_set_geometry(pnt_brush b, double radius, const float* core, int width
   std::vector<float> full((size_t)size_map * size_map, 0.0f);
   for (int i = 0; i < w; ++i)
     for (int j = 0; j < w; ++j) full[(size_t)(i + pad) * size_map + (j + pad)] = core[i * w + j];
+  *size_map_out = size_map;
+  return full;
+}
+
+void brush_set_geometry(pnt_brush b, double radius, const float* core, int width) {
+  int size_map = 0;
+  std::vector<float> full = padded_footprint(radius, core, width, &size_map);
+  const int w = width;
   const int64_t m = (int64_t)size_map * size_map;
   const int64_t mstride = round_up(m, 64);
   hipStream_t s = b->ctx->stream;
@@ -1167,6 +1175,7 @@ int pnt_brush_destroy(pnt_brush b) {
     (void)hipStreamSynchronize(b->ctx->stream);
     persistent_release(b->pers);
     chain_release(b->chain);
+    fused_release(b->fused);
     dev_free(b->fp);
     dev_free(b->pick);
     dev_free(b->snap);
@@ -1315,6 +1324,34 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
           launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), cv->ctx->stream);
           canvas_stamp_time(cv);
         }
+      }
+    }
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+int pnt_imprint_strokes(pnt_brush b, pnt_canvas cv, const pnt_stroke* S, int64_t ns, int mode) {
+  return guard([&] {
+    check_same_ctx(b, cv);
+    PNT_REQUIRE(ns >= 0 && (S || ns == 0), "invalid stroke list");
+    PNT_REQUIRE(mode == PNT_EXEC_PLAIN || mode == PNT_EXEC_PERSISTENT || mode == PNT_EXEC_FUSED ||
+                    mode == PNT_EXEC_CHAINED,
+                "unknown mode");
+    for (int64_t i = 0; i < ns; ++i)
+      PNT_REQUIRE(S[i].n_stamps >= 0 && (S[i].stamps || S[i].n_stamps == 0) && S[i].footprint,
+                  "invalid stroke");
+    b->ctx->use();
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    if (mode == PNT_EXEC_FUSED && cv->clock == PNT_CLOCK_OFF) {
+      run_fused_strokes(b, cv, S, ns);
+    } else {
+      for (int64_t i = 0; i < ns; ++i) {
+        if (!(std::fabs(b->radius - S[i].radius) < 0.5)) {  // FootprintBrush::setRadius
+          PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+          brush_set_geometry(b, S[i].radius, S[i].footprint, S[i].width);
+        }
+        const int rc = pnt_imprint_stamps(b, cv, S[i].stamps, S[i].n_stamps, mode);
+        if (rc != PNT_OK) throw Error(rc, pnt_last_error());
       }
     }
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));

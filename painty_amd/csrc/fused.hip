@@ -17,6 +17,7 @@
 // entirely, and tile t of every launch runs on the same XCD
 // (tile_of_block), so a tile's pixels and map cells stay in that XCD's L2.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -107,65 +108,343 @@ int ring_blocks_for(const RingBox& b, int threads) {
   return (int)std::min<int64_t>(std::max<int64_t>(blocks, 1), 1024);
 }
 
-template <int kT>
-void run_fused_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
-  hipStream_t s = b->ctx->stream;
-  const PlanDev D = upload_plan(b, cv, st, n);
-  FArgs A;
-  A.st = D.st;
-  A.canvas = cv->layer();
-  A.snap = b->use_snap ? b->snap_planes() : cv->layer();
-  A.pick = b->pick_planes();
-  A.fp = b->fp;
-  A.dirty = b->dirty;
-  A.dl = D.dl;
-  A.dl_off = D.dl_off;
-  A.use_snap = b->use_snap ? 1 : 0;
-  A.rows = cv->rows;
-  A.cols = cv->cols;
-  A.hr = (b->size_map - 1) / 2;
-  A.size_map = b->size_map;
-  const int side = 2 * A.hr + 1;
-  A.tiles_x = (side + kT - 1) / kT;
-  A.ntiles = A.tiles_x * A.tiles_x;
-  A.pickup_rate = b->pickup_rate;
-  A.dep_rate = b->dep_rate;
-  A.cap = b->cap;
+}  // namespace
+
+// ---------------------------------------------------- concurrent strokes --
+// A dip cleans the pickup map (FootprintBrush.hxx:150-160), so the stamps
+// after it depend on earlier stamps only through the canvas, the snapshot
+// and their dirty bytes -- and a stamp touches those only inside its
+// snapshot's allowed box (the window lies inside it).  The list is cut into
+// units: a run of stamps of one stroke (from a dip to the next dip) within
+// one drying phase (up to a PNT_STAMP_DRY_AFTER).  A unit depends on the
+// earlier units of its phase whose boxes (unions of their stamps' allowed
+// boxes) intersect its own, and a dryCanvas on everything before it.
+// Independent units run at once on up to kMaxChains streams ("chains"),
+// each chain with its own pickup map; every stamp still runs exactly the
+// kernels of the sequential order, on pixels no concurrent unit touches,
+// so the result is the same to the bit.  At the end the pickup map of the
+// list's last stroke becomes the brush's.
+constexpr int kMaxChains = 8;
+
+struct FusedState {
+  hipStream_t streams[kMaxChains] = {};  // [0] unused: chain 0 is the context's stream
+  float* pick[kMaxChains] = {};          // [0] unused: chain 0 is the brush's own map
+  int64_t pick_cap[kMaxChains] = {};
+  std::vector<hipEvent_t> events;
+};
+
+void fused_release(FusedState*& F) {
+  if (!F) return;
+  for (int c = 1; c < kMaxChains; ++c) {
+    if (F->streams[c]) (void)hipStreamSynchronize(F->streams[c]);
+    if (F->streams[c]) (void)hipStreamDestroy(F->streams[c]);
+    if (F->pick[c]) (void)hipFree(F->pick[c]);
+  }
+  for (hipEvent_t e : F->events) (void)hipEventDestroy(e);
+  delete F;
+  F = nullptr;
+}
+
+namespace {
+
+int fused_chains() {
+  int c = 4;
+  if (const char* e = std::getenv("PAINTY_FUSED_CHAINS")) c = std::atoi(e);
+  return std::min(std::max(c, 1), kMaxChains);
+}
+
+struct Unit {
+  int64_t s0, s1;             // stamps [s0, s1)
+  int x0, y0, x1, y1;         // union of the allowed boxes (x1 < x0: touches nothing)
+  int chain;
+  bool dry_after;             // the unit's last stamp dries the canvas
+  std::vector<int> deps;      // earlier units of the phase it must follow
+};
+
+bool boxes_meet(const Unit& a, const Unit& b) {
+  if (a.x1 < a.x0 || b.x1 < b.x0) return false;
+  return !(a.x1 < b.x0 || b.x1 < a.x0 || a.y1 < b.y0 || b.y1 < a.y0);
+}
+
+// Cuts the list into units, finds their dependencies, and assigns chains by
+// list scheduling on estimated durations (stamp counts): a unit goes to the
+// chain where it can start first; a unit that continues a stroke stays on
+// the stroke's chain, and one that continues the brush's current stroke
+// (no dip or geometry change before it) runs on chain 0.  clear[k]: the
+// pickup map is cleaned before stamp k (a new brush geometry).
+std::vector<Unit> plan_units(const pnt_stamp* st, const PStamp* h, const uint8_t* clear,
+                             int64_t n, int nchains) {
+  std::vector<Unit> U;
   for (int64_t k = 0; k < n; ++k) {
-    if (st[k].flags & PNT_STAMP_DIP) {  // dip(): clean() the pickup map, new paint
-      launch_fill(b->pick, 7 * b->mstride, 0.0f, s);
+    const bool dip = (st[k].flags & PNT_STAMP_DIP) != 0 || clear[k];  // a clean map
+    if (U.empty() || dip || U.back().dry_after) {
+      Unit u{k, k, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, -1, false, {}};
+      u.chain = dip ? -1 : (U.empty() ? 0 : -2);  // -2: the previous unit's chain
+      U.push_back(u);
+    }
+    Unit& u = U.back();
+    u.s1 = k + 1;
+    const RingBox& b = h[k].box;
+    if (b.a_x1 >= b.a_x0 && b.a_y1 >= b.a_y0) {
+      u.x0 = std::min(u.x0, b.a_x0);
+      u.y0 = std::min(u.y0, b.a_y0);
+      u.x1 = std::max(u.x1, b.a_x1);
+      u.y1 = std::max(u.y1, b.a_y1);
+    }
+    if (st[k].flags & PNT_STAMP_DRY_AFTER) u.dry_after = true;
+  }
+  std::vector<double> avail(nchains, 0.0), finish(U.size(), 0.0);
+  double phase_start = 0.0;
+  size_t phase0 = 0;
+  for (size_t i = 0; i < U.size(); ++i) {
+    Unit& u = U[i];
+    double dep_t = phase_start;
+    for (size_t j = phase0; j < i; ++j)
+      if (boxes_meet(U[j], u)) {
+        u.deps.push_back((int)j);
+        dep_t = std::max(dep_t, finish[j]);
+      }
+    if (u.chain == -2) u.chain = U[i - 1].chain;
+    if (u.chain < 0) {
+      int best = 0;
+      for (int c = 1; c < nchains; ++c)
+        if (std::max(avail[c], dep_t) < std::max(avail[best], dep_t)) best = c;
+      u.chain = best;
+    }
+    finish[i] = std::max(avail[u.chain], dep_t) + (double)(u.s1 - u.s0);
+    avail[u.chain] = finish[i];
+    if (u.dry_after) {  // dryCanvas: everything after waits for everything before
+      phase_start = *std::max_element(finish.begin(), finish.begin() + (std::ptrdiff_t)i + 1);
+      std::fill(avail.begin(), avail.end(), phase_start);
+      phase0 = i + 1;
+    }
+  }
+  return U;
+}
+
+// One brush geometry of a list: its padded footprint on the device, its side
+// and radius, and (for the brush's state after the list) the unpadded
+// footprint it came from; core == nullptr: the brush's current geometry.
+struct ListGeo {
+  const float* fp;
+  int size_map;
+  double radius;
+  const float* core;
+  int width;
+};
+
+// Stamps [0, n) of a list on the fused path.  Stamp k uses geometry
+// geo[k]; clear[k] cleans the pickup map before it (a geometry change).
+template <int kT>
+void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n,
+                    const PlanDev& D, const std::vector<ListGeo>& geos, const std::vector<int>& geo,
+                    const std::vector<uint8_t>& clear) {
+  hipStream_t s = b->ctx->stream;
+  FArgs A0;
+  A0.st = D.st;
+  A0.canvas = cv->layer();
+  A0.snap = b->use_snap ? b->snap_planes() : cv->layer();
+  A0.dirty = b->dirty;
+  A0.dl = D.dl;
+  A0.dl_off = D.dl_off;
+  A0.use_snap = b->use_snap ? 1 : 0;
+  A0.rows = cv->rows;
+  A0.cols = cv->cols;
+  A0.pickup_rate = b->pickup_rate;
+  A0.dep_rate = b->dep_rate;
+  A0.cap = b->cap;
+  std::vector<FArgs> GA(geos.size(), A0);
+  std::vector<int64_t> gstride(geos.size());
+  int64_t max_stride = 0;
+  for (size_t i = 0; i < geos.size(); ++i) {
+    FArgs& a = GA[i];
+    a.fp = geos[i].fp;
+    a.size_map = geos[i].size_map;
+    a.hr = (a.size_map - 1) / 2;
+    const int side = 2 * a.hr + 1;
+    a.tiles_x = (side + kT - 1) / kT;
+    a.ntiles = a.tiles_x * a.tiles_x;
+    gstride[i] = round_up((int64_t)a.size_map * a.size_map, 64);
+    max_stride = std::max(max_stride, gstride[i]);
+  }
+
+  // one stamp on stream `q` with pickup map `pick` (7 planes of its stride)
+  auto launch_stamp = [&](int64_t k, hipStream_t q, float* pick) {
+    FArgs a = GA[(size_t)geo[k]];
+    const int64_t ms = gstride[(size_t)geo[k]];
+    for (int c = 0; c < 7; ++c) a.pick.p[c] = pick + c * ms;
+    if (clear[k] || (st[k].flags & PNT_STAMP_DIP))  // setRadius / dip(): a clean pickup map
+      launch_fill(pick, 7 * ms, 0.0f, q);
+    if (st[k].flags & PNT_STAMP_DIP)
       for (int c = 0; c < 3; ++c) {
         b->paintK[c] = st[k].K[c];
         b->paintS[c] = st[k].S[c];
       }
-    }
     for (int c = 0; c < 3; ++c) {
-      A.pk[c] = b->paintK[c];
-      A.ps[c] = b->paintS[c];
+      a.pk[c] = b->paintK[c];
+      a.ps[c] = b->paintS[c];
     }
     const PStamp& p = D.host[k];
-    const int rb = A.use_snap ? ring_blocks_for(p.box, kT * kT) : 0;
+    const int rb = a.use_snap ? ring_blocks_for(p.box, kT * kT) : 0;
     const int jk = (int)k;
     if (!(p.flags & (kStDupX | kStDupY))) {
-      k_stamp_fused<kT><<<A.ntiles + rb, kT * kT, 0, s>>>(A, jk, A.ntiles, rb);
+      k_stamp_fused<kT><<<a.ntiles + rb, kT * kT, 0, q>>>(a, jk, a.ntiles, rb);
     } else {  // snapshot refresh first: a duplicated pixel is not refreshed by its visits
-      if (rb) k_stamp_fused<kT><<<rb, kT * kT, 0, s>>>(A, jk, 0, rb);
-      k_stamp_fused<kT><<<A.ntiles, kT * kT, 0, s>>>(A, jk, A.ntiles, 0);
-      if (p.dl_nlev) k_stamp_levels<<<1, 256, 0, s>>>(A, jk);
+      if (rb) k_stamp_fused<kT><<<rb, kT * kT, 0, q>>>(a, jk, 0, rb);
+      k_stamp_fused<kT><<<a.ntiles, kT * kT, 0, q>>>(a, jk, a.ntiles, 0);
+      if (p.dl_nlev) k_stamp_levels<<<1, 256, 0, q>>>(a, jk);
     }
     PNT_HIP_CHECK(hipGetLastError());
-    if (st[k].flags & PNT_STAMP_DRY_AFTER) {
-      if (b->use_snap)
-        launch_dry_dirty(cv->n, cv->layer(), cv->R0(), cv->h(), b->dirty, s);
-      else
-        launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), s);
+  };
+  auto dry = [&](hipStream_t q) {
+    if (b->use_snap)
+      launch_dry_dirty(cv->n, cv->layer(), cv->R0(), cv->h(), b->dirty, q);
+    else
+      launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), q);
+  };
+  // the brush ends with the geometry of the list's last stamp
+  auto finish_geometry = [&]() {
+    const ListGeo& g = geos[(size_t)geo[(size_t)n - 1]];
+    if (g.core) brush_set_geometry(b, g.radius, g.core, g.width);  // clears b->pick
+  };
+
+  const bool one_geo = geos.size() == 1 && !geos[0].core;
+  const int nchains = fused_chains();
+  std::vector<Unit> U;
+  if (nchains > 1 || !one_geo) U = plan_units(st, D.host, clear.data(), n, nchains);
+  int used = 1;
+  for (const Unit& u : U) used = std::max(used, u.chain + 1);
+  if (used <= 1 && one_geo) {  // one chain, the brush's own map: the list in stream order
+    for (int64_t k = 0; k < n; ++k) {
+      launch_stamp(k, s, b->pick);
+      if (st[k].flags & PNT_STAMP_DRY_AFTER) dry(s);
     }
+  } else {
+    if (!b->fused) b->fused = new FusedState;
+    FusedState& F = *b->fused;
+    hipStream_t S[kMaxChains];
+    float* pick[kMaxChains];
+    for (int c = 0; c < used; ++c) {
+      if (c > 0 && !F.streams[c])
+        PNT_HIP_CHECK(hipStreamCreateWithFlags(&F.streams[c], hipStreamNonBlocking));
+      if (F.pick_cap[c] < 7 * max_stride) {
+        if (c > 0) PNT_HIP_CHECK(hipStreamSynchronize(F.streams[c]));
+        PNT_HIP_CHECK(hipStreamSynchronize(s));
+        dev_free(F.pick[c]);
+        F.pick[c] = nullptr;
+        F.pick[c] = dev_alloc(7 * max_stride);
+        F.pick_cap[c] = 7 * max_stride;
+      }
+      S[c] = c == 0 ? s : F.streams[c];
+      pick[c] = F.pick[c];
+    }
+    const size_t nev = 2 * U.size() + 2;  // units, dries, start
+    while (F.events.size() < nev) {
+      hipEvent_t e;
+      PNT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      F.events.push_back(e);
+    }
+    size_t next_ev = 0;
+    auto record = [&](hipStream_t q) {
+      hipEvent_t e = F.events[next_ev++];
+      PNT_HIP_CHECK(hipEventRecord(e, q));
+      return e;
+    };
+    // a first unit that continues the brush's stroke starts from its map
+    if (!clear[0] && !(st[0].flags & PNT_STAMP_DIP))
+      PNT_HIP_CHECK(hipMemcpyAsync(pick[0], b->pick, 7 * gstride[(size_t)geo[0]] * sizeof(float),
+                                   hipMemcpyDeviceToDevice, s));
+    // the plan and everything before this call are on s
+    const hipEvent_t ready = record(s);
+    for (int c = 1; c < used; ++c) PNT_HIP_CHECK(hipStreamWaitEvent(S[c], ready, 0));
+    std::vector<hipEvent_t> done(U.size(), nullptr);
+    std::vector<int> last_unit(used, -1);  // per chain, in this phase
+    for (size_t i = 0; i < U.size(); ++i) {
+      const Unit& u = U[i];
+      hipStream_t q = S[u.chain];
+      for (int d : u.deps)
+        if (U[d].chain != u.chain) PNT_HIP_CHECK(hipStreamWaitEvent(q, done[d], 0));
+      for (int64_t k = u.s0; k < u.s1; ++k) launch_stamp(k, q, pick[u.chain]);
+      done[i] = record(q);
+      last_unit[u.chain] = (int)i;
+      if (u.dry_after) {  // join every chain on s, dry, fork again
+        for (int c = 1; c < used; ++c)
+          if (last_unit[c] >= 0) PNT_HIP_CHECK(hipStreamWaitEvent(s, done[last_unit[c]], 0));
+        dry(s);
+        const hipEvent_t dried = record(s);
+        for (int c = 1; c < used; ++c) PNT_HIP_CHECK(hipStreamWaitEvent(S[c], dried, 0));
+        std::fill(last_unit.begin(), last_unit.end(), -1);
+      }
+    }
+    for (int c = 1; c < used; ++c)
+      if (last_unit[c] >= 0) PNT_HIP_CHECK(hipStreamWaitEvent(s, done[last_unit[c]], 0));
+    finish_geometry();
+    // the pickup map of the list's last stroke is the brush's from now on
+    PNT_HIP_CHECK(hipMemcpyAsync(b->pick, pick[U.back().chain],
+                                 7 * gstride[(size_t)geo[(size_t)n - 1]] * sizeof(float),
+                                 hipMemcpyDeviceToDevice, s));
   }
   cv->epoch++;
   if (b->use_snap) {
     b->dirty_canvas = cv;
     b->dirty_epoch = cv->epoch;
   }
+}
+
+template <int kT>
+void run_fused_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
+  const PlanDev D = upload_plan(b, cv, st, n);
+  const std::vector<ListGeo> geos{ListGeo{b->fp, b->size_map, b->radius, nullptr, b->width}};
+  run_fused_list<kT>(b, cv, st, n, D, geos, std::vector<int>((size_t)n, 0),
+                     std::vector<uint8_t>((size_t)n, 0));
+}
+
+// A stroke list: each stroke is FootprintBrush::setRadius(radius) with its
+// footprint (a no-op within 0.5 of the current radius, FootprintBrush.hxx:
+// 46-50), then its stamps.
+template <int kT>
+void run_fused_strokes_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* S, int64_t ns) {
+  std::vector<ListGeo> geos{ListGeo{b->fp, b->size_map, b->radius, nullptr, b->width}};
+  std::vector<float> fps;
+  std::vector<int64_t> fp_off{0};
+  std::vector<PlanPart> parts;
+  std::vector<pnt_stamp> all;
+  std::vector<int> geo;
+  std::vector<uint8_t> clear;
+  int cur = 0;
+  double cur_r = b->radius;
+  bool pending_clear = false;  // setRadius cleaned the map; the next stamp starts from it
+  for (int64_t i = 0; i < ns; ++i) {
+    if (!(std::fabs(cur_r - S[i].radius) < 0.5)) {  // a new geometry, a clean map
+      int sm = 0;
+      std::vector<float> full = padded_footprint(S[i].radius, S[i].footprint, S[i].width, &sm);
+      fp_off.push_back((int64_t)fps.size());
+      fps.insert(fps.end(), full.begin(), full.end());
+      fps.resize((size_t)round_up((int64_t)fps.size(), 64), 0.0f);
+      geos.push_back(ListGeo{nullptr, sm, S[i].radius, S[i].footprint, S[i].width});
+      cur = (int)geos.size() - 1;
+      cur_r = S[i].radius;
+      pending_clear = true;
+    }
+    if (S[i].n_stamps == 0) continue;
+    clear.push_back(pending_clear ? 1 : 0);
+    clear.insert(clear.end(), (size_t)S[i].n_stamps - 1, 0);
+    pending_clear = false;
+    parts.push_back(PlanPart{geos[(size_t)cur].size_map, geos[(size_t)cur].radius, S[i].stamps,
+                             S[i].n_stamps});
+    all.insert(all.end(), S[i].stamps, S[i].stamps + S[i].n_stamps);
+    geo.insert(geo.end(), (size_t)S[i].n_stamps, cur);
+  }
+  const int64_t n = (int64_t)all.size();
+  if (n > 0) {
+    const float* d_fp = nullptr;
+    const PlanDev D = upload_plan_parts(b, cv, parts.data(), (int)parts.size(), fps, &d_fp);
+    for (size_t g = 1; g < geos.size(); ++g) geos[g].fp = d_fp + fp_off[g];
+    run_fused_list<kT>(b, cv, all.data(), n, D, geos, geo, clear);
+  }
+  // a geometry change after the last stamp (strokes without stamps)
+  if (cur != (n > 0 ? geo.back() : 0))
+    brush_set_geometry(b, geos[(size_t)cur].radius, geos[(size_t)cur].core, geos[(size_t)cur].width);
 }
 
 // ------------------------------------------------------------- chained --
@@ -464,6 +743,15 @@ void run_chained(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t 
     run_chained_t<16>(b, cv, st, n);
   else
     run_chained_t<32>(b, cv, st, n);
+}
+
+void run_fused_strokes(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* strokes, int64_t ns) {
+  int kT = 16;
+  if (const char* e = std::getenv("PAINTY_FUSED_TILE")) kT = std::atoi(e) == 32 ? 32 : 16;
+  if (kT == 32)
+    run_fused_strokes_t<32>(b, cv, strokes, ns);
+  else
+    run_fused_strokes_t<16>(b, cv, strokes, ns);
 }
 
 void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {

@@ -20,6 +20,7 @@ float* dev_alloc(int64_t nfloats);
 void dev_free(void* p);
 struct PersistentState;
 struct ChainState;
+struct FusedState;
 }  // namespace pnt
 
 struct pnt_ctx_s {
@@ -121,6 +122,7 @@ struct pnt_brush_s {
   uint64_t dirty_epoch = 0;
   pnt::PersistentState* pers = nullptr;
   pnt::ChainState* chain = nullptr;  // buffers of PNT_EXEC_CHAINED runs
+  pnt::FusedState* fused = nullptr;  // streams and maps of concurrent fused strokes
 
   pnt::Layer7 pick_planes() const {
     pnt::Layer7 L;
@@ -159,6 +161,9 @@ void brush_prepare_snapshot(pnt_brush b, pnt_canvas cv, hipStream_t s);
 // FootprintBrush::setRadius geometry with an injected width x width footprint
 // (api.hip): new footprint and a cleared pickup map.
 void brush_set_geometry(pnt_brush b, double radius, const float* core, int width);
+// The footprint padded with zeros to size_map^2 (FootprintBrush.hxx:52-58,
+// padded to exactly sizeMap), after the checks brush_set_geometry makes.
+std::vector<float> padded_footprint(double radius, const float* core, int width, int* size_map);
 // A canvas (cleared, background 1) on a caller buffer of at least
 // 11*round_up(rows*cols, 64) floats that pnt_canvas_destroy does not free.
 pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base);

@@ -314,10 +314,29 @@ void launch_dry_dirty(int64_t n, const Layer7& L, const Rgb3& R0, float* h, uint
 
 void plan_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t n,
                  std::vector<PStamp>& h_st, std::vector<int2>& h_dl, std::vector<int>& h_off) {
+  plan_stamps_geom(b->size_map, b->radius, cv, st, n, h_st, h_dl, h_off);
+}
+
+void plan_stamps_geom(int size_map, double radius, pnt_canvas cv, const pnt_stamp* st, int64_t n,
+                      std::vector<PStamp>& h_st, std::vector<int2>& h_dl, std::vector<int>& h_off) {
   std::vector<int2> list;
   std::vector<int> off;
   for (int64_t k = 0; k < n; ++k) {
-    const StampArgs a = stamp_args(b, cv, st[k].cx, st[k].cy, st[k].theta);
+    const double cx = st[k].cx, cy = st[k].cy, theta = st[k].theta;
+    PNT_REQUIRE(std::isfinite(cx) && std::isfinite(cy) && std::isfinite(theta) &&
+                    std::fabs(cx) < 1e9 && std::fabs(cy) < 1e9,
+                "stamp centre/angle must be finite and |centre| < 1e9");
+    StampArgs a{};
+    a.g.cx = cx;
+    a.g.cy = cy;
+    a.g.c = std::cos(-theta);
+    a.g.s = std::sin(-theta);
+    a.g.hr = (size_map - 1) / 2;
+    a.g.size_map = size_map;
+    a.g.rows = cv->rows;
+    a.g.cols = cv->cols;
+    a.dupx = stamp_dup_axis(cx, a.g.hr) ? 1 : 0;
+    a.dupy = stamp_dup_axis(cy, a.g.hr) ? 1 : 0;
     PStamp p{};
     p.cx = a.g.cx;
     p.cy = a.g.cy;
@@ -330,7 +349,7 @@ void plan_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t n,
     p.flags = (st[k].flags & PNT_STAMP_DIP) ? kStDip : 0u;
     if (a.dupx) p.flags |= kStDupX;
     if (a.dupy) p.flags |= kStDupY;
-    p.box = ring_box(a.g, b->radius);
+    p.box = ring_box(a.g, radius);
     if (a.dupx || a.dupy) {
       plan_shared_visits(a, list, off);
       if (!list.empty()) {
@@ -370,6 +389,8 @@ void brush_prepare_snapshot(pnt_brush b, pnt_canvas cv, hipStream_t s) {
 struct PersistentState {
   PStamp* d_st = nullptr;
   int64_t st_cap = 0;
+  float* d_fp = nullptr;  // footprints of a stroke list's geometries
+  int64_t fp_cap = 0;
   int2* d_dl = nullptr;
   int64_t dl_cap = 0;
   int* d_off = nullptr;
@@ -391,6 +412,7 @@ void persistent_release(PersistentState*& p) {
   (void)hipFree(p->d_st);
   (void)hipFree(p->d_dl);
   (void)hipFree(p->d_off);
+  (void)hipFree(p->d_fp);
   (void)hipFree(p->d_bar);
   if (p->stage) (void)hipHostFree(p->stage);
   if (p->uploaded) (void)hipEventDestroy(p->uploaded);
@@ -423,17 +445,45 @@ static PersistentState& plan_state(pnt_brush_s* b) {
 // and uploads the plan through pinned staging.  The host copy of the plan
 // stays in the brush's state until the next call.
 PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
+  const PlanPart part{b->size_map, b->radius, st, n};
+  return upload_plan_parts(b, cv, &part, 1, {}, nullptr);
+}
+
+PlanDev upload_plan_parts(pnt_brush_s* b, pnt_canvas_s* cv, const PlanPart* parts, int np,
+                          const std::vector<float>& fps, const float** d_fp) {
   hipStream_t s = b->ctx->stream;
   PersistentState& P = plan_state(b);
   P.h_st.clear();
   P.h_dl.clear();
   P.h_off.clear();
-  plan_stamps(b, cv, st, n, P.h_st, P.h_dl, P.h_off);
+  if (np == 1) {
+    plan_stamps_geom(parts[0].size_map, parts[0].radius, cv, parts[0].st, parts[0].n, P.h_st,
+                     P.h_dl, P.h_off);
+  } else {  // parts planned on their own, then concatenated
+    std::vector<PStamp> pst;
+    std::vector<int2> pdl;
+    std::vector<int> poff;
+    for (int i = 0; i < np; ++i) {
+      pst.clear();
+      pdl.clear();
+      poff.clear();
+      plan_stamps_geom(parts[i].size_map, parts[i].radius, cv, parts[i].st, parts[i].n, pst, pdl,
+                       poff);
+      const int off0 = (int)P.h_off.size(), dl0 = (int)P.h_dl.size();
+      for (PStamp p : pst) {
+        if (p.dl_nlev > 0) p.dl_lev += off0;
+        P.h_st.push_back(p);
+      }
+      for (int o : poff) P.h_off.push_back(o + dl0);
+      P.h_dl.insert(P.h_dl.end(), pdl.begin(), pdl.end());
+    }
+  }
   brush_prepare_snapshot(b, cv, s);
   const size_t bst = P.h_st.size() * sizeof(PStamp);
   const size_t bdl = P.h_dl.size() * sizeof(int2);
   const size_t boff = P.h_off.size() * sizeof(int);
-  const int64_t need = (int64_t)(bst + bdl + boff);
+  const size_t bfp = fps.size() * sizeof(float);
+  const int64_t need = (int64_t)(bst + bdl + boff + bfp);
   PNT_HIP_CHECK(hipEventSynchronize(P.uploaded));  // previous plan has been read
   if (need > P.stage_cap) {
     if (P.stage) PNT_HIP_CHECK(hipHostFree(P.stage));
@@ -445,6 +495,8 @@ PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
   std::memcpy(P.stage, P.h_st.data(), bst);
   if (bdl) std::memcpy(P.stage + bst, P.h_dl.data(), bdl);
   if (boff) std::memcpy(P.stage + bst + bdl, P.h_off.data(), boff);
+  if (bfp) std::memcpy(P.stage + bst + bdl + boff, fps.data(), bfp);
+  const int64_t n = (int64_t)P.h_st.size();
   ensure_dev(P.d_st, P.st_cap, std::max<int64_t>(n, 1), s);
   ensure_dev(P.d_dl, P.dl_cap, std::max<int64_t>((int64_t)P.h_dl.size(), 1), s);
   ensure_dev(P.d_off, P.off_cap, std::max<int64_t>((int64_t)P.h_off.size(), 1), s);
@@ -453,6 +505,12 @@ PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
     PNT_HIP_CHECK(hipMemcpyAsync(P.d_dl, P.stage + bst, bdl, hipMemcpyHostToDevice, s));
   if (boff)
     PNT_HIP_CHECK(hipMemcpyAsync(P.d_off, P.stage + bst + bdl, boff, hipMemcpyHostToDevice, s));
+  if (bfp) {
+    ensure_dev(P.d_fp, P.fp_cap, (int64_t)fps.size(), s);
+    PNT_HIP_CHECK(hipMemcpyAsync(P.d_fp, P.stage + bst + bdl + boff, bfp, hipMemcpyHostToDevice,
+                                 s));
+  }
+  if (d_fp) *d_fp = P.d_fp;
   PNT_HIP_CHECK(hipEventRecord(P.uploaded, s));
   return PlanDev{P.d_st, P.d_dl, P.d_off, P.h_st.data(), P.h_off.data()};
 }

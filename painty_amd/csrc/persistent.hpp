@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "painty_hip.h"
 
 struct pnt_brush_s;
@@ -26,11 +28,25 @@ struct PlanDev {
   const int* host_off;  // host copy of dl_off
 };
 PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
+// A list cut into parts that each have their own brush geometry (padded
+// footprint side, radius); `fps` (footprints of the list's geometries)
+// travels with the plan, to *d_fp.
+struct PlanPart {
+  int size_map;
+  double radius;
+  const pnt_stamp* st;
+  int64_t n;
+};
+PlanDev upload_plan_parts(pnt_brush_s* b, pnt_canvas_s* cv, const PlanPart* parts, int np,
+                          const std::vector<float>& fps, const float** d_fp);
 
 void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
+void run_fused_strokes(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* strokes, int64_t ns);
 void run_chained(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 struct ChainState;
 void chain_release(ChainState*& c);
+struct FusedState;
+void fused_release(FusedState*& f);
 void run_persistent(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 void persistent_release(PersistentState*& p);
 

@@ -440,5 +440,9 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
 // h_dl / h_off (offsets are global indices into h_dl).
 void plan_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t n,
                  std::vector<PStamp>& h_st, std::vector<int2>& h_dl, std::vector<int>& h_off);
+// The same for a brush geometry given explicitly (padded footprint side and
+// radius), so that a list can change geometry between strokes.
+void plan_stamps_geom(int size_map, double radius, pnt_canvas cv, const pnt_stamp* st, int64_t n,
+                      std::vector<PStamp>& h_st, std::vector<int2>& h_dl, std::vector<int>& h_off);
 
 }  // namespace pnt

@@ -323,6 +323,11 @@ class FootprintBrush:
         fp = _f32(footprint)
         call("pnt_brush_set_radius", self.h, float(radius), _pf(fp), int(fp.shape[0]))
 
+    def getSizeMap(self) -> int:
+        v = C.c_int()
+        call("pnt_brush_size_map", self.h, C.byref(v))
+        return v.value
+
     @property
     def sizeMap(self) -> int:
         v = C.c_int()
@@ -383,6 +388,23 @@ class FootprintBrush:
     def imprintStamps(self, stamps, canvas: Canvas, mode: int = _lib.EXEC_PLAIN) -> None:
         arr = stamps_to_c(stamps)
         call("pnt_imprint_stamps", self.h, canvas.h, arr, len(arr), int(mode))
+
+    def imprintStrokes(self, strokes, canvas: Canvas, mode: int = _lib.EXEC_FUSED) -> None:
+        """pnt_imprint_strokes over [(radius, footprint, stamps), ...]: per
+        stroke setRadius(radius, footprint), then its stamps."""
+        n = len(strokes)
+        arr = (_lib.PntStroke * n)()
+        keep = []
+        for k, (radius, fp, stamps) in enumerate(strokes):
+            f = _f32(fp)
+            st = stamps_to_c(stamps)
+            keep.append((f, st))
+            arr[k].radius = float(radius)
+            arr[k].footprint = _pf(f)
+            arr[k].width = int(f.shape[0])
+            arr[k].stamps = C.cast(st, C.POINTER(PntStamp)) if len(st) else None
+            arr[k].n_stamps = len(st)
+        call("pnt_imprint_strokes", self.h, canvas.h, arr, n, int(mode))
 
     def paintStroke(self, path, canvas: Canvas) -> None:
         p = np.ascontiguousarray(path, dtype=np.float64).reshape(-1, 2)

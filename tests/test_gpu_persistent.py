@@ -157,3 +157,87 @@ def test_planned_paths_equal_plain_bitwise_on_one_stroke(hip, lmode):
         x = getattr(a.getPaintLayer(), get)()
         y = getattr(b.getPaintLayer(), get)()
         assert np.array_equal(x, y), get
+
+
+@pytest.mark.parametrize("chains", [2, 4, 8])
+def test_fused_concurrent_strokes_equal_one_chain(hip, oracle, chains, monkeypatch):
+    """Strokes with disjoint snapshot boxes run on concurrent streams, each
+    with its own pickup map: the result equals the one-stream run to the
+    bit (and the oracle), with drying between phases and the last stroke's
+    pickup map left in the brush."""
+    wl = W.cfg2(seed=7, size=1024, n_stamps=400, dry_every=130, foot=(17, 41))
+    monkeypatch.setenv("PAINTY_FUSED_CHAINS", "1")
+    a, ab = run_gpu(wl, mode=FUSED)
+    monkeypatch.setenv("PAINTY_FUSED_CHAINS", str(chains))
+    b, bb = run_gpu(wl, mode=FUSED)
+    for get in ("getK_buffer", "getS_buffer", "getV_buffer"):
+        assert np.array_equal(getattr(a.getPaintLayer(), get)(), getattr(b.getPaintLayer(), get)()), get
+    assert np.array_equal(a.getR0(), b.getR0()) and np.array_equal(a.get_h(), b.get_h())
+    for x, y in zip(ab.getPickupMap(), bb.getPickupMap()):
+        assert np.array_equal(x, y)
+    ocv, ob = run_oracle(wl)
+    compare_canvas("fused%d" % chains, b, ocv, pickup=(bb, ob))
+
+
+def _strokes(wl):
+    from tests.parity import _groups
+    return [(wl.radii[bid], wl.cores[bid], g) for bid, g in _groups(wl.stamps)]
+
+
+def _fresh(hip, wl):
+    from painty_amd import hip as H
+    cv = H.Canvas(wl.rows, wl.cols)
+    cv.getPaintLayer().setBuffers(wl.wet.K, wl.wet.S, wl.wet.V)
+    cv.setR0(wl.R0)
+    dry = H.PaintLayer(wl.rows, wl.cols)
+    dry.setBuffers(wl.dry.K, wl.dry.S, wl.dry.V)
+    dry.composeOntoCanvas(cv)
+    br = H.FootprintBrush(wl.radii[0], wl.cores[0])
+    br.setPickupRate(wl.pickup_rate)
+    br.setDepositionRate(wl.deposition_rate)
+    return cv, br
+
+
+def _same(a, ab, b, bb):
+    for get in ("getK_buffer", "getS_buffer", "getV_buffer"):
+        assert np.array_equal(getattr(a.getPaintLayer(), get)(), getattr(b.getPaintLayer(), get)()), get
+    assert np.array_equal(a.getR0(), b.getR0()) and np.array_equal(a.get_h(), b.get_h())
+    assert ab.getSizeMap() == bb.getSizeMap()
+    for x, y in zip(ab.getPickupMap(), bb.getPickupMap()):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("chains", [1, 4])
+def test_imprint_strokes_equals_stroke_by_stroke(hip, oracle, chains, monkeypatch):
+    """pnt_imprint_strokes (one call, per-stroke footprints, concurrent
+    strokes) equals setRadius + imprintStamps stroke by stroke, to the bit,
+    and the oracle."""
+    wl = W.cfg2(seed=8, size=1024, n_stamps=400, dry_every=130, foot=(17, 41))
+    monkeypatch.setenv("PAINTY_FUSED_CHAINS", str(chains))
+    a, ab = run_gpu(wl, mode=FUSED)
+    b, bb = _fresh(hip, wl)
+    bb.imprintStrokes(_strokes(wl), b, FUSED)
+    _same(a, ab, b, bb)
+    ocv, ob = run_oracle(wl)
+    compare_canvas("strokes%d" % chains, b, ocv, pickup=(bb, ob))
+
+
+@pytest.mark.parametrize("mode", [PLAIN, FUSED])
+def test_imprint_strokes_geometry_rules(hip, oracle, mode):
+    """Radius changes within 0.5 keep the footprint and the pickup map; a
+    stroke without stamps still changes the geometry (and cleans the map);
+    the brush ends with the last stroke's geometry."""
+    wl = W.small(70, 80, 30, 5, 12, pickup=True, noisy=True, reach=0.3)
+    st = wl.stamps.copy()
+    st["flags"] = 0
+    st["flags"][0] = W.DIP
+    c5, c9 = W.gaussian_core(11, 3.0), W.gaussian_core(19, 5.0)
+    strokes = [(5.0, c5, st[:8]), (5.3, c5, st[8:14]), (9.0, c9, st[:0]), (9.2, c9, st[14:22]),
+               (5.0, c5, st[22:26]), (9.0, c9, st[:0])]
+    a, ab = _fresh(hip, wl)
+    for radius, core, part in strokes:  # FootprintBrush::setRadius + imprint, one by one
+        ab.setRadius(radius, core)
+        ab.imprintStamps(part, a, PLAIN)
+    b, bb = _fresh(hip, wl)
+    bb.imprintStrokes(strokes, b, mode)
+    _same(a, ab, b, bb)
