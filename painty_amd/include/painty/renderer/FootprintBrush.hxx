@@ -143,6 +143,30 @@ class FootprintBrush final : public BrushBase<vector_type> {
                                   mode));
     canvas.invalidate();
   }
+  // A painting session in one call: per stroke setRadius(radius, footprint)
+  // then its stamps (pnt_imprint_strokes); under PNT_EXEC_FUSED strokes whose
+  // snapshot boxes do not meet run concurrently, with the same result.
+  struct Stroke {
+    double radius;
+    Mat<double> footprint;
+    std::vector<pnt_stamp> stamps;
+  };
+  void imprintStrokes(const std::vector<Stroke>& strokes, Canvas<vector_type>& canvas,
+                      int mode = PNT_EXEC_FUSED) {
+    std::vector<std::vector<float>> fps;
+    std::vector<pnt_stroke> list;
+    fps.reserve(strokes.size());
+    for (const Stroke& s : strokes) {
+      check_square(s.footprint);
+      fps.push_back(gpu::to_f32(s.footprint.data(), s.footprint.total()));
+      list.push_back(pnt_stroke{s.radius, fps.back().data(), s.footprint.rows, s.stamps.data(),
+                                (int64_t)s.stamps.size()});
+    }
+    canvas.flush();
+    gpu::check(pnt_imprint_strokes(_h, canvas.handle(), list.data(), (int64_t)list.size(), mode));
+    canvas.invalidate();
+    _fp_valid = false;
+  }
   pnt_brush handle() const { return _h; }
   static void setFootprintSource(FootprintSource s) { source() = std::move(s); }
   // The reference's footprint width for a radius (FootprintBrush.hxx:50).

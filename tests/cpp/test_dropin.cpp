@@ -197,6 +197,48 @@ int api() {
     EXPECT(tc.getTimeMap()[1 * 5 + 2] == t0 - std::chrono::nanoseconds(
                t0.time_since_epoch().count() % 1000));  // microsecond time points
   }
+  // a stroke list in one call equals setRadius + imprint stroke by stroke
+  {
+    auto paint = [](Canvas<vec3>& c) {
+      for (int i = 0; i < 30 * 40; ++i) {
+        c.getPaintLayer().getK_buffer()(i) = vec3(0.3 + 0.001 * (i % 7), 0.5, 0.9);
+        c.getPaintLayer().getS_buffer()(i) = vec3(0.7, 0.2 + 0.002 * (i % 5), 0.4);
+        c.getPaintLayer().getV_buffer()(i) = 0.05 * (i % 3);
+      }
+    };
+    Canvas<vec3> c1(30, 40), c2(30, 40);
+    c1.useClock(PNT_CLOCK_OFF);
+    c2.useClock(PNT_CLOCK_OFF);
+    paint(c1);
+    paint(c2);
+    const Mat<double> f3(7, 7, 0.8), f5(11, 11, 0.6);
+    auto stamp = [](double x, double y, double th, bool dip) {
+      pnt_stamp s{};
+      s.cx = x;
+      s.cy = y;
+      s.theta = th;
+      s.K[0] = s.K[1] = s.K[2] = 0.4f;
+      s.S[0] = s.S[1] = s.S[2] = 0.9f;
+      s.flags = dip ? PNT_STAMP_DIP : 0u;
+      return s;
+    };
+    std::vector<FootprintBrush<vec3>::Stroke> strokes{
+        {3.0, f3, {stamp(8, 8, 0.2, true), stamp(10, 9, 0.3, false)}},
+        {5.0, f5, {stamp(25, 20, 1.0, true), stamp(27, 21, 1.1, false), stamp(29, 22, 1.2, false)}}};
+    FootprintBrush<vec3> b1(3.0, f3), b2(3.0, f3);
+    b1.imprintStrokes(strokes, c1);
+    for (const auto& st : strokes) {
+      b2.setRadius(st.radius, st.footprint);
+      b2.imprintStamps(st.stamps, c2, PNT_EXEC_PLAIN);
+    }
+    const Canvas<vec3>& k1 = c1;
+    const Canvas<vec3>& k2 = c2;
+    for (int i = 0; i < 30 * 40; ++i) {
+      EXPECT(k1.getPaintLayer().getV_buffer()(i) == k2.getPaintLayer().getV_buffer()(i));
+      EXPECT(k1.getPaintLayer().getK_buffer()(i)[1] == k2.getPaintLayer().getK_buffer()(i)[1]);
+    }
+    EXPECT(b1.getFootprint().rows == b2.getFootprint().rows);
+  }
   // errors map to the reference's exception types
   EXPECT(throws_invalid([] { Canvas<vec3> bad(-1, 4); }));
   EXPECT(throws_invalid([] { FootprintBrush<vec3> b(3.0, Mat<double>(6, 6, 0.5)); }));
