@@ -333,6 +333,26 @@ int pnt_brush_stamp_traffic(pnt_brush brush, int rows, int cols,
 /* mixSinglePaint(weights) -- mixer/src/PaintMixer.cxx:327-355: K = sum_l
  * norm*w_l*K_l (same for S), norm = 1/sum(w) unless |sum(w)-1| < 1e-5.
  * base_K / base_S: n_palette x 3; PNT_EINVAL when n_weights != n_palette. */
+/* ComputeScatteringAndAbsorption -- KubelkaMunk.hxx:84-124 (Curtis et al.
+ * 1997): K and S of n colours from their reflectance over black (Rb) and
+ * over white (Rw), AoS n*3 doubles each, computed on the device in FP64.
+ * Fails with PNT_EINVAL unless 0 < Rb < Rw < 1 on every channel (the
+ * reference throws std::invalid_argument). */
+int pnt_compute_scattering_absorption(pnt_ctx ctx, int64_t n, const double* Rb,
+                                      const double* Rw, double* K, double* S);
+/* LoadPalette / SavePalette -- mixer/src/Serialization.cxx:26-53: the JSON
+ * array [{"K": [k0, k1, k2], "S": [s0, s1, s2]}, ...].  from_json reads up
+ * to `capacity` entries into K, S (AoS n*3) and sets *n (other keys are
+ * ignored, "K" and "S" are required); to_json writes the compact form with
+ * the shortest round-trip decimals, NUL-terminated, and sets *len (fails
+ * when capacity <= *len). */
+int pnt_palette_from_json(const char* text, int64_t len, double* K, double* S,
+                          int64_t capacity, int64_t* n);
+int pnt_palette_to_json(const double* K, const double* S, int64_t n, char* text,
+                        int64_t capacity, int64_t* len);
+/* getThinningMedium -- mixer/src/Palette.cxx:453-463: the single entry of the
+ * reference's data/paint_palettes/thinning_medium.json, K = S = 0.01. */
+void pnt_thinning_medium(double K[3], double S[3]);
 int pnt_mix_single(int n_palette, const double* base_K, const double* base_S,
                    int n_weights, const double* weights, double K[3],
                    double S[3]);

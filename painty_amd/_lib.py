@@ -140,6 +140,11 @@ SIGNATURES = {
     "pnt_brush_stamp_traffic": [_vp, C.c_int, C.c_int, C.POINTER(PntStamp), C.c_int64,
                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "pnt_mix_single": [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp],
+    "pnt_compute_scattering_absorption": [_vp, C.c_int64, _dp, _dp, _dp, _dp],
+    "pnt_palette_from_json": [C.c_char_p, C.c_int64, _dp, _dp, C.c_int64,
+                              C.POINTER(C.c_int64)],
+    "pnt_palette_to_json": [_dp, _dp, C.c_int64, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)],
+    "pnt_thinning_medium": [_dp, _dp],
     "pnt_mixed": [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp],
     "pnt_compute_reflectance": [_vp, C.c_int64, _fp, _fp, _fp, _fp, _fp],
     "pnt_dev_compose": [_vp, C.c_int64, DevLayer, DevRgb, DevRgb, _vp],

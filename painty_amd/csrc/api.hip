@@ -1543,6 +1543,71 @@ int pnt_paint_stroke(pnt_brush b, pnt_canvas cv, const double* p, int64_t n) {
 }
 
 // -------------------------------------------------------------- PaintMixer --
+int pnt_compute_scattering_absorption(pnt_ctx ctx, int64_t n, const double* Rb, const double* Rw,
+                                      double* K, double* S) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && (n == 0 || (Rb && Rw && K && S)), "NULL argument");
+    PNT_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return;
+    ctx->use();
+    hipStream_t s = ctx->stream;
+    const int64_t n3 = 3 * n;
+    double* buf = nullptr;
+    const size_t bytes = (size_t)(4 * n3) * sizeof(double) + 256;
+    PNT_HIP_CHECK(hipMalloc((void**)&buf, bytes));
+    try {
+      unsigned* bad = reinterpret_cast<unsigned*>(buf + 4 * n3);
+      launch_fill_u32(bad, 1, 0u, s);
+      PNT_HIP_CHECK(hipMemcpyAsync(buf, Rb, n3 * sizeof(double), hipMemcpyHostToDevice, s));
+      PNT_HIP_CHECK(hipMemcpyAsync(buf + n3, Rw, n3 * sizeof(double), hipMemcpyHostToDevice, s));
+      launch_scatter_absorb(n3, buf, buf + n3, buf + 2 * n3, buf + 3 * n3, bad, s);
+      unsigned h_bad = 0;
+      PNT_HIP_CHECK(hipMemcpyAsync(&h_bad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+      PNT_HIP_CHECK(hipMemcpyAsync(K, buf + 2 * n3, n3 * sizeof(double), hipMemcpyDeviceToHost, s));
+      PNT_HIP_CHECK(hipMemcpyAsync(S, buf + 3 * n3, n3 * sizeof(double), hipMemcpyDeviceToHost, s));
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+      PNT_REQUIRE(!h_bad, "on black or white inputs violate one of the following conditions: "
+                          "0 < black < white < 1");
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(buf);
+      throw;
+    }
+    PNT_HIP_CHECK(hipFree(buf));
+  });
+}
+
+int pnt_palette_from_json(const char* text, int64_t len, double* K, double* S, int64_t capacity,
+                          int64_t* n) {
+  return guard([&] {
+    PNT_REQUIRE(text && n && len >= 0 && capacity >= 0, "invalid argument");
+    std::vector<double> k, s;
+    palette_from_json(text, len, k, s);
+    *n = (int64_t)(k.size() / 3);
+    PNT_REQUIRE(*n <= capacity, "palette has more entries than the capacity given");
+    PNT_REQUIRE((K && S) || *n == 0, "NULL output");
+    if (*n) {
+      std::memcpy(K, k.data(), k.size() * sizeof(double));
+      std::memcpy(S, s.data(), s.size() * sizeof(double));
+    }
+  });
+}
+
+int pnt_palette_to_json(const double* K, const double* S, int64_t n, char* text, int64_t capacity,
+                        int64_t* len) {
+  return guard([&] {
+    PNT_REQUIRE(len && n >= 0 && (n == 0 || (K && S)), "invalid argument");
+    const std::string js = palette_to_json(K, S, n);
+    *len = (int64_t)js.size();
+    PNT_REQUIRE(text && capacity > *len, "text buffer too small (needs *len + 1 bytes)");
+    std::memcpy(text, js.c_str(), js.size() + 1);
+  });
+}
+
+void pnt_thinning_medium(double K[3], double S[3]) {
+  for (int c = 0; c < 3; ++c) K[c] = S[c] = 0.01;
+}
+
 int pnt_mix_single(int n_palette, const double* base_K, const double* base_S, int n_weights,
                    const double* w, double K[3], double S[3]) {
   return guard([&] {

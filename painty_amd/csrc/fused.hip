@@ -147,9 +147,12 @@ void fused_release(FusedState*& F) {
 
 namespace {
 
-int fused_chains() {
-  int c = 4;
-  if (const char* e = std::getenv("PAINTY_FUSED_CHAINS")) c = std::atoi(e);
+// Concurrent chains of one call: `want` if given (> 0), else
+// PAINTY_FUSED_CHAINS, else 4 (the hardware queues of a process).
+int fused_chains(int want) {
+  int c = want > 0 ? want : 4;
+  if (want <= 0)
+    if (const char* e = std::getenv("PAINTY_FUSED_CHAINS")) c = std::atoi(e);
   return std::min(std::max(c, 1), kMaxChains);
 }
 
@@ -238,7 +241,7 @@ struct ListGeo {
 template <int kT>
 void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n,
                     const PlanDev& D, const std::vector<ListGeo>& geos, const std::vector<int>& geo,
-                    const std::vector<uint8_t>& clear) {
+                    const std::vector<uint8_t>& clear, int want_chains) {
   hipStream_t s = b->ctx->stream;
   FArgs A0;
   A0.st = D.st;
@@ -309,7 +312,7 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
   };
 
   const bool one_geo = geos.size() == 1 && !geos[0].core;
-  const int nchains = fused_chains();
+  const int nchains = fused_chains(want_chains);
   std::vector<Unit> U;
   if (nchains > 1 || !one_geo) U = plan_units(st, D.host, clear.data(), n, nchains);
   int used = 1;
@@ -392,11 +395,11 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
 }
 
 template <int kT>
-void run_fused_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
+void run_fused_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n, int chains) {
   const PlanDev D = upload_plan(b, cv, st, n);
   const std::vector<ListGeo> geos{ListGeo{b->fp, b->size_map, b->radius, nullptr, b->width}};
   run_fused_list<kT>(b, cv, st, n, D, geos, std::vector<int>((size_t)n, 0),
-                     std::vector<uint8_t>((size_t)n, 0));
+                     std::vector<uint8_t>((size_t)n, 0), chains);
 }
 
 // A stroke list: each stroke is FootprintBrush::setRadius(radius) with its
@@ -440,7 +443,7 @@ void run_fused_strokes_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* S, 
     const float* d_fp = nullptr;
     const PlanDev D = upload_plan_parts(b, cv, parts.data(), (int)parts.size(), fps, &d_fp);
     for (size_t g = 1; g < geos.size(); ++g) geos[g].fp = d_fp + fp_off[g];
-    run_fused_list<kT>(b, cv, all.data(), n, D, geos, geo, clear);
+    run_fused_list<kT>(b, cv, all.data(), n, D, geos, geo, clear, 0);
   }
   // a geometry change after the last stamp (strokes without stamps)
   if (cur != (n > 0 ? geo.back() : 0))
@@ -754,15 +757,15 @@ void run_fused_strokes(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* strok
     run_fused_strokes_t<16>(b, cv, strokes, ns);
 }
 
-void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
+void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n, int chains) {
   if (n == 0) return;
   // window tile side: 16 (256 threads), or 32 from PAINTY_FUSED_TILE
   int kT = 16;
   if (const char* e = std::getenv("PAINTY_FUSED_TILE")) kT = std::atoi(e) == 32 ? 32 : 16;
   if (kT == 32)
-    run_fused_t<32>(b, cv, st, n);
+    run_fused_t<32>(b, cv, st, n, chains);
   else
-    run_fused_t<16>(b, cv, st, n);
+    run_fused_t<16>(b, cv, st, n, chains);
 }
 
 }  // namespace pnt

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <string>
 #include <vector>
 
 #include "internal.hpp"
@@ -164,6 +165,10 @@ void brush_set_geometry(pnt_brush b, double radius, const float* core, int width
 // The footprint padded with zeros to size_map^2 (FootprintBrush.hxx:52-58,
 // padded to exactly sizeMap), after the checks brush_set_geometry makes.
 std::vector<float> padded_footprint(double radius, const float* core, int width, int* size_map);
+// Palette JSON (mixer/src/Serialization.cxx:26-53), ingest.hip.
+void palette_from_json(const char* text, int64_t len, std::vector<double>& K,
+                       std::vector<double>& S);
+std::string palette_to_json(const double* K, const double* S, int64_t n);
 // A canvas (cleared, background 1) on a caller buffer of at least
 // 11*round_up(rows*cols, 64) floats that pnt_canvas_destroy does not free.
 pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base);

@@ -80,6 +80,10 @@ void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 void launch_fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t s);
 void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
 
+// ---- ingest (ingest.hip) ----
+void launch_scatter_absorb(int64_t n3, const double* Rb, const double* Rw, double* K, double* S,
+                           unsigned* bad, hipStream_t s);
+
 // ---- display (display.hip) ----
 void launch_render(int rows, int cols, const Layer7& L, const Rgb3& R0, const Rgb3& out,
                    hipStream_t s);

@@ -40,7 +40,8 @@ struct PlanPart {
 PlanDev upload_plan_parts(pnt_brush_s* b, pnt_canvas_s* cv, const PlanPart* parts, int np,
                           const std::vector<float>& fps, const float** d_fp);
 
-void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
+// chains: concurrent stroke chains at most (0: PAINTY_FUSED_CHAINS or 4)
+void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n, int chains = 0);
 void run_fused_strokes(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* strokes, int64_t ns);
 void run_chained(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 struct ChainState;

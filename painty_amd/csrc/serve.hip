@@ -225,7 +225,8 @@ void pnt_server_s::run(int slot, Req* q) {
   b->use_snap = q->r.use_snapshot != 0;
   for (int c = 0; c < 3; ++c) b->paintK[c] = b->paintS[c] = 0.0f;
   if (b->use_snap) brush_full_snapshot(b, cv);
-  run_fused(b, cv, q->stamps.data(), (int64_t)q->stamps.size());
+  // one chain: the worker slots already keep the device busy with other canvases
+  run_fused(b, cv, q->stamps.data(), (int64_t)q->stamps.size(), 1);
   float* comp = R->block + 11 * cv->stride;
   const Rgb3 out{{comp, comp + cv->stride, comp + 2 * cv->stride}};
   launch_compose(cv->n, cv->layer(), cv->R0(), out, s);

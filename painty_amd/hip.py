@@ -287,6 +287,51 @@ class Renderer:
         return out
 
 
+def compute_scattering_absorption(Rb, Rw, ctx: Context | None = None):
+    """ComputeScatteringAndAbsorption (Curtis inversion) of (..., 3) colours
+    over black and over white -> (K, S), float64, on the device."""
+    ctx = ctx or default_context()
+    b = np.ascontiguousarray(Rb, dtype=np.float64)
+    w = np.ascontiguousarray(Rw, dtype=np.float64)
+    assert b.shape == w.shape and b.shape[-1] == 3
+    K, S = np.empty_like(b), np.empty_like(b)
+    dp = C.POINTER(C.c_double)
+    call("pnt_compute_scattering_absorption", ctx.h, b.size // 3, b.ctypes.data_as(dp),
+         w.ctypes.data_as(dp), K.ctypes.data_as(dp), S.ctypes.data_as(dp))
+    return K, S
+
+
+def palette_from_json(text: str):
+    """LoadPalette: JSON text -> (K, S), each (n, 3) float64."""
+    raw = text.encode()
+    n = C.c_int64()
+    cap = max(1, raw.count(b"{"))  # an upper bound on the entries
+    K, S = np.empty((cap, 3)), np.empty((cap, 3))
+    dp = C.POINTER(C.c_double)
+    call("pnt_palette_from_json", raw, len(raw), K.ctypes.data_as(dp), S.ctypes.data_as(dp), cap,
+         C.byref(n))
+    return K[: n.value].copy(), S[: n.value].copy()
+
+
+def palette_to_json(K, S) -> str:
+    """SavePalette: (n, 3) K and S -> compact JSON text."""
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    n = len(K)
+    ln = C.c_int64()
+    cap = 128 + 320 * n
+    buf = C.create_string_buffer(cap)
+    dp = C.POINTER(C.c_double)
+    call("pnt_palette_to_json", K.ctypes.data_as(dp), S.ctypes.data_as(dp), n, buf, cap, C.byref(ln))
+    return buf.value.decode()
+
+
+def thinning_medium():
+    K, S = (C.c_double * 3)(), (C.c_double * 3)()
+    load().pnt_thinning_medium(K, S)
+    return np.array(K), np.array(S)
+
+
 def encode(rgb, to_srgb: bool = True, bits: int = 8, ctx: Context | None = None) -> np.ndarray:
     """imSave's pixel transform: (..., 3) linear float RGB -> uint8 / uint16
     (optionally through sRGB), interleaved RGB of the same shape."""

@@ -3,5 +3,6 @@
 
 #include "painty/core/KubelkaMunk.hxx"
 #include "painty/mixer/PaintMixer.hxx"
+#include "painty/mixer/Serialization.hxx"
 #include "painty/renderer/FootprintBrush.hxx"
 #include "painty/renderer/Renderer.hxx"

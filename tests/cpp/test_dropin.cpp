@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <sstream>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -238,6 +239,19 @@ int api() {
       EXPECT(k1.getPaintLayer().getK_buffer()(i)[1] == k2.getPaintLayer().getK_buffer()(i)[1]);
     }
     EXPECT(b1.getFootprint().rows == b2.getFootprint().rows);
+  }
+  // palette JSON round trip and the thinning medium (Serialization.cxx:26-53,
+  // Palette.cxx:453-463); no device needed
+  {
+    std::stringstream ss;
+    painty::SavePalette(ss, pal);
+    painty::Palette back;
+    painty::LoadPalette(ss, back);
+    EXPECT(back.size() == pal.size());
+    for (size_t l = 0; l < pal.size(); ++l)
+      for (int c = 0; c < 3; ++c) EXPECT(back[l].K[c] == pal[l].K[c] && back[l].S[c] == pal[l].S[c]);
+    const auto tm = painty::getThinningMedium();
+    EXPECT(tm.K[0] == 0.01 && tm.S[2] == 0.01);
   }
   // errors map to the reference's exception types
   EXPECT(throws_invalid([] { Canvas<vec3> bad(-1, 4); }));
