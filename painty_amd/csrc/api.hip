@@ -126,6 +126,19 @@ std::vector<float> padded_footprint(double radius, const float* core, int width,
   return full;
 }
 
+void footprint_box(const std::vector<float>& fp, int size_map, int box[4]) {
+  box[0] = box[1] = size_map;
+  box[2] = box[3] = -1;
+  for (int y = 0; y < size_map; ++y)
+    for (int x = 0; x < size_map; ++x)
+      if (fp[(size_t)y * size_map + x] > 0.0f) {
+        box[0] = std::min(box[0], x);
+        box[1] = std::min(box[1], y);
+        box[2] = std::max(box[2], x);
+        box[3] = std::max(box[3], y);
+      }
+}
+
 void brush_set_geometry(pnt_brush b, double radius, const float* core, int width) {
   int size_map = 0;
   std::vector<float> full = padded_footprint(radius, core, width, &size_map);
@@ -164,6 +177,7 @@ void brush_set_geometry(pnt_brush b, double radius, const float* core, int width
     b->pick_cap = 7 * mstride;
   }
   b->fp = fp;
+  footprint_box(full, size_map, b->fbox);
   b->fp_host.swap(full);
   b->pick = pick;
   b->radius = radius;

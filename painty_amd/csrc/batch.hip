@@ -50,6 +50,7 @@ struct BJob {
   Rgb3 R0, out;
   float* h;
   const float* fp;
+  FpBox fb;  // nonzero footprint cells
   uint8_t* dirty;
   int compose, use_snap;
   int rows, cols, hr, size_map;
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(kWT* kWT) void k_batch(BArgs A) {
       }
       // ---- visits: 32x32 window tiles, round-robin over the group's blocks
       for (int tile = rank; tile < ntiles; tile += (int)nb)
-        window_tile<kWT>(T, J.pick.p, J.fp, g, st, dup, pull, tile, tiles_x, sm, pk, ps);
+        window_tile<kWT>(T, J.pick.p, J.fp, J.fb, g, st, dup, pull, tile, tiles_x, sm, pk, ps);
       // ---- visits sharing canvas column/row 0, level by level
       for (int L = 0; L < st.dl_nlev; ++L) {
         if (!group_sync(bar, nb, rank, A.host_err)) {
@@ -500,6 +501,7 @@ static BatchPlan plan_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
     J.R0 = cv->R0();
     J.h = cv->h();
     J.fp = b->fp;
+    J.fb = FpBox{b->fbox[0], b->fbox[1], b->fbox[2], b->fbox[3]};
     J.dirty = b->dirty;
     J.use_snap = b->use_snap ? 1 : 0;
     J.rows = cv->rows;

@@ -36,6 +36,7 @@ struct FArgs {
   const PStamp* st;
   Layer7 canvas, snap, pick;  // snap == canvas planes when the snapshot is off
   const float* fp;
+  FpBox fb;  // nonzero footprint cells
   uint8_t* dirty;
   const int2* dl;
   const int* dl_off;
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_fused(FArgs A, int j, int win_
   if ((int)blockIdx.x < win_blocks) {
     const VisitTarget T{&A.canvas, &A.snap, A.dirty, A.use_snap, A.pickup_rate, A.dep_rate,
                         A.cap};
-    window_tile<kT>(T, A.pick.p, A.fp, g, st, dup, pull, tile_of_block(blockIdx.x, A.ntiles),
+    window_tile<kT>(T, A.pick.p, A.fp, A.fb, g, st, dup, pull, tile_of_block(blockIdx.x, A.ntiles),
                     A.tiles_x, sm, A.pk, A.ps);
   } else {
     ring_scan<2>(A.canvas, A.snap, A.dirty, A.fp, g, st.box, pull,
@@ -234,6 +235,7 @@ struct ListGeo {
   double radius;
   const float* core;
   int width;
+  FpBox fb;  // nonzero cells of fp
 };
 
 // Stamps [0, n) of a list on the fused path.  Stamp k uses geometry
@@ -262,6 +264,7 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
   for (size_t i = 0; i < geos.size(); ++i) {
     FArgs& a = GA[i];
     a.fp = geos[i].fp;
+    a.fb = geos[i].fb;
     a.size_map = geos[i].size_map;
     a.hr = (a.size_map - 1) / 2;
     const int side = 2 * a.hr + 1;
@@ -397,7 +400,8 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
 template <int kT>
 void run_fused_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n, int chains) {
   const PlanDev D = upload_plan(b, cv, st, n);
-  const std::vector<ListGeo> geos{ListGeo{b->fp, b->size_map, b->radius, nullptr, b->width}};
+  const std::vector<ListGeo> geos{ListGeo{b->fp, b->size_map, b->radius, nullptr, b->width,
+                                          FpBox{b->fbox[0], b->fbox[1], b->fbox[2], b->fbox[3]}}};
   run_fused_list<kT>(b, cv, st, n, D, geos, std::vector<int>((size_t)n, 0),
                      std::vector<uint8_t>((size_t)n, 0), chains);
 }
@@ -407,7 +411,8 @@ void run_fused_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t 
 // 46-50), then its stamps.
 template <int kT>
 void run_fused_strokes_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* S, int64_t ns) {
-  std::vector<ListGeo> geos{ListGeo{b->fp, b->size_map, b->radius, nullptr, b->width}};
+  std::vector<ListGeo> geos{ListGeo{b->fp, b->size_map, b->radius, nullptr, b->width,
+                                    FpBox{b->fbox[0], b->fbox[1], b->fbox[2], b->fbox[3]}}};
   std::vector<float> fps;
   std::vector<int64_t> fp_off{0};
   std::vector<PlanPart> parts;
@@ -424,7 +429,10 @@ void run_fused_strokes_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* S, 
       fp_off.push_back((int64_t)fps.size());
       fps.insert(fps.end(), full.begin(), full.end());
       fps.resize((size_t)round_up((int64_t)fps.size(), 64), 0.0f);
-      geos.push_back(ListGeo{nullptr, sm, S[i].radius, S[i].footprint, S[i].width});
+      int box[4];
+      footprint_box(full, sm, box);
+      geos.push_back(ListGeo{nullptr, sm, S[i].radius, S[i].footprint, S[i].width,
+                             FpBox{box[0], box[1], box[2], box[3]}});
       cur = (int)geos.size() - 1;
       cur_r = S[i].radius;
       pending_clear = true;
@@ -548,7 +556,7 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_chain(CArgs A) {
     const VisitTarget T{&A.f.canvas, &A.f.snap, A.f.dirty, A.f.use_snap, A.f.pickup_rate,
                         A.f.dep_rate, A.f.cap};
     float pkr[3] = {pk[0], pk[1], pk[2]}, psr[3] = {ps[0], ps[1], ps[2]};
-    window_tile<kT>(T, A.f.pick.p, A.f.fp, g, st, dup, pull, off - r.x - r.y, A.f.tiles_x, sm,
+    window_tile<kT>(T, A.f.pick.p, A.f.fp, A.f.fb, g, st, dup, pull, off - r.x - r.y, A.f.tiles_x, sm,
                     pkr, psr);
   } else {  // visits sharing canvas column/row 0, by dependency level
     const VisitTarget T{&A.f.canvas, &A.f.snap, A.f.dirty, A.f.use_snap, A.f.pickup_rate,
@@ -620,6 +628,7 @@ static void run_chained_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st,
   F.snap = b->use_snap ? b->snap_planes() : cv->layer();
   F.pick = b->pick_planes();
   F.fp = b->fp;
+  F.fb = FpBox{b->fbox[0], b->fbox[1], b->fbox[2], b->fbox[3]};
   F.dirty = b->dirty;
   F.dl = D.dl;
   F.dl_off = D.dl_off;

@@ -18,14 +18,15 @@
 //   visits   k_adj_visits: each cell's visits replayed from the tape (the
 //            forward arithmetic of visit.hpp, so branches resolve the same
 //            way) and their adjoints applied in reverse order;
-//   shared   k_adj_shared: visits on a duplicated canvas column/row 0, one
-//            thread, replayed and reversed in plan order;
+//   shared   k_adj_shared: visits on a duplicated canvas column/row 0,
+//            replayed on the tape and reversed level by level;
 //   ring     k_adj_ring: the ring copy canvas -> snapshot moves the snapshot
 //            adjoint onto the canvas;
 //   dip      the pickup map was cleared: its adjoint is dropped.
 // The state is recomputed, not stored: memory is the checkpoints (a few MB
 // per segment) plus one segment of tapes.
 #include <algorithm>
+#include <initializer_list>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -210,7 +211,7 @@ struct AArgs {
   // shared-pixel visits of this stamp (plan lists) and their scratch
   const int2* dl;
   const int* dl_off;
-  float* scratch;  // tape-sized copy + per-visit records
+  float* scratch;  // per-visit records of the shared-pixel replay
 };
 
 __device__ __forceinline__ StampGeom geom_of(const AArgs& A, const PStamp& st) {
@@ -325,67 +326,70 @@ __global__ __launch_bounds__(kGT* kGT) void k_adj_visits(AArgs A) {
 }
 
 // Adjoint of a stamp's visits on a duplicated canvas column/row 0 and their
-// partners: thread 0 replays them in plan order on a private copy of the
-// tape (recording each visit's input), then applies the adjoints in reverse.
-// Only these visits touch those pixels and cells, so the tape holds their
-// true starting state.
+// partners.  Only these visits touch those pixels and cells, so the tape
+// holds their true starting state; the tape of stamp j is not read after
+// its adjoint, so the replay runs on it in place.  Visits are replayed level
+// by level (a level's visits share no map cell and no canvas pixel, so each
+// level runs in parallel), each visit's input recorded; then the adjoints
+// run level by level in reverse.
 __global__ __launch_bounds__(kBlk) void k_adj_shared(AArgs A) {
   const PStamp& st = A.st[A.j];
   if (st.dl_nlev <= 0) return;
-  const int e0 = A.dl_off[st.dl_lev], e1 = A.dl_off[st.dl_lev + st.dl_nlev];
-  const int64_t tape_floats = 14 * A.t.area + 7 * A.t.m;
-  float* work = A.scratch;                 // copy of the tape
-  VState* rec = reinterpret_cast<VState*>(A.scratch + tape_floats);
-  for (int64_t i = threadIdx.x; i < tape_floats; i += kBlk) work[i] = A.t.C[i];
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x != 0) return;
+  const int base = A.dl_off[st.dl_lev];
+  VState* rec = reinterpret_cast<VState*>(A.scratch);
   const StampGeom g = geom_of(A, st);
   const bool alias = !A.use_snap;
-  float* wC = work;
-  float* wX = work + 7 * A.t.area;
-  float* wP = work + 14 * A.t.area;
-  int nv = 0;
-  for (int e = e0; e < e1; ++e) {  // replay in plan order
-    const int2 v = A.dl[e];
-    int64_t ic;
-    const int mm = stamp_visit(g, v.x, v.y, &ic);
-    if (mm < 0 || !(A.fp[mm] > 0.0f)) continue;
-    const int64_t k = tape_index(A.t, ic, A.cols);
-    VState s;
+  float* wC = const_cast<float*>(A.t.C);  // C, X (7 planes of area each), P (7 of m)
+  float* wX = wC + 7 * A.t.area;
+  float* wP = wC + 14 * A.t.area;
+  for (int L = 0; L < st.dl_nlev; ++L) {  // replay in plan order, a level at a time
+    const int e0 = A.dl_off[st.dl_lev + L], e1 = A.dl_off[st.dl_lev + L + 1];
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlk) {
+      const int2 v = A.dl[e];
+      int64_t ic;
+      const int mm = stamp_visit(g, v.x, v.y, &ic);
+      if (mm < 0 || !(A.fp[mm] > 0.0f)) continue;
+      const int64_t k = tape_index(A.t, ic, A.cols);
+      VState sv;
 #pragma unroll
-    for (int c = 0; c < 7; ++c) {
-      s.P[c] = wP[c * A.t.m + mm];
-      s.C[c] = wC[c * A.t.area + k];
-      s.X[c] = A.use_snap ? wX[c * A.t.area + k] : 0.0f;
-    }
-    rec[nv++] = s;
-    visit_fwd(s, A.fp[mm], A.pr, A.dr, A.cap, A.pk, A.ps, alias);
+      for (int c = 0; c < 7; ++c) {
+        sv.P[c] = wP[c * A.t.m + mm];
+        sv.C[c] = wC[c * A.t.area + k];
+        sv.X[c] = A.use_snap ? wX[c * A.t.area + k] : 0.0f;
+      }
+      rec[e - base] = sv;
+      visit_fwd(sv, A.fp[mm], A.pr, A.dr, A.cap, A.pk, A.ps, alias);
 #pragma unroll
-    for (int c = 0; c < 7; ++c) {
-      wP[c * A.t.m + mm] = s.P[c];
-      wC[c * A.t.area + k] = s.C[c];
-      if (A.use_snap) wX[c * A.t.area + k] = s.X[c];
+      for (int c = 0; c < 7; ++c) {
+        wP[c * A.t.m + mm] = sv.P[c];
+        wC[c * A.t.area + k] = sv.C[c];
+        if (A.use_snap) wX[c * A.t.area + k] = sv.X[c];
+      }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // this level's stores before the next level's loads
+    __syncthreads();
   }
   float gB[6] = {0, 0, 0, 0, 0, 0};
-  for (int e = e1 - 1; e >= e0; --e) {  // adjoints in reverse
-    const int2 v = A.dl[e];
-    int64_t ic;
-    const int mm = stamp_visit(g, v.x, v.y, &ic);
-    if (mm < 0 || !(A.fp[mm] > 0.0f)) continue;
-    --nv;
-    VState gs;
+  for (int L = st.dl_nlev - 1; L >= 0; --L) {  // adjoints, levels in reverse
+    const int e0 = A.dl_off[st.dl_lev + L], e1 = A.dl_off[st.dl_lev + L + 1];
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlk) {
+      const int2 v = A.dl[e];
+      int64_t ic;
+      const int mm = stamp_visit(g, v.x, v.y, &ic);
+      if (mm < 0 || !(A.fp[mm] > 0.0f)) continue;
+      VState gs;
 #pragma unroll
-    for (int c = 0; c < 7; ++c) gs.P[c] = A.gP.p[c][mm];
-    load_g_pixel(A, ic, gs);
-    visit_adj(rec[nv], gs, gB, A.fp[mm], A.pr, A.dr, A.cap, A.pk, A.ps, alias);
-    store_g_pixel(A, ic, gs);
+      for (int c = 0; c < 7; ++c) gs.P[c] = A.gP.p[c][mm];
+      load_g_pixel(A, ic, gs);
+      visit_adj(rec[e - base], gs, gB, A.fp[mm], A.pr, A.dr, A.cap, A.pk, A.ps, alias);
+      store_g_pixel(A, ic, gs);
 #pragma unroll
-    for (int c = 0; c < 7; ++c) A.gP.p[c][mm] = gs.P[c];
+      for (int c = 0; c < 7; ++c) A.gP.p[c][mm] = gs.P[c];
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
   }
-#pragma unroll
-  for (int c = 0; c < 6; ++c) atomicAdd(&A.gB[c], gB[c]);
+  flush_gB(A, gB);
 }
 
 // Adjoint of the ring copy canvas -> snapshot (FootprintBrush.hxx:307-318).
@@ -489,18 +493,30 @@ __global__ __launch_bounds__(kBlk) void k_km_adj(int64_t n, Layer7 L, Rgb3 R0, R
 
 // Copies a w x h box at (x0, y0) of 7 planes with row length `cols` to or
 // from 7 dense planes of w*h floats (plane stride `stride`).
-__global__ __launch_bounds__(kBlk) void k_box_copy(Layer7 L, int cols, int x0, int y0, int w, int h,
-                                                   float* buf, int64_t stride, int to_buf) {
-  const int64_t total = (int64_t)w * h;
-  for (int64_t e = (int64_t)blockIdx.x * kBlk + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * kBlk) {
-    const int64_t i = (int64_t)(y0 + e / w) * cols + (x0 + e % w);
+// A w x h box of 7 planes (row stride cols) <-> a dense buffer (plane stride
+// `stride`); up to three boxes per launch.
+struct BoxRegion {
+  Layer7 L;
+  int cols, x0, y0, w, h;
+  float* buf;
+  int64_t stride;
+};
+
+__global__ __launch_bounds__(kBlk) void k_box_copy(BoxRegion r0, BoxRegion r1, BoxRegion r2,
+                                                   int64_t n0, int64_t n1, int64_t n2,
+                                                   int to_buf) {
+  const int64_t total = n0 + n1 + n2;
+  for (int64_t t = (int64_t)blockIdx.x * kBlk + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * kBlk) {
+    const BoxRegion& r = t < n0 ? r0 : (t < n0 + n1 ? r1 : r2);
+    const int64_t e = t < n0 ? t : (t < n0 + n1 ? t - n0 : t - n0 - n1);
+    const int64_t i = (int64_t)(r.y0 + e / r.w) * r.cols + (r.x0 + e % r.w);
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
       if (to_buf)
-        buf[c * stride + e] = L.p[c][i];
+        r.buf[c * r.stride + e] = r.L.p[c][i];
       else
-        L.p[c][i] = buf[c * stride + e];
+        r.L.p[c][i] = r.buf[c * r.stride + e];
     }
   }
 }
@@ -509,11 +525,21 @@ inline unsigned grid_for(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlk - 1) / kBlk, 4096));
 }
 
-void box_copy(const Layer7& L, int cols, int x0, int y0, int w, int h, float* buf, int64_t stride,
-              bool to_buf, hipStream_t s) {
-  if (w <= 0 || h <= 0) return;
-  k_box_copy<<<grid_for((int64_t)w * h), kBlk, 0, s>>>(L, cols, x0, y0, w, h, buf, stride,
-                                                       to_buf ? 1 : 0);
+// Copies up to three boxes in one launch (boxes with w or h <= 0 are skipped).
+void box_copy(std::initializer_list<BoxRegion> regions, bool to_buf, hipStream_t s) {
+  BoxRegion r[3] = {};
+  int64_t n[3] = {0, 0, 0};
+  int k = 0;
+  for (const BoxRegion& b : regions) {
+    if (b.w <= 0 || b.h <= 0) continue;
+    r[k] = b;
+    n[k] = (int64_t)b.w * b.h;
+    ++k;
+  }
+  const int64_t total = n[0] + n[1] + n[2];
+  if (total == 0) return;
+  for (int i = k; i < 3; ++i) r[i] = r[0];
+  k_box_copy<<<grid_for(total), kBlk, 0, s>>>(r[0], r[1], r[2], n[0], n[1], n[2], to_buf ? 1 : 0);
   PNT_HIP_CHECK(hipGetLastError());
 }
 
@@ -724,10 +750,12 @@ int pnt_grad_forward(pnt_grad g, pnt_dev_layer wet0, pnt_dev_rgb R0, const doubl
     for (const auto& sg : g->segs) {
       const int64_t area = (int64_t)sg.w * sg.h;
       float* c0 = g->ckpt + sg.off;
-      box_copy(L, cv->cols, sg.x0, sg.y0, sg.w, sg.h, c0, area, true, s);
-      if (b->use_snap) box_copy(b->snap_planes(), cv->cols, sg.x0, sg.y0, sg.w, sg.h, c0 + 7 * area, area, true, s);
-      box_copy(pick, b->size_map, 0, 0, b->size_map, b->size_map,
-               c0 + (b->use_snap ? 14 : 7) * area, m, true, s);
+      box_copy({BoxRegion{L, cv->cols, sg.x0, sg.y0, sg.w, sg.h, c0, area},
+                BoxRegion{b->snap_planes(), cv->cols, sg.x0, sg.y0, b->use_snap ? sg.w : 0, sg.h,
+                          c0 + 7 * area, area},
+                BoxRegion{pick, b->size_map, 0, 0, b->size_map, b->size_map,
+                          c0 + (b->use_snap ? 14 : 7) * area, m}},
+               true, s);
       run_fused(b, cv, g->st.data() + sg.s0, sg.s1 - sg.s0);
     }
     launch_compose(n, L, R0c, Rgb3{{R.p[0], R.p[1], R.p[2]}}, s);
@@ -779,21 +807,22 @@ int pnt_grad_backward(pnt_grad g, pnt_dev_rgb gR, pnt_dev_layer gwet, pnt_dev_rg
       const auto& sg = g->segs[k];
       const int cnt = sg.s1 - sg.s0;
       const PlanDev D = upload_plan(b, cv, g->st.data() + sg.s0, cnt);
-      // scratch of the shared-pixel replay: a tape copy + one record per visit
+      // scratch of the shared-pixel replay: one record per visit
       int64_t max_list = 0;
       for (int q = 0; q < cnt; ++q)
         if (D.host[q].dl_nlev > 0)
           max_list = std::max<int64_t>(
               max_list, D.host_off[D.host[q].dl_lev + D.host[q].dl_nlev] - D.host_off[D.host[q].dl_lev]);
       if (max_list)
-        ensure(g->scratch, g->scratch_cap, slot + max_list * (int64_t)(sizeof(VState) / sizeof(float)), s);
+        ensure(g->scratch, g->scratch_cap, max_list * (int64_t)(sizeof(VState) / sizeof(float)), s);
       // ---- restore the segment's checkpoint
       const int64_t carea = (int64_t)sg.w * sg.h;
       float* c0 = g->ckpt + sg.off;
-      box_copy(L, cv->cols, sg.x0, sg.y0, sg.w, sg.h, c0, carea, false, s);
-      if (b->use_snap)
-        box_copy(b->snap_planes(), cv->cols, sg.x0, sg.y0, sg.w, sg.h, c0 + 7 * carea, carea, false, s);
-      box_copy(pick, sm, 0, 0, sm, sm, c0 + (b->use_snap ? 14 : 7) * carea, m, false, s);
+      box_copy({BoxRegion{L, cv->cols, sg.x0, sg.y0, sg.w, sg.h, c0, carea},
+                BoxRegion{b->snap_planes(), cv->cols, sg.x0, sg.y0, b->use_snap ? sg.w : 0, sg.h,
+                          c0 + 7 * carea, carea},
+                BoxRegion{pick, sm, 0, 0, sm, sm, c0 + (b->use_snap ? 14 : 7) * carea, m}},
+               false, s);
       {  // brush paint at the segment start: its last dip (stamp 0 dips)
         int i = sg.s0;
         while (!(g->st[i].flags & PNT_STAMP_DIP)) --i;
@@ -828,10 +857,11 @@ int pnt_grad_backward(pnt_grad g, pnt_dev_rgb gR, pnt_dev_layer gwet, pnt_dev_rg
         t.area = area;
         t.m = m;
         window_box(p, cv->rows, cv->cols, &t.bx0, &t.by0, &t.bw, &t.bh);
-        box_copy(L, cv->cols, t.bx0, t.by0, t.bw, t.bh, base, area, true, s);
-        if (b->use_snap)
-          box_copy(b->snap_planes(), cv->cols, t.bx0, t.by0, t.bw, t.bh, base + 7 * area, area, true, s);
-        box_copy(pick, sm, 0, 0, sm, sm, base + 14 * area, m, true, s);
+        box_copy({BoxRegion{L, cv->cols, t.bx0, t.by0, t.bw, t.bh, base, area},
+                  BoxRegion{b->snap_planes(), cv->cols, t.bx0, t.by0, b->use_snap ? t.bw : 0, t.bh,
+                            base + 7 * area, area},
+                  BoxRegion{pick, sm, 0, 0, sm, sm, base + 14 * area, m}},
+                 true, s);
         launch_stamp_main(a, s);
         if (p.dl_nlev > 0) launch_stamp_serial(a, D.dl, D.dl_off + p.dl_lev, p.dl_nlev, s);
       }

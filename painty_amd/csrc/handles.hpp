@@ -104,6 +104,7 @@ struct pnt_brush_s {
   float* fp = nullptr;         // footprint, size_map^2
   int64_t fp_cap = 0, pick_cap = 0;  // floats allocated at fp / pick
   std::vector<float> fp_host;  // same footprint on the host (traffic replay)
+  int fbox[4] = {0, 0, 1 << 30, 1 << 30};  // nonzero footprint cells: x0 y0 x1 y1
   float* pick = nullptr;       // pickup map: 7 planes of mstride
   float* snap = nullptr;       // snapshot: 7 planes of snap_stride
   int snap_rows = 0, snap_cols = 0;
@@ -165,6 +166,9 @@ void brush_set_geometry(pnt_brush b, double radius, const float* core, int width
 // The footprint padded with zeros to size_map^2 (FootprintBrush.hxx:52-58,
 // padded to exactly sizeMap), after the checks brush_set_geometry makes.
 std::vector<float> padded_footprint(double radius, const float* core, int width, int* size_map);
+// The bounding box (x0, y0, x1, y1) of the nonzero cells of a padded
+// size_map^2 footprint; empty (x1 < x0) when all are zero.
+void footprint_box(const std::vector<float>& fp, int size_map, int box[4]);
 // Palette JSON (mixer/src/Serialization.cxx:26-53), ingest.hip.
 void palette_from_json(const char* text, int64_t len, std::vector<double>& K,
                        std::vector<double>& S);

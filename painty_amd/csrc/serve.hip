@@ -211,6 +211,7 @@ void pnt_server_s::run(int slot, Req* q) {
     for (int i = 0; i < w; ++i)
       for (int j = 0; j < w; ++j)
         b->fp_host[(size_t)(i + pad) * sm + (j + pad)] = q->footprint[(size_t)i * w + j];
+    footprint_box(b->fp_host, sm, b->fbox);
     PNT_HIP_CHECK(hipMemcpyAsync(b->fp, b->fp_host.data(), b->m * sizeof(float),
                                  hipMemcpyHostToDevice, s));
     launch_fill(b->pick, 7 * b->mstride, 0.0f, s);

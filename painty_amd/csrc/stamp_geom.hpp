@@ -42,6 +42,28 @@ __host__ __device__ __forceinline__ int stamp_visit(const StampGeom& g, int col,
   return my * g.size_map + mx;
 }
 
+// stamp_visit, and -1 also for a map cell outside the box [fx0, fx1] x
+// [fy0, fy1] (the footprint's nonzero cells: such a visit has fh = 0 and
+// changes nothing).
+__host__ __device__ __forceinline__ int stamp_visit_in(const StampGeom& g, int col, int row,
+                                                       int fx0, int fy0, int fx1, int fy1,
+                                                       int64_t* ic) {
+#pragma clang fp contract(off)
+  if (col < -g.hr || col > g.hr || row < -g.hr || row > g.hr) return -1;
+  const int x = (int)(col + g.cx);
+  const int y = (int)(row + g.cy);
+  if (y < 0 || x < 0 || x >= g.cols || y >= g.rows) return -1;
+  const double dc = (double)col, dr = (double)row;
+  const double rc = dc * g.c - dr * g.s;
+  const double rr = dc * g.s + dr * g.c;
+  const int mx = (int)round(rc + (double)g.hr);
+  const int my = (int)round(rr + (double)g.hr);
+  if (my < 0 || mx < 0 || mx >= g.size_map || my >= g.size_map) return -1;
+  if (mx < fx0 || mx > fx1 || my < fy0 || my > fy1) return -1;
+  *ic = (int64_t)y * g.cols + x;
+  return my * g.size_map + mx;
+}
+
 // Canvas column 0 (row 0) receives two window columns (rows) when a stamp
 // straddles the left (top) edge with a fractional centre: (int)(z) == 0 for
 // z in (-1, 1).  Returns true when some col in [-hr, hr-1] has

@@ -13,6 +13,14 @@
 
 namespace pnt {
 
+// Bounding box (map cells, inclusive) of the footprint's nonzero values: a
+// visit whose cell lies outside it has fh = 0 and changes nothing, so its
+// loads are skipped (an ALU test, no memory round trip).
+struct FpBox {
+  int x0, y0, x1, y1;
+};
+
+
 enum : uint32_t { kStDip = 1u, kStDupX = 2u, kStDupY = 4u };
 
 // One imprint, planned on the host (FP64 geometry as stamp_args computes it).
@@ -354,7 +362,7 @@ __device__ __forceinline__ void run_pixel(const VisitTarget& T, int64_t ic, bool
 // canvas pixel and source pixel are loaded in one batch.
 template <int kT>
 __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* pick,
-                                            const float* fp, const StampGeom& g,
+                                            const float* fp, const FpBox& fb, const StampGeom& g,
                                             const PStamp& st, bool dup, bool pull, int tile,
                                             int tiles_x, int (*sm)[kT + 2], const float pk[3],
                                             const float ps[3]) {
@@ -362,10 +370,12 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
   const int lx = t % kT, ly = t / kT;
   const int col0 = -g.hr + (tile % tiles_x) * kT;
   const int row0 = -g.hr + (tile / tiles_x) * kT;
+  // map cells of the tile and its halo; cells outside the footprint's
+  // nonzero box count as no visit (their visits have fh = 0)
   for (int e = t; e < (kT + 2) * (kT + 2); e += kT * kT) {
     const int hy = e / (kT + 2), hx = e % (kT + 2);
     int64_t ic;
-    sm[hy][hx] = stamp_visit(g, col0 + hx - 1, row0 + hy - 1, &ic);
+    sm[hy][hx] = stamp_visit_in(g, col0 + hx - 1, row0 + hy - 1, fb.x0, fb.y0, fb.x1, fb.y1, &ic);
   }
   __syncthreads();
   const int mm = sm[ly + 1][lx + 1];
