@@ -148,10 +148,18 @@ void fused_release(FusedState*& F) {
 
 namespace {
 
+// Concurrent units run as waves on one stream (the default), or on
+// concurrent streams with PAINTY_FUSED_SCHED=streams.
+bool fused_waves() {
+  const char* e = std::getenv("PAINTY_FUSED_SCHED");
+  return !(e && std::strcmp(e, "streams") == 0);
+}
+
 // Concurrent chains of one call: `want` if given (> 0), else
-// PAINTY_FUSED_CHAINS, else 4 (the hardware queues of a process).
+// PAINTY_FUSED_CHAINS, else 8 for waves (the slots of a wave) or 4 for
+// streams (the hardware queues of a process).
 int fused_chains(int want) {
-  int c = want > 0 ? want : 4;
+  int c = want > 0 ? want : (fused_waves() ? kMaxChains : 4);
   if (want <= 0)
     if (const char* e = std::getenv("PAINTY_FUSED_CHAINS")) c = std::atoi(e);
   return std::min(std::max(c, 1), kMaxChains);
@@ -224,6 +232,97 @@ std::vector<Unit> plan_units(const pnt_stamp* st, const PStamp* h, const uint8_t
     }
   }
   return U;
+}
+
+// ----------------------------------------------------------------- waves --
+// The concurrent strokes again, on one stream: each launch ("wave") runs
+// the next phase of every unit that is ready -- a stamp (window tiles and
+// ring scan side by side), or the ring / window / shared-pixel-levels steps
+// of a stamp that straddles canvas column/row 0, or the pickup-map clear of
+// a dip or a geometry change -- each on its own range of blocks.  Units of
+// one wave touch disjoint pixels and their own pickup maps, exactly as on
+// concurrent streams, and the stream orders the waves: no events, and no
+// limit from the hardware queues.
+constexpr int kWaveSlots = 8;
+enum : int { kPhClear = 0, kPhStamp = 1, kPhRing = 2, kPhWindow = 3, kPhLevels = 4 };
+
+struct WSlot {
+  int j;       // stamp index (not used by kPhClear)
+  int kind;    // kPh*
+  int start;   // first block of the slot (a multiple of 8: tiles keep their XCD)
+  int nblk;    // blocks with work (the slot's range may be padded past them)
+  int ntiles, rb;
+  int hr, size_map, tiles_x;
+  const float* fp;
+  FpBox fb;
+  float* pick;         // the unit's pickup map (7 planes of mstride)
+  int64_t mstride;
+  float pk[3], ps[3];  // the brush paint of this stamp
+};
+
+struct WArgs {
+  const PStamp* st;
+  Layer7 canvas, snap;
+  uint8_t* dirty;
+  const int2* dl;
+  const int* dl_off;
+  int use_snap, rows, cols;
+  float pickup_rate, dep_rate, cap;
+  int nslots;
+  WSlot slot[kWaveSlots];
+};
+
+template <int kT>
+__global__ __launch_bounds__(kT* kT) void k_stamp_wave(WArgs A) {
+  __shared__ int sm[kT + 2][kT + 2];
+  int si = 0;
+  while (si + 1 < A.nslots && (int)blockIdx.x >= A.slot[si + 1].start) ++si;
+  const WSlot& S = A.slot[si];
+  const int b = (int)blockIdx.x - S.start;
+  if (b >= S.nblk) return;
+  if (S.kind == kPhClear) {  // dip() / setRadius: a clean pickup map
+    const int64_t total = 7 * S.mstride;
+    for (int64_t i = (int64_t)b * kT * kT + threadIdx.x; i < total; i += (int64_t)S.nblk * kT * kT)
+      S.pick[i] = 0.0f;
+    return;
+  }
+  const PStamp& st = A.st[S.j];
+  StampGeom g;
+  g.cx = st.cx;
+  g.cy = st.cy;
+  g.c = st.c;
+  g.s = st.s;
+  g.hr = S.hr;
+  g.size_map = S.size_map;
+  g.rows = A.rows;
+  g.cols = A.cols;
+  const bool dup = (st.flags & (kStDupX | kStDupY)) != 0;
+  const bool pull = A.use_snap && !dup;
+  const VisitTarget T{&A.canvas, &A.snap, A.dirty, A.use_snap, A.pickup_rate, A.dep_rate, A.cap};
+  float* pick[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) pick[c] = S.pick + c * S.mstride;
+  const int win = (S.kind == kPhStamp || S.kind == kPhWindow) ? S.ntiles : 0;
+  if (S.kind == kPhLevels) {  // one block (nblk == 1)
+    for (int L = 0; L < st.dl_nlev; ++L) {
+      const int e0 = A.dl_off[st.dl_lev + L], e1 = A.dl_off[st.dl_lev + L + 1];
+      for (int e = e0 + (int)threadIdx.x; e < e1; e += kT * kT) {
+        const int2 v = A.dl[e];
+        int64_t ic;
+        const int m = stamp_visit(g, v.x, v.y, &ic);
+        if (m >= 0 && S.fp[m] > 0.0f)
+          cached_visit(T, GlobalPick{pick, m}, S.fp[m], ic, false, S.pk, S.ps);
+      }
+      __builtin_amdgcn_s_waitcnt(0);  // this level's stores before the next level's loads
+      __syncthreads();
+    }
+  } else if (b < win) {
+    window_tile<kT>(T, pick, S.fp, S.fb, g, st, dup, pull, tile_of_block(b, S.ntiles), S.tiles_x,
+                    sm, S.pk, S.ps);
+  } else if (b < win + S.rb) {
+    ring_scan<2>(A.canvas, A.snap, A.dirty, S.fp, g, st.box, pull, (b - win) * kT * kT +
+                 (int)threadIdx.x, S.rb * kT * kT);
+  }
 }
 
 // One brush geometry of a list: its padded footprint on the device, its side
@@ -314,6 +413,125 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
     if (g.core) brush_set_geometry(b, g.radius, g.core, g.width);  // clears b->pick
   };
 
+  // ---- waves: the units' phases, scheduled on s ----------------------------
+  auto run_waves = [&](const std::vector<Unit>& U, int used, float* const* pick) {
+    const int T2 = kT * kT;
+    struct Ph {
+      int64_t k;
+      int kind;
+    };
+    std::vector<std::vector<Ph>> ph(U.size());
+    for (size_t i = 0; i < U.size(); ++i)
+      for (int64_t k = U[i].s0; k < U[i].s1; ++k) {
+        if (clear[k] || (st[k].flags & PNT_STAMP_DIP)) ph[i].push_back(Ph{k, kPhClear});
+        const PStamp& p = D.host[k];
+        if (p.flags & (kStDupX | kStDupY)) {
+          if (b->use_snap) ph[i].push_back(Ph{k, kPhRing});
+          ph[i].push_back(Ph{k, kPhWindow});
+          if (p.dl_nlev > 0) ph[i].push_back(Ph{k, kPhLevels});
+        } else {
+          ph[i].push_back(Ph{k, kPhStamp});
+        }
+      }
+    // the brush paint of every stamp: that of the last dip up to it
+    std::vector<float> paint(6 * (size_t)n);
+    float cur[6] = {b->paintK[0], b->paintK[1], b->paintK[2],
+                    b->paintS[0], b->paintS[1], b->paintS[2]};
+    for (int64_t k = 0; k < n; ++k) {
+      if (st[k].flags & PNT_STAMP_DIP)
+        for (int c = 0; c < 3; ++c) {
+          cur[c] = st[k].K[c];
+          cur[3 + c] = st[k].S[c];
+        }
+      std::memcpy(&paint[6 * (size_t)k], cur, sizeof cur);
+    }
+    WArgs W{};
+    W.st = A0.st;
+    W.canvas = A0.canvas;
+    W.snap = A0.snap;
+    W.dirty = A0.dirty;
+    W.dl = A0.dl;
+    W.dl_off = A0.dl_off;
+    W.use_snap = A0.use_snap;
+    W.rows = A0.rows;
+    W.cols = A0.cols;
+    W.pickup_rate = A0.pickup_rate;
+    W.dep_rate = A0.dep_rate;
+    W.cap = A0.cap;
+    std::vector<size_t> next(U.size(), 0);
+    std::vector<char> done(U.size(), 0);
+    std::vector<int> prev_on_chain(U.size(), -1);
+    {
+      std::vector<int> last(used, -1);
+      for (size_t i = 0; i < U.size(); ++i) {
+        prev_on_chain[i] = last[U[i].chain];
+        last[U[i].chain] = (int)i;
+        if (ph[i].empty()) done[i] = 1;
+      }
+    }
+    size_t g0 = 0;  // first unit of the current drying phase
+    while (g0 < U.size()) {
+      size_t g1 = g0;  // units [g0, g1] form the phase (up to a dry)
+      while (g1 + 1 < U.size() && !U[g1].dry_after) ++g1;
+      for (;;) {
+        std::vector<int> ready;
+        for (size_t i = g0; i <= g1 && (int)ready.size() < kWaveSlots; ++i) {
+          if (done[i]) continue;
+          bool ok = prev_on_chain[i] < 0 || done[(size_t)prev_on_chain[i]];
+          for (int d : U[i].deps) ok = ok && done[(size_t)d];
+          if (ok) ready.push_back((int)i);
+        }
+        if (ready.empty()) break;  // the phase is done
+        W.nslots = (int)ready.size();
+        int start = 0;
+        for (int q = 0; q < W.nslots; ++q) {
+          const int i = ready[(size_t)q];
+          const Ph& f = ph[(size_t)i][next[(size_t)i]];
+          WSlot& S = W.slot[q];
+          const FArgs& a = GA[(size_t)geo[(size_t)f.k]];
+          S.j = (int)f.k;
+          S.kind = f.kind;
+          S.start = start;
+          S.ntiles = a.ntiles;
+          S.rb = a.use_snap ? ring_blocks_for(D.host[f.k].box, T2) : 0;
+          S.hr = a.hr;
+          S.size_map = a.size_map;
+          S.tiles_x = a.tiles_x;
+          S.fp = a.fp;
+          S.fb = a.fb;
+          S.mstride = gstride[(size_t)geo[(size_t)f.k]];
+          S.pick = pick[U[(size_t)i].chain];
+          for (int c = 0; c < 3; ++c) {
+            S.pk[c] = paint[6 * (size_t)f.k + c];
+            S.ps[c] = paint[6 * (size_t)f.k + 3 + c];
+          }
+          switch (f.kind) {
+            case kPhClear:
+              S.nblk = (int)std::min<int64_t>(64, (7 * S.mstride + 4 * T2 - 1) / (4 * T2));
+              break;
+            case kPhStamp: S.nblk = S.ntiles + S.rb; break;
+            case kPhRing: S.nblk = S.rb; break;
+            case kPhWindow: S.nblk = S.ntiles; break;
+            default: S.nblk = 1; break;
+          }
+          if (f.kind == kPhRing || f.kind == kPhWindow) {  // one part of a split stamp
+            if (f.kind == kPhRing) S.ntiles = 0;
+            else S.rb = 0;
+          }
+          start += (std::max(S.nblk, 1) + 7) / 8 * 8;
+        }
+        k_stamp_wave<kT><<<(unsigned)start, T2, 0, s>>>(W);
+        PNT_HIP_CHECK(hipGetLastError());
+        for (int i : ready)
+          if (++next[(size_t)i] == ph[(size_t)i].size()) done[(size_t)i] = 1;
+      }
+      if (U[g1].dry_after) dry(s);
+      g0 = g1 + 1;
+    }
+    std::memcpy(b->paintK, cur, 3 * sizeof(float));
+    std::memcpy(b->paintS, cur + 3, 3 * sizeof(float));
+  };
+
   const bool one_geo = geos.size() == 1 && !geos[0].core;
   const int nchains = fused_chains(want_chains);
   std::vector<Unit> U;
@@ -343,6 +561,24 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
       }
       S[c] = c == 0 ? s : F.streams[c];
       pick[c] = F.pick[c];
+    }
+    if (fused_waves()) {
+      // a first unit that continues the brush's stroke starts from its map
+      if (!clear[0] && !(st[0].flags & PNT_STAMP_DIP))
+        PNT_HIP_CHECK(hipMemcpyAsync(pick[0], b->pick,
+                                     7 * gstride[(size_t)geo[0]] * sizeof(float),
+                                     hipMemcpyDeviceToDevice, s));
+      run_waves(U, used, pick);
+      finish_geometry();
+      PNT_HIP_CHECK(hipMemcpyAsync(b->pick, pick[U.back().chain],
+                                   7 * gstride[(size_t)geo[(size_t)n - 1]] * sizeof(float),
+                                   hipMemcpyDeviceToDevice, s));
+      cv->epoch++;
+      if (b->use_snap) {
+        b->dirty_canvas = cv;
+        b->dirty_epoch = cv->epoch;
+      }
+      return;
     }
     const size_t nev = 2 * U.size() + 2;  // units, dries, start
     while (F.events.size() < nev) {

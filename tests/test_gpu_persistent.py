@@ -159,13 +159,15 @@ def test_planned_paths_equal_plain_bitwise_on_one_stroke(hip, lmode):
         assert np.array_equal(x, y), get
 
 
+@pytest.mark.parametrize("sched", ["waves", "streams"])
 @pytest.mark.parametrize("chains", [2, 4, 8])
-def test_fused_concurrent_strokes_equal_one_chain(hip, oracle, chains, monkeypatch):
-    """Strokes with disjoint snapshot boxes run on concurrent streams, each
-    with its own pickup map: the result equals the one-stream run to the
-    bit (and the oracle), with drying between phases and the last stroke's
-    pickup map left in the brush."""
+def test_fused_concurrent_strokes_equal_one_chain(hip, oracle, chains, sched, monkeypatch):
+    """Strokes with disjoint snapshot boxes run side by side (as waves of one
+    stream, or on concurrent streams), each with its own pickup map: the
+    result equals the one-chain run to the bit (and the oracle), with drying
+    between phases and the last stroke's pickup map left in the brush."""
     wl = W.cfg2(seed=7, size=1024, n_stamps=400, dry_every=130, foot=(17, 41))
+    monkeypatch.setenv("PAINTY_FUSED_SCHED", sched)
     monkeypatch.setenv("PAINTY_FUSED_CHAINS", "1")
     a, ab = run_gpu(wl, mode=FUSED)
     monkeypatch.setenv("PAINTY_FUSED_CHAINS", str(chains))
@@ -207,12 +209,14 @@ def _same(a, ab, b, bb):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("sched", ["waves", "streams"])
 @pytest.mark.parametrize("chains", [1, 4])
-def test_imprint_strokes_equals_stroke_by_stroke(hip, oracle, chains, monkeypatch):
+def test_imprint_strokes_equals_stroke_by_stroke(hip, oracle, chains, sched, monkeypatch):
     """pnt_imprint_strokes (one call, per-stroke footprints, concurrent
     strokes) equals setRadius + imprintStamps stroke by stroke, to the bit,
     and the oracle."""
     wl = W.cfg2(seed=8, size=1024, n_stamps=400, dry_every=130, foot=(17, 41))
+    monkeypatch.setenv("PAINTY_FUSED_SCHED", sched)
     monkeypatch.setenv("PAINTY_FUSED_CHAINS", str(chains))
     a, ab = run_gpu(wl, mode=FUSED)
     b, bb = _fresh(hip, wl)
