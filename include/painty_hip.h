@@ -460,6 +460,43 @@ int pnt_server_stats(pnt_server server, int64_t* resident_bytes,
 /* Finishes the queued work, then releases everything. */
 int pnt_server_destroy(pnt_server server);
 
+/* ------------------------------------------------------- TextureBrush --- */
+/* TextureBrush<vector_type> -- renderer/TextureBrush.hxx:19-238 with its
+ * Smudge (renderer/Smudge.hxx:22-229): a brush stroke texture (thickness
+ * map) warped along a stroke by mean value coordinates over the stroke's
+ * frame polygon (core/Math.hxx:73-133), an optional smudge pass, then
+ * volume-weighted mixing with V = max(Vtex, Vcanvas).  Paint, thickness map
+ * and smudge maps are FP64 as the reference's; the canvas is FP32. */
+typedef struct pnt_tbrush_s* pnt_tbrush;
+/* TextureBrush(sampleDir) -- TextureBrush.hxx:25-31: radius 0, paint 0.1,
+ * smudge on, thickness scale 1; the sample's thickness map is set apart
+ * (pnt_tbrush_set_thickness_map) since image IO is the caller's. */
+int pnt_tbrush_create(pnt_ctx ctx, pnt_tbrush* out);
+int pnt_tbrush_destroy(pnt_tbrush brush);
+/* setRadius -- TextureBrush.hxx:33-41: a change of 0.5 or more resets the
+ * smudge to a (int)(2 radius) square map when smudging is on. */
+int pnt_tbrush_set_radius(pnt_tbrush brush, double radius);
+/* dip -- TextureBrush.hxx:48-50 */
+int pnt_tbrush_dip(pnt_tbrush brush, const double K[3], const double S[3]);
+/* BrushStrokeSample::getThicknessMap (renderer/src/BrushStrokeSample.cxx:
+ * 24-31, 164): rows x cols row-major values, finite. */
+int pnt_tbrush_set_thickness_map(pnt_tbrush brush, const double* map, int rows,
+                                 int cols);
+/* BrushBase::setThicknessScale -- renderer/BrushBase.hxx:31-33 */
+int pnt_tbrush_set_thickness_scale(pnt_tbrush brush, double scale);
+/* enableSmudge -- TextureBrush.hxx:207-209 */
+int pnt_tbrush_set_use_smudge(pnt_tbrush brush, int enable);
+/* The smudge's pickup map (Smudge.hxx _pickupMapDst): side x side cells,
+ * K and S AoS (3 per cell) and V; any output may be NULL. */
+int pnt_tbrush_get_smudge_map(pnt_tbrush brush, int* side, double* K,
+                              double* S, double* V);
+/* paintStroke -- TextureBrush.hxx:52-205: xy holds n vertices (x, y);
+ * fewer than 2 do nothing.  Canvas::checkDry runs on textured pixels when
+ * the canvas clock is on.  Fails with PNT_EINVAL without a thickness map,
+ * or when the smudge would take more than 2^24 spine steps. */
+int pnt_tbrush_paint_stroke(pnt_tbrush brush, pnt_canvas canvas,
+                            const double* xy, int64_t n);
+
 /* ---------------------------------------------------- canvas device view - */
 /* Device planes of a canvas (valid while it lives): layer (7), R0 (3), h. */
 int pnt_canvas_device_planes(pnt_canvas canvas, pnt_dev_layer* layer,

@@ -90,6 +90,19 @@ def lib():
         L.orc_brush_imprint_timed.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp,
                                               C.c_double, C.c_double, C.c_double,
                                               C.POINTER(Clock)]
+        L.orc_tbrush_create.restype = C.c_void_p
+        L.orc_tbrush_destroy.argtypes = [C.c_void_p]
+        L.orc_tbrush_set_radius.argtypes = [C.c_void_p, C.c_double]
+        L.orc_tbrush_dip.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_tbrush_set_thickness_map.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int]
+        L.orc_tbrush_set_thickness_scale.argtypes = [C.c_void_p, C.c_double]
+        L.orc_tbrush_set_use_smudge.argtypes = [C.c_void_p, C.c_int]
+        L.orc_tbrush_get_smudge_map.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp, _dp, _dp]
+        L.orc_tbrush_paint_stroke.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp,
+                                              _dp, C.c_int, C.POINTER(Clock)]
+        L.orc_gbc.argtypes = [_dp, C.c_int, C.c_double, C.c_double, _dp, _dp]
+        L.orc_tbrush_set_smudge_map.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_tbrush_turn.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_mix_single.restype = C.c_int
         L.orc_mix_single.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.orc_mixed.argtypes = [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp]
@@ -394,3 +407,72 @@ def encode(rgb, to_srgb: bool = True, bits: int = 8) -> np.ndarray:
                      out.ctypes.data_as(C.POINTER(C.c_uint16)))
     return out.astype(np.uint8) if bits == 8 else out
 
+
+
+# ------------------------------------------------------- texture brush -----
+def gbc(poly, px: float, py: float, vals) -> np.ndarray:
+    """Mean value coordinate interpolation (core/Math.hxx:73-133) of the 2-d
+    values `vals` (np x 2) at the vertices of polygon `poly` (np x 2)."""
+    poly, vals = _f64(poly), _f64(vals)
+    out = np.empty(2)
+    lib().orc_gbc(_p(poly), int(poly.shape[0]), float(px), float(py), _p(vals), _p(out))
+    return out
+
+
+class TextureBrush:
+    """TextureBrush with Smudge (renderer/TextureBrush.hxx:19-238,
+    renderer/Smudge.hxx:22-229) over an FP64 Canvas."""
+
+    def __init__(self):
+        self.h = lib().orc_tbrush_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_tbrush_destroy(self.h)
+            self.h = None
+
+    def set_radius(self, radius: float) -> None:
+        lib().orc_tbrush_set_radius(self.h, float(radius))
+
+    def dip(self, K, S) -> None:
+        lib().orc_tbrush_dip(self.h, _p(_f64(K)), _p(_f64(S)))
+
+    def set_thickness_map(self, m) -> None:
+        m = _f64(m)
+        lib().orc_tbrush_set_thickness_map(self.h, _p(m), int(m.shape[0]), int(m.shape[1]))
+
+    def set_thickness_scale(self, s: float) -> None:
+        lib().orc_tbrush_set_thickness_scale(self.h, float(s))
+
+    def enable_smudge(self, on: bool) -> None:
+        lib().orc_tbrush_set_use_smudge(self.h, int(bool(on)))
+
+    def smudge_map(self):
+        side = C.c_int(0)
+        lib().orc_tbrush_get_smudge_map(self.h, C.byref(side), None, None, None)
+        n = side.value
+        K, S, V = np.zeros((n, n, 3)), np.zeros((n, n, 3)), np.zeros((n, n))
+        lib().orc_tbrush_get_smudge_map(self.h, C.byref(side), _p(K), _p(S), _p(V))
+        return K, S, V
+
+    def set_smudge_map(self, K, S, V) -> None:
+        n = self.smudge_map()[2].shape[0]
+        K, S, V = _f64(K), _f64(S), _f64(V)
+        assert K.shape == (n, n, 3) and S.shape == (n, n, 3) and V.shape == (n, n)
+        lib().orc_tbrush_set_smudge_map(self.h, _p(K), _p(S), _p(V))
+
+    def turn(self, hx: float, hy: float) -> None:
+        """Smudge::updateOrientation to the heading (hx, hy)."""
+        lib().orc_tbrush_turn(self.h, float(hx), float(hy))
+
+    def paint_stroke(self, xy, canvas: "Canvas", now_us=None, dry_ms=None) -> None:
+        """paintStroke; with now_us / dry_ms, Canvas::checkDry runs on the
+        textured pixels against canvas.tmap."""
+        xy = _f64(xy).reshape(-1, 2)
+        clk = None
+        if now_us is not None:
+            assert canvas.tmap.dtype == np.int64 and canvas.tmap.flags.c_contiguous
+            clk = C.byref(Clock(canvas.tmap.ctypes.data_as(C.POINTER(C.c_int64)), int(now_us),
+                                int(dry_ms), _p(canvas.R0), _p(canvas.h)))
+        lib().orc_tbrush_paint_stroke(self.h, canvas.rows, canvas.cols, _p(canvas.K),
+                                      _p(canvas.S), _p(canvas.V), _p(xy), int(xy.shape[0]), clk)

@@ -1036,8 +1036,12 @@ static const double kPi = 3.141592653589793238462643383279502884; /* Math.hxx:21
  * image/Mat.hxx:69-71) for p in [-1, len]: fedcba|abcdefgh|hgfedcb */
 static int reflect_index(int p, int len) {
   if (len == 1) return 0;
-  if (p < 0) return -p - 1;
-  if (p >= len) return 2 * len - p - 1;
+  do {
+    if (p < 0)
+      p = -p - 1;
+    else if (p >= len)
+      p = 2 * len - p - 1;
+  } while (p < 0 || p >= len);
   return p;
 }
 
@@ -1145,4 +1149,417 @@ void orc_encode(int64_t n, const double* rgb, int to_srgb, int bits,
     if (r > scale) r = scale;
     out[i] = (uint16_t)r;
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* TextureBrush (renderer/TextureBrush.hxx:52-205) with Smudge
+ * (renderer/Smudge.hxx:22-229): a brush-stroke texture (thickness map)
+ * warped along the stroke by generalized barycentric coordinates
+ * (core/Math.hxx:73-133, Hormann & Floater mean value coordinates), an
+ * optional smudge pass, then volume-weighted mixing with V = max. */
+
+/* SplineEval::catmullRom / catmullRomDerivativeFirst (core/Spline.hxx:150-
+ * 221) over n 2-d points: u in [0, 1] spans the n - 1 segments, indices
+ * clamped to the ends. */
+static void spline_cr(const double* v, int n, double u, int deriv, double out[2]) {
+  const double x = (double)(n - 1) * u;
+  const int index = (int)x;
+  const double t = x - floor(x);
+  const int id[4] = {index - 1, index, index + 1, index + 2};
+  double p[4][2];
+  for (int q = 0; q < 4; q++) {
+    const int i = id[q] < 0 ? 0 : (id[q] >= n ? n - 1 : id[q]);
+    p[q][0] = v[2 * i];
+    p[q][1] = v[2 * i + 1];
+  }
+  for (int c = 0; c < 2; c++)
+    out[c] = deriv ? orc_catmull_rom_d1(p[0][c], p[1][c], p[2][c], p[3][c], t)
+                   : orc_catmull_rom(p[0][c], p[1][c], p[2][c], p[3][c], t);
+}
+
+/* generalizedBarycentricCoordinatesInterpolate (core/Math.hxx:73-133) of
+ * 2-d values at the np vertices of a polygon. */
+void orc_gbc(const double* poly, int np, double px, double py, const double* vals,
+             double out[2]) {
+  const double Eps = DBL_EPSILON * 100.0;
+  if (np == 1) {
+    out[0] = vals[0];
+    out[1] = vals[1];
+    return;
+  }
+  double* s = (double*)malloc(sizeof(double) * 2 * np);
+  double* r = (double*)malloc(sizeof(double) * np);
+  double* A = (double*)malloc(sizeof(double) * np);
+  double* D = (double*)malloc(sizeof(double) * np);
+  for (int i = 0; i < np; i++) {
+    s[2 * i] = poly[2 * i] - px;
+    s[2 * i + 1] = poly[2 * i + 1] - py;
+  }
+  int done = 0;
+  for (int i = 0; i < np && !done; i++) {
+    const int i1 = (i == np - 1) ? 0 : i + 1;
+    const double* si = s + 2 * i;
+    const double* si1 = s + 2 * i1;
+    r[i] = sqrt(si[0] * si[0] + si[1] * si[1]);
+    if (fabs(r[i] - 0.0) < Eps) {
+      out[0] = vals[2 * i];
+      out[1] = vals[2 * i + 1];
+      done = 1;
+      break;
+    }
+    const double det = si[0] * si1[1] - si1[0] * si[1];
+    A[i] = det / 2.0;
+    D[i] = si[0] * si1[0] + si[1] * si1[1];
+    if (fabs(A[i] - 0.0) < Eps && D[i] < 0.0) {
+      const double ri1 = sqrt(si1[0] * si1[0] + si1[1] * si1[1]);
+      if (i == np - 1) r[0] = ri1; else r[i + 1] = ri1;
+      for (int c = 0; c < 2; c++)
+        out[c] = (ri1 * vals[2 * i + c] + r[i] * vals[2 * i1 + c]) * (1.0 / (r[i] + ri1));
+      done = 1;
+    }
+  }
+  if (!done) {
+    double f[2] = {0.0, 0.0}, W = 0.0;
+    for (int i = 0; i < np; i++) {
+      double w = 0.0;
+      const int im = (i == 0) ? np - 1 : i - 1;
+      const int ip = (i == np - 1) ? 0 : i + 1;
+      if (A[im] != 0.0) w = w + (r[im] - D[im] / r[i]) / A[im];
+      if (A[i] != 0.0) w = w + (r[ip] - D[i] / r[i]) / A[i];
+      f[0] = f[0] + w * vals[2 * i];
+      f[1] = f[1] + w * vals[2 * i + 1];
+      W = W + w;
+    }
+    if (!(fabs(W - 0.0) < Eps)) {
+      out[0] = f[0] * (1.0 / W);
+      out[1] = f[1] * (1.0 / W);
+    } else {
+      out[0] = vals[0];
+      out[1] = vals[1];
+    }
+  }
+  free(s);
+  free(r);
+  free(A);
+  free(D);
+}
+
+/* Interpolate (image/Mat.hxx:69-100) of a single-channel image with
+ * BORDER_REFLECT. */
+static double interp_reflect_n(const double* img, int rows, int cols, int nch, double x,
+                               double y) {
+  const int xi = (int)floor(x), yi = (int)floor(y);
+  const int x0 = reflect_index(xi, cols), x1 = reflect_index(xi + 1, cols);
+  const int y0 = reflect_index(yi, rows), y1 = reflect_index(yi + 1, rows);
+  const double a = x - (double)xi, c = y - (double)yi;
+#define PX(yy, xx) img[((int64_t)(yy) * cols + (xx)) * nch]
+  return (PX(y0, x0) * (1.0 - a) + PX(y0, x1) * a) * (1.0 - c) +
+         (PX(y1, x0) * (1.0 - a) + PX(y1, x1) * a) * c;
+#undef PX
+}
+
+static double interp_reflect(const double* img, int rows, int cols, double x, double y) {
+  return interp_reflect_n(img, rows, cols, 1, x, y);
+}
+
+/* The smudge pickup maps: AoS K, S (3 per cell) and V of map x map cells. */
+struct orc_tbrush {
+  double radius;
+  double paint[6]; /* K, S */
+  double* thick;   /* the brush stroke sample's thickness map */
+  int t_rows, t_cols;
+  double thickness_scale;
+  int use_smudge;
+  int map;     /* smudge map side (Smudge(size): size x size) */
+  int max_size; /* size, made odd */
+  double *dK, *dS, *dV, *sK, *sS, *sV; /* _pickupMapDst / _pickupMapSrc */
+  double cur_rot;
+};
+
+static void tb_smudge_reset(orc_tbrush* b, int size) {
+  free(b->dK); free(b->dS); free(b->dV); free(b->sK); free(b->sS); free(b->sV);
+  b->map = size < 0 ? 0 : size;
+  b->max_size = (size % 2 == 0) ? size + 1 : size;
+  const size_t m = (size_t)b->map * b->map;
+  b->dK = (double*)calloc(3 * m + 1, sizeof(double));
+  b->dS = (double*)calloc(3 * m + 1, sizeof(double));
+  b->dV = (double*)calloc(m + 1, sizeof(double));
+  b->sK = (double*)calloc(3 * m + 1, sizeof(double));
+  b->sS = (double*)calloc(3 * m + 1, sizeof(double));
+  b->sV = (double*)calloc(m + 1, sizeof(double));
+  b->cur_rot = 0.0;
+}
+
+orc_tbrush* orc_tbrush_create(void) {
+  orc_tbrush* b = (orc_tbrush*)calloc(1, sizeof(orc_tbrush));
+  for (int c = 0; c < 6; c++) b->paint[c] = 0.1; /* TextureBrush.hxx:27-29 */
+  b->thickness_scale = 1.0;
+  b->use_smudge = 1;
+  tb_smudge_reset(b, 0); /* Smudge(2.0 * _radius) with _radius = 0 */
+  return b;
+}
+
+void orc_tbrush_destroy(orc_tbrush* b) {
+  if (!b) return;
+  free(b->dK); free(b->dS); free(b->dV); free(b->sK); free(b->sS); free(b->sV);
+  free(b->thick);
+  free(b);
+}
+
+/* TextureBrush.hxx:31-39 */
+void orc_tbrush_set_radius(orc_tbrush* b, double radius) {
+  if (!(fabs(b->radius - radius) < 0.5)) {
+    b->radius = radius;
+    if (b->use_smudge) tb_smudge_reset(b, (int)(2.0 * radius));
+  }
+}
+
+void orc_tbrush_dip(orc_tbrush* b, const double K[3], const double S[3]) {
+  for (int c = 0; c < 3; c++) {
+    b->paint[c] = K[c];
+    b->paint[3 + c] = S[c];
+  }
+}
+
+void orc_tbrush_set_thickness_map(orc_tbrush* b, const double* map, int rows, int cols) {
+  free(b->thick);
+  b->thick = (double*)malloc(sizeof(double) * (size_t)rows * cols);
+  memcpy(b->thick, map, sizeof(double) * (size_t)rows * cols);
+  b->t_rows = rows;
+  b->t_cols = cols;
+}
+
+void orc_tbrush_set_thickness_scale(orc_tbrush* b, double s) { b->thickness_scale = s; }
+void orc_tbrush_set_use_smudge(orc_tbrush* b, int on) { b->use_smudge = on; }
+
+void orc_tbrush_get_smudge_map(const orc_tbrush* b, int* side, double* K, double* S, double* V) {
+  *side = b->map;
+  const size_t m = (size_t)b->map * b->map;
+  if (K) memcpy(K, b->dK, sizeof(double) * 3 * m);
+  if (S) memcpy(S, b->dS, sizeof(double) * 3 * m);
+  if (V) memcpy(V, b->dV, sizeof(double) * m);
+}
+
+static double normalize_angle(double a) { /* Smudge.hxx:200-210 */
+  const double HALF_PI = 0.5 * kPi;
+  while (a <= -HALF_PI) a += kPi;
+  while (a > HALF_PI) a -= kPi;
+  return a;
+}
+
+/* Smudge::updateOrientation (Smudge.hxx:164-198) */
+static void tb_update_orientation(orc_tbrush* b, const double heading[2]) {
+  const double theta = atan2(heading[1], heading[0]);
+  const double dtheta = normalize_angle(theta - b->cur_rot);
+  b->cur_rot = theta;
+  const int n = b->map;
+  const size_t m = (size_t)n * n;
+  memcpy(b->sK, b->dK, sizeof(double) * 3 * m);
+  memcpy(b->sS, b->dS, sizeof(double) * 3 * m);
+  memcpy(b->sV, b->dV, sizeof(double) * m);
+  const double cx = b->max_size / 2.0, cy = b->max_size / 2.0;
+  const double sn = sin(dtheta), cs = cos(dtheta);
+  for (int x = 0; x < n; x++) {
+    for (int y = 0; y < n; y++) {
+      const size_t i = (size_t)y * n + x;
+      const double px = x - cx, py = y - cy;
+      const double rx = px * cs - py * sn + cx, ry = px * sn + py * cs + cy;
+      if (rx < 0 || ry < 0 || rx >= n || ry >= n) {
+        for (int c = 0; c < 3; c++) {
+          b->dK[3 * i + c] = b->sK[3 * i + c];
+          b->dS[3 * i + c] = b->sS[3 * i + c];
+        }
+        b->dV[i] = b->sV[i];
+      } else {
+        b->dV[i] = interp_reflect(b->sV, n, n, rx, ry);
+        for (int c = 0; c < 3; c++) {
+          b->dK[3 * i + c] = interp_reflect_n(b->sK + c, n, n, 3, rx, ry);
+          b->dS[3 * i + c] = interp_reflect_n(b->sS + c, n, n, 3, rx, ry);
+        }
+      }
+    }
+  }
+}
+
+/* Test hooks: load the pickup map, and turn it to a heading. */
+void orc_tbrush_set_smudge_map(orc_tbrush* b, const double* K, const double* S,
+                               const double* V) {
+  const size_t m = (size_t)b->map * b->map;
+  memcpy(b->dK, K, sizeof(double) * 3 * m);
+  memcpy(b->dS, S, sizeof(double) * 3 * m);
+  memcpy(b->dV, V, sizeof(double) * m);
+}
+
+void orc_tbrush_turn(orc_tbrush* b, double hx, double hy) {
+  const double h[2] = {hx, hy};
+  tb_update_orientation(b, h);
+}
+
+/* Smudge::smudge (Smudge.hxx:40-158) */
+static void tb_smudge(orc_tbrush* b, int rows, int cols, double* K, double* S, double* V,
+                      const double bmin[2], const double* verts, int nv, double length,
+                      const double* tmap, int trows, int tcols) {
+  double maxD = 0.0;
+  for (int64_t i = 0; i < (int64_t)trows * tcols; i++) maxD = fmax(tmap[i], maxD);
+  if (maxD <= 0.0) return;
+  const double pickupRate = 0.1, depositionRate = 0.1, MinVolume = 0.001;
+  for (double u = 0.0; u <= 1.0; u += 1. / length) {
+    double center[2], t[2];
+    spline_cr(verts, nv, u, 0, center);
+    spline_cr(verts, nv, u, 1, t);
+    const double tn = sqrt(t[0] * t[0] + t[1] * t[1]);
+    if (tn > 0.0) { t[0] /= tn; t[1] /= tn; }
+    tb_update_orientation(b, t);
+    const double radius = b->max_size * 0.5;
+    const int roi_x = (int)(center[0] - b->map / 2.0);
+    const int roi_y = (int)(center[1] - b->map / 2.0);
+    for (int x = 0; x < b->map; x++) {
+      for (int y = 0; y < b->map; y++) {
+        const int cpx = x + roi_x, cpy = y + roi_y;
+        const int tpx = (int)(cpx - bmin[0]), tpy = (int)(cpy - bmin[1]);
+        if (cpx < 0 || cpy < 0 || cpx >= cols || cpy >= rows) continue;
+        if (tpx < 0 || tpy < 0 || tpx >= tcols || tpy >= trows) continue;
+        const double dx = center[0] - (double)cpx, dy = center[1] - (double)cpy;
+        if (sqrt(dx * dx + dy * dy) > radius) continue;
+        const double Dv = tmap[(int64_t)tpy * tcols + tpx];
+        if (Dv <= 0.0) continue;
+        const int64_t ic = (int64_t)cpy * cols + cpx;
+        const size_t ip = (size_t)y * b->map + x;
+        const double cV = V[ic], pV = b->dV[ip];
+        const double cVl = cV * depositionRate * Dv / maxD, cVr = cV - cVl;
+        const double pVl = pV * pickupRate * Dv / maxD, pVr = pV - pVl;
+        double cK[3], cS[3], pK[3], pS[3];
+        for (int c = 0; c < 3; c++) {
+          cK[c] = K[3 * ic + c];
+          cS[c] = S[3 * ic + c];
+          pK[c] = b->dK[3 * ip + c];
+          pS[c] = b->dS[3 * ip + c];
+        }
+        const double pVnew = pVr + cVl;
+        if (pVnew > MinVolume) {
+          const double inv = 1.0 / pVnew;
+          for (int c = 0; c < 3; c++) {
+            b->dK[3 * ip + c] = inv * (pVr * pK[c] + cVl * cK[c]);
+            b->dS[3 * ip + c] = inv * (pVr * pS[c] + cVl * cS[c]);
+          }
+          b->dV[ip] = fmax(pVnew, 0.0);
+        }
+        const double cVnew = cVr + pVl;
+        if (cVnew > MinVolume) {
+          const double inv = 1. / cVnew;
+          for (int c = 0; c < 3; c++) {
+            K[3 * ic + c] = inv * (cVr * cK[c] + pVl * pK[c]);
+            S[3 * ic + c] = inv * (cVr * cS[c] + pVl * pS[c]);
+          }
+          V[ic] = fmax(cVnew, 0.0);
+        }
+      }
+    }
+  }
+}
+
+/* TextureBrush::paintStroke (TextureBrush.hxx:52-205).  clk: Canvas::
+ * checkDry on every textured pixel (NULL: off). */
+void orc_tbrush_paint_stroke(orc_tbrush* b, int rows, int cols, double* K, double* S, double* V,
+                             const double* xy, int n, const orc_clock* clk) {
+  if (n < 2 || !b->thick) return;
+  const double r = b->radius;
+  const int nv = n + 2;
+  double* v = (double*)malloc(sizeof(double) * 2 * nv);
+  {
+    double d0[2] = {xy[2] - xy[0], xy[3] - xy[1]};
+    double l0 = sqrt(d0[0] * d0[0] + d0[1] * d0[1]);
+    double d1[2] = {xy[2 * n - 2] - xy[2 * n - 4], xy[2 * n - 1] - xy[2 * n - 3]};
+    double l1 = sqrt(d1[0] * d1[0] + d1[1] * d1[1]);
+    if (l0 > 0.0) { d0[0] /= l0; d0[1] /= l0; }
+    if (l1 > 0.0) { d1[0] /= l1; d1[1] /= l1; }
+    v[0] = xy[0] - d0[0] * r;
+    v[1] = xy[1] - d0[1] * r;
+    memcpy(v + 2, xy, sizeof(double) * 2 * n);
+    v[2 * nv - 2] = xy[2 * n - 2] + d1[0] * r;
+    v[2 * nv - 1] = xy[2 * n - 1] + d1[1] * r;
+  }
+  double bmin[2] = {v[0], v[1]}, bmax[2] = {v[0], v[1]};
+  for (int i = 0; i < nv; i++) {
+    bmin[0] = fmin(bmin[0], v[2 * i]);
+    bmin[1] = fmin(bmin[1], v[2 * i + 1]);
+    bmax[0] = fmax(bmax[0], v[2 * i]);
+    bmax[1] = fmax(bmax[1], v[2 * i + 1]);
+  }
+  bmin[0] = fmax(bmin[0] - r, 0.0);
+  bmax[0] = fmin(bmax[0] + r, (double)(cols - 1));
+  bmin[1] = fmax(bmin[1] - r, 0.0);
+  bmax[1] = fmin(bmax[1] + r, (double)(rows - 1));
+  double length = 0.0;
+  for (int i = 1; i < nv; i++) {
+    const double dx = v[2 * i] - v[2 * i - 2], dy = v[2 * i + 1] - v[2 * i - 1];
+    length += sqrt(dx * dx + dy * dy);
+  }
+  /* the frame polygon: the right side backwards, then the left side */
+  const int np = 2 * nv;
+  double* poly = (double*)malloc(sizeof(double) * 2 * np);
+  double* uv = (double*)malloc(sizeof(double) * 2 * np);
+  for (int i = 0; i < nv; i++) {
+    const double u = (double)i / (double)(nv - 1);
+    double c[2], t[2];
+    spline_cr(v, nv, u, 0, c);
+    spline_cr(v, nv, u, 1, t);
+    const double tn = sqrt(t[0] * t[0] + t[1] * t[1]);
+    if (tn > 0.0) { t[0] /= tn; t[1] /= tn; }
+    const double d[2] = {-t[1], t[0]};
+    /* up (left) at nv + i; down (right) reversed at nv - 1 - i */
+    poly[2 * (nv + i)] = c[0] - r * d[0];
+    poly[2 * (nv + i) + 1] = c[1] - r * d[1];
+    uv[2 * (nv + i)] = u;
+    uv[2 * (nv + i) + 1] = 0.0;
+    poly[2 * (nv - 1 - i)] = c[0] + r * d[0];
+    poly[2 * (nv - 1 - i) + 1] = c[1] + r * d[1];
+    uv[2 * (nv - 1 - i)] = u;
+    uv[2 * (nv - 1 - i) + 1] = 1.0;
+  }
+  const int trows = (int)(bmax[1] - bmin[1] + 1), tcols = (int)(bmax[0] - bmin[0] + 1);
+  double* tmap = (double*)calloc((size_t)(trows > 0 ? trows : 1) * (tcols > 0 ? tcols : 1), sizeof(double));
+  for (int x = (int)bmin[0]; x <= (int)bmax[0]; x++) {
+    for (int y = (int)bmin[1]; y <= (int)bmax[1]; y++) {
+      if (x < 0 || x >= cols || y < 0 || y >= rows) continue;
+      double tp[2];
+      orc_gbc(poly, np, (double)x, (double)y, uv, tp);
+      if (tp[0] < 0.0 || tp[0] > 1.0 || tp[1] < 0.0 || tp[1] > 1.0) continue;
+      tp[0] *= b->t_cols;
+      tp[1] *= b->t_rows;
+      const double vt = b->thickness_scale * interp_reflect(b->thick, b->t_rows, b->t_cols, tp[0], tp[1]);
+      if (vt > 0.0) {
+        const int s2 = x - (int)bmin[0], t2 = y - (int)bmin[1];
+        if (s2 >= 0 && t2 >= 0 && s2 < tcols && t2 < trows) {
+          if (clk) check_dry(clk, cols, x, y, K, S, V);
+          tmap[(int64_t)t2 * tcols + s2] = vt;
+        }
+      }
+    }
+  }
+  if (b->use_smudge)
+    tb_smudge(b, rows, cols, K, S, V, bmin, v, nv, length, tmap, trows, tcols);
+  for (int x = (int)bmin[0]; x <= (int)bmax[0]; x++) {
+    for (int y = (int)bmin[1]; y <= (int)bmax[1]; y++) {
+      if (x < 0 || x >= cols || y < 0 || y >= rows) continue;
+      const int s2 = x - (int)bmin[0], t2 = y - (int)bmin[1];
+      if (!(s2 >= 0 && t2 >= 0 && s2 < tcols && t2 < trows)) continue;
+      const double vt = tmap[(int64_t)t2 * tcols + s2];
+      if (!(vt > 0.0)) continue; /* not in the reference's pixel list */
+      const int64_t ic = (int64_t)y * cols + x;
+      const double vc = V[ic], vs = vc + vt;
+      if (vs > 0.0) {
+        const double sc = 1.0 / vs;
+        for (int c = 0; c < 3; c++) {
+          K[3 * ic + c] = (vc * K[3 * ic + c] + vt * b->paint[c]) * sc;
+          S[3 * ic + c] = (vc * S[3 * ic + c] + vt * b->paint[3 + c]) * sc;
+        }
+        V[ic] = fmax(vt, vc);
+      }
+    }
+  }
+  free(tmap);
+  free(poly);
+  free(uv);
+  free(v);
 }

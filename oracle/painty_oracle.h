@@ -178,6 +178,33 @@ double orc_rgb2srgb(double l);
 void orc_encode(int64_t n, const double* rgb, int to_srgb, int bits,
                 uint16_t* out);
 
+/* ---- TextureBrush (renderer/TextureBrush.hxx:19-238) with Smudge
+ * (renderer/Smudge.hxx:22-229) ----
+ * A brush stroke texture (thickness map) warped along a stroke polyline by
+ * generalized barycentric coordinates (core/Math.hxx:73-133), an optional
+ * smudge pass, then volume-weighted mixing with V = max(Vtex, Vcanvas).
+ * K, S of the canvas are AoS (rows*cols*3), V rows*cols; xy holds n points.
+ * orc_gbc: the interpolation of 2-d values at a polygon's np vertices. */
+typedef struct orc_tbrush orc_tbrush;
+orc_tbrush* orc_tbrush_create(void);
+void orc_tbrush_destroy(orc_tbrush* b);
+void orc_tbrush_set_radius(orc_tbrush* b, double radius);
+void orc_tbrush_dip(orc_tbrush* b, const double K[3], const double S[3]);
+void orc_tbrush_set_thickness_map(orc_tbrush* b, const double* map, int rows, int cols);
+void orc_tbrush_set_thickness_scale(orc_tbrush* b, double s);
+void orc_tbrush_set_use_smudge(orc_tbrush* b, int on);
+void orc_tbrush_get_smudge_map(const orc_tbrush* b, int* side, double* K, double* S,
+                               double* V);
+void orc_tbrush_paint_stroke(orc_tbrush* b, int rows, int cols, double* K, double* S,
+                             double* V, const double* xy, int n, const orc_clock* clk);
+void orc_gbc(const double* poly, int np, double px, double py, const double* vals,
+             double out[2]);
+/* Test hooks: overwrite the smudge's pickup map (side x side cells), and
+ * Smudge::updateOrientation to heading (hx, hy) (Smudge.hxx:164-198). */
+void orc_tbrush_set_smudge_map(orc_tbrush* b, const double* K, const double* S,
+                               const double* V);
+void orc_tbrush_turn(orc_tbrush* b, double hx, double hy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -145,6 +145,15 @@ SIGNATURES = {
                               C.POINTER(C.c_int64)],
     "pnt_palette_to_json": [_dp, _dp, C.c_int64, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)],
     "pnt_thinning_medium": [_dp, _dp],
+    "pnt_tbrush_create": [_vp, C.POINTER(_vp)],
+    "pnt_tbrush_destroy": [_vp],
+    "pnt_tbrush_set_radius": [_vp, C.c_double],
+    "pnt_tbrush_dip": [_vp, _dp, _dp],
+    "pnt_tbrush_set_thickness_map": [_vp, _dp, C.c_int, C.c_int],
+    "pnt_tbrush_set_thickness_scale": [_vp, C.c_double],
+    "pnt_tbrush_set_use_smudge": [_vp, C.c_int],
+    "pnt_tbrush_get_smudge_map": [_vp, C.POINTER(C.c_int), _dp, _dp, _dp],
+    "pnt_tbrush_paint_stroke": [_vp, _vp, _dp, C.c_int64],
     "pnt_mixed": [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp],
     "pnt_compute_reflectance": [_vp, C.c_int64, _fp, _fp, _fp, _fp, _fp],
     "pnt_dev_compose": [_vp, C.c_int64, DevLayer, DevRgb, DevRgb, _vp],

@@ -1,0 +1,649 @@
+// TextureBrush (renderer/TextureBrush.hxx:19-238) with its Smudge
+// (renderer/Smudge.hxx:22-229): a brush stroke texture -- a thickness map --
+// warped along the stroke by generalized barycentric coordinates
+// (core/Math.hxx:73-133, the mean value coordinates of Hormann and Floater
+// over the stroke's frame polygon), an optional smudge pass that trades
+// paint between the canvas and a rotating pickup map along the stroke, then
+// volume-weighted mixing with V = max(Vtex, Vcanvas).
+//
+// Kernels, one thread per pixel (or map cell):
+//   k_tex_sample   texture coordinates by the barycentric interpolation, in
+//                  FP64 as the reference; the texture sampled bilinearly
+//                  (Interpolate, image/Mat.hxx:69-100, reflected borders);
+//                  Canvas::checkDry on textured pixels when a clock is on;
+//   k_smudge_turn  Smudge::updateOrientation: the pickup map resampled at
+//                  the rotated positions (ping-pong buffers, FP64 maps);
+//   k_smudge_trade Smudge::smudge's exchange at one spine position;
+//   k_tex_deposit  the volume-weighted mix.
+// The smudge runs one turn and one trade per unit of stroke length, in
+// stream order; each is independent per cell (a map cell owns one canvas
+// pixel at a time).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <functional>
+#include <vector>
+
+#include "dry.hpp"
+#include "handles.hpp"
+#include "internal.hpp"
+
+// FP64 expressions evaluate as written (no fused multiply-adds), as the
+// reference's do: the texture coordinates decide which pixels a stroke
+// covers, and the smudge's map offsets are truncations.
+#pragma clang fp contract(off)
+
+namespace pnt {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kPi = 3.141592653589793238462643383279502884;  // Math.hxx:21-22
+
+__host__ __device__ __forceinline__ int reflect_index(int p, int len) {
+  if (len == 1) return 0;
+  do {
+    if (p < 0)
+      p = -p - 1;
+    else if (p >= len)
+      p = 2 * len - p - 1;
+  } while (p < 0 || p >= len);
+  return p;
+}
+
+// Interpolate (image/Mat.hxx:69-100): bilinear with BORDER_REFLECT; element
+// (y, x) of the image is img[(y * cols + x) * step].
+template <class T>
+__device__ __forceinline__ double interp(const T* img, int rows, int cols, int step, double x,
+                                         double y) {
+  const int xi = (int)floor(x), yi = (int)floor(y);
+  const int x0 = reflect_index(xi, cols), x1 = reflect_index(xi + 1, cols);
+  const int y0 = reflect_index(yi, rows), y1 = reflect_index(yi + 1, rows);
+  const double a = x - (double)xi, c = y - (double)yi;
+  const auto at = [&](int yy, int xx) { return (double)img[((int64_t)yy * cols + xx) * step]; };
+  return (at(y0, x0) * (1.0 - a) + at(y0, x1) * a) * (1.0 - c) +
+         (at(y1, x0) * (1.0 - a) + at(y1, x1) * a) * c;
+}
+
+// generalizedBarycentricCoordinatesInterpolate (core/Math.hxx:73-133) of
+// the 2-d values `val` at the np vertices of polygon `poly`.
+__device__ void gbc(const double2* poly, const double2* val, int np, double px, double py,
+                    double* ou, double* ov) {
+  const double Eps = DBL_EPSILON * 100.0;
+  if (np == 1) {
+    *ou = val[0].x;
+    *ov = val[0].y;
+    return;
+  }
+  // r, A, D of each vertex on the fly: vertex i needs those of i - 1 and i
+  auto sv = [&](int i) { return make_double2(poly[i].x - px, poly[i].y - py); };
+  // the early exits of the reference scan the vertices in order first
+  for (int i = 0; i < np; ++i) {
+    const int i1 = i == np - 1 ? 0 : i + 1;
+    const double2 si = sv(i), si1 = sv(i1);
+    const double ri = sqrt(si.x * si.x + si.y * si.y);
+    if (fabs(ri - 0.0) < Eps) {
+      *ou = val[i].x;
+      *ov = val[i].y;
+      return;
+    }
+    const double A = (si.x * si1.y - si1.x * si.y) / 2.0;
+    const double D = si.x * si1.x + si.y * si1.y;
+    if (fabs(A - 0.0) < Eps && D < 0.0) {
+      const double ri1 = sqrt(si1.x * si1.x + si1.y * si1.y);
+      const double w = 1.0 / (ri + ri1);
+      *ou = (ri1 * val[i].x + ri * val[i1].x) * w;
+      *ov = (ri1 * val[i].y + ri * val[i1].y) * w;
+      return;
+    }
+  }
+  double fu = 0.0, fv = 0.0, W = 0.0;
+  // A, D, r of vertex np - 1 (the predecessor of vertex 0)
+  double2 sp = sv(np - 1), s0 = sv(0);
+  double rp = sqrt(sp.x * sp.x + sp.y * sp.y);
+  double Ap = (sp.x * s0.y - s0.x * sp.y) / 2.0, Dp = sp.x * s0.x + sp.y * s0.y;
+  double2 si = s0;
+  double ri = sqrt(si.x * si.x + si.y * si.y);
+  for (int i = 0; i < np; ++i) {
+    const int i1 = i == np - 1 ? 0 : i + 1;
+    const double2 si1 = sv(i1);
+    const double ri1 = sqrt(si1.x * si1.x + si1.y * si1.y);
+    const double Ai = (si.x * si1.y - si1.x * si.y) / 2.0;
+    const double Di = si.x * si1.x + si.y * si1.y;
+    double w = 0.0;
+    if (Ap != 0.0) w = w + (rp - Dp / ri) / Ap;
+    if (Ai != 0.0) w = w + (ri1 - Di / ri) / Ai;
+    fu = fu + w * val[i].x;
+    fv = fv + w * val[i].y;
+    W = W + w;
+    rp = ri;
+    Ap = Ai;
+    Dp = Di;
+    si = si1;
+    ri = ri1;
+  }
+  if (!(fabs(W - 0.0) < Eps)) {
+    *ou = fu * (1.0 / W);
+    *ov = fv * (1.0 / W);
+  } else {
+    *ou = val[0].x;
+    *ov = val[0].y;
+  }
+}
+
+struct TexArgs {
+  Layer7 canvas;
+  int rows, cols;
+  const double2* poly;  // frame polygon, np vertices
+  const double2* uv;    // texture coordinates at the vertices
+  int np;
+  const double* thick;  // brush stroke sample thickness map (FP64, as the reference's Mat<double>)
+  int t_rows, t_cols;
+  double scale;         // thickness scale
+  int bx0, by0, bx1, by1;  // pixel loop: (int) of the bounding box corners
+  int mx0, my0;         // (int) boundMin: origin of the stroke's thickness map
+  int trows, tcols;     // its size
+  double* tmap;         // the stroke's thickness map (trows x tcols), zeroed
+  DryClock clk;         // checkDry; clk.tmap == nullptr: off
+  double paint[6];      // the brush's K, S
+};
+
+__global__ __launch_bounds__(kBlock) void k_tex_sample(TexArgs A) {
+  const int w = A.bx1 - A.bx0 + 1;
+  const int64_t total = (int64_t)w * (A.by1 - A.by0 + 1);
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= total) return;
+  const int x = A.bx0 + (int)(e % w), y = A.by0 + (int)(e / w);
+  if (x < 0 || x >= A.cols || y < 0 || y >= A.rows) return;
+  double u, v;
+  gbc(A.poly, A.uv, A.np, (double)x, (double)y, &u, &v);
+  if (u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0) return;
+  u *= A.t_cols;
+  v *= A.t_rows;
+  const double vt = A.scale * interp(A.thick, A.t_rows, A.t_cols, 1, u, v);
+  if (!(vt > 0.0)) return;
+  const int s = x - A.mx0, t = y - A.my0;
+  if (s < 0 || t < 0 || s >= A.tcols || t >= A.trows) return;
+  if (A.clk.tmap) check_dry(A.clk, A.canvas, (int64_t)y * A.cols + x);
+  A.tmap[(int64_t)t * A.tcols + s] = vt;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tex_deposit(TexArgs A) {
+  const int w = A.bx1 - A.bx0 + 1;
+  const int64_t total = (int64_t)w * (A.by1 - A.by0 + 1);
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= total) return;
+  const int x = A.bx0 + (int)(e % w), y = A.by0 + (int)(e / w);
+  if (x < 0 || x >= A.cols || y < 0 || y >= A.rows) return;
+  const int s = x - A.mx0, t = y - A.my0;
+  if (s < 0 || t < 0 || s >= A.tcols || t >= A.trows) return;
+  const double vt = A.tmap[(int64_t)t * A.tcols + s];
+  if (!(vt > 0.0)) return;  // not textured: not in the reference's pixel list
+  const int64_t ic = (int64_t)y * A.cols + x;
+  const double vc = A.canvas.p[6][ic];
+  const double vs = vc + vt;
+  if (vs > 0.0) {
+    const double sc = 1.0 / vs;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      A.canvas.p[c][ic] = (float)((vc * A.canvas.p[c][ic] + vt * A.paint[c]) * sc);
+      A.canvas.p[3 + c][ic] = (float)((vc * A.canvas.p[3 + c][ic] + vt * A.paint[3 + c]) * sc);
+    }
+    A.canvas.p[6][ic] = (float)fmax(vt, vc);
+  }
+}
+
+// Largest value of the stroke's thickness map (one workgroup).
+__global__ __launch_bounds__(kBlock) void k_max(const double* a, int64_t n, double* out) {
+  __shared__ double red[kBlock];
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) m = fmax(m, a[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// A smudge pickup map: K, S (3 channels, interleaved per cell) and V.
+struct SMap {
+  double* K;
+  double* S;
+  double* V;
+};
+
+// Smudge::updateOrientation (Smudge.hxx:164-198): dst(x, y) = src at
+// (x, y) rotated by dtheta about (max/2, max/2), bilinear; unchanged where
+// the rotated position leaves the map.
+__global__ __launch_bounds__(kBlock) void k_smudge_turn(SMap src, SMap dst, int n, double cx,
+                                                        double cy, double sn, double cs) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int64_t)n * n) return;
+  const int x = (int)(i % n), y = (int)(i / n);
+  const double px = x - cx, py = y - cy;
+  const double rx = px * cs - py * sn + cx, ry = px * sn + py * cs + cy;
+  if (rx < 0 || ry < 0 || rx >= n || ry >= n) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      dst.K[3 * i + c] = src.K[3 * i + c];
+      dst.S[3 * i + c] = src.S[3 * i + c];
+    }
+    dst.V[i] = src.V[i];
+  } else {
+    dst.V[i] = interp(src.V, n, n, 1, rx, ry);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      dst.K[3 * i + c] = interp(src.K + c, n, n, 3, rx, ry);
+      dst.S[3 * i + c] = interp(src.S + c, n, n, 3, rx, ry);
+    }
+  }
+}
+
+// Smudge::smudge's exchange at one spine position (Smudge.hxx:56-157).
+__global__ __launch_bounds__(kBlock) void k_smudge_trade(SMap m, int n, Layer7 canvas, int rows,
+                                                         int cols, int roi_x, int roi_y,
+                                                         double bmin_x, double bmin_y, double ctr_x,
+                                                         double ctr_y, double radius,
+                                                         const double* tmap, int trows, int tcols,
+                                                         const double* maxD) {
+  const int64_t ip = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (ip >= (int64_t)n * n) return;
+  const int x = (int)(ip % n), y = (int)(ip / n);
+  const int cpx = x + roi_x, cpy = y + roi_y;
+  const int tpx = (int)(cpx - bmin_x), tpy = (int)(cpy - bmin_y);
+  if (cpx < 0 || cpy < 0 || cpx >= cols || cpy >= rows) return;
+  if (tpx < 0 || tpy < 0 || tpx >= tcols || tpy >= trows) return;
+  const double dx = ctr_x - (double)cpx, dy = ctr_y - (double)cpy;
+  if (sqrt(dx * dx + dy * dy) > radius) return;
+  const double Dv = tmap[(int64_t)tpy * tcols + tpx];
+  if (Dv <= 0.0) return;
+  constexpr double pickupRate = 0.1, depositionRate = 0.1, MinVolume = 0.001;
+  const double md = *maxD;
+  const int64_t ic = (int64_t)cpy * cols + cpx;
+  const double cV = canvas.p[6][ic], pV = m.V[ip];
+  const double cVl = cV * depositionRate * Dv / md, cVr = cV - cVl;
+  const double pVl = pV * pickupRate * Dv / md, pVr = pV - pVl;
+  double cK[3], cS[3], pK[3], pS[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    cK[c] = canvas.p[c][ic];
+    cS[c] = canvas.p[3 + c][ic];
+    pK[c] = m.K[3 * ip + c];
+    pS[c] = m.S[3 * ip + c];
+  }
+  const double pVnew = pVr + cVl;
+  if (pVnew > MinVolume) {
+    const double inv = 1.0 / pVnew;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      m.K[3 * ip + c] = inv * (pVr * pK[c] + cVl * cK[c]);
+      m.S[3 * ip + c] = inv * (pVr * pS[c] + cVl * cS[c]);
+    }
+    m.V[ip] = fmax(pVnew, 0.0);
+  }
+  const double cVnew = cVr + pVl;
+  if (cVnew > MinVolume) {
+    const double inv = 1. / cVnew;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      canvas.p[c][ic] = (float)(inv * (cVr * cK[c] + pVl * pK[c]));
+      canvas.p[3 + c][ic] = (float)(inv * (cVr * cS[c] + pVl * pS[c]));
+    }
+    canvas.p[6][ic] = (float)fmax(cVnew, 0.0);
+  }
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// SplineEval::catmullRom / catmullRomDerivativeFirst (core/Spline.hxx:150-
+// 221) over points v: u in [0, 1] spans the segments, indices clamped.
+void spline_cr(const std::vector<double2>& v, double u, bool deriv, double out[2]) {
+  const int n = (int)v.size();
+  const double x = (double)(n - 1) * u;
+  const int index = (int)x;
+  const double t = x - std::floor(x);
+  const double tau = 0.5, t2 = t * t, t3 = t2 * t;
+  double b[4];
+  if (!deriv) {
+    b[0] = -tau * t + 2.0 * tau * t2 - tau * t3;
+    b[1] = 1.0 + (tau - 3.0) * t2 + (2.0 - tau) * t3;
+    b[2] = tau * t + (3.0 - 2.0 * tau) * t2 + (tau - 2.0) * t3;
+    b[3] = -tau * t2 + tau * t3;
+  } else {
+    b[0] = tau * (-3.0 * t2 + 4.0 * t - 1.0);
+    b[1] = -t * (-2.0 * tau + 3.0 * (tau - 2.0) * t + 6.0);
+    b[2] = (t - 1.0) * (3.0 * (tau - 2.0) * t - tau);
+    b[3] = tau * t * (3.0 * t - 2.0);
+  }
+  double2 p[4];
+  for (int q = 0; q < 4; ++q) p[q] = v[(size_t)std::min(std::max(index - 1 + q, 0), n - 1)];
+  out[0] = p[0].x * b[0] + p[1].x * b[1] + p[2].x * b[2] + p[3].x * b[3];
+  out[1] = p[0].y * b[0] + p[1].y * b[1] + p[2].y * b[2] + p[3].y * b[3];
+}
+
+void normalize2(double t[2]) {  // Eigen's normalized(): unchanged when zero
+  const double z = t[0] * t[0] + t[1] * t[1];
+  if (z > 0.0) {
+    const double n = std::sqrt(z);
+    t[0] /= n;
+    t[1] /= n;
+  }
+}
+
+double normalize_angle(double a) {  // Smudge.hxx:200-210
+  const double half = 0.5 * kPi;
+  while (a <= -half) a += kPi;
+  while (a > half) a -= kPi;
+  return a;
+}
+
+}  // namespace
+
+int guard_call(const std::function<void()>& f);  // api.hip: status + pnt_last_error
+}  // namespace pnt
+
+using namespace pnt;
+
+struct pnt_tbrush_s {
+  pnt_ctx ctx = nullptr;
+  double radius = 0.0;
+  double paint[6] = {0.1, 0.1, 0.1, 0.1, 0.1, 0.1};  // TextureBrush.hxx:27-29
+  double* thick = nullptr;  // thickness map of the brush stroke sample
+  int t_rows = 0, t_cols = 0;
+  double scale = 1.0;      // BrushBase thickness scale
+  bool use_smudge = true;  // TextureBrush.hxx:237
+  // Smudge(size): maps of size x size cells, _maxSize = size made odd
+  int map = 0, max_size = 1;
+  double* smap[2] = {nullptr, nullptr};  // ping-pong: 7 doubles per cell
+  int cur = 0;             // the map in use
+  double cur_rot = 0.0;
+  // scratch of a stroke
+  double* geo = nullptr;   // polygon + uv + maxD
+  int64_t geo_cap = 0;
+  double* tmap = nullptr;
+  int64_t tmap_cap = 0;
+  SMap maps(int k) const {
+    const int64_t m = (int64_t)map * map;
+    return SMap{smap[k], smap[k] + 3 * m, smap[k] + 6 * m};
+  }
+};
+
+namespace pnt {
+namespace {
+void tb_smudge_reset(pnt_tbrush_s* b, int size) {
+  hipStream_t s = b->ctx->stream;
+  PNT_HIP_CHECK(hipStreamSynchronize(s));
+  for (double*& p : b->smap) {
+    dev_free(p);
+    p = nullptr;
+  }
+  b->map = std::max(size, 0);
+  b->max_size = (size % 2 == 0) ? size + 1 : size;
+  const int64_t m = (int64_t)b->map * b->map;
+  for (double*& p : b->smap) {
+    p = reinterpret_cast<double*>(dev_alloc(2 * std::max<int64_t>(7 * m, 1)));
+    launch_fill(reinterpret_cast<float*>(p), 2 * std::max<int64_t>(7 * m, 1), 0.0f, s);
+  }
+  b->cur = 0;
+  b->cur_rot = 0.0;
+}
+}  // namespace
+}  // namespace pnt
+
+extern "C" {
+
+int pnt_tbrush_create(pnt_ctx ctx, pnt_tbrush* out) {
+  return guard_call([&] {
+    PNT_REQUIRE(ctx && out, "NULL argument");
+    ctx->use();
+    auto* b = new pnt_tbrush_s;
+    b->ctx = ctx;
+    try {
+      tb_smudge_reset(b, 0);  // Smudge(2.0 * _radius) with _radius = 0
+    } catch (...) {
+      for (double* p : b->smap) dev_free(p);
+      delete b;
+      throw;
+    }
+    ctx_retain(ctx);
+    *out = b;
+  });
+}
+
+int pnt_tbrush_destroy(pnt_tbrush b) {
+  return guard_call([&] {
+    if (!b) return;
+    pnt_ctx ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (double* p : b->smap) dev_free(p);
+    dev_free(b->thick);
+    dev_free(b->geo);
+    dev_free(b->tmap);
+    delete b;
+    ctx_release(ctx);
+  });
+}
+
+int pnt_tbrush_set_radius(pnt_tbrush b, double radius) {
+  return guard_call([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    PNT_REQUIRE(std::isfinite(radius) && radius >= 0.0 && radius < 4096.0,
+                "radius must be in [0, 4096)");
+    b->ctx->use();
+    if (!(std::fabs(b->radius - radius) < 0.5)) {  // TextureBrush.hxx:31-39
+      b->radius = radius;
+      if (b->use_smudge) tb_smudge_reset(b, (int)(2.0 * radius));
+    }
+  });
+}
+
+int pnt_tbrush_dip(pnt_tbrush b, const double K[3], const double S[3]) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && K && S, "NULL argument");
+    for (int c = 0; c < 3; ++c) {
+      b->paint[c] = K[c];
+      b->paint[3 + c] = S[c];
+    }
+  });
+}
+
+int pnt_tbrush_set_thickness_map(pnt_tbrush b, const double* map, int rows, int cols) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && map, "NULL argument");
+    PNT_REQUIRE(rows > 0 && cols > 0 && (int64_t)rows * cols < ((int64_t)1 << 28),
+                "thickness map size out of range");
+    for (int64_t i = 0; i < (int64_t)rows * cols; ++i)
+      PNT_REQUIRE(std::isfinite(map[i]), "thickness map values must be finite");
+    b->ctx->use();
+    hipStream_t s = b->ctx->stream;
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    dev_free(b->thick);
+    b->thick = nullptr;
+    b->thick = reinterpret_cast<double*>(dev_alloc(2 * (int64_t)rows * cols));
+    PNT_HIP_CHECK(hipMemcpyAsync(b->thick, map, (size_t)rows * cols * sizeof(double),
+                                 hipMemcpyHostToDevice, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    b->t_rows = rows;
+    b->t_cols = cols;
+  });
+}
+
+int pnt_tbrush_set_thickness_scale(pnt_tbrush b, double scale) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && std::isfinite(scale), "invalid argument");
+    b->scale = scale;
+  });
+}
+
+int pnt_tbrush_set_use_smudge(pnt_tbrush b, int on) {
+  return guard_call([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    b->use_smudge = on != 0;
+  });
+}
+
+int pnt_tbrush_get_smudge_map(pnt_tbrush b, int* side, double* K, double* S, double* V) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && side, "NULL argument");
+    b->ctx->use();
+    *side = b->map;
+    const int64_t m = (int64_t)b->map * b->map;
+    hipStream_t s = b->ctx->stream;
+    const SMap mp = b->maps(b->cur);
+    if (K) PNT_HIP_CHECK(hipMemcpyAsync(K, mp.K, 3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (S) PNT_HIP_CHECK(hipMemcpyAsync(S, mp.S, 3 * m * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (V) PNT_HIP_CHECK(hipMemcpyAsync(V, mp.V, m * sizeof(double), hipMemcpyDeviceToHost, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int pnt_tbrush_paint_stroke(pnt_tbrush b, pnt_canvas cv, const double* xy, int64_t n) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && cv && (xy || n == 0) && n >= 0, "invalid argument");
+    PNT_REQUIRE(b->ctx->device == cv->ctx->device, "brush and canvas on different devices");
+    for (int64_t i = 0; i < 2 * n; ++i) PNT_REQUIRE(std::isfinite(xy[i]), "non-finite vertex");
+    if (n < 2) return;  // TextureBrush.hxx:54-56
+    PNT_REQUIRE(b->thick, "no thickness map (pnt_tbrush_set_thickness_map)");
+    b->ctx->use();
+    hipStream_t s = b->ctx->stream;
+    if (cv->ctx->stream != s) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    cv->epoch++;
+    const double r = b->radius;
+    // the stroke extended by the radius at both ends (TextureBrush.hxx:58-66)
+    std::vector<double2> v;
+    v.reserve((size_t)n + 2);
+    double d0[2] = {xy[2] - xy[0], xy[3] - xy[1]};
+    double d1[2] = {xy[2 * n - 2] - xy[2 * n - 4], xy[2 * n - 1] - xy[2 * n - 3]};
+    normalize2(d0);
+    normalize2(d1);
+    v.push_back(make_double2(xy[0] - d0[0] * r, xy[1] - d0[1] * r));
+    for (int64_t i = 0; i < n; ++i) v.push_back(make_double2(xy[2 * i], xy[2 * i + 1]));
+    v.push_back(make_double2(xy[2 * n - 2] + d1[0] * r, xy[2 * n - 1] + d1[1] * r));
+    const int nv = (int)v.size();
+    double bmin[2] = {v[0].x, v[0].y}, bmax[2] = {v[0].x, v[0].y};
+    for (const double2& p : v) {
+      bmin[0] = std::min(bmin[0], p.x);
+      bmin[1] = std::min(bmin[1], p.y);
+      bmax[0] = std::max(bmax[0], p.x);
+      bmax[1] = std::max(bmax[1], p.y);
+    }
+    bmin[0] = std::max(bmin[0] - r, 0.0);
+    bmax[0] = std::min(bmax[0] + r, (double)(cv->cols - 1));
+    bmin[1] = std::max(bmin[1] - r, 0.0);
+    bmax[1] = std::min(bmax[1] + r, (double)(cv->rows - 1));
+    double length = 0.0;
+    for (int i = 1; i < nv; ++i)
+      length += std::sqrt((v[i].x - v[i - 1].x) * (v[i].x - v[i - 1].x) +
+                          (v[i].y - v[i - 1].y) * (v[i].y - v[i - 1].y));
+    // frame polygon: the right side backwards, then the left side (:95-121)
+    const int np = 2 * nv;
+    std::vector<double2> poly((size_t)np), uv((size_t)np);
+    for (int i = 0; i < nv; ++i) {
+      const double u = (double)i / (double)(nv - 1);
+      double c[2], t[2];
+      spline_cr(v, u, false, c);
+      spline_cr(v, u, true, t);
+      normalize2(t);
+      const double d[2] = {-t[1], t[0]};
+      poly[(size_t)(nv + i)] = make_double2(c[0] - r * d[0], c[1] - r * d[1]);
+      uv[(size_t)(nv + i)] = make_double2(u, 0.0);
+      poly[(size_t)(nv - 1 - i)] = make_double2(c[0] + r * d[0], c[1] + r * d[1]);
+      uv[(size_t)(nv - 1 - i)] = make_double2(u, 1.0);
+    }
+    const int trows = (int)(bmax[1] - bmin[1] + 1), tcols = (int)(bmax[0] - bmin[0] + 1);
+    // an empty thickness map covers no pixel; a negative size is an error
+    // (the reference's Mat would refuse it)
+    PNT_REQUIRE(trows >= 0 && tcols >= 0,
+                "stroke entirely off the canvas: its thickness map would have a negative size");
+    if (trows == 0 || tcols == 0) return;
+    const int64_t need = 4 * (int64_t)np + 2;
+    if (need > b->geo_cap) {
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+      dev_free(b->geo);
+      b->geo = nullptr;
+      b->geo = reinterpret_cast<double*>(dev_alloc(2 * need));
+      b->geo_cap = need;
+    }
+    const int64_t tn = (int64_t)trows * tcols;
+    if (tn > b->tmap_cap) {
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+      dev_free(b->tmap);
+      b->tmap = nullptr;
+      b->tmap = reinterpret_cast<double*>(dev_alloc(2 * tn));
+      b->tmap_cap = tn;
+    }
+    PNT_HIP_CHECK(hipMemcpyAsync(b->geo, poly.data(), np * sizeof(double2), hipMemcpyHostToDevice, s));
+    PNT_HIP_CHECK(hipMemcpyAsync(b->geo + 2 * np, uv.data(), np * sizeof(double2),
+                                 hipMemcpyHostToDevice, s));
+    launch_fill(reinterpret_cast<float*>(b->tmap), 2 * tn, 0.0f, s);
+    TexArgs A{};
+    A.canvas = cv->layer();
+    A.rows = cv->rows;
+    A.cols = cv->cols;
+    A.poly = reinterpret_cast<const double2*>(b->geo);
+    A.uv = reinterpret_cast<const double2*>(b->geo + 2 * np);
+    A.np = np;
+    A.thick = b->thick;
+    A.t_rows = b->t_rows;
+    A.t_cols = b->t_cols;
+    A.scale = b->scale;
+    A.bx0 = (int)bmin[0];
+    A.by0 = (int)bmin[1];
+    A.bx1 = (int)bmax[0];
+    A.by1 = (int)bmax[1];
+    A.mx0 = (int)bmin[0];
+    A.my0 = (int)bmin[1];
+    A.trows = trows;
+    A.tcols = tcols;
+    A.tmap = b->tmap;
+    for (int c = 0; c < 6; ++c) A.paint[c] = b->paint[c];
+    A.clk = DryClock{};
+    if (cv->tmap)  // Canvas::checkDry on every textured pixel (TextureBrush.hxx:133, 164)
+      A.clk = DryClock{cv->tmap, canvas_now_us(cv), cv->drying_ms, cv->R0(), cv->h()};
+    const int64_t npx = (int64_t)(A.bx1 - A.bx0 + 1) * (A.by1 - A.by0 + 1);
+    k_tex_sample<<<blocks_for(npx), kBlock, 0, s>>>(A);
+    PNT_HIP_CHECK(hipGetLastError());
+    if (b->use_smudge && b->map > 0) {  // Smudge::smudge (Smudge.hxx:40-158)
+      double* maxD = b->geo + 4 * np;
+      k_max<<<1, kBlock, 0, s>>>(b->tmap, tn, maxD);
+      PNT_HIP_CHECK(hipGetLastError());
+      double maxD_h = 0.0;
+      PNT_HIP_CHECK(hipMemcpyAsync(&maxD_h, maxD, sizeof(double), hipMemcpyDeviceToHost, s));
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+      // one turn and one trade per unit of length: bound the launches
+      PNT_REQUIRE(!(length > 0.0) || 1.0 / length >= 1.0 / (double)(1 << 24),
+                  "stroke too long for the smudge (more than 2^24 spine steps)");
+      PNT_REQUIRE(!(length > 0.0) || length >= 1e-3,
+                  "stroke shorter than 1e-3 pixels but not empty: too many smudge steps");
+      const int64_t m = (int64_t)b->map * b->map;
+      for (double u = 0.0; maxD_h > 0.0 && u <= 1.0; u += 1. / length) {
+        double center[2], t[2];
+        spline_cr(v, u, false, center);
+        spline_cr(v, u, true, t);
+        normalize2(t);
+        const double theta = std::atan2(t[1], t[0]);
+        const double dtheta = normalize_angle(theta - b->cur_rot);
+        b->cur_rot = theta;
+        k_smudge_turn<<<blocks_for(m), kBlock, 0, s>>>(
+            b->maps(b->cur), b->maps(1 - b->cur), b->map, b->max_size / 2.0, b->max_size / 2.0,
+            std::sin(dtheta), std::cos(dtheta));
+        b->cur = 1 - b->cur;
+        const int roi_x = (int)(center[0] - b->map / 2.0);
+        const int roi_y = (int)(center[1] - b->map / 2.0);
+        k_smudge_trade<<<blocks_for(m), kBlock, 0, s>>>(
+            b->maps(b->cur), b->map, cv->layer(), cv->rows, cv->cols, roi_x, roi_y, bmin[0],
+            bmin[1], center[0], center[1], b->max_size * 0.5, b->tmap, trows, tcols, maxD);
+        PNT_HIP_CHECK(hipGetLastError());
+      }
+    }
+    k_tex_deposit<<<blocks_for(npx), kBlock, 0, s>>>(A);
+    PNT_HIP_CHECK(hipGetLastError());
+    PNT_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors above end here
+  });
+}
+
+}  // extern "C"

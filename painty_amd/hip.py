@@ -7,6 +7,7 @@ so that code and tests read like the reference's own:
   Canvas          renderer/Canvas.hxx:20-198
   Renderer        renderer/Renderer.hxx:14-53
   FootprintBrush  renderer/FootprintBrush.hxx:22-502
+  TextureBrush    renderer/TextureBrush.hxx:19-238 (with Smudge.hxx:22-229)
   PaintMixer      mixer/PaintMixer.hxx:22-57 (mixSinglePaint, mixed)
   ComputeReflectance  core/KubelkaMunk.hxx:25-83
 
@@ -454,6 +455,68 @@ class FootprintBrush:
     def paintStroke(self, path, canvas: Canvas) -> None:
         p = np.ascontiguousarray(path, dtype=np.float64).reshape(-1, 2)
         call("pnt_paint_stroke", self.h, canvas.h, p.ctypes.data_as(_dp), len(p))
+
+
+class TextureBrush:
+    """Brush stroke texture warped along a stroke, with smudging
+    (renderer/TextureBrush.hxx:19-238).  The reference loads the sample's
+    thickness map from ``sampleDir/thickness_map.png``
+    (renderer/src/BrushStrokeSample.cxx:164); here the caller passes the
+    map (rows x cols, float64) to ``setThicknessMap``."""
+
+    def __init__(self, thickness_map=None, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        call("pnt_tbrush_create", self.ctx.h, C.byref(h))
+        self.h = h
+        if thickness_map is not None:
+            self.setThicknessMap(thickness_map)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                load().pnt_tbrush_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def setRadius(self, radius: float) -> None:
+        call("pnt_tbrush_set_radius", self.h, float(radius))
+
+    def dip(self, paint) -> None:
+        """paint = (K, S), 3 values each (TextureBrush.hxx:48-50)."""
+        K = np.ascontiguousarray(paint[0], np.float64).reshape(3)
+        S = np.ascontiguousarray(paint[1], np.float64).reshape(3)
+        call("pnt_tbrush_dip", self.h, K.ctypes.data_as(_dp), S.ctypes.data_as(_dp))
+
+    def setThicknessMap(self, m) -> None:
+        m = np.ascontiguousarray(m, np.float64)
+        if m.ndim != 2:
+            raise ValueError("thickness map must be 2-d")
+        call("pnt_tbrush_set_thickness_map", self.h, m.ctypes.data_as(_dp), int(m.shape[0]),
+             int(m.shape[1]))
+
+    def setThicknessScale(self, scale: float) -> None:
+        call("pnt_tbrush_set_thickness_scale", self.h, float(scale))
+
+    def enableSmudge(self, enable: bool) -> None:
+        call("pnt_tbrush_set_use_smudge", self.h, int(bool(enable)))
+
+    def getSmudgeMap(self):
+        """The smudge's pickup map: K, S (side, side, 3) and V (side, side)."""
+        side = C.c_int()
+        call("pnt_tbrush_get_smudge_map", self.h, C.byref(side), None, None, None)
+        n = side.value
+        K, S, V = np.zeros((n, n, 3)), np.zeros((n, n, 3)), np.zeros((n, n))
+        call("pnt_tbrush_get_smudge_map", self.h, C.byref(side), K.ctypes.data_as(_dp),
+             S.ctypes.data_as(_dp), V.ctypes.data_as(_dp))
+        return K, S, V
+
+    def paintStroke(self, vertices, canvas: Canvas) -> None:
+        """TextureBrush.hxx:52-205; vertices: (n, 2) x, y."""
+        xy = np.ascontiguousarray(vertices, np.float64).reshape(-1, 2)
+        call("pnt_tbrush_paint_stroke", self.h, canvas.h, xy.ctypes.data_as(_dp),
+             int(xy.shape[0]))
 
 
 def stamps_to_c(stamps):

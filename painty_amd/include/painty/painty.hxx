@@ -6,3 +6,4 @@
 #include "painty/mixer/Serialization.hxx"
 #include "painty/renderer/FootprintBrush.hxx"
 #include "painty/renderer/Renderer.hxx"
+#include "painty/renderer/TextureBrush.hxx"

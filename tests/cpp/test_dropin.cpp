@@ -26,6 +26,7 @@ using painty::FootprintBrush;
 using painty::Mat;
 using painty::PaintLayer;
 using painty::Renderer;
+using painty::TextureBrush;
 using painty::vec2;
 using painty::vec3;
 
@@ -150,6 +151,16 @@ bool throws_invalid(F f) {
   return false;
 }
 
+template <class F>
+bool throws_ios(F f) {
+  try {
+    f();
+  } catch (const std::ios_base::failure&) {
+    return true;
+  }
+  return false;
+}
+
 int api() {
   // PaintMixer (PaintMixer.cxx:327-355, 536-548)
   painty::Palette pal(3);
@@ -239,6 +250,35 @@ int api() {
       EXPECT(k1.getPaintLayer().getK_buffer()(i)[1] == k2.getPaintLayer().getK_buffer()(i)[1]);
     }
     EXPECT(b1.getFootprint().rows == b2.getFootprint().rows);
+  }
+  // TextureBrush as the reference's TextureBrushTest.cxx:16-36 drives it (a
+  // synthetic thickness map; the sample directory needs a loader)
+  {
+    Mat<double> tex(20, 60);
+    for (int i = 0; i < 20; ++i)
+      for (int j = 0; j < 60; ++j) tex(i, j) = 0.2 + 0.01 * j + 0.005 * i;
+    EXPECT(throws_ios([] { TextureBrush<vec3> t("data/sample_0"); }));
+    TextureBrush<vec3> tb(tex);
+    tb.dip({{{0.2, 0.3, 0.4}, {0.1, 0.23, 0.14}}});
+    Canvas<vec3> c(96, 128);
+    c.useClock(PNT_CLOCK_OFF);
+    c.clear();
+    tb.setRadius(8.0);
+    tb.paintStroke({{6.0, 40.0}, {60.0, 40.0}, {110.0, 40.0}}, c);
+    const Canvas<vec3>& k = c;
+    int painted = 0;
+    for (int i = 0; i < 96 * 128; ++i) painted += k.getPaintLayer().getV_buffer()(i) > 0.0;
+    EXPECT(painted > 1000);
+    EXPECT(std::abs(k.getPaintLayer().getK_buffer()(40, 60)[2] - 0.4) < 1e-6);
+    std::vector<double> mV;
+    EXPECT(tb.getSmudgeMap(nullptr, nullptr, &mV) == 16);
+    double held = 0.0;
+    for (double v : mV) held += v;
+    EXPECT(held == 0.0);  // a clean canvas gives the smudge nothing to pick up
+    tb.paintStroke({{10.0, 30.0}, {100.0, 50.0}}, c);
+    tb.getSmudgeMap(nullptr, nullptr, &mV);
+    for (double v : mV) held += v;
+    EXPECT(held > 0.0);
   }
   // palette JSON round trip and the thinning medium (Serialization.cxx:26-53,
   // Palette.cxx:453-463); no device needed
