@@ -432,6 +432,62 @@ def bench_copy(args):
                       "gb_s": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
 
 
+def bench_texture(args):
+    """TextureBrush::paintStroke with smudging on the reference test's stroke
+    (renderer/test/src/TextureBrushTest.cxx:16-36): a 768 x 1024 canvas,
+    radius 40, the path (50, 250) -> (400, 250) -> (650, 250), over a canvas
+    already holding paint (one stroke before the timed ones), with a
+    synthetic 43 x 200 brush stroke sample.  One step is one stroke."""
+    from painty_amd import hip as H
+    rows, cols, radius = 768, 1024, 40.0
+    rng = np.random.default_rng(7)
+    ty, tx = np.mgrid[0:43, 0:200]
+    tex = np.clip(np.sin(np.pi * ty / 42.0) * (0.6 + 0.4 * rng.random((43, 200))), 0, 1)
+    path = np.array([[50.0, 250.0], [400.0, 250.0], [650.0, 250.0]])
+    ctx = H.default_context()
+    cv = H.Canvas(rows, cols)
+    cv.clear()
+    br = H.TextureBrush(tex)
+    br.setRadius(radius)
+    br.dip(((0.2, 0.3, 0.4), (0.1, 0.23, 0.14)))
+    br.paintStroke(path + [0.0, 3.5], cv)
+    ev = HipEvents()
+    for _ in range(args.warmup):
+        br.paintStroke(path, cv)
+    ctx.synchronize()
+    e0, e1 = ev.event(), ev.event()
+    t0 = time.perf_counter()
+    ev.record(e0, ctx.stream)
+    for _ in range(args.steps):
+        br.paintStroke(path, cv)
+    ev.record(e1, ctx.stream)
+    ctx.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    gpu_ms = ev.elapsed_ms(e0, e1) / args.steps
+    result = {"metric": "texture-brush strokes/s (smudge on)", "value": round(1e3 / wall_ms, 2),
+              "unit": "strokes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+              "ms_per_step": round(wall_ms, 3), "stream_ms_per_step": round(gpu_ms, 3),
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+              "data": "synthetic",
+              "config": {"workload": "TextureBrushTest stroke: 768x1024 canvas, r=40, "
+                                     "(50,250)->(400,250)->(650,250), smudge on"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        oc = O.Canvas(rows, cols)
+        ob = O.TextureBrush()
+        ob.set_radius(radius)
+        ob.set_thickness_map(tex)
+        ob.dip([0.2, 0.3, 0.4], [0.1, 0.23, 0.14])
+        ob.paint_stroke(path + [0.0, 3.5], oc)
+        t0 = time.perf_counter()
+        ob.paint_stroke(path, oc)
+        t = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": round(1.0 / t, 3), "unit": "strokes/s", "cores": 1,
+                                  "kind": "port", "sample": "one stroke of the same job, FP64 "
+                                  "oracle (%.2fs)" % t}
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -443,7 +499,8 @@ def main():
     ap.add_argument("--cpu-stamps", type=int, default=300)
     ap.add_argument("--cpu-rows", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["single", "batched", "copy", "diff", "served"],
+    ap.add_argument("--config", choices=["single", "batched", "copy", "diff", "served",
+                                         "texture"],
                     default="single")
     ap.add_argument("--seg-len", type=int, default=32)
     ap.add_argument("--rates", default="200,500,1000,2000")
@@ -473,6 +530,8 @@ def main():
         return bench_diff(args)
     if args.config == "served":
         return bench_served(args)
+    if args.config == "texture":
+        return bench_texture(args)
 
     from painty_amd import _lib
     from painty_amd import hip as H
@@ -606,8 +665,15 @@ def main():
 
     stamps_s = args.stamps / (ms_per_step * 1e-3)
     stamp_gbs = stamp_bytes / (st_ms * 1e-3) / 1e9
+    waves = one_call and os.environ.get("PAINTY_FUSED_SCHED", "waves") != "streams"
     kernel = {"plain": "k_stamp_main+k_ring_refresh", "persistent": "k_persistent",
-              "fused": "k_stamp_fused<16>", "chained": "k_stamp_chain<16>"}[args.mode]
+              "fused": "k_stamp_wave<16>" if waves else "k_stamp_fused<16>",
+              "chained": "k_stamp_chain<16>"}[args.mode]
+    launches = None
+    if one_call:  # stamp-kernel launches of one step's stroke-list call
+        nl = C.c_int64(0)
+        _lib.call("pnt_brush_last_launches", brush.h, C.byref(nl))
+        launches = int(nl.value)
     result = {
         "metric": METRIC, "value": round(stamps_s, 1), "unit": "stamps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
@@ -626,11 +692,19 @@ def main():
                      "achieved": round(stamp_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(stamp_gbs / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(kernel),
+                     "launches_per_step": launches,
+                     "algorithmic_bytes_per_launch": int(stamp_bytes / launches)
+                     if launches else None,
+                     "avg_launch_us": round(st_ms * 1e3 / launches, 3) if launches else None,
                      "algorithmic_bytes_per_step": int(stamp_bytes),
                      "stamp_time": "stream time of the stamp calls"
-                                   + (" (incl. %d dries)" % n_dry if one_call else ""),
-                     "concurrent_chains": int(os.environ.get("PAINTY_FUSED_CHAINS", "4"))
-                     if one_call else 1},
+                                   + (" (incl. %d dries and the gaps between launches; "
+                                      "avg_launch_us = that time / launches)" % n_dry
+                                      if one_call else ""),
+                     "concurrent_chains": int(os.environ.get("PAINTY_FUSED_CHAINS",
+                                                             "8" if waves else "4"))
+                     if one_call else 1,
+                     "scheduler": ("waves" if waves else "streams") if one_call else None},
         "roofline_composite": {"bound": "hbm", "kernel": "k_compose_vec<1>",
                                "achieved": round(comp_gbs, 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(comp_gbs / HBM_PEAK_GBS, 4)},

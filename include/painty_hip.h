@@ -9,7 +9,7 @@
  * C/FFI layer.  Each entry point below replaces one member or free function
  * of those templates; the citation gives the reference file:line.  The C++
  * classes in painty_amd/include/painty/ (Canvas, PaintLayer, FootprintBrush,
- * Renderer, PaintMixer) are the drop-in binding over this ABI.
+ * TextureBrush, Renderer, PaintMixer) are the drop-in binding over this ABI.
  *
  * Conventions
  *   - Every function returns PNT_OK (0) or a negative status; the message of
@@ -328,6 +328,11 @@ int pnt_paint_stroke(pnt_brush brush, pnt_canvas canvas, const double* path_xy,
 int pnt_brush_stamp_traffic(pnt_brush brush, int rows, int cols,
                             const pnt_stamp* stamps, int64_t n,
                             int64_t* active_visits, int64_t* ring_pixels);
+
+/* Measurement helper (no reference counterpart): the number of stamp-kernel
+ * launches (k_stamp_fused, k_stamp_levels, k_stamp_wave) the brush's last
+ * PNT_EXEC_FUSED stamp-list call issued; 0 after other modes. */
+int pnt_brush_last_launches(pnt_brush brush, int64_t* launches);
 
 /* --------------------------------------------------------- PaintMixer --- */
 /* mixSinglePaint(weights) -- mixer/src/PaintMixer.cxx:327-355: K = sum_l

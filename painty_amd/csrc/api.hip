@@ -108,20 +108,47 @@ void dev_free(void* p) {
 
 // FootprintBrush::setRadius geometry (FootprintBrush.hxx:52-61) with an
 // injected width x width footprint, padded with zeros to size_map^2.
-std::vector<float> padded_footprint(double radius, const float* core, int width, int* size_map_out) {
+int footprint_size_map(double radius, const float* core, int width) {
   PNT_REQUIRE(core != nullptr, "footprint is NULL");
   PNT_REQUIRE(std::isfinite(radius) && radius > 0.0 && radius < 4096.0,
               "radius must be in (0, 4096)");
   const int w = (int)(2.0 * std::ceil(radius) + 1.0);
   PNT_REQUIRE(w == width, "footprint width must be 2*ceil(radius)+1");
-  for (int64_t i = 0; i < (int64_t)w * w; ++i)
-    PNT_REQUIRE(std::isfinite(core[i]) && core[i] >= 0.0f && core[i] <= 1.0f,
-                "footprint values must be in [0, 1]");
-  const int size_map = (int)std::ceil(std::sqrt(2.0) * w);
+  return (int)std::ceil(std::sqrt(2.0) * w);
+}
+
+void pad_footprint_into(const float* core, int w, int size_map, float* dst, int box[4]) {
   const int pad = (size_map - w) / 2;
-  std::vector<float> full((size_t)size_map * size_map, 0.0f);
-  for (int i = 0; i < w; ++i)
-    for (int j = 0; j < w; ++j) full[(size_t)(i + pad) * size_map + (j + pad)] = core[i * w + j];
+  box[0] = box[1] = size_map;
+  box[2] = box[3] = -1;
+  std::memset(dst, 0, sizeof(float) * (size_t)pad * size_map);
+  for (int i = 0; i < w; ++i) {
+    float* row = dst + (size_t)(i + pad) * size_map;
+    const float* src = core + (size_t)i * w;
+    std::memset(row, 0, sizeof(float) * (size_t)pad);
+    for (int j = 0; j < w; ++j) {
+      const float v = src[j];
+      PNT_REQUIRE(std::isfinite(v) && v >= 0.0f && v <= 1.0f,
+                  "footprint values must be in [0, 1]");
+      row[pad + j] = v;
+      if (v > 0.0f) {
+        box[0] = std::min(box[0], pad + j);
+        box[2] = std::max(box[2], pad + j);
+        box[1] = std::min(box[1], pad + i);
+        box[3] = std::max(box[3], pad + i);
+      }
+    }
+    std::memset(row + pad + w, 0, sizeof(float) * (size_t)(size_map - pad - w));
+  }
+  std::memset(dst + (size_t)(pad + w) * size_map, 0,
+              sizeof(float) * (size_t)(size_map - pad - w) * size_map);
+}
+
+std::vector<float> padded_footprint(double radius, const float* core, int width, int* size_map_out) {
+  const int size_map = footprint_size_map(radius, core, width);
+  std::vector<float> full((size_t)size_map * size_map);
+  int box[4];
+  pad_footprint_into(core, width, size_map, full.data(), box);
   *size_map_out = size_map;
   return full;
 }
@@ -1369,6 +1396,13 @@ int pnt_imprint_strokes(pnt_brush b, pnt_canvas cv, const pnt_stroke* S, int64_t
       }
     }
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+int pnt_brush_last_launches(pnt_brush b, int64_t* launches) {
+  return guard([&] {
+    PNT_REQUIRE(b && launches, "NULL argument");
+    *launches = b->last_launches;
   });
 }
 

@@ -17,10 +17,13 @@
 // entirely, and tile t of every launch runs on the same XCD
 // (tile_of_block), so a tile's pixels and map cells stay in that XCD's L2.
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <thread>
 #include <vector>
 
 #include "handles.hpp"
@@ -344,6 +347,7 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
                     const PlanDev& D, const std::vector<ListGeo>& geos, const std::vector<int>& geo,
                     const std::vector<uint8_t>& clear, int want_chains) {
   hipStream_t s = b->ctx->stream;
+  b->last_launches = 0;
   FArgs A0;
   A0.st = D.st;
   A0.canvas = cv->layer();
@@ -394,10 +398,12 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
     const int jk = (int)k;
     if (!(p.flags & (kStDupX | kStDupY))) {
       k_stamp_fused<kT><<<a.ntiles + rb, kT * kT, 0, q>>>(a, jk, a.ntiles, rb);
+      b->last_launches += 1;
     } else {  // snapshot refresh first: a duplicated pixel is not refreshed by its visits
       if (rb) k_stamp_fused<kT><<<rb, kT * kT, 0, q>>>(a, jk, 0, rb);
       k_stamp_fused<kT><<<a.ntiles, kT * kT, 0, q>>>(a, jk, a.ntiles, 0);
       if (p.dl_nlev) k_stamp_levels<<<1, 256, 0, q>>>(a, jk);
+      b->last_launches += (rb ? 1 : 0) + 1 + (p.dl_nlev ? 1 : 0);
     }
     PNT_HIP_CHECK(hipGetLastError());
   };
@@ -522,6 +528,7 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
         }
         k_stamp_wave<kT><<<(unsigned)start, T2, 0, s>>>(W);
         PNT_HIP_CHECK(hipGetLastError());
+        b->last_launches += 1;
         for (int i : ready)
           if (++next[(size_t)i] == ph[(size_t)i].size()) done[(size_t)i] = 1;
       }
@@ -649,7 +656,7 @@ template <int kT>
 void run_fused_strokes_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* S, int64_t ns) {
   std::vector<ListGeo> geos{ListGeo{b->fp, b->size_map, b->radius, nullptr, b->width,
                                     FpBox{b->fbox[0], b->fbox[1], b->fbox[2], b->fbox[3]}}};
-  std::vector<float> fps;
+  int64_t fp_floats = 0;
   std::vector<int64_t> fp_off{0};
   std::vector<PlanPart> parts;
   std::vector<pnt_stamp> all;
@@ -660,15 +667,10 @@ void run_fused_strokes_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* S, 
   bool pending_clear = false;  // setRadius cleaned the map; the next stamp starts from it
   for (int64_t i = 0; i < ns; ++i) {
     if (!(std::fabs(cur_r - S[i].radius) < 0.5)) {  // a new geometry, a clean map
-      int sm = 0;
-      std::vector<float> full = padded_footprint(S[i].radius, S[i].footprint, S[i].width, &sm);
-      fp_off.push_back((int64_t)fps.size());
-      fps.insert(fps.end(), full.begin(), full.end());
-      fps.resize((size_t)round_up((int64_t)fps.size(), 64), 0.0f);
-      int box[4];
-      footprint_box(full, sm, box);
-      geos.push_back(ListGeo{nullptr, sm, S[i].radius, S[i].footprint, S[i].width,
-                             FpBox{box[0], box[1], box[2], box[3]}});
+      const int sm = footprint_size_map(S[i].radius, S[i].footprint, S[i].width);
+      fp_off.push_back(fp_floats);
+      fp_floats = round_up(fp_floats + (int64_t)sm * sm, 64);
+      geos.push_back(ListGeo{nullptr, sm, S[i].radius, S[i].footprint, S[i].width, FpBox{}});
       cur = (int)geos.size() - 1;
       cur_r = S[i].radius;
       pending_clear = true;
@@ -685,7 +687,40 @@ void run_fused_strokes_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* S, 
   const int64_t n = (int64_t)all.size();
   if (n > 0) {
     const float* d_fp = nullptr;
-    const PlanDev D = upload_plan_parts(b, cv, parts.data(), (int)parts.size(), fps, &d_fp);
+    // the padded footprints go straight into the plan's pinned staging
+    // buffer, one host thread per few geometries
+    const auto fill = [&](float* dst) {
+      const int ng = (int)geos.size() - 1;
+      const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+      const int nt = std::min({hw, 8, (int)(1 + fp_floats / (1 << 18)), ng});
+      std::atomic<int> next{1};
+      std::vector<std::exception_ptr> errs((size_t)std::max(nt, 1));
+      auto work = [&](int t) {
+        try {
+          for (int g; (g = next.fetch_add(1)) <= ng;) {
+            ListGeo& G = geos[(size_t)g];
+            int box[4];
+            pad_footprint_into(G.core, G.width, G.size_map, dst + fp_off[(size_t)g], box);
+            G.fb = FpBox{box[0], box[1], box[2], box[3]};
+          }
+        } catch (...) {
+          errs[(size_t)t] = std::current_exception();
+          next.store(ng + 1);
+        }
+      };
+      if (nt <= 1) {
+        work(0);
+      } else {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+        work(0);
+        for (auto& th : pool) th.join();
+      }
+      for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
+    };
+    const PlanDev D =
+        upload_plan_parts(b, cv, parts.data(), (int)parts.size(), fp_floats, fill, &d_fp);
     for (size_t g = 1; g < geos.size(); ++g) geos[g].fp = d_fp + fp_off[g];
     run_fused_list<kT>(b, cv, all.data(), n, D, geos, geo, clear, 0);
   }

@@ -125,6 +125,7 @@ struct pnt_brush_s {
   pnt::PersistentState* pers = nullptr;
   pnt::ChainState* chain = nullptr;  // buffers of PNT_EXEC_CHAINED runs
   pnt::FusedState* fused = nullptr;  // streams and maps of concurrent fused strokes
+  int64_t last_launches = 0;  // stamp-kernel launches of the last fused stamp-list call
 
   pnt::Layer7 pick_planes() const {
     pnt::Layer7 L;
@@ -166,6 +167,11 @@ void brush_set_geometry(pnt_brush b, double radius, const float* core, int width
 // The footprint padded with zeros to size_map^2 (FootprintBrush.hxx:52-58,
 // padded to exactly sizeMap), after the checks brush_set_geometry makes.
 std::vector<float> padded_footprint(double radius, const float* core, int width, int* size_map);
+// The two steps of padded_footprint: the map side (validating radius and
+// width), and one pass that validates the core, writes it centred in a
+// size_map^2 zero map at dst and gives the box of its nonzero cells.
+int footprint_size_map(double radius, const float* core, int width);
+void pad_footprint_into(const float* core, int width, int size_map, float* dst, int box[4]);
 // The bounding box (x0, y0, x1, y1) of the nonzero cells of a padded
 // size_map^2 footprint; empty (x1 < x0) when all are zero.
 void footprint_box(const std::vector<float>& fp, int size_map, int box[4]);

@@ -446,11 +446,12 @@ static PersistentState& plan_state(pnt_brush_s* b) {
 // stays in the brush's state until the next call.
 PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
   const PlanPart part{b->size_map, b->radius, st, n};
-  return upload_plan_parts(b, cv, &part, 1, {}, nullptr);
+  return upload_plan_parts(b, cv, &part, 1, 0, nullptr, nullptr);
 }
 
 PlanDev upload_plan_parts(pnt_brush_s* b, pnt_canvas_s* cv, const PlanPart* parts, int np,
-                          const std::vector<float>& fps, const float** d_fp) {
+                          int64_t fp_floats, const std::function<void(float*)>& fill_fp,
+                          const float** d_fp) {
   hipStream_t s = b->ctx->stream;
   PersistentState& P = plan_state(b);
   P.h_st.clear();
@@ -482,7 +483,7 @@ PlanDev upload_plan_parts(pnt_brush_s* b, pnt_canvas_s* cv, const PlanPart* part
   const size_t bst = P.h_st.size() * sizeof(PStamp);
   const size_t bdl = P.h_dl.size() * sizeof(int2);
   const size_t boff = P.h_off.size() * sizeof(int);
-  const size_t bfp = fps.size() * sizeof(float);
+  const size_t bfp = (size_t)fp_floats * sizeof(float);
   const int64_t need = (int64_t)(bst + bdl + boff + bfp);
   PNT_HIP_CHECK(hipEventSynchronize(P.uploaded));  // previous plan has been read
   if (need > P.stage_cap) {
@@ -495,7 +496,7 @@ PlanDev upload_plan_parts(pnt_brush_s* b, pnt_canvas_s* cv, const PlanPart* part
   std::memcpy(P.stage, P.h_st.data(), bst);
   if (bdl) std::memcpy(P.stage + bst, P.h_dl.data(), bdl);
   if (boff) std::memcpy(P.stage + bst + bdl, P.h_off.data(), boff);
-  if (bfp) std::memcpy(P.stage + bst + bdl + boff, fps.data(), bfp);
+  if (bfp) fill_fp(reinterpret_cast<float*>(P.stage + bst + bdl + boff));
   const int64_t n = (int64_t)P.h_st.size();
   ensure_dev(P.d_st, P.st_cap, std::max<int64_t>(n, 1), s);
   ensure_dev(P.d_dl, P.dl_cap, std::max<int64_t>((int64_t)P.h_dl.size(), 1), s);
@@ -506,7 +507,7 @@ PlanDev upload_plan_parts(pnt_brush_s* b, pnt_canvas_s* cv, const PlanPart* part
   if (boff)
     PNT_HIP_CHECK(hipMemcpyAsync(P.d_off, P.stage + bst + bdl, boff, hipMemcpyHostToDevice, s));
   if (bfp) {
-    ensure_dev(P.d_fp, P.fp_cap, (int64_t)fps.size(), s);
+    ensure_dev(P.d_fp, P.fp_cap, fp_floats, s);
     PNT_HIP_CHECK(hipMemcpyAsync(P.d_fp, P.stage + bst + bdl + boff, bfp, hipMemcpyHostToDevice,
                                  s));
   }

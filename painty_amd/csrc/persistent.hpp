@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <vector>
 
 #include "painty_hip.h"
@@ -29,8 +30,9 @@ struct PlanDev {
 };
 PlanDev upload_plan(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 // A list cut into parts that each have their own brush geometry (padded
-// footprint side, radius); `fps` (footprints of the list's geometries)
-// travels with the plan, to *d_fp.
+// footprint side, radius).  The footprints of the list's geometries
+// (fp_floats floats) travel with the plan, to *d_fp: fill_fp writes them
+// into the pinned staging buffer once the previous plan's upload is done.
 struct PlanPart {
   int size_map;
   double radius;
@@ -38,7 +40,8 @@ struct PlanPart {
   int64_t n;
 };
 PlanDev upload_plan_parts(pnt_brush_s* b, pnt_canvas_s* cv, const PlanPart* parts, int np,
-                          const std::vector<float>& fps, const float** d_fp);
+                          int64_t fp_floats, const std::function<void(float*)>& fill_fp,
+                          const float** d_fp);
 
 // chains: concurrent stroke chains at most (0: PAINTY_FUSED_CHAINS or 4)
 void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n, int chains = 0);

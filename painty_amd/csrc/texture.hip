@@ -11,18 +11,22 @@
 //                  FP64 as the reference; the texture sampled bilinearly
 //                  (Interpolate, image/Mat.hxx:69-100, reflected borders);
 //                  Canvas::checkDry on textured pixels when a clock is on;
-//   k_smudge_turn  Smudge::updateOrientation: the pickup map resampled at
-//                  the rotated positions (ping-pong buffers, FP64 maps);
-//   k_smudge_trade Smudge::smudge's exchange at one spine position;
+//                  the largest Vtex folded in per workgroup (the smudge's
+//                  maximum thickness);
+//   k_smudge_step  one spine position of Smudge::smudge: the pickup map
+//                  turned (updateOrientation, resampled at the rotated
+//                  positions into the other of two FP64 buffers), then the
+//                  cell's exchange with the canvas pixel under it;
 //   k_tex_deposit  the volume-weighted mix.
-// The smudge runs one turn and one trade per unit of stroke length, in
-// stream order; each is independent per cell (a map cell owns one canvas
-// pixel at a time).
+// The smudge runs one step per unit of stroke length, in stream order.  A
+// step is independent per cell: a cell reads the previous map and writes
+// only itself, and a map cell meets one canvas pixel at a time.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 #include <functional>
 #include <vector>
 
@@ -147,26 +151,41 @@ struct TexArgs {
   double* tmap;         // the stroke's thickness map (trows x tcols), zeroed
   DryClock clk;         // checkDry; clk.tmap == nullptr: off
   double paint[6];      // the brush's K, S
+  unsigned long long* maxd;  // largest Vtex, as the bits of a non-negative double (zeroed)
 };
 
-__global__ __launch_bounds__(kBlock) void k_tex_sample(TexArgs A) {
+__device__ double tex_sample_pixel(const TexArgs& A, int64_t e) {
   const int w = A.bx1 - A.bx0 + 1;
   const int64_t total = (int64_t)w * (A.by1 - A.by0 + 1);
-  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (e >= total) return;
+  if (e >= total) return 0.0;
   const int x = A.bx0 + (int)(e % w), y = A.by0 + (int)(e / w);
-  if (x < 0 || x >= A.cols || y < 0 || y >= A.rows) return;
+  if (x < 0 || x >= A.cols || y < 0 || y >= A.rows) return 0.0;
   double u, v;
   gbc(A.poly, A.uv, A.np, (double)x, (double)y, &u, &v);
-  if (u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0) return;
+  if (u < 0.0 || u > 1.0 || v < 0.0 || v > 1.0) return 0.0;
   u *= A.t_cols;
   v *= A.t_rows;
   const double vt = A.scale * interp(A.thick, A.t_rows, A.t_cols, 1, u, v);
-  if (!(vt > 0.0)) return;
+  if (!(vt > 0.0)) return 0.0;
   const int s = x - A.mx0, t = y - A.my0;
-  if (s < 0 || t < 0 || s >= A.tcols || t >= A.trows) return;
+  if (s < 0 || t < 0 || s >= A.tcols || t >= A.trows) return 0.0;
   if (A.clk.tmap) check_dry(A.clk, A.canvas, (int64_t)y * A.cols + x);
   A.tmap[(int64_t)t * A.tcols + s] = vt;
+  return vt;
+}
+
+// One thread per bounding-box pixel; each workgroup also folds the largest
+// Vtex it stored into *A.maxd (ordered as integers: Vtex > 0).
+__global__ __launch_bounds__(kBlock) void k_tex_sample(TexArgs A) {
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = tex_sample_pixel(A, (int64_t)blockIdx.x * kBlock + threadIdx.x);
+  __syncthreads();
+  for (int k = kBlock / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0] > 0.0)
+    atomicMax(A.maxd, (unsigned long long)__double_as_longlong(red[0]));
 }
 
 __global__ __launch_bounds__(kBlock) void k_tex_deposit(TexArgs A) {
@@ -194,20 +213,6 @@ __global__ __launch_bounds__(kBlock) void k_tex_deposit(TexArgs A) {
   }
 }
 
-// Largest value of the stroke's thickness map (one workgroup).
-__global__ __launch_bounds__(kBlock) void k_max(const double* a, int64_t n, double* out) {
-  __shared__ double red[kBlock];
-  double m = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kBlock) m = fmax(m, a[i]);
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int k = kBlock / 2; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + k]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = red[0];
-}
-
 // A smudge pickup map: K, S (3 channels, interleaved per cell) and V.
 struct SMap {
   double* K;
@@ -215,85 +220,105 @@ struct SMap {
   double* V;
 };
 
-// Smudge::updateOrientation (Smudge.hxx:164-198): dst(x, y) = src at
-// (x, y) rotated by dtheta about (max/2, max/2), bilinear; unchanged where
-// the rotated position leaves the map.
-__global__ __launch_bounds__(kBlock) void k_smudge_turn(SMap src, SMap dst, int n, double cx,
-                                                        double cy, double sn, double cs) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= (int64_t)n * n) return;
-  const int x = (int)(i % n), y = (int)(i / n);
-  const double px = x - cx, py = y - cy;
-  const double rx = px * cs - py * sn + cx, ry = px * sn + py * cs + cy;
-  if (rx < 0 || ry < 0 || rx >= n || ry >= n) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      dst.K[3 * i + c] = src.K[3 * i + c];
-      dst.S[3 * i + c] = src.S[3 * i + c];
-    }
-    dst.V[i] = src.V[i];
-  } else {
-    dst.V[i] = interp(src.V, n, n, 1, rx, ry);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      dst.K[3 * i + c] = interp(src.K + c, n, n, 3, rx, ry);
-      dst.S[3 * i + c] = interp(src.S + c, n, n, 3, rx, ry);
-    }
-  }
-}
+// Where a smudge step meets the canvas: the spine position and the map's
+// top-left canvas pixel, the stroke's thickness map and its largest value.
+struct TradeArgs {
+  Layer7 canvas;
+  int rows, cols;
+  int roi_x, roi_y;
+  double bmin_x, bmin_y, ctr_x, ctr_y, radius;
+  const double* tmap;
+  int trows, tcols;
+  double maxD;
+};
 
-// Smudge::smudge's exchange at one spine position (Smudge.hxx:56-157).
-__global__ __launch_bounds__(kBlock) void k_smudge_trade(SMap m, int n, Layer7 canvas, int rows,
-                                                         int cols, int roi_x, int roi_y,
-                                                         double bmin_x, double bmin_y, double ctr_x,
-                                                         double ctr_y, double radius,
-                                                         const double* tmap, int trows, int tcols,
-                                                         const double* maxD) {
+// One spine position of Smudge::smudge (Smudge.hxx:49-157), for map cell
+// ip = y * n + x:
+//  - updateOrientation (Smudge.hxx:164-198): dst(x, y) = src at (x, y)
+//    rotated by dtheta about (max/2, max/2), bilinear; src(x, y) where the
+//    rotated position leaves the map;
+//  - the exchange of dst(x, y) with the canvas pixel under it.
+__global__ __launch_bounds__(kBlock) void k_smudge_step(SMap src, SMap dst, int n, double cx,
+                                                        double cy, double sn, double cs,
+                                                        TradeArgs T) {
   const int64_t ip = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (ip >= (int64_t)n * n) return;
   const int x = (int)(ip % n), y = (int)(ip / n);
-  const int cpx = x + roi_x, cpy = y + roi_y;
-  const int tpx = (int)(cpx - bmin_x), tpy = (int)(cpy - bmin_y);
-  if (cpx < 0 || cpy < 0 || cpx >= cols || cpy >= rows) return;
-  if (tpx < 0 || tpy < 0 || tpx >= tcols || tpy >= trows) return;
-  const double dx = ctr_x - (double)cpx, dy = ctr_y - (double)cpy;
-  if (sqrt(dx * dx + dy * dy) > radius) return;
-  const double Dv = tmap[(int64_t)tpy * tcols + tpx];
-  if (Dv <= 0.0) return;
-  constexpr double pickupRate = 0.1, depositionRate = 0.1, MinVolume = 0.001;
-  const double md = *maxD;
-  const int64_t ic = (int64_t)cpy * cols + cpx;
-  const double cV = canvas.p[6][ic], pV = m.V[ip];
-  const double cVl = cV * depositionRate * Dv / md, cVr = cV - cVl;
-  const double pVl = pV * pickupRate * Dv / md, pVr = pV - pVl;
-  double cK[3], cS[3], pK[3], pS[3];
+  double pK[3], pS[3], pV;
+  {
+    const double px = x - cx, py = y - cy;
+    const double rx = px * cs - py * sn + cx, ry = px * sn + py * cs + cy;
+    if (rx < 0 || ry < 0 || rx >= n || ry >= n) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        pK[c] = src.K[3 * ip + c];
+        pS[c] = src.S[3 * ip + c];
+      }
+      pV = src.V[ip];
+    } else {
+      pV = interp(src.V, n, n, 1, rx, ry);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        pK[c] = interp(src.K + c, n, n, 3, rx, ry);
+        pS[c] = interp(src.S + c, n, n, 3, rx, ry);
+      }
+    }
+  }
+  const int cpx = x + T.roi_x, cpy = y + T.roi_y;
+  const int tpx = (int)(cpx - T.bmin_x), tpy = (int)(cpy - T.bmin_y);
+  bool trade = !(cpx < 0 || cpy < 0 || cpx >= T.cols || cpy >= T.rows) &&
+               !(tpx < 0 || tpy < 0 || tpx >= T.tcols || tpy >= T.trows);
+  double Dv = 0.0;
+  if (trade) {
+    const double dx = T.ctr_x - (double)cpx, dy = T.ctr_y - (double)cpy;
+    trade = !(sqrt(dx * dx + dy * dy) > T.radius);
+  }
+  if (trade) {
+    Dv = T.tmap[(int64_t)tpy * T.tcols + tpx];
+    trade = !(Dv <= 0.0);
+  }
+  if (trade) {
+    constexpr double pickupRate = 0.1, depositionRate = 0.1, MinVolume = 0.001;
+    const double md = T.maxD;
+    const int64_t ic = (int64_t)cpy * T.cols + cpx;
+    const double cV = T.canvas.p[6][ic];
+    const double cVl = cV * depositionRate * Dv / md, cVr = cV - cVl;
+    const double pVl = pV * pickupRate * Dv / md, pVr = pV - pVl;
+    double cK[3], cS[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cK[c] = T.canvas.p[c][ic];
+      cS[c] = T.canvas.p[3 + c][ic];
+    }
+    const double cVnew = cVr + pVl;
+    if (cVnew > MinVolume) {
+      const double inv = 1. / cVnew;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        T.canvas.p[c][ic] = (float)(inv * (cVr * cK[c] + pVl * pK[c]));
+        T.canvas.p[3 + c][ic] = (float)(inv * (cVr * cS[c] + pVl * pS[c]));
+      }
+      T.canvas.p[6][ic] = (float)fmax(cVnew, 0.0);
+    }
+    const double pVnew = pVr + cVl;
+    if (pVnew > MinVolume) {
+      const double inv = 1.0 / pVnew;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double k = inv * (pVr * pK[c] + cVl * cK[c]);
+        const double q = inv * (pVr * pS[c] + cVl * cS[c]);
+        pK[c] = k;
+        pS[c] = q;
+      }
+      pV = fmax(pVnew, 0.0);
+    }
+  }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    cK[c] = canvas.p[c][ic];
-    cS[c] = canvas.p[3 + c][ic];
-    pK[c] = m.K[3 * ip + c];
-    pS[c] = m.S[3 * ip + c];
+    dst.K[3 * ip + c] = pK[c];
+    dst.S[3 * ip + c] = pS[c];
   }
-  const double pVnew = pVr + cVl;
-  if (pVnew > MinVolume) {
-    const double inv = 1.0 / pVnew;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      m.K[3 * ip + c] = inv * (pVr * pK[c] + cVl * cK[c]);
-      m.S[3 * ip + c] = inv * (pVr * pS[c] + cVl * cS[c]);
-    }
-    m.V[ip] = fmax(pVnew, 0.0);
-  }
-  const double cVnew = cVr + pVl;
-  if (cVnew > MinVolume) {
-    const double inv = 1. / cVnew;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      canvas.p[c][ic] = (float)(inv * (cVr * cK[c] + pVl * pK[c]));
-      canvas.p[3 + c][ic] = (float)(inv * (cVr * cS[c] + pVl * pS[c]));
-    }
-    canvas.p[6][ic] = (float)fmax(cVnew, 0.0);
-  }
+  dst.V[ip] = pV;
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -560,7 +585,7 @@ int pnt_tbrush_paint_stroke(pnt_tbrush b, pnt_canvas cv, const double* xy, int64
     PNT_REQUIRE(trows >= 0 && tcols >= 0,
                 "stroke entirely off the canvas: its thickness map would have a negative size");
     if (trows == 0 || tcols == 0) return;
-    const int64_t need = 4 * (int64_t)np + 2;
+    const int64_t need = 4 * (int64_t)np + 1;
     if (need > b->geo_cap) {
       PNT_HIP_CHECK(hipStreamSynchronize(s));
       dev_free(b->geo);
@@ -576,9 +601,11 @@ int pnt_tbrush_paint_stroke(pnt_tbrush b, pnt_canvas cv, const double* xy, int64
       b->tmap = reinterpret_cast<double*>(dev_alloc(2 * tn));
       b->tmap_cap = tn;
     }
-    PNT_HIP_CHECK(hipMemcpyAsync(b->geo, poly.data(), np * sizeof(double2), hipMemcpyHostToDevice, s));
-    PNT_HIP_CHECK(hipMemcpyAsync(b->geo + 2 * np, uv.data(), np * sizeof(double2),
-                                 hipMemcpyHostToDevice, s));
+    // polygon, texture coordinates, and a zero for the largest Vtex
+    std::vector<double> hg((size_t)need, 0.0);
+    std::memcpy(hg.data(), poly.data(), np * sizeof(double2));
+    std::memcpy(hg.data() + 2 * np, uv.data(), np * sizeof(double2));
+    PNT_HIP_CHECK(hipMemcpyAsync(b->geo, hg.data(), need * sizeof(double), hipMemcpyHostToDevice, s));
     launch_fill(reinterpret_cast<float*>(b->tmap), 2 * tn, 0.0f, s);
     TexArgs A{};
     A.canvas = cv->layer();
@@ -600,6 +627,7 @@ int pnt_tbrush_paint_stroke(pnt_tbrush b, pnt_canvas cv, const double* xy, int64
     A.trows = trows;
     A.tcols = tcols;
     A.tmap = b->tmap;
+    A.maxd = reinterpret_cast<unsigned long long*>(b->geo + 4 * np);
     for (int c = 0; c < 6; ++c) A.paint[c] = b->paint[c];
     A.clk = DryClock{};
     if (cv->tmap)  // Canvas::checkDry on every textured pixel (TextureBrush.hxx:133, 164)
@@ -608,11 +636,9 @@ int pnt_tbrush_paint_stroke(pnt_tbrush b, pnt_canvas cv, const double* xy, int64
     k_tex_sample<<<blocks_for(npx), kBlock, 0, s>>>(A);
     PNT_HIP_CHECK(hipGetLastError());
     if (b->use_smudge && b->map > 0) {  // Smudge::smudge (Smudge.hxx:40-158)
-      double* maxD = b->geo + 4 * np;
-      k_max<<<1, kBlock, 0, s>>>(b->tmap, tn, maxD);
-      PNT_HIP_CHECK(hipGetLastError());
       double maxD_h = 0.0;
-      PNT_HIP_CHECK(hipMemcpyAsync(&maxD_h, maxD, sizeof(double), hipMemcpyDeviceToHost, s));
+      PNT_HIP_CHECK(hipMemcpyAsync(&maxD_h, b->geo + 4 * np, sizeof(double), hipMemcpyDeviceToHost,
+                                   s));
       PNT_HIP_CHECK(hipStreamSynchronize(s));
       // one turn and one trade per unit of length: bound the launches
       PNT_REQUIRE(!(length > 0.0) || 1.0 / length >= 1.0 / (double)(1 << 24),
@@ -620,6 +646,17 @@ int pnt_tbrush_paint_stroke(pnt_tbrush b, pnt_canvas cv, const double* xy, int64
       PNT_REQUIRE(!(length > 0.0) || length >= 1e-3,
                   "stroke shorter than 1e-3 pixels but not empty: too many smudge steps");
       const int64_t m = (int64_t)b->map * b->map;
+      TradeArgs T{};
+      T.canvas = cv->layer();
+      T.rows = cv->rows;
+      T.cols = cv->cols;
+      T.bmin_x = bmin[0];
+      T.bmin_y = bmin[1];
+      T.radius = b->max_size * 0.5;
+      T.tmap = b->tmap;
+      T.trows = trows;
+      T.tcols = tcols;
+      T.maxD = maxD_h;
       for (double u = 0.0; maxD_h > 0.0 && u <= 1.0; u += 1. / length) {
         double center[2], t[2];
         spline_cr(v, u, false, center);
@@ -628,16 +665,16 @@ int pnt_tbrush_paint_stroke(pnt_tbrush b, pnt_canvas cv, const double* xy, int64
         const double theta = std::atan2(t[1], t[0]);
         const double dtheta = normalize_angle(theta - b->cur_rot);
         b->cur_rot = theta;
-        k_smudge_turn<<<blocks_for(m), kBlock, 0, s>>>(
-            b->maps(b->cur), b->maps(1 - b->cur), b->map, b->max_size / 2.0, b->max_size / 2.0,
-            std::sin(dtheta), std::cos(dtheta));
-        b->cur = 1 - b->cur;
-        const int roi_x = (int)(center[0] - b->map / 2.0);
-        const int roi_y = (int)(center[1] - b->map / 2.0);
-        k_smudge_trade<<<blocks_for(m), kBlock, 0, s>>>(
-            b->maps(b->cur), b->map, cv->layer(), cv->rows, cv->cols, roi_x, roi_y, bmin[0],
-            bmin[1], center[0], center[1], b->max_size * 0.5, b->tmap, trows, tcols, maxD);
+        T.roi_x = (int)(center[0] - b->map / 2.0);
+        T.roi_y = (int)(center[1] - b->map / 2.0);
+        T.ctr_x = center[0];
+        T.ctr_y = center[1];
+        k_smudge_step<<<blocks_for(m), kBlock, 0, s>>>(b->maps(b->cur), b->maps(1 - b->cur),
+                                                       b->map, b->max_size / 2.0,
+                                                       b->max_size / 2.0, std::sin(dtheta),
+                                                       std::cos(dtheta), T);
         PNT_HIP_CHECK(hipGetLastError());
+        b->cur = 1 - b->cur;
       }
     }
     k_tex_deposit<<<blocks_for(npx), kBlock, 0, s>>>(A);
