@@ -181,8 +181,11 @@ int pnt_compose_layer_host(pnt_layer layer, const float* R0, float* R,
 /* ------------------------------------- FootprintBrush<vec3> ------------- */
 /* FootprintBrush(radius) -- renderer/FootprintBrush.hxx:29-63.  The
  * reference loads ./data/footprint/footprint.png and resizes it to
- * width = 2*ceil(radius)+1; here the width x width footprint (values in
- * [0,1]) is passed in.  It is padded with zeros to sizeMap =
+ * width = 2*ceil(radius)+1 (pnt_image_read_gray + pnt_resize_lanczos4 do
+ * the same); here the width x width footprint (finite values; the Lanczos
+ * resize rings a little below 0 and above 1, and cells <= 0 take no part
+ * in pickup or deposit, FootprintBrush.hxx:355, 398) is passed in.  It is
+ * padded with zeros to sizeMap =
  * ceil(sqrt(2)*width) on each side (the reference pads one short when
  * sizeMap-width is odd and reads past its buffer; that row/col reads 0). */
 int pnt_brush_create(pnt_ctx ctx, double radius, const float* footprint,
@@ -501,6 +504,24 @@ int pnt_tbrush_get_smudge_map(pnt_tbrush brush, int* side, double* K,
  * or when the smudge would take more than 2^24 spine steps. */
 int pnt_tbrush_paint_stroke(pnt_tbrush brush, pnt_canvas canvas,
                             const double* xy, int64_t n);
+
+/* ---------------------------------------------------- footprint ingest - */
+/* io::imRead(buffer, Mat<double>& gray, convertFrom_sRGB) -- io/src/
+ * ImageIO.cxx:115-150 (the file form, :78-113, reads the same bytes): a PNG
+ * file's grey samples (8 or 16 bit; colour type grey or grey + alpha, the
+ * alpha dropped as IMREAD_GRAYSCALE does; not interlaced), times 1/255 or
+ * 1/65535, then sRGB -> linear (core/Color.hxx:189-195) when srgb != 0.
+ * Sets *rows, *cols; fills out (row-major, capacity >= rows*cols) unless it
+ * is NULL.  Other PNG kinds fail with PNT_EINVAL. */
+int pnt_image_read_gray(const uint8_t* data, int64_t len, int srgb, int* rows,
+                        int* cols, double* out, int64_t capacity);
+/* ScaledMat(input, rows, cols) -- image/Mat.hxx:141-148: cv::resize with
+ * INTER_LANCZOS4 (OpenCV's 8-tap kernel, float weights, replicated borders,
+ * FP64 sums) of a single-channel FP64 image, on the device.  The footprint
+ * of FootprintBrush::setRadius (FootprintBrush.hxx:48-58) is
+ * ScaledMat(imRead(footprint.png), width, width). */
+int pnt_resize_lanczos4(pnt_ctx ctx, const double* src, int rows, int cols,
+                        int out_rows, int out_cols, double* out);
 
 /* ---------------------------------------------------- canvas device view - */
 /* Device planes of a canvas (valid while it lives): layer (7), R0 (3), h. */

@@ -103,6 +103,11 @@ def lib():
         L.orc_gbc.argtypes = [_dp, C.c_int, C.c_double, C.c_double, _dp, _dp]
         L.orc_tbrush_set_smudge_map.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_tbrush_turn.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_srgb2rgb.restype = C.c_double
+        L.orc_srgb2rgb.argtypes = [C.c_double]
+        L.orc_gray_from_u16.argtypes = [C.c_int64, C.POINTER(C.c_uint16), C.c_int, C.c_int, _dp]
+        L.orc_lanczos4_coeffs.argtypes = [C.c_float, C.POINTER(C.c_float)]
+        L.orc_resize_lanczos4.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_mix_single.restype = C.c_int
         L.orc_mix_single.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.orc_mixed.argtypes = [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp]
@@ -476,3 +481,32 @@ class TextureBrush:
                                 int(dry_ms), _p(canvas.R0), _p(canvas.h)))
         lib().orc_tbrush_paint_stroke(self.h, canvas.rows, canvas.cols, _p(canvas.K),
                                       _p(canvas.S), _p(canvas.V), _p(xy), int(xy.shape[0]), clk)
+
+
+# ----------------------------------------------------- footprint ingest -----
+def srgb2rgb(s: float) -> float:
+    return lib().orc_srgb2rgb(float(s))
+
+
+def gray_from_u16(v, maxval: int, srgb: bool) -> np.ndarray:
+    """imRead's pixel transform of raw grey samples (uint16 array)."""
+    a = np.ascontiguousarray(v, dtype=np.uint16)
+    out = np.empty(a.shape)
+    lib().orc_gray_from_u16(a.size, a.ctypes.data_as(C.POINTER(C.c_uint16)), int(maxval),
+                            1 if srgb else 0, _p(out))
+    return out
+
+
+def lanczos4_coeffs(x: float) -> np.ndarray:
+    c = (C.c_float * 8)()
+    lib().orc_lanczos4_coeffs(float(x), c)
+    return np.array(c[:], dtype=np.float32)
+
+
+def resize_lanczos4(img, out_rows: int, out_cols: int) -> np.ndarray:
+    """ScaledMat(img, out_rows, out_cols): cv::resize INTER_LANCZOS4, FP64."""
+    a = _f64(img)
+    out = np.empty((out_rows, out_cols))
+    lib().orc_resize_lanczos4(_p(a), a.shape[0], a.shape[1], int(out_rows), int(out_cols),
+                              _p(out))
+    return out

@@ -1563,3 +1563,95 @@ void orc_tbrush_paint_stroke(orc_tbrush* b, int rows, int cols, double* K, doubl
   free(uv);
   free(v);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Footprint ingest (FootprintBrush.hxx:48-58): the PNG's grey values through
+ * io::imRead, then ScaledMat.  Decoding the PNG itself is checked against an
+ * independent decoder in the tests, not restated here. */
+
+/* ColorConverter::srgb2rgb (core/Color.hxx:189-195) */
+double orc_srgb2rgb(double s) {
+  if (s <= 0.0404482362771082) return s / 12.92;
+  return pow((s + 0.055) / 1.055, 2.4);
+}
+
+/* imRead's scale (io/src/ImageIO.cxx:91-104): Mat::convertTo(CV_64FC1, scale)
+ * multiplies by scale = 1.0 / 0xff or 1.0 / 0xffff */
+void orc_gray_from_u16(int64_t n, const uint16_t* v, int maxval, int srgb, double* out) {
+  const double scale = maxval == 65535 ? 1.0 / 0xffff : 1.0 / 0xff;
+  for (int64_t i = 0; i < n; i++) {
+    const double x = (double)v[i] * scale;
+    out[i] = srgb ? orc_srgb2rgb(x) : x;
+  }
+}
+
+/* OpenCV's interpolateLanczos4: L(t) = sin(pi t) sin(pi t / 4) / (pi^2 t^2 / 4)
+ * at t = x + 3 - i, i = 0..7, through one sin/cos pair and the angle-sum
+ * table, each rounded to float, then normalised by the float sum. */
+void orc_lanczos4_coeffs(float x, float c[8]) {
+  static const double s45 = 0.70710678118654752440084436210485;
+  static const double cs[8][2] = {{1, 0},   {-s45, -s45}, {0, 1},  {s45, -s45},
+                                  {-1, 0},  {s45, s45},   {0, -1}, {-s45, s45}};
+  if (x < FLT_EPSILON) {
+    for (int i = 0; i < 8; i++) c[i] = 0.0f;
+    c[3] = 1.0f;
+    return;
+  }
+  float sum = 0.0f;
+  const double pi = 3.14159265358979323846;
+  const float xp3 = x + 3.0f;
+  const double y0 = -(double)xp3 * pi * 0.25, s0 = sin(y0), c0 = cos(y0);
+  for (int i = 0; i < 8; i++) {
+    const float xi = x + 3.0f - (float)i;
+    const double y = -(double)xi * pi * 0.25;
+    c[i] = (float)((cs[i][0] * s0 + cs[i][1] * c0) / (y * y));
+    sum += c[i];
+  }
+  sum = 1.0f / sum;
+  for (int i = 0; i < 8; i++) c[i] *= sum;
+}
+
+static void lanczos_axis(int ssize, int dsize, int* ofs, float* alpha) {
+  const double inv_scale = (double)dsize / ssize;
+  const double scale = 1.0 / inv_scale;
+  for (int d = 0; d < dsize; d++) {
+    float f = (float)((d + 0.5) * scale - 0.5);
+    const int s = (int)floorf(f);
+    f -= (float)s;
+    ofs[d] = s;
+    orc_lanczos4_coeffs(f, alpha + 8 * d);
+  }
+}
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+void orc_resize_lanczos4(const double* src, int rows, int cols, int out_rows, int out_cols,
+                         double* dst) {
+  int* xofs = (int*)malloc(sizeof(int) * (size_t)out_cols);
+  int* yofs = (int*)malloc(sizeof(int) * (size_t)out_rows);
+  float* xa = (float*)malloc(sizeof(float) * 8 * (size_t)out_cols);
+  float* ya = (float*)malloc(sizeof(float) * 8 * (size_t)out_rows);
+  double* tmp = (double*)malloc(sizeof(double) * (size_t)rows * out_cols);
+  lanczos_axis(cols, out_cols, xofs, xa);
+  lanczos_axis(rows, out_rows, yofs, ya);
+  for (int r = 0; r < rows; r++)
+    for (int dx = 0; dx < out_cols; dx++) {
+      double v = 0.0;
+      for (int j = 0; j < 8; j++)
+        v += src[(int64_t)r * cols + clampi(xofs[dx] - 3 + j, 0, cols - 1)] * (double)xa[8 * dx + j];
+      tmp[(int64_t)r * out_cols + dx] = v;
+    }
+  for (int dy = 0; dy < out_rows; dy++)
+    for (int dx = 0; dx < out_cols; dx++) {
+      double v = 0.0;
+      for (int k = 0; k < 8; k++)
+        v += tmp[(int64_t)clampi(yofs[dy] - 3 + k, 0, rows - 1) * out_cols + dx] *
+             (double)ya[8 * dy + k];
+      dst[(int64_t)dy * out_cols + dx] = v;
+    }
+  free(xofs);
+  free(yofs);
+  free(xa);
+  free(ya);
+  free(tmp);
+}

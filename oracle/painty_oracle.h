@@ -205,6 +205,25 @@ void orc_tbrush_set_smudge_map(orc_tbrush* b, const double* K, const double* S,
                                const double* V);
 void orc_tbrush_turn(orc_tbrush* b, double hx, double hy);
 
+/* ---- Footprint ingest: io::imRead's pixel transform and ScaledMat ----
+ * orc_srgb2rgb: ColorConverter::srgb2rgb (core/Color.hxx:189-195).
+ * orc_gray_from_u16: imRead(.., Mat<double>&, convertFrom_sRGB)
+ * (io/src/ImageIO.cxx:78-113): v * (1/maxval) in FP64 (Mat::convertTo),
+ * then srgb2rgb when asked; maxval 255 or 65535.
+ * orc_lanczos4_coeffs / orc_resize_lanczos4: ScaledMat (image/Mat.hxx:
+ * 141-148) = cv::resize(INTER_LANCZOS4) of an FP64 single-channel image,
+ * restating OpenCV 4.x's published resize (imgproc/src/resize.cpp:
+ * interpolateLanczos4, resizeGeneric_ with HResizeLanczos4 /
+ * VResizeLanczos4<double, double, float>): source position
+ * fx = (float)((dx + 0.5) * scale - 0.5) with scale = 1 / (dst / src),
+ * 8 taps from floor(fx) - 3, float coefficients, replicated borders, FP64
+ * sums in tap order. */
+double orc_srgb2rgb(double s);
+void orc_gray_from_u16(int64_t n, const uint16_t* v, int maxval, int srgb, double* out);
+void orc_lanczos4_coeffs(float x, float c[8]);
+void orc_resize_lanczos4(const double* src, int rows, int cols, int out_rows, int out_cols,
+                         double* dst);
+
 #ifdef __cplusplus
 }
 #endif

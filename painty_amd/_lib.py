@@ -146,6 +146,9 @@ SIGNATURES = {
                               C.POINTER(C.c_int64)],
     "pnt_palette_to_json": [_dp, _dp, C.c_int64, C.c_char_p, C.c_int64, C.POINTER(C.c_int64)],
     "pnt_thinning_medium": [_dp, _dp],
+    "pnt_image_read_gray": [C.c_char_p, C.c_int64, C.c_int, C.POINTER(C.c_int),
+                            C.POINTER(C.c_int), _dp, C.c_int64],
+    "pnt_resize_lanczos4": [_vp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
     "pnt_tbrush_create": [_vp, C.POINTER(_vp)],
     "pnt_tbrush_destroy": [_vp],
     "pnt_tbrush_set_radius": [_vp, C.c_double],

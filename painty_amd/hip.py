@@ -457,6 +457,40 @@ class FootprintBrush:
         call("pnt_paint_stroke", self.h, canvas.h, p.ctypes.data_as(_dp), len(p))
 
 
+def read_image_gray(data: bytes, srgb: bool = True) -> np.ndarray:
+    """io::imRead(buffer, Mat<double>&, convertFrom_sRGB) of a PNG file's
+    bytes (io/src/ImageIO.cxx:115-150): grey samples / maxval, sRGB ->
+    linear when srgb.  Host work; no device needed."""
+    data = bytes(data)
+    r, c = C.c_int(), C.c_int()
+    call("pnt_image_read_gray", data, len(data), int(bool(srgb)), C.byref(r), C.byref(c), None, 0)
+    out = np.empty((r.value, c.value))
+    call("pnt_image_read_gray", data, len(data), int(bool(srgb)), C.byref(r), C.byref(c),
+         out.ctypes.data_as(_dp), out.size)
+    return out
+
+
+def resize_lanczos4(img, rows: int, cols: int, ctx: Context | None = None) -> np.ndarray:
+    """ScaledMat(img, rows, cols) (image/Mat.hxx:141-148): cv::resize with
+    INTER_LANCZOS4, FP64, on the device."""
+    ctx = ctx or default_context()
+    a = np.ascontiguousarray(img, np.float64)
+    out = np.empty((int(rows), int(cols)))
+    call("pnt_resize_lanczos4", ctx.h, a.ctypes.data_as(_dp), a.shape[0], a.shape[1], int(rows),
+         int(cols), out.ctypes.data_as(_dp))
+    return out
+
+
+def footprint_from_image(img, radius: float, ctx: Context | None = None) -> np.ndarray:
+    """The footprint FootprintBrush::setRadius builds (FootprintBrush.hxx:
+    48-55) from the full-size image (read_image_gray of footprint.png):
+    ScaledMat to width x width, width = 2*ceil(radius)+1, as float32 for
+    FootprintBrush(radius, footprint)."""
+    import math
+    w = int(2.0 * math.ceil(radius) + 1.0)
+    return resize_lanczos4(img, w, w, ctx).astype(np.float32)
+
+
 class TextureBrush:
     """Brush stroke texture warped along a stroke, with smudging
     (renderer/TextureBrush.hxx:19-238).  The reference loads the sample's

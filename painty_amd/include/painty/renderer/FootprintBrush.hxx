@@ -3,17 +3,27 @@
 // device through the C ABI.
 //
 // The reference reads its footprint from ./data/footprint/footprint.png and
-// resizes it to width x width (width = 2*ceil(radius)+1).  Here the
-// footprint comes from a footprint source: a width x width image in [0, 1]
-// passed to the constructor / setRadius, or, when none is passed, the
-// brush's footprint source (a Gaussian with sigma = width/4 and peak 1 by
-// default; setFootprintSource replaces it, e.g. with an image loader).
+// resizes it to width x width (width = 2*ceil(radius)+1) with Lanczos-4
+// (FootprintBrush.hxx:48-55).  Here the footprint is a width x width image
+// passed to the constructor / setRadius, or, when none is passed, comes from
+// the brush's footprint source.  The default source does what the reference
+// does when ./data/footprint/footprint.png can be read (pnt_image_read_gray,
+// then pnt_resize_lanczos4 on the device; the decoded image is kept per
+// path), and gives a Gaussian (sigma = width/4, peak 1) when it cannot,
+// where the reference would throw.  setFootprintSource replaces it, e.g.
+// with imageSource(path).
 #pragma once
 
 #include <array>
 #include <cmath>
+#include <fstream>
 #include <functional>
+#include <ios>
+#include <iterator>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "painty/renderer/BrushBase.hxx"
@@ -169,6 +179,19 @@ class FootprintBrush final : public BrushBase<vector_type> {
   }
   pnt_brush handle() const { return _h; }
   static void setFootprintSource(FootprintSource s) { source() = std::move(s); }
+  // ScaledMat(imRead(path, gray, true), width, width) (FootprintBrush.hxx:
+  // 48-55); throws std::ios_base::failure when the file cannot be read, as
+  // imRead does.
+  static Mat<double> fromImage(const std::string& path, int32_t width) {
+    const Mat<double>& full = decoded(path);
+    Mat<double> out(width, width);
+    gpu::check(pnt_resize_lanczos4(gpu::Device::get(), full.data(), full.rows, full.cols, width,
+                                   width, out.data()));
+    return out;
+  }
+  static FootprintSource imageSource(std::string path) {
+    return [path](int32_t width) { return fromImage(path, width); };
+  }
   // The reference's footprint width for a radius (FootprintBrush.hxx:50).
   static int32_t width_of(double radius) {
     return 2 * static_cast<int32_t>(std::ceil(radius)) + 1;
@@ -187,8 +210,31 @@ class FootprintBrush final : public BrushBase<vector_type> {
         f(i, j) = std::exp(-((i - c) * (i - c) + (j - c) * (j - c)) / (2.0 * sigma * sigma));
     return f;
   }
+  static Mat<double> reference_or_gaussian(int32_t width) {
+    static const std::string path = "./data/footprint/footprint.png";
+    if (std::ifstream(path, std::ios::binary).good()) return fromImage(path, width);
+    return gaussian(width);
+  }
+  // the linear grey image of each footprint file read so far
+  static const Mat<double>& decoded(const std::string& path) {
+    static std::mutex mu;
+    static std::map<std::string, Mat<double>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(path);
+    if (it != cache.end()) return it->second;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::ios_base::failure(path);
+    const std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(f)),
+                                     std::istreambuf_iterator<char>());
+    int rows = 0, cols = 0;
+    gpu::check(pnt_image_read_gray(bytes.data(), (int64_t)bytes.size(), 1, &rows, &cols, nullptr, 0));
+    Mat<double> img(rows, cols);
+    gpu::check(pnt_image_read_gray(bytes.data(), (int64_t)bytes.size(), 1, &rows, &cols, img.data(),
+                                   (int64_t)img.total()));
+    return cache.emplace(path, std::move(img)).first->second;
+  }
   static FootprintSource& source() {
-    static FootprintSource s = &FootprintBrush::gaussian;
+    static FootprintSource s = &FootprintBrush::reference_or_gaussian;
     return s;
   }
 

@@ -8,6 +8,8 @@
 //   test_dropin km <in.txt> <out.txt>       ComputeReflectance per case
 //   test_dropin stamp <in.bin> <out.bin>    one canvas, one brush, a stamp list
 //   test_dropin api                         mixer and error behaviour
+//   test_dropin footprint <r> <out> [png]   FootprintBrush(r)'s padded footprint,
+//                                           from png or the default source
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -305,13 +307,26 @@ int api() {
 
 }  // namespace
 
+int footprint(double radius, const char* out, const char* png) {
+  if (png) FootprintBrush<vec3>::setFootprintSource(FootprintBrush<vec3>::imageSource(png));
+  FootprintBrush<vec3> b(radius);
+  const Mat<double> fp = b.getFootprint();
+  std::ofstream o(out, std::ios::binary);
+  const int32_t side = fp.rows;
+  write(o, &side, 1);
+  write(o, fp.data(), fp.total());
+  return 0;
+}
+
 int main(int argc, char** argv) {
   try {
     const std::string mode = argc > 1 ? argv[1] : "";
     if (mode == "km" && argc == 4) return km(argv[2], argv[3]);
     if (mode == "stamp" && argc == 4) return stamp(argv[2], argv[3]);
     if (mode == "api") return api();
-    std::fprintf(stderr, "usage: test_dropin km|stamp <in> <out> | api\n");
+    if (mode == "footprint" && (argc == 4 || argc == 5))
+      return footprint(std::atof(argv[2]), argv[3], argc == 5 ? argv[4] : nullptr);
+    std::fprintf(stderr, "usage: test_dropin km|stamp <in> <out> | api | footprint <r> <out> [png]\n");
     return 2;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "test_dropin: %s\n", e.what());
