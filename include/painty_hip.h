@@ -124,6 +124,14 @@ int pnt_canvas_dry_step(pnt_canvas canvas, float portion);
  * default is the reference's 15000000 ms (Canvas.hxx:32).  The drying time
  * acts through the canvas clock below. */
 int pnt_canvas_set_drying_time_ms(pnt_canvas canvas, int64_t ms);
+/* The drying timer of the reference's GPU render thread
+ * (SbrRenderThread.cxx:37-41: every 250 ms, CanvasGpu::dryStep(0.01),
+ * CanvasGpu.hxx:29), on the canvas clock: every period_ms of clock time
+ * one pnt_canvas_dry_step(portion).  The steps whose ticks have passed run
+ * at the start of the next call that reads or changes the canvas (so a
+ * manual clock makes them reproducible); the timer starts now.
+ * period_ms = 0 stops it; it does nothing while the clock is off. */
+int pnt_canvas_set_dry_timer(pnt_canvas canvas, int64_t period_ms, float portion);
 int pnt_canvas_get_drying_time_ms(pnt_canvas canvas, int64_t* ms);
 /* The clock of Canvas::checkDry (Canvas.hxx:123-155), which the reference's
  * imprint runs on every visited pixel before its pickup

@@ -138,6 +138,7 @@ SIGNATURES = {
     "pnt_batch_graph_launch": [_vp],
     "pnt_batch_graph_destroy": [_vp],
     "pnt_brush_last_launches": [_vp, C.POINTER(C.c_int64)],
+    "pnt_canvas_set_dry_timer": [_vp, C.c_int64, C.c_float],
     "pnt_brush_stamp_traffic": [_vp, C.c_int, C.c_int, C.POINTER(PntStamp), C.c_int64,
                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "pnt_mix_single": [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp],

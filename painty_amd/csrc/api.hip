@@ -269,6 +269,37 @@ int64_t canvas_now_us(const pnt_canvas_s* cv) {
       .count();
 }
 
+// The drying timer of the reference's GPU render thread
+// (SbrRenderThread.cxx:37-41): every 250 ms, CanvasGpu::dryStep(0.01)
+// (CanvasGpu.hxx:29).  Here it ticks on the canvas clock and is applied
+// lazily: the steps whose ticks have passed since the last call run before
+// the next call that reads or changes the canvas.  The step is per pixel, so
+// k steps run as k iterations per pixel in one launch, cut short once no
+// pixel holds paint (steps then change nothing).
+void canvas_tick(pnt_canvas cv) {
+  if (cv->dry_period_us <= 0 || cv->clock == PNT_CLOCK_OFF) return;
+  const int64_t now = canvas_now_us(cv);
+  if (now <= cv->dry_last_us) return;
+  int64_t k = (now - cv->dry_last_us) / cv->dry_period_us;
+  if (k <= 0) return;
+  cv->dry_last_us += k * cv->dry_period_us;
+  cv->ctx->use();
+  hipStream_t s = cv->ctx->stream;
+  cv->epoch++;
+  if (!cv->dry_wet) cv->dry_wet = reinterpret_cast<unsigned*>(dev_alloc(1));
+  constexpr int64_t kChunk = 4096;  // iterations per launch
+  while (k > 0) {
+    const int steps = (int)std::min<int64_t>(k, kChunk);
+    unsigned wet = 0;
+    PNT_HIP_CHECK(hipMemcpyAsync(cv->dry_wet, &wet, sizeof(unsigned), hipMemcpyHostToDevice, s));
+    launch_dry_steps(cv->n, cv->layer(), cv->R0(), cv->dry_portion, steps, cv->dry_wet, s);
+    PNT_HIP_CHECK(hipMemcpyAsync(&wet, cv->dry_wet, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    k -= steps;
+    if (!wet) break;
+  }
+}
+
 void canvas_stamp_time(pnt_canvas cv) {
   if (cv->tmap) launch_fill_i64(cv->tmap, cv->n, canvas_now_us(cv), cv->ctx->stream);
 }
@@ -643,6 +674,7 @@ int pnt_layer_upload(pnt_layer L, const float* K, const float* S, const float* V
     PNT_REQUIRE(L, "NULL layer");
     check_layout(layout);
     L->ctx->use();
+    if (L->parent) canvas_tick(L->parent);
     L->touched();
     const Layer7 P = L->planes();
     if (K) upload_planes(P.p, 3, K, L->n, layout, L->ctx->stream);
@@ -656,6 +688,7 @@ int pnt_layer_download(pnt_layer L, float* K, float* S, float* V, int layout) {
     PNT_REQUIRE(L, "NULL layer");
     check_layout(layout);
     L->ctx->use();
+    if (L->parent) canvas_tick(L->parent);
     const Layer7 P = L->planes();
     if (K) download_planes(K, 3, P.p, L->n, layout, L->ctx->stream);
     if (S) download_planes(S, 3, P.p + 3, L->n, layout, L->ctx->stream);
@@ -734,6 +767,7 @@ int pnt_layer_compose_onto_canvas(pnt_layer L, pnt_canvas cv) {
     PNT_REQUIRE(L->rows == cv->rows && L->cols == cv->cols, "layer and canvas sizes differ");
     PNT_REQUIRE(L->ctx->device == cv->ctx->device, "layer and canvas on different devices");
     cv->ctx->use();
+    canvas_tick(cv);
     if (L->ctx->stream != cv->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(L->ctx->stream));
     launch_compose(L->n, L->planes(), cv->R0(), cv->R0(), cv->ctx->stream);
   });
@@ -831,6 +865,7 @@ int pnt_canvas_destroy(pnt_canvas cv) {
     if (cv->owns_base) dev_free(cv->base);
     dev_free(cv->scratch);
     dev_free(cv->tmap);
+    dev_free(cv->dry_wet);
     delete cv;
     ctx_release(ctx);
   });
@@ -855,6 +890,7 @@ int pnt_canvas_clear(pnt_canvas cv) {
   return guard([&] {
     PNT_REQUIRE(cv, "NULL canvas");
     cv->ctx->use();
+    canvas_tick(cv);
     canvas_clear(cv);
   });
 }
@@ -864,6 +900,7 @@ int pnt_canvas_set_background(pnt_canvas cv, const float* R0, int layout) {
     PNT_REQUIRE(cv && R0, "NULL argument");
     check_layout(layout);
     cv->ctx->use();
+    canvas_tick(cv);
     canvas_clear(cv);
     Rgb3 r = cv->R0();
     upload_planes(r.p, 3, R0, cv->n, layout, cv->ctx->stream);
@@ -875,6 +912,7 @@ int pnt_canvas_upload_R0(pnt_canvas cv, const float* R0, int layout) {
     PNT_REQUIRE(cv && R0, "NULL argument");
     check_layout(layout);
     cv->ctx->use();
+    canvas_tick(cv);
     Rgb3 r = cv->R0();
     upload_planes(r.p, 3, R0, cv->n, layout, cv->ctx->stream);
   });
@@ -885,6 +923,7 @@ int pnt_canvas_download_R0(pnt_canvas cv, float* R0, int layout) {
     PNT_REQUIRE(cv && R0, "NULL argument");
     check_layout(layout);
     cv->ctx->use();
+    canvas_tick(cv);
     Rgb3 r = cv->R0();
     download_planes(R0, 3, r.p, cv->n, layout, cv->ctx->stream);
   });
@@ -894,6 +933,7 @@ int pnt_canvas_upload_h(pnt_canvas cv, const float* h) {
   return guard([&] {
     PNT_REQUIRE(cv && h, "NULL argument");
     cv->ctx->use();
+    canvas_tick(cv);
     float* p = cv->h();
     upload_planes(&p, 1, h, cv->n, PNT_PLANAR, cv->ctx->stream);
   });
@@ -903,6 +943,7 @@ int pnt_canvas_download_h(pnt_canvas cv, float* h) {
   return guard([&] {
     PNT_REQUIRE(cv && h, "NULL argument");
     cv->ctx->use();
+    canvas_tick(cv);
     float* p = cv->h();
     download_planes(h, 1, &p, cv->n, PNT_PLANAR, cv->ctx->stream);
   });
@@ -912,6 +953,7 @@ int pnt_canvas_dry(pnt_canvas cv) {
   return guard([&] {
     PNT_REQUIRE(cv, "NULL canvas");
     cv->ctx->use();
+    canvas_tick(cv);
     cv->epoch++;
     launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), cv->ctx->stream);
     canvas_stamp_time(cv);  // Canvas.hxx:106, 118
@@ -923,8 +965,22 @@ int pnt_canvas_dry_step(pnt_canvas cv, float portion) {
     PNT_REQUIRE(cv, "NULL canvas");
     PNT_REQUIRE(std::isfinite(portion), "portion must be finite");
     cv->ctx->use();
+    canvas_tick(cv);
     cv->epoch++;
     launch_dry_step(cv->n, cv->layer(), cv->R0(), portion, cv->ctx->stream);
+  });
+}
+
+int pnt_canvas_set_dry_timer(pnt_canvas cv, int64_t period_ms, float portion) {
+  return guard([&] {
+    PNT_REQUIRE(cv, "NULL canvas");
+    PNT_REQUIRE(period_ms >= 0, "timer period must be >= 0");
+    PNT_REQUIRE(period_ms == 0 || (std::isfinite(portion) && portion > 0.0f),
+                "dry portion must be positive");
+    canvas_tick(cv);  // steps of the previous timer up to now
+    cv->dry_period_us = period_ms * 1000;
+    cv->dry_portion = portion;
+    cv->dry_last_us = canvas_now_us(cv);
   });
 }
 
@@ -963,6 +1019,7 @@ int pnt_canvas_check_dry(pnt_canvas cv, int x, int y, int64_t now_us) {
     PNT_REQUIRE(x >= 0 && y >= 0 && x < cv->cols && y < cv->rows, "pixel outside the canvas");
     PNT_REQUIRE(cv->tmap, "the canvas has no clock (pnt_canvas_set_clock)");
     cv->ctx->use();
+    canvas_tick(cv);
     cv->epoch++;
     const DryClock c{cv->tmap, now_us, cv->drying_ms, cv->R0(), cv->h()};
     launch_check_dry_pixel(c, cv->layer(), (int64_t)y * cv->cols + x, cv->ctx->stream);
@@ -1015,6 +1072,7 @@ int pnt_canvas_get_drying_time_ms(pnt_canvas cv, int64_t* ms) {
 int pnt_canvas_device_planes(pnt_canvas cv, pnt_dev_layer* layer, pnt_dev_rgb* R0, float** h) {
   return guard([&] {
     PNT_REQUIRE(cv, "NULL canvas");
+    canvas_tick(cv);
     cv->epoch++;  // the caller may write the planes
     if (layer) {
       const Layer7 L = cv->layer();
@@ -1034,6 +1092,7 @@ int pnt_compose(pnt_canvas cv, float* R, int layout) {
     PNT_REQUIRE(cv && R, "NULL argument");
     check_layout(layout);
     cv->ctx->use();
+    canvas_tick(cv);
     if (!cv->scratch) cv->scratch = dev_alloc(3 * cv->stride);
     Rgb3 out{{cv->scratch, cv->scratch + cv->stride, cv->scratch + 2 * cv->stride}};
     launch_compose(cv->n, cv->layer(), cv->R0(), out, cv->ctx->stream);
@@ -1046,6 +1105,7 @@ int pnt_render(pnt_canvas cv, float* R, int layout) {
     PNT_REQUIRE(cv && R, "NULL argument");
     check_layout(layout);
     cv->ctx->use();
+    canvas_tick(cv);
     if (!cv->scratch) cv->scratch = dev_alloc(3 * cv->stride);
     Rgb3 out{{cv->scratch, cv->scratch + cv->stride, cv->scratch + 2 * cv->stride}};
     launch_render(cv->rows, cv->cols, cv->layer(), cv->R0(), out, cv->ctx->stream);
@@ -1057,6 +1117,7 @@ int pnt_dev_render(pnt_canvas cv, pnt_dev_rgb R) {
   return guard([&] {
     PNT_REQUIRE(cv && R.p[0] && R.p[1] && R.p[2], "NULL argument");
     cv->ctx->use();
+    canvas_tick(cv);
     Rgb3 out{{R.p[0], R.p[1], R.p[2]}};
     launch_render(cv->rows, cv->cols, cv->layer(), cv->R0(), out, cv->ctx->stream);
   });
@@ -1324,6 +1385,7 @@ int pnt_brush_update_snapshot(pnt_brush b, pnt_canvas cv) {
   return guard([&] {
     check_same_ctx(b, cv);
     b->ctx->use();
+    canvas_tick(cv);
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
     brush_full_snapshot(b, cv);
   });
@@ -1333,6 +1395,7 @@ int pnt_imprint(pnt_brush b, pnt_canvas cv, double cx, double cy, double theta) 
   return guard([&] {
     check_same_ctx(b, cv);
     b->ctx->use();
+    canvas_tick(cv);
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
     imprint_plain(b, cv, cx, cy, theta);
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
@@ -1347,6 +1410,7 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
                     mode == PNT_EXEC_CHAINED,
                 "unknown mode");
     b->ctx->use();
+    canvas_tick(cv);
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
     if (cv->clock != PNT_CLOCK_OFF) mode = PNT_EXEC_PLAIN;  // checkDry runs on the plain path
     if (mode == PNT_EXEC_PERSISTENT) {
@@ -1381,6 +1445,7 @@ int pnt_imprint_strokes(pnt_brush b, pnt_canvas cv, const pnt_stroke* S, int64_t
       PNT_REQUIRE(S[i].n_stamps >= 0 && (S[i].stamps || S[i].n_stamps == 0) && S[i].footprint,
                   "invalid stroke");
     b->ctx->use();
+    canvas_tick(cv);
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
     if (mode == PNT_EXEC_FUSED && cv->clock == PNT_CLOCK_OFF) {
       run_fused_strokes(b, cv, S, ns);
@@ -1565,6 +1630,7 @@ int pnt_paint_stroke(pnt_brush b, pnt_canvas cv, const double* p, int64_t n) {
     check_same_ctx(b, cv);
     PNT_REQUIRE(n >= 0 && (p || n == 0), "invalid path");
     b->ctx->use();
+    canvas_tick(cv);
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
     // FootprintBrush.hxx:251-267
     for (int64_t i = 0; i + 1 < n; ++i) {

@@ -80,6 +80,11 @@ struct pnt_canvas_s {
   // while a clock is on
   int clock = PNT_CLOCK_OFF;
   int64_t manual_us = 0;
+  // drying timer (pnt_canvas_set_dry_timer): dryStep(dry_portion) every
+  // dry_period_us of canvas time, last applied at dry_last_us; 0: off
+  int64_t dry_period_us = 0, dry_last_us = 0;
+  float dry_portion = 0.01f;
+  unsigned* dry_wet = nullptr;  // device word: a pixel still holds paint
   int64_t* tmap = nullptr;
   float bg[3] = {1.f, 1.f, 1.f};
   // Bumped by every change of the wet layer; a brush's snapshot dirty bytes
@@ -184,6 +189,10 @@ std::string palette_to_json(const double* K, const double* S, int64_t n);
 pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base);
 // The canvas clock's current time point (microseconds).
 int64_t canvas_now_us(const pnt_canvas_s* cv);
+// Runs the drying-timer steps whose ticks have passed (nothing when the
+// timer or the clock is off); every call that reads or changes a canvas
+// starts with it.
+void canvas_tick(pnt_canvas_s* cv);
 // Every pixel's time point set to now (Canvas::clear, dryCanvas), if a clock is on.
 void canvas_stamp_time(pnt_canvas cv);
 }  // namespace pnt

@@ -76,6 +76,10 @@ void launch_dry_step(int64_t n, const Layer7& L, const Rgb3& R0, float portion,
 void launch_reflectance_aos(int64_t n, const float* K, const float* S, const float* R0,
                             const float* d, float* R, hipStream_t s);
 void launch_fill(float* p, int64_t n, float v, hipStream_t s);
+// `steps` dry steps of `portion` (k_dry_step repeated per pixel); *wet is
+// raised when a pixel still holds paint after them (zero it first).
+void launch_dry_steps(int64_t n, const Layer7& L, const Rgb3& R0, float portion, int steps,
+                      unsigned* wet, hipStream_t s);
 void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 void launch_fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t s);
 void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);

@@ -166,6 +166,39 @@ __global__ __launch_bounds__(kBlock) void k_dry_step(int64_t n, Layer7 L, Rgb3 R
   L.p[6][i] = rem;
 }
 
+// `steps` applications of k_dry_step's per-pixel update, stopping early once
+// the pixel holds no paint (further steps change nothing).
+__global__ __launch_bounds__(kBlock) void k_dry_steps(int64_t n, Layer7 L, Rgb3 R0, float portion,
+                                                      int steps, unsigned* wet) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  bool still = false;
+  if (i < n) {
+    float d = L.p[6][i];
+    if (d != 0.0f) {  // V = 0 stays as it is under any number of steps
+      float K[3], S[3], R[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        K[c] = L.p[c][i];
+        S[c] = L.p[3 + c][i];
+        R[c] = R0.p[c][i];
+      }
+      for (int k = 0; k < steps; ++k) {
+        const float rem = fmaxf(0.0f, d - portion);
+        const float leave = d - rem;
+        if (__builtin_fabsf(leave) > 10e-6f) km_pixel_glsl(K, S, R, leave);
+        d = rem;
+        if (!(d > 0.0f)) break;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) R0.p[c][i] = R[c];
+      L.p[6][i] = d;
+      still = d > 0.0f;
+    }
+  }
+  // one atomic per wave that still holds paint
+  if (__ballot(still) != 0 && (threadIdx.x & 63) == 0) atomicOr(wet, 1u);
+}
+
 // ComputeReflectance over AoS arrays (KubelkaMunk.hxx:25-83).
 __global__ __launch_bounds__(kBlock) void k_reflectance_aos(int64_t n, const float* K,
                                                             const float* S, const float* R0,
@@ -387,6 +420,13 @@ void launch_reflectance_aos(int64_t n, const float* K, const float* S, const flo
                             const float* d, float* R, hipStream_t s) {
   if (n <= 0) return;
   k_reflectance_aos<<<blocks_for(n), kBlock, 0, s>>>(n, K, S, R0, d, R);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dry_steps(int64_t n, const Layer7& L, const Rgb3& R0, float portion, int steps,
+                      unsigned* wet, hipStream_t s) {
+  if (n <= 0 || steps <= 0) return;
+  k_dry_steps<<<blocks_for(n), kBlock, 0, s>>>(n, L, R0, portion, steps, wet);
   PNT_HIP_CHECK(hipGetLastError());
 }
 

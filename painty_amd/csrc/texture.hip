@@ -534,6 +534,7 @@ int pnt_tbrush_paint_stroke(pnt_tbrush b, pnt_canvas cv, const double* xy, int64
     if (n < 2) return;  // TextureBrush.hxx:54-56
     PNT_REQUIRE(b->thick, "no thickness map (pnt_tbrush_set_thickness_map)");
     b->ctx->use();
+    canvas_tick(cv);
     hipStream_t s = b->ctx->stream;
     if (cv->ctx->stream != s) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
     cv->epoch++;

@@ -222,6 +222,11 @@ class Canvas:
     def dryStep(self, portion: float) -> None:
         call("pnt_canvas_dry_step", self.h, float(portion))
 
+    def setDryTimer(self, period_ms: int = 250, portion: float = 0.01) -> None:
+        """The GPU render thread's drying timer (SbrRenderThread.cxx:37-41):
+        dryStep(portion) every period_ms of canvas clock time; 0 stops it."""
+        call("pnt_canvas_set_dry_timer", self.h, int(period_ms), float(portion))
+
     def setDryingTime(self, ms: int) -> None:
         call("pnt_canvas_set_drying_time_ms", self.h, int(ms))
 

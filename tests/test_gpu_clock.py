@@ -8,7 +8,7 @@ import pytest
 
 from painty_amd import _lib
 from painty_amd import workload as W
-from tests.parity import compare_canvas
+from tests.parity import assert_close, compare_canvas
 from tests.test_gpu_stamp import PA, PB
 
 pytestmark = pytest.mark.gpu
@@ -147,3 +147,44 @@ def test_batch_refuses_clocked_canvas(hip):
     gb = hip.FootprintBrush(wl.radii[0], wl.cores[0])
     with pytest.raises(hip.InvalidArgument):
         hip.batch_run([(gc, gb, wl.stamps, None)])
+
+
+def test_dry_timer(hip, oracle):
+    """The GPU render thread's drying timer (SbrRenderThread.cxx:37-41):
+    dryStep(0.01) every 250 ms of canvas clock, applied before the next call
+    that reads the canvas; against the oracle's DryingStep repeated."""
+    rows, cols = 24, 32
+    rng = np.random.default_rng(8)
+    K = rng.uniform(0.05, 1.0, (rows, cols, 3)).astype(np.float32)
+    S = rng.uniform(0.05, 1.0, (rows, cols, 3)).astype(np.float32)
+    V = rng.uniform(0.0, 0.08, (rows, cols)).astype(np.float32)
+    V[:, :4] = 0.0
+    R0 = rng.uniform(0.2, 1.0, (rows, cols, 3)).astype(np.float32)
+    oK, oS, oV, oR0 = (a.astype(np.float64) for a in (K, S, V, R0))
+    gc = hip.Canvas(rows, cols)
+    gc.getPaintLayer().setBuffers(K, S, V)
+    gc.setR0(R0)
+    gc.setClock(_lib.CLOCK_MANUAL)
+    gc.setTime(NOW)
+    gc.setDryTimer(250, 0.01)
+    done = 0
+    for t_ms, steps in [(1100, 4), (1300, 5), (1300, 5), (3_600_000, 14400)]:
+        gc.setTime(NOW + t_ms * 1000)
+        R = hip.Renderer.compose(gc)  # reads the canvas: the timer's steps run first
+        for _ in range(min(steps, 20) - done):  # V <= 0.08 is gone after 8 steps
+            oracle.dry_step(oK, oS, oV, oR0, 0.01)
+        done = min(steps, 20)
+        tag = "timer/%dms" % t_ms
+        # V - k * 0.01 in FP32 (as DryingStep.compute.glsl computes it) loses
+        # the low bits of what remains: an absolute bound of a few FP32 ulps
+        np.testing.assert_allclose(gc.getPaintLayer().getV_buffer(), oV, rtol=1e-5, atol=5e-8,
+                                   err_msg=tag + ".V")
+        assert_close(tag + ".R0", gc.getR0(), oR0)
+        assert_close(tag + ".R", R, oracle.compose(oK, oS, oV, oR0))
+    assert (gc.getPaintLayer().getV_buffer() == 0).all()
+    gc.getPaintLayer().setBuffers(V=V)  # paint again, then stop the timer: nothing dries
+    before = gc.getR0()
+    gc.setDryTimer(0)
+    gc.setTime(NOW + 7_200_000_000)
+    np.testing.assert_array_equal(gc.getR0(), before)
+    np.testing.assert_array_equal(gc.getPaintLayer().getV_buffer(), V)
