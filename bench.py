@@ -464,13 +464,29 @@ def bench_texture(args):
     ctx.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     gpu_ms = ev.elapsed_ms(e0, e1) / args.steps
+    # the GL brush (TextureBrushGpu) on the same stroke, smudging on as in the
+    # reference's GpuTest.cxx:29-45
+    gl = H.TextureBrushGpu()
+    gl.setRadius(radius)
+    gl.setTexture(tex)
+    gl.enableSmudge(True)
+    gl.dip(((0.2, 0.3, 0.4), (0.1, 0.23, 0.14)))
+    for _ in range(max(args.warmup, 1)):
+        gl.paintStroke(path, cv)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        gl.paintStroke(path, cv)
+    ctx.synchronize()
+    gl_ms = (time.perf_counter() - t1) * 1e3 / args.steps
     result = {"metric": "texture-brush strokes/s (smudge on)", "value": round(1e3 / wall_ms, 2),
               "unit": "strokes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
               "ms_per_step": round(wall_ms, 3), "stream_ms_per_step": round(gpu_ms, 3),
               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
               "data": "synthetic",
               "config": {"workload": "TextureBrushTest stroke: 768x1024 canvas, r=40, "
-                                     "(50,250)->(400,250)->(650,250), smudge on"}}
+                                     "(50,250)->(400,250)->(650,250), smudge on"},
+              "gl_brush_ms_per_stroke": round(gl_ms, 3)}
     if not args.no_cpu:
         from oracle import oracle as O
         oc = O.Canvas(rows, cols)

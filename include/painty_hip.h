@@ -513,6 +513,55 @@ int pnt_tbrush_get_smudge_map(pnt_tbrush brush, int* side, double* K,
 int pnt_tbrush_paint_stroke(pnt_tbrush brush, pnt_canvas canvas,
                             const double* xy, int64_t n);
 
+/* ---------------------------------------------------- TextureBrushGpu --- */
+/* TextureBrushGpu -- renderer/TextureBrushGpu.hxx:21-64,
+ * renderer/src/TextureBrushGpu.cxx:17-305: the reference's GL texture brush
+ * (used by its render thread, SbrRenderThread).  A stroke rasterises the
+ * brush texture along a triangle strip around the spline through the
+ * vertices (BrushTextureWarp.vert/.frag), optionally smudges
+ * (Smudge.compute: pickup and deposition rates 0.7, the pickup map read at
+ * the rotated position), then mixes by volume with V = Vcan + Vtex and
+ * stores the lighting thickness (PaintTextureFootprintOnCanvas.compute), and
+ * clears the warped texture (ClearTextureFootprint.compute).  The GL images
+ * are y-flipped (gpu/GpuMat.hxx:43-56), and the smudge's dispatch offset is
+ * taken from the canvas position unflipped, as in the reference.  Where GL
+ * leaves behaviour to the implementation this fixes one choice: vertices
+ * snapped to 1/256 pixel, pixel centres inside a triangle with the top-left
+ * rule (window y up), linear interpolation of texture coordinates, later
+ * triangles overwriting earlier ones, GL_LINEAR + GL_CLAMP_TO_EDGE sampling,
+ * and for the smudge's writes to one pickup cell: every invocation reads the
+ * state before the dispatch and the highest invocation index wins.  The
+ * reference picks the brush texture at random from its texture dictionary
+ * (TextureBrushDictionary.cxx:24-68); here the caller sets it. */
+typedef struct pnt_tbgl_s* pnt_tbgl;
+/* TextureBrushGpu() -- TextureBrushGpu.cxx:17-46: radius 5 (an 11 x 11
+ * smudge window), smudging off, paint 0. */
+int pnt_tbgl_create(pnt_ctx ctx, pnt_tbgl* out);
+int pnt_tbgl_destroy(pnt_tbgl brush);
+/* setRadius -- TextureBrushGpu.cxx:48-60: a change of 0.5 or more makes new,
+ * cleared smudge maps of (int)(2 radius) + 1 square. */
+int pnt_tbgl_set_radius(pnt_tbgl brush, double radius);
+/* dip -- TextureBrushGpu.cxx:62-64 */
+int pnt_tbgl_dip(pnt_tbgl brush, const double K[3], const double S[3]);
+/* The brush texture as a texture dictionary entry makes it
+ * (TextureBrushDictionary.cxx:106-123): the height map (rows x cols,
+ * row-major, finite) normalised to [0, 1] by its minimum and maximum,
+ * flipped, FP32. */
+int pnt_tbgl_set_texture(pnt_tbgl brush, const double* img, int rows, int cols);
+/* BrushBase::setThicknessScale -- renderer/BrushBase.hxx:31-33 */
+int pnt_tbgl_set_thickness_scale(pnt_tbgl brush, double scale);
+/* enableSmudge -- TextureBrushGpu.cxx:249-251 */
+int pnt_tbgl_enable_smudge(pnt_tbgl brush, int enable);
+/* The smudge maps (side x side RGBA: K or S, then V); either may be NULL. */
+int pnt_tbgl_get_smudge(pnt_tbgl brush, int* side, float* K4, float* S4);
+/* paintStroke(vertices, CanvasGpu&) -- TextureBrushGpu.cxx:73-166; fewer
+ * than 2 vertices do nothing; PNT_EINVAL without a brush texture. */
+int pnt_tbgl_paint_stroke(pnt_tbgl brush, pnt_canvas canvas, const double* xy,
+                          int64_t n);
+/* The canvas' lighting thickness (the GL renderer's S.a plane), rows x cols;
+ * zero until a pnt_tbgl stroke writes it. */
+int pnt_canvas_download_lighting(pnt_canvas canvas, float* out);
+
 /* ---------------------------------------------------- footprint ingest - */
 /* io::imRead(buffer, Mat<double>& gray, convertFrom_sRGB) -- io/src/
  * ImageIO.cxx:115-150 (the file form, :78-113, reads the same bytes): a PNG

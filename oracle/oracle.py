@@ -108,6 +108,17 @@ def lib():
         L.orc_gray_from_u16.argtypes = [C.c_int64, C.POINTER(C.c_uint16), C.c_int, C.c_int, _dp]
         L.orc_lanczos4_coeffs.argtypes = [C.c_float, C.POINTER(C.c_float)]
         L.orc_resize_lanczos4.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_tbgl_create.restype = C.c_void_p
+        L.orc_tbgl_destroy.argtypes = [C.c_void_p]
+        L.orc_tbgl_set_radius.argtypes = [C.c_void_p, C.c_double]
+        L.orc_tbgl_dip.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_tbgl_set_texture.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int]
+        L.orc_tbgl_set_thickness_scale.argtypes = [C.c_void_p, C.c_double]
+        L.orc_tbgl_enable_smudge.argtypes = [C.c_void_p, C.c_int]
+        _f = C.POINTER(C.c_float)
+        L.orc_tbgl_get_smudge.argtypes = [C.c_void_p, C.POINTER(C.c_int), _f, _f]
+        L.orc_tbgl_paint_stroke.argtypes = [C.c_void_p, C.c_int, C.c_int, _f, _f, _f, _f, _dp,
+                                            C.c_int]
         L.orc_mix_single.restype = C.c_int
         L.orc_mix_single.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.orc_mixed.argtypes = [_dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp]
@@ -510,3 +521,63 @@ def resize_lanczos4(img, out_rows: int, out_cols: int) -> np.ndarray:
     lib().orc_resize_lanczos4(_p(a), a.shape[0], a.shape[1], int(out_rows), int(out_cols),
                               _p(out))
     return out
+
+
+# --------------------------------------------------- GL texture brush -----
+class GlCanvas:
+    """FP32 canvas planes as the GL renderer holds them (canvas row order):
+    K, S (rows, cols, 3), V and the lighting thickness (rows, cols)."""
+
+    def __init__(self, K, S, V, light=None):
+        self.K = np.ascontiguousarray(K, np.float32).copy()
+        self.S = np.ascontiguousarray(S, np.float32).copy()
+        self.V = np.ascontiguousarray(V, np.float32).copy()
+        self.light = (np.zeros(self.V.shape, np.float32) if light is None
+                      else np.ascontiguousarray(light, np.float32).copy())
+        self.rows, self.cols = self.V.shape
+
+
+def _pf32(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class TextureBrushGpu:
+    """TextureBrushGpu (renderer/src/TextureBrushGpu.cxx:17-305)."""
+
+    def __init__(self):
+        self.h = lib().orc_tbgl_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_tbgl_destroy(self.h)
+            self.h = None
+
+    def set_radius(self, r: float) -> None:
+        lib().orc_tbgl_set_radius(self.h, float(r))
+
+    def dip(self, K, S) -> None:
+        lib().orc_tbgl_dip(self.h, _p(_f64(K)), _p(_f64(S)))
+
+    def set_texture(self, img) -> None:
+        a = _f64(img)
+        lib().orc_tbgl_set_texture(self.h, _p(a), a.shape[0], a.shape[1])
+
+    def set_thickness_scale(self, s: float) -> None:
+        lib().orc_tbgl_set_thickness_scale(self.h, float(s))
+
+    def enable_smudge(self, on: bool) -> None:
+        lib().orc_tbgl_enable_smudge(self.h, int(bool(on)))
+
+    def smudge(self):
+        side = C.c_int(0)
+        lib().orc_tbgl_get_smudge(self.h, C.byref(side), None, None)
+        n = side.value
+        K, S = np.zeros((n, n, 4), np.float32), np.zeros((n, n, 4), np.float32)
+        lib().orc_tbgl_get_smudge(self.h, C.byref(side), _pf32(K), _pf32(S))
+        return K, S
+
+    def paint_stroke(self, xy, cv: GlCanvas) -> None:
+        xy = _f64(xy).reshape(-1, 2)
+        lib().orc_tbgl_paint_stroke(self.h, cv.rows, cv.cols, _pf32(cv.K), _pf32(cv.S),
+                                    _pf32(cv.V), _pf32(cv.light), _p(xy), int(xy.shape[0]))

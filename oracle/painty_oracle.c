@@ -1655,3 +1655,320 @@ void orc_resize_lanczos4(const double* src, int rows, int cols, int out_rows, in
   free(ya);
   free(tmp);
 }
+
+/* ------------------------------------------------------------------------ */
+/* TextureBrushGpu (renderer/src/TextureBrushGpu.cxx:17-305) and its shaders,
+ * in FP32 where the shaders compute (highp float) and FP64 where the host
+ * code does.  The canvas planes are FP32 in canvas row order; the warped
+ * brush texture is kept in GL row order (row g = canvas row H-1-g). */
+struct orc_tbgl {
+  double radius;
+  float paint[6];
+  float* tex; /* brush texture as uploaded: min-max normalised, flipped */
+  int t_rows, t_cols;
+  double thickness_scale;
+  int smudge;
+  int n;          /* smudge window side: (int)(2 r) + 1 */
+  float* smK;     /* n*n RGBA: K rgb, V */
+  float* smS;     /* n*n RGBA: S rgb, V */
+  float* warp;    /* rows*cols, GL rows */
+  int w_rows, w_cols;
+};
+
+static void tbgl_reset_smudge(orc_tbgl* b) {
+  free(b->smK);
+  free(b->smS);
+  b->smK = (float*)calloc(4 * (size_t)b->n * b->n + 4, sizeof(float));
+  b->smS = (float*)calloc(4 * (size_t)b->n * b->n + 4, sizeof(float));
+}
+
+/* TextureBrushGpu.cxx:48-60 */
+void orc_tbgl_set_radius(orc_tbgl* b, double radius) {
+  if (!(fabs(b->radius - radius) < 0.5)) {
+    b->radius = radius;
+    b->n = (int)(unsigned)(2.0 * radius) + 1;
+    tbgl_reset_smudge(b);
+  }
+}
+
+orc_tbgl* orc_tbgl_create(void) {
+  orc_tbgl* b = (orc_tbgl*)calloc(1, sizeof(orc_tbgl));
+  b->thickness_scale = 1.0;
+  b->smudge = 0;          /* TextureBrushGpu.hxx:63 */
+  b->n = 11;              /* _smudgeK(Size{10, 10}) then setRadius(5.0) */
+  tbgl_reset_smudge(b);
+  orc_tbgl_set_radius(b, 5.0);
+  return b;
+}
+
+void orc_tbgl_destroy(orc_tbgl* b) {
+  if (!b) return;
+  free(b->smK); free(b->smS); free(b->tex); free(b->warp);
+  free(b);
+}
+
+void orc_tbgl_dip(orc_tbgl* b, const double K[3], const double S[3]) {
+  for (int c = 0; c < 3; c++) {
+    b->paint[c] = (float)K[c];
+    b->paint[3 + c] = (float)S[c];
+  }
+}
+
+/* TextureBrushDictionary.cxx:106-123: NORM_MINMAX to [0, 1], flipped, FP32 */
+void orc_tbgl_set_texture(orc_tbgl* b, const double* img, int rows, int cols) {
+  free(b->tex);
+  b->tex = (float*)malloc(sizeof(float) * (size_t)rows * cols);
+  double lo = img[0], hi = img[0];
+  for (int64_t i = 0; i < (int64_t)rows * cols; i++) {
+    lo = fmin(lo, img[i]);
+    hi = fmax(hi, img[i]);
+  }
+  const double sc = hi > lo ? 1.0 / (hi - lo) : 0.0;
+  for (int r = 0; r < rows; r++)
+    for (int c = 0; c < cols; c++)
+      b->tex[(int64_t)(rows - 1 - r) * cols + c] = (float)((img[(int64_t)r * cols + c] - lo) * sc);
+  b->t_rows = rows;
+  b->t_cols = cols;
+}
+
+void orc_tbgl_set_thickness_scale(orc_tbgl* b, double s) { b->thickness_scale = s; }
+void orc_tbgl_enable_smudge(orc_tbgl* b, int on) { b->smudge = on; }
+
+void orc_tbgl_get_smudge(const orc_tbgl* b, int* side, float* K4, float* S4) {
+  *side = b->n;
+  if (K4) memcpy(K4, b->smK, sizeof(float) * 4 * (size_t)b->n * b->n);
+  if (S4) memcpy(S4, b->smS, sizeof(float) * 4 * (size_t)b->n * b->n);
+}
+
+/* GL_LINEAR + GL_CLAMP_TO_EDGE at texture coordinates (s, t) */
+static double tbgl_sample(const orc_tbgl* b, double s, double t) {
+  const double x = s * b->t_cols - 0.5, y = t * b->t_rows - 0.5;
+  const double fx0 = floor(x), fy0 = floor(y);
+  const double ax = x - fx0, ay = y - fy0;
+  const int x0 = (int)fx0, y0 = (int)fy0;
+  const int xa = clampi(x0, 0, b->t_cols - 1), xb = clampi(x0 + 1, 0, b->t_cols - 1);
+  const int ya = clampi(y0, 0, b->t_rows - 1), yb = clampi(y0 + 1, 0, b->t_rows - 1);
+#define TX(yy, xx) ((double)b->tex[(int64_t)(yy) * b->t_cols + (xx)])
+  return (TX(ya, xa) * (1.0 - ax) + TX(ya, xb) * ax) * (1.0 - ay) +
+         (TX(yb, xa) * (1.0 - ax) + TX(yb, xb) * ax) * ay;
+#undef TX
+}
+
+static int64_t tbgl_cross(int64_t ax, int64_t ay, int64_t bx, int64_t by, int64_t px, int64_t py) {
+  return (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+}
+
+static int tbgl_top_left(int64_t ax, int64_t ay, int64_t bx, int64_t by) {
+  const int64_t dx = bx - ax, dy = by - ay;
+  return (dy == 0 && dx < 0) || dy < 0;
+}
+
+/* generateWarpedTexture (TextureBrushGpu.cxx:168-247) + BrushTextureWarp:
+ * the strip r0 l0 r1 l1 ... drawn into the canvas-sized warp image */
+static void tbgl_warp(orc_tbgl* b, int rows, int cols, const double* xy, int n) {
+  const int nvb = 2 * n;
+  float* vx = (float*)malloc(sizeof(float) * nvb);
+  float* vy = (float*)malloc(sizeof(float) * nvb);
+  float* tu = (float*)malloc(sizeof(float) * nvb);
+  float* tv = (float*)malloc(sizeof(float) * nvb);
+  for (int i = 0; i < n; i++) {
+    const double u = (double)i / (double)(n - 1);
+    double c[2], t[2];
+    spline_cr(xy, n, u, 0, c);
+    spline_cr(xy, n, u, 1, t);
+    const double tn = sqrt(t[0] * t[0] + t[1] * t[1]);
+    if (tn > 0.0) { t[0] /= tn; t[1] /= tn; }
+    const double d[2] = {-t[1], t[0]};
+    vx[2 * i] = (float)(c[0] + b->radius * d[0]);  /* r: texture v = 0 */
+    vy[2 * i] = (float)(c[1] + b->radius * d[1]);
+    vx[2 * i + 1] = (float)(c[0] - b->radius * d[0]);  /* l: v = 1 */
+    vy[2 * i + 1] = (float)(c[1] - b->radius * d[1]);
+    tu[2 * i] = tu[2 * i + 1] = (float)u;
+    tv[2 * i] = 0.0f;
+    tv[2 * i + 1] = 1.0f;
+  }
+  for (int j = 0; j + 2 < nvb; j++) {
+    int id[3] = {j, j + 1, j + 2};
+    int64_t X[3], Y[3];
+    for (int q = 0; q < 3; q++) {  /* window coordinates (y up), 1/256 px */
+      X[q] = (int64_t)llrint((double)vx[id[q]] * 256.0);
+      Y[q] = (int64_t)llrint((double)((float)rows - vy[id[q]]) * 256.0);
+    }
+    int64_t area = tbgl_cross(X[0], Y[0], X[1], Y[1], X[2], Y[2]);
+    if (area == 0) continue;
+    if (area < 0) {  /* counter-clockwise in y-up coordinates */
+      int64_t tx = X[1], ty = Y[1];
+      X[1] = X[2]; Y[1] = Y[2]; X[2] = tx; Y[2] = ty;
+      int ti = id[1]; id[1] = id[2]; id[2] = ti;
+      area = -area;
+    }
+    int64_t xmin = X[0], xmax = X[0], ymin = Y[0], ymax = Y[0];
+    for (int q = 1; q < 3; q++) {
+      xmin = X[q] < xmin ? X[q] : xmin; xmax = X[q] > xmax ? X[q] : xmax;
+      ymin = Y[q] < ymin ? Y[q] : ymin; ymax = Y[q] > ymax ? Y[q] : ymax;
+    }
+    int px0 = (int)((xmin - 128) >> 8), px1 = (int)((xmax - 128) >> 8) + 1;
+    int py0 = (int)((ymin - 128) >> 8), py1 = (int)((ymax - 128) >> 8) + 1;
+    px0 = px0 < 0 ? 0 : px0; py0 = py0 < 0 ? 0 : py0;
+    px1 = px1 > cols - 1 ? cols - 1 : px1; py1 = py1 > rows - 1 ? rows - 1 : py1;
+    for (int py = py0; py <= py1; py++)
+      for (int px = px0; px <= px1; px++) {
+        const int64_t cx = (int64_t)px * 256 + 128, cy = (int64_t)py * 256 + 128;
+        const int64_t e0 = tbgl_cross(X[1], Y[1], X[2], Y[2], cx, cy);
+        const int64_t e1 = tbgl_cross(X[2], Y[2], X[0], Y[0], cx, cy);
+        const int64_t e2 = tbgl_cross(X[0], Y[0], X[1], Y[1], cx, cy);
+        const int in0 = e0 > 0 || (e0 == 0 && tbgl_top_left(X[1], Y[1], X[2], Y[2]));
+        const int in1 = e1 > 0 || (e1 == 0 && tbgl_top_left(X[2], Y[2], X[0], Y[0]));
+        const int in2 = e2 > 0 || (e2 == 0 && tbgl_top_left(X[0], Y[0], X[1], Y[1]));
+        if (!(in0 && in1 && in2)) continue;
+        const double w0 = (double)e0 / (double)area, w1 = (double)e1 / (double)area,
+                     w2 = (double)e2 / (double)area;
+        const double s = w0 * tu[id[0]] + w1 * tu[id[1]] + w2 * tu[id[2]];
+        const double t = w0 * tv[id[0]] + w1 * tv[id[1]] + w2 * tv[id[2]];
+        b->warp[(int64_t)py * cols + px] = (float)tbgl_sample(b, (double)(float)s, (double)(float)t);
+      }
+  }
+  free(vx); free(vy); free(tu); free(tv);
+}
+
+static int tbgl_round16(int v) { return (v + 15) / 16 * 16; }
+
+/* TextureBrushGpu::smudge (TextureBrushGpu.cxx:253-305) + Smudge.compute */
+static void tbgl_smudge(orc_tbgl* b, int rows, int cols, float* K, float* S, float* V,
+                        float* light, const double* xy, int n) {
+  double length = 0.0;
+  for (int i = 1; i < n; i++) {
+    const double dx = xy[2 * i] - xy[2 * i - 2], dy = xy[2 * i + 1] - xy[2 * i - 1];
+    length += sqrt(dx * dx + dy * dy);
+  }
+  const int m = b->n;
+  const int dw = tbgl_round16(m), dh = tbgl_round16(m);
+  const float rcx = (float)(m * 0.5), rcy = (float)(m * 0.5);
+  const float scale = (float)b->thickness_scale;
+  const size_t cells = (size_t)m * m;
+  float* oK = (float*)malloc(sizeof(float) * 4 * cells);  /* state before the dispatch */
+  float* oS = (float*)malloc(sizeof(float) * 4 * cells);
+  int* owner = (int*)malloc(sizeof(int) * cells);
+  float* nK = (float*)malloc(sizeof(float) * 4 * cells);
+  float* nS = (float*)malloc(sizeof(float) * 4 * cells);
+  int lastx = -1, lasty = -1;
+  for (double u = 0.0; u <= 1.0; u += (0.25 / length)) {
+    double c[2], t[2];
+    spline_cr(xy, n, u, 0, c);
+    if ((int)c[0] == lastx && (int)c[1] == lasty) continue;
+    lastx = (int)c[0];
+    lasty = (int)c[1];
+    spline_cr(xy, n, u, 1, t);
+    const double tn = sqrt(t[0] * t[0] + t[1] * t[1]);
+    if (tn > 0.0) { t[0] /= tn; t[1] /= tn; }
+    const double theta = atan2(t[1], t[0]);
+    const int tlx = (int)((double)(int)c[0] - (double)m * 0.5);
+    const int tly = (int)((double)(int)c[1] - (double)m * 0.5);
+    const float th = (float)theta, cs = cosf(th), sn = sinf(th);
+    memcpy(oK, b->smK, sizeof(float) * 4 * cells);
+    memcpy(oS, b->smS, sizeof(float) * 4 * cells);
+    for (size_t i = 0; i < cells; i++) owner[i] = -1;
+    for (int gy = 0; gy < dh; gy++)
+      for (int gx = 0; gx < dw; gx++) {
+        const int px = tlx + gx, pyg = tly + gy; /* GL coordinates */
+        if (px < 0 || pyg < 0 || px >= cols || pyg >= rows) continue;
+        const float vTex = scale * b->warp[(int64_t)pyg * cols + px];
+        if (vTex < 10e-6f) continue;
+        const float ptx = (float)gx - rcx, pty = (float)gy - rcy;
+        const float rx = (ptx * cs - pty * sn) + rcx, ry = (ptx * sn + pty * cs) + rcy;
+        const int qx = (int)rx, qy = (int)ry;
+        if (qx < 0 || qy < 0 || qx >= cols || qy >= rows) continue;
+        const int64_t ic = (int64_t)(rows - 1 - pyg) * cols + px;  /* canvas row */
+        const int inmap = qx < m && qy < m;
+        const size_t ip = inmap ? (size_t)qy * m + qx : 0;
+        float pK[3] = {0, 0, 0}, pS[3] = {0, 0, 0}, pV = 0.0f;
+        if (inmap) {
+          for (int ch = 0; ch < 3; ch++) { pK[ch] = oK[4 * ip + ch]; pS[ch] = oS[4 * ip + ch]; }
+          pV = oK[4 * ip + 3];
+        }
+        float cK[3], cS[3];
+        for (int ch = 0; ch < 3; ch++) { cK[ch] = K[3 * ic + ch]; cS[ch] = S[3 * ic + ch]; }
+        const float cV = V[ic];
+        const float cVl = cV * 0.7f * vTex, cVr = cV - cVl;
+        const float pVl = pV * 0.7f * vTex, pVr = pV - pVl;
+        const float pVnew = pVr + cVl;
+        if (pVnew > 0.0f && inmap) {
+          const int inv = gx + gy * dw;
+          if (inv > owner[ip]) {
+            owner[ip] = inv;
+            for (int ch = 0; ch < 3; ch++) {
+              nK[4 * ip + ch] = (pVr * pK[ch] + cVl * cK[ch]) / pVnew;
+              nS[4 * ip + ch] = (pVr * pS[ch] + cVl * cS[ch]) / pVnew;
+            }
+            nK[4 * ip + 3] = fmaxf(pVnew, 0.0f);
+            nS[4 * ip + 3] = fmaxf(pVnew, 0.0f);
+          }
+        }
+        const float cVnew = cVr + pVl;
+        if (cVnew > 0.0f) {
+          for (int ch = 0; ch < 3; ch++) {
+            K[3 * ic + ch] = (cVr * pK[ch] + pVl * cK[ch]) / cVnew;
+            S[3 * ic + ch] = (cVr * pS[ch] + pVl * cS[ch]) / cVnew;
+          }
+          V[ic] = fmaxf(cVnew, 0.0f);
+          light[ic] = vTex;
+        }
+      }
+    for (size_t i = 0; i < cells; i++)
+      if (owner[i] >= 0) {
+        memcpy(b->smK + 4 * i, nK + 4 * i, sizeof(float) * 4);
+        memcpy(b->smS + 4 * i, nS + 4 * i, sizeof(float) * 4);
+      }
+  }
+  free(oK); free(oS); free(owner); free(nK); free(nS);
+}
+
+/* TextureBrushGpu::paintStroke(vertices, CanvasGpu&) (TextureBrushGpu.cxx:73-166) */
+void orc_tbgl_paint_stroke(orc_tbgl* b, int rows, int cols, float* K, float* S, float* V,
+                           float* light, const double* xy, int n) {
+  if (n < 2 || !b->tex) return;
+  if (!b->warp || b->w_rows != rows || b->w_cols != cols) {
+    free(b->warp);
+    b->warp = (float*)calloc((size_t)rows * cols, sizeof(float));
+    b->w_rows = rows;
+    b->w_cols = cols;
+  }
+  double bmin[2] = {xy[0], xy[1]}, bmax[2] = {xy[0], xy[1]};
+  for (int i = 0; i < n; i++) {
+    bmin[0] = fmin(bmin[0], xy[2 * i]); bmin[1] = fmin(bmin[1], xy[2 * i + 1]);
+    bmax[0] = fmax(bmax[0], xy[2 * i]); bmax[1] = fmax(bmax[1], xy[2 * i + 1]);
+  }
+  bmin[0] = fmax(bmin[0] - b->radius, 0.0);
+  bmax[0] = fmin(bmax[0] + b->radius, (double)(cols - 1));
+  bmin[1] = fmax(bmin[1] - b->radius, 0.0);
+  bmax[1] = fmin(bmax[1] + b->radius, (double)(rows - 1));
+  tbgl_warp(b, rows, cols, xy, n);
+  const int aw = (int)(bmax[0] - bmin[0]) + 1, ah = (int)(bmax[1] - bmin[1]) + 1;
+  const int ox = (int)bmin[0], oy = (int)bmin[1];
+  if (b->smudge) tbgl_smudge(b, rows, cols, K, S, V, light, xy, n);
+  const float Kb[3] = {b->paint[0], b->paint[1], b->paint[2]};
+  const float Sb[3] = {b->paint[3], b->paint[4], b->paint[5]};
+  const float scale = (float)b->thickness_scale;
+  const int gw = tbgl_round16(aw), gh = tbgl_round16(ah);
+  for (int gy = 0; gy < gh; gy++)  /* PaintTextureFootprintOnCanvas */
+    for (int gx = 0; gx < gw; gx++) {
+      const int px = ox + gx, pyg = rows - (oy + gy) - 1;
+      if (px < 0 || pyg < 0 || px >= cols || pyg >= rows) continue;
+      const int64_t ic = (int64_t)(rows - 1 - pyg) * cols + px;
+      const float vCan = V[ic], vTex = scale * b->warp[(int64_t)pyg * cols + px];
+      const float vSum = vCan + vTex;
+      if (vSum < 10e-6f) continue;
+      for (int ch = 0; ch < 3; ch++) {
+        K[3 * ic + ch] = (vCan * K[3 * ic + ch] + vTex * Kb[ch]) / vSum;
+        S[3 * ic + ch] = (vCan * S[3 * ic + ch] + vTex * Sb[ch]) / vSum;
+      }
+      V[ic] = vSum;
+      light[ic] = (vTex > 0.0f) ? vTex : vCan;
+    }
+  for (int gy = 0; gy < gh; gy++)  /* ClearTextureFootprint */
+    for (int gx = 0; gx < gw; gx++) {
+      const int px = ox + gx, pyg = rows - (oy + gy) - 1;
+      if (px < 0 || pyg < 0 || px >= cols || pyg >= rows) continue;
+      b->warp[(int64_t)pyg * cols + px] = 0.0f;
+    }
+}

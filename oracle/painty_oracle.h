@@ -224,6 +224,32 @@ void orc_lanczos4_coeffs(float x, float c[8]);
 void orc_resize_lanczos4(const double* src, int rows, int cols, int out_rows, int out_cols,
                          double* dst);
 
+/* ---- TextureBrushGpu (renderer/src/TextureBrushGpu.cxx:17-305) with its
+ * shaders BrushTextureWarp.vert/.frag, Smudge.compute,
+ * PaintTextureFootprintOnCanvas.compute and ClearTextureFootprint.compute.
+ * The GL renderer keeps its images y-flipped (gpu/GpuMat.hxx:43-56,
+ * 107-132): GL row g is canvas row H-1-g.  Where GL leaves behaviour to the
+ * implementation this restatement fixes it (and the product follows):
+ * vertices snapped to 1/256 pixel, a pixel covered when its centre is inside
+ * a triangle with the top-left rule on edges (window y up), texture
+ * coordinates interpolated linearly at the centre, the brush texture
+ * sampled with GL_LINEAR and GL_CLAMP_TO_EDGE, later triangles of the strip
+ * overwriting earlier ones, and the smudge shader's writes to one pickup
+ * cell resolved as: every invocation reads the state before the dispatch,
+ * the highest invocation index (x + y * width of the dispatch) wins.
+ * light: the canvas S.a plane (the GL renderer's lighting thickness). */
+typedef struct orc_tbgl orc_tbgl;
+orc_tbgl* orc_tbgl_create(void);
+void orc_tbgl_destroy(orc_tbgl* b);
+void orc_tbgl_set_radius(orc_tbgl* b, double radius);
+void orc_tbgl_dip(orc_tbgl* b, const double K[3], const double S[3]);
+void orc_tbgl_set_texture(orc_tbgl* b, const double* img, int rows, int cols);
+void orc_tbgl_set_thickness_scale(orc_tbgl* b, double s);
+void orc_tbgl_enable_smudge(orc_tbgl* b, int on);
+void orc_tbgl_get_smudge(const orc_tbgl* b, int* side, float* K4, float* S4);
+void orc_tbgl_paint_stroke(orc_tbgl* b, int rows, int cols, float* K, float* S, float* V,
+                           float* light, const double* xy, int n);
+
 #ifdef __cplusplus
 }
 #endif

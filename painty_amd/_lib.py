@@ -150,6 +150,16 @@ SIGNATURES = {
     "pnt_image_read_gray": [C.c_char_p, C.c_int64, C.c_int, C.POINTER(C.c_int),
                             C.POINTER(C.c_int), _dp, C.c_int64],
     "pnt_resize_lanczos4": [_vp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
+    "pnt_tbgl_create": [_vp, C.POINTER(_vp)],
+    "pnt_tbgl_destroy": [_vp],
+    "pnt_tbgl_set_radius": [_vp, C.c_double],
+    "pnt_tbgl_dip": [_vp, _dp, _dp],
+    "pnt_tbgl_set_texture": [_vp, _dp, C.c_int, C.c_int],
+    "pnt_tbgl_set_thickness_scale": [_vp, C.c_double],
+    "pnt_tbgl_enable_smudge": [_vp, C.c_int],
+    "pnt_tbgl_get_smudge": [_vp, C.POINTER(C.c_int), _fp, _fp],
+    "pnt_tbgl_paint_stroke": [_vp, _vp, _dp, C.c_int64],
+    "pnt_canvas_download_lighting": [_vp, _fp],
     "pnt_tbrush_create": [_vp, C.POINTER(_vp)],
     "pnt_tbrush_destroy": [_vp],
     "pnt_tbrush_set_radius": [_vp, C.c_double],

@@ -866,6 +866,7 @@ int pnt_canvas_destroy(pnt_canvas cv) {
     dev_free(cv->scratch);
     dev_free(cv->tmap);
     dev_free(cv->dry_wet);
+    dev_free(cv->light);
     delete cv;
     ctx_release(ctx);
   });

@@ -85,6 +85,9 @@ struct pnt_canvas_s {
   int64_t dry_period_us = 0, dry_last_us = 0;
   float dry_portion = 0.01f;
   unsigned* dry_wet = nullptr;  // device word: a pixel still holds paint
+  // the GL renderer's lighting thickness (its S.a plane), made zero by the
+  // first GL texture-brush stroke; nullptr until then
+  float* light = nullptr;
   int64_t* tmap = nullptr;
   float bg[3] = {1.f, 1.f, 1.f};
   // Bumped by every change of the wet layer; a brush's snapshot dirty bytes

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -681,6 +682,569 @@ int pnt_tbrush_paint_stroke(pnt_tbrush b, pnt_canvas cv, const double* xy, int64
     k_tex_deposit<<<blocks_for(npx), kBlock, 0, s>>>(A);
     PNT_HIP_CHECK(hipGetLastError());
     PNT_HIP_CHECK(hipStreamSynchronize(s));  // the host vectors above end here
+  });
+}
+
+}  // extern "C"
+
+// ===================================================================== GL --
+// TextureBrushGpu (renderer/src/TextureBrushGpu.cxx:17-305): the reference's
+// GL texture brush.  Its images are y-flipped (gpu/GpuMat.hxx:43-56,
+// 107-132), so GL row g is canvas row H-1-g; the warped brush texture is
+// kept in GL row order here too.  Where GL leaves behaviour to the
+// implementation this follows one fixed choice:
+//  - rasterisation: vertices snapped to 1/256 pixel; a pixel is covered when
+//    its centre lies inside a triangle, edges by the top-left rule in window
+//    coordinates (y up); texture coordinates interpolated linearly at the
+//    centre; later triangles of the strip overwrite earlier ones;
+//  - the brush texture is sampled with GL_LINEAR and GL_CLAMP_TO_EDGE;
+//  - the smudge shader's writes to one pickup cell: every invocation reads
+//    the pickup state from before the dispatch and the highest invocation
+//    index wins.
+// Shader arithmetic is FP32 (highp float), host geometry FP64.
+namespace pnt {
+namespace {
+
+struct GlTri {  // a strip vertex: snapped window position, texture coordinates
+  long long X, Y;
+  float s, t;
+};
+
+__device__ __forceinline__ long long gl_cross(long long ax, long long ay, long long bx, long long by,
+                                              long long px, long long py) {
+  return (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+}
+__device__ __forceinline__ bool gl_top_left(long long ax, long long ay, long long bx, long long by) {
+  const long long dx = bx - ax, dy = by - ay;
+  return (dy == 0 && dx < 0) || dy < 0;
+}
+
+struct GlWarpArgs {
+  const GlTri* v;  // strip vertices
+  int nv;
+  int px0, py0, pw, ph;  // pixel box (GL coordinates) the strip can cover
+  int cols;
+  const float* tex;  // brush texture as uploaded (t_rows x t_cols)
+  int t_rows, t_cols;
+  float* warp;       // canvas-sized, GL rows
+};
+
+__device__ double gl_sample(const GlWarpArgs& A, double s, double t) {
+  const double x = s * A.t_cols - 0.5, y = t * A.t_rows - 0.5;
+  const double fx0 = floor(x), fy0 = floor(y);
+  const double ax = x - fx0, ay = y - fy0;
+  const int x0 = (int)fx0, y0 = (int)fy0;
+  const int xa = min(max(x0, 0), A.t_cols - 1), xb = min(max(x0 + 1, 0), A.t_cols - 1);
+  const int ya = min(max(y0, 0), A.t_rows - 1), yb = min(max(y0 + 1, 0), A.t_rows - 1);
+  const auto at = [&](int yy, int xx) { return (double)A.tex[(int64_t)yy * A.t_cols + xx]; };
+  return (at(ya, xa) * (1.0 - ax) + at(ya, xb) * ax) * (1.0 - ay) +
+         (at(yb, xa) * (1.0 - ax) + at(yb, xb) * ax) * ay;
+}
+
+// One thread per pixel of the strip's box: the last triangle covering the
+// pixel centre gives the fragment (BrushTextureWarp.frag: texture().r).
+__global__ __launch_bounds__(kBlock) void k_gl_warp(GlWarpArgs A) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (int64_t)A.pw * A.ph) return;
+  const int px = A.px0 + (int)(e % A.pw), py = A.py0 + (int)(e / A.pw);
+  const long long cx = (long long)px * 256 + 128, cy = (long long)py * 256 + 128;
+  int hit = -1;
+  long long he0 = 0, he1 = 0, he2 = 0, harea = 1;
+  int hid[3] = {0, 0, 0};
+  for (int j = 0; j + 2 < A.nv; ++j) {
+    int id[3] = {j, j + 1, j + 2};
+    long long X[3], Y[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      X[q] = A.v[id[q]].X;
+      Y[q] = A.v[id[q]].Y;
+    }
+    long long area = gl_cross(X[0], Y[0], X[1], Y[1], X[2], Y[2]);
+    if (area == 0) continue;
+    if (area < 0) {
+      long long tx = X[1], ty = Y[1];
+      X[1] = X[2];
+      Y[1] = Y[2];
+      X[2] = tx;
+      Y[2] = ty;
+      const int ti = id[1];
+      id[1] = id[2];
+      id[2] = ti;
+      area = -area;
+    }
+    const long long e0 = gl_cross(X[1], Y[1], X[2], Y[2], cx, cy);
+    const long long e1 = gl_cross(X[2], Y[2], X[0], Y[0], cx, cy);
+    const long long e2 = gl_cross(X[0], Y[0], X[1], Y[1], cx, cy);
+    const bool in0 = e0 > 0 || (e0 == 0 && gl_top_left(X[1], Y[1], X[2], Y[2]));
+    const bool in1 = e1 > 0 || (e1 == 0 && gl_top_left(X[2], Y[2], X[0], Y[0]));
+    const bool in2 = e2 > 0 || (e2 == 0 && gl_top_left(X[0], Y[0], X[1], Y[1]));
+    if (in0 && in1 && in2) {
+      hit = j;
+      he0 = e0;
+      he1 = e1;
+      he2 = e2;
+      harea = area;
+      hid[0] = id[0];
+      hid[1] = id[1];
+      hid[2] = id[2];
+    }
+  }
+  if (hit < 0) return;
+  const double w0 = (double)he0 / (double)harea, w1 = (double)he1 / (double)harea,
+               w2 = (double)he2 / (double)harea;
+  const double s = w0 * A.v[hid[0]].s + w1 * A.v[hid[1]].s + w2 * A.v[hid[2]].s;
+  const double t = w0 * A.v[hid[0]].t + w1 * A.v[hid[1]].t + w2 * A.v[hid[2]].t;
+  A.warp[(int64_t)py * A.cols + px] = (float)gl_sample(A, (double)(float)s, (double)(float)t);
+}
+
+struct GlStep {
+  int tlx, tly;  // dispatch offset (GL coordinates)
+  float cs, sn;  // cos / sin of the heading
+};
+
+struct GlSmudgeArgs {
+  Layer7 canvas;
+  float* light;
+  int rows, cols;
+  const float* warp;
+  float scale;
+  float* smK;  // m*m RGBA
+  float* smS;
+  int m, dw;   // pickup side; dispatch width (m rounded up to 16)
+  float rc;    // rotation centre m * 0.5
+  unsigned long long* owner;  // per cell: (step << 32) | (invocation + 1)
+  float* cand;  // per invocation: 8 candidate values (K rgb V, S rgb V)
+};
+
+// Smudge.compute for one spine step, first half: the canvas update of every
+// invocation, and each invocation's candidate for its pickup cell with a
+// claim on the cell.
+__global__ __launch_bounds__(kBlock) void k_gl_smudge_a(GlSmudgeArgs A, GlStep St,
+                                                        unsigned long long step) {
+  const int64_t inv = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (inv >= (int64_t)A.dw * A.dw) return;
+  const int gx = (int)(inv % A.dw), gy = (int)(inv / A.dw);
+  const int px = St.tlx + gx, pyg = St.tly + gy;
+  if (px < 0 || pyg < 0 || px >= A.cols || pyg >= A.rows) return;
+  const float vTex = A.scale * A.warp[(int64_t)pyg * A.cols + px];
+  if (vTex < 10e-6f) return;
+  const float ptx = (float)gx - A.rc, pty = (float)gy - A.rc;
+  const float rx = (ptx * St.cs - pty * St.sn) + A.rc, ry = (ptx * St.sn + pty * St.cs) + A.rc;
+  const int qx = (int)rx, qy = (int)ry;
+  if (qx < 0 || qy < 0 || qx >= A.cols || qy >= A.rows) return;
+  const int64_t ic = (int64_t)(A.rows - 1 - pyg) * A.cols + px;
+  const bool inmap = qx < A.m && qy < A.m;
+  const int64_t ip = inmap ? (int64_t)qy * A.m + qx : 0;
+  float pK[3] = {0.0f, 0.0f, 0.0f}, pS[3] = {0.0f, 0.0f, 0.0f}, pV = 0.0f;
+  if (inmap) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pK[c] = A.smK[4 * ip + c];
+      pS[c] = A.smS[4 * ip + c];
+    }
+    pV = A.smK[4 * ip + 3];
+  }
+  float cK[3], cS[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    cK[c] = A.canvas.p[c][ic];
+    cS[c] = A.canvas.p[3 + c][ic];
+  }
+  const float cV = A.canvas.p[6][ic];
+  const float cVl = cV * 0.7f * vTex, cVr = cV - cVl;
+  const float pVl = pV * 0.7f * vTex, pVr = pV - pVl;
+  const float pVnew = pVr + cVl;
+  if (pVnew > 0.0f && inmap) {
+    float* o = A.cand + 8 * inv;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      o[c] = (pVr * pK[c] + cVl * cK[c]) / pVnew;
+      o[4 + c] = (pVr * pS[c] + cVl * cS[c]) / pVnew;
+    }
+    o[3] = o[7] = fmaxf(pVnew, 0.0f);
+    atomicMax(A.owner + ip, (step << 32) | (unsigned long long)(inv + 1));
+  }
+  const float cVnew = cVr + pVl;
+  if (cVnew > 0.0f) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      A.canvas.p[c][ic] = (cVr * pK[c] + pVl * cK[c]) / cVnew;
+      A.canvas.p[3 + c][ic] = (cVr * pS[c] + pVl * cS[c]) / cVnew;
+    }
+    A.canvas.p[6][ic] = fmaxf(cVnew, 0.0f);
+    A.light[ic] = vTex;
+  }
+}
+
+// Second half: each claimed cell takes its winner's candidate.
+__global__ __launch_bounds__(kBlock) void k_gl_smudge_b(GlSmudgeArgs A, unsigned long long step) {
+  const int64_t ip = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (ip >= (int64_t)A.m * A.m) return;
+  const unsigned long long w = A.owner[ip];
+  if ((w >> 32) != step) return;
+  const float* o = A.cand + 8 * (int64_t)((w & 0xffffffffull) - 1);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    A.smK[4 * ip + c] = o[c];
+    A.smS[4 * ip + c] = o[4 + c];
+  }
+}
+
+// PaintTextureFootprintOnCanvas.compute then ClearTextureFootprint.compute
+// over the stroke's box rounded up to 16 x 16 workgroups.
+__global__ __launch_bounds__(kBlock) void k_gl_imprint(Layer7 canvas, float* light, int rows,
+                                                       int cols, float* warp, float scale,
+                                                       float3 Kb, float3 Sb, int ox, int oy,
+                                                       int gw, int gh) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (int64_t)gw * gh) return;
+  const int gx = (int)(e % gw), gy = (int)(e / gw);
+  const int px = ox + gx, pyg = rows - (oy + gy) - 1;
+  if (px < 0 || pyg < 0 || px >= cols || pyg >= rows) return;
+  const int64_t iw = (int64_t)pyg * cols + px;
+  const int64_t ic = (int64_t)(rows - 1 - pyg) * cols + px;
+  const float vCan = canvas.p[6][ic], vTex = scale * warp[iw];
+  const float vSum = vCan + vTex;
+  if (!(vSum < 10e-6f)) {
+    const float kb[3] = {Kb.x, Kb.y, Kb.z}, sb[3] = {Sb.x, Sb.y, Sb.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      canvas.p[c][ic] = (vCan * canvas.p[c][ic] + vTex * kb[c]) / vSum;
+      canvas.p[3 + c][ic] = (vCan * canvas.p[3 + c][ic] + vTex * sb[c]) / vSum;
+    }
+    canvas.p[6][ic] = vSum;
+    light[ic] = (vTex > 0.0f) ? vTex : vCan;
+  }
+  warp[iw] = 0.0f;
+}
+
+int round16(int v) { return (v + 15) / 16 * 16; }
+
+}  // namespace
+}  // namespace pnt
+
+struct pnt_tbgl_s {
+  pnt_ctx ctx = nullptr;
+  double radius = 0.0;
+  float paint[6] = {0, 0, 0, 0, 0, 0};
+  float* tex = nullptr;
+  int t_rows = 0, t_cols = 0;
+  double scale = 1.0;
+  bool smudge = false;  // TextureBrushGpu.hxx:63
+  int m = 0;            // smudge window side
+  float* smK = nullptr;
+  float* smS = nullptr;
+  float* warp = nullptr;  // canvas-sized, GL rows
+  int w_rows = 0, w_cols = 0;
+  // per-stroke scratch
+  GlTri* verts = nullptr;
+  int64_t verts_cap = 0;
+  unsigned long long* owner = nullptr;
+  float* cand = nullptr;
+  int64_t sm_cap = 0;  // cells for owner / invocations for cand
+};
+
+namespace pnt {
+namespace {
+void gl_reset_smudge(pnt_tbgl_s* b, int m) {
+  hipStream_t s = b->ctx->stream;
+  PNT_HIP_CHECK(hipStreamSynchronize(s));
+  dev_free(b->smK);
+  dev_free(b->smS);
+  dev_free(b->owner);
+  dev_free(b->cand);
+  b->smK = b->smS = b->cand = nullptr;
+  b->owner = nullptr;
+  b->sm_cap = 0;
+  b->m = m;
+  const int64_t cells = (int64_t)m * m;
+  b->smK = dev_alloc(4 * cells + 4);
+  b->smS = dev_alloc(4 * cells + 4);
+  launch_fill(b->smK, 4 * cells + 4, 0.0f, s);
+  launch_fill(b->smS, 4 * cells + 4, 0.0f, s);
+  const int dw = round16(m);
+  b->owner = reinterpret_cast<unsigned long long*>(dev_alloc(2 * cells + 2));
+  b->cand = dev_alloc(8 * (int64_t)dw * dw + 8);
+  launch_fill(reinterpret_cast<float*>(b->owner), 2 * cells + 2, 0.0f, s);
+  b->sm_cap = cells;
+}
+void gl_set_radius(pnt_tbgl_s* b, double radius) {  // TextureBrushGpu.cxx:48-60
+  if (!(std::fabs(b->radius - radius) < 0.5)) {
+    b->radius = radius;
+    gl_reset_smudge(b, (int)(uint32_t)(2.0 * radius) + 1);
+  }
+}
+}  // namespace
+}  // namespace pnt
+
+extern "C" {
+
+int pnt_tbgl_create(pnt_ctx ctx, pnt_tbgl* out) {
+  return guard_call([&] {
+    PNT_REQUIRE(ctx && out, "NULL argument");
+    ctx->use();
+    auto* b = new pnt_tbgl_s;
+    b->ctx = ctx;
+    try {
+      gl_set_radius(b, 5.0);  // TextureBrushGpu.cxx:45
+    } catch (...) {
+      dev_free(b->smK);
+      dev_free(b->smS);
+      dev_free(b->owner);
+      dev_free(b->cand);
+      delete b;
+      throw;
+    }
+    ctx_retain(ctx);
+    *out = b;
+  });
+}
+
+int pnt_tbgl_destroy(pnt_tbgl b) {
+  return guard_call([&] {
+    if (!b) return;
+    pnt_ctx ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : {(void*)b->tex, (void*)b->smK, (void*)b->smS, (void*)b->warp,
+                    (void*)b->verts, (void*)b->owner, (void*)b->cand})
+      dev_free(p);
+    delete b;
+    ctx_release(ctx);
+  });
+}
+
+int pnt_tbgl_set_radius(pnt_tbgl b, double radius) {
+  return guard_call([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    PNT_REQUIRE(std::isfinite(radius) && radius >= 0.0 && radius < 2048.0,
+                "radius must be in [0, 2048)");
+    b->ctx->use();
+    gl_set_radius(b, radius);
+  });
+}
+
+int pnt_tbgl_dip(pnt_tbgl b, const double K[3], const double S[3]) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && K && S, "NULL argument");
+    for (int c = 0; c < 3; ++c) {
+      b->paint[c] = (float)K[c];
+      b->paint[3 + c] = (float)S[c];
+    }
+  });
+}
+
+int pnt_tbgl_set_texture(pnt_tbgl b, const double* img, int rows, int cols) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && img, "NULL argument");
+    PNT_REQUIRE(rows > 0 && cols > 0 && (int64_t)rows * cols < ((int64_t)1 << 28),
+                "texture size out of range");
+    const int64_t n = (int64_t)rows * cols;
+    double lo = img[0], hi = img[0];
+    for (int64_t i = 0; i < n; ++i) {
+      PNT_REQUIRE(std::isfinite(img[i]), "texture values must be finite");
+      lo = std::min(lo, img[i]);
+      hi = std::max(hi, img[i]);
+    }
+    // TextureBrushDictionary.cxx:106-123: NORM_MINMAX, flipped, FP32
+    const double sc = hi > lo ? 1.0 / (hi - lo) : 0.0;
+    std::vector<float> t((size_t)n);
+    for (int r = 0; r < rows; ++r)
+      for (int c = 0; c < cols; ++c)
+        t[(size_t)(rows - 1 - r) * cols + c] = (float)((img[(int64_t)r * cols + c] - lo) * sc);
+    b->ctx->use();
+    hipStream_t s = b->ctx->stream;
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    dev_free(b->tex);
+    b->tex = nullptr;
+    b->tex = dev_alloc(n);
+    PNT_HIP_CHECK(hipMemcpyAsync(b->tex, t.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    b->t_rows = rows;
+    b->t_cols = cols;
+  });
+}
+
+int pnt_tbgl_set_thickness_scale(pnt_tbgl b, double scale) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && std::isfinite(scale), "invalid argument");
+    b->scale = scale;
+  });
+}
+
+int pnt_tbgl_enable_smudge(pnt_tbgl b, int enable) {
+  return guard_call([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    b->smudge = enable != 0;
+  });
+}
+
+int pnt_tbgl_get_smudge(pnt_tbgl b, int* side, float* K4, float* S4) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && side, "NULL argument");
+    b->ctx->use();
+    *side = b->m;
+    const int64_t c = (int64_t)b->m * b->m;
+    hipStream_t s = b->ctx->stream;
+    if (K4) PNT_HIP_CHECK(hipMemcpyAsync(K4, b->smK, 4 * c * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (S4) PNT_HIP_CHECK(hipMemcpyAsync(S4, b->smS, 4 * c * sizeof(float), hipMemcpyDeviceToHost, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int pnt_canvas_download_lighting(pnt_canvas cv, float* out) {
+  return guard_call([&] {
+    PNT_REQUIRE(cv && out, "NULL argument");
+    cv->ctx->use();
+    canvas_tick(cv);
+    hipStream_t s = cv->ctx->stream;
+    if (cv->light)
+      PNT_HIP_CHECK(hipMemcpyAsync(out, cv->light, cv->n * sizeof(float), hipMemcpyDeviceToHost, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    if (!cv->light) std::fill(out, out + cv->n, 0.0f);
+  });
+}
+
+int pnt_tbgl_paint_stroke(pnt_tbgl b, pnt_canvas cv, const double* xy, int64_t n) {
+  return guard_call([&] {
+    PNT_REQUIRE(b && cv && (xy || n == 0) && n >= 0, "invalid argument");
+    PNT_REQUIRE(b->ctx->device == cv->ctx->device, "brush and canvas on different devices");
+    PNT_REQUIRE(n <= (1 << 20), "too many vertices");
+    for (int64_t i = 0; i < 2 * n; ++i) PNT_REQUIRE(std::isfinite(xy[i]), "non-finite vertex");
+    if (n < 2) return;
+    PNT_REQUIRE(b->tex, "no brush texture (pnt_tbgl_set_texture)");
+    b->ctx->use();
+    canvas_tick(cv);
+    hipStream_t s = b->ctx->stream;
+    if (cv->ctx->stream != s) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    const int rows = cv->rows, cols = cv->cols;
+    cv->epoch++;
+    if (!cv->light) {
+      cv->light = dev_alloc(cv->n);
+      launch_fill(cv->light, cv->n, 0.0f, s);
+    }
+    if (!b->warp || b->w_rows != rows || b->w_cols != cols) {  // TextureBrushGpu.cxx:170-178
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+      dev_free(b->warp);
+      b->warp = nullptr;
+      b->warp = dev_alloc((int64_t)rows * cols);
+      launch_fill(b->warp, (int64_t)rows * cols, 0.0f, s);
+      b->w_rows = rows;
+      b->w_cols = cols;
+    }
+    const double r = b->radius;
+    std::vector<double2> v((size_t)n);
+    for (int64_t i = 0; i < n; ++i) v[(size_t)i] = make_double2(xy[2 * i], xy[2 * i + 1]);
+    double bmin[2] = {xy[0], xy[1]}, bmax[2] = {xy[0], xy[1]};
+    for (int64_t i = 0; i < n; ++i) {
+      bmin[0] = std::min(bmin[0], xy[2 * i]);
+      bmin[1] = std::min(bmin[1], xy[2 * i + 1]);
+      bmax[0] = std::max(bmax[0], xy[2 * i]);
+      bmax[1] = std::max(bmax[1], xy[2 * i + 1]);
+    }
+    bmin[0] = std::max(bmin[0] - r, 0.0);
+    bmax[0] = std::min(bmax[0] + r, (double)(cols - 1));
+    bmin[1] = std::max(bmin[1] - r, 0.0);
+    bmax[1] = std::min(bmax[1] + r, (double)(rows - 1));
+    // the strip r0 l0 r1 l1 ... (generateWarpedTexture, TextureBrushGpu.cxx:186-212)
+    const int nv = (int)(2 * n);
+    std::vector<GlTri> tri((size_t)nv);
+    long long xmin = LLONG_MAX, xmax = LLONG_MIN, ymin = LLONG_MAX, ymax = LLONG_MIN;
+    for (int i = 0; i < (int)n; ++i) {
+      const double u = (double)i / (double)(n - 1);
+      double c[2], t[2];
+      spline_cr(v, u, false, c);
+      spline_cr(v, u, true, t);
+      normalize2(t);
+      const double d[2] = {-t[1], t[0]};
+      const float rx = (float)(c[0] + r * d[0]), ry = (float)(c[1] + r * d[1]);
+      const float lx = (float)(c[0] - r * d[0]), ly = (float)(c[1] - r * d[1]);
+      const float pos[2][2] = {{rx, ry}, {lx, ly}};
+      for (int k = 0; k < 2; ++k) {
+        GlTri& g = tri[(size_t)(2 * i + k)];
+        g.X = (long long)std::llrint((double)pos[k][0] * 256.0);
+        g.Y = (long long)std::llrint((double)((float)rows - pos[k][1]) * 256.0);
+        g.s = (float)u;
+        g.t = k == 0 ? 0.0f : 1.0f;
+        xmin = std::min(xmin, g.X);
+        xmax = std::max(xmax, g.X);
+        ymin = std::min(ymin, g.Y);
+        ymax = std::max(ymax, g.Y);
+      }
+    }
+    const int px0 = (int)std::max<long long>(0, (xmin - 128) >> 8);
+    const int px1 = (int)std::min<long long>(cols - 1, ((xmax - 128) >> 8) + 1);
+    const int py0 = (int)std::max<long long>(0, (ymin - 128) >> 8);
+    const int py1 = (int)std::min<long long>(rows - 1, ((ymax - 128) >> 8) + 1);
+    if (nv > b->verts_cap) {
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+      dev_free(b->verts);
+      b->verts = nullptr;
+      b->verts = reinterpret_cast<GlTri*>(dev_alloc((int64_t)nv * (int64_t)(sizeof(GlTri) / 4)));
+      b->verts_cap = nv;
+    }
+    PNT_HIP_CHECK(hipMemcpyAsync(b->verts, tri.data(), nv * sizeof(GlTri), hipMemcpyHostToDevice, s));
+    if (px1 >= px0 && py1 >= py0) {
+      GlWarpArgs W{b->verts, nv, px0, py0, px1 - px0 + 1, py1 - py0 + 1, cols, b->tex,
+                   b->t_rows, b->t_cols, b->warp};
+      k_gl_warp<<<blocks_for((int64_t)W.pw * W.ph), kBlock, 0, s>>>(W);
+      PNT_HIP_CHECK(hipGetLastError());
+    }
+    if (b->smudge) {  // TextureBrushGpu::smudge (TextureBrushGpu.cxx:253-305)
+      double length = 0.0;
+      for (int64_t i = 1; i < n; ++i)
+        length += std::sqrt((xy[2 * i] - xy[2 * i - 2]) * (xy[2 * i] - xy[2 * i - 2]) +
+                            (xy[2 * i + 1] - xy[2 * i - 1]) * (xy[2 * i + 1] - xy[2 * i - 1]));
+      PNT_REQUIRE(!(length > 0.0) || length <= (double)(1 << 22),
+                  "stroke too long for the smudge");
+      PNT_REQUIRE(!(length > 0.0) || length >= 1e-3,
+                  "stroke shorter than 1e-3 pixels but not empty: too many smudge steps");
+      GlSmudgeArgs A{};
+      A.canvas = cv->layer();
+      A.light = cv->light;
+      A.rows = rows;
+      A.cols = cols;
+      A.warp = b->warp;
+      A.scale = (float)b->scale;
+      A.smK = b->smK;
+      A.smS = b->smS;
+      A.m = b->m;
+      A.dw = round16(b->m);
+      A.rc = (float)(b->m * 0.5);
+      A.owner = b->owner;
+      A.cand = b->cand;
+      unsigned long long step = 0;
+      int lastx = -1, lasty = -1;
+      for (double u = 0.0; u <= 1.0; u += (0.25 / length)) {
+        double c[2], t[2];
+        spline_cr(v, u, false, c);
+        if ((int)c[0] == lastx && (int)c[1] == lasty) continue;
+        lastx = (int)c[0];
+        lasty = (int)c[1];
+        spline_cr(v, u, true, t);
+        normalize2(t);
+        const float th = (float)std::atan2(t[1], t[0]);
+        GlStep St{(int)((double)(int)c[0] - (double)b->m * 0.5),
+                  (int)((double)(int)c[1] - (double)b->m * 0.5), std::cos(th), std::sin(th)};
+        ++step;
+        PNT_REQUIRE(step < (1ull << 31), "too many smudge steps");
+        k_gl_smudge_a<<<blocks_for((int64_t)A.dw * A.dw), kBlock, 0, s>>>(A, St, step);
+        k_gl_smudge_b<<<blocks_for((int64_t)A.m * A.m), kBlock, 0, s>>>(A, step);
+        PNT_HIP_CHECK(hipGetLastError());
+      }
+      // the claims carry the step number; start the next stroke from zero
+      launch_fill(reinterpret_cast<float*>(b->owner), 2 * (int64_t)b->m * b->m + 2, 0.0f, s);
+    }
+    const int aw = (int)(bmax[0] - bmin[0]) + 1, ah = (int)(bmax[1] - bmin[1]) + 1;
+    if (aw > 0 && ah > 0) {
+      const int gw = round16(aw), gh = round16(ah);
+      k_gl_imprint<<<blocks_for((int64_t)gw * gh), kBlock, 0, s>>>(
+          cv->layer(), cv->light, rows, cols, b->warp, (float)b->scale,
+          make_float3(b->paint[0], b->paint[1], b->paint[2]),
+          make_float3(b->paint[3], b->paint[4], b->paint[5]), (int)bmin[0], (int)bmin[1], gw, gh);
+      PNT_HIP_CHECK(hipGetLastError());
+    }
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
   });
 }
 

@@ -558,6 +558,63 @@ class TextureBrush:
              int(xy.shape[0]))
 
 
+class TextureBrushGpu:
+    """The reference's GL texture brush (renderer/src/TextureBrushGpu.cxx:
+    17-305) on the device; the brush texture is set by the caller
+    (setTexture) instead of drawn from the texture dictionary."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        call("pnt_tbgl_create", self.ctx.h, C.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                load().pnt_tbgl_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def setRadius(self, radius: float) -> None:
+        call("pnt_tbgl_set_radius", self.h, float(radius))
+
+    def dip(self, paint) -> None:
+        K = np.ascontiguousarray(paint[0], np.float64).reshape(3)
+        S = np.ascontiguousarray(paint[1], np.float64).reshape(3)
+        call("pnt_tbgl_dip", self.h, K.ctypes.data_as(_dp), S.ctypes.data_as(_dp))
+
+    def setTexture(self, img) -> None:
+        a = np.ascontiguousarray(img, np.float64)
+        call("pnt_tbgl_set_texture", self.h, a.ctypes.data_as(_dp), a.shape[0], a.shape[1])
+
+    def setThicknessScale(self, scale: float) -> None:
+        call("pnt_tbgl_set_thickness_scale", self.h, float(scale))
+
+    def enableSmudge(self, enable: bool) -> None:
+        call("pnt_tbgl_enable_smudge", self.h, int(bool(enable)))
+
+    def getSmudge(self):
+        side = C.c_int()
+        call("pnt_tbgl_get_smudge", self.h, C.byref(side), None, None)
+        n = side.value
+        K, S = np.zeros((n, n, 4), np.float32), np.zeros((n, n, 4), np.float32)
+        call("pnt_tbgl_get_smudge", self.h, C.byref(side), _pf(K), _pf(S))
+        return K, S
+
+    def paintStroke(self, vertices, canvas: Canvas) -> None:
+        xy = np.ascontiguousarray(vertices, np.float64).reshape(-1, 2)
+        call("pnt_tbgl_paint_stroke", self.h, canvas.h, xy.ctypes.data_as(_dp), int(xy.shape[0]))
+
+
+def canvas_lighting(canvas: Canvas) -> np.ndarray:
+    """The GL renderer's lighting thickness plane (S.a) of a canvas."""
+    out = np.empty((canvas.rows, canvas.cols), np.float32)
+    call("pnt_canvas_download_lighting", canvas.h, _pf(out))
+    return out
+
+
 def stamps_to_c(stamps):
     """A workload.STAMP_DTYPE array (or a C array already) -> PntStamp array."""
     if isinstance(stamps, C.Array):

@@ -1,0 +1,93 @@
+"""The GL texture brush (TextureBrushGpu, renderer/src/TextureBrushGpu.cxx:
+17-305) on the device against the oracle's restatement: the same FP32
+arithmetic in the same order, so the canvas planes, the lighting plane and
+the smudge maps agree to 1e-6 relative (in practice bit for bit)."""
+import numpy as np
+import pytest
+
+from tests.test_oracle_texture import sample_thickness
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(hip, oracle, rows, cols, seed, radius, tex, smudge, scale=1.0,
+          paint=((0.2, 0.3, 0.4), (0.1, 0.23, 0.14))):
+    rng = np.random.default_rng(seed)
+    K = rng.uniform(0.05, 0.9, (rows, cols, 3)).astype(np.float32)
+    S = rng.uniform(0.05, 0.9, (rows, cols, 3)).astype(np.float32)
+    V = rng.uniform(0.0, 0.6, (rows, cols)).astype(np.float32)
+    gc = hip.Canvas(rows, cols)
+    gc.getPaintLayer().setBuffers(K, S, V)
+    oc = oracle.GlCanvas(K, S, V)
+    gb, ob = hip.TextureBrushGpu(), oracle.TextureBrushGpu()
+    for b, gpu in ((gb, True), (ob, False)):
+        if gpu:
+            b.setRadius(radius)
+            b.setTexture(tex)
+            b.setThicknessScale(scale)
+            b.enableSmudge(smudge)
+            b.dip(paint)
+        else:
+            b.set_radius(radius)
+            b.set_texture(tex)
+            b.set_thickness_scale(scale)
+            b.enable_smudge(smudge)
+            b.dip(*paint)
+    return gc, oc, gb, ob
+
+
+def _check(tag, hip, gc, oc, gb, ob):
+    L = gc.getPaintLayer()
+    got = {"K": L.getK_buffer(), "S": L.getS_buffer(), "V": L.getV_buffer(),
+           "light": hip.canvas_lighting(gc)}
+    want = {"K": oc.K, "S": oc.S, "V": oc.V, "light": oc.light}
+    gK, gS = gb.getSmudge()
+    oK, oS = ob.smudge()
+    got.update(smK=gK, smS=gS)
+    want.update(smK=oK, smS=oS)
+    for k in got:
+        a, b = np.asarray(got[k], np.float64), np.asarray(want[k], np.float64)
+        err = np.abs(a - b) / np.maximum(np.abs(b), 1e-6)
+        print("parity %s.%s max_rel=%.3e exact=%s" % (tag, k, err.max() if err.size else 0.0,
+                                                      np.array_equal(a, b)), flush=True)
+        assert err.max() <= 1e-6 if err.size else True, (tag, k)
+
+
+STROKES = [np.array([[6.3, 30.4], [40.2, 22.7], [70.8, 35.1], [90.6, 28.3]]),  # curved
+           np.array([[10.2, 12.4], [80.7, 14.1], [12.6, 20.3]]),                 # hairpin
+           np.array([[-8.4, 50.2], [30.1, 58.9], [60.3, 70.5]])]                 # off canvas
+
+
+@pytest.mark.parametrize("smudge", [False, True])
+def test_strokes_match_oracle(hip, oracle, smudge):
+    gc, oc, gb, ob = _pair(hip, oracle, 64, 96, 3, 7.3, sample_thickness(), smudge, 0.8)
+    for k, xy in enumerate(STROKES):
+        gb.paintStroke(xy, gc)
+        ob.paint_stroke(xy, oc)
+        _check("gl/smudge%d/stroke%d" % (smudge, k), hip, gc, oc, gb, ob)
+
+
+def test_radius_change_and_middle_rows(hip, oracle):
+    """A stroke along the middle rows (where the smudge's unflipped window
+    meets it), a radius change (new, cleared smudge maps), a second stroke."""
+    gc, oc, gb, ob = _pair(hip, oracle, 64, 80, 5, 4.2, sample_thickness(), True)
+    xy = np.array([[5.4, 31.6], [40.3, 32.2], [75.1, 31.1]])
+    gb.paintStroke(xy, gc)
+    ob.paint_stroke(xy, oc)
+    _check("gl/middle", hip, gc, oc, gb, ob)
+    gb.setRadius(9.7)
+    ob.set_radius(9.7)
+    xy2 = np.array([[70.2, 20.3], [40.1, 40.7], [8.8, 44.4]])
+    gb.paintStroke(xy2, gc)
+    ob.paint_stroke(xy2, oc)
+    _check("gl/radius", hip, gc, oc, gb, ob)
+    assert gb.getSmudge()[0].shape[0] == 20
+
+
+def test_no_texture_is_an_error(hip):
+    from painty_amd import _lib
+    gb = hip.TextureBrushGpu()
+    gc = hip.Canvas(16, 16)
+    gb.paintStroke(np.array([[3.0, 3.0]]), gc)  # one vertex: nothing
+    with pytest.raises(_lib.InvalidArgument):
+        gb.paintStroke(np.array([[2.0, 2.0], [9.0, 9.0]]), gc)
