@@ -682,8 +682,9 @@ def main():
     stamps_s = args.stamps / (ms_per_step * 1e-3)
     stamp_gbs = stamp_bytes / (st_ms * 1e-3) / 1e9
     waves = one_call and os.environ.get("PAINTY_FUSED_SCHED", "waves") != "streams"
+    tile = 32 if os.environ.get("PAINTY_FUSED_TILE", "16") == "32" else 16
     kernel = {"plain": "k_stamp_main+k_ring_refresh", "persistent": "k_persistent",
-              "fused": "k_stamp_wave<16>" if waves else "k_stamp_fused<16>",
+              "fused": ("k_stamp_wave<%d>" if waves else "k_stamp_fused<%d>") % tile,
               "chained": "k_stamp_chain<16>"}[args.mode]
     launches = None
     if one_call:  # stamp-kernel launches of one step's stroke-list call
