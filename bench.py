@@ -128,7 +128,7 @@ def bench_batched(args):
     from painty_amd import hip as H
     from painty_amd import workload as W
     t0 = time.perf_counter()
-    bw = W.cfg3(n_canvas=args.canvases)
+    bw = W.batch256(n_canvas=args.canvases)
     log("bench: %s generated in %.1fs" % (bw.name, time.perf_counter() - t0))
     ctx = H.default_context()
     stream = ctx.stream
@@ -239,7 +239,7 @@ def bench_diff(args):
     from painty_amd import hip as H
     from painty_amd import workload as W
     t0 = time.perf_counter()
-    dw = W.cfg4(size=args.size if args.size != 4096 else 2048, n_stamps=500)
+    dw = W.diff2048_500stamps(size=args.size if args.size != 4096 else 2048, n_stamps=500)
     wl = dw.wl
     log("bench: %s generated in %.1fs" % (wl.name, time.perf_counter() - t0))
     dev = "cuda:0"
@@ -554,7 +554,7 @@ def main():
     from painty_amd import workload as W
 
     t0 = time.perf_counter()
-    wl = W.cfg2(size=args.size, n_stamps=args.stamps)
+    wl = W.canvas4096_2000stamps(size=args.size, n_stamps=args.stamps)
     log("bench: workload %s generated in %.1fs (%d strokes)" % (wl.name, time.perf_counter() - t0,
                                                                len(wl.cores)))
     ctx = H.default_context()

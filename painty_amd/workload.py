@@ -197,18 +197,18 @@ class Workload:
     deposition_rate: float = 0.05
 
 
-def cfg1(pickup: bool = False, seed: int = 1, size: int = 1024, n_stamps: int = 200) -> Workload:
+def canvas1024_200stamps(pickup: bool = False, seed: int = 1, size: int = 1024, n_stamps: int = 200) -> Workload:
     """1024x1024 canvas, 200 stamps of a 65x65 Gaussian footprint (sigma 16, radius 32)."""
     st = strokes(size, size, n_stamps, seed, stamps_per_stroke=(20, 60))
     st["brush"][:] = 0
     core = gaussian_core(65, 16.0)
-    return Workload("cfg1-%d-%dstamps-%s" % (size, n_stamps, "pickup" if pickup else "deposit"),
+    return Workload("canvas%d-%dstamps-%s" % (size, n_stamps, "pickup" if pickup else "deposit"),
                     size, size, paint_layer(size, size, seed, 100), paint_layer(size, size, seed, 0),
                     substrate(size, size, seed), st, [core], [32.0],
                     pickup_rate=0.9 if pickup else 0.0)
 
 
-def cfg2(seed: int = 2, size: int = 4096, n_stamps: int = 2000, dry_every: int = 500,
+def canvas4096_2000stamps(seed: int = 2, size: int = 4096, n_stamps: int = 2000, dry_every: int = 500,
          foot=(129, 257)) -> Workload:
     """4096x4096 canvas, 2000 stamps, per-stroke odd footprint in [129, 257]
     (Gaussian sigma=F/4 times value noise), pickup on, dry every 500 stamps."""
@@ -221,7 +221,7 @@ def cfg2(seed: int = 2, size: int = 4096, n_stamps: int = 2000, dry_every: int =
         radii.append((F - 1) / 2.0)
     if dry_every:
         st["flags"][dry_every - 1::dry_every] |= DRY_AFTER
-    return Workload("cfg2-%d-%dstamps" % (size, n_stamps), size, size,
+    return Workload("canvas%d-%dstamps" % (size, n_stamps), size, size,
                     paint_layer(size, size, seed, 100), paint_layer(size, size, seed, 0),
                     substrate(size, size, seed), st, cores, radii)
 
@@ -249,7 +249,7 @@ class _Pool:
                       self.V[idx].reshape(rows, cols)), self.R0[idx].reshape(rows, cols, 3))
 
 
-def cfg3(n_canvas: int = 256, seed: int = 3, sides=(256, 2048), stamps=(50, 500),
+def batch256(n_canvas: int = 256, seed: int = 3, sides=(256, 2048), stamps=(50, 500),
          foot=(32, 128), pool: bool = True) -> BatchWorkload:
     """Batched: n_canvas canvases with independent sides uniform in
     [sides], stamps uniform in [stamps], one footprint per canvas with
@@ -277,9 +277,9 @@ def cfg3(n_canvas: int = 256, seed: int = 3, sides=(256, 2048), stamps=(50, 500)
         sts = strokes(rows, cols, n_st, seed * 1000 + j, stamps_per_stroke=(20, 80),
                       reach=0.15)
         sts["brush"][:] = 0
-        jobs.append(Workload("cfg3-%d-%dx%d" % (j, rows, cols), rows, cols, dry, wet, R0, sts,
+        jobs.append(Workload("batch-job%d-%dx%d" % (j, rows, cols), rows, cols, dry, wet, R0, sts,
                              [gaussian_core(width, width / 4.0)], [float(radius)]))
-    return BatchWorkload("cfg3-%d" % n_canvas, jobs)
+    return BatchWorkload("batch-%d" % n_canvas, jobs)
 
 
 def small(rows: int, cols: int, n_stamps: int, radius: int, seed: int, pickup: bool = True,
@@ -321,7 +321,7 @@ def dirichlet_rows(n: int, k: int, st: Stream) -> np.ndarray:
     return e / e.sum(axis=1, keepdims=True)
 
 
-def cfg4(seed: int = 4, size: int = 2048, n_stamps: int = 500, n_paint: int = 64,
+def diff2048_500stamps(seed: int = 4, size: int = 2048, n_stamps: int = 500, n_paint: int = 64,
          radius: int = 32, pickup: bool = True) -> DiffWorkload:
     """Differentiable configuration: size x size canvas, n_stamps stamps of a
     (2r+1)^2 Gaussian footprint (sigma = width/4) along seeded strokes, each
@@ -331,7 +331,7 @@ def cfg4(seed: int = 4, size: int = 2048, n_stamps: int = 500, n_paint: int = 64
     st = strokes(size, size, n_stamps, seed, stamps_per_stroke=(20, 60), reach=0.15)
     st["brush"][:] = 0
     width = 2 * radius + 1
-    wl = Workload("cfg4-%d-%dstamps" % (size, n_stamps), size, size,
+    wl = Workload("diff%d-%dstamps" % (size, n_stamps), size, size,
                   paint_layer(size, size, seed, 100), paint_layer(size, size, seed, 0),
                   substrate(size, size, seed), st, [gaussian_core(width, width / 4.0)],
                   [float(radius)], pickup_rate=0.9 if pickup else 0.0)

@@ -79,10 +79,10 @@ def test_batch_big_and_small_jobs(hip, oracle):
         assert_close("bigsmall/job%d.out" % k, out.getR0(), ocv.compose())
 
 
-def test_batch_cfg3_reduced(hip, oracle):
+def test_batch_reduced(hip, oracle):
     """The batched configuration at reduced size: 24 canvases 128..384 px,
     10..60 stamps, footprints 32..128 px, pixels from the shared pool."""
-    bw = W.cfg3(n_canvas=24, seed=5, sides=(128, 384), stamps=(10, 60), foot=(32, 128))
+    bw = W.batch256(n_canvas=24, seed=5, sides=(128, 384), stamps=(10, 60), foot=(32, 128))
     gpu = []
     for wl in bw.jobs:
         cv, br, out = _setup(hip, wl)
@@ -90,8 +90,8 @@ def test_batch_cfg3_reduced(hip, oracle):
     hip.batch_run(gpu, mode=0)
     for k, (wl, (cv, br, _, out)) in enumerate(zip(bw.jobs, gpu)):
         ocv, ob = run_oracle(wl)
-        compare_canvas("cfg3r/job%d" % k, cv, ocv, pickup=(br, ob))
-        assert_close("cfg3r/job%d.out" % k, out.getR0(), ocv.compose())
+        compare_canvas("batch-reduced/job%d" % k, cv, ocv, pickup=(br, ob))
+        assert_close("batch-reduced/job%d.out" % k, out.getR0(), ocv.compose())
 
 
 def test_batch_then_single_calls_continue_state(hip, oracle):

@@ -156,10 +156,10 @@ def test_autograd_function_end_to_end(hip, oracle):
     check_grad("autograd", g, ref)
 
 
-def test_cfg4_full_size_against_oracle(hip, oracle):
+def test_diff2048_full_size_against_oracle(hip, oracle):
     """The differentiable configuration at full size: 2048^2, 500 stamps,
     64 paints x 14 pigments, L2 loss against smooth noise."""
-    dw = W.cfg4()
+    dw = W.diff2048_500stamps()
     wl = dw.wl
     st = wl.stamps
     c = dict(radius=wl.radii[0], core=wl.cores[0].astype(np.float64),
@@ -171,6 +171,6 @@ def test_cfg4_full_size_against_oracle(hip, oracle):
     Rr, _ = oracle_gradient(oracle, c, np.zeros_like(c["R0"]))
     gR = 2.0 * (Rr - c["T"])
     R, g = gpu_run(c, gR)
-    assert_close("grad.R/cfg4", R, Rr)
+    assert_close("grad.R/diff2048", R, Rr)
     _, ref = oracle_gradient(oracle, c, gR)
-    check_grad("cfg4", g, ref)
+    check_grad("diff2048", g, ref)

@@ -46,18 +46,18 @@ def test_seeded_strokes(hip, oracle, seed, radius, noisy, lmode):
     compare_canvas("%s/" % lmode + "strokes/s%d/r%d" % (seed, radius), gcv, ocv, pickup=(gb, ob))
 
 
-def test_cfg1_pickup(hip, oracle, lmode):
-    wl = W.cfg1(pickup=True)
+def test_canvas1024_pickup(hip, oracle, lmode):
+    wl = W.canvas1024_200stamps(pickup=True)
     ocv, ob = run_oracle(wl)
     gcv, gb = run_gpu(wl, mode=lmode)
-    compare_canvas("%s/" % lmode + "cfg1", gcv, ocv, pickup=(gb, ob))
+    compare_canvas("%s/" % lmode + "canvas1024", gcv, ocv, pickup=(gb, ob))
 
 
 def test_multi_brush_with_drying(hip, oracle, lmode):
-    wl = W.cfg2(seed=4, size=512, n_stamps=150, dry_every=50, foot=(41, 129))
+    wl = W.canvas4096_2000stamps(seed=4, size=512, n_stamps=150, dry_every=50, foot=(41, 129))
     ocv, ob = run_oracle(wl)
     gcv, gb = run_gpu(wl, mode=lmode)
-    compare_canvas("%s/" % lmode + "cfg2-small", gcv, ocv, pickup=(gb, ob))
+    compare_canvas("%s/" % lmode + "canvas-small", gcv, ocv, pickup=(gb, ob))
 
 
 def _pair(rows, cols, radius, seed):
@@ -166,7 +166,7 @@ def test_fused_concurrent_strokes_equal_one_chain(hip, oracle, chains, sched, mo
     stream, or on concurrent streams), each with its own pickup map: the
     result equals the one-chain run to the bit (and the oracle), with drying
     between phases and the last stroke's pickup map left in the brush."""
-    wl = W.cfg2(seed=7, size=1024, n_stamps=400, dry_every=130, foot=(17, 41))
+    wl = W.canvas4096_2000stamps(seed=7, size=1024, n_stamps=400, dry_every=130, foot=(17, 41))
     monkeypatch.setenv("PAINTY_FUSED_SCHED", sched)
     monkeypatch.setenv("PAINTY_FUSED_CHAINS", "1")
     a, ab = run_gpu(wl, mode=FUSED)
@@ -215,7 +215,7 @@ def test_imprint_strokes_equals_stroke_by_stroke(hip, oracle, chains, sched, mon
     """pnt_imprint_strokes (one call, per-stroke footprints, concurrent
     strokes) equals setRadius + imprintStamps stroke by stroke, to the bit,
     and the oracle."""
-    wl = W.cfg2(seed=8, size=1024, n_stamps=400, dry_every=130, foot=(17, 41))
+    wl = W.canvas4096_2000stamps(seed=8, size=1024, n_stamps=400, dry_every=130, foot=(17, 41))
     monkeypatch.setenv("PAINTY_FUSED_SCHED", sched)
     monkeypatch.setenv("PAINTY_FUSED_CHAINS", str(chains))
     a, ab = run_gpu(wl, mode=FUSED)

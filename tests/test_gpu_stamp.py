@@ -70,23 +70,23 @@ def test_seeded_strokes(hip, oracle, seed, radius, noisy, pickup):
     compare_canvas("strokes/s%d/r%d/p%d" % (seed, radius, pickup), gcv, ocv, pickup=(gb, ob))
 
 
-def test_cfg1_deposit_and_pickup(hip, oracle):
+def test_canvas1024_deposit_and_pickup(hip, oracle):
     """1024x1024, 200 stamps of the 65x65 Gaussian (sigma 16): the headline
     deposit-only case and the pickup-on variant."""
     for pickup in (False, True):
-        wl = W.cfg1(pickup=pickup)
+        wl = W.canvas1024_200stamps(pickup=pickup)
         ocv, ob = run_oracle(wl)
         gcv, gb = run_gpu(wl)
-        compare_canvas("cfg1/pickup%d" % pickup, gcv, ocv, pickup=(gb, ob))
+        compare_canvas("canvas1024/pickup%d" % pickup, gcv, ocv, pickup=(gb, ob))
 
 
 def test_multi_brush_with_drying(hip, oracle):
     """Per-stroke footprint sizes through setRadius, noisy footprints, pickup
     on, dryCanvas after every 40th stamp (a reduced large-canvas config)."""
-    wl = W.cfg2(seed=4, size=384, n_stamps=120, dry_every=40, foot=(21, 61))
+    wl = W.canvas4096_2000stamps(seed=4, size=384, n_stamps=120, dry_every=40, foot=(21, 61))
     ocv, ob = run_oracle(wl)
     gcv, gb = run_gpu(wl)
-    compare_canvas("cfg2-small", gcv, ocv, pickup=(gb, ob))
+    compare_canvas("canvas-small", gcv, ocv, pickup=(gb, ob))
 
 
 def test_update_snapshot_and_clean(hip, oracle):
