@@ -741,60 +741,100 @@ __device__ double gl_sample(const GlWarpArgs& A, double s, double t) {
          (at(yb, xa) * (1.0 - ax) + at(yb, xb) * ax) * ay;
 }
 
-// One thread per pixel of the strip's box: the last triangle covering the
-// pixel centre gives the fragment (BrushTextureWarp.frag: texture().r).
-__global__ __launch_bounds__(kBlock) void k_gl_warp(GlWarpArgs A) {
+// A strip triangle's pixel box and the offset of its pixels in the flattened
+// claim launch.
+struct GlBox {
+  int tx0, ty0, tw, th;
+  long long off;
+};
+
+// The triangle j of the strip (vertices j, j+1, j+2), counter-clockwise in
+// window coordinates (y up); false when it has no area.
+__device__ __forceinline__ bool gl_tri(const GlTri* v, int j, long long X[3], long long Y[3],
+                                       int id[3], long long* area) {
+  id[0] = j;
+  id[1] = j + 1;
+  id[2] = j + 2;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    X[q] = v[id[q]].X;
+    Y[q] = v[id[q]].Y;
+  }
+  long long a = gl_cross(X[0], Y[0], X[1], Y[1], X[2], Y[2]);
+  if (a == 0) return false;
+  if (a < 0) {
+    long long tx = X[1], ty = Y[1];
+    X[1] = X[2];
+    Y[1] = Y[2];
+    X[2] = tx;
+    Y[2] = ty;
+    const int ti = id[1];
+    id[1] = id[2];
+    id[2] = ti;
+    a = -a;
+  }
+  *area = a;
+  return true;
+}
+
+// Edge functions of a pixel centre against triangle (X, Y); covered by the
+// top-left rule.
+__device__ __forceinline__ bool gl_covers(const long long X[3], const long long Y[3], long long cx,
+                                          long long cy, long long e[3]) {
+  e[0] = gl_cross(X[1], Y[1], X[2], Y[2], cx, cy);
+  e[1] = gl_cross(X[2], Y[2], X[0], Y[0], cx, cy);
+  e[2] = gl_cross(X[0], Y[0], X[1], Y[1], cx, cy);
+  const bool in0 = e[0] > 0 || (e[0] == 0 && gl_top_left(X[1], Y[1], X[2], Y[2]));
+  const bool in1 = e[1] > 0 || (e[1] == 0 && gl_top_left(X[2], Y[2], X[0], Y[0]));
+  const bool in2 = e[2] > 0 || (e[2] == 0 && gl_top_left(X[0], Y[0], X[1], Y[1]));
+  return in0 && in1 && in2;
+}
+
+// First pass: one thread per pixel of every triangle's box; a covered pixel
+// is claimed by the latest triangle of the strip that covers it (draw order:
+// later triangles overwrite earlier ones).
+__global__ __launch_bounds__(kBlock) void k_gl_claim(const GlTri* v, const GlBox* bx, int ntri,
+                                                     long long total, int cols, int* claim) {
+  const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= total) return;
+  int lo = 0, hi = ntri - 1;  // the triangle whose pixel range holds e
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (bx[mid].off <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const GlBox B = bx[lo];
+  const long long k = e - B.off;
+  if (k >= (long long)B.tw * B.th) return;
+  const int px = B.tx0 + (int)(k % B.tw), py = B.ty0 + (int)(k / B.tw);
+  long long X[3], Y[3], area, ed[3];
+  int id[3];
+  if (!gl_tri(v, lo, X, Y, id, &area)) return;
+  if (gl_covers(X, Y, (long long)px * 256 + 128, (long long)py * 256 + 128, ed))
+    atomicMax(claim + (int64_t)py * cols + px, lo + 1);
+}
+
+// Second pass: one thread per pixel of the strip's box; the claiming
+// triangle's texture coordinates at the pixel centre, sampled
+// (BrushTextureWarp.frag: texture().r).  Clears the claim for the next
+// stroke.
+__global__ __launch_bounds__(kBlock) void k_gl_shade(GlWarpArgs A, int* claim) {
   const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (e >= (int64_t)A.pw * A.ph) return;
   const int px = A.px0 + (int)(e % A.pw), py = A.py0 + (int)(e / A.pw);
-  const long long cx = (long long)px * 256 + 128, cy = (long long)py * 256 + 128;
-  int hit = -1;
-  long long he0 = 0, he1 = 0, he2 = 0, harea = 1;
-  int hid[3] = {0, 0, 0};
-  for (int j = 0; j + 2 < A.nv; ++j) {
-    int id[3] = {j, j + 1, j + 2};
-    long long X[3], Y[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      X[q] = A.v[id[q]].X;
-      Y[q] = A.v[id[q]].Y;
-    }
-    long long area = gl_cross(X[0], Y[0], X[1], Y[1], X[2], Y[2]);
-    if (area == 0) continue;
-    if (area < 0) {
-      long long tx = X[1], ty = Y[1];
-      X[1] = X[2];
-      Y[1] = Y[2];
-      X[2] = tx;
-      Y[2] = ty;
-      const int ti = id[1];
-      id[1] = id[2];
-      id[2] = ti;
-      area = -area;
-    }
-    const long long e0 = gl_cross(X[1], Y[1], X[2], Y[2], cx, cy);
-    const long long e1 = gl_cross(X[2], Y[2], X[0], Y[0], cx, cy);
-    const long long e2 = gl_cross(X[0], Y[0], X[1], Y[1], cx, cy);
-    const bool in0 = e0 > 0 || (e0 == 0 && gl_top_left(X[1], Y[1], X[2], Y[2]));
-    const bool in1 = e1 > 0 || (e1 == 0 && gl_top_left(X[2], Y[2], X[0], Y[0]));
-    const bool in2 = e2 > 0 || (e2 == 0 && gl_top_left(X[0], Y[0], X[1], Y[1]));
-    if (in0 && in1 && in2) {
-      hit = j;
-      he0 = e0;
-      he1 = e1;
-      he2 = e2;
-      harea = area;
-      hid[0] = id[0];
-      hid[1] = id[1];
-      hid[2] = id[2];
-    }
-  }
-  if (hit < 0) return;
-  const double w0 = (double)he0 / (double)harea, w1 = (double)he1 / (double)harea,
-               w2 = (double)he2 / (double)harea;
-  const double s = w0 * A.v[hid[0]].s + w1 * A.v[hid[1]].s + w2 * A.v[hid[2]].s;
-  const double t = w0 * A.v[hid[0]].t + w1 * A.v[hid[1]].t + w2 * A.v[hid[2]].t;
-  A.warp[(int64_t)py * A.cols + px] = (float)gl_sample(A, (double)(float)s, (double)(float)t);
+  const int64_t ip = (int64_t)py * A.cols + px;
+  const int c = claim[ip];
+  if (c <= 0) return;
+  claim[ip] = 0;
+  long long X[3], Y[3], area, ed[3];
+  int id[3];
+  gl_tri(A.v, c - 1, X, Y, id, &area);
+  gl_covers(X, Y, (long long)px * 256 + 128, (long long)py * 256 + 128, ed);
+  const double w0 = (double)ed[0] / (double)area, w1 = (double)ed[1] / (double)area,
+               w2 = (double)ed[2] / (double)area;
+  const double s = w0 * A.v[id[0]].s + w1 * A.v[id[1]].s + w2 * A.v[id[2]].s;
+  const double t = w0 * A.v[id[0]].t + w1 * A.v[id[1]].t + w2 * A.v[id[2]].t;
+  A.warp[ip] = (float)gl_sample(A, (double)(float)s, (double)(float)t);
 }
 
 struct GlStep {
@@ -936,9 +976,12 @@ struct pnt_tbgl_s {
   float* smS = nullptr;
   float* warp = nullptr;  // canvas-sized, GL rows
   int w_rows = 0, w_cols = 0;
+  int* claim = nullptr;    // canvas-sized, zero between strokes
   // per-stroke scratch
   GlTri* verts = nullptr;
   int64_t verts_cap = 0;
+  GlBox* boxes = nullptr;
+  int64_t boxes_cap = 0;
   unsigned long long* owner = nullptr;
   float* cand = nullptr;
   int64_t sm_cap = 0;  // cells for owner / invocations for cand
@@ -1007,7 +1050,8 @@ int pnt_tbgl_destroy(pnt_tbgl b) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (void* p : {(void*)b->tex, (void*)b->smK, (void*)b->smS, (void*)b->warp,
-                    (void*)b->verts, (void*)b->owner, (void*)b->cand})
+                    (void*)b->claim, (void*)b->verts, (void*)b->boxes, (void*)b->owner,
+                    (void*)b->cand})
       dev_free(p);
     delete b;
     ctx_release(ctx);
@@ -1126,9 +1170,13 @@ int pnt_tbgl_paint_stroke(pnt_tbgl b, pnt_canvas cv, const double* xy, int64_t n
     if (!b->warp || b->w_rows != rows || b->w_cols != cols) {  // TextureBrushGpu.cxx:170-178
       PNT_HIP_CHECK(hipStreamSynchronize(s));
       dev_free(b->warp);
+      dev_free(b->claim);
       b->warp = nullptr;
+      b->claim = nullptr;
       b->warp = dev_alloc((int64_t)rows * cols);
+      b->claim = reinterpret_cast<int*>(dev_alloc((int64_t)rows * cols));
       launch_fill(b->warp, (int64_t)rows * cols, 0.0f, s);
+      launch_fill(reinterpret_cast<float*>(b->claim), (int64_t)rows * cols, 0.0f, s);
       b->w_rows = rows;
       b->w_cols = cols;
     }
@@ -1184,10 +1232,48 @@ int pnt_tbgl_paint_stroke(pnt_tbgl b, pnt_canvas cv, const double* xy, int64_t n
       b->verts_cap = nv;
     }
     PNT_HIP_CHECK(hipMemcpyAsync(b->verts, tri.data(), nv * sizeof(GlTri), hipMemcpyHostToDevice, s));
-    if (px1 >= px0 && py1 >= py0) {
+    // each triangle's pixel box (the centres it may cover), flattened
+    const int ntri = nv - 2;
+    std::vector<GlBox> bx((size_t)std::max(ntri, 1));
+    long long total = 0;
+    for (int j = 0; j < ntri; ++j) {
+      long long x0 = LLONG_MAX, x1 = LLONG_MIN, y0 = LLONG_MAX, y1 = LLONG_MIN;
+      for (int q = 0; q < 3; ++q) {
+        x0 = std::min(x0, tri[(size_t)(j + q)].X);
+        x1 = std::max(x1, tri[(size_t)(j + q)].X);
+        y0 = std::min(y0, tri[(size_t)(j + q)].Y);
+        y1 = std::max(y1, tri[(size_t)(j + q)].Y);
+      }
+      const long long tx0 = std::max<long long>(0, (x0 - 128) >> 8);
+      const long long tx1 = std::min<long long>(cols - 1, ((x1 - 128) >> 8) + 1);
+      const long long ty0 = std::max<long long>(0, (y0 - 128) >> 8);
+      const long long ty1 = std::min<long long>(rows - 1, ((y1 - 128) >> 8) + 1);
+      GlBox& B = bx[(size_t)j];
+      B.tx0 = (int)std::min<long long>(tx0, cols - 1);
+      B.ty0 = (int)std::min<long long>(ty0, rows - 1);
+      B.tw = tx1 >= tx0 ? (int)(tx1 - tx0 + 1) : 0;
+      B.th = ty1 >= ty0 ? (int)(ty1 - ty0 + 1) : 0;
+      B.off = total;
+      total += (long long)B.tw * B.th;
+    }
+    PNT_REQUIRE(total < ((long long)1 << 40), "stroke too large");
+    if (ntri > b->boxes_cap) {
+      PNT_HIP_CHECK(hipStreamSynchronize(s));
+      dev_free(b->boxes);
+      b->boxes = nullptr;
+      b->boxes = reinterpret_cast<GlBox*>(dev_alloc((int64_t)ntri * (int64_t)(sizeof(GlBox) / 4)));
+      b->boxes_cap = ntri;
+    }
+    PNT_HIP_CHECK(hipMemcpyAsync(b->boxes, bx.data(), ntri * sizeof(GlBox), hipMemcpyHostToDevice,
+                                 s));
+    if (total > 0 && px1 >= px0 && py1 >= py0) {
+      k_gl_claim<<<(unsigned)((total + kBlock - 1) / kBlock), kBlock, 0, s>>>(b->verts, b->boxes,
+                                                                              ntri, total, cols,
+                                                                              b->claim);
+      PNT_HIP_CHECK(hipGetLastError());
       GlWarpArgs W{b->verts, nv, px0, py0, px1 - px0 + 1, py1 - py0 + 1, cols, b->tex,
                    b->t_rows, b->t_cols, b->warp};
-      k_gl_warp<<<blocks_for((int64_t)W.pw * W.ph), kBlock, 0, s>>>(W);
+      k_gl_shade<<<blocks_for((int64_t)W.pw * W.ph), kBlock, 0, s>>>(W, b->claim);
       PNT_HIP_CHECK(hipGetLastError());
     }
     if (b->smudge) {  // TextureBrushGpu::smudge (TextureBrushGpu.cxx:253-305)
