@@ -91,3 +91,35 @@ def test_no_texture_is_an_error(hip):
     gb.paintStroke(np.array([[3.0, 3.0]]), gc)  # one vertex: nothing
     with pytest.raises(_lib.InvalidArgument):
         gb.paintStroke(np.array([[2.0, 2.0], [9.0, 9.0]]), gc)
+
+
+def test_reference_gputest_smudge_strokes(hip, oracle):
+    """The reference's own GL case (renderer/test/src/GpuTest.cxx:29-60): a
+    1024 x 768 canvas, smudging on, radius 40, a horizontal stroke
+    (50, 400) -> (800, 400) of the paint ComputeScatteringAndAbsorption
+    makes from (0.01, 0.02) per channel, then a vertical stroke
+    (500, 50) -> (500, 800) of the paint from (0.8, 0.9).  The dictionary's
+    random texture is replaced by the reference's brush stroke sample."""
+    rows, cols = 768, 1024
+    Ka, Sa = oracle.compute_scattering_absorption([0.01] * 3, [0.02] * 3)
+    Kb, Sb = oracle.compute_scattering_absorption([0.8] * 3, [0.9] * 3)
+    gc = hip.Canvas(rows, cols)
+    gc.clear()
+    oc = oracle.GlCanvas(np.zeros((rows, cols, 3)), np.zeros((rows, cols, 3)),
+                         np.zeros((rows, cols)))
+    gb, ob = hip.TextureBrushGpu(), oracle.TextureBrushGpu()
+    tex = sample_thickness()
+    gb.setTexture(tex)
+    ob.set_texture(tex)
+    gb.enableSmudge(True)
+    ob.enable_smudge(True)
+    gb.setRadius(40.0)
+    ob.set_radius(40.0)
+    for (K, S), xy in [((Ka, Sa), [[50.0, 400.0], [800.0, 400.0]]),
+                       ((Kb, Sb), [[500.0, 50.0], [500.0, 800.0]])]:
+        gb.dip((K, S))
+        ob.dip(K, S)
+        gb.paintStroke(np.array(xy), gc)
+        ob.paint_stroke(np.array(xy), oc)
+    _check("gl/gputest", hip, gc, oc, gb, ob)
+    assert (oc.V > 0).sum() > 50000
