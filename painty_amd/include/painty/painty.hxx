@@ -2,6 +2,7 @@
 #pragma once
 
 #include "painty/core/KubelkaMunk.hxx"
+#include "painty/core/Spline.hxx"
 #include "painty/mixer/PaintMixer.hxx"
 #include "painty/mixer/Serialization.hxx"
 #include "painty/renderer/FootprintBrush.hxx"

@@ -8,6 +8,7 @@
 //   test_dropin km <in.txt> <out.txt>       ComputeReflectance per case
 //   test_dropin stamp <in.bin> <out.bin>    one canvas, one brush, a stamp list
 //   test_dropin api                         mixer and error behaviour
+//   test_dropin gui <in.txt> <out.bin>      the GUI's mouse-move stroke driver
 //   test_dropin footprint <r> <out> [png]   FootprintBrush(r)'s padded footprint,
 //                                           from png or the default source
 #include <chrono>
@@ -295,6 +296,16 @@ int api() {
     const auto tm = painty::getThinningMedium();
     EXPECT(tm.K[0] == 0.01 && tm.S[2] == 0.01);
   }
+  // spline helpers (core/test/src/SplineTest.cxx:14-41) and SplineEval
+  {
+    EXPECT(std::abs(painty::CatmullRom(0.0, 2.0, 4.5, 5.9, 0.6) - 3.5552) < 1e-7);
+    EXPECT(painty::Cubic(1.0, 2.0, 4.0, 7.0, 0.0) == 2.0 && painty::Cubic(1.0, 2.0, 4.0, 7.0, 1.0) == 4.0);
+    const std::vector<vec2> pts{{0.0, 0.0}, {10.0, 0.0}, {20.0, 5.0}};
+    painty::SplineEval<std::vector<vec2>::const_iterator> sp(pts.cbegin(), pts.cend());
+    EXPECT(sp.catmullRom(0.0) == pts[0] && sp.catmullRom(0.5) == pts[1]);
+    EXPECT(sp.linear(0.75)[0] == 15.0 && sp.cubic(1.0) == pts[2]);
+    EXPECT(std::abs(sp.catmullRomDerivativeFirst(0.0).normalized().norm() - 1.0) < 1e-12);
+  }
   // errors map to the reference's exception types
   EXPECT(throws_invalid([] { Canvas<vec3> bad(-1, 4); }));
   EXPECT(throws_invalid([] { FootprintBrush<vec3> b(3.0, Mat<double>(6, 6, 0.5)); }));
@@ -318,12 +329,63 @@ int footprint(double radius, const char* out, const char* png) {
   return 0;
 }
 
+// The GUI's footprint-brush stroke driver (apps/painty_gui/DigitalCanvas.cxx:
+// 98-125) over a mouse path read from `in` ("rows cols radius n" then n
+// "x y" lines): each new point imprints at every pixel of arc length between
+// the last two points, on the Catmull-Rom through the last three.  Writes
+// the stamps it placed, the padded footprint and the canvas.
+int gui(const char* in, const char* out) {
+  std::ifstream f(in);
+  int rows = 0, cols = 0, n = 0;
+  double radius = 0.0;
+  f >> rows >> cols >> radius >> n;
+  Canvas<vec3> canvas(rows, cols);
+  canvas.useClock(PNT_CLOCK_OFF);
+  FootprintBrush<vec3> brush(radius);
+  brush.dip({vec3(0.3, 0.5, 0.7), vec3(0.6, 0.2, 0.4)});
+  std::vector<vec2> path;
+  std::vector<double> placed;
+  for (int k = 0; k < n; ++k) {
+    vec2 p;
+    f >> p[0] >> p[1];
+    if (path.empty() || path.back() != p) path.push_back(p);
+    if (path.size() < 2) continue;
+    const int sz = static_cast<int>(path.size());
+    const vec2 p0 = path[static_cast<size_t>(std::max(0, sz - 3))];
+    const vec2 p1 = path[static_cast<size_t>(std::max(0, sz - 2))];
+    const vec2 p2 = path[static_cast<size_t>(std::max(0, sz - 1))];
+    const double dist = (p2 - p1).norm();
+    for (int pd = 1; pd <= static_cast<int>(dist); ++pd) {
+      const double t = static_cast<double>(pd) / dist;
+      const vec2 dir = painty::CatmullRomDerivativeFirst(p0, p1, p2, p2, t);
+      const vec2 c = painty::CatmullRom(p0, p1, p2, p2, t);
+      const double theta = std::atan2(dir[1], dir[0]);
+      brush.imprint(c, theta, canvas);
+      placed.insert(placed.end(), {c[0], c[1], theta});
+    }
+  }
+  std::ofstream o(out, std::ios::binary);
+  const int64_t ns = static_cast<int64_t>(placed.size() / 3);
+  write(o, &ns, 1);
+  write(o, placed.data(), placed.size());
+  const Mat<double>& fp = brush.getFootprint();
+  const int32_t side = fp.rows;
+  write(o, &side, 1);
+  write(o, fp.data(), fp.total());
+  const Canvas<vec3>& cc = canvas;
+  for (const vec3& v : cc.getPaintLayer().getK_buffer()) write(o, v.data(), 3);
+  for (const vec3& v : cc.getPaintLayer().getS_buffer()) write(o, v.data(), 3);
+  write(o, cc.getPaintLayer().getV_buffer().data(), static_cast<size_t>(rows) * cols);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   try {
     const std::string mode = argc > 1 ? argv[1] : "";
     if (mode == "km" && argc == 4) return km(argv[2], argv[3]);
     if (mode == "stamp" && argc == 4) return stamp(argv[2], argv[3]);
     if (mode == "api") return api();
+    if (mode == "gui" && argc == 4) return gui(argv[2], argv[3]);
     if (mode == "footprint" && (argc == 4 || argc == 5))
       return footprint(std::atof(argv[2]), argv[3], argc == 5 ? argv[4] : nullptr);
     std::fprintf(stderr, "usage: test_dropin km|stamp <in> <out> | api | footprint <r> <out> [png]\n");

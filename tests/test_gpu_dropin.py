@@ -4,6 +4,7 @@ program the way the reference's code drives its templates, checked against
 the reference's own known answers and the FP64 oracle (1e-5 relative per
 channel)."""
 import json
+import math
 import os
 import subprocess
 
@@ -96,3 +97,60 @@ def test_stamp_scenario_through_cpp(tmp_path, oracle, snapshot):
 
 def test_mixer_lifecycle_and_errors_through_cpp():
     assert "api ok" in _run("api")
+
+
+def test_gui_stroke_driver_through_cpp(tmp_path, oracle):
+    """The GUI's footprint-brush mouse handler (apps/painty_gui/
+    DigitalCanvas.cxx:98-125) compiled against the drop-in headers: every
+    pixel of arc length between the last two mouse points gets a stamp on the
+    Catmull-Rom through the last three.  The stamps equal the ones the
+    oracle's Catmull-Rom places, to the bit, and the canvas equals the
+    oracle's after the same stamps."""
+    rows, cols, radius = 96, 128, 6.0
+    pts = [(10.2, 12.7), (18.9, 15.1), (31.4, 22.8), (31.4, 22.8), (45.0, 40.6), (70.3, 44.2),
+           (96.8, 30.5), (110.1, 28.0)]
+    inp = tmp_path / "gui.txt"
+    inp.write_text("%d %d %r %d\n" % (rows, cols, radius, len(pts)) +
+                   "".join("%r %r\n" % p for p in pts))
+    out = str(tmp_path / "gui.bin")
+    _run("gui", str(inp), out)
+    raw = open(out, "rb").read()
+    ns = int(np.frombuffer(raw[:8], np.int64)[0])
+    placed = np.frombuffer(raw[8:8 + 24 * ns], np.float64).reshape(ns, 3)
+    off = 8 + 24 * ns
+    side = int(np.frombuffer(raw[off:off + 4], np.int32)[0])
+    fp = np.frombuffer(raw[off + 4:off + 4 + 8 * side * side], np.float64).reshape(side, side)
+    off += 4 + 8 * side * side
+    K = np.frombuffer(raw[off:off + 24 * rows * cols], np.float64).reshape(rows, cols, 3)
+    off += 24 * rows * cols
+    S = np.frombuffer(raw[off:off + 24 * rows * cols], np.float64).reshape(rows, cols, 3)
+    off += 24 * rows * cols
+    V = np.frombuffer(raw[off:off + 8 * rows * cols], np.float64).reshape(rows, cols)
+    # the same placement from the oracle's Catmull-Rom (SplineTest-pinned)
+    path, want = [], []
+    for p in pts:
+        if not path or path[-1] != p:
+            path.append(p)
+        if len(path) < 2:
+            continue
+        p0, p1, p2 = path[max(0, len(path) - 3)], path[max(0, len(path) - 2)], path[-1]
+        dx, dy = p2[0] - p1[0], p2[1] - p1[1]
+        dist = math.sqrt(dx * dx + dy * dy)  # as vec::norm computes it
+        for pd in range(1, int(dist) + 1):
+            t = pd / dist
+            c = [oracle.catmull_rom(p0[i], p1[i], p2[i], p2[i], t) for i in range(2)]
+            d = [oracle.catmull_rom_d1(p0[i], p1[i], p2[i], p2[i], t) for i in range(2)]
+            want.append((c[0], c[1], math.atan2(d[1], d[0])))
+    np.testing.assert_array_equal(placed, np.array(want))
+    w = 2 * int(np.ceil(radius)) + 1
+    pad = (side - w) // 2
+    ob = oracle.Brush(radius, np.ascontiguousarray(fp[pad:pad + w, pad:pad + w]))
+    ob.dip([0.3, 0.5, 0.7], [0.6, 0.2, 0.4])
+    oc = oracle.Canvas(rows, cols)
+    for cx, cy, th in want:
+        ob.imprint(cx, cy, th, oc)
+    from tests.parity import assert_close
+    assert_close("gui.K", K, oc.K)
+    assert_close("gui.S", S, oc.S)
+    assert_close("gui.V", V, oc.V)
+    assert (V > 0).sum() > 500
