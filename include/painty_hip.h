@@ -292,6 +292,13 @@ typedef struct {
 #define PNT_BATCH_ONE_LAUNCH 0
 #define PNT_BATCH_PLAIN 1
 int pnt_batch_run(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int mode);
+/* The same one-launch batch as a submit / wait pair (no reference
+ * counterpart; for callers that overlap host work with the batch): submit
+ * enqueues it on the context's stream and returns once the plan is staged
+ * (the stamp lists may then be reused); wait blocks until it is done and
+ * reports an aborted kernel as PNT_EKERNEL. */
+int pnt_batch_submit(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs);
+int pnt_batch_wait(pnt_ctx ctx);
 
 /* Shape of a one-launch batch (no reference counterpart): window tiles of
  * tile x tile pixels (16 or 32; one thread each) and at most max_group

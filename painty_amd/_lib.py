@@ -131,6 +131,8 @@ SIGNATURES = {
     "pnt_imprint_strokes": [_vp, _vp, C.POINTER(PntStroke), C.c_int64, C.c_int],
     "pnt_paint_stroke": [_vp, _vp, _dp, C.c_int64],
     "pnt_batch_run": [_vp, C.POINTER(BatchJob), C.c_int, C.c_int],
+    "pnt_batch_submit": [_vp, C.POINTER(BatchJob), C.c_int],
+    "pnt_batch_wait": [_vp],
     "pnt_batch_set_config": [_vp, C.POINTER(BatchConfig)],
     "pnt_batch_get_config": [_vp, C.POINTER(BatchConfig)],
     "pnt_batch_tune": [_vp, C.POINTER(BatchJob), C.c_int, C.POINTER(BatchConfig), _dp],

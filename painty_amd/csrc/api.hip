@@ -1546,6 +1546,12 @@ static void check_config(const pnt_batch_config* c) {
   PNT_REQUIRE(c->max_group >= 1 && c->max_group <= 256, "max_group must be in [1, 256]");
 }
 
+int pnt_batch_submit(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs) {
+  return pnt_batch_run(ctx, jobs, n_jobs, PNT_BATCH_ONE_LAUNCH);
+}
+
+int pnt_batch_wait(pnt_ctx ctx) { return pnt_ctx_synchronize(ctx); }
+
 int pnt_batch_run(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int mode) {
   return guard([&] {
     PNT_REQUIRE(mode == PNT_BATCH_ONE_LAUNCH || mode == PNT_BATCH_PLAIN, "unknown batch mode");

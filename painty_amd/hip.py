@@ -656,6 +656,17 @@ def batch_run(jobs, mode: int = _lib.BATCH_ONE_LAUNCH, ctx: Context | None = Non
     call("pnt_batch_run", ctx.h, arr, len(jobs), int(mode))
 
 
+def batch_submit(jobs, ctx: Context | None = None) -> None:
+    """pnt_batch_submit: the one-launch batch, enqueued; batch_wait joins it."""
+    ctx = ctx or default_context()
+    arr, _keep = _batch_jobs(jobs)
+    call("pnt_batch_submit", ctx.h, arr, len(jobs))
+
+
+def batch_wait(ctx: Context | None = None) -> None:
+    call("pnt_batch_wait", (ctx or default_context()).h)
+
+
 def batch_set_config(tile: int, max_group: int, ctx: Context | None = None) -> None:
     ctx = ctx or default_context()
     c = _lib.BatchConfig(tile, max_group)

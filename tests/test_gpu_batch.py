@@ -240,3 +240,19 @@ def test_objects_outlive_their_context(hip, oracle):
     del cv
     g.close()
     del br
+
+
+def test_batch_submit_then_wait(hip):
+    """pnt_batch_submit returns with the batch enqueued; after pnt_batch_wait
+    every canvas equals the same jobs run by pnt_batch_run, bit for bit."""
+    bw = W.batch256(n_canvas=12, seed=9, sides=(128, 320), stamps=(10, 50), foot=(32, 96))
+    a = [(_setup(hip, wl)) for wl in bw.jobs]
+    b = [(_setup(hip, wl)) for wl in bw.jobs]
+    hip.batch_run([(cv, br, wl.stamps, out) for (cv, br, out), wl in zip(a, bw.jobs)], mode=0)
+    hip.batch_submit([(cv, br, wl.stamps, out) for (cv, br, out), wl in zip(b, bw.jobs)])
+    hip.batch_wait()
+    for (ca, _, oa), (cb, _, ob) in zip(a, b):
+        La, Lb = ca.getPaintLayer(), cb.getPaintLayer()
+        np.testing.assert_array_equal(La.getK_buffer(), Lb.getK_buffer())
+        np.testing.assert_array_equal(La.getV_buffer(), Lb.getV_buffer())
+        np.testing.assert_array_equal(oa.getR0(), ob.getR0())
