@@ -852,17 +852,46 @@ struct GlSmudgeArgs {
   float* smS;
   int m, dw;   // pickup side; dispatch width (m rounded up to 16)
   float rc;    // rotation centre m * 0.5
-  unsigned long long* owner;  // per cell: (step << 32) | (invocation + 1)
-  float* cand;  // per invocation: 8 candidate values (K rgb V, S rgb V)
+  // Claims and candidates, two of each, by step parity.  Per cell:
+  // (step << 32) | (invocation + 1); per invocation: 8 candidate values
+  // (K rgb V, S rgb V).
+  unsigned long long* owner[2];
+  float* cand[2];
 };
 
-// Smudge.compute for one spine step, first half: the canvas update of every
-// invocation, and each invocation's candidate for its pickup cell with a
-// claim on the cell.
-__global__ __launch_bounds__(kBlock) void k_gl_smudge_a(GlSmudgeArgs A, GlStep St,
-                                                        unsigned long long step) {
+// The pickup cell ip as the step before left it, where that step claimed
+// the cell (step > 1): the winning candidate.  nullptr: the map holds it.
+__device__ __forceinline__ const float* gl_cell(const GlSmudgeArgs& A, unsigned long long step,
+                                                int64_t ip) {
+  if (step > 1) {
+    const unsigned long long w = A.owner[(step - 1) & 1][ip];
+    if ((w >> 32) == step - 1)
+      return A.cand[(step - 1) & 1] + 8 * (int64_t)((w & 0xffffffffull) - 1);
+  }
+  return nullptr;
+}
+
+// Smudge.compute for one spine step.  Each invocation updates its canvas
+// pixel and leaves a candidate for its pickup cell with a claim on the cell;
+// the highest claiming invocation wins the cell.  The winners are applied to
+// the map by the next step's launch (or k_gl_smudge_flush after the last):
+// there, threads ip < m*m copy the previous step's winners into the map while
+// every reader of a cell the previous step claimed takes the winner's
+// candidate directly, so no thread reads a map cell that another writes.
+__global__ __launch_bounds__(kBlock) void k_gl_smudge(GlSmudgeArgs A, GlStep St,
+                                                      unsigned long long step) {
   const int64_t inv = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (inv >= (int64_t)A.dw * A.dw) return;
+  if (inv < (int64_t)A.m * A.m) {  // apply the step before's winner of cell inv
+    const float* o = gl_cell(A, step, inv);
+    if (o) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        A.smK[4 * inv + c] = o[c];
+        A.smS[4 * inv + c] = o[4 + c];
+      }
+    }
+  }
   const int gx = (int)(inv % A.dw), gy = (int)(inv / A.dw);
   const int px = St.tlx + gx, pyg = St.tly + gy;
   if (px < 0 || pyg < 0 || px >= A.cols || pyg >= A.rows) return;
@@ -877,12 +906,15 @@ __global__ __launch_bounds__(kBlock) void k_gl_smudge_a(GlSmudgeArgs A, GlStep S
   const int64_t ip = inmap ? (int64_t)qy * A.m + qx : 0;
   float pK[3] = {0.0f, 0.0f, 0.0f}, pS[3] = {0.0f, 0.0f, 0.0f}, pV = 0.0f;
   if (inmap) {
+    const float* o = gl_cell(A, step, ip);
+    const float* k = o ? o : A.smK + 4 * ip;
+    const float* sv = o ? o + 4 : A.smS + 4 * ip;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      pK[c] = A.smK[4 * ip + c];
-      pS[c] = A.smS[4 * ip + c];
+      pK[c] = k[c];
+      pS[c] = sv[c];
     }
-    pV = A.smK[4 * ip + 3];
+    pV = k[3];
   }
   float cK[3], cS[3];
 #pragma unroll
@@ -895,14 +927,14 @@ __global__ __launch_bounds__(kBlock) void k_gl_smudge_a(GlSmudgeArgs A, GlStep S
   const float pVl = pV * 0.7f * vTex, pVr = pV - pVl;
   const float pVnew = pVr + cVl;
   if (pVnew > 0.0f && inmap) {
-    float* o = A.cand + 8 * inv;
+    float* o = A.cand[step & 1] + 8 * inv;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       o[c] = (pVr * pK[c] + cVl * cK[c]) / pVnew;
       o[4 + c] = (pVr * pS[c] + cVl * cS[c]) / pVnew;
     }
     o[3] = o[7] = fmaxf(pVnew, 0.0f);
-    atomicMax(A.owner + ip, (step << 32) | (unsigned long long)(inv + 1));
+    atomicMax(A.owner[step & 1] + ip, (step << 32) | (unsigned long long)(inv + 1));
   }
   const float cVnew = cVr + pVl;
   if (cVnew > 0.0f) {
@@ -916,13 +948,13 @@ __global__ __launch_bounds__(kBlock) void k_gl_smudge_a(GlSmudgeArgs A, GlStep S
   }
 }
 
-// Second half: each claimed cell takes its winner's candidate.
-__global__ __launch_bounds__(kBlock) void k_gl_smudge_b(GlSmudgeArgs A, unsigned long long step) {
+// After the last step (number step - 1): its winners into the map.
+__global__ __launch_bounds__(kBlock) void k_gl_smudge_flush(GlSmudgeArgs A,
+                                                            unsigned long long step) {
   const int64_t ip = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (ip >= (int64_t)A.m * A.m) return;
-  const unsigned long long w = A.owner[ip];
-  if ((w >> 32) != step) return;
-  const float* o = A.cand + 8 * (int64_t)((w & 0xffffffffull) - 1);
+  const float* o = gl_cell(A, step, ip);
+  if (!o) return;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     A.smK[4 * ip + c] = o[c];
@@ -1006,9 +1038,9 @@ void gl_reset_smudge(pnt_tbgl_s* b, int m) {
   launch_fill(b->smK, 4 * cells + 4, 0.0f, s);
   launch_fill(b->smS, 4 * cells + 4, 0.0f, s);
   const int dw = round16(m);
-  b->owner = reinterpret_cast<unsigned long long*>(dev_alloc(2 * cells + 2));
-  b->cand = dev_alloc(8 * (int64_t)dw * dw + 8);
-  launch_fill(reinterpret_cast<float*>(b->owner), 2 * cells + 2, 0.0f, s);
+  b->owner = reinterpret_cast<unsigned long long*>(dev_alloc(4 * cells + 4));  // two by parity
+  b->cand = dev_alloc(16 * (int64_t)dw * dw + 8);
+  launch_fill(reinterpret_cast<float*>(b->owner), 4 * cells + 4, 0.0f, s);
   b->sm_cap = cells;
 }
 void gl_set_radius(pnt_tbgl_s* b, double radius) {  // TextureBrushGpu.cxx:48-60
@@ -1297,8 +1329,10 @@ int pnt_tbgl_paint_stroke(pnt_tbgl b, pnt_canvas cv, const double* xy, int64_t n
       A.m = b->m;
       A.dw = round16(b->m);
       A.rc = (float)(b->m * 0.5);
-      A.owner = b->owner;
-      A.cand = b->cand;
+      A.owner[0] = b->owner;
+      A.owner[1] = b->owner + (int64_t)A.m * A.m;
+      A.cand[0] = b->cand;
+      A.cand[1] = b->cand + 8 * (int64_t)A.dw * A.dw;
       unsigned long long step = 0;
       int lastx = -1, lasty = -1;
       for (double u = 0.0; u <= 1.0; u += (0.25 / length)) {
@@ -1314,12 +1348,15 @@ int pnt_tbgl_paint_stroke(pnt_tbgl b, pnt_canvas cv, const double* xy, int64_t n
                   (int)((double)(int)c[1] - (double)b->m * 0.5), std::cos(th), std::sin(th)};
         ++step;
         PNT_REQUIRE(step < (1ull << 31), "too many smudge steps");
-        k_gl_smudge_a<<<blocks_for((int64_t)A.dw * A.dw), kBlock, 0, s>>>(A, St, step);
-        k_gl_smudge_b<<<blocks_for((int64_t)A.m * A.m), kBlock, 0, s>>>(A, step);
+        k_gl_smudge<<<blocks_for((int64_t)A.dw * A.dw), kBlock, 0, s>>>(A, St, step);
+        PNT_HIP_CHECK(hipGetLastError());
+      }
+      if (step > 0) {
+        k_gl_smudge_flush<<<blocks_for((int64_t)A.m * A.m), kBlock, 0, s>>>(A, step + 1);
         PNT_HIP_CHECK(hipGetLastError());
       }
       // the claims carry the step number; start the next stroke from zero
-      launch_fill(reinterpret_cast<float*>(b->owner), 2 * (int64_t)b->m * b->m + 2, 0.0f, s);
+      launch_fill(reinterpret_cast<float*>(b->owner), 4 * (int64_t)b->m * b->m + 4, 0.0f, s);
     }
     const int aw = (int)(bmax[0] - bmin[0]) + 1, ah = (int)(bmax[1] - bmin[1]) + 1;
     if (aw > 0 && ah > 0) {
