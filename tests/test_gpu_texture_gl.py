@@ -84,6 +84,24 @@ def test_radius_change_and_middle_rows(hip, oracle):
     assert gb.getSmudge()[0].shape[0] == 20
 
 
+def test_short_smudge_strokes_in_a_row(hip, oracle):
+    """Strokes of one smudge step (all spine samples in one integer cell),
+    two steps, and a few, back to back: each step's claims are applied by the
+    next step's launch or by the flush after the last, and every stroke
+    starts from cleared claims."""
+    gc, oc, gb, ob = _pair(hip, oracle, 64, 80, 11, 6.1, sample_thickness(), True)
+    strokes = [np.array([[20.2, 30.3], [20.7, 30.6]]),   # one step
+               np.array([[20.6, 30.3], [21.4, 30.3]]),   # two
+               np.array([[21.3, 30.2], [20.8, 31.7]]),   # two, the other way
+               np.array([[18.1, 28.4], [23.9, 33.2]]),   # a few
+               np.array([[20.5, 30.5], [20.6, 30.5]])]   # one again
+    for k, xy in enumerate(strokes):
+        gb.paintStroke(xy, gc)
+        ob.paint_stroke(xy, oc)
+        _check("gl/short/stroke%d" % k, hip, gc, oc, gb, ob)
+    assert np.asarray(gb.getSmudge()[0])[..., 3].max() > 0  # the map took paint up
+
+
 def test_no_texture_is_an_error(hip):
     from painty_amd import _lib
     gb = hip.TextureBrushGpu()
