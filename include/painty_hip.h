@@ -300,13 +300,19 @@ int pnt_batch_run(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int mode);
 int pnt_batch_submit(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs);
 int pnt_batch_wait(pnt_ctx ctx);
 
-/* Shape of a one-launch batch (no reference counterpart): window tiles of
- * tile x tile pixels (16 or 32; one thread each) and at most max_group
- * workgroups on one job's stamps.  A context starts with {32, 16}. */
+/* How a batch runs (no reference counterpart).  waves == 0: one persistent
+ * cooperative launch, window tiles of tile x tile pixels (16 or 32; one
+ * thread each) and at most max_group workgroups on one job's stamps.
+ * waves == 1: the batch planned on the host into a sequence of launches
+ * ("waves"), each carrying the next step of every ready stroke of every
+ * canvas, with at most `chunk` strokes (1, 2 or 4) of one canvas in flight
+ * at once (each with its own pickup map); 16 x 16 tiles.  A context starts
+ * with {32, 16, 0, 1}. */
 typedef struct {
   int tile;
   int max_group;
-  int reserved[2];
+  int waves;
+  int chunk;
 } pnt_batch_config;
 int pnt_batch_set_config(pnt_ctx ctx, const pnt_batch_config* cfg);
 int pnt_batch_get_config(pnt_ctx ctx, pnt_batch_config* cfg);
@@ -317,7 +323,8 @@ int pnt_batch_get_config(pnt_ctx ctx, pnt_batch_config* cfg);
 int pnt_batch_tune(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs,
                    pnt_batch_config* best, double* best_ms);
 /* The batch planned once and captured as a HIP graph (plan copy, counter
- * clear, k_batch); each launch replays it on the jobs' current state -- the
+ * clear, k_batch -- or, in the waves configuration, the plan copy and every
+ * wave); each launch replays it on the jobs' current state -- the
  * same result as pnt_batch_run(PNT_BATCH_ONE_LAUNCH) with the same jobs.
  * Every stamp list must start with PNT_STAMP_DIP; a canvas or brush changed
  * outside the graph since its last launch makes the next launch fail
@@ -351,6 +358,14 @@ int pnt_brush_stamp_traffic(pnt_brush brush, int rows, int cols,
  * launches (k_stamp_fused, k_stamp_levels, k_stamp_wave) the brush's last
  * PNT_EXEC_FUSED stamp-list call issued; 0 after other modes. */
 int pnt_brush_last_launches(pnt_brush brush, int64_t* launches);
+/* Per-launch kernel timing of a brush's stamp-list calls (measurement; no
+ * counterpart in the reference, which has no device kernels): while on, each
+ * stamp-kernel launch of the next pnt_imprint_stamps / pnt_imprint_strokes
+ * call on the brush's own stream is bracketed by a pair of HIP events.
+ * pnt_brush_kernel_time waits for that call and gives the sum of the
+ * bracketed kernels' durations (ms) and how many were timed. */
+int pnt_brush_set_kernel_timing(pnt_brush brush, int on);
+int pnt_brush_kernel_time(pnt_brush brush, double* ms, int64_t* launches);
 
 /* --------------------------------------------------------- PaintMixer --- */
 /* mixSinglePaint(weights) -- mixer/src/PaintMixer.cxx:327-355: K = sum_l
@@ -442,17 +457,22 @@ int pnt_grad_backward(pnt_grad g, pnt_dev_rgb gR, pnt_dev_layer gwet,
 
 /* ------------------------------------------------------------- server -- */
 /* No reference counterpart (the reference paints on one canvas in one GUI
- * thread).  A request server for stroke + composite work on canvases kept
- * resident on one device: n_slots worker slots (a host thread, a HIP stream
- * and a brush each) run admitted requests concurrently; requests on one
- * canvas run in submission order; a request whose canvas is not resident is
- * admitted when it fits hbm_budget_bytes (canvas planes + composite planes,
- * 56 B/px), evicting idle canvases least recently used first; an evicted
- * canvas' state is dropped and later requests on its id fail with
- * PNT_ESTATE.  Each request
- * runs as a fresh FootprintBrush (FootprintBrush.hxx:29-63): its stamps
- * (PNT_EXEC_FUSED), then Renderer::compose (Renderer.hxx:48-53) into the
- * canvas' resident composite planes, copied to `out` when it is not NULL. */
+ * thread; its GPU renderer queues tasks FIFO, gpu/GpuTaskQueue.hxx:32-46).
+ * A request server for stroke + composite work on canvases kept resident on
+ * one device, with continuous batching: up to n_slots admitted requests run
+ * at once, and their stamps share device launches ("waves": one launch
+ * carries the next phase of every ready stroke of every admitted request);
+ * requests that arrive meanwhile join the next wave.  Requests on one canvas
+ * run in submission order, one at a time; a request whose canvas is not
+ * resident is admitted when it fits hbm_budget_bytes (canvas planes +
+ * composite planes, 56 B/px), spilling idle canvases to host memory, least
+ * recently used first; a later request on a spilled canvas restores it.
+ * Each request runs as a fresh FootprintBrush (FootprintBrush.hxx:29-63):
+ * its stamps (with any PNT_STAMP_DRY_AFTER), then Renderer::compose
+ * (Renderer.hxx:48-53) into the canvas' resident composite planes, copied
+ * to `out` when it is not NULL.  Each request's result is bit for bit the
+ * same as running it alone through pnt_imprint_stamps(PNT_EXEC_FUSED) with
+ * a fresh brush, then pnt_compose. */
 typedef struct pnt_server_s* pnt_server;
 typedef struct {
   int64_t canvas_id;       /* resident canvas key; a new id creates a canvas */

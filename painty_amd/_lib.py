@@ -47,7 +47,7 @@ class BatchJob(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("tile", C.c_int), ("max_group", C.c_int), ("reserved", C.c_int * 2)]
+    _fields_ = [("tile", C.c_int), ("max_group", C.c_int), ("waves", C.c_int), ("chunk", C.c_int)]
 
 
 class PntRequest(C.Structure):
@@ -140,6 +140,8 @@ SIGNATURES = {
     "pnt_batch_graph_launch": [_vp],
     "pnt_batch_graph_destroy": [_vp],
     "pnt_brush_last_launches": [_vp, C.POINTER(C.c_int64)],
+    "pnt_brush_set_kernel_timing": [_vp, C.c_int],
+    "pnt_brush_kernel_time": [_vp, _dp, C.POINTER(C.c_int64)],
     "pnt_canvas_set_dry_timer": [_vp, C.c_int64, C.c_float],
     "pnt_brush_stamp_traffic": [_vp, C.c_int, C.c_int, C.POINTER(PntStamp), C.c_int64,
                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
@@ -226,3 +228,67 @@ def check(status: int) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(load(), name)(*args))
+
+
+# ------------------------------------------------------------- lifetimes --
+# Objects that own a library handle release it exactly once: through their
+# close(), through the garbage collector, or -- for whatever is still alive
+# when the interpreter exits -- through shutdown(), which runs first among
+# the exit handlers and releases in dependency order: captured batches,
+# servers and gradient runners (they hold brushes and canvases), texture
+# brushes, brushes, canvases, layers, and the contexts last.  Nothing is
+# left for the finalizers of a half-torn-down interpreter.
+import atexit  # noqa: E402
+import threading  # noqa: E402
+import weakref  # noqa: E402
+
+_ORDER = ("graph", "server", "grad", "tbrush", "brush", "canvas", "layer", "ctx")
+_live = {k: weakref.WeakSet() for k in _ORDER}
+_live_lock = threading.Lock()
+_shut = False
+
+
+class Owned:
+    """Base of the Python objects that own a handle: subclasses set `_kind`
+    (one of _ORDER) and `_destroy` (the pnt_*_destroy name) and call
+    `_own(handle)` once the handle exists."""
+
+    _kind = ""
+    _destroy = ""
+    h = None
+
+    def _own(self, h) -> None:
+        self.h = h
+        with _live_lock:
+            _live[self._kind].add(self)
+
+    def close(self) -> None:
+        h, self.h = self.h, None
+        if h and _lib is not None:
+            status = getattr(_lib, self._destroy)(h)
+            if status != OK and not _shut:
+                check(status)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def shutdown() -> None:
+    """Releases every handle still owned by a Python object, in dependency
+    order (registered with atexit; safe to call more than once)."""
+    global _shut
+    with _live_lock:
+        groups = [list(_live[k]) for k in _ORDER]
+    for objs in groups:
+        for o in objs:
+            try:
+                o.close()
+            except Exception:
+                pass
+    _shut = True
+
+
+atexit.register(shutdown)

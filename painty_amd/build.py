@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libpainty_hip.so")
-SOURCES = ["kernels.hip", "persistent.hip", "fused.hip", "batch.hip", "grad.hip", "serve.hip", "display.hip", "ingest.hip", "texture.hip", "image.hip", "api.hip"]
+SOURCES = ["kernels.hip", "persistent.hip", "fused.hip", "batch.hip", "batchwave.hip", "grad.hip", "serve.hip", "display.hip", "ingest.hip", "texture.hip", "image.hip", "api.hip"]
 ARCH = os.environ.get("PAINTY_HIP_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

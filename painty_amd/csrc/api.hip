@@ -788,7 +788,7 @@ static void canvas_clear(pnt_canvas cv) {
 namespace pnt {
 // A canvas on a caller-provided buffer of at least 11*round_up(rows*cols, 64)
 // floats, which pnt_canvas_destroy leaves alone.
-pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base) {
+pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base, bool clear) {
   PNT_REQUIRE(ctx && base, "NULL argument");
   PNT_REQUIRE(rows > 0 && cols > 0 && (int64_t)rows * cols < ((int64_t)1 << 31),
               "rows/cols out of range");
@@ -810,7 +810,7 @@ pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base) {
   cv->view.owner = false;
   cv->view.parent = cv;
   try {
-    canvas_clear(cv);
+    if (clear) canvas_clear(cv);
   } catch (...) {
     delete cv;
     throw;
@@ -1278,6 +1278,7 @@ int pnt_brush_destroy(pnt_brush b) {
     persistent_release(b->pers);
     chain_release(b->chain);
     fused_release(b->fused);
+    ktimer_release(b->ktimer);
     dev_free(b->fp);
     dev_free(b->pick);
     dev_free(b->snap);
@@ -1471,6 +1472,24 @@ int pnt_brush_last_launches(pnt_brush b, int64_t* launches) {
   });
 }
 
+int pnt_brush_set_kernel_timing(pnt_brush b, int on) {
+  return guard([&] {
+    PNT_REQUIRE(b, "NULL brush");
+    if (!b->ktimer) b->ktimer = new KernelTimer;
+    b->ktimer->on = on != 0;
+    b->ktimer->used = 0;
+  });
+}
+
+int pnt_brush_kernel_time(pnt_brush b, double* ms, int64_t* launches) {
+  return guard([&] {
+    PNT_REQUIRE(b && ms && launches, "NULL argument");
+    b->ctx->use();
+    *ms = b->ktimer ? ktimer_total_ms(*b->ktimer) : 0.0;
+    *launches = b->ktimer ? (int64_t)(b->ktimer->used / 2) : 0;
+  });
+}
+
 int pnt_brush_stamp_traffic(pnt_brush b, int rows, int cols, const pnt_stamp* st, int64_t n,
                             int64_t* visits, int64_t* ring) {
   return guard([&] {
@@ -1544,6 +1563,8 @@ static void check_config(const pnt_batch_config* c) {
   PNT_REQUIRE(c, "NULL configuration");
   PNT_REQUIRE(c->tile == 16 || c->tile == 32, "tile must be 16 or 32");
   PNT_REQUIRE(c->max_group >= 1 && c->max_group <= 256, "max_group must be in [1, 256]");
+  PNT_REQUIRE(c->waves == 0 || c->waves == 1, "waves must be 0 or 1");
+  PNT_REQUIRE(c->chunk == 1 || c->chunk == 2 || c->chunk == 4, "chunk must be 1, 2 or 4");
 }
 
 int pnt_batch_submit(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs) {

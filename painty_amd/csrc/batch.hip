@@ -36,6 +36,7 @@
 #include "km.hpp"
 #include "persistent.hpp"
 #include "stampcore.hpp"
+#include "wave.hpp"
 
 namespace pnt {
 
@@ -302,15 +303,37 @@ __global__ __launch_bounds__(kWT* kWT) void k_batch(BArgs A) {
 // ------------------------------------------------------------------- host --
 struct BatchState {
   int max_blocks16 = 0, max_blocks32 = 0;
-  pnt_batch_config cfg{32, 16, {0, 0}};  // the configuration runs use (pnt_batch_tune sets it)
+  pnt_batch_config cfg{32, 16, 0, 1};  // the configuration runs use (pnt_batch_tune sets it)
+  float* pool = nullptr;  // pickup maps of the waves' extra stroke chains
+  int64_t pool_cap = 0;   // floats
   char* stage = nullptr;
   int64_t stage_cap = 0;
   char* dev = nullptr;
   int64_t dev_cap = 0;
   hipEvent_t uploaded = nullptr;
+  hipEvent_t finished = nullptr;  // the last batch on the context's stream
 };
 
 static BatchState* batch_state(pnt_ctx ctx);
+
+// A batch runs on the context's stream; a job whose canvas or brush belongs
+// to another context (same device) is ordered after the batch on that
+// context's stream too, so its next call there sees the batch's results.
+static void order_job_streams(pnt_ctx ctx, const pnt_batch_job* jobs, int n) {
+  BatchState& B = *batch_state(ctx);
+  bool recorded = false;
+  std::vector<hipStream_t> done;
+  for (int j = 0; j < n; ++j)
+    for (hipStream_t q : {jobs[j].canvas->ctx->stream, jobs[j].brush->ctx->stream}) {
+      if (q == ctx->stream || std::find(done.begin(), done.end(), q) != done.end()) continue;
+      if (!recorded) {
+        PNT_HIP_CHECK(hipEventRecord(B.finished, ctx->stream));
+        recorded = true;
+      }
+      PNT_HIP_CHECK(hipStreamWaitEvent(q, B.finished, 0));
+      done.push_back(q);
+    }
+}
 
 // cost of one stamp for a group of g blocks, in tile-iteration units
 static double stamp_cost(int hr, int g, int tile) {
@@ -631,13 +654,59 @@ static void finish_jobs(const pnt_batch_job* jobs, int n) {
   }
 }
 
+// Grows a device buffer of the batch state (after the stream's earlier work).
+static void grow_dev(char*& p, int64_t& cap, int64_t need, hipStream_t s) {
+  if (need <= cap) return;
+  PNT_HIP_CHECK(hipStreamSynchronize(s));
+  if (p) PNT_HIP_CHECK(hipFree(p));
+  p = nullptr;
+  cap = std::max<int64_t>(need, 2 * cap);
+  PNT_HIP_CHECK(hipMalloc((void**)&p, (size_t)cap));
+}
+
+static void grow_stage(char*& p, int64_t& cap, int64_t need) {
+  if (need <= cap) return;
+  if (p) PNT_HIP_CHECK(hipHostFree(p));
+  p = nullptr;
+  cap = std::max<int64_t>(need, 2 * cap);
+  PNT_HIP_CHECK(hipHostMalloc((void**)&p, (size_t)cap));
+}
+
+// The batch as waves (batchwave.hip) on the context's stream.
+static void run_batch_waves(pnt_ctx ctx, const pnt_batch_job* jobs, int n, int chunk) {
+  hipStream_t s = ctx->stream;
+  BatchState& B = *batch_state(ctx);
+  const int64_t need = plan_waves_pool(jobs, n, chunk);
+  if (need > B.pool_cap) {
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    dev_free(B.pool);
+    B.pool = nullptr;
+    B.pool_cap = std::max<int64_t>(need, 2 * B.pool_cap);
+    B.pool = dev_alloc(B.pool_cap);
+  }
+  const BWPlan W = plan_waves(jobs, n, chunk, B.pool, s);
+  PNT_HIP_CHECK(hipEventSynchronize(B.uploaded));  // the previous plan's copy has read the stage
+  grow_stage(B.stage, B.stage_cap, W.total);
+  grow_dev(B.dev, B.dev_cap, W.total, s);
+  std::memcpy(B.stage, W.blob.data(), W.blob.size());
+  enqueue_waves(W, B.stage, B.dev, s);
+  PNT_HIP_CHECK(hipEventRecord(B.uploaded, s));
+  order_job_streams(ctx, jobs, n);
+  finish_jobs(jobs, n);
+}
+
 void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
                           const pnt_batch_config* cfg) {
   hipStream_t s = ctx->stream;
   if (*(volatile unsigned*)ctx->kernel_error)
     throw Error(PNT_EKERNEL, "an earlier persistent stamp kernel aborted");
   BatchState& B = *batch_state(ctx);
-  const BatchPlan P = plan_batch(ctx, jobs, n, cfg ? *cfg : B.cfg);
+  const pnt_batch_config C = cfg ? *cfg : B.cfg;
+  if (C.waves) {
+    run_batch_waves(ctx, jobs, n, C.chunk);
+    return;
+  }
+  const BatchPlan P = plan_batch(ctx, jobs, n, C);
   PNT_HIP_CHECK(hipEventSynchronize(B.uploaded));
   if (P.total > B.stage_cap) {
     if (B.stage) PNT_HIP_CHECK(hipHostFree(B.stage));
@@ -657,6 +726,7 @@ void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
   PNT_HIP_CHECK(hipHostGetDevicePointer((void**)&host_err, ctx->kernel_error, 0));
   enqueue_plan(P, B.stage, B.dev, s, host_err, true);
   PNT_HIP_CHECK(hipEventRecord(B.uploaded, s));
+  order_job_streams(ctx, jobs, n);
   finish_jobs(jobs, n);
 }
 
@@ -733,8 +803,11 @@ void tune_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n, pnt_batch_config*
   for (int j = 0; j < n; ++j) brush_prepare_snapshot(jobs[j].brush, jobs[j].canvas, s);
   std::vector<SavedJob> sv(n);
   save_or_restore(jobs, n, sv, true, s);
-  const pnt_batch_config cands[] = {{32, 16, {0, 0}}, {32, 4, {0, 0}}, {32, 1, {0, 0}},
-                                    {16, 64, {0, 0}}, {16, 16, {0, 0}}, {16, 4, {0, 0}}};
+  // one persistent launch: tile x workgroups per job; waves: strokes of one
+  // canvas in flight (the stamp-chunk of a wave)
+  const pnt_batch_config cands[] = {{32, 16, 0, 1}, {32, 4, 0, 1}, {32, 1, 0, 1},
+                                    {16, 64, 0, 1}, {16, 16, 0, 1}, {16, 4, 0, 1},
+                                    {16, 1, 1, 1},  {16, 1, 1, 2},  {16, 1, 1, 4}};
   hipEvent_t e0, e1;
   PNT_HIP_CHECK(hipEventCreate(&e0));
   PNT_HIP_CHECK(hipEventCreate(&e1));
@@ -817,6 +890,7 @@ struct BatchGraph {
   std::vector<uint64_t> epochs;  // canvas epochs a launch expects
   char* stage = nullptr;
   char* dev = nullptr;
+  float* pool = nullptr;  // waves: pickup maps of the extra stroke chains
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -841,16 +915,33 @@ BatchGraph* capture_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n) {
       G->jobs[j].stamps = G->stamps[j].data();
     }
     hipStream_t s = ctx->stream;
-    const BatchPlan P = plan_batch(ctx, G->jobs.data(), n, B.cfg);
+    const pnt_batch_config C = B.cfg;
+    BatchPlan P;
+    BWPlan W;
+    int64_t total = 0;
+    if (C.waves) {  // the waves and their own pool of pickup maps
+      G->pool = dev_alloc(std::max<int64_t>(plan_waves_pool(G->jobs.data(), n, C.chunk), 1));
+      W = plan_waves(G->jobs.data(), n, C.chunk, G->pool, s);
+      total = W.total;
+    } else {
+      P = plan_batch(ctx, G->jobs.data(), n, C);
+      total = P.total;
+    }
     PNT_HIP_CHECK(hipStreamSynchronize(s));  // snapshot preparation done before capture
-    PNT_HIP_CHECK(hipHostMalloc((void**)&G->stage, (size_t)P.total));
-    PNT_HIP_CHECK(hipMalloc((void**)&G->dev, (size_t)P.total));
-    std::memcpy(G->stage, P.blob.data(), P.blob.size());
+    PNT_HIP_CHECK(hipHostMalloc((void**)&G->stage, (size_t)std::max<int64_t>(total, 1)));
+    PNT_HIP_CHECK(hipMalloc((void**)&G->dev, (size_t)std::max<int64_t>(total, 1)));
+    if (C.waves)
+      std::memcpy(G->stage, W.blob.data(), W.blob.size());
+    else
+      std::memcpy(G->stage, P.blob.data(), P.blob.size());
     unsigned* host_err = nullptr;
     PNT_HIP_CHECK(hipHostGetDevicePointer((void**)&host_err, ctx->kernel_error, 0));
     PNT_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-      enqueue_plan(P, G->stage, G->dev, s, host_err, false);
+      if (C.waves)
+        enqueue_waves(W, G->stage, G->dev, s);
+      else
+        enqueue_plan(P, G->stage, G->dev, s, host_err, false);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(s, &g);
@@ -884,6 +975,7 @@ void launch_batch_graph(BatchGraph* G) {
                 "a canvas or brush of the captured batch changed outside it; capture again");
   }
   PNT_HIP_CHECK(hipGraphLaunch(G->exec, ctx->stream));
+  order_job_streams(ctx, G->jobs.data(), (int)G->jobs.size());
   finish_jobs(G->jobs.data(), (int)G->jobs.size());
   for (size_t j = 0; j < G->jobs.size(); ++j) G->epochs[j] = G->jobs[j].canvas->epoch;
 }
@@ -895,46 +987,52 @@ void release_batch_graph(BatchGraph*& G) {
   if (G->graph) (void)hipGraphDestroy(G->graph);
   if (G->stage) (void)hipHostFree(G->stage);
   if (G->dev) (void)hipFree(G->dev);
+  if (G->pool) (void)hipFree(G->pool);
   pnt_ctx ctx = G->retained ? G->ctx : nullptr;
   delete G;
   G = nullptr;
   if (ctx) ctx_release(ctx);
 }
 
-// One BatchState per context (the plan buffers of its batches).
-static std::vector<std::pair<pnt_ctx, BatchState*>> g_batch_states;
-
+// The context's BatchState (plan buffers of its batches), made on first use;
+// it lives in the context handle and goes with it.
 static BatchState* batch_state(pnt_ctx ctx) {
-  for (auto& e : g_batch_states)
-    if (e.first == ctx) return e.second;
+  if (ctx->batch) return ctx->batch;
   auto* B = new BatchState;
-  PNT_HIP_CHECK(hipEventCreateWithFlags(&B->uploaded, hipEventDisableTiming));
-  int dev = 0, coop = 0, ncu = 0, per_cu = 0;
-  PNT_HIP_CHECK(hipGetDevice(&dev));
-  PNT_HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-  if (!coop) throw Error(PNT_ESTATE, "device has no cooperative launch");
-  PNT_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void*>(k_batch<32>), 1024, 0));
-  B->max_blocks32 = ncu * std::max(per_cu, 1);
-  PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void*>(k_batch<16>), 256, 0));
-  B->max_blocks16 = ncu * std::max(per_cu, 1);
-  g_batch_states.push_back({ctx, B});
+  try {
+    PNT_HIP_CHECK(hipEventCreateWithFlags(&B->uploaded, hipEventDisableTiming));
+    PNT_HIP_CHECK(hipEventCreateWithFlags(&B->finished, hipEventDisableTiming));
+    int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+    PNT_HIP_CHECK(hipGetDevice(&dev));
+    PNT_HIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (!coop) throw Error(PNT_ESTATE, "device has no cooperative launch");
+    PNT_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_batch<32>), 1024, 0));
+    B->max_blocks32 = ncu * std::max(per_cu, 1);
+    PNT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_batch<16>), 256, 0));
+    B->max_blocks16 = ncu * std::max(per_cu, 1);
+  } catch (...) {
+    if (B->uploaded) (void)hipEventDestroy(B->uploaded);
+    if (B->finished) (void)hipEventDestroy(B->finished);
+    delete B;
+    throw;
+  }
+  ctx->batch = B;
   return B;
 }
 
 void batch_release(pnt_ctx ctx) {
-  for (size_t i = 0; i < g_batch_states.size(); ++i)
-    if (g_batch_states[i].first == ctx) {
-      BatchState* B = g_batch_states[i].second;
-      if (B->stage) (void)hipHostFree(B->stage);
-      if (B->dev) (void)hipFree(B->dev);
-      if (B->uploaded) (void)hipEventDestroy(B->uploaded);
-      delete B;
-      g_batch_states.erase(g_batch_states.begin() + i);
-      return;
-    }
+  BatchState* B = ctx->batch;
+  if (!B) return;
+  if (B->stage) (void)hipHostFree(B->stage);
+  if (B->dev) (void)hipFree(B->dev);
+  if (B->pool) (void)hipFree(B->pool);
+  if (B->uploaded) (void)hipEventDestroy(B->uploaded);
+  if (B->finished) (void)hipEventDestroy(B->finished);
+  delete B;
+  ctx->batch = nullptr;
 }
 
 }  // namespace pnt

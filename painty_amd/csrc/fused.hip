@@ -102,6 +102,8 @@ __global__ __launch_bounds__(256) void k_stamp_levels(FArgs A, int j) {
   }
 }
 
+}  // namespace
+
 // 4-pixel groups of box b, and the blocks that scan them (each thread
 // takes two groups per pass).
 int ring_blocks_for(const RingBox& b, int threads) {
@@ -112,7 +114,6 @@ int ring_blocks_for(const RingBox& b, int threads) {
   return (int)std::min<int64_t>(std::max<int64_t>(blocks, 1), 1024);
 }
 
-}  // namespace
 
 // ---------------------------------------------------- concurrent strokes --
 // A dip cleans the pickup map (FootprintBrush.hxx:150-160), so the stamps
@@ -137,6 +138,38 @@ struct FusedState {
   std::vector<hipEvent_t> events;
 };
 
+void ktimer_release(KernelTimer*& t) {
+  if (!t) return;
+  for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
+  delete t;
+  t = nullptr;
+}
+
+double ktimer_total_ms(KernelTimer& t) {
+  if (t.used == 0) return 0.0;
+  PNT_HIP_CHECK(hipEventSynchronize(t.ev[t.used - 1]));
+  double ms = 0.0;
+  for (size_t k = 0; k + 1 < t.used; k += 2) {
+    float x = 0.0f;
+    PNT_HIP_CHECK(hipEventElapsedTime(&x, t.ev[k], t.ev[k + 1]));
+    ms += x;
+  }
+  return ms;
+}
+
+// Records the next event of a timed launch on s (a start before the launch,
+// an end after it); nothing when timing is off.
+static void ktimer_mark(pnt_brush_s* b, hipStream_t s) {
+  KernelTimer* t = b->ktimer;
+  if (!t || !t->on) return;
+  if (t->used == t->ev.size()) {
+    hipEvent_t e;
+    PNT_HIP_CHECK(hipEventCreate(&e));
+    t->ev.push_back(e);
+  }
+  PNT_HIP_CHECK(hipEventRecord(t->ev[t->used++], s));
+}
+
 void fused_release(FusedState*& F) {
   if (!F) return;
   for (int c = 1; c < kMaxChains; ++c) {
@@ -149,34 +182,7 @@ void fused_release(FusedState*& F) {
   F = nullptr;
 }
 
-namespace {
-
-// Concurrent units run as waves on one stream (the default), or on
-// concurrent streams with PAINTY_FUSED_SCHED=streams.
-bool fused_waves() {
-  const char* e = std::getenv("PAINTY_FUSED_SCHED");
-  return !(e && std::strcmp(e, "streams") == 0);
-}
-
-// Concurrent chains of one call: `want` if given (> 0), else
-// PAINTY_FUSED_CHAINS, else 8 for waves (the slots of a wave) or 4 for
-// streams (the hardware queues of a process).
-int fused_chains(int want) {
-  int c = want > 0 ? want : (fused_waves() ? kMaxChains : 4);
-  if (want <= 0)
-    if (const char* e = std::getenv("PAINTY_FUSED_CHAINS")) c = std::atoi(e);
-  return std::min(std::max(c, 1), kMaxChains);
-}
-
-struct Unit {
-  int64_t s0, s1;             // stamps [s0, s1)
-  int x0, y0, x1, y1;         // union of the allowed boxes (x1 < x0: touches nothing)
-  int chain;
-  bool dry_after;             // the unit's last stamp dries the canvas
-  std::vector<int> deps;      // earlier units of the phase it must follow
-};
-
-bool boxes_meet(const Unit& a, const Unit& b) {
+static bool boxes_meet(const Unit& a, const Unit& b) {
   if (a.x1 < a.x0 || b.x1 < b.x0) return false;
   return !(a.x1 < b.x0 || b.x1 < a.x0 || a.y1 < b.y0 || b.y1 < a.y0);
 }
@@ -235,6 +241,25 @@ std::vector<Unit> plan_units(const pnt_stamp* st, const PStamp* h, const uint8_t
     }
   }
   return U;
+}
+
+namespace {
+
+// Concurrent units run as waves on one stream (the default), or on
+// concurrent streams with PAINTY_FUSED_SCHED=streams.
+bool fused_waves() {
+  const char* e = std::getenv("PAINTY_FUSED_SCHED");
+  return !(e && std::strcmp(e, "streams") == 0);
+}
+
+// Concurrent chains of one call: `want` if given (> 0), else
+// PAINTY_FUSED_CHAINS, else 8 for waves (the slots of a wave) or 4 for
+// streams (the hardware queues of a process).
+int fused_chains(int want) {
+  int c = want > 0 ? want : (fused_waves() ? kMaxChains : 4);
+  if (want <= 0)
+    if (const char* e = std::getenv("PAINTY_FUSED_CHAINS")) c = std::atoi(e);
+  return std::min(std::max(c, 1), kMaxChains);
 }
 
 // ----------------------------------------------------------------- waves --
@@ -348,6 +373,7 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
                     const std::vector<uint8_t>& clear, int want_chains) {
   hipStream_t s = b->ctx->stream;
   b->last_launches = 0;
+  if (b->ktimer) b->ktimer->used = 0;
   FArgs A0;
   A0.st = D.st;
   A0.canvas = cv->layer();
@@ -397,7 +423,9 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
     const int rb = a.use_snap ? ring_blocks_for(p.box, kT * kT) : 0;
     const int jk = (int)k;
     if (!(p.flags & (kStDupX | kStDupY))) {
+      if (q == s) ktimer_mark(b, q);
       k_stamp_fused<kT><<<a.ntiles + rb, kT * kT, 0, q>>>(a, jk, a.ntiles, rb);
+      if (q == s) ktimer_mark(b, q);
       b->last_launches += 1;
     } else {  // snapshot refresh first: a duplicated pixel is not refreshed by its visits
       if (rb) k_stamp_fused<kT><<<rb, kT * kT, 0, q>>>(a, jk, 0, rb);
@@ -526,8 +554,10 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
           }
           start += (std::max(S.nblk, 1) + 7) / 8 * 8;
         }
+        ktimer_mark(b, s);
         k_stamp_wave<kT><<<(unsigned)start, T2, 0, s>>>(W);
         PNT_HIP_CHECK(hipGetLastError());
+        ktimer_mark(b, s);
         b->last_launches += 1;
         for (int i : ready)
           if (++next[(size_t)i] == ph[(size_t)i].size()) done[(size_t)i] = 1;

@@ -22,6 +22,8 @@ void dev_free(void* p);
 struct PersistentState;
 struct ChainState;
 struct FusedState;
+struct BatchState;
+struct KernelTimer;
 }  // namespace pnt
 
 struct pnt_ctx_s {
@@ -35,6 +37,9 @@ struct pnt_ctx_s {
   // Pinned host word that persistent kernels raise when a bounded wait
   // expires; checked by pnt_ctx_synchronize and at the next persistent run.
   unsigned* kernel_error = nullptr;
+  // plan buffers and launch configuration of the context's batches
+  // (batch.hip); made by the first batch call, freed with the context
+  pnt::BatchState* batch = nullptr;
   void use() const { PNT_HIP_CHECK(hipSetDevice(device)); }
 };
 
@@ -134,6 +139,7 @@ struct pnt_brush_s {
   pnt::ChainState* chain = nullptr;  // buffers of PNT_EXEC_CHAINED runs
   pnt::FusedState* fused = nullptr;  // streams and maps of concurrent fused strokes
   int64_t last_launches = 0;  // stamp-kernel launches of the last fused stamp-list call
+  pnt::KernelTimer* ktimer = nullptr;  // per-launch event pairs (pnt_brush_set_kernel_timing)
 
   pnt::Layer7 pick_planes() const {
     pnt::Layer7 L;
@@ -187,9 +193,10 @@ void footprint_box(const std::vector<float>& fp, int size_map, int box[4]);
 void palette_from_json(const char* text, int64_t len, std::vector<double>& K,
                        std::vector<double>& S);
 std::string palette_to_json(const double* K, const double* S, int64_t n);
-// A canvas (cleared, background 1) on a caller buffer of at least
-// 11*round_up(rows*cols, 64) floats that pnt_canvas_destroy does not free.
-pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base);
+// A canvas on a caller buffer of at least 11*round_up(rows*cols, 64) floats
+// that pnt_canvas_destroy does not free: cleared (background 1) when `clear`,
+// else a view of the planes as they are.
+pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base, bool clear = true);
 // The canvas clock's current time point (microseconds).
 int64_t canvas_now_us(const pnt_canvas_s* cv);
 // Runs the drying-timer steps whose ticks have passed (nothing when the

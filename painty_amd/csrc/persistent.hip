@@ -319,6 +319,12 @@ void plan_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t n,
 
 void plan_stamps_geom(int size_map, double radius, pnt_canvas cv, const pnt_stamp* st, int64_t n,
                       std::vector<PStamp>& h_st, std::vector<int2>& h_dl, std::vector<int>& h_off) {
+  plan_stamps_rc(size_map, radius, cv->rows, cv->cols, st, n, h_st, h_dl, h_off);
+}
+
+void plan_stamps_rc(int size_map, double radius, int rows, int cols, const pnt_stamp* st,
+                    int64_t n, std::vector<PStamp>& h_st, std::vector<int2>& h_dl,
+                    std::vector<int>& h_off) {
   std::vector<int2> list;
   std::vector<int> off;
   for (int64_t k = 0; k < n; ++k) {
@@ -333,8 +339,8 @@ void plan_stamps_geom(int size_map, double radius, pnt_canvas cv, const pnt_stam
     a.g.s = std::sin(-theta);
     a.g.hr = (size_map - 1) / 2;
     a.g.size_map = size_map;
-    a.g.rows = cv->rows;
-    a.g.cols = cv->cols;
+    a.g.rows = rows;
+    a.g.cols = cols;
     a.dupx = stamp_dup_axis(cx, a.g.hr) ? 1 : 0;
     a.dupy = stamp_dup_axis(cy, a.g.hr) ? 1 : 0;
     PStamp p{};

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "painty_hip.h"
+#include "internal.hpp"
 
 struct pnt_brush_s;
 struct pnt_canvas_s;
@@ -43,6 +44,29 @@ PlanDev upload_plan_parts(pnt_brush_s* b, pnt_canvas_s* cv, const PlanPart* part
                           int64_t fp_floats, const std::function<void(float*)>& fill_fp,
                           const float** d_fp);
 
+// A run of stamps of one stroke (from a dip to the next dip) within one
+// drying phase, and the earlier units of its phase whose snapshot boxes meet
+// its own (fused.hip, plan_units).
+struct Unit {
+  int64_t s0, s1;             // stamps [s0, s1)
+  int x0, y0, x1, y1;         // union of the allowed boxes (x1 < x0: touches nothing)
+  int chain;
+  bool dry_after;             // the unit's last stamp dries the canvas
+  std::vector<int> deps;      // earlier units of the phase it must follow
+};
+// Cuts stamps [0, n) (planned in h) into units and assigns them to at most
+// nchains pickup-map chains by list scheduling; clear[k]: the map is clean
+// before stamp k.
+std::vector<Unit> plan_units(const pnt_stamp* st, const PStamp* h, const uint8_t* clear,
+                             int64_t n, int nchains);
+// Blocks of `threads` threads that scan the snapshot ring of box b.
+int ring_blocks_for(const RingBox& b, int threads);
+// Plans stamps [0, n) for a brush of footprint side size_map and radius on a
+// rows x cols canvas (FP64 geometry, ring boxes, shared-pixel levels).
+void plan_stamps_rc(int size_map, double radius, int rows, int cols, const pnt_stamp* st,
+                    int64_t n, std::vector<PStamp>& h_st, std::vector<int2>& h_dl,
+                    std::vector<int>& h_off);
+
 // chains: concurrent stroke chains at most (0: PAINTY_FUSED_CHAINS or 4)
 void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n, int chains = 0);
 void run_fused_strokes(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* strokes, int64_t ns);
@@ -51,6 +75,16 @@ struct ChainState;
 void chain_release(ChainState*& c);
 struct FusedState;
 void fused_release(FusedState*& f);
+// Event pairs around each stamp-kernel launch of a brush's stamp-list calls,
+// for measuring the kernels' own durations on their stream.
+struct KernelTimer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // start, end of launch k at 2k, 2k+1
+  size_t used = 0;
+};
+void ktimer_release(KernelTimer*& t);
+// Sum of the timed launches' durations in ms (waits for the last one).
+double ktimer_total_ms(KernelTimer& t);
 void run_persistent(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 void persistent_release(PersistentState*& p);
 
@@ -67,5 +101,21 @@ BatchGraph* capture_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n);
 void launch_batch_graph(BatchGraph* g);
 void release_batch_graph(BatchGraph*& g);
 void batch_release(pnt_ctx ctx);
+// A batch as waves (batchwave.hip): the device image (jobs, slots, group
+// tables, stamps, visit lists, paints) and where each wave's slots and group
+// table start; pickup maps beyond the brushes' own come from a pool given
+// when the image is built (plan_waves_pool floats).
+struct BWPlan {
+  std::vector<char> blob;
+  int64_t total = 0;
+  int64_t pool_floats = 0;
+  int64_t o_jobs = 0;
+  std::vector<int64_t> wave_slots, wave_grp;
+  std::vector<int> wave_blocks;
+  int njobs = 0;
+};
+int64_t plan_waves_pool(const pnt_batch_job* jobs, int n, int chunk);
+BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipStream_t s);
+void enqueue_waves(const BWPlan& W, const char* stage, char* dev, hipStream_t s);
 
 }  // namespace pnt

@@ -1,22 +1,41 @@
-// A request server for stroke + composite work on resident canvases.
+// A request server for stroke + composite work on resident canvases, with
+// continuous batching on the device.
 //
 // Requests (a stamp list and a composite on a canvas identified by an id)
-// are submitted from any thread and return a ticket.  Worker slots -- one
-// host thread, one HIP stream and one brush each -- take admitted requests
-// and run them concurrently on the device: the stamps as PNT_EXEC_FUSED
-// launches (FootprintBrush::imprint, renderer/FootprintBrush.hxx:73-143) and
-// the composite (Renderer::compose, renderer/Renderer.hxx:48-53) into the
-// canvas' resident output planes, optionally copied to a host buffer.  A
-// request starts as the reference's fresh FootprintBrush would: pickup map
-// clean, snapshot copied whole from the canvas on its first imprint.
+// are submitted from any thread and return a ticket.  Each request runs as
+// the reference's fresh FootprintBrush would (renderer/FootprintBrush.hxx:
+// 29-63): pickup map clean, snapshot copied whole from the canvas before its
+// first imprint (:281-284), its stamps (FootprintBrush::imprint, :73-143)
+// with dryCanvas() where a stamp asks for it (Canvas.hxx:105-121), then
+// Renderer::compose (Renderer.hxx:48-53) into the canvas' resident composite
+// planes, copied to the caller's host buffer when one is given.
+//
+// Execution.  Planner threads plan each request on arrival (FP64 stamp
+// geometry, snapshot ring boxes, shared-pixel levels; its stamps cut into
+// units -- runs of one stroke -- with the units they depend on; see
+// plan_units in fused.hip).  One engine thread admits planned requests into
+// up to n_slots request slots and drives the device with "waves": each wave
+// is ONE launch of k_serve_wave on the engine's stream, carrying the next
+// phase of every ready unit of every admitted request (up to kSlots phases:
+// a canvas initialisation, a full snapshot copy, a pickup-map clear, a stamp
+// -- window tiles and ring scan side by side -- or the split parts of a
+// stamp that straddles canvas column/row 0, a dryCanvas, or the composite),
+// oldest request first.  Requests that arrive while others run join the
+// next wave: their work overlaps on the device, in the same launches.
+// Phases in one wave touch disjoint memory (different canvases, or units of
+// one canvas with disjoint snapshot boxes and their own pickup maps), and the
+// stream orders the waves, so each request computes exactly the arithmetic
+// of running alone: the same to the bit.  At most kAhead waves are queued
+// ahead of the host, so a new request waits for few.
 //
 // Admission: requests on one canvas run in submission order, one at a time;
 // a request whose canvas is not resident is admitted when its canvas fits
-// the HBM budget, after evicting canvases that no queued or running request
-// needs, least recently used first.  An evicted canvas' state is dropped (a
-// cache, not a store): later requests on its id fail with PNT_ESTATE.  Requests that arrive while others run
-// are admitted as soon as a slot and memory are free, so the device always
-// holds the work of up to n_slots requests.
+// the HBM budget (canvas + composite planes, 56 B/px), after evicting
+// canvases that no queued or running request needs, least recently used
+// first.  An evicted canvas is spilled to host memory by the copier thread,
+// and restored from there when a later request names it.  A request slot's
+// working memory (snapshot 28 B/px, dirty bytes 1 B/px, pickup maps, plan)
+// is kept by the slot for the next request and is not counted in the budget.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -31,7 +50,10 @@
 
 #include "handles.hpp"
 #include "internal.hpp"
+#include "km.hpp"
 #include "persistent.hpp"
+#include "stampcore.hpp"
+#include "wave.hpp"
 
 namespace pnt {
 int guard_call(const std::function<void()>& f);  // api.hip: status + pnt_last_error
@@ -42,17 +64,50 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-struct Resident {
-  pnt_canvas cv = nullptr;
-  float* block = nullptr;  // canvas planes (11 x stride) then composite planes (3 x stride)
-  int64_t cap = 0;         // floats in block (allocated or to allocate)
-  int busy = 0;            // a request on it is admitted or running
-  Clock::time_point last_use;
+constexpr int kSlots = 16;         // phases per wave
+constexpr int kChainsPerReq = 4;   // pickup maps of one request (concurrent units)
+constexpr int kAhead = 3;          // waves queued ahead of the engine
+constexpr int kT = kWT;            // window tile side
+constexpr int kT2 = kWT2;          // threads per block
+
+struct SArgs {
+  int nslots;
+  int pad;
+  SSlot slot[kSlots];
+};
+static_assert(sizeof(SArgs) <= 4000, "wave arguments must fit the kernel argument segment");
+
+// One wave of the server: its phases arrive as kernel arguments.
+__global__ __launch_bounds__(kWT2) void k_serve_wave(SArgs A) {
+  __shared__ int sm[kWT + 2][kWT + 2];
+  int si = 0;
+  while (si + 1 < A.nslots && (int)blockIdx.x >= A.slot[si + 1].start) ++si;
+  const SSlot& S = A.slot[si];
+  const int b = (int)blockIdx.x - S.start;
+  if (b >= S.nblk) return;
+  run_wave_slot(S, b, sm);
+}
+
+int stream_blocks(int64_t items) { return wave_stream_blocks(items); }
+
+int64_t canvas_floats(int rows, int cols) {
+  return (11 + 3) * round_up((int64_t)rows * cols, 64);  // canvas planes + composite planes
+}
+int64_t canvas_bytes(int rows, int cols) { return canvas_floats(rows, cols) * 4; }
+int64_t dirty_bytes_for(int64_t n) { return round_up(n, 16) + 16; }
+
+struct Phase {
+  int64_t k;
+  int kind;
 };
 
+struct Resident;
+
+enum Stage : int { kPrologue = 0, kUnits, kDrying, kComposite, kIssued };
+
 struct Req {
-  int64_t ticket;
-  pnt_request r;
+  int64_t ticket = 0;
+  pnt_request r{};
   std::vector<pnt_stamp> stamps;
   std::vector<float> footprint;
   Clock::time_point t_submit;
@@ -60,209 +115,726 @@ struct Req {
   int status = PNT_OK;
   std::string error;
   bool done = false;
+  // plan (planner thread)
+  bool planned = false;
+  int size_map = 0, hr = 0, tiles_x = 0, ntiles = 0;
+  int64_t mstride = 0;
+  int fbox[4] = {0, 0, -1, -1};
+  std::vector<float> fp;  // padded footprint
+  std::vector<PStamp> st;
+  std::vector<int2> dl;
+  std::vector<int> off;
+  std::vector<Unit> U;
+  std::vector<std::vector<Phase>> ph;  // per unit
+  std::vector<int> prev_on_chain;      // per unit
+  std::vector<float> paint;            // 6 per stamp: the brush paint at that stamp
+  std::vector<int> rb;                 // ring-scan blocks per stamp
+  int64_t plan_bytes = 0;
+  // engine
+  int slot = -1;
+  Resident* res = nullptr;
+  bool fresh = false;  // a new canvas: initialised by the prologue
+  int stage = kPrologue;
+  size_t g0 = 0, g1 = 0;  // units of the current drying phase
+  std::vector<size_t> next;
+  std::vector<char> udone;
+  const PStamp* d_st = nullptr;
+  const int2* d_dl = nullptr;
+  const int* d_off = nullptr;
+  const float* d_fp = nullptr;
 };
 
-int64_t canvas_floats(int rows, int cols) {
-  return (11 + 3) * round_up((int64_t)rows * cols, 64);  // canvas planes + composite planes
-}
-int64_t canvas_bytes(int rows, int cols) { return canvas_floats(rows, cols) * 4; }
+enum Where : int { kResident = 0, kSpilling, kSpilled, kRestoring };
+
+struct Resident {
+  float* block = nullptr;  // canvas planes (11 x stride) then composite planes (3 x stride)
+  int64_t cap = 0;         // floats of block
+  int rows = 0, cols = 0;
+  int64_t stride = 0;
+  float bg[3] = {1.f, 1.f, 1.f};
+  bool initialised = false;
+  int active = 0;  // admitted requests not yet complete
+  int where = kResident;
+  std::vector<float> host;     // spilled canvas planes (11 x stride)
+  pnt_canvas view = nullptr;   // handle on the query context, made on demand
+  Clock::time_point last_use;
+};
+
+struct SlotRes {
+  float* snap = nullptr;
+  int64_t snap_cap = 0;  // floats
+  uint8_t* dirty = nullptr;
+  int64_t dirty_cap = 0;
+  float* pick = nullptr;
+  int64_t pick_cap = 0;  // floats
+  char* plan = nullptr;
+  int64_t plan_cap = 0;
+  char* stage = nullptr;  // pinned
+  int64_t stage_cap = 0;
+  hipEvent_t fin = nullptr;  // after the request's composite
+  Req* req = nullptr;
+};
+
+struct CopyJob {
+  int kind;  // 0: composite to the caller, 1: spill, 2: restore
+  Req* q = nullptr;
+  int64_t canvas_id = 0;
+};
 
 }  // namespace
 
 struct pnt_server_s {
   int device = 0;
   int64_t budget = 0;  // bytes
-  int64_t held = 0;    // bytes of canvas blocks held: resident + pooled
+  int64_t held = 0;    // bytes of canvas blocks held: resident + pooled + in transit
   std::multimap<int64_t, float*> pool;  // idle blocks by capacity (floats)
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
-  std::deque<Req*> queue;
+  std::condition_variable cv_engine, cv_done, cv_plan, cv_copy;
+  std::deque<Req*> planq;   // to plan
+  std::deque<Req*> queue;   // submitted, in order, not yet admitted
   std::unordered_map<int64_t, Req*> tickets;
   std::unordered_map<int64_t, Resident> canvases;
   std::unordered_map<int64_t, int> pending;  // queued requests per canvas
-  std::unordered_map<int64_t, char> dropped;  // evicted canvases (state gone)
-  int64_t next_ticket = 1;
+  std::deque<CopyJob> copyq;
+  std::vector<SlotRes> slots;
+  std::vector<int> free_slots;
   bool stopping = false;
-  std::vector<std::thread> workers;
-  std::vector<pnt_ctx> ctxs;
-  std::vector<pnt_brush> brushes;
   int64_t completed = 0;
+  int64_t ticket_seq = 1;
+  int spills = 0;  // spills in progress
+  // engine-thread state
+  std::vector<Req*> active;  // admitted, in admission order
+  pnt_ctx ctx = nullptr;     // engine stream
+  pnt_ctx qctx = nullptr;    // views of resident canvases (pnt_server_canvas)
+  pnt_ctx cctx = nullptr;    // copier stream
+  hipEvent_t wave_ev[kAhead + 1] = {};
+  int64_t waves = 0;
+  std::thread engine, copier;
+  std::vector<std::thread> planners;
 
-  // Next admissible request (caller holds mu), or nullptr.  A request is
-  // admissible when no earlier queued request is on its canvas, its canvas
-  // is idle, and a new canvas fits the budget (evicting idle ones).  The
-  // canvas is marked busy and a new one's memory reserved before the lock
-  // is released.
-  Req* take() {
-    std::vector<int64_t> seen;
-    for (auto it = queue.begin(); it != queue.end(); ++it) {
-      Req* q = *it;
-      const int64_t id = q->r.canvas_id;
-      if (std::find(seen.begin(), seen.end(), id) != seen.end()) continue;
-      seen.push_back(id);
-      auto c = canvases.find(id);
-      if (c != canvases.end()) {
-        if (c->second.busy) continue;
-      } else if (dropped.count(id)) {
-        queue.erase(it);  // fails in run(): its canvas was evicted
-        if (--pending[id] == 0) pending.erase(id);
-        q->status = PNT_ESTATE;
-        return q;
-      } else {
-        Resident nr;
-        if (!reserve(canvas_floats(q->r.rows, q->r.cols), &nr)) continue;
-        c = canvases.emplace(id, nr).first;
-      }
-      c->second.busy = 1;
-      if (--pending[id] == 0) pending.erase(id);
-      queue.erase(it);
-      return q;
-    }
-    return nullptr;
-  }
-
-  // A block of at least `need` floats for a new canvas (caller holds mu): an
-  // idle pooled block of close size, or room for a new one within the budget
-  // -- evicting idle canvases into the pool, least recently used first, and
-  // releasing pooled blocks that fit nothing.  False when nothing can give
-  // way now.
-  bool reserve(int64_t need, Resident* r) {
-    for (;;) {
-      auto it = pool.lower_bound(need);
-      if (it != pool.end() && it->first <= 2 * need) {
-        r->cap = it->first;
-        r->block = it->second;
-        pool.erase(it);
-        return true;
-      }
-      if (held + need * 4 <= budget) {
-        r->cap = need;
-        r->block = nullptr;  // allocated by the worker
-        held += need * 4;
-        return true;
-      }
-      if (evict_one()) continue;
-      if (pool.empty()) return false;
-      auto big = std::prev(pool.end());  // release the largest idle block
-      dev_free(big->second);
-      held -= big->first * 4;
-      pool.erase(big);
-    }
-  }
-
-  // Moves the least recently used idle canvas with no queued requests into
-  // the pool; its state is dropped.  Caller holds mu.
-  bool evict_one() {
-    int64_t victim = -1;
-    Clock::time_point oldest = Clock::time_point::max();
-    for (auto& kv : canvases) {
-      const Resident& R = kv.second;
-      if (R.busy || pending.count(kv.first)) continue;
-      if (R.last_use < oldest) {
-        oldest = R.last_use;
-        victim = kv.first;
-      }
-    }
-    if (victim < 0) return false;
-    Resident& R = canvases[victim];
-    if (R.cv) pnt_canvas_destroy(R.cv);  // leaves the block alone
-    if (R.block) {
-      pool.emplace(R.cap, R.block);
-    } else {
-      held -= R.cap * 4;
-    }
-    canvases.erase(victim);
-    dropped[victim] = 1;
-    return true;
-  }
-
-  void run(int slot, Req* q);
-  void worker(int slot);
+  void plan(Req* q);
+  void planner();
+  bool reserve(int64_t need, float** block, int64_t* cap);
+  bool evict_one();
+  std::vector<Req*> admit();
+  void setup(Req* q);
+  bool collect(Req* q, SArgs& A, int& start, std::vector<std::pair<Req*, int>>& used);
+  void after_wave(const std::vector<std::pair<Req*, int>>& used);
+  void complete(Req* q);
+  void finish(Req* q, int status, const std::string& err);
+  void run_engine();
+  void run_copier();
+  void copy_job(const CopyJob& j);
 };
 
-void pnt_server_s::run(int slot, Req* q) {
-  pnt_ctx ctx = ctxs[slot];
-  pnt_brush b = brushes[slot];
-  hipStream_t s = ctx->stream;
-  Resident* R = nullptr;
+// ------------------------------------------------------------- planning --
+void pnt_server_s::plan(Req* q) {
+  const pnt_request& r = q->r;
+  q->size_map = footprint_size_map(r.radius, q->footprint.data(), r.width);
+  q->hr = (q->size_map - 1) / 2;
+  const int side = 2 * q->hr + 1;
+  q->tiles_x = (side + kT - 1) / kT;
+  q->ntiles = q->tiles_x * q->tiles_x;
+  q->mstride = round_up((int64_t)q->size_map * q->size_map, 64);
+  q->fp.assign((size_t)q->size_map * q->size_map, 0.0f);
+  pad_footprint_into(q->footprint.data(), r.width, q->size_map, q->fp.data(), q->fbox);
+  const int64_t n = (int64_t)q->stamps.size();
+  plan_stamps_rc(q->size_map, r.radius, r.rows, r.cols, q->stamps.data(), n, q->st, q->dl,
+                 q->off);
+  std::vector<uint8_t> clear((size_t)n, 0);
+  if (n) clear[0] = 1;  // a fresh brush starts from a clean pickup map
+  q->U = n ? plan_units(q->stamps.data(), q->st.data(), clear.data(), n, kChainsPerReq)
+           : std::vector<Unit>{};
+  const bool snap = r.use_snapshot != 0;
+  q->ph.assign(q->U.size(), {});
+  q->rb.assign((size_t)n, 0);
+  for (size_t i = 0; i < q->U.size(); ++i)
+    for (int64_t k = q->U[i].s0; k < q->U[i].s1; ++k) {
+      if (clear[(size_t)k] || (q->stamps[(size_t)k].flags & PNT_STAMP_DIP))
+        q->ph[i].push_back(Phase{k, kSClear});
+      const PStamp& p = q->st[(size_t)k];
+      q->rb[(size_t)k] = snap ? ring_blocks_for(p.box, kT2) : 0;
+      if (p.flags & (kStDupX | kStDupY)) {
+        if (snap) q->ph[i].push_back(Phase{k, kSRing});
+        q->ph[i].push_back(Phase{k, kSWindow});
+        if (p.dl_nlev > 0) q->ph[i].push_back(Phase{k, kSLevels});
+      } else {
+        q->ph[i].push_back(Phase{k, kSStamp});
+      }
+    }
+  q->prev_on_chain.assign(q->U.size(), -1);
   {
-    std::lock_guard<std::mutex> lk(mu);  // entries are stable while busy
-    if (q->status == PNT_ESTATE)
-      throw Error(PNT_ESTATE, "canvas " + std::to_string(q->r.canvas_id) +
-                                  " was evicted from the HBM budget; its state is gone");
-    R = &canvases.find(q->r.canvas_id)->second;
+    std::vector<int> last(kChainsPerReq, -1);
+    for (size_t i = 0; i < q->U.size(); ++i) {
+      q->prev_on_chain[i] = last[(size_t)q->U[i].chain];
+      last[(size_t)q->U[i].chain] = (int)i;
+    }
   }
-  if (!R->cv) {  // a new canvas: substrate `background`, no paint
-    if (!R->block) R->block = dev_alloc(R->cap);
-    R->cv = canvas_over(ctx, q->r.rows, q->r.cols, R->block);
-    const Rgb3 r0 = R->cv->R0();
-    for (int c = 0; c < 3; ++c) launch_fill(r0.p[c], R->cv->n, q->r.background[c], s);
-    R->cv->bg[0] = q->r.background[0];
-    R->cv->bg[1] = q->r.background[1];
-    R->cv->bg[2] = q->r.background[2];
+  q->paint.assign(6 * (size_t)n, 0.0f);
+  float cur[6] = {0, 0, 0, 0, 0, 0};  // a fresh brush holds no paint
+  for (int64_t k = 0; k < n; ++k) {
+    if (q->stamps[(size_t)k].flags & PNT_STAMP_DIP)
+      for (int c = 0; c < 3; ++c) {
+        cur[c] = q->stamps[(size_t)k].K[c];
+        cur[3 + c] = q->stamps[(size_t)k].S[c];
+      }
+    std::memcpy(&q->paint[6 * (size_t)k], cur, sizeof cur);
   }
-  PNT_REQUIRE(R->cv->rows == q->r.rows && R->cv->cols == q->r.cols,
-              "request size differs from its resident canvas");
-  pnt_canvas cv = R->cv;
-  // a fresh brush for this request: FootprintBrush(radius), pickup map clean
-  if (b->width == q->r.width) {  // same geometry: new footprint into the same buffers
-    const int w = q->r.width, sm = b->size_map, pad = (sm - w) / 2;
-    std::fill(b->fp_host.begin(), b->fp_host.end(), 0.0f);
-    for (int i = 0; i < w; ++i)
-      for (int j = 0; j < w; ++j)
-        b->fp_host[(size_t)(i + pad) * sm + (j + pad)] = q->footprint[(size_t)i * w + j];
-    footprint_box(b->fp_host, sm, b->fbox);
-    PNT_HIP_CHECK(hipMemcpyAsync(b->fp, b->fp_host.data(), b->m * sizeof(float),
-                                 hipMemcpyHostToDevice, s));
-    launch_fill(b->pick, 7 * b->mstride, 0.0f, s);
-    PNT_HIP_CHECK(hipStreamSynchronize(s));  // fp_host is rewritten by the next request
-    b->radius = q->r.radius;
-  } else {
-    PNT_HIP_CHECK(hipStreamSynchronize(s));
-    brush_set_geometry(b, q->r.radius, q->footprint.data(), q->r.width);
-  }
-  b->pickup_rate = q->r.pickup_rate;
-  b->dep_rate = q->r.deposition_rate;
-  b->use_snap = q->r.use_snapshot != 0;
-  for (int c = 0; c < 3; ++c) b->paintK[c] = b->paintS[c] = 0.0f;
-  if (b->use_snap) brush_full_snapshot(b, cv);
-  // one chain: the worker slots already keep the device busy with other canvases
-  run_fused(b, cv, q->stamps.data(), (int64_t)q->stamps.size(), 1);
-  float* comp = R->block + 11 * cv->stride;
-  const Rgb3 out{{comp, comp + cv->stride, comp + 2 * cv->stride}};
-  launch_compose(cv->n, cv->layer(), cv->R0(), out, s);
-  if (q->r.out)
-    for (int c = 0; c < 3; ++c)
-      PNT_HIP_CHECK(hipMemcpyAsync(q->r.out + c * cv->n, out.p[c], cv->n * sizeof(float),
-                                   hipMemcpyDeviceToHost, s));
-  PNT_HIP_CHECK(hipStreamSynchronize(s));
-  if (*(volatile unsigned*)ctx->kernel_error) throw Error(PNT_EKERNEL, "a stamp kernel aborted");
+  auto al = [](int64_t x) { return round_up(x, 256); };
+  q->plan_bytes = al((int64_t)(q->st.size() * sizeof(PStamp))) +
+                  al((int64_t)(q->dl.size() * sizeof(int2))) +
+                  al((int64_t)(q->off.size() * sizeof(int))) +
+                  al((int64_t)(q->fp.size() * sizeof(float)));
+  q->next.assign(q->U.size(), 0);
+  q->udone.assign(q->U.size(), 0);
+  for (size_t i = 0; i < q->U.size(); ++i)
+    if (q->ph[i].empty()) q->udone[i] = 1;
+  q->g0 = q->g1 = 0;  // the first drying phase: units [0, g1]
+  while (q->g1 + 1 < q->U.size() && !q->U[q->g1].dry_after) ++q->g1;
 }
 
-void pnt_server_s::worker(int slot) {
+void pnt_server_s::planner() {
   (void)hipSetDevice(device);
   for (;;) {
     Req* q = nullptr;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_work.wait(lk, [&] { return stopping || (q = take()) != nullptr; });
-      if (!q) return;  // stopping and nothing admissible
+      cv_plan.wait(lk, [&] { return stopping || !planq.empty(); });
+      if (planq.empty()) return;
+      q = planq.front();
+      planq.pop_front();
     }
-    const int st = guard_call([&] { run(slot, q); });
+    const int st = guard_call([&] { plan(q); });
     std::lock_guard<std::mutex> lk(mu);
-    q->status = st;
-    if (st != PNT_OK) q->error = pnt_last_error();
-    auto c = canvases.find(q->r.canvas_id);
-    if (c != canvases.end()) {
-      c->second.busy = 0;
-      c->second.last_use = Clock::now();
+    if (st != PNT_OK) {
+      q->status = st;
+      q->error = pnt_last_error();
     }
-    q->latency_ms =
-        std::chrono::duration<double, std::milli>(Clock::now() - q->t_submit).count();
-    q->done = true;
-    ++completed;
-    cv_done.notify_all();
-    cv_work.notify_all();  // the canvas is free again; memory may be too
+    q->planned = true;
+    cv_engine.notify_all();
+  }
+}
+
+// ------------------------------------------------------------ admission --
+// A block of at least `need` floats for a canvas (caller holds mu): an idle
+// pooled block of close size, or room for a new one within the budget --
+// spilling idle canvases, least recently used first, and releasing pooled
+// blocks that fit nothing.  False when nothing can give way now (a spill in
+// progress returns its block to the pool when its copy is done).
+bool pnt_server_s::reserve(int64_t need, float** block, int64_t* cap) {
+  for (;;) {
+    auto it = pool.lower_bound(need);
+    if (it != pool.end() && it->first <= 2 * need) {
+      *cap = it->first;
+      *block = it->second;
+      pool.erase(it);
+      return true;
+    }
+    if (held + need * 4 <= budget) {
+      *cap = need;
+      *block = nullptr;  // allocated by the engine
+      held += need * 4;
+      return true;
+    }
+    if (!pool.empty()) {  // release the largest idle block
+      auto big = std::prev(pool.end());
+      dev_free(big->second);
+      held -= big->first * 4;
+      pool.erase(big);
+      continue;
+    }
+    if (!evict_one()) return false;
+  }
+}
+
+// Starts spilling the least recently used idle canvas (no queued or running
+// request) to host memory; its block goes to the pool when the copy is done.
+// Caller holds mu.  False when no canvas can be evicted now.
+bool pnt_server_s::evict_one() {
+  if (spills > 0) return false;  // one at a time: its block may be enough
+  int64_t victim = -1;
+  Clock::time_point oldest = Clock::time_point::max();
+  for (auto& kv : canvases) {
+    const Resident& R = kv.second;
+    if (R.where != kResident || R.active || pending.count(kv.first) || !R.block) continue;
+    if (R.last_use < oldest) {
+      oldest = R.last_use;
+      victim = kv.first;
+    }
+  }
+  if (victim < 0) return false;
+  canvases[victim].where = kSpilling;
+  ++spills;
+  copyq.push_back(CopyJob{1, nullptr, victim});
+  cv_copy.notify_all();
+  return false;  // the block arrives in the pool later
+}
+
+// Admits queued requests, in submission order, while slots and memory allow
+// (caller holds mu).  A request waits while an earlier request on its canvas
+// is queued or running.
+std::vector<Req*> pnt_server_s::admit() {
+  std::vector<Req*> out;
+  std::vector<int64_t> seen;
+  for (auto it = queue.begin(); it != queue.end();) {
+    Req* q = *it;
+    const int64_t id = q->r.canvas_id;
+    if (std::find(seen.begin(), seen.end(), id) != seen.end()) {
+      ++it;
+      continue;
+    }
+    seen.push_back(id);
+    if (!q->planned) {
+      ++it;
+      continue;
+    }
+    if (q->status != PNT_OK) {  // planning failed
+      it = queue.erase(it);
+      if (--pending[id] == 0) pending.erase(id);
+      finish(q, q->status, q->error);
+      continue;
+    }
+    if (free_slots.empty()) break;
+    auto c = canvases.find(id);
+    if (c != canvases.end()) {
+      Resident& R = c->second;
+      if (R.rows != q->r.rows || R.cols != q->r.cols) {
+        it = queue.erase(it);
+        if (--pending[id] == 0) pending.erase(id);
+        finish(q, PNT_EINVAL, "request size differs from its resident canvas");
+        continue;
+      }
+      if (R.active || R.where == kSpilling || R.where == kRestoring) {
+        ++it;
+        continue;
+      }
+      if (R.where == kSpilled) {  // bring it back first
+        float* blk = nullptr;
+        int64_t cap = 0;
+        if (reserve(canvas_floats(R.rows, R.cols), &blk, &cap)) {
+          if (!blk) blk = dev_alloc(cap);
+          R.block = blk;
+          R.cap = cap;
+          R.where = kRestoring;
+          copyq.push_back(CopyJob{2, nullptr, id});
+          cv_copy.notify_all();
+        }
+        ++it;
+        continue;
+      }
+    } else {
+      float* blk = nullptr;
+      int64_t cap = 0;
+      if (!reserve(canvas_floats(q->r.rows, q->r.cols), &blk, &cap)) {
+        ++it;
+        continue;
+      }
+      Resident nr;
+      nr.block = blk;
+      nr.cap = cap;
+      nr.rows = q->r.rows;
+      nr.cols = q->r.cols;
+      nr.stride = round_up((int64_t)q->r.rows * q->r.cols, 64);
+      for (int k = 0; k < 3; ++k) nr.bg[k] = q->r.background[k];
+      c = canvases.emplace(id, std::move(nr)).first;
+    }
+    Resident& R = c->second;
+    R.active++;
+    q->res = &R;
+    q->fresh = !R.initialised;
+    R.initialised = true;
+    q->slot = free_slots.back();
+    free_slots.pop_back();
+    slots[(size_t)q->slot].req = q;
+    if (--pending[id] == 0) pending.erase(id);
+    it = queue.erase(it);
+    out.push_back(q);
+  }
+  return out;
+}
+
+// Device memory and the plan upload of an admitted request (engine thread).
+void pnt_server_s::setup(Req* q) {
+  SlotRes& S = slots[(size_t)q->slot];
+  Resident& R = *q->res;
+  hipStream_t s = ctx->stream;
+  if (!R.block) R.block = dev_alloc(R.cap);
+  auto grow_f = [&](float*& p, int64_t& cap, int64_t need) {
+    if (need <= cap) return;
+    dev_free(p);
+    p = nullptr;
+    cap = std::max(need, 2 * cap);
+    p = dev_alloc(cap);
+  };
+  const bool snap = q->r.use_snapshot != 0;
+  if (snap) {
+    grow_f(S.snap, S.snap_cap, 7 * R.stride);
+    const int64_t need = dirty_bytes_for((int64_t)R.rows * R.cols);
+    if (need > S.dirty_cap) {
+      dev_free(S.dirty);
+      S.dirty = nullptr;
+      S.dirty_cap = std::max(need, 2 * S.dirty_cap);
+      PNT_HIP_CHECK(hipMalloc((void**)&S.dirty, (size_t)S.dirty_cap));
+    }
+  }
+  grow_f(S.pick, S.pick_cap, kChainsPerReq * 7 * q->mstride);
+  if (q->plan_bytes > S.plan_cap) {
+    dev_free(S.plan);
+    S.plan = nullptr;
+    S.plan_cap = std::max(q->plan_bytes, 2 * S.plan_cap);
+    PNT_HIP_CHECK(hipMalloc((void**)&S.plan, (size_t)S.plan_cap));
+  }
+  if (q->plan_bytes > S.stage_cap) {
+    if (S.stage) PNT_HIP_CHECK(hipHostFree(S.stage));
+    S.stage = nullptr;
+    S.stage_cap = std::max(q->plan_bytes, 2 * S.stage_cap);
+    PNT_HIP_CHECK(hipHostMalloc((void**)&S.stage, (size_t)S.stage_cap));
+  }
+  // the slot's previous request completed (its fin event passed), so the
+  // staging buffer and device plan are free to rewrite
+  int64_t o = 0;
+  auto put = [&](const void* src, size_t bytes) {
+    const int64_t at = o;
+    if (bytes) std::memcpy(S.stage + at, src, bytes);
+    o += round_up((int64_t)bytes, 256);
+    return at;
+  };
+  const int64_t o_st = put(q->st.data(), q->st.size() * sizeof(PStamp));
+  const int64_t o_dl = put(q->dl.data(), q->dl.size() * sizeof(int2));
+  const int64_t o_off = put(q->off.data(), q->off.size() * sizeof(int));
+  const int64_t o_fp = put(q->fp.data(), q->fp.size() * sizeof(float));
+  if (o) PNT_HIP_CHECK(hipMemcpyAsync(S.plan, S.stage, (size_t)o, hipMemcpyHostToDevice, s));
+  q->d_st = reinterpret_cast<const PStamp*>(S.plan + o_st);
+  q->d_dl = reinterpret_cast<const int2*>(S.plan + o_dl);
+  q->d_off = reinterpret_cast<const int*>(S.plan + o_off);
+  q->d_fp = reinterpret_cast<const float*>(S.plan + o_fp);
+}
+
+// ---------------------------------------------------------------- waves --
+// Adds the request's ready phases to the wave being built (engine thread).
+// A unit's phase is ready when its previous phase, its chain's previous unit
+// and the units it depends on were issued in earlier waves.
+bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
+                           std::vector<std::pair<Req*, int>>& used) {
+  const SlotRes& SR = slots[(size_t)q->slot];
+  const Resident& R = *q->res;
+  const bool snap = q->r.use_snapshot != 0;
+  const int64_t n = (int64_t)R.rows * R.cols;
+  auto base = [&](int kind) -> SSlot& {
+    SSlot& S = A.slot[A.nslots];
+    S = SSlot{};
+    S.canvas = R.block;
+    S.cstride = (int)R.stride;
+    S.snap = snap ? SR.snap : R.block;
+    S.dirty = snap ? SR.dirty : nullptr;
+    S.dirty_bytes = snap ? (int)dirty_bytes_for(n) : 0;
+    for (int k = 0; k < 3; ++k) S.out[k] = R.block + (11 + k) * R.stride;
+    S.rows = R.rows;
+    S.cols = R.cols;
+    S.kind = kind;
+    S.start = start;
+    S.pickup_rate = q->r.pickup_rate;
+    S.dep_rate = q->r.deposition_rate;
+    S.cap = 1.0f;  // FootprintBrush.hxx:483
+    S.use_snap = snap ? 1 : 0;
+    for (int k = 0; k < 3; ++k) S.bg[k] = R.bg[k];
+    S.st = q->d_st;
+    S.fp = q->d_fp;
+    S.dl = q->d_dl;
+    S.dl_off = q->d_off;
+    S.mstride = (int)q->mstride;
+    S.hr = q->hr;
+    S.size_map = q->size_map;
+    S.tiles_x = q->tiles_x;
+    S.ntiles = q->ntiles;
+    for (int k = 0; k < 4; ++k) S.fb[k] = q->fbox[k];
+    return S;
+  };
+  auto push = [&](SSlot& S, int unit) {
+    start += (std::max(S.nblk, 1) + 7) / 8 * 8;  // slots start on a multiple of 8 blocks
+    A.nslots++;
+    used.push_back({q, unit});
+  };
+  bool any = false;
+  for (;;) {  // stage changes that need no phase of their own
+    if (q->stage == kPrologue && !q->fresh && !snap) {  // an existing canvas, no snapshot
+      q->stage = q->U.empty() ? kComposite : kUnits;
+      continue;
+    }
+    if (q->stage == kUnits) {
+      bool all = true;
+      for (size_t i = q->g0; i <= q->g1 && all; ++i) all = q->udone[i] != 0;
+      if (!all) break;
+      if (q->U[q->g1].dry_after) {
+        q->stage = kDrying;
+        break;
+      }
+      q->g0 = q->g1 + 1;
+      if (q->g0 >= q->U.size()) {
+        q->stage = kComposite;
+        break;
+      }
+      q->g1 = q->g0;
+      while (q->g1 + 1 < q->U.size() && !q->U[q->g1].dry_after) ++q->g1;
+      continue;
+    }
+    break;
+  }
+  if (A.nslots == kSlots) return false;
+  switch (q->stage) {
+    case kPrologue: {
+      if (q->fresh) {
+        SSlot& S = base(kSInit);
+        S.nblk = stream_blocks(11 * R.stride / 4);
+        push(S, -1);
+        return true;
+      }
+      SSlot& S = base(kSSnap);
+      S.nblk = stream_blocks(7 * R.stride / 4);
+      push(S, -1);
+      return true;
+    }
+    case kUnits: {
+      for (size_t i = q->g0; i <= q->g1 && A.nslots < kSlots; ++i) {
+        if (q->udone[i]) continue;
+        bool ok = q->prev_on_chain[i] < 0 || q->udone[(size_t)q->prev_on_chain[i]];
+        for (int d : q->U[i].deps) ok = ok && q->udone[(size_t)d];
+        if (!ok) continue;
+        const Phase& f = q->ph[i][q->next[i]];
+        SSlot& S = base(f.kind);
+        S.st = q->d_st + f.k;
+        S.pick = SR.pick + (int64_t)q->U[i].chain * 7 * q->mstride;
+        for (int c = 0; c < 3; ++c) {
+          S.pk[c] = q->paint[6 * (size_t)f.k + c];
+          S.ps[c] = q->paint[6 * (size_t)f.k + 3 + c];
+        }
+        const int rb = q->rb[(size_t)f.k];
+        switch (f.kind) {
+          case kSClear:
+            S.nblk = (int)std::min<int64_t>(64, (7 * q->mstride + 4 * kT2 - 1) / (4 * kT2));
+            break;
+          case kSStamp: S.rb = rb; S.nblk = q->ntiles + rb; break;
+          case kSRing: S.rb = rb; S.ntiles = 0; S.nblk = rb; break;
+          case kSWindow: S.rb = 0; S.nblk = q->ntiles; break;
+          default: S.nblk = 1; break;  // kSLevels
+        }
+        if (S.nblk == 0) S.nblk = 1;  // an empty ring: the block returns at once
+        push(S, (int)i);
+        any = true;
+      }
+      return any;
+    }
+    case kDrying: {
+      SSlot& S = base(kSDry);
+      S.nblk = stream_blocks(n);
+      push(S, -2);
+      return true;
+    }
+    case kComposite: {
+      SSlot& S = base(kSCompose);
+      S.nblk = stream_blocks(n);
+      push(S, -3);
+      return true;
+    }
+    default:
+      return false;
+  }
+}
+
+// Advances the requests that took part in the wave just launched.
+void pnt_server_s::after_wave(const std::vector<std::pair<Req*, int>>& used) {
+  for (const auto& u : used) {
+    Req* q = u.first;
+    if (u.second >= 0) {
+      const size_t i = (size_t)u.second;
+      if (++q->next[i] == q->ph[i].size()) q->udone[i] = 1;
+    } else if (u.second == -1) {  // prologue issued
+      q->stage = q->U.empty() ? kComposite : kUnits;
+    } else if (u.second == -2) {  // dry issued: the next drying phase
+      q->g0 = q->g1 + 1;
+      if (q->g0 >= q->U.size()) {
+        q->stage = kComposite;
+      } else {
+        q->g1 = q->g0;
+        while (q->g1 + 1 < q->U.size() && !q->U[q->g1].dry_after) ++q->g1;
+        q->stage = kUnits;
+      }
+    } else if (u.second == -3) {  // composite issued: the request's last phase
+      q->stage = kIssued;
+      PNT_HIP_CHECK(hipEventRecord(slots[(size_t)q->slot].fin, ctx->stream));
+    }
+  }
+}
+
+// Marks a request done (caller holds mu).
+void pnt_server_s::finish(Req* q, int status, const std::string& err) {
+  q->status = status;
+  q->error = err;
+  q->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - q->t_submit).count();
+  q->done = true;
+  ++completed;
+  cv_done.notify_all();
+}
+
+// The device finished a request (engine thread): its slot is free; the
+// composite goes to the caller through the copier, or the request is done.
+void pnt_server_s::complete(Req* q) {
+  std::lock_guard<std::mutex> lk(mu);
+  free_slots.push_back(q->slot);
+  slots[(size_t)q->slot].req = nullptr;
+  if (q->r.out) {
+    copyq.push_back(CopyJob{0, q, q->r.canvas_id});
+    cv_copy.notify_all();
+    return;
+  }
+  q->res->active--;
+  q->res->last_use = Clock::now();
+  finish(q, PNT_OK, "");
+}
+
+void pnt_server_s::run_engine() {
+  (void)hipSetDevice(device);
+  hipStream_t s = ctx->stream;
+  SArgs A;
+  std::vector<std::pair<Req*, int>> used;
+  for (;;) {
+    std::vector<Req*> fresh;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      for (;;) {
+        if (stopping && queue.empty() && active.empty() && planq.empty()) return;
+        fresh = admit();
+        if (!fresh.empty() || !active.empty()) break;
+        cv_engine.wait_for(lk, std::chrono::milliseconds(2));
+      }
+    }
+    for (Req* q : fresh) {
+      const int st = guard_call([&] { setup(q); });
+      if (st != PNT_OK) {
+        const std::string e = pnt_last_error();
+        std::lock_guard<std::mutex> lk(mu);
+        free_slots.push_back(q->slot);
+        slots[(size_t)q->slot].req = nullptr;
+        q->res->active--;
+        finish(q, st, e);
+        continue;
+      }
+      active.push_back(q);
+    }
+    // one wave from the ready phases, oldest request first
+    A.nslots = 0;
+    int start = 0;
+    used.clear();
+    for (Req* q : active) {
+      if (A.nslots == kSlots) break;
+      if (q->stage == kIssued) continue;
+      collect(q, A, start, used);
+    }
+    if (A.nslots > 0) {
+      if (waves >= kAhead)  // at most kAhead waves queued ahead of the host
+        PNT_HIP_CHECK(hipEventSynchronize(wave_ev[(waves - kAhead) % (kAhead + 1)]));
+      k_serve_wave<<<(unsigned)start, kT2, 0, s>>>(A);
+      PNT_HIP_CHECK(hipGetLastError());
+      PNT_HIP_CHECK(hipEventRecord(wave_ev[waves % (kAhead + 1)], s));
+      ++waves;
+      after_wave(used);
+    }
+    // requests whose device work is done
+    bool progressed = A.nslots > 0;
+    for (size_t i = 0; i < active.size();) {
+      Req* q = active[i];
+      if (q->stage == kIssued && hipEventQuery(slots[(size_t)q->slot].fin) == hipSuccess) {
+        complete(q);
+        active.erase(active.begin() + (std::ptrdiff_t)i);
+        progressed = true;
+        continue;
+      }
+      ++i;
+    }
+    if (!progressed) {  // everything issued is still running: wait a little
+      std::unique_lock<std::mutex> lk(mu);
+      cv_engine.wait_for(lk, std::chrono::microseconds(50));
+    }
+  }
+}
+
+// ---------------------------------------------------------------- copier --
+void pnt_server_s::copy_job(const CopyJob& j) {
+  hipStream_t s = cctx->stream;
+  if (j.kind == 0) {  // the composite to the caller's buffer
+    Req* q = j.q;
+    const Resident& R = *q->res;
+    const int64_t n = (int64_t)R.rows * R.cols;
+    for (int c = 0; c < 3; ++c)
+      PNT_HIP_CHECK(hipMemcpyAsync(q->r.out + c * n, R.block + (11 + c) * R.stride,
+                                   (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+    return;
+  }
+  Resident* R = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    R = &canvases.find(j.canvas_id)->second;
+  }
+  const int64_t floats = 11 * R->stride;
+  if (j.kind == 1) {  // spill to host memory
+    R->host.resize((size_t)floats);
+    PNT_HIP_CHECK(hipMemcpyAsync(R->host.data(), R->block, (size_t)floats * sizeof(float),
+                                 hipMemcpyDeviceToHost, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+  } else {  // restore from host memory
+    PNT_HIP_CHECK(hipMemcpyAsync(R->block, R->host.data(), (size_t)floats * sizeof(float),
+                                 hipMemcpyHostToDevice, s));
+    PNT_HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+void pnt_server_s::run_copier() {
+  (void)hipSetDevice(device);
+  for (;;) {
+    CopyJob j;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_copy.wait(lk, [&] { return stopping || !copyq.empty(); });
+      if (copyq.empty()) return;
+      j = copyq.front();
+      copyq.pop_front();
+    }
+    const int st = guard_call([&] { copy_job(j); });
+    const std::string err = st == PNT_OK ? std::string() : std::string(pnt_last_error());
+    std::lock_guard<std::mutex> lk(mu);
+    if (j.kind == 0) {
+      j.q->res->active--;
+      j.q->res->last_use = Clock::now();
+      finish(j.q, st, err);
+    } else {
+      Resident& R = canvases.find(j.canvas_id)->second;
+      if (j.kind == 1) {
+        --spills;
+        if (R.view) {
+          pnt_canvas_destroy(R.view);
+          R.view = nullptr;
+        }
+        if (st == PNT_OK) {
+          pool.emplace(R.cap, R.block);
+          R.block = nullptr;
+          R.where = kSpilled;
+        } else {
+          R.where = kResident;  // keep it; admission may try another
+          std::vector<float>().swap(R.host);
+        }
+      } else {
+        R.where = kResident;
+        if (st == PNT_OK) {
+          std::vector<float>().swap(R.host);
+        } else {  // the state could not come back: the block stays empty
+          pool.emplace(R.cap, R.block);
+          R.block = nullptr;
+          R.where = kSpilled;
+        }
+      }
+    }
+    cv_engine.notify_all();
   }
 }
 
@@ -277,24 +849,30 @@ int pnt_server_create(int device, int n_slots, int64_t hbm_budget_bytes, pnt_ser
     S->device = device;
     S->budget = hbm_budget_bytes;
     try {
-      for (int i = 0; i < n_slots; ++i) {
-        pnt_ctx c = nullptr;
-        int st = pnt_ctx_create(device, &c);
+      for (pnt_ctx* c : {&S->ctx, &S->qctx, &S->cctx}) {
+        const int st = pnt_ctx_create(device, c);
         if (st != PNT_OK) throw Error(st, pnt_last_error());
-        S->ctxs.push_back(c);
-        const float ones[9] = {1, 1, 1, 1, 1, 1, 1, 1, 1};
-        pnt_brush b = nullptr;
-        st = pnt_brush_create(c, 1.0, ones, 3, &b);  // placeholder geometry, set per request
-        if (st != PNT_OK) throw Error(st, pnt_last_error());
-        S->brushes.push_back(b);
       }
+      S->slots.resize((size_t)n_slots);
+      for (int i = n_slots - 1; i >= 0; --i) S->free_slots.push_back(i);
+      for (SlotRes& r : S->slots)
+        PNT_HIP_CHECK(hipEventCreateWithFlags(&r.fin, hipEventDisableTiming));
+      for (hipEvent_t& e : S->wave_ev)
+        PNT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     } catch (...) {
-      for (auto b : S->brushes) pnt_brush_destroy(b);
-      for (auto c : S->ctxs) pnt_ctx_destroy(c);
+      for (SlotRes& r : S->slots)
+        if (r.fin) (void)hipEventDestroy(r.fin);
+      for (hipEvent_t e : S->wave_ev)
+        if (e) (void)hipEventDestroy(e);
+      for (pnt_ctx c : {S->ctx, S->qctx, S->cctx})
+        if (c) pnt_ctx_destroy(c);
       delete S;
       throw;
     }
-    for (int i = 0; i < n_slots; ++i) S->workers.emplace_back([S, i] { S->worker(i); });
+    S->engine = std::thread([S] { S->run_engine(); });
+    S->copier = std::thread([S] { S->run_copier(); });
+    const int np = (int)std::min(4u, std::max(1u, std::thread::hardware_concurrency() / 4));
+    for (int i = 0; i < np; ++i) S->planners.emplace_back([S] { S->planner(); });
     *out = S;
   });
 }
@@ -317,15 +895,23 @@ int pnt_server_submit(pnt_server S, const pnt_request* r, int64_t* ticket) {
     q->r.footprint = nullptr;
     q->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(S->mu);
-    PNT_REQUIRE(!S->stopping, "server is shutting down");
-    q->ticket = S->next_ticket++;
+    if (S->stopping) {
+      delete q;
+      throw Error(PNT_ESTATE, "server is shutting down");
+    }
+    q->ticket = S->ticket_seq++;
     S->tickets[q->ticket] = q;
     S->pending[r->canvas_id]++;
     S->queue.push_back(q);
+    S->planq.push_back(q);
     *ticket = q->ticket;
-    S->cv_work.notify_one();
+    S->cv_plan.notify_one();
   });
 }
+
+}  // extern "C"
+
+extern "C" {
 
 int pnt_server_wait(pnt_server S, int64_t ticket, double* latency_ms) {
   return guard_call([&] {
@@ -352,13 +938,14 @@ int pnt_server_canvas(pnt_server S, int64_t canvas_id, pnt_canvas* canvas, pnt_d
     PNT_REQUIRE(S && canvas, "NULL argument");
     std::lock_guard<std::mutex> lk(S->mu);
     auto c = S->canvases.find(canvas_id);
-    PNT_REQUIRE(c != S->canvases.end() && c->second.cv, "canvas is not resident");
-    PNT_REQUIRE(!c->second.busy, "canvas has a running request");
-    *canvas = c->second.cv;
-    if (composite) {
-      const int64_t st = c->second.cv->stride;
-      for (int k = 0; k < 3; ++k) composite->p[k] = c->second.block + (11 + k) * st;
-    }
+    PNT_REQUIRE(c != S->canvases.end(), "unknown canvas");
+    Resident& R = c->second;
+    PNT_REQUIRE(R.where == kResident && R.block, "canvas is not resident (spilled to the host)");
+    PNT_REQUIRE(!R.active && !S->pending.count(canvas_id), "canvas has a queued or running request");
+    if (!R.view) R.view = canvas_over(S->qctx, R.rows, R.cols, R.block, false);
+    *canvas = R.view;
+    if (composite)
+      for (int k = 0; k < 3; ++k) composite->p[k] = R.block + (11 + k) * R.stride;
   });
 }
 
@@ -367,12 +954,14 @@ int pnt_server_stats(pnt_server S, int64_t* resident_bytes, int64_t* resident_ca
   return guard_call([&] {
     PNT_REQUIRE(S, "NULL server");
     std::lock_guard<std::mutex> lk(S->mu);
-    if (resident_bytes) {
-      int64_t b = 0;
-      for (auto& kv : S->canvases) b += kv.second.cap * 4;
-      *resident_bytes = b;
-    }
-    if (resident_canvases) *resident_canvases = (int64_t)S->canvases.size();
+    int64_t b = 0, c = 0;
+    for (auto& kv : S->canvases)
+      if (kv.second.block) {
+        b += kv.second.cap * 4;
+        ++c;
+      }
+    if (resident_bytes) *resident_bytes = b;
+    if (resident_canvases) *resident_canvases = c;
     if (completed) *completed = S->completed;
   });
 }
@@ -382,23 +971,40 @@ int pnt_server_destroy(pnt_server S) {
     if (!S) return;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      S->cv_done.wait(lk, [&] {  // let queued work finish
+      S->cv_done.wait(lk, [&] {  // let submitted work finish
         for (auto& kv : S->tickets)
           if (!kv.second->done) return false;
         return S->queue.empty();
       });
       S->stopping = true;
     }
-    S->cv_work.notify_all();
-    for (auto& t : S->workers) t.join();
+    S->cv_engine.notify_all();
+    S->cv_plan.notify_all();
+    S->cv_copy.notify_all();
+    S->engine.join();
+    for (auto& t : S->planners) t.join();
+    S->copier.join();
+    (void)hipSetDevice(S->device);
+    (void)hipStreamSynchronize(S->ctx->stream);
     for (auto& kv : S->tickets) delete kv.second;
     for (auto& kv : S->canvases) {
-      if (kv.second.cv) pnt_canvas_destroy(kv.second.cv);
+      if (kv.second.view) pnt_canvas_destroy(kv.second.view);
       dev_free(kv.second.block);
     }
     for (auto& kv : S->pool) dev_free(kv.second);
-    for (auto b : S->brushes) pnt_brush_destroy(b);
-    for (auto c : S->ctxs) pnt_ctx_destroy(c);
+    for (SlotRes& r : S->slots) {
+      dev_free(r.snap);
+      dev_free(r.dirty);
+      dev_free(r.pick);
+      dev_free(r.plan);
+      if (r.stage) (void)hipHostFree(r.stage);
+      if (r.fin) (void)hipEventDestroy(r.fin);
+    }
+    for (hipEvent_t e : S->wave_ev)
+      if (e) (void)hipEventDestroy(e);
+    pnt_ctx_destroy(S->ctx);
+    pnt_ctx_destroy(S->qctx);
+    pnt_ctx_destroy(S->cctx);
     delete S;
   });
 }

@@ -1,0 +1,206 @@
+// One phase of a "wave": a single launch that advances many independent
+// stamp chains at once, each on its own range of blocks (serve.hip runs a
+// server's admitted requests this way, batch.hip a batch of canvases).
+//
+// A phase is one step of one chain: a canvas initialisation, a full
+// snapshot copy (updateSnapshot(canvas), FootprintBrush.hxx:168-172), a
+// pickup-map clear (dip()/clean(), :150-160), one imprint (:73-143) -- its
+// window tiles and snapshot ring scan side by side -- or the ring / window /
+// shared-pixel-levels parts of an imprint that straddles canvas column/row
+// 0, a dryCanvas (Canvas.hxx:105-121), a composite (Renderer.hxx:26-41), or
+// a pickup-map copy.  Phases of one wave touch disjoint memory; the stream
+// orders the waves.  Every kind runs the same visit and KM arithmetic as the
+// single-canvas kernels, so a chain gives the same bits as run alone.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "internal.hpp"
+#include "km.hpp"
+#include "stampcore.hpp"
+
+namespace pnt {
+
+constexpr int kWT = 16;          // window tile side of wave phases
+constexpr int kWT2 = kWT * kWT;  // threads per block
+
+enum : int {
+  kSInit = 0,  // canvas planes: no paint, R0 = bg, h = 0 (and the snapshot of it)
+  kSSnap,      // snapshot = canvas, dirty bytes 0
+  kSClear,     // pickup map = 0
+  kSStamp,     // window tiles + ring scan of one imprint
+  kSRing,      // ring scan of a split imprint
+  kSWindow,    // window tiles of a split imprint
+  kSLevels,    // shared-pixel visits of a split imprint, by level (one block)
+  kSDry,       // dryCanvas, marking the snapshot dirty bytes it changes
+  kSCompose,   // composite of the wet layer over R0 into out
+  kSCopyPick,  // pickup map -> pick_dst
+};
+
+// Canvas planes: K0..2 S0..2 V R0_0..2 h, cstride floats each.
+struct SSlot {
+  const PStamp* st;
+  float* canvas;
+  float* snap;      // 7 planes of cstride (the canvas itself when the snapshot is off)
+  uint8_t* dirty;   // snapshot dirty bytes (dirty_bytes of them)
+  float* pick;      // the chain's pickup map: 7 planes of mstride
+  float* pick_dst;  // kSCopyPick target
+  const float* fp;  // padded footprint, size_map^2
+  const int2* dl;   // shared-pixel visit lists
+  const int* dl_off;
+  float* out[3];    // composite planes
+  int cstride, mstride, dirty_bytes;
+  int rows, cols;
+  int kind, start, nblk, ntiles, rb, hr, size_map, tiles_x;
+  int fb[4];
+  float pk[3], ps[3];
+  float pickup_rate, dep_rate, cap;
+  int use_snap;
+  float bg[3];
+};
+
+__device__ __forceinline__ void wave_fill4(float* p, int64_t n4, float v, int tid, int nth) {
+  const float4 x = make_float4(v, v, v, v);
+  for (int64_t q = tid; q < n4; q += nth) reinterpret_cast<float4*>(p)[q] = x;
+}
+
+// Canvas::dryCanvas (Canvas.hxx:105-121) of pixel i; marks the snapshot
+// dirty byte of a pixel it changes (when dirty is given).
+__device__ __forceinline__ void wave_dry_pixel(const Layer7& L, float* R0, float* h, int64_t cs,
+                                               uint8_t* dirty, int64_t i) {
+  float K[3], S[3], R[3];
+  bool nz = false;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    K[c] = L.p[c][i];
+    S[c] = L.p[3 + c][i];
+    R[c] = R0[c * cs + i];
+    nz |= (K[c] != 0.0f) | (S[c] != 0.0f);
+  }
+  const float v = L.p[6][i];
+  nz |= v != 0.0f;
+  if (!nz) return;  // drying an empty pixel changes nothing
+  h[i] += v;
+  km_pixel(K, S, R, v);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    R0[c * cs + i] = R[c];
+    L.p[c][i] = 0.0f;
+    L.p[3 + c][i] = 0.0f;
+  }
+  L.p[6][i] = 0.0f;
+  if (dirty) dirty[i] = 1;
+}
+
+// Renderer::compose (Renderer.hxx:26-41) of pixel i.
+__device__ __forceinline__ void wave_compose_pixel(const Layer7& L, const float* R0,
+                                                   float* const out[3], int64_t cs, int64_t i) {
+  float K[3], S[3], R[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    K[c] = L.p[c][i];
+    S[c] = L.p[3 + c][i];
+    R[c] = R0[c * cs + i];
+  }
+  km_pixel(K, S, R, L.p[6][i]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[c][i] = R[c];
+}
+
+// Block b (of S.nblk) of phase S; all kWT2 threads of the block call it.
+__device__ __forceinline__ void run_wave_slot(const SSlot& S, int b, int (*sm)[kWT + 2]) {
+  const int tid = b * kWT2 + (int)threadIdx.x, nth = S.nblk * kWT2;
+  const int64_t cs = S.cstride;
+  Layer7 canvas, snap;
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    canvas.p[c] = S.canvas + c * cs;
+    snap.p[c] = S.snap + c * cs;
+  }
+  float* R0 = S.canvas + 7 * cs;
+  const int64_t n = (int64_t)S.rows * S.cols;
+  switch (S.kind) {
+    case kSInit: {
+      for (int p = 0; p < 11; ++p)
+        wave_fill4(S.canvas + p * cs, cs / 4, (p >= 7 && p < 10) ? S.bg[p - 7] : 0.0f, tid, nth);
+      if (S.use_snap) {  // a fresh brush's snapshot of the new canvas
+        for (int p = 0; p < 7; ++p) wave_fill4(S.snap + p * cs, cs / 4, 0.0f, tid, nth);
+        for (int q = tid; q < S.dirty_bytes / 16; q += nth)
+          reinterpret_cast<uint4*>(S.dirty)[q] = make_uint4(0u, 0u, 0u, 0u);
+      }
+      return;
+    }
+    case kSSnap: {
+      for (int p = 0; p < 7; ++p)
+        for (int64_t q = tid; q < cs / 4; q += nth)
+          reinterpret_cast<float4*>(S.snap + p * cs)[q] =
+              reinterpret_cast<const float4*>(S.canvas + p * cs)[q];
+      for (int q = tid; q < S.dirty_bytes / 16; q += nth)
+        reinterpret_cast<uint4*>(S.dirty)[q] = make_uint4(0u, 0u, 0u, 0u);
+      return;
+    }
+    case kSClear:
+      wave_fill4(S.pick, 7 * (int64_t)S.mstride / 4, 0.0f, tid, nth);
+      return;
+    case kSCopyPick:
+      for (int64_t q = tid; q < 7 * (int64_t)S.mstride / 4; q += nth)
+        reinterpret_cast<float4*>(S.pick_dst)[q] = reinterpret_cast<const float4*>(S.pick)[q];
+      return;
+    case kSDry:
+      for (int64_t i = tid; i < n; i += nth)
+        wave_dry_pixel(canvas, R0, S.canvas + 10 * cs, cs, S.use_snap ? S.dirty : nullptr, i);
+      return;
+    case kSCompose:
+      for (int64_t i = tid; i < n; i += nth) wave_compose_pixel(canvas, R0, S.out, cs, i);
+      return;
+    default:
+      break;
+  }
+  const PStamp& st = *S.st;
+  StampGeom g;
+  g.cx = st.cx;
+  g.cy = st.cy;
+  g.c = st.c;
+  g.s = st.s;
+  g.hr = S.hr;
+  g.size_map = S.size_map;
+  g.rows = S.rows;
+  g.cols = S.cols;
+  const bool dup = (st.flags & (kStDupX | kStDupY)) != 0;
+  const bool pull = S.use_snap && !dup;
+  const VisitTarget T{&canvas, &snap, S.dirty, S.use_snap, S.pickup_rate, S.dep_rate, S.cap};
+  float* pick[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) pick[c] = S.pick + c * (int64_t)S.mstride;
+  const FpBox fb{S.fb[0], S.fb[1], S.fb[2], S.fb[3]};
+  const int win = (S.kind == kSStamp || S.kind == kSWindow) ? S.ntiles : 0;
+  if (S.kind == kSLevels) {  // one block
+    for (int L = 0; L < st.dl_nlev; ++L) {
+      const int e0 = S.dl_off[st.dl_lev + L], e1 = S.dl_off[st.dl_lev + L + 1];
+      for (int e = e0 + (int)threadIdx.x; e < e1; e += kWT2) {
+        const int2 v = S.dl[e];
+        int64_t ic;
+        const int m = stamp_visit(g, v.x, v.y, &ic);
+        if (m >= 0 && S.fp[m] > 0.0f)
+          cached_visit(T, GlobalPick{pick, m}, S.fp[m], ic, false, S.pk, S.ps);
+      }
+      __builtin_amdgcn_s_waitcnt(0);  // this level's stores before the next level's loads
+      __syncthreads();
+    }
+  } else if (b < win) {
+    window_tile<kWT>(T, pick, S.fp, fb, g, st, dup, pull, tile_of_block(b, S.ntiles), S.tiles_x,
+                     sm, S.pk, S.ps);
+  } else if (b < win + S.rb) {
+    ring_scan<2>(canvas, snap, S.dirty, S.fp, g, st.box, pull, (b - win) * kWT2 + (int)threadIdx.x,
+                 S.rb * kWT2);
+  }
+}
+
+// Blocks of a streaming phase over `items` work items (at most 2048).
+inline int wave_stream_blocks(int64_t items) {
+  const int64_t b = (items + kWT2 - 1) / kWT2;
+  return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+}  // namespace pnt

@@ -35,9 +35,11 @@ def _planes(*tensors):
     return out
 
 
-class GradRunner:
+class GradRunner(_lib.Owned):
     """One differentiable stamp list (pnt_grad).  Holds the checkpoints of its
     last forward; backward applies to that forward."""
+
+    _kind, _destroy = "grad", "pnt_grad_destroy"
 
     def __init__(self, rows: int, cols: int, brush: H.FootprintBrush, stamps, paint_of, palK,
                  palS, n_paint: int, seg_len: int = 32, ctx: H.Context | None = None):
@@ -54,12 +56,7 @@ class GradRunner:
                   po.ctypes.data_as(C.POINTER(C.c_int32)), self.n_pig,
                   self.palK.ctypes.data_as(_lib._dp), self.palS.ctypes.data_as(_lib._dp),
                   self.n_paint, int(seg_len), C.byref(h))
-        self.h = h
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            _lib.load().pnt_grad_destroy(self.h)
-            self.h = None
+        self._own(h)
 
     def forward(self, K, S, V, R0, W) -> torch.Tensor:
         rc = (self.rows, self.cols)

@@ -23,7 +23,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import AOS, PLANAR, InvalidArgument, PaintyError, PntStamp, call, load
+from ._lib import AOS, PLANAR, InvalidArgument, Owned, PaintyError, PntStamp, call, load
 
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
@@ -40,13 +40,15 @@ def _pf(a: np.ndarray):
     return a.ctypes.data_as(_fp) if a is not None else None
 
 
-class Context:
+class Context(Owned):
     """A device and one HIP stream on it."""
+
+    _kind, _destroy = "ctx", "pnt_ctx_destroy"
 
     def __init__(self, device: int = 0):
         h = C.c_void_p()
         call("pnt_ctx_create", device, C.byref(h))
-        self.h = h
+        self._own(h)
         self.device = device
 
     def synchronize(self) -> None:
@@ -55,17 +57,6 @@ class Context:
     @property
     def stream(self) -> int:
         return load().pnt_ctx_stream(self.h) or 0
-
-    def close(self) -> None:
-        if getattr(self, "h", None):
-            load().pnt_ctx_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
 
 _default: Context | None = None
@@ -78,7 +69,9 @@ def default_context() -> Context:
     return _default
 
 
-class PaintLayer:
+class PaintLayer(Owned):
+    _kind, _destroy = "layer", "pnt_layer_destroy"
+
     def __init__(self, rows: int, cols: int, ctx: Context | None = None, _handle=None,
                  _parent=None):
         self.ctx = ctx or default_context()
@@ -86,18 +79,17 @@ class PaintLayer:
         if _handle is None:
             h = C.c_void_p()
             call("pnt_layer_create", self.ctx.h, rows, cols, C.byref(h))
-            self.h = h
+            self._own(h)
             self._owner = True
         else:
             self.h = _handle
             self._owner = False
 
-    def __del__(self):
-        if getattr(self, "_owner", False) and getattr(self, "h", None):
-            try:
-                load().pnt_layer_destroy(self.h)
-            except Exception:
-                pass
+    def close(self) -> None:
+        if getattr(self, "_owner", False):
+            super().close()
+        else:  # a view of a canvas's wet layer: the canvas owns it
+            self.h = None
             self.h = None
 
     def getRows(self) -> int:
@@ -167,21 +159,15 @@ class PaintLayer:
         call("pnt_layer_set", self.h, i, j, _pf(k), _pf(s), float(v))
 
 
-class Canvas:
+class Canvas(Owned):
+    _kind, _destroy = "canvas", "pnt_canvas_destroy"
+
     def __init__(self, rows: int, cols: int, ctx: Context | None = None):
         self.ctx = ctx or default_context()
         h = C.c_void_p()
         call("pnt_canvas_create", self.ctx.h, rows, cols, C.byref(h))
-        self.h = h
+        self._own(h)
         self.rows, self.cols = rows, cols
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            try:
-                load().pnt_canvas_destroy(self.h)
-            except Exception:
-                pass
-            self.h = None
 
     def clear(self) -> None:
         call("pnt_canvas_clear", self.h)
@@ -351,8 +337,10 @@ def encode(rgb, to_srgb: bool = True, bits: int = 8, ctx: Context | None = None)
     return out
 
 
-class FootprintBrush:
+class FootprintBrush(Owned):
     """Imprint brush with an explicit footprint (width = 2*ceil(radius)+1)."""
+
+    _kind, _destroy = "brush", "pnt_brush_destroy"
 
     def __init__(self, radius: float, footprint, ctx: Context | None = None):
         self.ctx = ctx or default_context()
@@ -360,15 +348,7 @@ class FootprintBrush:
         h = C.c_void_p()
         call("pnt_brush_create", self.ctx.h, float(radius), _pf(fp), int(fp.shape[0]),
              C.byref(h))
-        self.h = h
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            try:
-                load().pnt_brush_destroy(self.h)
-            except Exception:
-                pass
-            self.h = None
+        self._own(h)
 
     def setRadius(self, radius: float, footprint) -> None:
         fp = _f32(footprint)
@@ -496,28 +476,22 @@ def footprint_from_image(img, radius: float, ctx: Context | None = None) -> np.n
     return resize_lanczos4(img, w, w, ctx).astype(np.float32)
 
 
-class TextureBrush:
+class TextureBrush(Owned):
     """Brush stroke texture warped along a stroke, with smudging
     (renderer/TextureBrush.hxx:19-238).  The reference loads the sample's
     thickness map from ``sampleDir/thickness_map.png``
     (renderer/src/BrushStrokeSample.cxx:164); here the caller passes the
     map (rows x cols, float64) to ``setThicknessMap``."""
 
+    _kind, _destroy = "tbrush", "pnt_tbrush_destroy"
+
     def __init__(self, thickness_map=None, ctx: Context | None = None):
         self.ctx = ctx or default_context()
         h = C.c_void_p()
         call("pnt_tbrush_create", self.ctx.h, C.byref(h))
-        self.h = h
+        self._own(h)
         if thickness_map is not None:
             self.setThicknessMap(thickness_map)
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            try:
-                load().pnt_tbrush_destroy(self.h)
-            except Exception:
-                pass
-            self.h = None
 
     def setRadius(self, radius: float) -> None:
         call("pnt_tbrush_set_radius", self.h, float(radius))
@@ -558,24 +532,18 @@ class TextureBrush:
              int(xy.shape[0]))
 
 
-class TextureBrushGpu:
+class TextureBrushGpu(Owned):
     """The reference's GL texture brush (renderer/src/TextureBrushGpu.cxx:
     17-305) on the device; the brush texture is set by the caller
     (setTexture) instead of drawn from the texture dictionary."""
+
+    _kind, _destroy = "tbrush", "pnt_tbgl_destroy"
 
     def __init__(self, ctx: Context | None = None):
         self.ctx = ctx or default_context()
         h = C.c_void_p()
         call("pnt_tbgl_create", self.ctx.h, C.byref(h))
-        self.h = h
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            try:
-                load().pnt_tbgl_destroy(self.h)
-            except Exception:
-                pass
-            self.h = None
+        self._own(h)
 
     def setRadius(self, radius: float) -> None:
         call("pnt_tbgl_set_radius", self.h, float(radius))
@@ -667,52 +635,51 @@ def batch_wait(ctx: Context | None = None) -> None:
     call("pnt_batch_wait", (ctx or default_context()).h)
 
 
-def batch_set_config(tile: int, max_group: int, ctx: Context | None = None) -> None:
+def batch_set_config(tile: int, max_group: int, waves: int = 0, chunk: int = 1,
+                     ctx: Context | None = None) -> None:
+    """pnt_batch_set_config: waves=0 -- one persistent launch with tile x tile
+    window tiles and at most max_group workgroups per job; waves=1 -- the
+    batch as a sequence of wave launches, `chunk` strokes of a canvas in
+    flight at once."""
     ctx = ctx or default_context()
-    c = _lib.BatchConfig(tile, max_group)
+    c = _lib.BatchConfig(tile, max_group, waves, chunk)
     call("pnt_batch_set_config", ctx.h, C.byref(c))
 
 
 def batch_get_config(ctx: Context | None = None):
+    """(tile, max_group, waves, chunk) of the context's batches."""
     ctx = ctx or default_context()
     c = _lib.BatchConfig()
     call("pnt_batch_get_config", ctx.h, C.byref(c))
-    return c.tile, c.max_group
+    return c.tile, c.max_group, c.waves, c.chunk
 
 
 def batch_tune(jobs, ctx: Context | None = None):
-    """pnt_batch_tune: returns ((tile, max_group), ms) of the fastest
-    configuration, which the context keeps; the jobs' state is unchanged."""
+    """pnt_batch_tune: returns ((tile, max_group, waves, chunk), ms) of the
+    fastest configuration, which the context keeps; the jobs' state is
+    unchanged."""
     ctx = ctx or default_context()
     arr, _keep = _batch_jobs(jobs)
     c, ms = _lib.BatchConfig(), C.c_double()
     call("pnt_batch_tune", ctx.h, arr, len(jobs), C.byref(c), C.byref(ms))
-    return (c.tile, c.max_group), ms.value
+    return (c.tile, c.max_group, c.waves, c.chunk), ms.value
 
 
-class BatchGraph:
+class BatchGraph(Owned):
     """pnt_batch_capture: the batch captured once as a HIP graph;
     ``launch()`` replays it on the jobs' current state."""
+
+    _kind, _destroy = "graph", "pnt_batch_graph_destroy"
 
     def __init__(self, jobs, ctx: Context | None = None):
         self.ctx = ctx or default_context()
         arr, _keep = _batch_jobs(jobs)
         h = C.c_void_p()
         call("pnt_batch_capture", self.ctx.h, arr, len(jobs), C.byref(h))
-        self.h = h
+        self._own(h)
 
     def launch(self) -> None:
         call("pnt_batch_graph_launch", self.h)
-
-    def close(self) -> None:
-        if getattr(self, "h", None):
-            call("pnt_batch_graph_destroy", self.h)
-            self.h = None
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            _lib.load().pnt_batch_graph_destroy(self.h)
-            self.h = None
 
 
 class PaintMixer:

@@ -12,22 +12,18 @@ from painty_amd import _lib
 from painty_amd import hip as H
 
 
-class Server:
+class Server(_lib.Owned):
+    _kind, _destroy = "server", "pnt_server_destroy"
+
     def __init__(self, n_slots: int = 4, hbm_budget_bytes: int = 64 << 30, device: int = 0):
+        self._keep = {}  # ticket -> host buffers that must outlive the request
         h = C.c_void_p()
         _lib.call("pnt_server_create", device, n_slots, int(hbm_budget_bytes), C.byref(h))
-        self.h = h
-        self._keep = {}  # ticket -> host buffers that must outlive the request
+        self._own(h)
 
     def close(self) -> None:
-        if self.h:
-            _lib.call("pnt_server_destroy", self.h)
-            self.h = None
-            self._keep.clear()
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            self.close()
+        super().close()  # waits for the queued requests
+        self._keep.clear()
 
     def submit(self, canvas_id: int, rows: int, cols: int, stamps, radius: float, footprint,
                background=(1.0, 1.0, 1.0), pickup_rate: float = 0.9,

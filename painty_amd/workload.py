@@ -344,3 +344,49 @@ def diff2048_500stamps(seed: int = 4, size: int = 2048, n_stamps: int = 500, n_p
     tn = Stream(seed, 61)
     target = np.stack([0.2 + 0.7 * value_noise(size, tn, cell=128.0) for _ in range(3)], axis=-1)
     return DiffWorkload(wl, palK, palS, W, paint_of, target.astype(np.float32))
+
+
+@dataclasses.dataclass
+class ServedRequest:
+    """One request of the served configuration: stamps (starting with a dip)
+    and a composite on canvas `canvas_id` (square, `side` px), with one
+    Gaussian footprint of radius `radius`."""
+    canvas_id: int
+    side: int
+    stamps: np.ndarray
+    radius: float
+    footprint: np.ndarray
+    background: tuple = (0.85, 0.85, 0.85)
+
+
+def served_requests(n: int, seed: int, side=(512, 4096), stamps=(10, 1000), foot=(32, 128),
+                    follow: float = 0.3, recent: int = 32) -> list:
+    """The served configuration's request stream: square canvases with side
+    log-uniform in `side`, stamps U{stamps} (strokes of 20..80 stamps at 2 px
+    spacing, scaled from a seeded pool of 16 stroke lists), footprint
+    diameter U{foot}; with probability `follow` a request goes to one of the
+    `recent` last new canvases (a follow-up on a resident canvas)."""
+    rs = Stream(seed, 95)
+    pool = [strokes(1000, 1000, stamps[1], seed * 1000 + k, stamps_per_stroke=(20, 80),
+                    reach=0.15) for k in range(16)]
+    cores = {}
+    reqs, created = [], []
+    for i in range(n):
+        if created and rs.uniform(1)[0] < follow:
+            cid, s_ = created[int(rs.integers(1, max(0, len(created) - recent),
+                                              len(created) - 1)[0])]
+        else:
+            s_ = int(round(math.exp(rs.uniform(1, math.log(side[0]), math.log(side[1]))[0])))
+            cid = len(created)
+            created.append((cid, s_))
+        n_st = int(rs.integers(1, stamps[0], stamps[1])[0])
+        st = pool[i % len(pool)][:n_st].copy()
+        st["cx"] *= s_ / 1000.0
+        st["cy"] *= s_ / 1000.0
+        st["flags"][0] |= DIP
+        d = int(rs.integers(1, foot[0], foot[1])[0])
+        r = max(1, d // 2)
+        if r not in cores:
+            cores[r] = gaussian_core(2 * r + 1, (2 * r + 1) / 4.0)
+        reqs.append(ServedRequest(cid, s_, st, float(r), cores[r]))
+    return reqs

@@ -130,12 +130,16 @@ def restore_config(hip):
     hip.batch_set_config(*saved)
 
 
-@pytest.mark.parametrize("tile,max_group", [(16, 64), (16, 4), (16, 1), (32, 1), (32, 4)])
-def test_batch_configs_match_oracle(hip, oracle, restore_config, tile, max_group):
-    """Every launch shape the autotuner may pick (16- or 32-pixel window
-    tiles, one to 64 workgroups per job) gives the oracle's result."""
-    hip.batch_set_config(tile, max_group)
-    assert hip.batch_get_config() == (tile, max_group)
+@pytest.mark.parametrize("tile,max_group,waves,chunk",
+                         [(16, 64, 0, 1), (16, 4, 0, 1), (16, 1, 0, 1), (32, 1, 0, 1),
+                          (32, 4, 0, 1), (16, 1, 1, 1), (16, 1, 1, 2), (16, 1, 1, 4)])
+def test_batch_configs_match_oracle(hip, oracle, restore_config, tile, max_group, waves, chunk):
+    """Every configuration the autotuner may pick -- one persistent launch
+    with 16- or 32-pixel window tiles and one to 64 workgroups per job, or
+    waves with one, two or four strokes of a canvas in flight -- gives the
+    oracle's result."""
+    hip.batch_set_config(tile, max_group, waves, chunk)
+    assert hip.batch_get_config() == (tile, max_group, waves, chunk)
     jobs = _big_small()
     gpu = []
     for wl in jobs:
@@ -150,9 +154,10 @@ def test_batch_configs_match_oracle(hip, oracle, restore_config, tile, max_group
 
 
 def test_batch_config_rejects_bad_shapes(hip, restore_config):
-    for tile, g in [(8, 4), (64, 4), (32, 0), (16, 257)]:
+    for cfg in [(8, 4), (64, 4), (32, 0), (16, 257), (16, 4, 2, 1), (16, 4, 1, 3),
+                (16, 4, 1, 0)]:
         with pytest.raises(hip.InvalidArgument):
-            hip.batch_set_config(tile, g)
+            hip.batch_set_config(*cfg)
 
 
 def test_batch_tune_keeps_state_and_config(hip, oracle, restore_config):
@@ -164,9 +169,10 @@ def test_batch_tune_keeps_state_and_config(hip, oracle, restore_config):
     for wl, snap in jobs:
         cv, br, out = _setup(hip, wl, snap)
         gpu.append((cv, br, wl.stamps, out))
-    (tile, g), ms = hip.batch_tune(gpu)
-    assert tile in (16, 32) and 1 <= g <= 256 and ms > 0
-    assert hip.batch_get_config() == (tile, g)
+    (tile, g, waves, chunk), ms = hip.batch_tune(gpu)
+    assert tile in (16, 32) and 1 <= g <= 256 and waves in (0, 1) and chunk in (1, 2, 4)
+    assert ms > 0
+    assert hip.batch_get_config() == (tile, g, waves, chunk)
     hip.batch_run(gpu, mode=0)
     for k, ((wl, snap), (cv, br, _, out)) in enumerate(zip(jobs, gpu)):
         ocv, ob = run_oracle(wl, use_snapshot=snap)
@@ -179,10 +185,12 @@ def _layers(cv):
     return [lay.getK_buffer(), lay.getS_buffer(), lay.getV_buffer(), cv.getR0()]
 
 
-def test_batch_graph_replays_equal_eager_and_oracle(hip, oracle):
+@pytest.mark.parametrize("waves", [0, 1])
+def test_batch_graph_replays_equal_eager_and_oracle(hip, oracle, restore_config, waves):
     """A captured batch launched twice equals the eager batch run twice (to
-    the bit: same kernel, same plan) and the oracle running each stamp list
-    twice."""
+    the bit: same kernels, same plan) and the oracle running each stamp list
+    twice -- for one persistent launch and for waves."""
+    hip.batch_set_config(16, 16, waves, 4)
     jobs = _big_small(70)
     eager, graph = [], []
     for wl in jobs:
@@ -256,3 +264,38 @@ def test_batch_submit_then_wait(hip):
         np.testing.assert_array_equal(La.getK_buffer(), Lb.getK_buffer())
         np.testing.assert_array_equal(La.getV_buffer(), Lb.getV_buffer())
         np.testing.assert_array_equal(oa.getR0(), ob.getR0())
+
+
+def _setup_on(hip, wl, ctx):
+    cv = hip.Canvas(wl.rows, wl.cols, ctx)
+    cv.getPaintLayer().setBuffers(wl.wet.K, wl.wet.S, wl.wet.V)
+    cv.setR0(wl.R0)
+    br = hip.FootprintBrush(wl.radii[0], wl.cores[0], ctx)
+    br.setPickupRate(wl.pickup_rate)
+    br.setDepositionRate(wl.deposition_rate)
+    return cv, br, hip.Canvas(wl.rows, wl.cols, ctx)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_jobs_on_another_context_read_back_without_wait(hip, oracle, graph):
+    """Canvases and brushes made on a second context, the batch submitted on
+    the default one, and the results read back through the jobs' own context
+    with no pnt_batch_wait: the batch's end orders the other context's stream,
+    so the read sees the finished batch."""
+    bw = W.batch256(n_canvas=6, seed=13, sides=(200, 420), stamps=(40, 90), foot=(48, 96))
+    other = hip.Context(0)
+    jobs = [_setup_on(hip, wl, other) for wl in bw.jobs]
+    other.synchronize()
+    arg = [(cv, br, wl.stamps, out) for (cv, br, out), wl in zip(jobs, bw.jobs)]
+    if graph:
+        g = hip.BatchGraph(arg)
+        g.launch()
+    else:
+        hip.batch_submit(arg)
+    for k, ((cv, br, out), wl) in enumerate(zip(jobs, bw.jobs)):
+        ocv, ob = run_oracle(wl, bake_dry=False)
+        compare_canvas("otherctx%d/job%d" % (graph, k), cv, ocv, pickup=(br, ob))
+        assert_close("otherctx%d/job%d.out" % (graph, k), out.getR0(), ocv.compose())
+    if graph:
+        g.close()
+    hip.batch_wait()

@@ -16,7 +16,7 @@ from tests.parity import assert_close
 pytestmark = pytest.mark.gpu
 
 
-def make_requests(seed, n_canvas=6, n_req=18, sides=(48, 160), max_stamps=40):
+def make_requests(seed, n_canvas=6, n_req=18, sides=(48, 160), max_stamps=40, dry=False):
     rs = W.Stream(seed, 91)
     canvases = {}
     for cid in range(n_canvas):
@@ -30,6 +30,8 @@ def make_requests(seed, n_canvas=6, n_req=18, sides=(48, 160), max_stamps=40):
         radius = int(rs.integers(1, 2, 10)[0])
         n = int(rs.integers(1, 1, max_stamps)[0])
         st = W.strokes(rows, cols, n, seed * 100 + k, stamps_per_stroke=(3, 15), reach=0.4)
+        if dry and k % 3 == 0 and n > 4:  # dryCanvas() in the middle of the request
+            st["flags"][n // 2] |= W.DRY_AFTER
         core = W.noisy_core(2 * radius + 1, W.Stream(seed * 100 + k, 41))
         reqs.append(dict(canvas_id=cid, rows=rows, cols=cols, background=bg, stamps=st,
                          radius=float(radius), footprint=core,
@@ -59,9 +61,9 @@ def run_alone(canvases, reqs):
     return out
 
 
-@pytest.mark.parametrize("slots", [1, 4])
-def test_mixed_equals_alone(hip, slots):
-    canvases, reqs = make_requests(7)
+@pytest.mark.parametrize("slots,dry", [(1, False), (4, False), (16, True)])
+def test_mixed_equals_alone(hip, slots, dry):
+    canvases, reqs = make_requests(7, dry=dry)
     alone = run_alone(canvases, reqs)
     srv = SV.Server(n_slots=slots, hbm_budget_bytes=1 << 30)
     tickets = [srv.submit(**q) for q in reqs]  # all in flight together
@@ -107,15 +109,46 @@ def test_served_matches_oracle(hip, oracle):
     srv.close()
 
 
-def test_eviction_drops_state_and_fails_loudly(hip):
+def test_eviction_spills_and_restores(hip):
+    """With room for two canvases, a third evicts the least recently used one
+    to host memory; a later request on it restores it and continues from its
+    state: every canvas equals its requests run alone."""
     core = W.gaussian_core(9, 2.25)
-    st = W.strokes(64, 64, 10, 5, stamps_per_stroke=(5, 10), reach=0.3)
     one = 56 * 64 * 64  # canvas + composite planes of a 64x64 canvas
+    reqs = []
+    for k, cid in enumerate((1, 2, 3, 1, 3, 2, 1)):
+        st = W.strokes(64, 64, 12, 50 + k, stamps_per_stroke=(4, 8), reach=0.3)
+        reqs.append(dict(canvas_id=cid, rows=64, cols=64, background=(0.8, 0.85, 0.9),
+                         stamps=st, radius=4.0, footprint=core, use_snapshot=True))
+    canvases = {c: (64, 64, np.float32([0.8, 0.85, 0.9])) for c in (1, 2, 3)}
+    alone = run_alone(canvases, reqs)
     srv = SV.Server(n_slots=2, hbm_budget_bytes=2 * one + 1024)
-    for cid in (1, 2, 3):  # the third evicts the least recently used (1)
-        srv.wait(srv.submit(cid, 64, 64, st, 4.0, core))
-    assert srv.stats()["resident_canvases"] == 2
+    for q in reqs:  # one at a time, so the eviction order is fixed
+        srv.wait(srv.submit(**q))
+        assert srv.stats()["resident_canvases"] <= 2
+    spilled = 0
+    for cid in (1, 2, 3):
+        try:
+            got = srv.canvas_state(cid)
+        except _lib.PaintyError:  # spilled to the host: a request with no stamps restores it
+            spilled += 1
+            srv.wait(srv.submit(cid, 64, 64, reqs[0]["stamps"][:0], 4.0, core,
+                                background=(0.8, 0.85, 0.9)))
+            got = srv.canvas_state(cid)
+        K, S, V, R0, R = got
+        aK, aS, aV, aR0, aR = alone[cid]
+        for name, x, y in (("K", K, aK), ("S", S, aS), ("V", V, aV), ("R0", R0, aR0)):
+            assert np.array_equal(x, y), (cid, name)
+    assert spilled >= 1
+    srv.close()
+
+
+def test_server_rejects_size_change(hip):
+    core = W.gaussian_core(9, 2.25)
+    st = W.strokes(64, 64, 6, 5, stamps_per_stroke=(3, 6), reach=0.3)
+    srv = SV.Server(n_slots=2, hbm_budget_bytes=1 << 28)
+    srv.wait(srv.submit(7, 64, 64, st, 4.0, core))
     with pytest.raises(_lib.PaintyError):
-        srv.wait(srv.submit(1, 64, 64, st, 4.0, core))
-    srv.wait(srv.submit(3, 64, 64, st, 4.0, core))  # a resident one still works
+        srv.wait(srv.submit(7, 80, 64, st, 4.0, core))
+    srv.wait(srv.submit(7, 64, 64, st, 4.0, core))
     srv.close()
