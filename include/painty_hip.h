@@ -211,11 +211,13 @@ int pnt_brush_get_footprint(pnt_brush brush, float* out);
 int pnt_brush_get_pickup_map(pnt_brush brush, float* K, float* S, float* V,
                              int layout);
 /* set/getPickupRate, set/getDepositionRate, set/getUseSnapshotBuffer --
- * FootprintBrush.hxx:182-204 (defaults 0.9, 0.05, true: :477-495) */
-int pnt_brush_set_pickup_rate(pnt_brush brush, float rate);
-int pnt_brush_get_pickup_rate(pnt_brush brush, float* rate);
-int pnt_brush_set_deposition_rate(pnt_brush brush, float rate);
-int pnt_brush_get_deposition_rate(pnt_brush brush, float* rate);
+ * FootprintBrush.hxx:182-204 (defaults 0.9, 0.05, true: :477-495).  The
+ * rates are T = double in the reference and here: the visit arithmetic runs
+ * in FP64 on the stored FP32 values. */
+int pnt_brush_set_pickup_rate(pnt_brush brush, double rate);
+int pnt_brush_get_pickup_rate(pnt_brush brush, double* rate);
+int pnt_brush_set_deposition_rate(pnt_brush brush, double rate);
+int pnt_brush_get_deposition_rate(pnt_brush brush, double* rate);
 int pnt_brush_set_use_snapshot(pnt_brush brush, int use);
 int pnt_brush_get_use_snapshot(pnt_brush brush, int* use);
 /* dip(paint) -- FootprintBrush.hxx:150-154: clean(), then paint = {K, S} */
@@ -254,7 +256,6 @@ typedef struct {
 #define PNT_EXEC_CHAINED 3
 int pnt_imprint_stamps(pnt_brush brush, pnt_canvas canvas,
                        const pnt_stamp* stamps, int64_t n, int mode);
-
 /* A stroke list: for each stroke, FootprintBrush::setRadius(radius) with
  * its width x width footprint (FootprintBrush.hxx:46-63: nothing changes
  * within 0.5 of the current radius, otherwise a new footprint and a clean
@@ -481,7 +482,7 @@ typedef struct {
   double radius;           /* brush radius and its width x width footprint */
   const float* footprint;  /* (copied at submit) */
   int width;
-  float pickup_rate, deposition_rate;
+  double pickup_rate, deposition_rate;
   int use_snapshot;
   const pnt_stamp* stamps; /* stamp list (copied at submit); PNT_STAMP_DIP sets paint */
   int64_t n_stamps;

@@ -16,7 +16,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# PAINTY_ORACLE_LIB selects another build of the same sources (sanitize.sh
+# points it at the AddressSanitizer / UndefinedBehaviorSanitizer build)
+_LIB_PATH = os.environ.get("PAINTY_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _dp = C.POINTER(C.c_double)

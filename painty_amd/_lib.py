@@ -54,7 +54,7 @@ class PntRequest(C.Structure):
     _fields_ = [("canvas_id", C.c_int64), ("rows", C.c_int), ("cols", C.c_int),
                 ("background", C.c_float * 3), ("radius", C.c_double),
                 ("footprint", C.POINTER(C.c_float)), ("width", C.c_int),
-                ("pickup_rate", C.c_float), ("deposition_rate", C.c_float),
+                ("pickup_rate", C.c_double), ("deposition_rate", C.c_double),
                 ("use_snapshot", C.c_int), ("stamps", C.POINTER(PntStamp)),
                 ("n_stamps", C.c_int64), ("out", C.POINTER(C.c_float))]
 
@@ -117,10 +117,10 @@ SIGNATURES = {
     "pnt_brush_size_map": [_vp, C.POINTER(C.c_int)],
     "pnt_brush_get_footprint": [_vp, _fp],
     "pnt_brush_get_pickup_map": [_vp, _fp, _fp, _fp, C.c_int],
-    "pnt_brush_set_pickup_rate": [_vp, C.c_float],
-    "pnt_brush_get_pickup_rate": [_vp, _fp],
-    "pnt_brush_set_deposition_rate": [_vp, C.c_float],
-    "pnt_brush_get_deposition_rate": [_vp, _fp],
+    "pnt_brush_set_pickup_rate": [_vp, C.c_double],
+    "pnt_brush_get_pickup_rate": [_vp, _dp],
+    "pnt_brush_set_deposition_rate": [_vp, C.c_double],
+    "pnt_brush_get_deposition_rate": [_vp, _dp],
     "pnt_brush_set_use_snapshot": [_vp, C.c_int],
     "pnt_brush_get_use_snapshot": [_vp, C.POINTER(C.c_int)],
     "pnt_brush_dip": [_vp, _fp, _fp],

@@ -1330,25 +1330,25 @@ int pnt_brush_get_pickup_map(pnt_brush b, float* K, float* S, float* V, int layo
   });
 }
 
-int pnt_brush_set_pickup_rate(pnt_brush b, float rate) {
+int pnt_brush_set_pickup_rate(pnt_brush b, double rate) {
   return guard([&] {
     PNT_REQUIRE(b, "NULL brush");
     b->pickup_rate = rate;
   });
 }
-int pnt_brush_get_pickup_rate(pnt_brush b, float* rate) {
+int pnt_brush_get_pickup_rate(pnt_brush b, double* rate) {
   return guard([&] {
     PNT_REQUIRE(b && rate, "NULL argument");
     *rate = b->pickup_rate;
   });
 }
-int pnt_brush_set_deposition_rate(pnt_brush b, float rate) {
+int pnt_brush_set_deposition_rate(pnt_brush b, double rate) {
   return guard([&] {
     PNT_REQUIRE(b, "NULL brush");
     b->dep_rate = rate;
   });
 }
-int pnt_brush_get_deposition_rate(pnt_brush b, float* rate) {
+int pnt_brush_get_deposition_rate(pnt_brush b, double* rate) {
   return guard([&] {
     PNT_REQUIRE(b && rate, "NULL argument");
     *rate = b->dep_rate;
@@ -1389,6 +1389,16 @@ int pnt_brush_update_snapshot(pnt_brush b, pnt_canvas cv) {
     b->ctx->use();
     canvas_tick(cv);
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    // the dirty bytes that go with the snapshot (all clean after a full copy)
+    const int64_t need = round_up(cv->n, 16) + 16;
+    if (need > b->dirty_cap) {
+      PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+      dev_free(b->dirty);
+      b->dirty = nullptr;
+      PNT_HIP_CHECK(hipMalloc((void**)&b->dirty, (size_t)need));
+      b->dirty_cap = need;
+    }
+    b->dirty_n = cv->n;
     brush_full_snapshot(b, cv);
   });
 }
@@ -1414,7 +1424,10 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
     b->ctx->use();
     canvas_tick(cv);
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
-    if (cv->clock != PNT_CLOCK_OFF) mode = PNT_EXEC_PLAIN;  // checkDry runs on the plain path
+    // checkDry runs on the plain and fused paths; the one-launch variants
+    // (persistent, chained) of a canvas with a drying clock run fused
+    if (cv->clock != PNT_CLOCK_OFF && (mode == PNT_EXEC_PERSISTENT || mode == PNT_EXEC_CHAINED))
+      mode = PNT_EXEC_FUSED;
     if (mode == PNT_EXEC_PERSISTENT) {
       run_persistent(b, cv, st, n);
     } else if (mode == PNT_EXEC_FUSED) {
@@ -1449,7 +1462,7 @@ int pnt_imprint_strokes(pnt_brush b, pnt_canvas cv, const pnt_stroke* S, int64_t
     b->ctx->use();
     canvas_tick(cv);
     if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
-    if (mode == PNT_EXEC_FUSED && cv->clock == PNT_CLOCK_OFF) {
+    if (mode == PNT_EXEC_FUSED) {
       run_fused_strokes(b, cv, S, ns);
     } else {
       for (int64_t i = 0; i < ns; ++i) {
@@ -1537,8 +1550,6 @@ static void validate_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs) {
   for (int j = 0; j < n_jobs; ++j) {
     const pnt_batch_job& J = jobs[j];
     PNT_REQUIRE(J.canvas && J.brush, "batch job without canvas or brush");
-    PNT_REQUIRE(J.canvas->clock == PNT_CLOCK_OFF,
-                "a canvas with a drying clock stamps on the plain path, not in a batch");
     PNT_REQUIRE(J.canvas->ctx->device == ctx->device && J.brush->ctx->device == ctx->device,
                 "batch job on another device");
     PNT_REQUIRE(J.n_stamps >= 0 && (J.stamps || J.n_stamps == 0), "invalid stamp list");
@@ -1552,6 +1563,7 @@ static void validate_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs) {
   std::sort(seen.begin(), seen.end());
   PNT_REQUIRE(std::adjacent_find(seen.begin(), seen.end()) == seen.end(),
               "canvases and brushes must be distinct across the jobs of a batch");
+  for (int j = 0; j < n_jobs; ++j) canvas_tick(jobs[j].canvas);  // due drying-timer steps
   for (int j = 0; j < n_jobs; ++j)  // other streams' work on these objects comes first
     if (jobs[j].canvas->ctx->stream != ctx->stream || jobs[j].brush->ctx->stream != ctx->stream) {
       PNT_HIP_CHECK(hipStreamSynchronize(jobs[j].canvas->ctx->stream));

@@ -55,7 +55,7 @@ struct BJob {
   uint8_t* dirty;
   int compose, use_snap;
   int rows, cols, hr, size_map;
-  float pickup_rate, dep_rate, cap;
+  double pickup_rate, dep_rate, cap;
   float paintK[3], paintS[3];
   int s0, s1;  // stamps [s0, s1) of the plan
   int pad[2];
@@ -702,8 +702,10 @@ void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
     throw Error(PNT_EKERNEL, "an earlier persistent stamp kernel aborted");
   BatchState& B = *batch_state(ctx);
   const pnt_batch_config C = cfg ? *cfg : B.cfg;
-  if (C.waves) {
-    run_batch_waves(ctx, jobs, n, C.chunk);
+  bool clocked = false;  // checkDry runs in the waves, not in the persistent kernel
+  for (int j = 0; j < n; ++j) clocked = clocked || jobs[j].canvas->clock != PNT_CLOCK_OFF;
+  if (C.waves || clocked) {
+    run_batch_waves(ctx, jobs, n, C.waves ? C.chunk : 4);
     return;
   }
   const BatchPlan P = plan_batch(ctx, jobs, n, C);
@@ -868,7 +870,7 @@ struct BrushKey {
   int64_t snap_stride;
   int size_map;
   bool use_snap;
-  float pickup_rate, dep_rate, cap;
+  double pickup_rate, dep_rate, cap;
   bool operator==(const BrushKey& o) const {
     return fp == o.fp && pick == o.pick && snap == o.snap && dirty == o.dirty &&
            snap_stride == o.snap_stride && size_map == o.size_map && use_snap == o.use_snap &&
@@ -915,7 +917,9 @@ BatchGraph* capture_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n) {
       G->jobs[j].stamps = G->stamps[j].data();
     }
     hipStream_t s = ctx->stream;
-    const pnt_batch_config C = B.cfg;
+    pnt_batch_config C = B.cfg;
+    for (int j = 0; j < n; ++j)  // checkDry runs in the waves, not in the persistent kernel
+      if (jobs[j].canvas->clock != PNT_CLOCK_OFF && !C.waves) C = pnt_batch_config{16, 1, 1, 4};
     BatchPlan P;
     BWPlan W;
     int64_t total = 0;

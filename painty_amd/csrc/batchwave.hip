@@ -45,8 +45,10 @@ struct BWJob {
   int64_t o_st, o_dl, o_off, o_paint;  // offsets into the plan image
   int cstride, mstride, dirty_bytes, rows, cols, hr, size_map, tiles_x, ntiles, use_snap;
   int fb[4];
-  float pickup_rate, dep_rate, cap;
+  double pickup_rate, dep_rate, cap;
   int pad;
+  int64_t* tmap;  // drying-clock time map of the canvas (nullptr: no clock)
+  int64_t dry_ms;
 };
 
 struct BWSlot {
@@ -100,6 +102,8 @@ __global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJ
   S.cap = J.cap;
   S.use_snap = J.use_snap;
   S.bg[0] = S.bg[1] = S.bg[2] = 0.0f;
+  S.tmap = J.tmap;
+  S.dry_ms = J.dry_ms;
   run_wave_slot(S, b, sm);
 }
 
@@ -300,8 +304,8 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
             add(j, f.k, f.kind, nblk, f.kind == kSWindow ? 0 : rb, J.perm[J.U[i].chain], (int)i);
           }
           break;
-        case kJDrying:
-          add(j, 0, kSDry, wave_stream_blocks(npx), 0, 0, -1);
+        case kJDrying:  // its stamp: the drying phase's last (its time point)
+          add(j, J.U[J.g1].s1 - 1, kSDry, wave_stream_blocks(npx), 0, 0, -1);
           break;
         case kJCopy:
           add(j, 0, kSCopyPick, 64, 0, J.perm[J.U.back().chain], -2);
@@ -402,6 +406,8 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
     d.pickup_rate = b->pickup_rate;
     d.dep_rate = b->dep_rate;
     d.cap = b->cap;
+    d.tmap = cv->clock != PNT_CLOCK_OFF ? cv->tmap : nullptr;
+    d.dry_ms = cv->drying_ms;
     std::memcpy(W.blob.data() + (size_t)j * sizeof(BWJob), &d, sizeof d);
     if (!J.st.empty()) std::memcpy(W.blob.data() + o_st[j], J.st.data(), J.st.size() * sizeof(PStamp));
     if (!J.dl.empty()) std::memcpy(W.blob.data() + o_dl[j], J.dl.data(), J.dl.size() * sizeof(int2));

@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -44,9 +45,19 @@ struct FArgs {
   const int2* dl;
   const int* dl_off;
   int use_snap, rows, cols, hr, size_map, tiles_x, ntiles;
-  float pickup_rate, dep_rate, cap;
+  double pickup_rate, dep_rate, cap;
   float pk[3], ps[3];  // brush paint (the last dip)
+  DryClock clk;        // the canvas' drying clock; clk.tmap == nullptr: none
 };
+
+// The clock of stamp `st` (its own time point), or nullptr without one.
+__device__ __forceinline__ const DryClock* stamp_clock(const DryClock& base, const PStamp& st,
+                                                       DryClock& out) {
+  if (!base.tmap) return nullptr;
+  out = base;
+  out.now_us = st.now_us;
+  return &out;
+}
 
 __device__ __forceinline__ StampGeom geom_of(const FArgs& A, const PStamp& st) {
   StampGeom g;
@@ -71,14 +82,17 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_fused(FArgs A, int j, int win_
   const StampGeom g = geom_of(A, st);
   const bool dup = (st.flags & (kStDupX | kStDupY)) != 0;
   const bool pull = A.use_snap && !dup;
+  DryClock ck;
+  const DryClock* clk = stamp_clock(A.clk, st, ck);
   if ((int)blockIdx.x < win_blocks) {
     const VisitTarget T{&A.canvas, &A.snap, A.dirty, A.use_snap, A.pickup_rate, A.dep_rate,
                         A.cap};
     window_tile<kT>(T, A.pick.p, A.fp, A.fb, g, st, dup, pull, tile_of_block(blockIdx.x, A.ntiles),
-                    A.tiles_x, sm, A.pk, A.ps);
+                    A.tiles_x, sm, A.pk, A.ps, clk);
   } else {
     ring_scan<2>(A.canvas, A.snap, A.dirty, A.fp, g, st.box, pull,
-              ((int)blockIdx.x - win_blocks) * kT * kT + (int)threadIdx.x, ring_blocks * kT * kT);
+                 ((int)blockIdx.x - win_blocks) * kT * kT + (int)threadIdx.x,
+                 ring_blocks * kT * kT, clk != nullptr);
   }
 }
 
@@ -88,18 +102,9 @@ __global__ __launch_bounds__(256) void k_stamp_levels(FArgs A, int j) {
   const PStamp& st = A.st[j];
   const StampGeom g = geom_of(A, st);
   const VisitTarget T{&A.canvas, &A.snap, A.dirty, A.use_snap, A.pickup_rate, A.dep_rate, A.cap};
-  for (int L = 0; L < st.dl_nlev; ++L) {
-    const int e0 = A.dl_off[st.dl_lev + L], e1 = A.dl_off[st.dl_lev + L + 1];
-    for (int e = e0 + (int)threadIdx.x; e < e1; e += 256) {
-      const int2 v = A.dl[e];
-      int64_t ic;
-      const int m = stamp_visit(g, v.x, v.y, &ic);
-      if (m >= 0 && A.fp[m] > 0.0f)
-        cached_visit(T, GlobalPick{A.pick.p, m}, A.fp[m], ic, false, A.pk, A.ps);
-    }
-    __builtin_amdgcn_s_waitcnt(0);  // this level's stores before the next level's loads
-    __syncthreads();
-  }
+  DryClock ck;
+  level_visits<256>(T, A.pick.p, A.fp, g, st, A.dl, A.dl_off, A.pk, A.ps,
+                    stamp_clock(A.clk, st, ck));
 }
 
 }  // namespace
@@ -286,19 +291,24 @@ struct WSlot {
   float* pick;         // the unit's pickup map (7 planes of mstride)
   int64_t mstride;
   float pk[3], ps[3];  // the brush paint of this stamp
+  PStamp stamp;        // the stamp's plan, passed with the launch: its blocks start
+                       // without a dependent load of it
 };
 
 struct WArgs {
-  const PStamp* st;
+  const PStamp* st;  // (the slots carry their stamps' plans; this is the whole list)
   Layer7 canvas, snap;
   uint8_t* dirty;
   const int2* dl;
   const int* dl_off;
   int use_snap, rows, cols;
-  float pickup_rate, dep_rate, cap;
+  double pickup_rate, dep_rate, cap;
+  DryClock clk;  // the canvas' drying clock; clk.tmap == nullptr: none
   int nslots;
   WSlot slot[kWaveSlots];
 };
+static_assert(sizeof(WArgs) <= 4000, "wave arguments must fit the kernel argument segment");
+
 
 template <int kT>
 __global__ __launch_bounds__(kT* kT) void k_stamp_wave(WArgs A) {
@@ -314,7 +324,7 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_wave(WArgs A) {
       S.pick[i] = 0.0f;
     return;
   }
-  const PStamp& st = A.st[S.j];
+  const PStamp& st = S.stamp;
   StampGeom g;
   g.cx = st.cx;
   g.cy = st.cy;
@@ -330,26 +340,17 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_wave(WArgs A) {
   float* pick[7];
 #pragma unroll
   for (int c = 0; c < 7; ++c) pick[c] = S.pick + c * S.mstride;
+  DryClock ck;
+  const DryClock* clk = stamp_clock(A.clk, st, ck);
   const int win = (S.kind == kPhStamp || S.kind == kPhWindow) ? S.ntiles : 0;
   if (S.kind == kPhLevels) {  // one block (nblk == 1)
-    for (int L = 0; L < st.dl_nlev; ++L) {
-      const int e0 = A.dl_off[st.dl_lev + L], e1 = A.dl_off[st.dl_lev + L + 1];
-      for (int e = e0 + (int)threadIdx.x; e < e1; e += kT * kT) {
-        const int2 v = A.dl[e];
-        int64_t ic;
-        const int m = stamp_visit(g, v.x, v.y, &ic);
-        if (m >= 0 && S.fp[m] > 0.0f)
-          cached_visit(T, GlobalPick{pick, m}, S.fp[m], ic, false, S.pk, S.ps);
-      }
-      __builtin_amdgcn_s_waitcnt(0);  // this level's stores before the next level's loads
-      __syncthreads();
-    }
+    level_visits<kT * kT>(T, pick, S.fp, g, st, A.dl, A.dl_off, S.pk, S.ps, clk);
   } else if (b < win) {
     window_tile<kT>(T, pick, S.fp, S.fb, g, st, dup, pull, tile_of_block(b, S.ntiles), S.tiles_x,
-                    sm, S.pk, S.ps);
+                    sm, S.pk, S.ps, clk);
   } else if (b < win + S.rb) {
     ring_scan<2>(A.canvas, A.snap, A.dirty, S.fp, g, st.box, pull, (b - win) * kT * kT +
-                 (int)threadIdx.x, S.rb * kT * kT);
+                 (int)threadIdx.x, S.rb * kT * kT, clk != nullptr);
   }
 }
 
@@ -387,6 +388,11 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
   A0.pickup_rate = b->pickup_rate;
   A0.dep_rate = b->dep_rate;
   A0.cap = b->cap;
+  // Canvas::checkDry on every visited pixel when the canvas has a drying
+  // clock; each stamp carries its own time point (PStamp::now_us)
+  A0.clk = cv->clock != PNT_CLOCK_OFF && cv->tmap
+               ? DryClock{cv->tmap, 0, cv->drying_ms, cv->R0(), cv->h()}
+               : DryClock{};
   std::vector<FArgs> GA(geos.size(), A0);
   std::vector<int64_t> gstride(geos.size());
   int64_t max_stride = 0;
@@ -440,6 +446,8 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
       launch_dry_dirty(cv->n, cv->layer(), cv->R0(), cv->h(), b->dirty, q);
     else
       launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), q);
+    if (A0.clk.tmap)  // dryCanvas stamps every pixel's time (Canvas.hxx:105-119)
+      launch_fill_i64(cv->tmap, cv->n, canvas_now_us(cv), q);
   };
   // the brush ends with the geometry of the list's last stamp
   auto finish_geometry = [&]() {
@@ -492,6 +500,7 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
     W.pickup_rate = A0.pickup_rate;
     W.dep_rate = A0.dep_rate;
     W.cap = A0.cap;
+    W.clk = A0.clk;
     std::vector<size_t> next(U.size(), 0);
     std::vector<char> done(U.size(), 0);
     std::vector<int> prev_on_chain(U.size(), -1);
@@ -524,6 +533,7 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
           WSlot& S = W.slot[q];
           const FArgs& a = GA[(size_t)geo[(size_t)f.k]];
           S.j = (int)f.k;
+          S.stamp = D.host[f.k];
           S.kind = f.kind;
           S.start = start;
           S.ntiles = a.ntiles;
@@ -944,6 +954,7 @@ static void run_chained_t(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st,
   F.pickup_rate = b->pickup_rate;
   F.dep_rate = b->dep_rate;
   F.cap = b->cap;
+  F.clk = DryClock{};  // a canvas with a drying clock stamps on the fused path
   // host plan of the whole list: roles, tickets, paints
   std::vector<int4> roles(n);
   std::vector<int> start(n + 1);

@@ -54,73 +54,82 @@ struct VState {
 // holds the adjoints of the outputs (P, C and, with the snapshot, X); on
 // return the adjoints of the inputs.  gB accumulates the adjoint of the
 // brush paint.  The intermediates are recomputed with the forward's own
-// operations, so every threshold branch is the one the forward took.
-__device__ __forceinline__ void visit_adj(const VState& in, VState& g, float gB[6], float fh,
-                                          float pr, float dr, float cap, const float pk[3],
+// operations (FP64 on the stored FP32 values), so every threshold branch is
+// the one the forward took.
+__device__ __forceinline__ void visit_adj(const VState& in, VState& g, float gB[6], float fh_f,
+                                          double pr, double dr, double cap, const float pk[3],
                                           const float ps[3], bool alias) {
-  const float B[6] = {pk[0], pk[1], pk[2], ps[0], ps[1], ps[2]};
-  const float* P = in.P;
-  const float* C = in.C;
-  const float* X = alias ? in.C : in.X;
-  const float vP = P[6], vC = X[6];
-  const float vLeave = (pr * vC) * fh;
-  const bool a = vLeave >= kMinVolumeF;
-  float P1[7];
+  const double fh = fh_f;
+  const double B[6] = {pk[0], pk[1], pk[2], ps[0], ps[1], ps[2]};
+  const float* Pf = in.P;
+  const float* Cf = in.C;
+  const float* Xf = alias ? in.C : in.X;
+  double P[7], C[7], X[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    P[c] = Pf[c];
+    C[c] = Cf[c];
+    X[c] = Xf[c];
+  }
+  const double vP = P[6], vC = X[6];
+  const double vLeave = (pr * vC) * fh;
+  const bool a = vLeave > kMinVolume;
+  double P1[7];
 #pragma unroll
   for (int c = 0; c < 7; ++c) P1[c] = P[c];
-  float X6n = vC, iv1 = 0.0f;
+  double X6n = vC, iv1 = 0.0;
   bool bb = false;
   if (a) {
-    X6n = vC - vLeave;
-    const float vt1 = vP + vLeave;
-    bb = vt1 >= kMinVolumeF;
+    X6n = (float)(vC - vLeave);  // the stored source volume
+    const double vt1 = vP + vLeave;
+    bb = vt1 > kMinVolume;
     if (bb) {
-      iv1 = rcpf(vt1);
+      iv1 = 1.0 / vt1;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) P1[c] = blendf(vP, P[c], vLeave, X[c], iv1);
+      for (int c = 0; c < 6; ++c) P1[c] = (float)blendd(vP, P[c], vLeave, X[c], iv1);
     }
-    P1[6] = vP + vLeave;
+    P1[6] = (float)(vP + vLeave);
   }
-  const float u = P1[6];
-  const bool f = cap - u > 0.0f;
-  const float vFree = fmaxf(0.0f, cap - u);
-  const float vt2 = u + vFree;
-  const bool d = vt2 >= kMinVolumeF;
-  const float iv2 = d ? rcpf(vt2) : 0.0f;
-  float src[6];
+  const double u = P1[6];
+  const bool f = cap - u > 0.0;
+  const double vFree = fmax(0.0, cap - u);
+  const double vt2 = u + vFree;
+  const bool d = vt2 > kMinVolume;
+  const double iv2 = d ? 1.0 / vt2 : 0.0;
+  double src[6];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) src[c] = d ? blendf(u, P1[c], vFree, B[c], iv2) : P1[c];
-  float Cd[7];  // the canvas pixel as depositPaint reads it
+  for (int c = 0; c < 6; ++c) src[c] = d ? blendd(u, P1[c], vFree, B[c], iv2) : P1[c];
+  double Cd[7];  // the canvas pixel as depositPaint reads it
 #pragma unroll
   for (int c = 0; c < 6; ++c) Cd[c] = C[c];
   Cd[6] = alias ? X6n : C[6];
-  const float vB = cap * fh, vCc = Cd[6], vt3 = vB + vCc;
-  const bool e = vt3 >= kMinVolumeF;
-  const float iv3 = e ? rcpf(vt3) : 0.0f;
+  const double vB = cap * fh, vCc = Cd[6], vt3 = vB + vCc;
+  const bool e = vt3 > kMinVolume;
+  const double iv3 = e ? 1.0 / vt3 : 0.0;
   // ---- depositPaint
-  float gCd[7], gsrc[6];
-  float gvCc = g.C[6];
+  double gCd[7], gsrc[6];
+  double gvCc = g.C[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     if (e) {
-      const float C1 = blendf(vB, src[c], vCc, Cd[c], iv3);
+      const double C1 = blendd(vB, src[c], vCc, Cd[c], iv3);
       gsrc[c] = g.C[c] * vB * iv3;
       gCd[c] = g.C[c] * vCc * iv3;
       gvCc += g.C[c] * (Cd[c] - C1) * iv3;
     } else {
       gsrc[c] = g.C[c];
-      gCd[c] = 0.0f;
+      gCd[c] = 0.0;
     }
   }
   gCd[6] = gvCc;
-  float gP1[7];
-  float gu = g.P[6] * (1.0f - dr * fh);
+  double gP1[7];
+  double gu = g.P[6] * (1.0 - dr * fh);
   if (d) {
-    float gvFree = 0.0f;
+    double gvFree = 0.0;
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       gP1[c] = g.P[c] + gsrc[c] * u * iv2;
-      gB[c] += gsrc[c] * vFree * iv2;
+      gB[c] += (float)(gsrc[c] * vFree * iv2);
       gu += gsrc[c] * (P1[c] - src[c]) * iv2;
       gvFree += gsrc[c] * (B[c] - src[c]) * iv2;
     }
@@ -131,14 +140,14 @@ __device__ __forceinline__ void visit_adj(const VState& in, VState& g, float gB[
   }
   gP1[6] = gu;
   // ---- pickupPaint
-  float gXo[7];
+  double gXo[7];
 #pragma unroll
   for (int c = 0; c < 7; ++c) gXo[c] = alias ? gCd[c] : g.X[c];
-  float gPin[7], gXin[7];
+  double gPin[7], gXin[7];
 #pragma unroll
   for (int c = 0; c < 7; ++c) gXin[c] = gXo[c];
   if (a) {
-    float gvP = gP1[6], gvLeave = gP1[6] - gXo[6], gvC = gXo[6];
+    double gvP = gP1[6], gvLeave = gP1[6] - gXo[6], gvC = gXo[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       if (bb) {
@@ -159,15 +168,15 @@ __device__ __forceinline__ void visit_adj(const VState& in, VState& g, float gB[
   }
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
-    g.P[c] = gPin[c];
-    g.C[c] = alias ? gXin[c] : gCd[c];
-    g.X[c] = alias ? 0.0f : gXin[c];
+    g.P[c] = (float)gPin[c];
+    g.C[c] = (float)(alias ? gXin[c] : gCd[c]);
+    g.X[c] = alias ? 0.0f : (float)gXin[c];
   }
 }
 
 // The forward visit on a state held in registers (the same instructions as
 // every stamp kernel).
-__device__ __forceinline__ void visit_fwd(VState& s, float fh, float pr, float dr, float cap,
+__device__ __forceinline__ void visit_fwd(VState& s, float fh, double pr, double dr, double cap,
                                           const float pk[3], const float ps[3], bool alias) {
   CachedVisit<RegPick> acc;
 #pragma unroll
@@ -206,7 +215,7 @@ struct AArgs {
   Tape t;
   Layer7 gC, gX, gP;  // adjoints: canvas (caller's planes), snapshot, pickup map
   float* gB;          // 6 floats of this stamp's paint adjoint
-  float pr, dr, cap;
+  double pr, dr, cap;
   float pk[3], ps[3];
   // shared-pixel visits of this stamp (plan lists) and their scratch
   const int2* dl;

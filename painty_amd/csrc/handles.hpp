@@ -123,7 +123,7 @@ struct pnt_brush_s {
   int snap_rows = 0, snap_cols = 0;
   int64_t snap_stride = 0, snap_cap = 0;  // floats allocated at snap
   bool use_snap = true;
-  float pickup_rate = 0.9f, dep_rate = 0.05f, cap = 1.0f;
+  double pickup_rate = 0.9, dep_rate = 0.05, cap = 1.0;  // FootprintBrush.hxx:483-495
   float paintK[3] = {0, 0, 0}, paintS[3] = {0, 0, 0};
   // dependency lists of the current stamp's shared-pixel visits (plain path)
   int2* d_list = nullptr;

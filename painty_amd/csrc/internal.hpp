@@ -36,7 +36,7 @@ struct Rgb3 {
 struct BrushArgs {
   const float* fp;   // footprint, size_map^2
   Layer7 pick;       // pickup map, size_map^2 planes
-  float pickup_rate, dep_rate, cap;
+  double pickup_rate, dep_rate, cap;
   float paintK[3], paintS[3];
 };
 

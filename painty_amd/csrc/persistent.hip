@@ -46,7 +46,7 @@ struct PArgs {
   Layer7 pick, canvas, src;
   uint8_t* dirty;
   int use_snap;
-  float pickup_rate, dep_rate, cap;
+  double pickup_rate, dep_rate, cap;
   float paintK[3], paintS[3];
   const int2* dl;     // shared-pixel visits (col,row), grouped by level
   const int* dl_off;  // level offsets into dl
@@ -319,7 +319,10 @@ void plan_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t n,
 
 void plan_stamps_geom(int size_map, double radius, pnt_canvas cv, const pnt_stamp* st, int64_t n,
                       std::vector<PStamp>& h_st, std::vector<int2>& h_dl, std::vector<int>& h_off) {
+  const size_t first = h_st.size();
   plan_stamps_rc(size_map, radius, cv->rows, cv->cols, st, n, h_st, h_dl, h_off);
+  if (cv->clock != PNT_CLOCK_OFF)  // each imprint's time point: system_clock::now() per imprint
+    for (size_t k = first; k < h_st.size(); ++k) h_st[k].now_us = canvas_now_us(cv);
 }
 
 void plan_stamps_rc(int size_map, double radius, int rows, int cols, const pnt_stamp* st,

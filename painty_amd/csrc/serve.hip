@@ -538,6 +538,8 @@ bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
     S.cap = 1.0f;  // FootprintBrush.hxx:483
     S.use_snap = snap ? 1 : 0;
     for (int k = 0; k < 3; ++k) S.bg[k] = R.bg[k];
+    S.tmap = nullptr;  // served canvases have no drying clock
+    S.dry_ms = 0;
     S.st = q->d_st;
     S.fp = q->d_fp;
     S.dl = q->d_dl;

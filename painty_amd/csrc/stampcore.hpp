@@ -8,6 +8,7 @@
 
 #include <vector>
 
+#include "dry.hpp"
 #include "internal.hpp"
 #include "visit.hpp"
 
@@ -31,7 +32,9 @@ struct PStamp {
   int32_t dl_lev;    // first entry of this stamp's level offsets (dup stamps)
   int32_t dl_nlev;   // number of shared-pixel levels (0 if none)
   RingBox box;
-  int32_t pad[6];
+  int32_t pad0;
+  int64_t now_us;  // the imprint's time point when the canvas has a drying clock
+  int32_t pad[4];
 };
 static_assert(sizeof(PStamp) % 16 == 0, "PStamp alignment");
 
@@ -148,7 +151,7 @@ struct VisitTarget {
   const Layer7* src;  // snapshot planes (unused when use_snap == 0)
   uint8_t* dirty;
   int use_snap;
-  float pickup_rate, dep_rate, cap;
+  double pickup_rate, dep_rate, cap;
 };
 
 template <class PickAcc>
@@ -210,15 +213,18 @@ __device__ __forceinline__ bool window_offset(int x, double c, int hr, int* k) {
 }
 
 // Is canvas pixel (x, y), on the ring of this stamp, also visited with a
-// footprint height > 0 by it (and therefore refreshed by that visit)?
+// footprint height > 0 by it (and therefore refreshed by that visit)?  With
+// a drying clock every visited pixel counts: its window tile refreshes it
+// before checkDry changes the canvas.
 __device__ __forceinline__ bool ring_pixel_visited(const StampGeom& g, const RingBox& b,
-                                                   const float* fp, int x, int y) {
+                                                   const float* fp, int x, int y,
+                                                   bool any_visit = false) {
   if (x < b.tl_x || x > b.br_x || y < b.tl_y || y > b.br_y) return false;
   int col, row;
   if (!window_offset(x, g.cx, g.hr, &col) || !window_offset(y, g.cy, g.hr, &row)) return false;
   int64_t ic;
   const int m = stamp_visit(g, col, row, &ic);
-  return m >= 0 && fp[m] > 0.0f;
+  return m >= 0 && (any_visit || fp[m] > 0.0f);
 }
 
 __device__ __forceinline__ bool on_window_border(const RingBox& b, int x, int y) {
@@ -239,7 +245,8 @@ __device__ __forceinline__ bool on_window_border(const RingBox& b, int x, int y)
 template <int kG>
 __device__ __forceinline__ void ring_scan(const Layer7& canvas, const Layer7& snap, uint8_t* dirty,
                                           const float* fp, const StampGeom& g, const RingBox& b,
-                                          bool pull, int tid, int nthreads) {
+                                          bool pull, int tid, int nthreads,
+                                          bool clocked = false) {
   if (b.a_x1 < b.a_x0 || b.a_y1 < b.a_y0) return;
   const int rows = b.a_y1 - b.a_y0 + 1;
   const int span = b.a_x1 - b.a_x0 + 1;
@@ -274,7 +281,7 @@ __device__ __forceinline__ void ring_scan(const Layer7& canvas, const Layer7& sn
         if (i < lo || i >= lo + span) continue;
         const int col = x0 + q;
         if (interior_row && col > b.tl_x && col < b.br_x) continue;
-        if (pull && ring_pixel_visited(g, b, fp, col, row)) continue;
+        if (pull && ring_pixel_visited(g, b, fp, col, row, clocked)) continue;
         m |= 1u << q;
       }
       mask[u] = m;
@@ -352,6 +359,52 @@ __device__ __forceinline__ void run_pixel(const VisitTarget& T, int64_t ic, bool
   }
 }
 
+// Canvas::checkDry (Canvas.hxx:123-155) of a visited pixel whose canvas
+// values are held in registers (cv[7]); R0, h and the time map are in
+// memory.  The same arithmetic as check_dry (dry.hpp).
+__device__ __forceinline__ void check_dry_regs(const DryClock& c, float cv[7], int64_t ic) {
+  const float v = cv[6];
+  if (c.dry_ms > 0 && v >= 0.001f) {
+    const int64_t elapsed_ms = (c.now_us - c.tmap[ic]) / 1000;  // duration_cast truncation
+    if (elapsed_ms >= c.dry_ms) {
+      float K[3], S[3], R[3];
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        K[ch] = cv[ch];
+        S[ch] = cv[3 + ch];
+        R[ch] = c.R0.p[ch][ic];
+      }
+      c.h[ic] += v;
+      km_pixel(K, S, R, v);
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        c.R0.p[ch][ic] = R[ch];
+        cv[ch] = 0.0f;
+        cv[3 + ch] = 0.0f;
+      }
+      cv[6] = 0.0f;
+    }
+  }
+  c.tmap[ic] = c.now_us;
+}
+
+// checkDry of a visited pixel with footprint height <= 0 (no pickup or
+// deposit follows).  On the window border of a stamp that refreshes its
+// border itself (`pull`), the snapshot refresh comes first, as
+// updateSnapshot runs before the visits (FootprintBrush.hxx:80-84).  A
+// pixel that dries no longer equals its snapshot.
+__device__ __forceinline__ void check_only(const VisitTarget& T, const DryClock& c, int64_t ic,
+                                           bool pv) {
+  if (pv) {
+#pragma unroll
+    for (int ch = 0; ch < 7; ++ch) T.src->p[ch][ic] = T.canvas->p[ch][ic];
+    T.dirty[ic] = 0;
+  }
+  const float v0 = T.canvas->p[6][ic];
+  check_dry(c, *T.canvas, ic);
+  if (T.use_snap && v0 >= 0.001f && T.canvas->p[6][ic] != v0) T.dirty[ic] = 1;
+}
+
 // The visits of one kT x kT tile of a stamp's window, by one workgroup of
 // kT*kT threads (all of which must call it).  The map cells of the tile and
 // a one-pixel halo go to LDS (`sm`); a visit whose left or upper neighbour
@@ -360,22 +413,30 @@ __device__ __forceinline__ void run_pixel(const VisitTarget& T, int64_t ic, bool
 // Visits on a duplicated canvas column/row 0 (and their partners) are left
 // out; they run afterwards by dependency level.  Each visit's pickup cell,
 // canvas pixel and source pixel are loaded in one batch.
+//
+// With a drying clock (clk != nullptr) every visited pixel -- in the canvas
+// and in the map, whatever its footprint height -- gets Canvas::checkDry
+// before its pickup and deposit (FootprintBrush.hxx:115), by the thread that
+// runs its visit: map cells are then taken without the footprint-box cut,
+// so the pairs are those of the plain kernel.
 template <int kT>
 __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* pick,
                                             const float* fp, const FpBox& fb, const StampGeom& g,
                                             const PStamp& st, bool dup, bool pull, int tile,
                                             int tiles_x, int (*sm)[kT + 2], const float pk[3],
-                                            const float ps[3]) {
+                                            const float ps[3], const DryClock* clk = nullptr) {
   const int t = threadIdx.x;
   const int lx = t % kT, ly = t / kT;
   const int col0 = -g.hr + (tile % tiles_x) * kT;
   const int row0 = -g.hr + (tile / tiles_x) * kT;
-  // map cells of the tile and its halo; cells outside the footprint's
-  // nonzero box count as no visit (their visits have fh = 0)
+  // map cells of the tile and its halo; without a clock, cells outside the
+  // footprint's nonzero box count as no visit (their visits have fh = 0)
   for (int e = t; e < (kT + 2) * (kT + 2); e += kT * kT) {
     const int hy = e / (kT + 2), hx = e % (kT + 2);
     int64_t ic;
-    sm[hy][hx] = stamp_visit_in(g, col0 + hx - 1, row0 + hy - 1, fb.x0, fb.y0, fb.x1, fb.y1, &ic);
+    sm[hy][hx] = clk ? stamp_visit(g, col0 + hx - 1, row0 + hy - 1, &ic)
+                     : stamp_visit_in(g, col0 + hx - 1, row0 + hy - 1, fb.x0, fb.y0, fb.x1,
+                                      fb.y1, &ic);
   }
   __syncthreads();
   const int mm = sm[ly + 1][lx + 1];
@@ -399,6 +460,8 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
       act = true;
       ic = canvas_index(g, col, row);
       pv = pull && on_window_border(st.box, (int)(ic % g.cols), (int)(ic / g.cols));
+      // the footprint height and every value the visits read go out in one
+      // batch (a visit with fh <= 0 then uses none of them)
       fh = fp[mm];
 #pragma unroll
       for (int c = 0; c < 7; ++c) acc.v[c] = pick[c][mm];
@@ -421,8 +484,18 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
       }
     }
   }
+  if (act && clk && !(fh > 0.0f)) {  // visits that only check
+    check_only(T, *clk, ic, pv);
+    if (pdir) check_only(T, *clk, pic, ppull);
+  }
   act = act && fh > 0.0f;
   if (act) {
+    if (clk) {
+      check_dry_regs(*clk, acc.cv, ic);
+      if (!T.use_snap)  // the pickup source is the canvas itself
+#pragma unroll
+        for (int c = 0; c < 7; ++c) acc.sv[c] = acc.cv[c];
+    }
     run_pixel(T, ic, pv, fh, pk, ps, acc);
     if (pdir) {
       acc.alias = !T.use_snap;
@@ -436,12 +509,43 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
       } else {
         load_pixel(T, pic, ppull, acc);  // the pickup cell stays in acc.v
       }
+      if (clk) {
+        check_dry_regs(*clk, acc.cv, pic);
+        if (!T.use_snap)
+#pragma unroll
+          for (int c = 0; c < 7; ++c) acc.sv[c] = acc.cv[c];
+      }
       run_pixel(T, pic, ppull, fh, pk, ps, acc);
     }
 #pragma unroll
     for (int c = 0; c < 7; ++c) pick[c][mm] = acc.v[c];
   }
   __syncthreads();  // sm is rewritten by the block's next tile
+}
+
+// Visits sharing a canvas pixel on column/row 0 (and their map-cell
+// partners), listed in reference order and grouped by dependency level; the
+// threads of one workgroup, levels separated by barriers.  With a clock,
+// each listed visit first checks its pixel (the second check of a pixel in
+// one imprint finds its time already at the imprint's and changes nothing).
+template <int kThreads>
+__device__ __forceinline__ void level_visits(const VisitTarget& T, float* const* pick,
+                                             const float* fp, const StampGeom& g, const PStamp& st,
+                                             const int2* dl, const int* dl_off, const float pk[3],
+                                             const float ps[3], const DryClock* clk = nullptr) {
+  for (int L = 0; L < st.dl_nlev; ++L) {
+    const int e0 = dl_off[st.dl_lev + L], e1 = dl_off[st.dl_lev + L + 1];
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += kThreads) {
+      const int2 v = dl[e];
+      int64_t ic;
+      const int m = stamp_visit(g, v.x, v.y, &ic);
+      if (m < 0) continue;
+      if (clk) check_only(T, *clk, ic, false);
+      if (fp[m] > 0.0f) cached_visit(T, GlobalPick{pick, m}, fp[m], ic, false, pk, ps);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this level's stores before the next level's loads
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------- host side --

@@ -1,23 +1,35 @@
 // The arithmetic of one imprint visit, shared by every stamp kernel.
 //
 // pickupPaint (renderer/FootprintBrush.hxx:349-384), then depositPaint
-// (:393-431), with blend() (:331-340) as blendf below.  Storage is reached
+// (:393-431), with blend() (:331-340) as blendd below.  Storage is reached
 // through an accessor so the plain kernel (pickup map in global memory) and
 // the persistent kernel (pickup map in LDS) execute the same instructions and
-// produce the same bits.  Products that the reference rounds separately are
-// combined only through explicit fmaf, never by compiler contraction.
+// produce the same bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace pnt {
 
-// MinVolume = 0.001 (FootprintBrush.hxx:26) compared in FP64; for a float
-// x, x > 0.001 holds exactly when x >= 0.001f.
-constexpr float kMinVolumeF = 0.001f;
+// MinVolume = 0.001 (FootprintBrush.hxx:26).
+constexpr double kMinVolume = 0.001;
 
-// (va*a + vb*b) / vt, with inv_vt = 1/vt shared by the channels of one blend
-// (v_rcp_f32, 1 ulp; the product adds at most 1.5 ulp over a divide).
+// The arithmetic runs in FP64, as the reference's does (vec3 is Eigen's
+// double vector): loads widen the stored FP32 values, every product, sum,
+// quotient and threshold test is the reference's own in double precision,
+// and each result is rounded to FP32 once, when it is stored.  Over the
+// hundreds of dependent visits a pixel and a pickup cell take in a long
+// stamp list, only that storage rounding accumulates (an FP32 chain would
+// add the rounding of every intermediate step as well).
+//
+// blend(va, a, vb, b) = vt > MinVolume ? (va*a + vb*b) / vt : a with
+// vt = va + vb (FootprintBrush.hxx:331-340); inv_vt = 1/vt is shared by the
+// channels of one blend.
+__device__ __forceinline__ double blendd(double va, double a, double vb, double b, double inv_vt) {
+  return (va * a + vb * b) * inv_vt;
+}
+
+// Kept for the FP32 adjoint (grad.hip): the FP32 form of blend.
 __device__ __forceinline__ float blendf(float va, float a, float vb, float b, float inv_vt) {
   return __builtin_fmaf(va, a, vb * b) * inv_vt;
 }
@@ -29,52 +41,59 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 //   float src(int c)  / void set_src_v(float)          pickup source (snapshot or canvas)
 //   float canvas(int c) / void set_canvas(int c, float)
 template <class Acc>
-__device__ __forceinline__ void paint_visit_math(Acc& a, float fh, float pickup_rate,
-                                                 float dep_rate, float cap, const float pK[3],
+__device__ __forceinline__ void paint_visit_math(Acc& a, float fh_f, double pickup_rate,
+                                                 double dep_rate, double cap, const float pK[3],
                                                  const float pS[3]) {
-  {  // pickupPaint
-    const float vP = a.pick(6);
-    const float vC = a.src(6);
-    const float vLeave = (pickup_rate * vC) * fh;
-    if (vLeave >= kMinVolumeF) {
-      a.set_src_v(vC - vLeave);
-      const float vt = vP + vLeave;
-      if (vt >= kMinVolumeF) {
-        const float iv = rcpf(vt);
+  const double fh = fh_f;
+  {  // pickupPaint (FootprintBrush.hxx:349-384)
+    const double vP = a.pick(6);
+    const double vC = a.src(6);
+    const double vLeave = (pickup_rate * vC) * fh;
+    if (vLeave > kMinVolume) {
+      a.set_src_v((float)(vC - vLeave));
+      const double vt = vP + vLeave;
+      if (vt > kMinVolume) {
+        const double iv = 1.0 / vt;
+        double nv[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) a.set_pick(c, blendf(vP, a.pick(c), vLeave, a.src(c), iv));
+        for (int c = 0; c < 6; ++c) nv[c] = blendd(vP, a.pick(c), vLeave, a.src(c), iv);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a.set_pick(c, (float)nv[c]);
       }
-      a.set_pick(6, vP + vLeave);
+      a.set_pick(6, (float)(vP + vLeave));
     }
   }
-  {  // depositPaint
-    const float vP = a.pick(6);
-    const float vFree = fmaxf(0.0f, cap - vP);
-    float src[6];
+  {  // depositPaint (FootprintBrush.hxx:393-431)
+    const double vP = a.pick(6);
+    const double vFree = fmax(0.0, cap - vP);
+    double src[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) src[c] = a.pick(c);
-    const float vt = vP + vFree;
-    if (vt >= kMinVolumeF) {
-      const float iv = rcpf(vt);
+    const double vt = vP + vFree;
+    if (vt > kMinVolume) {
+      const double iv = 1.0 / vt;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        src[c] = blendf(vP, src[c], vFree, pK[c], iv);
-        src[3 + c] = blendf(vP, src[3 + c], vFree, pS[c], iv);
+        src[c] = blendd(vP, src[c], vFree, (double)pK[c], iv);
+        src[3 + c] = blendd(vP, src[3 + c], vFree, (double)pS[c], iv);
       }
     }
-    a.set_pick(6, __builtin_fmaf(-(dep_rate * vP), fh, vP));
-    const float vC = a.canvas(6);
-    const float vB = cap * fh;
-    const float vt2 = vB + vC;
-    if (vt2 >= kMinVolumeF) {
-      const float iv = rcpf(vt2);
+    a.set_pick(6, (float)(vP - (dep_rate * vP) * fh));
+    const double vC = a.canvas(6);
+    const double vB = cap * fh;
+    const double vt2 = vB + vC;
+    if (vt2 > kMinVolume) {
+      const double iv = 1.0 / vt2;
+      double nv[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) a.set_canvas(c, blendf(vB, src[c], vC, a.canvas(c), iv));
+      for (int c = 0; c < 6; ++c) nv[c] = blendd(vB, src[c], vC, a.canvas(c), iv);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.set_canvas(c, (float)nv[c]);
     } else {
 #pragma unroll
-      for (int c = 0; c < 6; ++c) a.set_canvas(c, src[c]);
+      for (int c = 0; c < 6; ++c) a.set_canvas(c, (float)src[c]);
     }
-    a.set_canvas(6, vB + vC);
+    a.set_canvas(6, (float)(vB + vC));
   }
 }
 

@@ -55,9 +55,11 @@ struct SSlot {
   int kind, start, nblk, ntiles, rb, hr, size_map, tiles_x;
   int fb[4];
   float pk[3], ps[3];
-  float pickup_rate, dep_rate, cap;
+  double pickup_rate, dep_rate, cap;
   int use_snap;
   float bg[3];
+  int64_t* tmap;   // the canvas' drying-clock time map (nullptr: no clock)
+  int64_t dry_ms;  // its drying time
 };
 
 __device__ __forceinline__ void wave_fill4(float* p, int64_t n4, float v, int tid, int nth) {
@@ -147,10 +149,14 @@ __device__ __forceinline__ void run_wave_slot(const SSlot& S, int b, int (*sm)[k
       for (int64_t q = tid; q < 7 * (int64_t)S.mstride / 4; q += nth)
         reinterpret_cast<float4*>(S.pick_dst)[q] = reinterpret_cast<const float4*>(S.pick)[q];
       return;
-    case kSDry:
-      for (int64_t i = tid; i < n; i += nth)
+    case kSDry: {  // with a clock, dryCanvas also stamps every pixel's time (its last imprint's)
+      const int64_t now = S.tmap ? S.st->now_us : 0;
+      for (int64_t i = tid; i < n; i += nth) {
         wave_dry_pixel(canvas, R0, S.canvas + 10 * cs, cs, S.use_snap ? S.dirty : nullptr, i);
+        if (S.tmap) S.tmap[i] = now;
+      }
       return;
+    }
     case kSCompose:
       for (int64_t i = tid; i < n; i += nth) wave_compose_pixel(canvas, R0, S.out, cs, i);
       return;
@@ -174,26 +180,20 @@ __device__ __forceinline__ void run_wave_slot(const SSlot& S, int b, int (*sm)[k
 #pragma unroll
   for (int c = 0; c < 7; ++c) pick[c] = S.pick + c * (int64_t)S.mstride;
   const FpBox fb{S.fb[0], S.fb[1], S.fb[2], S.fb[3]};
+  // Canvas::checkDry on every visited pixel when the canvas has a clock
+  const DryClock ck{S.tmap, st.now_us, S.dry_ms,
+                    Rgb3{{S.canvas + 7 * cs, S.canvas + 8 * cs, S.canvas + 9 * cs}},
+                    S.canvas + 10 * cs};
+  const DryClock* clk = S.tmap ? &ck : nullptr;
   const int win = (S.kind == kSStamp || S.kind == kSWindow) ? S.ntiles : 0;
   if (S.kind == kSLevels) {  // one block
-    for (int L = 0; L < st.dl_nlev; ++L) {
-      const int e0 = S.dl_off[st.dl_lev + L], e1 = S.dl_off[st.dl_lev + L + 1];
-      for (int e = e0 + (int)threadIdx.x; e < e1; e += kWT2) {
-        const int2 v = S.dl[e];
-        int64_t ic;
-        const int m = stamp_visit(g, v.x, v.y, &ic);
-        if (m >= 0 && S.fp[m] > 0.0f)
-          cached_visit(T, GlobalPick{pick, m}, S.fp[m], ic, false, S.pk, S.ps);
-      }
-      __builtin_amdgcn_s_waitcnt(0);  // this level's stores before the next level's loads
-      __syncthreads();
-    }
+    level_visits<kWT2>(T, pick, S.fp, g, st, S.dl, S.dl_off, S.pk, S.ps, clk);
   } else if (b < win) {
     window_tile<kWT>(T, pick, S.fp, fb, g, st, dup, pull, tile_of_block(b, S.ntiles), S.tiles_x,
-                     sm, S.pk, S.ps);
+                     sm, S.pk, S.ps, clk);
   } else if (b < win + S.rb) {
     ring_scan<2>(canvas, snap, S.dirty, S.fp, g, st.box, pull, (b - win) * kWT2 + (int)threadIdx.x,
-                 S.rb * kWT2);
+                 S.rb * kWT2, clk != nullptr);
   }
 }
 

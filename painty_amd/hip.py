@@ -382,7 +382,7 @@ class FootprintBrush(Owned):
         call("pnt_brush_set_pickup_rate", self.h, float(r))
 
     def getPickupRate(self) -> float:
-        v = C.c_float()
+        v = C.c_double()
         call("pnt_brush_get_pickup_rate", self.h, C.byref(v))
         return v.value
 
@@ -390,7 +390,7 @@ class FootprintBrush(Owned):
         call("pnt_brush_set_deposition_rate", self.h, float(r))
 
     def getDepositionRate(self) -> float:
-        v = C.c_float()
+        v = C.c_double()
         call("pnt_brush_get_deposition_rate", self.h, C.byref(v))
         return v.value
 

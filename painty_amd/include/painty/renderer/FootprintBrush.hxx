@@ -109,17 +109,17 @@ class FootprintBrush final : public BrushBase<vector_type> {
   }
 
   // FootprintBrush.hxx:182-204
-  void setPickupRate(const T rate) { gpu::check(pnt_brush_set_pickup_rate(_h, float(rate))); }
+  void setPickupRate(const T rate) { gpu::check(pnt_brush_set_pickup_rate(_h, double(rate))); }
   void setDepositionRate(const T rate) {
-    gpu::check(pnt_brush_set_deposition_rate(_h, float(rate)));
+    gpu::check(pnt_brush_set_deposition_rate(_h, double(rate)));
   }
   T getPickupRate() const {
-    float r = 0;
+    double r = 0;
     gpu::check(pnt_brush_get_pickup_rate(_h, &r));
     return static_cast<T>(r);
   }
   T getDepositionRate() const {
-    float r = 0;
+    double r = 0;
     gpu::check(pnt_brush_get_deposition_rate(_h, &r));
     return static_cast<T>(r);
   }

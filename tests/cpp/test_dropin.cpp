@@ -73,7 +73,7 @@ int stamp(const char* in, const char* out) {
   int32_t hdr[5];
   read(f, hdr, 5);
   const int rows = hdr[0], cols = hdr[1], width = hdr[2], nst = hdr[3], snap = hdr[4];
-  float rates[3];
+  double rates[3];  // pickup rate, deposition rate, radius (T = double, as the reference's)
   read(f, rates, 3);
   const size_t n = (size_t)rows * cols;
   std::vector<float> K(3 * n), S(3 * n), V(n), R0(3 * n), fp((size_t)width * width);

@@ -69,7 +69,7 @@ def test_stamp_scenario_through_cpp(tmp_path, oracle, snapshot):
     core = wl.cores[0].astype(np.float32)
     with open(tmp_path / "in.bin", "wb") as f:
         np.array([wl.rows, wl.cols, core.shape[0], len(st), int(snapshot)], "<i4").tofile(f)
-        np.array([wl.pickup_rate, wl.deposition_rate, wl.radii[0]], "<f4").tofile(f)
+        np.array([wl.pickup_rate, wl.deposition_rate, wl.radii[0]], "<f8").tofile(f)
         for a in (wl.wet.K, wl.wet.S, wl.wet.V, wl.R0, core):
             np.ascontiguousarray(a, dtype="<f4").tofile(f)
         rec.tofile(f)

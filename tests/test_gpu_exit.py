@@ -27,6 +27,7 @@ for ctx in (None, other):
     out = H.Canvas(wl.rows, wl.cols, ctx)
     keep.append((cv, br, out, H.PaintLayer(8, 8, ctx)))
 jobs = [(cv, br, wl.stamps, out) for cv, br, out, _ in keep]
+H.batch_set_config(16, 1, 1, 4)  # waves (the persistent launch is covered elsewhere)
 H.batch_run(jobs)
 g = H.BatchGraph(jobs)          # a captured batch, never closed
 g.launch()
