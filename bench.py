@@ -581,6 +581,8 @@ def _serve_rate(srv, reqs, rate, seed, offset):
             time.sleep(0.0001)
         tickets[i] = srv.submit(q.canvas_id + offset, q.side, q.side, q.stamps, q.radius,
                                 q.footprint, background=q.background)
+        if q.last:  # the client ends the canvas after this request
+            srv.release_canvas(q.canvas_id + offset)
     th.join()
     el = time.perf_counter() - t0
     ok = np.array([x for x in lat if x is not None])

@@ -499,6 +499,11 @@ int pnt_server_wait(pnt_server server, int64_t ticket, double* latency_ms);
 /* A resident canvas (no request running on it) and its composite planes. */
 int pnt_server_canvas(pnt_server server, int64_t canvas_id, pnt_canvas* canvas,
                       pnt_dev_rgb* composite);
+/* Ends a canvas: once no request on it is queued or running, its device
+ * block returns to the server's pool and any host copy is freed; a later
+ * request with the same id starts a new canvas.  A request submitted on the
+ * id before that happens cancels the release. */
+int pnt_server_release(pnt_server server, int64_t canvas_id);
 int pnt_server_stats(pnt_server server, int64_t* resident_bytes,
                      int64_t* resident_canvases, int64_t* completed);
 /* Finishes the queued work, then releases everything. */

@@ -191,6 +191,7 @@ SIGNATURES = {
     "pnt_server_submit": [_vp, C.POINTER(PntRequest), C.POINTER(C.c_int64)],
     "pnt_server_wait": [_vp, C.c_int64, C.POINTER(C.c_double)],
     "pnt_server_canvas": [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(DevRgb)],
+    "pnt_server_release": [_vp, C.c_int64],
     "pnt_server_stats": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                          C.POINTER(C.c_int64)],
     "pnt_server_destroy": [_vp],

@@ -9,14 +9,15 @@
 //   R = (1 - R0*(a - b*coth x)) / (a - R0 + b*coth x)
 // Multiplying numerator and denominator by S, with A = S*a = S+K and
 // B = S*b = sqrt(K*(K+2S)), x = B*d, E = expm1(2x), coth x = 1 + 2/E, and
-// a - b = 1/(a+b):
-//   S*(a - b*coth x) = S^2/(A+B) - 2B/E
-//   S*(a - R0 + b*coth x) = A + B - R0*S + 2B/E
-// so R = (S - R0*(S^2/(A+B) - 2B/E)) / (A + B - R0*S + 2B/E).
-// There is no subtraction of nearly equal large terms (cosh/sinh in FP32 loses
-// ~1.5e-5 relative; this form stays within a few ulp).  The reference returns
-// coth = 1 for x > 20; here 2x is clamped to 40, where 2/E = 8.5e-18 relative
-// to 1, i.e. the same number in FP32.
+// a - b = 1/(a+b), so that S*(a - b*coth x) = S - S*(K+B)/(A+B) - 2B/E:
+//   numerator   S*(1 - R0*(a - b coth x)) = S*(1-R0) + R0*(S*(K+B)/(A+B) + 2B/E)
+//   denominator S*(a - R0 + b coth x)     = S*(1-R0) + K + B + 2B/E
+// For 0 <= R0 <= 1 and K, S >= 0 every term is non-negative: no subtraction
+// of nearly equal terms anywhere (1 - R0 is exact for R0 >= 0.5), so the
+// result stays within a few ulp even where the reference's own form cancels
+// (R0 near 1 under a weakly absorbing, thick layer).  The reference returns
+// coth = 1 for x > 20; here 2x is clamped to 40, where 2/E = 8.5e-18
+// relative to 1, i.e. the same number in FP32.
 //
 // Thresholds: 1e4*DBL_EPSILON = 10000*2^-52 = 0x1.388p-39 is exact in FP32,
 // so |d| < thr and |S| > thr compare exactly as the FP64 reference does.
@@ -34,6 +35,7 @@ constexpr float kSClamp = 1e-11f;        // KubelkaMunk.hxx:41
 // into hi + lo (fma) so the exponent carries ~48 bits, then -1 (y >= 0.5 keeps
 // the subtraction within 1.3 bits).
 __device__ __forceinline__ float km_expm1(float y) {
+#pragma clang fp contract(off)
   // Taylor: y*(1 + y/2*(1 + y/3*(1 + ... )))
   float p = 1.0f + y * (1.0f / 10.0f);
   p = 1.0f + y * (1.0f / 9.0f) * p;
@@ -55,21 +57,24 @@ __device__ __forceinline__ float km_expm1(float y) {
   return (y < 0.5f) ? small : big;
 }
 
-// One channel, d already known to satisfy |d| >= kKmThr.
+// One channel, d already known to satisfy |d| >= kKmThr.  Contraction is
+// off and every fused multiply-add is written out, so each kernel that
+// inlines this computes the same bits (composite, dry, checkDry, waves).
 __device__ __forceinline__ float km_channel(float K, float S, float R0, float d) {
+#pragma clang fp contract(off)
   S = (__builtin_fabsf(S) > kKmThr) ? S : kSClamp;
-  const float A = S + K;
-  const float q = K * (K + 2.0f * S);
+  const float q = K * __builtin_fmaf(2.0f, S, K);
   const float B = __builtin_amdgcn_sqrtf(q > 0.0f ? q : 0.0f);
   float y = 2.0f * B * d;
   y = y < 40.0f ? y : 40.0f;
   const float E = km_expm1(y);
-  const float AB = A + B;
-  const float r = __builtin_amdgcn_rcpf(E * AB);   // 1/(E (A+B))
-  const float t = 2.0f * B * (AB * r);             // 2B/E
-  const float m = S * S * (E * r) - t;             // S^2/(A+B) - 2B/E
-  const float num = S - R0 * m;
-  const float den = AB - R0 * S + t;
+  const float KB = K + B;
+  const float AB = S + KB;
+  const float r = __builtin_amdgcn_rcpf(E * AB);  // 1/(E (A+B))
+  const float t = 2.0f * B * (AB * r);            // 2B/E
+  const float w = S * (1.0f - R0);
+  const float num = __builtin_fmaf(R0, __builtin_fmaf(S * KB, E * r, t), w);
+  const float den = w + KB + t;
   return num * __builtin_amdgcn_rcpf(den);
 }
 
@@ -85,6 +90,7 @@ __device__ __forceinline__ void km_pixel(const float K[3], const float S[3], flo
 // no S clamp; r0 returned when any channel has |b S d| < 1e-5.
 __device__ __forceinline__ void km_pixel_glsl(const float K[3], const float S[3], float R0[3],
                                               float d) {
+#pragma clang fp contract(off)
   bool small = false;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -96,17 +102,18 @@ __device__ __forceinline__ void km_pixel_glsl(const float K[3], const float S[3]
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const float s = S[c];
-    const float A = s + K[c];
-    const float q = K[c] * (K[c] + 2.0f * s);
+    const float q = K[c] * __builtin_fmaf(2.0f, s, K[c]);
     const float B = __builtin_amdgcn_sqrtf(q > 0.0f ? q : 0.0f);
     float y = 2.0f * B * d;
     y = y < 40.0f ? y : 40.0f;
     const float E = km_expm1(y);
-    const float AB = A + B;
+    const float KB = K[c] + B;
+    const float AB = s + KB;
     const float r = __builtin_amdgcn_rcpf(E * AB);
     const float t = 2.0f * B * (AB * r);
-    const float m = s * s * (E * r) - t;
-    R0[c] = (s - R0[c] * m) * __builtin_amdgcn_rcpf(AB - R0[c] * s + t);
+    const float w = s * (1.0f - R0[c]);
+    const float num = __builtin_fmaf(R0[c], __builtin_fmaf(s * KB, E * r, t), w);
+    R0[c] = num * __builtin_amdgcn_rcpf(w + KB + t);
   }
 }
 

@@ -46,6 +46,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "handles.hpp"
@@ -195,6 +196,7 @@ struct pnt_server_s {
   std::unordered_map<int64_t, Req*> tickets;
   std::unordered_map<int64_t, Resident> canvases;
   std::unordered_map<int64_t, int> pending;  // queued requests per canvas
+  std::unordered_set<int64_t> closing;       // released once idle (pnt_server_release)
   std::deque<CopyJob> copyq;
   std::vector<SlotRes> slots;
   std::vector<int> free_slots;
@@ -222,6 +224,7 @@ struct pnt_server_s {
   void after_wave(const std::vector<std::pair<Req*, int>>& used);
   void complete(Req* q);
   void finish(Req* q, int status, const std::string& err);
+  void release_if_closing(int64_t id);
   void run_engine();
   void run_copier();
   void copy_job(const CopyJob& j);
@@ -678,6 +681,29 @@ void pnt_server_s::finish(Req* q, int status, const std::string& err) {
   cv_done.notify_all();
 }
 
+// A canvas named by pnt_server_release goes once nothing needs it: no queued
+// or running request and no copy in flight (caller holds mu).  Its device
+// block returns to the pool; its host spill, if any, is freed.
+void pnt_server_s::release_if_closing(int64_t id) {
+  if (!closing.count(id) || pending.count(id)) return;
+  auto c = canvases.find(id);
+  if (c == canvases.end()) {
+    closing.erase(id);
+    return;
+  }
+  Resident& R = c->second;
+  if (R.active || R.where == kSpilling || R.where == kRestoring) return;
+  if (R.view) pnt_canvas_destroy(R.view);
+  if (R.where == kResident) {
+    if (R.block)
+      pool.emplace(R.cap, R.block);
+    else
+      held -= R.cap * 4;  // reserved, never allocated
+  }
+  canvases.erase(c);
+  closing.erase(id);
+}
+
 // The device finished a request (engine thread): its slot is free; the
 // composite goes to the caller through the copier, or the request is done.
 void pnt_server_s::complete(Req* q) {
@@ -692,6 +718,7 @@ void pnt_server_s::complete(Req* q) {
   q->res->active--;
   q->res->last_use = Clock::now();
   finish(q, PNT_OK, "");
+  release_if_closing(q->r.canvas_id);
 }
 
 void pnt_server_s::run_engine() {
@@ -719,6 +746,7 @@ void pnt_server_s::run_engine() {
         slots[(size_t)q->slot].req = nullptr;
         q->res->active--;
         finish(q, st, e);
+        release_if_closing(q->r.canvas_id);
         continue;
       }
       active.push_back(q);
@@ -808,7 +836,9 @@ void pnt_server_s::run_copier() {
     if (j.kind == 0) {
       j.q->res->active--;
       j.q->res->last_use = Clock::now();
+      const int64_t id = j.q->r.canvas_id;
       finish(j.q, st, err);
+      release_if_closing(id);
     } else {
       Resident& R = canvases.find(j.canvas_id)->second;
       if (j.kind == 1) {
@@ -835,6 +865,7 @@ void pnt_server_s::run_copier() {
           R.where = kSpilled;
         }
       }
+      release_if_closing(j.canvas_id);
     }
     cv_engine.notify_all();
   }
@@ -904,6 +935,7 @@ int pnt_server_submit(pnt_server S, const pnt_request* r, int64_t* ticket) {
     q->ticket = S->ticket_seq++;
     S->tickets[q->ticket] = q;
     S->pending[r->canvas_id]++;
+    S->closing.erase(r->canvas_id);  // a new request on the id cancels its release
     S->queue.push_back(q);
     S->planq.push_back(q);
     *ticket = q->ticket;
@@ -948,6 +980,17 @@ int pnt_server_canvas(pnt_server S, int64_t canvas_id, pnt_canvas* canvas, pnt_d
     *canvas = R.view;
     if (composite)
       for (int k = 0; k < 3; ++k) composite->p[k] = R.block + (11 + k) * R.stride;
+  });
+}
+
+int pnt_server_release(pnt_server S, int64_t canvas_id) {
+  return guard_call([&] {
+    PNT_REQUIRE(S, "NULL server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    PNT_REQUIRE(S->canvases.count(canvas_id) || S->pending.count(canvas_id), "unknown canvas");
+    S->closing.insert(canvas_id);
+    S->release_if_closing(canvas_id);
+    S->cv_engine.notify_all();
   });
 }
 

@@ -83,6 +83,11 @@ class Server(_lib.Owned):
             assert hip.hipMemcpy(R[c].ctypes.data, comp.p[c], n * 4, 2) == 0  # device -> host
         return K, S, V, R0, R.reshape(3, rows.value, cols.value)
 
+    def release_canvas(self, canvas_id: int) -> None:
+        """Ends a canvas: released once no request on it is queued or
+        running (a request submitted on the id before then cancels this)."""
+        _lib.call("pnt_server_release", self.h, int(canvas_id))
+
     def stats(self):
         b, c, d = C.c_int64(), C.c_int64(), C.c_int64()
         _lib.call("pnt_server_stats", self.h, C.byref(b), C.byref(c), C.byref(d))

@@ -357,6 +357,7 @@ class ServedRequest:
     radius: float
     footprint: np.ndarray
     background: tuple = (0.85, 0.85, 0.85)
+    last: bool = False  # the canvas' last request in the stream: its client ends it
 
 
 def served_requests(n: int, seed: int, side=(512, 4096), stamps=(10, 1000), foot=(32, 128),
@@ -365,7 +366,9 @@ def served_requests(n: int, seed: int, side=(512, 4096), stamps=(10, 1000), foot
     log-uniform in `side`, stamps U{stamps} (strokes of 20..80 stamps at 2 px
     spacing, scaled from a seeded pool of 16 stroke lists), footprint
     diameter U{foot}; with probability `follow` a request goes to one of the
-    `recent` last new canvases (a follow-up on a resident canvas)."""
+    `recent` last new canvases (a follow-up on a resident canvas).  A
+    canvas' last request carries `last`: its client ends the canvas there,
+    so the set of live canvases stays bounded by the follow-up window."""
     rs = Stream(seed, 95)
     pool = [strokes(1000, 1000, stamps[1], seed * 1000 + k, stamps_per_stroke=(20, 80),
                     reach=0.15) for k in range(16)]
@@ -389,4 +392,9 @@ def served_requests(n: int, seed: int, side=(512, 4096), stamps=(10, 1000), foot
         if r not in cores:
             cores[r] = gaussian_core(2 * r + 1, (2 * r + 1) / 4.0)
         reqs.append(ServedRequest(cid, s_, st, float(r), cores[r]))
+    final = {}
+    for i, q in enumerate(reqs):
+        final[q.canvas_id] = i
+    for i in final.values():
+        reqs[i].last = True
     return reqs

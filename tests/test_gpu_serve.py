@@ -152,3 +152,38 @@ def test_server_rejects_size_change(hip):
         srv.wait(srv.submit(7, 80, 64, st, 4.0, core))
     srv.wait(srv.submit(7, 64, 64, st, 4.0, core))
     srv.close()
+
+
+def test_release_ends_a_canvas(hip):
+    """pnt_server_release: the canvas goes once its requests have run (also
+    while it is spilled to the host); a later request with the same id
+    starts a new canvas, equal to that request run alone on a fresh one."""
+    core = W.gaussian_core(9, 2.25)
+    one = 56 * 64 * 64
+    st = [W.strokes(64, 64, 10, 70 + k, stamps_per_stroke=(4, 8), reach=0.3) for k in range(4)]
+    bg = (0.8, 0.85, 0.9)
+    srv = SV.Server(n_slots=2, hbm_budget_bytes=2 * one + 1024)
+    t = srv.submit(1, 64, 64, st[0], 4.0, core, background=bg)
+    srv.release_canvas(1)  # queued or running: released after the request
+    srv.wait(t)
+    srv.wait(srv.submit(2, 64, 64, st[1], 4.0, core, background=bg))
+    assert srv.stats()["resident_canvases"] == 1
+    with pytest.raises(_lib.PaintyError):
+        srv.canvas_state(1)
+    # a spilled canvas: 2 is evicted by 3 and 4, then released from the host
+    srv.wait(srv.submit(3, 64, 64, st[2], 4.0, core, background=bg))
+    srv.wait(srv.submit(4, 64, 64, st[3], 4.0, core, background=bg))
+    srv.release_canvas(2)
+    srv.release_canvas(3)
+    assert srv.stats()["resident_canvases"] == 1
+    with pytest.raises(_lib.PaintyError):
+        srv.release_canvas(3)  # unknown now
+    # id 1 again: a new canvas
+    srv.wait(srv.submit(1, 64, 64, st[3], 4.0, core, background=bg))
+    K, S, V, R0, R = srv.canvas_state(1)
+    q = dict(canvas_id=1, rows=64, cols=64, background=bg, stamps=st[3], radius=4.0,
+             footprint=core, use_snapshot=True)
+    aK, aS, aV, aR0, aR = run_alone({1: (64, 64, np.float32(bg))}, [q])[1]
+    for name, x, y in (("K", K, aK), ("S", S, aS), ("V", V, aV), ("R0", R0, aR0)):
+        assert np.array_equal(x, y), name
+    srv.close()
