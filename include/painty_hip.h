@@ -256,6 +256,23 @@ typedef struct {
 #define PNT_EXEC_CHAINED 3
 int pnt_imprint_stamps(pnt_brush brush, pnt_canvas canvas,
                        const pnt_stamp* stamps, int64_t n, int mode);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: the same stamps and results as
+ * pnt_imprint_stamps(PNT_EXEC_FUSED) (stamps in stream order on the brush's
+ * own pickup map), with the stamp plan in a fixed 4 MiB device array of the
+ * library instead of library-allocated buffers; the call allocates no device
+ * or pinned memory.  Limits: at most 24576 stamps per call (and 1 MiB of
+ * shared-pixel visit lists), else PNT_EINVAL.  The brush's snapshot buffers
+ * must already fit the canvas (a previous stamp call on a canvas at least as
+ * large, or pnt_brush_update_snapshot), else PNT_EINVAL; so is a canvas
+ * with a drying clock (pnt_canvas_set_clock).  Returns once the
+ * plan is on the device and the stamps are enqueued.  Every call on one GPU
+ * uses that GPU's one array: calls that overlap on one GPU (other threads or
+ * streams) race and must be serialised by the caller. */
+int pnt_imprint_stamps_noalloc(pnt_brush brush, pnt_canvas canvas, const pnt_stamp* stamps,
+                               int64_t n);
+
 /* A stroke list: for each stroke, FootprintBrush::setRadius(radius) with
  * its width x width footprint (FootprintBrush.hxx:46-63: nothing changes
  * within 0.5 of the current radius, otherwise a new footprint and a clean

@@ -128,6 +128,7 @@ SIGNATURES = {
     "pnt_brush_update_snapshot": [_vp, _vp],
     "pnt_imprint": [_vp, _vp, C.c_double, C.c_double, C.c_double],
     "pnt_imprint_stamps": [_vp, _vp, C.POINTER(PntStamp), C.c_int64, C.c_int],
+    "pnt_imprint_stamps_noalloc": [_vp, _vp, C.POINTER(PntStamp), C.c_int64],
     "pnt_imprint_strokes": [_vp, _vp, C.POINTER(PntStroke), C.c_int64, C.c_int],
     "pnt_paint_stroke": [_vp, _vp, _dp, C.c_int64],
     "pnt_batch_run": [_vp, C.POINTER(BatchJob), C.c_int, C.c_int],

@@ -1449,6 +1449,19 @@ int pnt_imprint_stamps(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t 
   });
 }
 
+int pnt_imprint_stamps_noalloc(pnt_brush b, pnt_canvas cv, const pnt_stamp* st, int64_t n) {
+  return guard([&] {
+    check_same_ctx(b, cv);
+    PNT_REQUIRE(n >= 0 && (st || n == 0), "invalid stamp list");
+    PNT_REQUIRE(cv->clock == PNT_CLOCK_OFF,
+                "the no-allocation call takes canvases without a drying clock");
+    b->ctx->use();
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(cv->ctx->stream));
+    run_fused_noalloc(b, cv, st, n);
+    if (cv->ctx->stream != b->ctx->stream) PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
 int pnt_imprint_strokes(pnt_brush b, pnt_canvas cv, const pnt_stroke* S, int64_t ns, int mode) {
   return guard([&] {
     check_same_ctx(b, cv);

@@ -1069,6 +1069,133 @@ void run_chained(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t 
     run_chained_t<32>(b, cv, st, n);
 }
 
+// ------------------------------------------------------ no-allocation call --
+// pnt_imprint_stamps_noalloc: the stamps of a list in stream order on the
+// brush's own pickup map (PNT_EXEC_FUSED, one chain), with the plan -- the
+// stamp descriptors and shared-pixel visit lists the kernels read -- in a
+// file-scope device array instead of library-allocated buffers, so the call
+// allocates no device or pinned memory.  Each GPU holds its own copy of the
+// array; its address is looked up once per device with hipGetSymbolAddress
+// and kept here, and only that address is handed to kernels and copies.
+constexpr int64_t kScratchBytes = 4 << 20;
+constexpr int64_t kScratchStampBytes = 3 << 20;  // stamp descriptors; visit lists after
+__device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[64] = {};  // per device: the array's device address
+}  // namespace
+
+// The device address of `scratch` on the current device (looked up on the
+// first call there).
+static char* scratch_on(int device) {
+  PNT_REQUIRE(device >= 0 && device < 64, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    PNT_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<char*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
+void run_fused_noalloc(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n) {
+  hipStream_t s = b->ctx->stream;
+  char* dev = scratch_on(b->ctx->device);
+  // host plan (thread-local buffers keep their capacity between calls)
+  thread_local std::vector<PStamp> h_st;
+  thread_local std::vector<int2> h_dl;
+  thread_local std::vector<int> h_off;
+  h_st.clear();
+  h_dl.clear();
+  h_off.clear();
+  plan_stamps(b, cv, st, n, h_st, h_dl, h_off);
+  const int64_t bst = (int64_t)(h_st.size() * sizeof(PStamp));
+  const int64_t bdl = (int64_t)(h_dl.size() * sizeof(int2));
+  const int64_t boff = (int64_t)(h_off.size() * sizeof(int));
+  PNT_REQUIRE(bst <= kScratchStampBytes,
+              "stamp list too long for the no-allocation call (at most 24576 stamps)");
+  PNT_REQUIRE(round_up(bdl, 256) + boff <= kScratchBytes - kScratchStampBytes,
+              "shared-pixel visit lists too long for the no-allocation call");
+  // the brush's snapshot and dirty bytes must already fit this canvas
+  if (b->use_snap) {
+    PNT_REQUIRE(b->dirty && round_up(cv->n, 16) + 16 <= b->dirty_cap &&
+                    b->snap && 7 * round_up(std::max<int64_t>(cv->n, 1), 64) <= b->snap_cap,
+                "the brush's snapshot buffers are not sized for this canvas (no-allocation "
+                "call); stamp this canvas once with pnt_imprint_stamps first");
+    brush_prepare_snapshot(b, cv, s);  // copies or marks; allocates nothing at these sizes
+  }
+  // the plan into the array, then the kernels read it (the copy is waited for
+  // before returning control of the host buffers)
+  if (bst) PNT_HIP_CHECK(hipMemcpyAsync(dev, h_st.data(), (size_t)bst, hipMemcpyHostToDevice, s));
+  char* ldl = dev + kScratchStampBytes;
+  char* loff = ldl + round_up(bdl, 256);
+  if (bdl) PNT_HIP_CHECK(hipMemcpyAsync(ldl, h_dl.data(), (size_t)bdl, hipMemcpyHostToDevice, s));
+  if (boff)
+    PNT_HIP_CHECK(hipMemcpyAsync(loff, h_off.data(), (size_t)boff, hipMemcpyHostToDevice, s));
+  PNT_HIP_CHECK(hipStreamSynchronize(s));
+  FArgs a;
+  a.st = reinterpret_cast<const PStamp*>(dev);
+  a.canvas = cv->layer();
+  a.snap = b->use_snap ? b->snap_planes() : cv->layer();
+  a.pick = b->pick_planes();
+  a.fp = b->fp;
+  a.fb = FpBox{b->fbox[0], b->fbox[1], b->fbox[2], b->fbox[3]};
+  a.dirty = b->dirty;
+  a.dl = reinterpret_cast<const int2*>(ldl);
+  a.dl_off = reinterpret_cast<const int*>(loff);
+  a.use_snap = b->use_snap ? 1 : 0;
+  a.rows = cv->rows;
+  a.cols = cv->cols;
+  a.hr = (b->size_map - 1) / 2;
+  a.size_map = b->size_map;
+  const int side = 2 * a.hr + 1;
+  a.tiles_x = (side + 15) / 16;
+  a.ntiles = a.tiles_x * a.tiles_x;
+  a.pickup_rate = b->pickup_rate;
+  a.dep_rate = b->dep_rate;
+  a.cap = b->cap;
+  a.clk = cv->clock != PNT_CLOCK_OFF && cv->tmap
+              ? DryClock{cv->tmap, 0, cv->drying_ms, cv->R0(), cv->h()}
+              : DryClock{};
+  for (int64_t k = 0; k < n; ++k) {
+    if (st[k].flags & PNT_STAMP_DIP) {  // dip(): clean() and the new paint
+      launch_fill(b->pick, 7 * b->mstride, 0.0f, s);
+      for (int c = 0; c < 3; ++c) {
+        b->paintK[c] = st[k].K[c];
+        b->paintS[c] = st[k].S[c];
+      }
+    }
+    for (int c = 0; c < 3; ++c) {
+      a.pk[c] = b->paintK[c];
+      a.ps[c] = b->paintS[c];
+    }
+    const PStamp& p = h_st[(size_t)k];
+    const int rb = a.use_snap ? ring_blocks_for(p.box, 256) : 0;
+    const int jk = (int)k;
+    if (!(p.flags & (kStDupX | kStDupY))) {
+      k_stamp_fused<16><<<a.ntiles + rb, 256, 0, s>>>(a, jk, a.ntiles, rb);
+    } else {
+      if (rb) k_stamp_fused<16><<<rb, 256, 0, s>>>(a, jk, 0, rb);
+      k_stamp_fused<16><<<a.ntiles, 256, 0, s>>>(a, jk, a.ntiles, 0);
+      if (p.dl_nlev) k_stamp_levels<<<1, 256, 0, s>>>(a, jk);
+    }
+    PNT_HIP_CHECK(hipGetLastError());
+    if (st[k].flags & PNT_STAMP_DRY_AFTER) {
+      if (b->use_snap)
+        launch_dry_dirty(cv->n, cv->layer(), cv->R0(), cv->h(), b->dirty, s);
+      else
+        launch_dry(cv->n, cv->layer(), cv->R0(), cv->h(), s);
+      if (a.clk.tmap) launch_fill_i64(cv->tmap, cv->n, canvas_now_us(cv), s);
+    }
+  }
+  cv->epoch++;
+  if (b->use_snap) {
+    b->dirty_canvas = cv;
+    b->dirty_epoch = cv->epoch;
+  }
+}
+
 void run_fused_strokes(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* strokes, int64_t ns) {
   int kT = 16;
   if (const char* e = std::getenv("PAINTY_FUSED_TILE")) kT = std::atoi(e) == 32 ? 32 : 16;

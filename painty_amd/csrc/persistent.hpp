@@ -70,6 +70,9 @@ void plan_stamps_rc(int size_map, double radius, int rows, int cols, const pnt_s
 // chains: concurrent stroke chains at most (0: PAINTY_FUSED_CHAINS or 4)
 void run_fused(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n, int chains = 0);
 void run_fused_strokes(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stroke* strokes, int64_t ns);
+// The fused stamps of a list in stream order with the plan in a file-scope
+// device array (pnt_imprint_stamps_noalloc).
+void run_fused_noalloc(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 void run_chained(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64_t n);
 struct ChainState;
 void chain_release(ChainState*& c);
