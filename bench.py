@@ -13,8 +13,8 @@ prints them in ONE JSON line:
   duration measured with HIP events around each launch on its stream.
 * composite: the KM composite on paint-bearing input (the same canvas before
   stamping: 70% of pixels carry paint in each layer), single layer
-  (Renderer::compose, 52 B/px) and two layers (dry composed onto the
-  substrate, then wet, 80 B/px), with rooflines.
+  (Renderer::compose, 56 B/px) and two layers (dry composed onto the
+  substrate, then wet, 88 B/px), with rooflines.
 * batched: 256 canvases (sides 256..2048, 50..500 stamps, footprints
   32..128 px) stamped and composited in one launch vs the same jobs as plain
   per-canvas launches.
@@ -50,11 +50,12 @@ sys.path.insert(0, ROOT)
 METRIC = "KM composite Gpixel/s + stamps/s; p50/p99 served latency vs plain GPU path"
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes (FP32 SoA minimum traffic)
-BYTES_COMPOSE1 = 52   # K3 S3 V R0_3 in, R_3 out
-BYTES_COMPOSE2 = 80   # dry K3 S3 V + wet K3 S3 V + R0_3 in, R_3 out
-BYTES_DRY = 88        # 11 planes read, 11 written
-BYTES_VISIT = 148     # canvas K,S,V rw 56 + snapshot K,S,V r + V w 32 + pickup rw 56 + footprint 4
-BYTES_RING = 56       # snapshot refresh: 7 floats read + 7 written
+# K, S, R0, R, h and the pickup map are FP32; the volume V is FP64 (8 B).
+BYTES_COMPOSE1 = 56   # K3 S3 V R0_3 in, R_3 out
+BYTES_COMPOSE2 = 88   # dry K3 S3 V + wet K3 S3 V + R0_3 in, R_3 out
+BYTES_DRY = 96        # K3 S3 V R0_3 h read, the same written
+BYTES_VISIT = 164     # canvas K,S,V rw 64 + snapshot K,S,V r + V w 40 + pickup rw 56 + footprint 4
+BYTES_RING = 64       # snapshot refresh: K,S,V read (32) + written (32)
 
 
 class HipEvents:
@@ -208,9 +209,9 @@ def cpu_baseline(wl, n_stamps: int, comp_rows: int, threads: int):
 # ------------------------------------------------------ large single canvas --
 def leg_composite(wl, reps: int = 20):
     """The KM composite on paint-bearing input: the canvas's wet layer over
-    the raw substrate (k_compose_vec<1>, Renderer::compose, 52 B/px), and the
+    the raw substrate (k_compose_vec<1>, Renderer::compose, 56 B/px), and the
     dry layer then the wet layer over the substrate in one pass
-    (k_compose_vec<2>, 80 B/px).  Live HIP events on the library's stream."""
+    (k_compose_vec<2>, 88 B/px).  Live HIP events on the library's stream."""
     from painty_amd import _lib
     from painty_amd import hip as H
     ctx = H.default_context()
@@ -225,8 +226,7 @@ def leg_composite(wl, reps: int = 20):
     Lw, R0, _ = wet.device_planes()
     Ld, _, _ = dry.device_planes()
     _, Ro, _ = out.device_planes()
-    lw = _lib.DevLayer((C.c_void_p * 7)(*Lw))
-    ld = _lib.DevLayer((C.c_void_p * 7)(*Ld))
+    lw, ld = Lw, Ld
     r0 = _lib.DevRgb((C.c_void_p * 3)(*R0))
     ro = _lib.DevRgb((C.c_void_p * 3)(*Ro))
     ctx.synchronize()
@@ -234,9 +234,9 @@ def leg_composite(wl, reps: int = 20):
            "wet_fraction": round(float((wl.wet.V != 0).mean()), 4),
            "dry_fraction": round(float((wl.dry.V != 0).mean()), 4)}
     for name, call, nbytes, kernel in (
-            ("single_layer", lambda: _lib.call("pnt_dev_compose", ctx.h, n, lw, r0, ro, None),
+            ("single_layer", lambda: _lib.call("pnt_dev_compose_wet", ctx.h, n, lw, r0, ro, None),
              BYTES_COMPOSE1, "k_compose_vec<1>"),
-            ("two_layer", lambda: _lib.call("pnt_dev_compose2", ctx.h, n, ld, lw, r0, ro, None),
+            ("two_layer", lambda: _lib.call("pnt_dev_compose2_wet", ctx.h, n, ld, lw, r0, ro, None),
              BYTES_COMPOSE2, "k_compose_vec<2>")):
         call()
         a, b = ev.event(), ev.event()
@@ -279,7 +279,7 @@ def leg_single(args, wl):
     out = H.Canvas(wl.rows, wl.cols)  # its R0 planes receive the composite
     Lc, Rc, _ = cv.device_planes()
     _, Ro, _ = out.device_planes()
-    dev_layer = _lib.DevLayer((C.c_void_p * 7)(*Lc))
+    dev_layer = Lc
     dev_r0 = _lib.DevRgb((C.c_void_p * 3)(*Rc))
     dev_out = _lib.DevRgb((C.c_void_p * 3)(*Ro))
     n_px = wl.rows * wl.cols
@@ -335,7 +335,7 @@ def leg_single(args, wl):
             ev.record(call_events[0], stream)
             _lib.call("pnt_imprint_strokes", brush.h, cv.h, strokes, len(segs), mode)
             ev.record(call_events[1], stream)
-            _lib.call("pnt_dev_compose", ctx.h, n_px, dev_layer, dev_r0, dev_out, None)
+            _lib.call("pnt_dev_compose_wet", ctx.h, n_px, dev_layer, dev_r0, dev_out, None)
             return
         for k, (bid, arr, dry_after) in enumerate(segs):
             brush.setRadius(wl.radii[bid], wl.cores[bid])
@@ -344,7 +344,7 @@ def leg_single(args, wl):
             ev.record(seg_events[k][1], stream)
             if dry_after:
                 _lib.call("pnt_canvas_dry", cv.h)
-        _lib.call("pnt_dev_compose", ctx.h, n_px, dev_layer, dev_r0, dev_out, None)
+        _lib.call("pnt_dev_compose_wet", ctx.h, n_px, dev_layer, dev_r0, dev_out, None)
 
     def stamp_ms():
         if one_call:  # the dries run inside the call

@@ -77,11 +77,16 @@ int pnt_layer_shape(pnt_layer layer, int* rows, int* cols);
 /* clear() -- PaintLayer.hxx:68-74 */
 int pnt_layer_clear(pnt_layer layer);
 /* Write / read the K, S, V buffers (getK_buffer/getS_buffer/getV_buffer,
- * PaintLayer.hxx:32-54).  Any pointer may be NULL to skip that buffer. */
+ * PaintLayer.hxx:32-54).  Any pointer may be NULL to skip that buffer.  The
+ * device holds K and S in FP32 and V in FP64 (as the reference's Mat<double>
+ * V): FP32 V is widened exactly on upload and rounded on download. */
 int pnt_layer_upload(pnt_layer layer, const float* K, const float* S,
                      const float* V, int layout);
 int pnt_layer_download(pnt_layer layer, float* K, float* S, float* V,
                        int layout);
+/* The V buffer in FP64, exactly as the device holds it (rows*cols doubles). */
+int pnt_layer_upload_volume(pnt_layer layer, const double* V);
+int pnt_layer_download_volume(pnt_layer layer, double* V);
 /* copyTo(other) -- PaintLayer.hxx:103-111 (reallocates on size mismatch) */
 int pnt_layer_copy_to(pnt_layer src, pnt_layer dst);
 /* set(i, j, k, s, v) -- PaintLayer.hxx:122-127 */
@@ -435,6 +440,13 @@ typedef struct {
 typedef struct {
   float* p[3];
 } pnt_dev_rgb;
+/* A canvas' wet layer on the device: K0 K1 K2 S0 S1 S2 as FP32 planes and
+ * the volume V as an FP64 plane (the reference holds V in double, and its
+ * MinVolume and drying tests see it). */
+typedef struct {
+  float* ks[6];
+  double* v;
+} pnt_dev_wet;
 /* Renderer::render into caller device planes (see pnt_render). */
 int pnt_dev_render(pnt_canvas canvas, pnt_dev_rgb rgb);
 /* The imSave pixel transform (see pnt_encode) from device planes to device
@@ -448,6 +460,12 @@ int pnt_dev_compose(pnt_ctx ctx, int64_t n, pnt_dev_layer layer,
 int pnt_dev_compose2(pnt_ctx ctx, int64_t n, pnt_dev_layer dry,
                      pnt_dev_layer wet, pnt_dev_rgb R0, pnt_dev_rgb R,
                      void* stream);
+/* The same two on canvas wet layers (pnt_canvas_device_planes). */
+int pnt_dev_compose_wet(pnt_ctx ctx, int64_t n, pnt_dev_wet layer,
+                        pnt_dev_rgb R0, pnt_dev_rgb R, void* stream);
+int pnt_dev_compose2_wet(pnt_ctx ctx, int64_t n, pnt_dev_wet dry,
+                         pnt_dev_wet wet, pnt_dev_rgb R0, pnt_dev_rgb R,
+                         void* stream);
 
 /* ------------------------------------- differentiable composite+stamp -- */
 /* No reference counterpart: the reference has no gradients.  A stamp list
@@ -631,8 +649,9 @@ int pnt_resize_lanczos4(pnt_ctx ctx, const double* src, int rows, int cols,
                         int out_rows, int out_cols, double* out);
 
 /* ---------------------------------------------------- canvas device view - */
-/* Device planes of a canvas (valid while it lives): layer (7), R0 (3), h. */
-int pnt_canvas_device_planes(pnt_canvas canvas, pnt_dev_layer* layer,
+/* Device planes of a canvas (valid while it lives): wet layer (K, S FP32,
+ * V FP64), R0 (3), h. */
+int pnt_canvas_device_planes(pnt_canvas canvas, pnt_dev_wet* layer,
                              pnt_dev_rgb* R0, float** h);
 
 #ifdef __cplusplus

@@ -67,6 +67,11 @@ class DevRgb(C.Structure):
     _fields_ = [("p", C.c_void_p * 3)]
 
 
+class DevWet(C.Structure):
+    """pnt_dev_wet: a canvas' wet layer, K/S FP32 planes and the FP64 V plane."""
+    _fields_ = [("ks", C.c_void_p * 6), ("v", C.c_void_p)]
+
+
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
 _vp = C.c_void_p
@@ -85,6 +90,8 @@ SIGNATURES = {
     "pnt_layer_clear": [_vp],
     "pnt_layer_upload": [_vp, _fp, _fp, _fp, C.c_int],
     "pnt_layer_download": [_vp, _fp, _fp, _fp, C.c_int],
+    "pnt_layer_upload_volume": [_vp, _dp],
+    "pnt_layer_download_volume": [_vp, _dp],
     "pnt_layer_copy_to": [_vp, _vp],
     "pnt_layer_set": [_vp, C.c_int, C.c_int, _fp, _fp, C.c_float],
     "pnt_layer_compose_onto_host": [_vp, _fp, C.c_int, C.c_int, C.c_int],
@@ -182,7 +189,9 @@ SIGNATURES = {
     "pnt_encode": [_vp, C.c_int64, _fp, C.c_int, C.c_int, C.c_int, _vp],
     "pnt_dev_encode": [_vp, C.c_int64, DevRgb, C.c_int, C.c_int, _vp, _vp],
     "pnt_dev_compose2": [_vp, C.c_int64, DevLayer, DevLayer, DevRgb, DevRgb, _vp],
-    "pnt_canvas_device_planes": [_vp, C.POINTER(DevLayer), C.POINTER(DevRgb),
+    "pnt_dev_compose_wet": [_vp, C.c_int64, DevWet, DevRgb, DevRgb, _vp],
+    "pnt_dev_compose2_wet": [_vp, C.c_int64, DevWet, DevWet, DevRgb, DevRgb, _vp],
+    "pnt_canvas_device_planes": [_vp, C.POINTER(DevWet), C.POINTER(DevRgb),
                                  C.POINTER(_vp)],
     "pnt_grad_create": [_vp, C.c_int, C.c_int, _vp, C.POINTER(PntStamp), C.c_int64,
                         C.POINTER(C.c_int32), C.c_int, _dp, _dp, C.c_int, C.c_int,

@@ -65,6 +65,20 @@ void upload_planes(float* const* dst, int nplanes, const float* src, int64_t n, 
   PNT_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Host FP32 volumes <-> a layer's FP64 V plane (widened exactly, rounded back).
+void upload_volume(double* dst, const float* src, int64_t n, hipStream_t s) {
+  std::vector<double> tmp(src, src + n);
+  PNT_HIP_CHECK(hipMemcpyAsync(dst, tmp.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  PNT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void download_volume(float* dst, const double* src, int64_t n, hipStream_t s) {
+  std::vector<double> tmp((size_t)n);
+  PNT_HIP_CHECK(hipMemcpyAsync(tmp.data(), src, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+  PNT_HIP_CHECK(hipStreamSynchronize(s));
+  for (int64_t i = 0; i < n; ++i) dst[i] = (float)tmp[(size_t)i];
+}
+
 void download_planes(float* dst, int nplanes, float* const* src, int64_t n, int layout,
                      hipStream_t s) {
   if (nplanes == 1 || layout == PNT_PLANAR) {
@@ -218,12 +232,12 @@ void brush_full_snapshot(pnt_brush b, pnt_canvas cv) {
   hipStream_t s = b->ctx->stream;
   if (b->snap_rows != cv->rows || b->snap_cols != cv->cols) {
     const int64_t stride = round_up(std::max<int64_t>(cv->n, 1), 64);
-    if (7 * stride > b->snap_cap) {  // grow; a smaller canvas reuses the buffer
-      float* snap = dev_alloc(7 * stride);
+    if (kLayerStrides * stride > b->snap_cap) {  // grow; a smaller canvas reuses the buffer
+      float* snap = dev_alloc(kLayerStrides * stride);
       PNT_HIP_CHECK(hipStreamSynchronize(s));
       dev_free(b->snap);
       b->snap = snap;
-      b->snap_cap = 7 * stride;
+      b->snap_cap = kLayerStrides * stride;
     }
     b->snap_stride = stride;
     b->snap_rows = cv->rows;
@@ -626,7 +640,7 @@ int pnt_layer_create(pnt_ctx ctx, int rows, int cols, pnt_layer* out) {
     L->ctx = ctx;
     try {
       L->alloc(rows, cols);
-      launch_fill(L->base, 7 * L->stride, 0.0f, ctx->stream);
+      launch_fill(L->base, kLayerStrides * L->stride, 0.0f, ctx->stream);  // zero bits: 0.0 too
       PNT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     } catch (...) {
       dev_free(L->base);
@@ -665,7 +679,7 @@ int pnt_layer_clear(pnt_layer L) {
     PNT_REQUIRE(L, "NULL layer");
     L->ctx->use();
     L->touched();
-    launch_fill(L->base, 7 * L->stride, 0.0f, L->ctx->stream);
+    launch_fill(L->base, kLayerStrides * L->stride, 0.0f, L->ctx->stream);
   });
 }
 
@@ -677,9 +691,9 @@ int pnt_layer_upload(pnt_layer L, const float* K, const float* S, const float* V
     if (L->parent) canvas_tick(L->parent);
     L->touched();
     const Layer7 P = L->planes();
-    if (K) upload_planes(P.p, 3, K, L->n, layout, L->ctx->stream);
-    if (S) upload_planes(P.p + 3, 3, S, L->n, layout, L->ctx->stream);
-    if (V) upload_planes(P.p + 6, 1, V, L->n, layout, L->ctx->stream);
+    if (K) upload_planes(P.ks, 3, K, L->n, layout, L->ctx->stream);
+    if (S) upload_planes(P.ks + 3, 3, S, L->n, layout, L->ctx->stream);
+    if (V) upload_volume(P.v, V, L->n, L->ctx->stream);
   });
 }
 
@@ -690,9 +704,32 @@ int pnt_layer_download(pnt_layer L, float* K, float* S, float* V, int layout) {
     L->ctx->use();
     if (L->parent) canvas_tick(L->parent);
     const Layer7 P = L->planes();
-    if (K) download_planes(K, 3, P.p, L->n, layout, L->ctx->stream);
-    if (S) download_planes(S, 3, P.p + 3, L->n, layout, L->ctx->stream);
-    if (V) download_planes(V, 1, P.p + 6, L->n, layout, L->ctx->stream);
+    if (K) download_planes(K, 3, P.ks, L->n, layout, L->ctx->stream);
+    if (S) download_planes(S, 3, P.ks + 3, L->n, layout, L->ctx->stream);
+    if (V) download_volume(V, P.v, L->n, L->ctx->stream);
+  });
+}
+
+int pnt_layer_upload_volume(pnt_layer L, const double* V) {
+  return guard([&] {
+    PNT_REQUIRE(L && V, "NULL argument");
+    L->ctx->use();
+    if (L->parent) canvas_tick(L->parent);
+    L->touched();
+    PNT_HIP_CHECK(hipMemcpyAsync(L->planes().v, V, (size_t)L->n * sizeof(double),
+                                 hipMemcpyHostToDevice, L->ctx->stream));
+    PNT_HIP_CHECK(hipStreamSynchronize(L->ctx->stream));
+  });
+}
+
+int pnt_layer_download_volume(pnt_layer L, double* V) {
+  return guard([&] {
+    PNT_REQUIRE(L && V, "NULL argument");
+    L->ctx->use();
+    if (L->parent) canvas_tick(L->parent);
+    PNT_HIP_CHECK(hipMemcpyAsync(V, L->planes().v, (size_t)L->n * sizeof(double),
+                                 hipMemcpyDeviceToHost, L->ctx->stream));
+    PNT_HIP_CHECK(hipStreamSynchronize(L->ctx->stream));
   });
 }
 
@@ -706,7 +743,7 @@ int pnt_layer_copy_to(pnt_layer src, pnt_layer dst) {
     if (dst->rows != src->rows || dst->cols != src->cols) {
       PNT_REQUIRE(dst->owner, "cannot resize the paint layer of a canvas");
       PNT_HIP_CHECK(hipStreamSynchronize(dst->ctx->stream));
-      float* nb = dev_alloc(7 * round_up(std::max<int64_t>(src->n, 1), 64));
+      float* nb = dev_alloc(kLayerStrides * round_up(std::max<int64_t>(src->n, 1), 64));
       dev_free(dst->base);
       dst->base = nb;
       dst->rows = src->rows;
@@ -727,11 +764,14 @@ int pnt_layer_set(pnt_layer L, int i, int j, const float k[3], const float s[3],
     L->ctx->use();
     L->touched();
     const int64_t idx = (int64_t)i * L->cols + j;
-    const float vals[7] = {k[0], k[1], k[2], s[0], s[1], s[2], v};
+    const float vals[6] = {k[0], k[1], k[2], s[0], s[1], s[2]};
+    const double vd = v;
     const Layer7 P = L->planes();
-    for (int c = 0; c < 7; ++c)
-      PNT_HIP_CHECK(hipMemcpyAsync(P.p[c] + idx, &vals[c], sizeof(float), hipMemcpyHostToDevice,
+    for (int c = 0; c < 6; ++c)
+      PNT_HIP_CHECK(hipMemcpyAsync(P.ks[c] + idx, &vals[c], sizeof(float), hipMemcpyHostToDevice,
                                    L->ctx->stream));
+    PNT_HIP_CHECK(hipMemcpyAsync(P.v + idx, &vd, sizeof(double), hipMemcpyHostToDevice,
+                                 L->ctx->stream));
     PNT_HIP_CHECK(hipStreamSynchronize(L->ctx->stream));
   });
 }
@@ -777,8 +817,8 @@ int pnt_layer_compose_onto_canvas(pnt_layer L, pnt_canvas cv) {
 static void canvas_clear(pnt_canvas cv) {
   hipStream_t s = cv->ctx->stream;
   cv->epoch++;
-  launch_fill(cv->base, 7 * cv->stride, 0.0f, s);
-  for (int c = 0; c < 3; ++c) launch_fill(cv->base + (7 + c) * cv->stride, cv->n, cv->bg[c], s);
+  launch_fill(cv->base, kLayerStrides * cv->stride, 0.0f, s);  // zero bits: V = 0.0 too
+  for (int c = 0; c < 3; ++c) launch_fill(cv->R0().p[c], cv->n, cv->bg[c], s);
   launch_fill(cv->h(), cv->n, 0.0f, s);
   canvas_stamp_time(cv);  // Canvas.hxx:47-50
 }
@@ -786,8 +826,9 @@ static void canvas_clear(pnt_canvas cv) {
 }  // extern "C"
 
 namespace pnt {
-// A canvas on a caller-provided buffer of at least 11*round_up(rows*cols, 64)
-// floats, which pnt_canvas_destroy leaves alone.
+// A canvas on a caller-provided buffer of at least
+// kCanvasStrides*round_up(rows*cols, 64) floats, which pnt_canvas_destroy
+// leaves alone.
 pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base, bool clear) {
   PNT_REQUIRE(ctx && base, "NULL argument");
   PNT_REQUIRE(rows > 0 && cols > 0 && (int64_t)rows * cols < ((int64_t)1 << 31),
@@ -835,7 +876,7 @@ int pnt_canvas_create(pnt_ctx ctx, int rows, int cols, pnt_canvas* out) {
     cv->n = (int64_t)rows * cols;
     cv->stride = round_up(cv->n, 64);
     try {
-      cv->base = dev_alloc(11 * cv->stride);
+      cv->base = dev_alloc(kCanvasStrides * cv->stride);
       canvas_clear(cv);
       PNT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     } catch (...) {
@@ -1070,14 +1111,15 @@ int pnt_canvas_get_drying_time_ms(pnt_canvas cv, int64_t* ms) {
   });
 }
 
-int pnt_canvas_device_planes(pnt_canvas cv, pnt_dev_layer* layer, pnt_dev_rgb* R0, float** h) {
+int pnt_canvas_device_planes(pnt_canvas cv, pnt_dev_wet* layer, pnt_dev_rgb* R0, float** h) {
   return guard([&] {
     PNT_REQUIRE(cv, "NULL canvas");
     canvas_tick(cv);
     cv->epoch++;  // the caller may write the planes
     if (layer) {
       const Layer7 L = cv->layer();
-      for (int i = 0; i < 7; ++i) layer->p[i] = L.p[i];
+      for (int i = 0; i < 6; ++i) layer->ks[i] = L.ks[i];
+      layer->v = L.v;
     }
     if (R0) {
       const Rgb3 r = cv->R0();
@@ -1214,12 +1256,22 @@ int pnt_compute_reflectance(pnt_ctx ctx, int64_t n, const float* K, const float*
   });
 }
 
-static Layer7 to7(const pnt_dev_layer& l) {
-  Layer7 L;
+static Plane7 to7(const pnt_dev_layer& l) {
+  Plane7 L;
   for (int i = 0; i < 7; ++i) {
     PNT_REQUIRE(l.p[i] != nullptr, "NULL device plane");
     L.p[i] = l.p[i];
   }
+  return L;
+}
+static Layer7 to_wet(const pnt_dev_wet& l) {
+  Layer7 L;
+  for (int i = 0; i < 6; ++i) {
+    PNT_REQUIRE(l.ks[i] != nullptr, "NULL device plane");
+    L.ks[i] = l.ks[i];
+  }
+  PNT_REQUIRE(l.v != nullptr, "NULL device plane");
+  L.v = l.v;
   return L;
 }
 static Rgb3 to3(const pnt_dev_rgb& r) {
@@ -1247,6 +1299,25 @@ int pnt_dev_compose2(pnt_ctx ctx, int64_t n, pnt_dev_layer dry, pnt_dev_layer we
     PNT_REQUIRE(ctx && n >= 0, "invalid argument");
     ctx->use();
     launch_compose2(n, to7(dry), to7(wet), to3(R0), to3(R),
+                    stream ? (hipStream_t)stream : ctx->stream);
+  });
+}
+
+int pnt_dev_compose_wet(pnt_ctx ctx, int64_t n, pnt_dev_wet layer, pnt_dev_rgb R0, pnt_dev_rgb R,
+                        void* stream) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && n >= 0, "invalid argument");
+    ctx->use();
+    launch_compose(n, to_wet(layer), to3(R0), to3(R), stream ? (hipStream_t)stream : ctx->stream);
+  });
+}
+
+int pnt_dev_compose2_wet(pnt_ctx ctx, int64_t n, pnt_dev_wet dry, pnt_dev_wet wet,
+                         pnt_dev_rgb R0, pnt_dev_rgb R, void* stream) {
+  return guard([&] {
+    PNT_REQUIRE(ctx && n >= 0, "invalid argument");
+    ctx->use();
+    launch_compose2(n, to_wet(dry), to_wet(wet), to3(R0), to3(R),
                     stream ? (hipStream_t)stream : ctx->stream);
   });
 }
@@ -1323,7 +1394,7 @@ int pnt_brush_get_pickup_map(pnt_brush b, float* K, float* S, float* V, int layo
     PNT_REQUIRE(b, "NULL brush");
     check_layout(layout);
     b->ctx->use();
-    const Layer7 P = b->pick_planes();
+    const Plane7 P = b->pick_planes();
     if (K) download_planes(K, 3, P.p, b->m, layout, b->ctx->stream);
     if (S) download_planes(S, 3, P.p + 3, b->m, layout, b->ctx->stream);
     if (V) download_planes(V, 1, P.p + 6, b->m, layout, b->ctx->stream);

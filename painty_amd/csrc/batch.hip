@@ -47,7 +47,8 @@ namespace {
 constexpr int64_t kComposeItem = 1 << 18;  // pixels per composite work item
 
 struct BJob {
-  Layer7 canvas, pick, snap;
+  Layer7 canvas, snap;
+  Plane7 pick;
   Rgb3 R0, out;
   float* h;
   const float* fp;
@@ -205,23 +206,21 @@ __global__ __launch_bounds__(kWT* kWT) void k_batch(BArgs A) {
           bool nz = false;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            K[c] = J.canvas.p[c][i];
-            S[c] = J.canvas.p[3 + c][i];
+            K[c] = J.canvas.ks[c][i];
+            S[c] = J.canvas.ks[3 + c][i];
             R[c] = J.R0.p[c][i];
             nz |= (K[c] != 0.0f) | (S[c] != 0.0f);
           }
-          const float v = J.canvas.p[6][i];
-          nz |= v != 0.0f;
+          const double v = J.canvas.v[i];
+          nz |= v != 0.0;
           if (!nz) continue;
-          J.h[i] += v;
-          km_pixel(K, S, R, v);
+          J.h[i] = h_plus(J.h[i], v);
+          km_pixel(K, S, R, (float)v);
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            J.R0.p[c][i] = R[c];
-            J.canvas.p[c][i] = 0.0f;
-            J.canvas.p[3 + c][i] = 0.0f;
-          }
-          J.canvas.p[6][i] = 0.0f;
+          for (int c = 0; c < 3; ++c) J.R0.p[c][i] = R[c];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) J.canvas.ks[c][i] = 0.0f;
+          J.canvas.v[i] = 0.0;
           if (J.use_snap) J.dirty[i] = 1;
         }
         if (!group_sync(bar, nb, rank, A.host_err)) {
@@ -262,11 +261,12 @@ __global__ __launch_bounds__(kWT* kWT) void k_batch(BArgs A) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         r[c] = reinterpret_cast<const float4*>(J.R0.p[c])[q];
-        k[c] = reinterpret_cast<const float4*>(J.canvas.p[c])[q];
-        s[c] = reinterpret_cast<const float4*>(J.canvas.p[3 + c])[q];
+        k[c] = reinterpret_cast<const float4*>(J.canvas.ks[c])[q];
+        s[c] = reinterpret_cast<const float4*>(J.canvas.ks[3 + c])[q];
       }
-      const float4 v = reinterpret_cast<const float4*>(J.canvas.p[6])[q];
-      const float vv[4] = {v.x, v.y, v.z, v.w};
+      const double2 v01 = reinterpret_cast<const double2*>(J.canvas.v)[2 * q];
+      const double2 v23 = reinterpret_cast<const double2*>(J.canvas.v)[2 * q + 1];
+      const float vv[4] = {(float)v01.x, (float)v01.y, (float)v23.x, (float)v23.y};
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
         float K[3], S[3], R[3];
@@ -287,11 +287,11 @@ __global__ __launch_bounds__(kWT* kWT) void k_batch(BArgs A) {
       float K[3], S[3], R[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        K[c] = J.canvas.p[c][i];
-        S[c] = J.canvas.p[3 + c][i];
+        K[c] = J.canvas.ks[c][i];
+        S[c] = J.canvas.ks[3 + c][i];
         R[c] = J.R0.p[c][i];
       }
-      km_pixel(K, S, R, J.canvas.p[6][i]);
+      km_pixel(K, S, R, (float)J.canvas.v[i]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) J.out.p[c][i] = R[c];
     }
@@ -762,9 +762,9 @@ static void save_or_restore(const pnt_batch_job* jobs, int n, std::vector<SavedJ
     pnt_canvas cv = jobs[j].canvas;
     SavedJob& S = sv[j];
     if (save) {
-      S.canvas_n = 11 * cv->stride;
+      S.canvas_n = kCanvasStrides * cv->stride;
       S.pick_n = 7 * b->mstride;
-      S.snap_n = b->snap ? 7 * b->snap_stride : 0;
+      S.snap_n = b->snap ? kLayerStrides * b->snap_stride : 0;
       S.dirty_n = b->dirty ? b->dirty_cap : 0;
       S.canvas = dev_alloc(S.canvas_n);
       S.pick = dev_alloc(S.pick_n);

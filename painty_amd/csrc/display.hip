@@ -61,17 +61,17 @@ __global__ __launch_bounds__(kBlock) void k_render(RenderArgs a) {
   float K[3], S[3], Kd[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    K[c] = a.L.p[c][p];
-    S[c] = a.L.p[3 + c][p];
+    K[c] = a.L.ks[c][p];
+    S[c] = a.L.ks[3 + c][p];
     Kd[c] = a.R0.p[c][p];
   }
-  const float* V = a.L.p[6];
-  const float s11 = V[p];
+  const double* V = a.L.v;  // thickness, evaluated in FP32 below
+  const float s11 = (float)V[p];
   km_pixel(K, S, Kd, s11);  // compR = compose(canvas)
-  const float s01 = V[(int64_t)i * a.cols + reflect_index(j - 1, a.cols)];
-  const float s21 = V[(int64_t)i * a.cols + reflect_index(j + 1, a.cols)];
-  const float s10 = V[(int64_t)reflect_index(i - 1, a.rows) * a.cols + j];
-  const float s12 = V[(int64_t)reflect_index(i + 1, a.rows) * a.cols + j];
+  const float s01 = (float)V[(int64_t)i * a.cols + reflect_index(j - 1, a.cols)];
+  const float s21 = (float)V[(int64_t)i * a.cols + reflect_index(j + 1, a.cols)];
+  const float s10 = (float)V[(int64_t)reflect_index(i - 1, a.rows) * a.cols + j];
+  const float s12 = (float)V[(int64_t)reflect_index(i + 1, a.rows) * a.cols + j];
   // normal: normalized(va x vb) with z negated (Renderer.hxx:113-121)
   const F3 va = normalized(F3{2.0f, 0.0f, s21 - s01});
   const F3 vb = normalized(F3{0.0f, 2.0f, s12 - s10});

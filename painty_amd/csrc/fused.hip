@@ -38,7 +38,8 @@ namespace {
 
 struct FArgs {
   const PStamp* st;
-  Layer7 canvas, snap, pick;  // snap == canvas planes when the snapshot is off
+  Layer7 canvas, snap;  // snap == canvas planes when the snapshot is off
+  Plane7 pick;
   const float* fp;
   FpBox fb;  // nonzero footprint cells
   uint8_t* dirty;
@@ -1120,7 +1121,7 @@ void run_fused_noalloc(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, in
   // the brush's snapshot and dirty bytes must already fit this canvas
   if (b->use_snap) {
     PNT_REQUIRE(b->dirty && round_up(cv->n, 16) + 16 <= b->dirty_cap &&
-                    b->snap && 7 * round_up(std::max<int64_t>(cv->n, 1), 64) <= b->snap_cap,
+                    b->snap && kLayerStrides * round_up(std::max<int64_t>(cv->n, 1), 64) <= b->snap_cap,
                 "the brush's snapshot buffers are not sized for this canvas (no-allocation "
                 "call); stamp this canvas once with pnt_imprint_stamps first");
     brush_prepare_snapshot(b, cv, s);  // copies or marks; allocates nothing at these sizes

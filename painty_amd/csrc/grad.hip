@@ -47,7 +47,8 @@ constexpr int kBlk = 256;
 
 // ------------------------------------------------------ visit adjoint ----
 struct VState {
-  float P[7], C[7], X[7];  // pickup cell, canvas pixel, source pixel
+  float P[7];          // pickup cell (FP32, as stored)
+  double C[7], X[7];   // canvas and source pixel (K, S from FP32; V is FP64)
 };
 
 // Adjoint of paint_visit_math (visit.hpp) at input state `in`.  On entry g
@@ -62,8 +63,8 @@ __device__ __forceinline__ void visit_adj(const VState& in, VState& g, float gB[
   const double fh = fh_f;
   const double B[6] = {pk[0], pk[1], pk[2], ps[0], ps[1], ps[2]};
   const float* Pf = in.P;
-  const float* Cf = in.C;
-  const float* Xf = alias ? in.C : in.X;
+  const double* Cf = in.C;
+  const double* Xf = alias ? in.C : in.X;
   double P[7], C[7], X[7];
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
@@ -80,7 +81,7 @@ __device__ __forceinline__ void visit_adj(const VState& in, VState& g, float gB[
   double X6n = vC, iv1 = 0.0;
   bool bb = false;
   if (a) {
-    X6n = (float)(vC - vLeave);  // the stored source volume
+    X6n = vC - vLeave;  // the stored source volume (FP64)
     const double vt1 = vP + vLeave;
     bb = vt1 > kMinVolume;
     if (bb) {
@@ -169,8 +170,8 @@ __device__ __forceinline__ void visit_adj(const VState& in, VState& g, float gB[
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
     g.P[c] = (float)gPin[c];
-    g.C[c] = (float)(alias ? gXin[c] : gCd[c]);
-    g.X[c] = alias ? 0.0f : (float)gXin[c];
+    g.C[c] = alias ? gXin[c] : gCd[c];
+    g.X[c] = alias ? 0.0 : gXin[c];
   }
 }
 
@@ -180,28 +181,34 @@ __device__ __forceinline__ void visit_fwd(VState& s, float fh, double pr, double
                                           const float pk[3], const float ps[3], bool alias) {
   CachedVisit<RegPick> acc;
 #pragma unroll
-  for (int c = 0; c < 7; ++c) {
-    acc.v[c] = s.P[c];
-    acc.cv[c] = s.C[c];
-    acc.sv[c] = alias ? s.C[c] : s.X[c];
+  for (int c = 0; c < 7; ++c) acc.v[c] = s.P[c];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    acc.cv[c] = (float)s.C[c];
+    acc.sv[c] = (float)(alias ? s.C[c] : s.X[c]);
   }
+  acc.cvv = s.C[6];
+  acc.svv = alias ? s.C[6] : s.X[6];
   acc.alias = alias;
   acc.src_moved = false;
   paint_visit_math(acc, fh, pr, dr, cap, pk, ps);
 #pragma unroll
-  for (int c = 0; c < 7; ++c) {
-    s.P[c] = acc.v[c];
+  for (int c = 0; c < 7; ++c) s.P[c] = acc.v[c];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
     s.C[c] = acc.cv[c];
     s.X[c] = alias ? acc.cv[c] : acc.sv[c];
   }
+  s.C[6] = acc.cvv;
+  s.X[6] = alias ? acc.cvv : acc.svv;
 }
 
 // One stamp's tape: canvas and snapshot of the window box as the visits
-// find them (7 planes of `area` floats each), then the pickup map (7 planes
-// of m).
+// find them (7 FP64 planes of `area` each: K, S widened, V as stored), then
+// the pickup map (7 FP32 planes of m).
 struct Tape {
-  const float* C;
-  const float* X;
+  const double* C;
+  const double* X;
   const float* P;
   int64_t area, m;
   int bx0, by0, bw, bh;
@@ -213,7 +220,7 @@ struct AArgs {
   int hr, size_map, rows, cols, tiles_x, ntiles, use_snap;
   const float* fp;
   Tape t;
-  Layer7 gC, gX, gP;  // adjoints: canvas (caller's planes), snapshot, pickup map
+  Plane7 gC, gX, gP;  // adjoints: canvas (caller's planes), snapshot, pickup map
   float* gB;          // 6 floats of this stamp's paint adjoint
   double pr, dr, cap;
   float pk[3], ps[3];
@@ -247,7 +254,7 @@ __device__ __forceinline__ void load_in(const AArgs& A, int64_t ic, int mm, VSta
   for (int c = 0; c < 7; ++c) {
     s.P[c] = A.t.P[c * A.t.m + mm];
     s.C[c] = A.t.C[c * A.t.area + k];
-    s.X[c] = A.use_snap ? A.t.X[c * A.t.area + k] : 0.0f;
+    s.X[c] = A.use_snap ? A.t.X[c * A.t.area + k] : 0.0;
   }
 }
 
@@ -262,8 +269,8 @@ __device__ __forceinline__ void load_g_pixel(const AArgs& A, int64_t ic, VState&
 __device__ __forceinline__ void store_g_pixel(const AArgs& A, int64_t ic, const VState& g) {
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
-    A.gC.p[c][ic] = g.C[c];
-    if (A.use_snap) A.gX.p[c][ic] = g.X[c];
+    A.gC.p[c][ic] = (float)g.C[c];
+    if (A.use_snap) A.gX.p[c][ic] = (float)g.X[c];
   }
 }
 
@@ -318,7 +325,7 @@ __global__ __launch_bounds__(kGT* kGT) void k_adj_visits(AArgs A) {
 #pragma unroll
         for (int c = 0; c < 7; ++c) {
           in2.C[c] = A.t.C[c * A.t.area + k2];
-          in2.X[c] = A.use_snap ? A.t.X[c * A.t.area + k2] : 0.0f;
+          in2.X[c] = A.use_snap ? A.t.X[c * A.t.area + k2] : 0.0;
         }
         load_g_pixel(A, ic2, gs);
         visit_adj(in2, gs, gB, fh, A.pr, A.dr, A.cap, A.pk, A.ps, alias);
@@ -348,9 +355,9 @@ __global__ __launch_bounds__(kBlk) void k_adj_shared(AArgs A) {
   VState* rec = reinterpret_cast<VState*>(A.scratch);
   const StampGeom g = geom_of(A, st);
   const bool alias = !A.use_snap;
-  float* wC = const_cast<float*>(A.t.C);  // C, X (7 planes of area each), P (7 of m)
-  float* wX = wC + 7 * A.t.area;
-  float* wP = wC + 14 * A.t.area;
+  double* wC = const_cast<double*>(A.t.C);  // C, X (7 planes of area each), P (7 of m)
+  double* wX = const_cast<double*>(A.t.X);
+  float* wP = const_cast<float*>(A.t.P);
   for (int L = 0; L < st.dl_nlev; ++L) {  // replay in plan order, a level at a time
     const int e0 = A.dl_off[st.dl_lev + L], e1 = A.dl_off[st.dl_lev + L + 1];
     for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlk) {
@@ -364,7 +371,7 @@ __global__ __launch_bounds__(kBlk) void k_adj_shared(AArgs A) {
       for (int c = 0; c < 7; ++c) {
         sv.P[c] = wP[c * A.t.m + mm];
         sv.C[c] = wC[c * A.t.area + k];
-        sv.X[c] = A.use_snap ? wX[c * A.t.area + k] : 0.0f;
+        sv.X[c] = A.use_snap ? wX[c * A.t.area + k] : 0.0;
       }
       rec[e - base] = sv;
       visit_fwd(sv, A.fp[mm], A.pr, A.dr, A.cap, A.pk, A.ps, alias);
@@ -402,7 +409,7 @@ __global__ __launch_bounds__(kBlk) void k_adj_shared(AArgs A) {
 }
 
 // Adjoint of the ring copy canvas -> snapshot (FootprintBrush.hxx:307-318).
-__global__ __launch_bounds__(kBlk) void k_adj_ring(RingBox b, int cols, Layer7 gC, Layer7 gX) {
+__global__ __launch_bounds__(kBlk) void k_adj_ring(RingBox b, int cols, Plane7 gC, Plane7 gX) {
   const int w = b.a_x1 - b.a_x0 + 1;
   const int64_t total = (int64_t)w * (b.a_y1 - b.a_y0 + 1);
   for (int64_t e = (int64_t)blockIdx.x * kBlk + threadIdx.x; e < total;
@@ -419,7 +426,7 @@ __global__ __launch_bounds__(kBlk) void k_adj_ring(RingBox b, int cols, Layer7 g
 }
 
 // Adjoint of the first imprint's full canvas -> snapshot copy.
-__global__ __launch_bounds__(kBlk) void k_adj_full_copy(int64_t n, Layer7 gC, Layer7 gX) {
+__global__ __launch_bounds__(kBlk) void k_adj_full_copy(int64_t n, Plane7 gC, Plane7 gX) {
   for (int64_t i = (int64_t)blockIdx.x * kBlk + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlk)
 #pragma unroll
     for (int c = 0; c < 7; ++c) gC.p[c][i] += gX.p[c][i];
@@ -474,10 +481,10 @@ __device__ __forceinline__ void km_adj_channel(float K, float Sin, float R0, flo
   *gR0 = gR * dR0;
 }
 
-__global__ __launch_bounds__(kBlk) void k_km_adj(int64_t n, Layer7 L, Rgb3 R0, Rgb3 gR, Layer7 gL,
+__global__ __launch_bounds__(kBlk) void k_km_adj(int64_t n, Layer7 L, Rgb3 R0, Rgb3 gR, Plane7 gL,
                                                  Rgb3 gR0) {
   for (int64_t i = (int64_t)blockIdx.x * kBlk + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlk) {
-    const float d = L.p[6][i];
+    const float d = (float)L.v[i];
     float gd = 0.0f;
     if (__builtin_fabsf(d) < kKmThr) {  // R = R0
 #pragma unroll
@@ -490,7 +497,7 @@ __global__ __launch_bounds__(kBlk) void k_km_adj(int64_t n, Layer7 L, Rgb3 R0, R
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         float gk, gs, gr0;
-        km_adj_channel(L.p[c][i], L.p[3 + c][i], R0.p[c][i], d, gR.p[c][i], &gk, &gs, &gd, &gr0);
+        km_adj_channel(L.ks[c][i], L.ks[3 + c][i], R0.p[c][i], d, gR.p[c][i], &gk, &gs, &gd, &gr0);
         gL.p[c][i] = gk;
         gL.p[3 + c][i] = gs;
         gR0.p[c][i] = gr0;
@@ -500,16 +507,48 @@ __global__ __launch_bounds__(kBlk) void k_km_adj(int64_t n, Layer7 L, Rgb3 R0, R
   }
 }
 
-// Copies a w x h box at (x0, y0) of 7 planes with row length `cols` to or
-// from 7 dense planes of w*h floats (plane stride `stride`).
-// A w x h box of 7 planes (row stride cols) <-> a dense buffer (plane stride
-// `stride`); up to three boxes per launch.
+// A w x h box at (x0, y0) of a layer's planes (row stride `cols`) <-> a
+// dense buffer of 7 planes (plane stride `stride`): FP64 planes for a
+// canvas or snapshot layer (K, S widened exactly, V as stored), FP32 planes
+// for the pickup map.  Up to three boxes per launch.
 struct BoxRegion {
-  Layer7 L;
+  Layer7 L;   // when layer
+  Plane7 P;   // otherwise
+  int layer;
   int cols, x0, y0, w, h;
-  float* buf;
+  void* buf;  // double* (layer) or float*
   int64_t stride;
 };
+
+inline BoxRegion layer_box(const Layer7& L, int cols, int x0, int y0, int w, int h, double* buf,
+                           int64_t stride) {
+  BoxRegion r{};
+  r.L = L;
+  r.layer = 1;
+  r.cols = cols;
+  r.x0 = x0;
+  r.y0 = y0;
+  r.w = w;
+  r.h = h;
+  r.buf = buf;
+  r.stride = stride;
+  return r;
+}
+
+inline BoxRegion plane_box(const Plane7& P, int cols, int x0, int y0, int w, int h, float* buf,
+                           int64_t stride) {
+  BoxRegion r{};
+  r.P = P;
+  r.layer = 0;
+  r.cols = cols;
+  r.x0 = x0;
+  r.y0 = y0;
+  r.w = w;
+  r.h = h;
+  r.buf = buf;
+  r.stride = stride;
+  return r;
+}
 
 __global__ __launch_bounds__(kBlk) void k_box_copy(BoxRegion r0, BoxRegion r1, BoxRegion r2,
                                                    int64_t n0, int64_t n1, int64_t n2,
@@ -520,12 +559,26 @@ __global__ __launch_bounds__(kBlk) void k_box_copy(BoxRegion r0, BoxRegion r1, B
     const BoxRegion& r = t < n0 ? r0 : (t < n0 + n1 ? r1 : r2);
     const int64_t e = t < n0 ? t : (t < n0 + n1 ? t - n0 : t - n0 - n1);
     const int64_t i = (int64_t)(r.y0 + e / r.w) * r.cols + (r.x0 + e % r.w);
+    if (r.layer) {
+      double* b = static_cast<double*>(r.buf);
+      if (to_buf) {
 #pragma unroll
-    for (int c = 0; c < 7; ++c) {
-      if (to_buf)
-        r.buf[c * r.stride + e] = r.L.p[c][i];
-      else
-        r.L.p[c][i] = r.buf[c * r.stride + e];
+        for (int c = 0; c < 6; ++c) b[c * r.stride + e] = r.L.ks[c][i];
+        b[6 * r.stride + e] = r.L.v[i];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) r.L.ks[c][i] = (float)b[c * r.stride + e];
+        r.L.v[i] = b[6 * r.stride + e];
+      }
+    } else {
+      float* b = static_cast<float*>(r.buf);
+#pragma unroll
+      for (int c = 0; c < 7; ++c) {
+        if (to_buf)
+          b[c * r.stride + e] = r.P.p[c][i];
+        else
+          r.P.p[c][i] = b[c * r.stride + e];
+      }
     }
   }
 }
@@ -604,8 +657,8 @@ void ensure(T*& p, int64_t& cap, int64_t need, hipStream_t s) {
   cap = need;
 }
 
-Layer7 dense7(float* base, int64_t stride) {
-  Layer7 L;
+Plane7 dense7(float* base, int64_t stride) {
+  Plane7 L;
   for (int c = 0; c < 7; ++c) L.p[c] = base + c * stride;
   return L;
 }
@@ -703,8 +756,9 @@ int pnt_grad_forward(pnt_grad g, pnt_dev_layer wet0, pnt_dev_rgb R0, const doubl
     PNT_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
     const Layer7 L = cv->layer();
     const Rgb3 R0c = cv->R0();
-    for (int c = 0; c < 7; ++c)
-      PNT_HIP_CHECK(hipMemcpyAsync(L.p[c], wet0.p[c], n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    Plane7 W0;
+    for (int c = 0; c < 7; ++c) W0.p[c] = wet0.p[c];
+    launch_layer_from_planes(n, W0, L, s);  // V widened to FP64
     for (int c = 0; c < 3; ++c)
       PNT_HIP_CHECK(hipMemcpyAsync(R0c.p[c], R0.p[c], n * sizeof(float), hipMemcpyDeviceToDevice, s));
     cv->epoch++;
@@ -750,20 +804,22 @@ int pnt_grad_forward(pnt_grad g, pnt_dev_layer wet0, pnt_dev_rgb R0, const doubl
         y1 = std::max(y1, by1);
       }
       pnt_grad_s::Seg sg{s0, s1, x0, y0, std::max(x1 - x0 + 1, 0), std::max(y1 - y0 + 1, 0), total};
-      total += (int64_t)sg.w * sg.h * (b->use_snap ? 14 : 7) + 7 * m;
+      // canvas and snapshot boxes as 7 FP64 planes (14 floats per pixel
+      // each), the pickup map as 7 FP32 planes; offsets kept 8-byte aligned
+      total += round_up((int64_t)sg.w * sg.h * (b->use_snap ? 28 : 14) + 7 * m, 2);
       g->segs.push_back(sg);
       s0 = s1;
     }
     ensure(g->ckpt, g->ckpt_cap, std::max<int64_t>(total, 1), s);
-    const Layer7 pick = b->pick_planes();
+    const Plane7 pick = b->pick_planes();
     for (const auto& sg : g->segs) {
       const int64_t area = (int64_t)sg.w * sg.h;
       float* c0 = g->ckpt + sg.off;
-      box_copy({BoxRegion{L, cv->cols, sg.x0, sg.y0, sg.w, sg.h, c0, area},
-                BoxRegion{b->snap_planes(), cv->cols, sg.x0, sg.y0, b->use_snap ? sg.w : 0, sg.h,
-                          c0 + 7 * area, area},
-                BoxRegion{pick, b->size_map, 0, 0, b->size_map, b->size_map,
-                          c0 + (b->use_snap ? 14 : 7) * area, m}},
+      box_copy({layer_box(L, cv->cols, sg.x0, sg.y0, sg.w, sg.h, reinterpret_cast<double*>(c0), area),
+                layer_box(b->snap_planes(), cv->cols, sg.x0, sg.y0, b->use_snap ? sg.w : 0, sg.h,
+                          reinterpret_cast<double*>(c0 + 14 * area), area),
+                plane_box(pick, b->size_map, 0, 0, b->size_map, b->size_map,
+                          c0 + (b->use_snap ? 28 : 14) * area, m)},
                true, s);
       run_fused(b, cv, g->st.data() + sg.s0, sg.s1 - sg.s0);
     }
@@ -787,7 +843,7 @@ int pnt_grad_backward(pnt_grad g, pnt_dev_rgb gR, pnt_dev_layer gwet, pnt_dev_rg
     const int sm = b->size_map, hr = (sm - 1) / 2;
     const int ns = (int)g->st.size();
     const Layer7 L = cv->layer();
-    const Layer7 gC{{gwet.p[0], gwet.p[1], gwet.p[2], gwet.p[3], gwet.p[4], gwet.p[5], gwet.p[6]}};
+    const Plane7 gC{{gwet.p[0], gwet.p[1], gwet.p[2], gwet.p[3], gwet.p[4], gwet.p[5], gwet.p[6]}};
     // ---- composite adjoint at the final state
     k_km_adj<<<grid_for(n), kBlk, 0, s>>>(n, L, cv->R0(), Rgb3{{gR.p[0], gR.p[1], gR.p[2]}}, gC,
                                           Rgb3{{gR0.p[0], gR0.p[1], gR0.p[2]}});
@@ -801,16 +857,18 @@ int pnt_grad_backward(pnt_grad g, pnt_dev_rgb gR, pnt_dev_layer gwet, pnt_dev_rg
     ensure(g->gB, g->gB_cap, 6 * (int64_t)ns, s);
     launch_fill(g->gP, 7 * m, 0.0f, s);
     launch_fill(g->gB, 6 * (int64_t)ns, 0.0f, s);
-    const Layer7 gXL = b->use_snap ? dense7(g->gX, nst) : gC;
-    const Layer7 gPL = dense7(g->gP, m);
+    const Plane7 gXL = b->use_snap ? dense7(g->gX, nst) : gC;
+    const Plane7 gPL = dense7(g->gP, m);
     // tapes of one segment
     const int64_t bwmax = 2 * (int64_t)hr + 2;
     const int64_t area = bwmax * bwmax;
-    const int64_t slot = 14 * area + 7 * m;
+    // per stamp: C and X as 7 FP64 planes of area (28 floats per pixel),
+    // then P as 7 FP32 planes of m
+    const int64_t slot = round_up(28 * area + 7 * m, 2);
     ensure(g->tape, g->tape_cap, slot * g->seg_len, s);
     const int side = 2 * hr + 1;
     const int tiles_x = (side + kGT - 1) / kGT;
-    const Layer7 pick = b->pick_planes();
+    const Plane7 pick = b->pick_planes();
     std::vector<float> paints(6 * (size_t)g->seg_len);
     for (int k = (int)g->segs.size() - 1; k >= 0; --k) {
       const auto& sg = g->segs[k];
@@ -827,10 +885,10 @@ int pnt_grad_backward(pnt_grad g, pnt_dev_rgb gR, pnt_dev_layer gwet, pnt_dev_rg
       // ---- restore the segment's checkpoint
       const int64_t carea = (int64_t)sg.w * sg.h;
       float* c0 = g->ckpt + sg.off;
-      box_copy({BoxRegion{L, cv->cols, sg.x0, sg.y0, sg.w, sg.h, c0, carea},
-                BoxRegion{b->snap_planes(), cv->cols, sg.x0, sg.y0, b->use_snap ? sg.w : 0, sg.h,
-                          c0 + 7 * carea, carea},
-                BoxRegion{pick, sm, 0, 0, sm, sm, c0 + (b->use_snap ? 14 : 7) * carea, m}},
+      box_copy({layer_box(L, cv->cols, sg.x0, sg.y0, sg.w, sg.h, reinterpret_cast<double*>(c0), carea),
+                layer_box(b->snap_planes(), cv->cols, sg.x0, sg.y0, b->use_snap ? sg.w : 0, sg.h,
+                          reinterpret_cast<double*>(c0 + 14 * carea), carea),
+                plane_box(pick, sm, 0, 0, sm, sm, c0 + (b->use_snap ? 28 : 14) * carea, m)},
                false, s);
       {  // brush paint at the segment start: its last dip (stamp 0 dips)
         int i = sg.s0;
@@ -860,16 +918,19 @@ int pnt_grad_backward(pnt_grad g, pnt_dev_rgb gR, pnt_dev_layer gwet, pnt_dev_rg
         if (b->use_snap) launch_ring_refresh(p.box, cv->cols, L, b->snap_planes(), s);
         Tape& t = tapes[q];
         float* base = g->tape + slot * q;
-        t.C = base;
-        t.X = base + 7 * area;
-        t.P = base + 14 * area;
+        double* tC = reinterpret_cast<double*>(base);
+        double* tX = reinterpret_cast<double*>(base + 14 * area);
+        float* tP = base + 28 * area;
+        t.C = tC;
+        t.X = tX;
+        t.P = tP;
         t.area = area;
         t.m = m;
         window_box(p, cv->rows, cv->cols, &t.bx0, &t.by0, &t.bw, &t.bh);
-        box_copy({BoxRegion{L, cv->cols, t.bx0, t.by0, t.bw, t.bh, base, area},
-                  BoxRegion{b->snap_planes(), cv->cols, t.bx0, t.by0, b->use_snap ? t.bw : 0, t.bh,
-                            base + 7 * area, area},
-                  BoxRegion{pick, sm, 0, 0, sm, sm, base + 14 * area, m}},
+        box_copy({layer_box(L, cv->cols, t.bx0, t.by0, t.bw, t.bh, tC, area),
+                  layer_box(b->snap_planes(), cv->cols, t.bx0, t.by0, b->use_snap ? t.bw : 0, t.bh,
+                            tX, area),
+                  plane_box(pick, sm, 0, 0, sm, sm, tP, m)},
                  true, s);
         launch_stamp_main(a, s);
         if (p.dl_nlev > 0) launch_stamp_serial(a, D.dl, D.dl_off + p.dl_lev, p.dl_nlev, s);

@@ -57,17 +57,13 @@ struct pnt_layer_s {
   float* base = nullptr;
   bool owner = true;
   pnt_canvas_s* parent = nullptr;  // the canvas whose wet layer this views
-  pnt::Layer7 planes() const {
-    pnt::Layer7 L;
-    for (int i = 0; i < 7; ++i) L.p[i] = base + i * stride;
-    return L;
-  }
+  pnt::Layer7 planes() const { return pnt::layer_at(base, stride); }
   void alloc(int r, int c) {
     rows = r;
     cols = c;
     n = (int64_t)r * c;
     stride = pnt::round_up(n > 0 ? n : 1, 64);
-    base = pnt::dev_alloc(7 * stride);
+    base = pnt::dev_alloc(pnt::kLayerStrides * stride);
   }
   void touched();  // the wet layer changed outside a brush's tracking
 };
@@ -76,7 +72,7 @@ struct pnt_canvas_s {
   pnt_ctx ctx = nullptr;
   int rows = 0, cols = 0;
   int64_t n = 0, stride = 0;
-  float* base = nullptr;     // planes 0..6 wet layer, 7..9 R0, 10 h
+  float* base = nullptr;     // wet layer (strides 0..7: K, S, then V in FP64), 8..10 R0, 11 h
   bool owns_base = true;     // false: base belongs to the creator (canvas_over)
   float* scratch = nullptr;  // 3 planes for host-side compose results
   pnt_layer_s view;
@@ -100,9 +96,9 @@ struct pnt_canvas_s {
   uint64_t epoch = 1;
   pnt::Layer7 layer() const { return view.planes(); }
   pnt::Rgb3 R0() const {
-    return pnt::Rgb3{{base + 7 * stride, base + 8 * stride, base + 9 * stride}};
+    return pnt::Rgb3{{base + 8 * stride, base + 9 * stride, base + 10 * stride}};
   }
-  float* h() const { return base + 10 * stride; }
+  float* h() const { return base + 11 * stride; }
 };
 
 inline void pnt_layer_s::touched() {
@@ -118,8 +114,8 @@ struct pnt_brush_s {
   int64_t fp_cap = 0, pick_cap = 0;  // floats allocated at fp / pick
   std::vector<float> fp_host;  // same footprint on the host (traffic replay)
   int fbox[4] = {0, 0, 1 << 30, 1 << 30};  // nonzero footprint cells: x0 y0 x1 y1
-  float* pick = nullptr;       // pickup map: 7 planes of mstride
-  float* snap = nullptr;       // snapshot: 7 planes of snap_stride
+  float* pick = nullptr;       // pickup map: 7 FP32 planes of mstride
+  float* snap = nullptr;       // snapshot: a layer (8 strides of snap_stride)
   int snap_rows = 0, snap_cols = 0;
   int64_t snap_stride = 0, snap_cap = 0;  // floats allocated at snap
   bool use_snap = true;
@@ -141,16 +137,12 @@ struct pnt_brush_s {
   int64_t last_launches = 0;  // stamp-kernel launches of the last fused stamp-list call
   pnt::KernelTimer* ktimer = nullptr;  // per-launch event pairs (pnt_brush_set_kernel_timing)
 
-  pnt::Layer7 pick_planes() const {
-    pnt::Layer7 L;
+  pnt::Plane7 pick_planes() const {
+    pnt::Plane7 L;
     for (int i = 0; i < 7; ++i) L.p[i] = pick + i * mstride;
     return L;
   }
-  pnt::Layer7 snap_planes() const {
-    pnt::Layer7 L;
-    for (int i = 0; i < 7; ++i) L.p[i] = snap + i * snap_stride;
-    return L;
-  }
+  pnt::Layer7 snap_planes() const { return pnt::layer_at(snap, snap_stride); }
   pnt::BrushArgs args() const {
     pnt::BrushArgs b;
     b.fp = fp;
@@ -193,7 +185,7 @@ void footprint_box(const std::vector<float>& fp, int size_map, int box[4]);
 void palette_from_json(const char* text, int64_t len, std::vector<double>& K,
                        std::vector<double>& S);
 std::string palette_to_json(const double* K, const double* S, int64_t n);
-// A canvas on a caller buffer of at least 11*round_up(rows*cols, 64) floats
+// A canvas on a caller buffer of at least kCanvasStrides*round_up(rows*cols, 64) floats
 // that pnt_canvas_destroy does not free: cleared (background 1) when `clear`,
 // else a view of the planes as they are.
 pnt_canvas canvas_over(pnt_ctx ctx, int rows, int cols, float* base, bool clear = true);

@@ -24,10 +24,32 @@ struct Error : std::runtime_error {
       throw ::pnt::Error(PNT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// A paint layer on the device: 7 planes K0 K1 K2 S0 S1 S2 V.
+// A paint layer on the device: FP32 planes K0 K1 K2 S0 S1 S2 and the
+// volume plane V in FP64.  Volume feeds the reference's MinVolume and
+// drying thresholds (FootprintBrush.hxx:26, Canvas.hxx:123-155); held in
+// FP64, as the reference holds it, every such test takes the reference's
+// branch.  V lies at float offset 6*stride of the layer's block (two float
+// strides wide).
 struct Layer7 {
+  float* ks[6];
+  double* v;
+};
+// Seven FP32 planes K0 K1 K2 S0 S1 S2 V: the pickup map, and caller-owned
+// device layers (pnt_dev_layer).
+struct Plane7 {
   float* p[7];
 };
+// The layer whose planes start at `base`, `stride` floats apart.
+__host__ __device__ inline Layer7 layer_at(float* base, int64_t stride) {
+  Layer7 L;
+  for (int i = 0; i < 6; ++i) L.ks[i] = base + i * stride;
+  L.v = reinterpret_cast<double*>(base + 6 * stride);
+  return L;
+}
+// Strides a layer takes (6 FP32 planes + the FP64 plane), and a canvas
+// block: its wet layer, R0 (3 planes, strides 8..10) and h (stride 11).
+constexpr int kLayerStrides = 8;
+constexpr int kCanvasStrides = 12;
 struct Rgb3 {
   float* p[3];
 };
@@ -35,7 +57,7 @@ struct Rgb3 {
 // Brush state and parameters one imprint needs.
 struct BrushArgs {
   const float* fp;   // footprint, size_map^2
-  Layer7 pick;       // pickup map, size_map^2 planes
+  Plane7 pick;       // pickup map, size_map^2 planes
   double pickup_rate, dep_rate, cap;
   float paintK[3], paintS[3];
 };
@@ -70,6 +92,10 @@ struct RingBox {
 void launch_compose(int64_t n, const Layer7& L, const Rgb3& R0, const Rgb3& R, hipStream_t s);
 void launch_compose2(int64_t n, const Layer7& dry, const Layer7& wet, const Rgb3& R0,
                      const Rgb3& R, hipStream_t s);
+// The same over caller-owned FP32 planes (pnt_dev_layer).
+void launch_compose(int64_t n, const Plane7& L, const Rgb3& R0, const Rgb3& R, hipStream_t s);
+void launch_compose2(int64_t n, const Plane7& dry, const Plane7& wet, const Rgb3& R0,
+                     const Rgb3& R, hipStream_t s);
 void launch_dry(int64_t n, const Layer7& L, const Rgb3& R0, float* h, hipStream_t s);
 void launch_dry_step(int64_t n, const Layer7& L, const Rgb3& R0, float portion,
                      hipStream_t s);
@@ -94,6 +120,9 @@ void launch_render(int rows, int cols, const Layer7& L, const Rgb3& R0, const Rg
 void launch_encode(int64_t n, const Rgb3& in, int to_srgb, int bits, void* out, hipStream_t s);
 void launch_check_dry_pixel(const DryClock& c, const Layer7& canvas, int64_t ic, hipStream_t s);
 void launch_copy_layer(int64_t n, const Layer7& src, const Layer7& dst, hipStream_t s);
+// A layer from seven FP32 planes (V widened exactly), and back (V rounded).
+void launch_layer_from_planes(int64_t n, const Plane7& src, const Layer7& dst, hipStream_t s);
+void launch_planes_from_layer(int64_t n, const Layer7& src, const Plane7& dst, hipStream_t s);
 
 // ---- stamping (kernels.hip) ----
 void launch_ring_refresh(const RingBox& box, int cols, const Layer7& canvas, const Layer7& snap,

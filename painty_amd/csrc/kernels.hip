@@ -1,7 +1,8 @@
 // gfx950 kernels: KM composite, drying, imprint stamping.
 //
-// Composite and dry are streaming kernels over FP32 planes (HBM-bound): one
-// thread handles four consecutive pixels with 16-byte loads from every plane.
+// Composite and dry are streaming kernels over the layer planes (FP32 K/S,
+// FP64 V; HBM-bound): one thread handles four consecutive pixels with
+// 16-byte loads from every FP32 plane and 32 bytes of volumes.
 // Stamping follows renderer/FootprintBrush.hxx:73-143 visit by visit; see the
 // comment above k_stamp_main for how the reference's serial order is kept.
 #include <algorithm>
@@ -26,16 +27,44 @@ __device__ __forceinline__ float& lane(float4& v, int j) {
   return reinterpret_cast<float*>(&v)[j];
 }
 
+// Four FP64 volumes of pixels 4i..4i+3 (32-byte aligned).
+__device__ __forceinline__ void ldd4(const double* p, int64_t i, double v[4]) {
+  const double2 a = reinterpret_cast<const double2*>(p)[2 * i];
+  const double2 b = reinterpret_cast<const double2*>(p)[2 * i + 1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ __forceinline__ void std4(double* p, int64_t i, const double v[4]) {
+  reinterpret_cast<double2*>(p)[2 * i] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2*>(p)[2 * i + 1] = make_double2(v[2], v[3]);
+}
+
 // ------------------------------------------------------------ composite ---
+// The K/S planes and the thickness (FP32 for the KM evaluation) of a
+// canvas layer (FP64 volume) or of a caller's seven FP32 planes.
+__device__ __forceinline__ const float* ksp(const Layer7& L, int c) { return L.ks[c]; }
+__device__ __forceinline__ const float* ksp(const Plane7& L, int c) { return L.p[c]; }
+__device__ __forceinline__ float4 thick4(const Layer7& L, int64_t i) {
+  double v[4];
+  ldd4(L.v, i, v);
+  return make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+}
+__device__ __forceinline__ float4 thick4(const Plane7& L, int64_t i) { return ld4(L.p[6], i); }
+__device__ __forceinline__ float thick1(const Layer7& L, int64_t i) { return (float)L.v[i]; }
+__device__ __forceinline__ float thick1(const Plane7& L, int64_t i) { return L.p[6][i]; }
+
 // Applies one layer (K, S, V planes) to the reflectance r of pixels 4i..4i+3.
-__device__ __forceinline__ void apply_layer4(const Layer7& L, int64_t i, float4 r[3]) {
+template <class LT>
+__device__ __forceinline__ void apply_layer4(const LT& L, int64_t i, float4 r[3]) {
   float4 k[3], s[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    k[c] = ld4(L.p[c], i);
-    s[c] = ld4(L.p[3 + c], i);
+    k[c] = ld4(ksp(L, c), i);
+    s[c] = ld4(ksp(L, 3 + c), i);
   }
-  float4 v = ld4(L.p[6], i);
+  float4 v = thick4(L, i);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     float K[3], S[3], R[3];
@@ -51,22 +80,22 @@ __device__ __forceinline__ void apply_layer4(const Layer7& L, int64_t i, float4 
   }
 }
 
-__device__ __forceinline__ void apply_layer1(const Layer7& L, int64_t i, float R[3]) {
+template <class LT>
+__device__ __forceinline__ void apply_layer1(const LT& L, int64_t i, float R[3]) {
   float K[3], S[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    K[c] = L.p[c][i];
-    S[c] = L.p[3 + c][i];
+    K[c] = ksp(L, c)[i];
+    S[c] = ksp(L, 3 + c)[i];
   }
-  km_pixel(K, S, R, L.p[6][i]);
+  km_pixel(K, S, R, thick1(L, i));
 }
 
 // Renderer::compose (Renderer.hxx:26-41) for kLayers == 1; for kLayers == 2
 // the dry layer A is composed first (PaintLayer::composeOnto, :81-96) and the
 // wet layer B over the result, without writing the intermediate.
-template <int kLayers>
-__global__ __launch_bounds__(kBlock) void k_compose_vec(int64_t n4, Layer7 A, Layer7 B, Rgb3 R0,
-                                                        Rgb3 R) {
+template <int kLayers, class LT>
+__global__ __launch_bounds__(kBlock) void k_compose_vec(int64_t n4, LT A, LT B, Rgb3 R0, Rgb3 R) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n4) return;
   float4 r[3];
@@ -78,9 +107,9 @@ __global__ __launch_bounds__(kBlock) void k_compose_vec(int64_t n4, Layer7 A, La
   for (int c = 0; c < 3; ++c) st4(R.p[c], i, r[c]);
 }
 
-template <int kLayers>
-__global__ __launch_bounds__(kBlock) void k_compose_scalar(int64_t begin, int64_t n, Layer7 A,
-                                                           Layer7 B, Rgb3 R0, Rgb3 R) {
+template <int kLayers, class LT>
+__global__ __launch_bounds__(kBlock) void k_compose_scalar(int64_t begin, int64_t n, LT A, LT B,
+                                                           Rgb3 R0, Rgb3 R) {
   const int64_t i = begin + (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   float r[3];
@@ -99,20 +128,18 @@ __device__ __forceinline__ void dry1(const Layer7& L, const Rgb3& R0, float* h, 
   float K[3], S[3], R[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    K[c] = L.p[c][i];
-    S[c] = L.p[3 + c][i];
+    K[c] = L.ks[c][i];
+    S[c] = L.ks[3 + c][i];
     R[c] = R0.p[c][i];
   }
-  const float v = L.p[6][i];
-  h[i] += v;
-  km_pixel(K, S, R, v);
+  const double v = L.v[i];
+  h[i] = h_plus(h[i], v);
+  km_pixel(K, S, R, (float)v);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    R0.p[c][i] = R[c];
-    L.p[c][i] = 0.0f;
-    L.p[3 + c][i] = 0.0f;
-  }
-  L.p[6][i] = 0.0f;
+  for (int c = 0; c < 3; ++c) R0.p[c][i] = R[c];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) L.ks[c][i] = 0.0f;
+  L.v[i] = 0.0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_dry_vec(int64_t n4, Layer7 L, Rgb3 R0, float* h) {
@@ -122,18 +149,18 @@ __global__ __launch_bounds__(kBlock) void k_dry_vec(int64_t n4, Layer7 L, Rgb3 R
 #pragma unroll
   for (int c = 0; c < 3; ++c) r[c] = ld4(R0.p[c], i);
   float4 hv = ld4(h, i);
-  const float4 v = ld4(L.p[6], i);
+  double v[4];
+  ldd4(L.v, i, v);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) lane(hv, j) += reinterpret_cast<const float*>(&v)[j];
+  for (int j = 0; j < 4; ++j) lane(hv, j) = h_plus(lane(hv, j), v[j]);
   apply_layer4(L, i, r);
   const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    st4(R0.p[c], i, r[c]);
-    st4(L.p[c], i, z);
-    st4(L.p[3 + c], i, z);
-  }
-  st4(L.p[6], i, z);
+  for (int c = 0; c < 3; ++c) st4(R0.p[c], i, r[c]);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) st4(L.ks[c], i, z);
+  const double zd[4] = {0.0, 0.0, 0.0, 0.0};
+  std4(L.v, i, zd);
   st4(h, i, hv);
 }
 
@@ -144,26 +171,30 @@ __global__ __launch_bounds__(kBlock) void k_dry_scalar(int64_t begin, int64_t n,
 }
 
 // DryingStep.compute.glsl:66-76: V leaves up to `portion`; R0 takes it.
+// The volume arithmetic is FP64 (V is); the shader's KM runs in FP32.
+__device__ __forceinline__ double dry_step_px(double d, float portion, float K[3], float S[3],
+                                              float R[3]) {
+#pragma clang fp contract(off)
+  const double rem = fmax(0.0, d - (double)portion);
+  const float leave = (float)(d - rem);
+  if (__builtin_fabsf(leave) > 10e-6f) km_pixel_glsl(K, S, R, leave);
+  return rem;
+}
+
 __global__ __launch_bounds__(kBlock) void k_dry_step(int64_t n, Layer7 L, Rgb3 R0,
                                                      float portion) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const float d = L.p[6][i];
-  const float rem = fmaxf(0.0f, d - portion);
-  const float leave = d - rem;
-  if (__builtin_fabsf(leave) > 10e-6f) {
-    float K[3], S[3], R[3];
+  float K[3], S[3], R[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      K[c] = L.p[c][i];
-      S[c] = L.p[3 + c][i];
-      R[c] = R0.p[c][i];
-    }
-    km_pixel_glsl(K, S, R, leave);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) R0.p[c][i] = R[c];
+  for (int c = 0; c < 3; ++c) {
+    K[c] = L.ks[c][i];
+    S[c] = L.ks[3 + c][i];
+    R[c] = R0.p[c][i];
   }
-  L.p[6][i] = rem;
+  L.v[i] = dry_step_px(L.v[i], portion, K, S, R);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) R0.p[c][i] = R[c];
 }
 
 // `steps` applications of k_dry_step's per-pixel update, stopping early once
@@ -173,26 +204,23 @@ __global__ __launch_bounds__(kBlock) void k_dry_steps(int64_t n, Layer7 L, Rgb3 
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   bool still = false;
   if (i < n) {
-    float d = L.p[6][i];
-    if (d != 0.0f) {  // V = 0 stays as it is under any number of steps
+    double d = L.v[i];
+    if (d != 0.0) {  // V = 0 stays as it is under any number of steps
       float K[3], S[3], R[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        K[c] = L.p[c][i];
-        S[c] = L.p[3 + c][i];
+        K[c] = L.ks[c][i];
+        S[c] = L.ks[3 + c][i];
         R[c] = R0.p[c][i];
       }
       for (int k = 0; k < steps; ++k) {
-        const float rem = fmaxf(0.0f, d - portion);
-        const float leave = d - rem;
-        if (__builtin_fabsf(leave) > 10e-6f) km_pixel_glsl(K, S, R, leave);
-        d = rem;
-        if (!(d > 0.0f)) break;
+        d = dry_step_px(d, portion, K, S, R);
+        if (!(d > 0.0)) break;
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) R0.p[c][i] = R[c];
-      L.p[6][i] = d;
-      still = d > 0.0f;
+      L.v[i] = d;
+      still = d > 0.0;
     }
   }
   // one atomic per wave that still holds paint
@@ -231,7 +259,25 @@ __global__ __launch_bounds__(kBlock) void k_copy_layer(int64_t n, Layer7 src, La
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
 #pragma unroll
-  for (int c = 0; c < 7; ++c) dst.p[c][i] = src.p[c][i];
+  for (int c = 0; c < 6; ++c) dst.ks[c][i] = src.ks[c][i];
+  dst.v[i] = src.v[i];
+}
+
+// A layer from seven FP32 planes (V widened exactly) and back (V rounded).
+__global__ __launch_bounds__(kBlock) void k_layer_from_planes(int64_t n, Plane7 src, Layer7 dst) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) dst.ks[c][i] = src.p[c][i];
+  dst.v[i] = (double)src.p[6][i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_planes_from_layer(int64_t n, Layer7 src, Plane7 dst) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) dst.p[c][i] = src.ks[c][i];
+  dst.p[6][i] = (float)src.v[i];
 }
 
 // ------------------------------------------------------------ stamping ---
@@ -243,10 +289,13 @@ struct GlobalVisit {
   int im;
   __device__ float pick(int c) const { return a.b.pick.p[c][im]; }
   __device__ void set_pick(int c, float v) const { a.b.pick.p[c][im] = v; }
-  __device__ float src(int c) const { return a.src.p[c][ic]; }
-  __device__ void set_src_v(float v) const { a.src.p[6][ic] = v; }
-  __device__ float canvas(int c) const { return a.canvas.p[c][ic]; }
-  __device__ void set_canvas(int c, float v) const { a.canvas.p[c][ic] = v; }
+  __device__ float src(int c) const { return a.src.ks[c][ic]; }
+  __device__ double src_v() const { return a.src.v[ic]; }
+  __device__ void set_src_v(double v) const { a.src.v[ic] = v; }
+  __device__ float canvas(int c) const { return a.canvas.ks[c][ic]; }
+  __device__ double canvas_v() const { return a.canvas.v[ic]; }
+  __device__ void set_canvas(int c, float v) const { a.canvas.ks[c][ic] = v; }
+  __device__ void set_canvas_v(double v) const { a.canvas.v[ic] = v; }
 };
 
 // Canvas::checkDry of one pixel, called directly (Canvas.hxx:123-155).
@@ -275,7 +324,8 @@ __global__ __launch_bounds__(kBlock) void k_ring_refresh(RingBox b, int cols, La
     if (row > b.tl_y && row < b.br_y && col > b.tl_x && col < b.br_x) continue;
     const int64_t i = (int64_t)row * cols + col;
 #pragma unroll
-    for (int c = 0; c < 7; ++c) snap.p[c][i] = canvas.p[c][i];
+    for (int c = 0; c < 6; ++c) snap.ks[c][i] = canvas.ks[c][i];
+    snap.v[i] = canvas.v[i];
   }
 }
 
@@ -359,8 +409,13 @@ __global__ __launch_bounds__(kBlock) void k_stamp_serial(StampArgs a, const int2
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-inline bool aligned16(const float* p) { return ((uintptr_t)p & 15u) == 0; }
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 inline bool aligned16(const Layer7& L) {
+  for (auto* p : L.ks)
+    if (!aligned16(p)) return false;
+  return ((uintptr_t)L.v & 31u) == 0;
+}
+inline bool aligned16(const Plane7& L) {
   for (auto* p : L.p)
     if (!aligned16(p)) return false;
   return true;
@@ -374,26 +429,33 @@ inline bool aligned16(const Rgb3& L) {
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
-template <int kL>
-static void launch_compose_t(int64_t n, const Layer7& A, const Layer7& B, const Rgb3& R0,
-                             const Rgb3& R, hipStream_t s) {
+template <int kL, class LT>
+static void launch_compose_t(int64_t n, const LT& A, const LT& B, const Rgb3& R0, const Rgb3& R,
+                             hipStream_t s) {
   if (n <= 0) return;
   int64_t done = 0;
   if (aligned16(A) && aligned16(B) && aligned16(R0) && aligned16(R)) {
     const int64_t n4 = n / 4;
-    if (n4 > 0) k_compose_vec<kL><<<blocks_for(n4), kBlock, 0, s>>>(n4, A, B, R0, R);
+    if (n4 > 0) k_compose_vec<kL, LT><<<blocks_for(n4), kBlock, 0, s>>>(n4, A, B, R0, R);
     done = n4 * 4;
   }
   if (done < n)
-    k_compose_scalar<kL><<<blocks_for(n - done), kBlock, 0, s>>>(done, n, A, B, R0, R);
+    k_compose_scalar<kL, LT><<<blocks_for(n - done), kBlock, 0, s>>>(done, n, A, B, R0, R);
   PNT_HIP_CHECK(hipGetLastError());
 }
 
 void launch_compose(int64_t n, const Layer7& L, const Rgb3& R0, const Rgb3& R, hipStream_t s) {
   launch_compose_t<1>(n, L, L, R0, R, s);
 }
+void launch_compose(int64_t n, const Plane7& L, const Rgb3& R0, const Rgb3& R, hipStream_t s) {
+  launch_compose_t<1>(n, L, L, R0, R, s);
+}
 
 void launch_compose2(int64_t n, const Layer7& dry, const Layer7& wet, const Rgb3& R0,
+                     const Rgb3& R, hipStream_t s) {
+  launch_compose_t<2>(n, dry, wet, R0, R, s);
+}
+void launch_compose2(int64_t n, const Plane7& dry, const Plane7& wet, const Rgb3& R0,
                      const Rgb3& R, hipStream_t s) {
   launch_compose_t<2>(n, dry, wet, R0, R, s);
 }
@@ -444,6 +506,18 @@ void launch_check_dry_pixel(const DryClock& c, const Layer7& canvas, int64_t ic,
 void launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s) {
   if (n <= 0) return;
   k_fill_i64<<<blocks_for(n), kBlock, 0, s>>>(p, n, v);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_layer_from_planes(int64_t n, const Plane7& src, const Layer7& dst, hipStream_t s) {
+  if (n <= 0) return;
+  k_layer_from_planes<<<blocks_for(n), kBlock, 0, s>>>(n, src, dst);
+  PNT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_planes_from_layer(int64_t n, const Layer7& src, const Plane7& dst, hipStream_t s) {
+  if (n <= 0) return;
+  k_planes_from_layer<<<blocks_for(n), kBlock, 0, s>>>(n, src, dst);
   PNT_HIP_CHECK(hipGetLastError());
 }
 

@@ -43,7 +43,8 @@ struct PArgs {
   int n;
   int hr, size_map, rows, cols, tiles_x, ntiles;
   const float* fp;
-  Layer7 pick, canvas, src;
+  Plane7 pick;
+  Layer7 canvas, src;
   uint8_t* dirty;
   int use_snap;
   double pickup_rate, dep_rate, cap;
@@ -148,7 +149,8 @@ __global__ __launch_bounds__(kT* kT) void k_persistent(PArgs A) {
           if (!A.dirty[i]) continue;
           if (pull && ring_pixel_visited(g, b, A.fp, col, row)) continue;
 #pragma unroll
-          for (int c = 0; c < 7; ++c) A.src.p[c][i] = A.canvas.p[c][i];
+          for (int c = 0; c < 6; ++c) A.src.ks[c][i] = A.canvas.ks[c][i];
+          A.src.v[i] = A.canvas.v[i];
           A.dirty[i] = 0;
         }
       }
@@ -269,23 +271,21 @@ __global__ __launch_bounds__(256) void k_dry_dirty(int64_t n, Layer7 L, Rgb3 R0,
   bool nz = false;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    K[c] = L.p[c][i];
-    S[c] = L.p[3 + c][i];
+    K[c] = L.ks[c][i];
+    S[c] = L.ks[3 + c][i];
     R[c] = R0.p[c][i];
     nz |= (K[c] != 0.0f) | (S[c] != 0.0f);
   }
-  const float v = L.p[6][i];
-  nz |= v != 0.0f;
+  const double v = L.v[i];
+  nz |= v != 0.0;
   if (!nz) return;  // drying an empty pixel changes nothing
-  h[i] += v;
-  km_pixel(K, S, R, v);
+  h[i] = h_plus(h[i], v);
+  km_pixel(K, S, R, (float)v);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    R0.p[c][i] = R[c];
-    L.p[c][i] = 0.0f;
-    L.p[3 + c][i] = 0.0f;
-  }
-  L.p[6][i] = 0.0f;
+  for (int c = 0; c < 3; ++c) R0.p[c][i] = R[c];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) L.ks[c][i] = 0.0f;
+  L.v[i] = 0.0;
   dirty[i] = 1;
 }
 

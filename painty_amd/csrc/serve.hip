@@ -30,11 +30,11 @@
 //
 // Admission: requests on one canvas run in submission order, one at a time;
 // a request whose canvas is not resident is admitted when its canvas fits
-// the HBM budget (canvas + composite planes, 56 B/px), after evicting
+// the HBM budget (canvas + composite planes, 60 B/px), after evicting
 // canvases that no queued or running request needs, least recently used
 // first.  An evicted canvas is spilled to host memory by the copier thread,
 // and restored from there when a later request names it.  A request slot's
-// working memory (snapshot 28 B/px, dirty bytes 1 B/px, pickup maps, plan)
+// working memory (snapshot 32 B/px, dirty bytes 1 B/px, pickup maps, plan)
 // is kept by the slot for the next request and is not counted in the budget.
 #include <algorithm>
 #include <chrono>
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kWT2) void k_serve_wave(SArgs A) {
 int stream_blocks(int64_t items) { return wave_stream_blocks(items); }
 
 int64_t canvas_floats(int rows, int cols) {
-  return (11 + 3) * round_up((int64_t)rows * cols, 64);  // canvas planes + composite planes
+  return (kCanvasStrides + 3) * round_up((int64_t)rows * cols, 64);  // canvas + composite planes
 }
 int64_t canvas_bytes(int rows, int cols) { return canvas_floats(rows, cols) * 4; }
 int64_t dirty_bytes_for(int64_t n) { return round_up(n, 16) + 16; }
@@ -148,7 +148,7 @@ struct Req {
 enum Where : int { kResident = 0, kSpilling, kSpilled, kRestoring };
 
 struct Resident {
-  float* block = nullptr;  // canvas planes (11 x stride) then composite planes (3 x stride)
+  float* block = nullptr;  // canvas planes (kCanvasStrides x stride) then composite (3 x stride)
   int64_t cap = 0;         // floats of block
   int rows = 0, cols = 0;
   int64_t stride = 0;
@@ -156,7 +156,7 @@ struct Resident {
   bool initialised = false;
   int active = 0;  // admitted requests not yet complete
   int where = kResident;
-  std::vector<float> host;     // spilled canvas planes (11 x stride)
+  std::vector<float> host;     // spilled canvas planes (kCanvasStrides x stride)
   pnt_canvas view = nullptr;   // handle on the query context, made on demand
   Clock::time_point last_use;
 };
@@ -471,7 +471,7 @@ void pnt_server_s::setup(Req* q) {
   };
   const bool snap = q->r.use_snapshot != 0;
   if (snap) {
-    grow_f(S.snap, S.snap_cap, 7 * R.stride);
+    grow_f(S.snap, S.snap_cap, kLayerStrides * R.stride);
     const int64_t need = dirty_bytes_for((int64_t)R.rows * R.cols);
     if (need > S.dirty_cap) {
       dev_free(S.dirty);
@@ -531,7 +531,7 @@ bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
     S.snap = snap ? SR.snap : R.block;
     S.dirty = snap ? SR.dirty : nullptr;
     S.dirty_bytes = snap ? (int)dirty_bytes_for(n) : 0;
-    for (int k = 0; k < 3; ++k) S.out[k] = R.block + (11 + k) * R.stride;
+    for (int k = 0; k < 3; ++k) S.out[k] = R.block + (kCanvasStrides + k) * R.stride;
     S.rows = R.rows;
     S.cols = R.cols;
     S.kind = kind;
@@ -590,12 +590,12 @@ bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
     case kPrologue: {
       if (q->fresh) {
         SSlot& S = base(kSInit);
-        S.nblk = stream_blocks(11 * R.stride / 4);
+        S.nblk = stream_blocks(kCanvasStrides * R.stride / 4);
         push(S, -1);
         return true;
       }
       SSlot& S = base(kSSnap);
-      S.nblk = stream_blocks(7 * R.stride / 4);
+      S.nblk = stream_blocks(kLayerStrides * R.stride / 4);
       push(S, -1);
       return true;
     }
@@ -796,7 +796,7 @@ void pnt_server_s::copy_job(const CopyJob& j) {
     const Resident& R = *q->res;
     const int64_t n = (int64_t)R.rows * R.cols;
     for (int c = 0; c < 3; ++c)
-      PNT_HIP_CHECK(hipMemcpyAsync(q->r.out + c * n, R.block + (11 + c) * R.stride,
+      PNT_HIP_CHECK(hipMemcpyAsync(q->r.out + c * n, R.block + (kCanvasStrides + c) * R.stride,
                                    (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
     PNT_HIP_CHECK(hipStreamSynchronize(s));
     return;
@@ -806,7 +806,7 @@ void pnt_server_s::copy_job(const CopyJob& j) {
     std::lock_guard<std::mutex> lk(mu);
     R = &canvases.find(j.canvas_id)->second;
   }
-  const int64_t floats = 11 * R->stride;
+  const int64_t floats = kCanvasStrides * R->stride;
   if (j.kind == 1) {  // spill to host memory
     R->host.resize((size_t)floats);
     PNT_HIP_CHECK(hipMemcpyAsync(R->host.data(), R->block, (size_t)floats * sizeof(float),
@@ -979,7 +979,7 @@ int pnt_server_canvas(pnt_server S, int64_t canvas_id, pnt_canvas* canvas, pnt_d
     if (!R.view) R.view = canvas_over(S->qctx, R.rows, R.cols, R.block, false);
     *canvas = R.view;
     if (composite)
-      for (int k = 0; k < 3; ++k) composite->p[k] = R.block + (11 + k) * R.stride;
+      for (int k = 0; k < 3; ++k) composite->p[k] = R.block + (kCanvasStrides + k) * R.stride;
   });
 }
 

@@ -125,18 +125,40 @@ __device__ __forceinline__ bool group_sync(unsigned* bar, unsigned nblocks, unsi
 // PickAcc supplies pick(c) / set_pick(c, v) for the visit's map cell.
 template <class PickAcc>
 struct CachedVisit : PickAcc {
-  float sv[7], cv[7];
-  bool alias;      // pickup source is the canvas itself (snapshot off)
-  bool src_moved;  // the pickup changed the source volume
+  float sv[6], cv[6];  // K, S of the pickup source and of the canvas pixel
+  double svv, cvv;     // their volumes
+  bool alias;          // pickup source is the canvas itself (snapshot off)
+  bool src_moved;      // the pickup changed the source volume
   __device__ float src(int c) const { return sv[c]; }
-  __device__ void set_src_v(float v) {
-    sv[6] = v;
-    if (alias) cv[6] = v;
+  __device__ double src_v() const { return svv; }
+  __device__ void set_src_v(double v) {
+    svv = v;
+    if (alias) cvv = v;
     src_moved = true;
   }
   __device__ float canvas(int c) const { return cv[c]; }
+  __device__ double canvas_v() const { return cvv; }
   __device__ void set_canvas(int c, float v) { cv[c] = v; }
+  __device__ void set_canvas_v(double v) { cvv = v; }
+  // the pickup source is the canvas pixel itself
+  __device__ void src_from_canvas() {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sv[c] = cv[c];
+    svv = cvv;
+  }
 };
+
+// One pixel of a layer: K, S (FP32) and V (FP64).
+__device__ __forceinline__ void ld_px(const Layer7& L, int64_t i, float ks[6], double& v) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) ks[c] = L.ks[c][i];
+  v = L.v[i];
+}
+__device__ __forceinline__ void st_px(const Layer7& L, int64_t i, const float ks[6], double v) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) L.ks[c][i] = ks[c];
+  L.v[i] = v;
+}
 
 struct GlobalPick {  // pickup map cell in global memory
   float* const* p;
@@ -158,26 +180,19 @@ template <class PickAcc>
 __device__ __forceinline__ void cached_visit(const VisitTarget& T, const PickAcc& pick, float fh,
                                              int64_t ic, bool pull, const float pk[3],
                                              const float ps[3]) {
-  CachedVisit<PickAcc> acc{pick, {}, {}, !T.use_snap, false};
-#pragma unroll
-  for (int c = 0; c < 7; ++c) acc.cv[c] = T.canvas->p[c][ic];
-  if (T.use_snap && !pull) {
-#pragma unroll
-    for (int c = 0; c < 7; ++c) acc.sv[c] = T.src->p[c][ic];
-  } else {
-#pragma unroll
-    for (int c = 0; c < 7; ++c) acc.sv[c] = acc.cv[c];
-  }
+  CachedVisit<PickAcc> acc{pick, {}, {}, 0.0, 0.0, !T.use_snap, false};
+  ld_px(*T.canvas, ic, acc.cv, acc.cvv);
+  if (T.use_snap && !pull)
+    ld_px(*T.src, ic, acc.sv, acc.svv);
+  else
+    acc.src_from_canvas();
   paint_visit_math(acc, fh, T.pickup_rate, T.dep_rate, T.cap, pk, ps);
-#pragma unroll
-  for (int c = 0; c < 7; ++c) T.canvas->p[c][ic] = acc.cv[c];
+  st_px(*T.canvas, ic, acc.cv, acc.cvv);
   if (T.use_snap) {
-    if (pull) {
-#pragma unroll
-      for (int c = 0; c < 7; ++c) T.src->p[c][ic] = acc.sv[c];
-    } else if (acc.src_moved) {
-      T.src->p[6][ic] = acc.sv[6];
-    }
+    if (pull)
+      st_px(*T.src, ic, acc.sv, acc.svv);
+    else if (acc.src_moved)
+      T.src->v[ic] = acc.svv;
     T.dirty[ic] = 1;
   }
 }
@@ -287,26 +302,32 @@ __device__ __forceinline__ void ring_scan(const Layer7& canvas, const Layer7& sn
       mask[u] = m;
       base[u] = bs;
     }
-    float4 v[kG][7];
+    float4 v[kG][6];
+    double2 w[kG][2];  // the four volumes
 #pragma unroll
     for (int u = 0; u < kG; ++u) {
       if (!mask[u]) continue;
 #pragma unroll
-      for (int c = 0; c < 7; ++c) v[u][c] = reinterpret_cast<const float4*>(canvas.p[c])[base[u] >> 2];
+      for (int c = 0; c < 6; ++c) v[u][c] = reinterpret_cast<const float4*>(canvas.ks[c])[base[u] >> 2];
+      w[u][0] = reinterpret_cast<const double2*>(canvas.v)[base[u] >> 1];
+      w[u][1] = reinterpret_cast<const double2*>(canvas.v)[(base[u] >> 1) + 1];
     }
 #pragma unroll
     for (int u = 0; u < kG; ++u) {
       if (!mask[u]) continue;
       if (mask[u] == 0xfu) {
 #pragma unroll
-        for (int c = 0; c < 7; ++c) reinterpret_cast<float4*>(snap.p[c])[base[u] >> 2] = v[u][c];
+        for (int c = 0; c < 6; ++c) reinterpret_cast<float4*>(snap.ks[c])[base[u] >> 2] = v[u][c];
+        reinterpret_cast<double2*>(snap.v)[base[u] >> 1] = w[u][0];
+        reinterpret_cast<double2*>(snap.v)[(base[u] >> 1) + 1] = w[u][1];
         *reinterpret_cast<unsigned*>(dirty + base[u]) = 0u;
       } else {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           if (!((mask[u] >> q) & 1u)) continue;
 #pragma unroll
-          for (int c = 0; c < 7; ++c) snap.p[c][base[u] + q] = reinterpret_cast<const float*>(&v[u][c])[q];
+          for (int c = 0; c < 6; ++c) snap.ks[c][base[u] + q] = reinterpret_cast<const float*>(&v[u][c])[q];
+          snap.v[base[u] + q] = reinterpret_cast<const double*>(&w[u][0])[q];
           dirty[base[u] + q] = 0;
         }
       }
@@ -329,15 +350,11 @@ __device__ __forceinline__ int64_t canvas_index(const StampGeom& g, int col, int
 // Loads everything one visit reads: canvas and pickup-source pixel.
 __device__ __forceinline__ void load_pixel(const VisitTarget& T, int64_t ic, bool pull,
                                            CachedVisit<RegPick>& acc) {
-#pragma unroll
-  for (int c = 0; c < 7; ++c) acc.cv[c] = T.canvas->p[c][ic];
-  if (T.use_snap && !pull) {
-#pragma unroll
-    for (int c = 0; c < 7; ++c) acc.sv[c] = T.src->p[c][ic];
-  } else {
-#pragma unroll
-    for (int c = 0; c < 7; ++c) acc.sv[c] = acc.cv[c];
-  }
+  ld_px(*T.canvas, ic, acc.cv, acc.cvv);
+  if (T.use_snap && !pull)
+    ld_px(*T.src, ic, acc.sv, acc.svv);
+  else
+    acc.src_from_canvas();
   acc.src_moved = false;
 }
 
@@ -346,43 +363,34 @@ __device__ __forceinline__ void run_pixel(const VisitTarget& T, int64_t ic, bool
                                           const float pk[3], const float ps[3],
                                           CachedVisit<RegPick>& acc) {
   paint_visit_math(acc, fh, T.pickup_rate, T.dep_rate, T.cap, pk, ps);
-#pragma unroll
-  for (int c = 0; c < 7; ++c) T.canvas->p[c][ic] = acc.cv[c];
+  st_px(*T.canvas, ic, acc.cv, acc.cvv);
   if (T.use_snap) {
-    if (pull) {
-#pragma unroll
-      for (int c = 0; c < 7; ++c) T.src->p[c][ic] = acc.sv[c];
-    } else if (acc.src_moved) {
-      T.src->p[6][ic] = acc.sv[6];
-    }
+    if (pull)
+      st_px(*T.src, ic, acc.sv, acc.svv);
+    else if (acc.src_moved)
+      T.src->v[ic] = acc.svv;
     T.dirty[ic] = 1;
   }
 }
 
 // Canvas::checkDry (Canvas.hxx:123-155) of a visited pixel whose canvas
-// values are held in registers (cv[7]); R0, h and the time map are in
+// values are held in registers (ks[6], v); R0, h and the time map are in
 // memory.  The same arithmetic as check_dry (dry.hpp).
-__device__ __forceinline__ void check_dry_regs(const DryClock& c, float cv[7], int64_t ic) {
-  const float v = cv[6];
-  if (c.dry_ms > 0 && v >= 0.001f) {
+__device__ __forceinline__ void check_dry_regs(const DryClock& c, float ks[6], double& v,
+                                               int64_t ic) {
+  if (c.dry_ms > 0 && v > 0.001) {
     const int64_t elapsed_ms = (c.now_us - c.tmap[ic]) / 1000;  // duration_cast truncation
     if (elapsed_ms >= c.dry_ms) {
-      float K[3], S[3], R[3];
+      float K[3], S[3];
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
-        K[ch] = cv[ch];
-        S[ch] = cv[3 + ch];
-        R[ch] = c.R0.p[ch][ic];
+        K[ch] = ks[ch];
+        S[ch] = ks[3 + ch];
       }
-      c.h[ic] += v;
-      km_pixel(K, S, R, v);
+      dry_pixel_into(c, K, S, v, ic);
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        c.R0.p[ch][ic] = R[ch];
-        cv[ch] = 0.0f;
-        cv[3 + ch] = 0.0f;
-      }
-      cv[6] = 0.0f;
+      for (int ch = 0; ch < 6; ++ch) ks[ch] = 0.0f;
+      v = 0.0;
     }
   }
   c.tmap[ic] = c.now_us;
@@ -396,13 +404,15 @@ __device__ __forceinline__ void check_dry_regs(const DryClock& c, float cv[7], i
 __device__ __forceinline__ void check_only(const VisitTarget& T, const DryClock& c, int64_t ic,
                                            bool pv) {
   if (pv) {
-#pragma unroll
-    for (int ch = 0; ch < 7; ++ch) T.src->p[ch][ic] = T.canvas->p[ch][ic];
+    float ks[6];
+    double v;
+    ld_px(*T.canvas, ic, ks, v);
+    st_px(*T.src, ic, ks, v);
     T.dirty[ic] = 0;
   }
-  const float v0 = T.canvas->p[6][ic];
+  const double v0 = T.canvas->v[ic];
   check_dry(c, *T.canvas, ic);
-  if (T.use_snap && v0 >= 0.001f && T.canvas->p[6][ic] != v0) T.dirty[ic] = 1;
+  if (T.use_snap && v0 > 0.001 && T.canvas->v[ic] != v0) T.dirty[ic] = 1;
 }
 
 // The visits of one kT x kT tile of a stamp's window, by one workgroup of
@@ -451,7 +461,8 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
   bool pv = false, ppull = false;
   float fh = 0.0f;
   CachedVisit<RegPick> acc;
-  float pcv[kPre ? 7 : 1], psv[kPre ? 7 : 1];
+  float pcv[kPre ? 6 : 1], psv[kPre ? 6 : 1];
+  double pcvv = 0.0, psvv = 0.0;
   if (mm >= 0 && sm[ly + 1][lx] != mm && sm[ly][lx + 1] != mm) {
     const int col = col0 + lx, row = row0 + ly;
     pdir = sm[ly + 1][lx + 2] == mm ? 1 : (sm[ly + 2][lx + 1] == mm ? 2 : 0);
@@ -471,14 +482,13 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
         pic = canvas_index(g, col + (pdir == 1), row + (pdir == 2));
         ppull = pull && on_window_border(st.box, (int)(pic % g.cols), (int)(pic / g.cols));
         if constexpr (kPre) {
-#pragma unroll
-          for (int c = 0; c < 7; ++c) pcv[c] = T.canvas->p[c][pic];
+          ld_px(*T.canvas, pic, pcv, pcvv);
           if (T.use_snap && !ppull) {
-#pragma unroll
-            for (int c = 0; c < 7; ++c) psv[c] = T.src->p[c][pic];
+            ld_px(*T.src, pic, psv, psvv);
           } else {
 #pragma unroll
-            for (int c = 0; c < 7; ++c) psv[c] = pcv[c];
+            for (int c = 0; c < 6; ++c) psv[c] = pcv[c];
+            psvv = pcvv;
           }
         }
       }
@@ -491,29 +501,27 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
   act = act && fh > 0.0f;
   if (act) {
     if (clk) {
-      check_dry_regs(*clk, acc.cv, ic);
-      if (!T.use_snap)  // the pickup source is the canvas itself
-#pragma unroll
-        for (int c = 0; c < 7; ++c) acc.sv[c] = acc.cv[c];
+      check_dry_regs(*clk, acc.cv, acc.cvv, ic);
+      if (!T.use_snap) acc.src_from_canvas();  // the pickup source is the canvas itself
     }
     run_pixel(T, ic, pv, fh, pk, ps, acc);
     if (pdir) {
       acc.alias = !T.use_snap;
       if constexpr (kPre) {
 #pragma unroll
-        for (int c = 0; c < 7; ++c) {
+        for (int c = 0; c < 6; ++c) {
           acc.cv[c] = pcv[c];
           acc.sv[c] = psv[c];
         }
+        acc.cvv = pcvv;
+        acc.svv = psvv;
         acc.src_moved = false;
       } else {
         load_pixel(T, pic, ppull, acc);  // the pickup cell stays in acc.v
       }
       if (clk) {
-        check_dry_regs(*clk, acc.cv, pic);
-        if (!T.use_snap)
-#pragma unroll
-          for (int c = 0; c < 7; ++c) acc.sv[c] = acc.cv[c];
+        check_dry_regs(*clk, acc.cv, acc.cvv, pic);
+        if (!T.use_snap) acc.src_from_canvas();
       }
       run_pixel(T, pic, ppull, fh, pk, ps, acc);
     }

@@ -205,16 +205,16 @@ __global__ __launch_bounds__(kBlock) void k_tex_deposit(TexArgs A) {
   const double vt = A.tmap[(int64_t)t * A.tcols + s];
   if (!(vt > 0.0)) return;  // not textured: not in the reference's pixel list
   const int64_t ic = (int64_t)y * A.cols + x;
-  const double vc = A.canvas.p[6][ic];
+  const double vc = A.canvas.v[ic];
   const double vs = vc + vt;
   if (vs > 0.0) {
     const double sc = 1.0 / vs;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      A.canvas.p[c][ic] = (float)((vc * A.canvas.p[c][ic] + vt * A.paint[c]) * sc);
-      A.canvas.p[3 + c][ic] = (float)((vc * A.canvas.p[3 + c][ic] + vt * A.paint[3 + c]) * sc);
+      A.canvas.ks[c][ic] = (float)((vc * A.canvas.ks[c][ic] + vt * A.paint[c]) * sc);
+      A.canvas.ks[3 + c][ic] = (float)((vc * A.canvas.ks[3 + c][ic] + vt * A.paint[3 + c]) * sc);
     }
-    A.canvas.p[6][ic] = (float)fmax(vt, vc);
+    A.canvas.v[ic] = fmax(vt, vc);
   }
 }
 
@@ -286,24 +286,24 @@ __global__ __launch_bounds__(kStepBlock) void k_smudge_step(SMap src, SMap dst, 
     constexpr double pickupRate = 0.1, depositionRate = 0.1, MinVolume = 0.001;
     const double md = T.maxD;
     const int64_t ic = (int64_t)cpy * T.cols + cpx;
-    const double cV = T.canvas.p[6][ic];
+    const double cV = T.canvas.v[ic];
     const double cVl = cV * depositionRate * Dv / md, cVr = cV - cVl;
     const double pVl = pV * pickupRate * Dv / md, pVr = pV - pVl;
     double cK[3], cS[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      cK[c] = T.canvas.p[c][ic];
-      cS[c] = T.canvas.p[3 + c][ic];
+      cK[c] = T.canvas.ks[c][ic];
+      cS[c] = T.canvas.ks[3 + c][ic];
     }
     const double cVnew = cVr + pVl;
     if (cVnew > MinVolume) {
       const double inv = 1. / cVnew;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        T.canvas.p[c][ic] = (float)(inv * (cVr * cK[c] + pVl * pK[c]));
-        T.canvas.p[3 + c][ic] = (float)(inv * (cVr * cS[c] + pVl * pS[c]));
+        T.canvas.ks[c][ic] = (float)(inv * (cVr * cK[c] + pVl * pK[c]));
+        T.canvas.ks[3 + c][ic] = (float)(inv * (cVr * cS[c] + pVl * pS[c]));
       }
-      T.canvas.p[6][ic] = (float)fmax(cVnew, 0.0);
+      T.canvas.v[ic] = fmax(cVnew, 0.0);
     }
     const double pVnew = pVr + cVl;
     if (pVnew > MinVolume) {
@@ -925,10 +925,10 @@ __global__ __launch_bounds__(kBlock) void k_gl_smudge(GlSmudgeArgs A, GlStep St,
   float cK[3], cS[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    cK[c] = A.canvas.p[c][ic];
-    cS[c] = A.canvas.p[3 + c][ic];
+    cK[c] = A.canvas.ks[c][ic];
+    cS[c] = A.canvas.ks[3 + c][ic];
   }
-  const float cV = A.canvas.p[6][ic];
+  const float cV = (float)A.canvas.v[ic];
   const float cVl = cV * 0.7f * vTex, cVr = cV - cVl;
   const float pVl = pV * 0.7f * vTex, pVr = pV - pVl;
   const float pVnew = pVr + cVl;
@@ -946,10 +946,10 @@ __global__ __launch_bounds__(kBlock) void k_gl_smudge(GlSmudgeArgs A, GlStep St,
   if (cVnew > 0.0f) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      A.canvas.p[c][ic] = (cVr * pK[c] + pVl * cK[c]) / cVnew;
-      A.canvas.p[3 + c][ic] = (cVr * pS[c] + pVl * cS[c]) / cVnew;
+      A.canvas.ks[c][ic] = (cVr * pK[c] + pVl * cK[c]) / cVnew;
+      A.canvas.ks[3 + c][ic] = (cVr * pS[c] + pVl * cS[c]) / cVnew;
     }
-    A.canvas.p[6][ic] = fmaxf(cVnew, 0.0f);
+    A.canvas.v[ic] = fmaxf(cVnew, 0.0f);
     A.light[ic] = vTex;
   }
 }
@@ -981,16 +981,16 @@ __global__ __launch_bounds__(kBlock) void k_gl_imprint(Layer7 canvas, float* lig
   if (px < 0 || pyg < 0 || px >= cols || pyg >= rows) return;
   const int64_t iw = (int64_t)pyg * cols + px;
   const int64_t ic = (int64_t)(rows - 1 - pyg) * cols + px;
-  const float vCan = canvas.p[6][ic], vTex = scale * warp[iw];
+  const float vCan = (float)canvas.v[ic], vTex = scale * warp[iw];
   const float vSum = vCan + vTex;
   if (!(vSum < 10e-6f)) {
     const float kb[3] = {Kb.x, Kb.y, Kb.z}, sb[3] = {Sb.x, Sb.y, Sb.z};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      canvas.p[c][ic] = (vCan * canvas.p[c][ic] + vTex * kb[c]) / vSum;
-      canvas.p[3 + c][ic] = (vCan * canvas.p[3 + c][ic] + vTex * sb[c]) / vSum;
+      canvas.ks[c][ic] = (vCan * canvas.ks[c][ic] + vTex * kb[c]) / vSum;
+      canvas.ks[3 + c][ic] = (vCan * canvas.ks[3 + c][ic] + vTex * sb[c]) / vSum;
     }
-    canvas.p[6][ic] = vSum;
+    canvas.v[ic] = vSum;
     light[ic] = (vTex > 0.0f) ? vTex : vCan;
   }
   warp[iw] = 0.0f;

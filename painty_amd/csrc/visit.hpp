@@ -17,7 +17,8 @@ constexpr double kMinVolume = 0.001;
 // The arithmetic runs in FP64, as the reference's does (vec3 is Eigen's
 // double vector): loads widen the stored FP32 values, every product, sum,
 // quotient and threshold test is the reference's own in double precision,
-// and each result is rounded to FP32 once, when it is stored.  Over the
+// and each K/S result is rounded to FP32 once, when it is stored (canvas
+// and snapshot volumes are stored in FP64).  Over the
 // hundreds of dependent visits a pixel and a pickup cell take in a long
 // stamp list, only that storage rounding accumulates (an FP32 chain would
 // add the rounding of every intermediate step as well).
@@ -38,19 +39,25 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 
 // Accessor interface:
 //   float pick(int c) / void set_pick(int c, float)   c = 0..2 K, 3..5 S, 6 V
-//   float src(int c)  / void set_src_v(float)          pickup source (snapshot or canvas)
-//   float canvas(int c) / void set_canvas(int c, float)
+//   float src(int c) (c < 6) / double src_v() / void set_src_v(double)
+//                                                      pickup source (snapshot or canvas)
+//   float canvas(int c) (c < 6) / double canvas_v()
+//   void set_canvas(int c, float) (c < 6) / void set_canvas_v(double)
+// Canvas and source volumes are FP64 in memory, so every volume that a
+// MinVolume test sees is the reference's own number; contraction is off so
+// that each product and sum rounds as the reference's does.
 template <class Acc>
 __device__ __forceinline__ void paint_visit_math(Acc& a, float fh_f, double pickup_rate,
                                                  double dep_rate, double cap, const float pK[3],
                                                  const float pS[3]) {
+#pragma clang fp contract(off)
   const double fh = fh_f;
   {  // pickupPaint (FootprintBrush.hxx:349-384)
     const double vP = a.pick(6);
-    const double vC = a.src(6);
+    const double vC = a.src_v();
     const double vLeave = (pickup_rate * vC) * fh;
     if (vLeave > kMinVolume) {
-      a.set_src_v((float)(vC - vLeave));
+      a.set_src_v(vC - vLeave);
       const double vt = vP + vLeave;
       if (vt > kMinVolume) {
         const double iv = 1.0 / vt;
@@ -79,7 +86,7 @@ __device__ __forceinline__ void paint_visit_math(Acc& a, float fh_f, double pick
       }
     }
     a.set_pick(6, (float)(vP - (dep_rate * vP) * fh));
-    const double vC = a.canvas(6);
+    const double vC = a.canvas_v();
     const double vB = cap * fh;
     const double vt2 = vB + vC;
     if (vt2 > kMinVolume) {
@@ -93,7 +100,7 @@ __device__ __forceinline__ void paint_visit_math(Acc& a, float fh_f, double pick
 #pragma unroll
       for (int c = 0; c < 6; ++c) a.set_canvas(c, (float)src[c]);
     }
-    a.set_canvas(6, (float)(vB + vC));
+    a.set_canvas_v(vB + vC);
   }
 }
 

@@ -38,11 +38,12 @@ enum : int {
   kSCopyPick,  // pickup map -> pick_dst
 };
 
-// Canvas planes: K0..2 S0..2 V R0_0..2 h, cstride floats each.
+// A canvas block: K0..2 S0..2 (FP32), V (FP64, two strides), R0_0..2, h;
+// strides of cstride floats (kCanvasStrides of them).
 struct SSlot {
   const PStamp* st;
   float* canvas;
-  float* snap;      // 7 planes of cstride (the canvas itself when the snapshot is off)
+  float* snap;      // a layer of cstride (the canvas itself when the snapshot is off)
   uint8_t* dirty;   // snapshot dirty bytes (dirty_bytes of them)
   float* pick;      // the chain's pickup map: 7 planes of mstride
   float* pick_dst;  // kSCopyPick target
@@ -75,23 +76,21 @@ __device__ __forceinline__ void wave_dry_pixel(const Layer7& L, float* R0, float
   bool nz = false;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    K[c] = L.p[c][i];
-    S[c] = L.p[3 + c][i];
+    K[c] = L.ks[c][i];
+    S[c] = L.ks[3 + c][i];
     R[c] = R0[c * cs + i];
     nz |= (K[c] != 0.0f) | (S[c] != 0.0f);
   }
-  const float v = L.p[6][i];
-  nz |= v != 0.0f;
+  const double v = L.v[i];
+  nz |= v != 0.0;
   if (!nz) return;  // drying an empty pixel changes nothing
-  h[i] += v;
-  km_pixel(K, S, R, v);
+  h[i] = h_plus(h[i], v);
+  km_pixel(K, S, R, (float)v);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    R0[c * cs + i] = R[c];
-    L.p[c][i] = 0.0f;
-    L.p[3 + c][i] = 0.0f;
-  }
-  L.p[6][i] = 0.0f;
+  for (int c = 0; c < 3; ++c) R0[c * cs + i] = R[c];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) L.ks[c][i] = 0.0f;
+  L.v[i] = 0.0;
   if (dirty) dirty[i] = 1;
 }
 
@@ -101,11 +100,11 @@ __device__ __forceinline__ void wave_compose_pixel(const Layer7& L, const float*
   float K[3], S[3], R[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    K[c] = L.p[c][i];
-    S[c] = L.p[3 + c][i];
+    K[c] = L.ks[c][i];
+    S[c] = L.ks[3 + c][i];
     R[c] = R0[c * cs + i];
   }
-  km_pixel(K, S, R, L.p[6][i]);
+  km_pixel(K, S, R, (float)L.v[i]);
 #pragma unroll
   for (int c = 0; c < 3; ++c) out[c][i] = R[c];
 }
@@ -114,27 +113,25 @@ __device__ __forceinline__ void wave_compose_pixel(const Layer7& L, const float*
 __device__ __forceinline__ void run_wave_slot(const SSlot& S, int b, int (*sm)[kWT + 2]) {
   const int tid = b * kWT2 + (int)threadIdx.x, nth = S.nblk * kWT2;
   const int64_t cs = S.cstride;
-  Layer7 canvas, snap;
-#pragma unroll
-  for (int c = 0; c < 7; ++c) {
-    canvas.p[c] = S.canvas + c * cs;
-    snap.p[c] = S.snap + c * cs;
-  }
-  float* R0 = S.canvas + 7 * cs;
+  const Layer7 canvas = layer_at(S.canvas, cs);
+  const Layer7 snap = layer_at(S.snap, cs);
+  float* R0 = S.canvas + 8 * cs;
+  float* hp = S.canvas + 11 * cs;
   const int64_t n = (int64_t)S.rows * S.cols;
   switch (S.kind) {
     case kSInit: {
-      for (int p = 0; p < 11; ++p)
-        wave_fill4(S.canvas + p * cs, cs / 4, (p >= 7 && p < 10) ? S.bg[p - 7] : 0.0f, tid, nth);
+      // zero bits are 0.0f and 0.0: the wet layer (8 strides) and h
+      for (int p = 0; p < kCanvasStrides; ++p)
+        wave_fill4(S.canvas + p * cs, cs / 4, (p >= 8 && p < 11) ? S.bg[p - 8] : 0.0f, tid, nth);
       if (S.use_snap) {  // a fresh brush's snapshot of the new canvas
-        for (int p = 0; p < 7; ++p) wave_fill4(S.snap + p * cs, cs / 4, 0.0f, tid, nth);
+        for (int p = 0; p < kLayerStrides; ++p) wave_fill4(S.snap + p * cs, cs / 4, 0.0f, tid, nth);
         for (int q = tid; q < S.dirty_bytes / 16; q += nth)
           reinterpret_cast<uint4*>(S.dirty)[q] = make_uint4(0u, 0u, 0u, 0u);
       }
       return;
     }
     case kSSnap: {
-      for (int p = 0; p < 7; ++p)
+      for (int p = 0; p < kLayerStrides; ++p)  // raw copy: K, S and the FP64 V
         for (int64_t q = tid; q < cs / 4; q += nth)
           reinterpret_cast<float4*>(S.snap + p * cs)[q] =
               reinterpret_cast<const float4*>(S.canvas + p * cs)[q];
@@ -152,7 +149,7 @@ __device__ __forceinline__ void run_wave_slot(const SSlot& S, int b, int (*sm)[k
     case kSDry: {  // with a clock, dryCanvas also stamps every pixel's time (its last imprint's)
       const int64_t now = S.tmap ? S.st->now_us : 0;
       for (int64_t i = tid; i < n; i += nth) {
-        wave_dry_pixel(canvas, R0, S.canvas + 10 * cs, cs, S.use_snap ? S.dirty : nullptr, i);
+        wave_dry_pixel(canvas, R0, hp, cs, S.use_snap ? S.dirty : nullptr, i);
         if (S.tmap) S.tmap[i] = now;
       }
       return;
@@ -181,9 +178,7 @@ __device__ __forceinline__ void run_wave_slot(const SSlot& S, int b, int (*sm)[k
   for (int c = 0; c < 7; ++c) pick[c] = S.pick + c * (int64_t)S.mstride;
   const FpBox fb{S.fb[0], S.fb[1], S.fb[2], S.fb[3]};
   // Canvas::checkDry on every visited pixel when the canvas has a clock
-  const DryClock ck{S.tmap, st.now_us, S.dry_ms,
-                    Rgb3{{S.canvas + 7 * cs, S.canvas + 8 * cs, S.canvas + 9 * cs}},
-                    S.canvas + 10 * cs};
+  const DryClock ck{S.tmap, st.now_us, S.dry_ms, Rgb3{{R0, R0 + cs, R0 + 2 * cs}}, hp};
   const DryClock* clk = S.tmap ? &ck : nullptr;
   const int win = (S.kind == kSStamp || S.kind == kSWindow) ? S.ntiles : 0;
   if (S.kind == kSLevels) {  // one block

@@ -124,14 +124,25 @@ class PaintLayer(Owned):
         return self._download("S")
 
     def getV_buffer(self) -> np.ndarray:
-        return self._download("V")
+        """V as the device holds it: FP64 (rows, cols)."""
+        rows, cols = self.shape
+        out = np.empty((rows, cols), np.float64)
+        call("pnt_layer_download_volume", self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
 
     def setBuffers(self, K=None, S=None, V=None) -> None:
+        """K, S go to FP32 planes; V to the FP64 plane (a float64 V exactly,
+        a float32 V widened)."""
         rows, cols = self.shape
         K = None if K is None else _f32(K, (rows, cols, 3))
         S = None if S is None else _f32(S, (rows, cols, 3))
-        V = None if V is None else _f32(V, (rows, cols))
-        call("pnt_layer_upload", self.h, _pf(K), _pf(S), _pf(V), AOS)
+        v64 = V is not None and np.asarray(V).dtype == np.float64
+        V32 = None if V is None or v64 else _f32(V, (rows, cols))
+        call("pnt_layer_upload", self.h, _pf(K), _pf(S), _pf(V32), AOS)
+        if v64:
+            Vd = np.ascontiguousarray(V, dtype=np.float64)
+            assert Vd.shape == (rows, cols)
+            call("pnt_layer_upload_volume", self.h, Vd.ctypes.data_as(C.POINTER(C.c_double)))
 
     def composeOnto(self, R0: np.ndarray) -> np.ndarray:
         """In place on R0 (float32 (rows, cols, 3)); returns it.  A
@@ -249,10 +260,11 @@ class Canvas(Owned):
         call("pnt_canvas_upload_time_map", self.h, t.ctypes.data_as(C.POINTER(C.c_int64)))
 
     def device_planes(self):
-        """(layer planes[7], R0 planes[3], h) as device addresses."""
-        L, R, h = _lib.DevLayer(), _lib.DevRgb(), C.c_void_p()
+        """(wet layer as a DevWet -- K/S FP32 planes, V FP64 --, R0 planes[3],
+        h) as device addresses."""
+        L, R, h = _lib.DevWet(), _lib.DevRgb(), C.c_void_p()
         call("pnt_canvas_device_planes", self.h, C.byref(L), C.byref(R), C.byref(h))
-        return list(L.p), list(R.p), h.value
+        return L, list(R.p), h.value
 
 
 class Renderer:

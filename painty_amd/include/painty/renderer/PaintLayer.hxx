@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <utility>
+#include <vector>
 
 #include "painty/core/Vec.hxx"
 #include "painty/gpu/Device.hxx"
@@ -94,8 +95,10 @@ class PaintLayer final {
     const std::size_t n = static_cast<std::size_t>(_rows) * _cols;
     const auto k = gpu::to_f32(_K.data()->data(), 3 * n);
     const auto s = gpu::to_f32(_S.data()->data(), 3 * n);
-    const auto v = gpu::to_f32(_V.data(), n);
-    gpu::check(pnt_layer_upload(_h, k.data(), s.data(), v.data(), PNT_AOS));
+    gpu::check(pnt_layer_upload(_h, k.data(), s.data(), nullptr, PNT_AOS));
+    // V travels in FP64, as the device holds it
+    std::vector<double> v(_V.data(), _V.data() + n);
+    gpu::check(pnt_layer_upload_volume(_h, v.data()));
     _m = gpu::Mirror::Clean;
   }
   // The device data changed behind the mirrors.
@@ -113,14 +116,16 @@ class PaintLayer final {
   void pull() const {
     if (_m != gpu::Mirror::Stale) return;
     const std::size_t n = static_cast<std::size_t>(_rows) * _cols;
-    std::vector<float> k(3 * n), s(3 * n), v(n);
-    gpu::check(pnt_layer_download(_h, k.data(), s.data(), v.data(), PNT_AOS));
+    std::vector<float> k(3 * n), s(3 * n);
+    std::vector<double> v(n);
+    gpu::check(pnt_layer_download(_h, k.data(), s.data(), nullptr, PNT_AOS));
+    gpu::check(pnt_layer_download_volume(_h, v.data()));
     _K = Mat<vector_type>(_rows, _cols);
     _S = Mat<vector_type>(_rows, _cols);
     _V = Mat<T>(_rows, _cols);
     gpu::from_f32(k, _K.data()->data());
     gpu::from_f32(s, _S.data()->data());
-    gpu::from_f32(v, _V.data());
+    for (std::size_t i = 0; i < n; ++i) _V.data()[i] = static_cast<T>(v[i]);
     _m = gpu::Mirror::Clean;
   }
   void touch() {

@@ -70,11 +70,12 @@ class Server(_lib.Owned):
         n = rows.value * cols.value
         K = np.empty((rows.value, cols.value, 3), np.float32)
         S = np.empty_like(K)
-        V = np.empty((rows.value, cols.value), np.float32)
+        V = np.empty((rows.value, cols.value), np.float64)  # FP64, as the device holds it
         R0 = np.empty_like(K)
         lay = C.c_void_p()
         _lib.call("pnt_canvas_layer", cvh, C.byref(lay))
-        _lib.call("pnt_layer_download", lay, H._pf(K), H._pf(S), H._pf(V), _lib.AOS)
+        _lib.call("pnt_layer_download", lay, H._pf(K), H._pf(S), None, _lib.AOS)
+        _lib.call("pnt_layer_download_volume", lay, V.ctypes.data_as(C.POINTER(C.c_double)))
         _lib.call("pnt_canvas_download_R0", cvh, H._pf(R0), _lib.AOS)
         R = np.empty((3, n), np.float32)
         hip = C.CDLL("libamdhip64.so")

@@ -96,8 +96,9 @@ def test_compose_onto(hip, oracle):
 
 
 def test_two_layer_compose_device_planes(hip, oracle):
-    """pnt_dev_compose2: wet over (dry over R0) in one pass, R0 untouched; also
-    the unaligned (scalar) path through planes offset by one float."""
+    """pnt_dev_compose2_wet: wet over (dry over R0) in one pass on canvas
+    layers (K/S FP32, V FP64), R0 untouched; also the unaligned (scalar) path
+    through planes offset by one element."""
     import ctypes as C
     from painty_amd import _lib
     rows, cols = 300, 301
@@ -112,8 +113,7 @@ def test_two_layer_compose_device_planes(hip, oracle):
     _, Ro, _ = out.device_planes()
     n = rows * cols
     ctx = hip.default_context()
-    _lib.call("pnt_dev_compose2", ctx.h, n, _lib.DevLayer((C.c_void_p * 7)(*La)),
-              _lib.DevLayer((C.c_void_p * 7)(*Lb)), _lib.DevRgb((C.c_void_p * 3)(*Ra)),
+    _lib.call("pnt_dev_compose2_wet", ctx.h, n, La, Lb, _lib.DevRgb((C.c_void_p * 3)(*Ra)),
               _lib.DevRgb((C.c_void_p * 3)(*Ro)), None)
     ref = R0.astype(np.float64)
     oracle.compose_onto(dry.K, dry.S, dry.V, ref)
@@ -122,12 +122,48 @@ def test_two_layer_compose_device_planes(hip, oracle):
     np.testing.assert_array_equal(a.getR0(), R0)
     # unaligned planes: skip the first pixel of every plane
     sh = lambda ps: [p + 4 for p in ps]  # noqa: E731
-    _lib.call("pnt_dev_compose", ctx.h, n - 1, _lib.DevLayer((C.c_void_p * 7)(*sh(Lb))),
+    Lu = _lib.DevWet((C.c_void_p * 6)(*sh(list(Lb.ks))), Lb.v + 8)
+    _lib.call("pnt_dev_compose_wet", ctx.h, n - 1, Lu,
               _lib.DevRgb((C.c_void_p * 3)(*sh(Ra))), _lib.DevRgb((C.c_void_p * 3)(*sh(Ro))),
               None)
     got = out.getR0().reshape(-1, 3)[1:]
     ref1 = oracle.compose(wet.K, wet.S, wet.V, R0).reshape(-1, 3)[1:]
     assert_close("compose(dev,unaligned)", got, ref1)
+
+
+def test_compose_caller_fp32_planes(hip, oracle):
+    """pnt_dev_compose / pnt_dev_compose2 on caller-owned FP32 planes (torch
+    tensors): seven planes per layer, V in FP32; aligned and unaligned."""
+    import ctypes as C
+
+    import torch
+    from painty_amd import _lib
+    rows, cols = 97, 203
+    dry, wet = W.paint_layer(rows, cols, 5, 100), W.paint_layer(rows, cols, 5, 0)
+    R0 = W.substrate(rows, cols, 5)
+    dev = "cuda:0"
+    planes = lambda L: torch.from_numpy(np.concatenate(  # noqa: E731
+        [L.K.transpose(2, 0, 1), L.S.transpose(2, 0, 1), L.V[None]]).astype(np.float32)
+        .reshape(7, -1)).contiguous().to(dev)
+    Pd, Pw = planes(dry), planes(wet)
+    r0 = torch.from_numpy(R0.transpose(2, 0, 1).reshape(3, -1).copy()).to(dev)
+    out = torch.zeros_like(r0)
+    n = rows * cols
+    lay = lambda P, o=0: _lib.DevLayer((C.c_void_p * 7)(*[P[c].data_ptr() + 4 * o for c in range(7)]))  # noqa: E731
+    rgb = lambda T, o=0: _lib.DevRgb((C.c_void_p * 3)(*[T[c].data_ptr() + 4 * o for c in range(3)]))  # noqa: E731
+    ctx = hip.default_context()
+    torch.cuda.synchronize()  # torch's copies and fill are on its own stream
+    _lib.call("pnt_dev_compose2", ctx.h, n, lay(Pd), lay(Pw), rgb(r0), rgb(out), None)
+    ctx.synchronize()
+    ref = R0.astype(np.float64)
+    oracle.compose_onto(dry.K, dry.S, dry.V, ref)
+    ref2 = oracle.compose(wet.K, wet.S, wet.V, ref)
+    assert_close("compose2(fp32 planes)", out.cpu().numpy().reshape(3, rows, cols).transpose(1, 2, 0), ref2)
+    _lib.call("pnt_dev_compose", ctx.h, n - 1, lay(Pw, 1), rgb(r0, 1), rgb(out, 1), None)
+    ctx.synchronize()
+    got = out.cpu().numpy().reshape(3, -1).T[1:]
+    ref1 = oracle.compose(wet.K, wet.S, wet.V, R0).reshape(-1, 3)[1:]
+    assert_close("compose(fp32 planes,unaligned)", got, ref1)
 
 
 def test_dry_canvas(hip, oracle):

@@ -114,7 +114,7 @@ def test_eviction_spills_and_restores(hip):
     to host memory; a later request on it restores it and continues from its
     state: every canvas equals its requests run alone."""
     core = W.gaussian_core(9, 2.25)
-    one = 56 * 64 * 64  # canvas + composite planes of a 64x64 canvas
+    one = 60 * 64 * 64  # canvas + composite planes of a 64x64 canvas (60 B/px)
     reqs = []
     for k, cid in enumerate((1, 2, 3, 1, 3, 2, 1)):
         st = W.strokes(64, 64, 12, 50 + k, stamps_per_stroke=(4, 8), reach=0.3)
@@ -159,7 +159,7 @@ def test_release_ends_a_canvas(hip):
     while it is spilled to the host); a later request with the same id
     starts a new canvas, equal to that request run alone on a fresh one."""
     core = W.gaussian_core(9, 2.25)
-    one = 56 * 64 * 64
+    one = 60 * 64 * 64
     st = [W.strokes(64, 64, 10, 70 + k, stamps_per_stroke=(4, 8), reach=0.3) for k in range(4)]
     bg = (0.8, 0.85, 0.9)
     srv = SV.Server(n_slots=2, hbm_budget_bytes=2 * one + 1024)
