@@ -840,7 +840,7 @@ def parse():
     ap.add_argument("--rates", default="200,500,1000,2000")
     ap.add_argument("--rates2048", default="500,1000,2000")
     ap.add_argument("--seconds", type=float, default=3.0)
-    ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--hbm-gb", type=float, default=96.0)
     ap.add_argument("--plain-requests", type=int, default=40)
     ap.add_argument("--canvases", type=int, default=256)

@@ -51,13 +51,11 @@ struct FArgs {
   DryClock clk;        // the canvas' drying clock; clk.tmap == nullptr: none
 };
 
-// The clock of stamp `st` (its own time point), or nullptr without one.
-__device__ __forceinline__ const DryClock* stamp_clock(const DryClock& base, const PStamp& st,
-                                                       DryClock& out) {
-  if (!base.tmap) return nullptr;
-  out = base;
-  out.now_us = st.now_us;
-  return &out;
+// The clock of stamp `st` (its own time point); tmap == nullptr without one.
+__device__ __forceinline__ DryClock stamp_clock(const DryClock& base, const PStamp& st) {
+  DryClock c = base;
+  c.now_us = st.now_us;
+  return c;
 }
 
 __device__ __forceinline__ StampGeom geom_of(const FArgs& A, const PStamp& st) {
@@ -83,8 +81,7 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_fused(FArgs A, int j, int win_
   const StampGeom g = geom_of(A, st);
   const bool dup = (st.flags & (kStDupX | kStDupY)) != 0;
   const bool pull = A.use_snap && !dup;
-  DryClock ck;
-  const DryClock* clk = stamp_clock(A.clk, st, ck);
+  const DryClock clk = stamp_clock(A.clk, st);
   if ((int)blockIdx.x < win_blocks) {
     const VisitTarget T{&A.canvas, &A.snap, A.dirty, A.use_snap, A.pickup_rate, A.dep_rate,
                         A.cap};
@@ -93,7 +90,7 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_fused(FArgs A, int j, int win_
   } else {
     ring_scan<2>(A.canvas, A.snap, A.dirty, A.fp, g, st.box, pull,
                  ((int)blockIdx.x - win_blocks) * kT * kT + (int)threadIdx.x,
-                 ring_blocks * kT * kT, clk != nullptr);
+                 ring_blocks * kT * kT, clk.tmap != nullptr);
   }
 }
 
@@ -103,9 +100,8 @@ __global__ __launch_bounds__(256) void k_stamp_levels(FArgs A, int j) {
   const PStamp& st = A.st[j];
   const StampGeom g = geom_of(A, st);
   const VisitTarget T{&A.canvas, &A.snap, A.dirty, A.use_snap, A.pickup_rate, A.dep_rate, A.cap};
-  DryClock ck;
   level_visits<256>(T, A.pick.p, A.fp, g, st, A.dl, A.dl_off, A.pk, A.ps,
-                    stamp_clock(A.clk, st, ck));
+                    stamp_clock(A.clk, st));
 }
 
 }  // namespace
@@ -341,8 +337,7 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_wave(WArgs A) {
   float* pick[7];
 #pragma unroll
   for (int c = 0; c < 7; ++c) pick[c] = S.pick + c * S.mstride;
-  DryClock ck;
-  const DryClock* clk = stamp_clock(A.clk, st, ck);
+  const DryClock clk = stamp_clock(A.clk, st);
   const int win = (S.kind == kPhStamp || S.kind == kPhWindow) ? S.ntiles : 0;
   if (S.kind == kPhLevels) {  // one block (nblk == 1)
     level_visits<kT * kT>(T, pick, S.fp, g, st, A.dl, A.dl_off, S.pk, S.ps, clk);
@@ -351,7 +346,7 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_wave(WArgs A) {
                     sm, S.pk, S.ps, clk);
   } else if (b < win + S.rb) {
     ring_scan<2>(A.canvas, A.snap, A.dirty, S.fp, g, st.box, pull, (b - win) * kT * kT +
-                 (int)threadIdx.x, S.rb * kT * kT, clk != nullptr);
+                 (int)threadIdx.x, S.rb * kT * kT, clk.tmap != nullptr);
   }
 }
 

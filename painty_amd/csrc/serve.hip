@@ -16,7 +16,7 @@
 // plan_units in fused.hip).  One engine thread admits planned requests into
 // up to n_slots request slots and drives the device with "waves": each wave
 // is ONE launch of k_serve_wave on the engine's stream, carrying the next
-// phase of every ready unit of every admitted request (up to kSlots phases:
+// phase of every ready unit of every admitted request (up to kPhases phases:
 // a canvas initialisation, a full snapshot copy, a pickup-map clear, a stamp
 // -- window tiles and ring scan side by side -- or the split parts of a
 // stamp that straddles canvas column/row 0, a dryCanvas, or the composite),
@@ -65,27 +65,109 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-constexpr int kSlots = 16;         // phases per wave
+constexpr int kPhases = 160;       // phases per wave
 constexpr int kChainsPerReq = 4;   // pickup maps of one request (concurrent units)
 constexpr int kAhead = 3;          // waves queued ahead of the engine
 constexpr int kT = kWT;            // window tile side
 constexpr int kT2 = kWT2;          // threads per block
 
-struct SArgs {
-  int nslots;
+// What every phase of one admitted request shares: written to the device
+// once, when the request is set up in its slot.
+struct SReq {
+  const PStamp* st;
+  float* canvas;
+  float* snap;      // the canvas itself when the snapshot is off
+  uint8_t* dirty;
+  float* pick;      // chain c's map at pick + c * 7 * mstride
+  const float* fp;
+  const int2* dl;
+  const int* dl_off;
+  const float* paint;  // 6 floats per stamp: the brush paint at that stamp
+  float* out[3];
+  int cstride, mstride, dirty_bytes, rows, cols, hr, size_map, tiles_x, ntiles, use_snap;
+  int fb[4];
+  float bg[3];
   int pad;
-  SSlot slot[kSlots];
+  double pickup_rate, dep_rate, cap;
+};
+
+// One phase of a wave (a kernel argument): its request slot, its kind, the
+// stamp it runs and the blocks it takes.
+struct SPhase {
+  int16_t slot;
+  int8_t kind;
+  int8_t chain;
+  int k;      // stamp index (stamp kinds)
+  int start;  // first block
+  int nblk;
+  int rb;     // ring-scan blocks (stamp and ring kinds)
+};
+
+struct SArgs {
+  const SReq* reqs;
+  int nphases;
+  int pad;
+  SPhase ph[kPhases];
 };
 static_assert(sizeof(SArgs) <= 4000, "wave arguments must fit the kernel argument segment");
 
-// One wave of the server: its phases arrive as kernel arguments.
+// One wave of the server: its phases arrive as kernel arguments, each
+// request's shared fields from its device descriptor.
 __global__ __launch_bounds__(kWT2) void k_serve_wave(SArgs A) {
   __shared__ int sm[kWT + 2][kWT + 2];
-  int si = 0;
-  while (si + 1 < A.nslots && (int)blockIdx.x >= A.slot[si + 1].start) ++si;
-  const SSlot& S = A.slot[si];
-  const int b = (int)blockIdx.x - S.start;
-  if (b >= S.nblk) return;
+  int lo = 0, hi = A.nphases - 1;  // the last phase with start <= blockIdx.x
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (A.ph[mid].start <= (int)blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SPhase& P = A.ph[lo];
+  const int b = (int)blockIdx.x - P.start;
+  if (b >= P.nblk) return;
+  const SReq& Q = A.reqs[P.slot];
+  SSlot S;
+  S.st = Q.st + P.k;
+  S.canvas = Q.canvas;
+  S.snap = Q.snap;
+  S.dirty = Q.dirty;
+  S.pick = Q.pick + (int64_t)P.chain * 7 * Q.mstride;
+  S.pick_dst = S.pick;
+  S.fp = Q.fp;
+  S.dl = Q.dl;
+  S.dl_off = Q.dl_off;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) S.out[c] = Q.out[c];
+  S.cstride = Q.cstride;
+  S.mstride = Q.mstride;
+  S.dirty_bytes = Q.dirty_bytes;
+  S.rows = Q.rows;
+  S.cols = Q.cols;
+  S.kind = P.kind;
+  S.start = P.start;
+  S.nblk = P.nblk;
+  S.ntiles = P.kind == kSRing ? 0 : Q.ntiles;
+  S.rb = P.rb;
+  S.hr = Q.hr;
+  S.size_map = Q.size_map;
+  S.tiles_x = Q.tiles_x;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) S.fb[c] = Q.fb[c];
+  const bool stamp_kind = P.kind >= kSStamp && P.kind <= kSLevels;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    S.pk[c] = stamp_kind ? Q.paint[6 * (int64_t)P.k + c] : 0.0f;
+    S.ps[c] = stamp_kind ? Q.paint[6 * (int64_t)P.k + 3 + c] : 0.0f;
+  }
+  S.pickup_rate = Q.pickup_rate;
+  S.dep_rate = Q.dep_rate;
+  S.cap = Q.cap;
+  S.use_snap = Q.use_snap;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) S.bg[c] = Q.bg[c];
+  S.tmap = nullptr;  // served canvases have no drying clock
+  S.dry_ms = 0;
   run_wave_slot(S, b, sm);
 }
 
@@ -140,6 +222,7 @@ struct Req {
   std::vector<size_t> next;
   std::vector<char> udone;
   const PStamp* d_st = nullptr;
+  const float* d_paint = nullptr;
   const int2* d_dl = nullptr;
   const int* d_off = nullptr;
   const float* d_fp = nullptr;
@@ -210,6 +293,7 @@ struct pnt_server_s {
   pnt_ctx qctx = nullptr;    // views of resident canvases (pnt_server_canvas)
   pnt_ctx cctx = nullptr;    // copier stream
   hipEvent_t wave_ev[kAhead + 1] = {};
+  SReq* d_reqs = nullptr;    // one descriptor per request slot (device)
   int64_t waves = 0;
   std::thread engine, copier;
   std::vector<std::thread> planners;
@@ -221,6 +305,7 @@ struct pnt_server_s {
   std::vector<Req*> admit();
   void setup(Req* q);
   bool collect(Req* q, SArgs& A, int& start, std::vector<std::pair<Req*, int>>& used);
+  void write_descriptor(Req* q, int64_t& o);
   void after_wave(const std::vector<std::pair<Req*, int>>& used);
   void complete(Req* q);
   void finish(Req* q, int status, const std::string& err);
@@ -287,7 +372,8 @@ void pnt_server_s::plan(Req* q) {
   q->plan_bytes = al((int64_t)(q->st.size() * sizeof(PStamp))) +
                   al((int64_t)(q->dl.size() * sizeof(int2))) +
                   al((int64_t)(q->off.size() * sizeof(int))) +
-                  al((int64_t)(q->fp.size() * sizeof(float)));
+                  al((int64_t)(q->fp.size() * sizeof(float))) +
+                  al((int64_t)(q->paint.size() * sizeof(float))) + al((int64_t)sizeof(SReq));
   q->next.assign(q->U.size(), 0);
   q->udone.assign(q->U.size(), 0);
   for (size_t i = 0; i < q->U.size(); ++i)
@@ -506,11 +592,53 @@ void pnt_server_s::setup(Req* q) {
   const int64_t o_dl = put(q->dl.data(), q->dl.size() * sizeof(int2));
   const int64_t o_off = put(q->off.data(), q->off.size() * sizeof(int));
   const int64_t o_fp = put(q->fp.data(), q->fp.size() * sizeof(float));
-  if (o) PNT_HIP_CHECK(hipMemcpyAsync(S.plan, S.stage, (size_t)o, hipMemcpyHostToDevice, s));
+  const int64_t o_paint = put(q->paint.data(), q->paint.size() * sizeof(float));
+  const int64_t o_plan = o;
+  if (o_plan) PNT_HIP_CHECK(hipMemcpyAsync(S.plan, S.stage, (size_t)o_plan, hipMemcpyHostToDevice, s));
   q->d_st = reinterpret_cast<const PStamp*>(S.plan + o_st);
   q->d_dl = reinterpret_cast<const int2*>(S.plan + o_dl);
   q->d_off = reinterpret_cast<const int*>(S.plan + o_off);
   q->d_fp = reinterpret_cast<const float*>(S.plan + o_fp);
+  q->d_paint = reinterpret_cast<const float*>(S.plan + o_paint);
+  write_descriptor(q, o);
+}
+
+// The request's shared phase fields into its slot's device descriptor, on
+// the engine stream (after the waves of the slot's previous request).
+void pnt_server_s::write_descriptor(Req* q, int64_t& o) {
+  SlotRes& S = slots[(size_t)q->slot];
+  const Resident& R = *q->res;
+  const bool snap = q->r.use_snapshot != 0;
+  SReq D{};
+  D.st = q->d_st;
+  D.canvas = R.block;
+  D.snap = snap ? S.snap : R.block;
+  D.dirty = snap ? S.dirty : nullptr;
+  D.pick = S.pick;
+  D.fp = q->d_fp;
+  D.dl = q->d_dl;
+  D.dl_off = q->d_off;
+  D.paint = q->d_paint;
+  for (int k = 0; k < 3; ++k) D.out[k] = R.block + (kCanvasStrides + k) * R.stride;
+  D.cstride = (int)R.stride;
+  D.mstride = (int)q->mstride;
+  D.dirty_bytes = snap ? (int)dirty_bytes_for((int64_t)R.rows * R.cols) : 0;
+  D.rows = R.rows;
+  D.cols = R.cols;
+  D.hr = q->hr;
+  D.size_map = q->size_map;
+  D.tiles_x = q->tiles_x;
+  D.ntiles = q->ntiles;
+  D.use_snap = snap ? 1 : 0;
+  for (int k = 0; k < 4; ++k) D.fb[k] = q->fbox[k];
+  for (int k = 0; k < 3; ++k) D.bg[k] = R.bg[k];
+  D.pickup_rate = q->r.pickup_rate;
+  D.dep_rate = q->r.deposition_rate;
+  D.cap = 1.0;  // FootprintBrush.hxx:483
+  std::memcpy(S.stage + o, &D, sizeof D);
+  PNT_HIP_CHECK(hipMemcpyAsync(d_reqs + q->slot, S.stage + o, sizeof D, hipMemcpyHostToDevice,
+                               ctx->stream));
+  o += round_up((int64_t)sizeof D, 256);
 }
 
 // ---------------------------------------------------------------- waves --
@@ -519,45 +647,20 @@ void pnt_server_s::setup(Req* q) {
 // and the units it depends on were issued in earlier waves.
 bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
                            std::vector<std::pair<Req*, int>>& used) {
-  const SlotRes& SR = slots[(size_t)q->slot];
   const Resident& R = *q->res;
   const bool snap = q->r.use_snapshot != 0;
   const int64_t n = (int64_t)R.rows * R.cols;
-  auto base = [&](int kind) -> SSlot& {
-    SSlot& S = A.slot[A.nslots];
-    S = SSlot{};
-    S.canvas = R.block;
-    S.cstride = (int)R.stride;
-    S.snap = snap ? SR.snap : R.block;
-    S.dirty = snap ? SR.dirty : nullptr;
-    S.dirty_bytes = snap ? (int)dirty_bytes_for(n) : 0;
-    for (int k = 0; k < 3; ++k) S.out[k] = R.block + (kCanvasStrides + k) * R.stride;
-    S.rows = R.rows;
-    S.cols = R.cols;
-    S.kind = kind;
-    S.start = start;
-    S.pickup_rate = q->r.pickup_rate;
-    S.dep_rate = q->r.deposition_rate;
-    S.cap = 1.0f;  // FootprintBrush.hxx:483
-    S.use_snap = snap ? 1 : 0;
-    for (int k = 0; k < 3; ++k) S.bg[k] = R.bg[k];
-    S.tmap = nullptr;  // served canvases have no drying clock
-    S.dry_ms = 0;
-    S.st = q->d_st;
-    S.fp = q->d_fp;
-    S.dl = q->d_dl;
-    S.dl_off = q->d_off;
-    S.mstride = (int)q->mstride;
-    S.hr = q->hr;
-    S.size_map = q->size_map;
-    S.tiles_x = q->tiles_x;
-    S.ntiles = q->ntiles;
-    for (int k = 0; k < 4; ++k) S.fb[k] = q->fbox[k];
-    return S;
+  auto base = [&](int kind) -> SPhase& {
+    SPhase& P = A.ph[A.nphases];
+    P = SPhase{};
+    P.slot = (int16_t)q->slot;
+    P.kind = (int8_t)kind;
+    P.start = start;
+    return P;
   };
-  auto push = [&](SSlot& S, int unit) {
-    start += (std::max(S.nblk, 1) + 7) / 8 * 8;  // slots start on a multiple of 8 blocks
-    A.nslots++;
+  auto push = [&](SPhase& P, int unit) {
+    start += (std::max(P.nblk, 1) + 7) / 8 * 8;  // phases start on a multiple of 8 blocks
+    A.nphases++;
     used.push_back({q, unit});
   };
   bool any = false;
@@ -585,60 +688,56 @@ bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
     }
     break;
   }
-  if (A.nslots == kSlots) return false;
+  if (A.nphases == kPhases) return false;
   switch (q->stage) {
     case kPrologue: {
       if (q->fresh) {
-        SSlot& S = base(kSInit);
-        S.nblk = stream_blocks(kCanvasStrides * R.stride / 4);
-        push(S, -1);
+        SPhase& P = base(kSInit);
+        P.nblk = stream_blocks(kCanvasStrides * R.stride / 4);
+        push(P, -1);
         return true;
       }
-      SSlot& S = base(kSSnap);
-      S.nblk = stream_blocks(kLayerStrides * R.stride / 4);
-      push(S, -1);
+      SPhase& P = base(kSSnap);
+      P.nblk = stream_blocks(kLayerStrides * R.stride / 4);
+      push(P, -1);
       return true;
     }
     case kUnits: {
-      for (size_t i = q->g0; i <= q->g1 && A.nslots < kSlots; ++i) {
+      for (size_t i = q->g0; i <= q->g1 && A.nphases < kPhases; ++i) {
         if (q->udone[i]) continue;
         bool ok = q->prev_on_chain[i] < 0 || q->udone[(size_t)q->prev_on_chain[i]];
         for (int d : q->U[i].deps) ok = ok && q->udone[(size_t)d];
         if (!ok) continue;
         const Phase& f = q->ph[i][q->next[i]];
-        SSlot& S = base(f.kind);
-        S.st = q->d_st + f.k;
-        S.pick = SR.pick + (int64_t)q->U[i].chain * 7 * q->mstride;
-        for (int c = 0; c < 3; ++c) {
-          S.pk[c] = q->paint[6 * (size_t)f.k + c];
-          S.ps[c] = q->paint[6 * (size_t)f.k + 3 + c];
-        }
+        SPhase& P = base(f.kind);
+        P.k = (int)f.k;
+        P.chain = (int8_t)q->U[i].chain;
         const int rb = q->rb[(size_t)f.k];
         switch (f.kind) {
           case kSClear:
-            S.nblk = (int)std::min<int64_t>(64, (7 * q->mstride + 4 * kT2 - 1) / (4 * kT2));
+            P.nblk = (int)std::min<int64_t>(64, (7 * q->mstride + 4 * kT2 - 1) / (4 * kT2));
             break;
-          case kSStamp: S.rb = rb; S.nblk = q->ntiles + rb; break;
-          case kSRing: S.rb = rb; S.ntiles = 0; S.nblk = rb; break;
-          case kSWindow: S.rb = 0; S.nblk = q->ntiles; break;
-          default: S.nblk = 1; break;  // kSLevels
+          case kSStamp: P.rb = rb; P.nblk = q->ntiles + rb; break;
+          case kSRing: P.rb = rb; P.nblk = rb; break;
+          case kSWindow: P.rb = 0; P.nblk = q->ntiles; break;
+          default: P.nblk = 1; break;  // kSLevels
         }
-        if (S.nblk == 0) S.nblk = 1;  // an empty ring: the block returns at once
-        push(S, (int)i);
+        if (P.nblk == 0) P.nblk = 1;  // an empty ring: the block returns at once
+        push(P, (int)i);
         any = true;
       }
       return any;
     }
     case kDrying: {
-      SSlot& S = base(kSDry);
-      S.nblk = stream_blocks(n);
-      push(S, -2);
+      SPhase& P = base(kSDry);
+      P.nblk = stream_blocks(n);
+      push(P, -2);
       return true;
     }
     case kComposite: {
-      SSlot& S = base(kSCompose);
-      S.nblk = stream_blocks(n);
-      push(S, -3);
+      SPhase& P = base(kSCompose);
+      P.nblk = stream_blocks(n);
+      push(P, -3);
       return true;
     }
     default:
@@ -752,15 +851,16 @@ void pnt_server_s::run_engine() {
       active.push_back(q);
     }
     // one wave from the ready phases, oldest request first
-    A.nslots = 0;
+    A.reqs = d_reqs;
+    A.nphases = 0;
     int start = 0;
     used.clear();
     for (Req* q : active) {
-      if (A.nslots == kSlots) break;
+      if (A.nphases == kPhases) break;
       if (q->stage == kIssued) continue;
       collect(q, A, start, used);
     }
-    if (A.nslots > 0) {
+    if (A.nphases > 0) {
       if (waves >= kAhead)  // at most kAhead waves queued ahead of the host
         PNT_HIP_CHECK(hipEventSynchronize(wave_ev[(waves - kAhead) % (kAhead + 1)]));
       k_serve_wave<<<(unsigned)start, kT2, 0, s>>>(A);
@@ -770,7 +870,7 @@ void pnt_server_s::run_engine() {
       after_wave(used);
     }
     // requests whose device work is done
-    bool progressed = A.nslots > 0;
+    bool progressed = A.nphases > 0;
     for (size_t i = 0; i < active.size();) {
       Req* q = active[i];
       if (q->stage == kIssued && hipEventQuery(slots[(size_t)q->slot].fin) == hipSuccess) {
@@ -892,7 +992,9 @@ int pnt_server_create(int device, int n_slots, int64_t hbm_budget_bytes, pnt_ser
         PNT_HIP_CHECK(hipEventCreateWithFlags(&r.fin, hipEventDisableTiming));
       for (hipEvent_t& e : S->wave_ev)
         PNT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      PNT_HIP_CHECK(hipMalloc((void**)&S->d_reqs, (size_t)n_slots * sizeof(SReq)));
     } catch (...) {
+      if (S->d_reqs) (void)hipFree(S->d_reqs);
       for (SlotRes& r : S->slots)
         if (r.fin) (void)hipEventDestroy(r.fin);
       for (hipEvent_t e : S->wave_ev)
@@ -1037,6 +1139,7 @@ int pnt_server_destroy(pnt_server S) {
       dev_free(kv.second.block);
     }
     for (auto& kv : S->pool) dev_free(kv.second);
+    if (S->d_reqs) (void)hipFree(S->d_reqs);
     for (SlotRes& r : S->slots) {
       dev_free(r.snap);
       dev_free(r.dirty);

@@ -424,7 +424,7 @@ __device__ __forceinline__ void check_only(const VisitTarget& T, const DryClock&
 // out; they run afterwards by dependency level.  Each visit's pickup cell,
 // canvas pixel and source pixel are loaded in one batch.
 //
-// With a drying clock (clk != nullptr) every visited pixel -- in the canvas
+// With a drying clock (clk.tmap != nullptr) every visited pixel -- in the canvas
 // and in the map, whatever its footprint height -- gets Canvas::checkDry
 // before its pickup and deposit (FootprintBrush.hxx:115), by the thread that
 // runs its visit: map cells are then taken without the footprint-box cut,
@@ -434,7 +434,8 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
                                             const float* fp, const FpBox& fb, const StampGeom& g,
                                             const PStamp& st, bool dup, bool pull, int tile,
                                             int tiles_x, int (*sm)[kT + 2], const float pk[3],
-                                            const float ps[3], const DryClock* clk = nullptr) {
+                                            const float ps[3], const DryClock& clk = DryClock{}) {
+  const bool clocked = clk.tmap != nullptr;
   const int t = threadIdx.x;
   const int lx = t % kT, ly = t / kT;
   const int col0 = -g.hr + (tile % tiles_x) * kT;
@@ -444,7 +445,7 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
   for (int e = t; e < (kT + 2) * (kT + 2); e += kT * kT) {
     const int hy = e / (kT + 2), hx = e % (kT + 2);
     int64_t ic;
-    sm[hy][hx] = clk ? stamp_visit(g, col0 + hx - 1, row0 + hy - 1, &ic)
+    sm[hy][hx] = clocked ? stamp_visit(g, col0 + hx - 1, row0 + hy - 1, &ic)
                      : stamp_visit_in(g, col0 + hx - 1, row0 + hy - 1, fb.x0, fb.y0, fb.x1,
                                       fb.y1, &ic);
   }
@@ -494,14 +495,14 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
       }
     }
   }
-  if (act && clk && !(fh > 0.0f)) {  // visits that only check
-    check_only(T, *clk, ic, pv);
-    if (pdir) check_only(T, *clk, pic, ppull);
+  if (act && clocked && !(fh > 0.0f)) {  // visits that only check
+    check_only(T, clk, ic, pv);
+    if (pdir) check_only(T, clk, pic, ppull);
   }
   act = act && fh > 0.0f;
   if (act) {
-    if (clk) {
-      check_dry_regs(*clk, acc.cv, acc.cvv, ic);
+    if (clocked) {
+      check_dry_regs(clk, acc.cv, acc.cvv, ic);
       if (!T.use_snap) acc.src_from_canvas();  // the pickup source is the canvas itself
     }
     run_pixel(T, ic, pv, fh, pk, ps, acc);
@@ -519,8 +520,8 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
       } else {
         load_pixel(T, pic, ppull, acc);  // the pickup cell stays in acc.v
       }
-      if (clk) {
-        check_dry_regs(*clk, acc.cv, acc.cvv, pic);
+      if (clocked) {
+        check_dry_regs(clk, acc.cv, acc.cvv, pic);
         if (!T.use_snap) acc.src_from_canvas();
       }
       run_pixel(T, pic, ppull, fh, pk, ps, acc);
@@ -540,7 +541,8 @@ template <int kThreads>
 __device__ __forceinline__ void level_visits(const VisitTarget& T, float* const* pick,
                                              const float* fp, const StampGeom& g, const PStamp& st,
                                              const int2* dl, const int* dl_off, const float pk[3],
-                                             const float ps[3], const DryClock* clk = nullptr) {
+                                             const float ps[3],
+                                             const DryClock& clk = DryClock{}) {
   for (int L = 0; L < st.dl_nlev; ++L) {
     const int e0 = dl_off[st.dl_lev + L], e1 = dl_off[st.dl_lev + L + 1];
     for (int e = e0 + (int)threadIdx.x; e < e1; e += kThreads) {
@@ -548,7 +550,7 @@ __device__ __forceinline__ void level_visits(const VisitTarget& T, float* const*
       int64_t ic;
       const int m = stamp_visit(g, v.x, v.y, &ic);
       if (m < 0) continue;
-      if (clk) check_only(T, *clk, ic, false);
+      if (clk.tmap) check_only(T, clk, ic, false);
       if (fp[m] > 0.0f) cached_visit(T, GlobalPick{pick, m}, fp[m], ic, false, pk, ps);
     }
     __builtin_amdgcn_s_waitcnt(0);  // this level's stores before the next level's loads

@@ -179,16 +179,15 @@ __device__ __forceinline__ void run_wave_slot(const SSlot& S, int b, int (*sm)[k
   const FpBox fb{S.fb[0], S.fb[1], S.fb[2], S.fb[3]};
   // Canvas::checkDry on every visited pixel when the canvas has a clock
   const DryClock ck{S.tmap, st.now_us, S.dry_ms, Rgb3{{R0, R0 + cs, R0 + 2 * cs}}, hp};
-  const DryClock* clk = S.tmap ? &ck : nullptr;
   const int win = (S.kind == kSStamp || S.kind == kSWindow) ? S.ntiles : 0;
   if (S.kind == kSLevels) {  // one block
-    level_visits<kWT2>(T, pick, S.fp, g, st, S.dl, S.dl_off, S.pk, S.ps, clk);
+    level_visits<kWT2>(T, pick, S.fp, g, st, S.dl, S.dl_off, S.pk, S.ps, ck);
   } else if (b < win) {
     window_tile<kWT>(T, pick, S.fp, fb, g, st, dup, pull, tile_of_block(b, S.ntiles), S.tiles_x,
-                     sm, S.pk, S.ps, clk);
+                     sm, S.pk, S.ps, ck);
   } else if (b < win + S.rb) {
     ring_scan<2>(canvas, snap, S.dirty, S.fp, g, st.box, pull, (b - win) * kWT2 + (int)threadIdx.x,
-                 S.rb * kWT2, clk != nullptr);
+                 S.rb * kWT2, S.tmap != nullptr);
   }
 }
 
