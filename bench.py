@@ -837,8 +837,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seg-len", type=int, default=32)
     ap.add_argument("--diff-steps", type=int, default=3)
-    ap.add_argument("--rates", default="200,500,1000,2000")
-    ap.add_argument("--rates2048", default="500,1000,2000")
+    ap.add_argument("--rates", default="200,500,1000,1250,1500,2000")
+    ap.add_argument("--rates2048", default="500,1000,1250,1500,2000")
     ap.add_argument("--seconds", type=float, default=3.0)
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--hbm-gb", type=float, default=96.0)

@@ -8,8 +8,9 @@
 //
 // Per job (a canvas, its brush and stamps): the stamps are cut into units
 // (runs of one stroke, plan_units in fused.hip) whose snapshot boxes decide
-// which may run at once; `chunk` bounds how many units of one canvas are in
-// flight (each has its own pickup map).  The brush's own map takes the chain
+// which may run at once; `chunk` bounds how many units of the longest job
+// are in flight (each has its own pickup map; shorter jobs get
+// proportionally fewer).  The brush's own map takes the chain
 // of the job's last unit, so it ends holding the last stroke's map, as it
 // would after the same stamps run alone; a job whose first unit continues
 // the brush's current stroke keeps that map on its first chain and copies
@@ -53,6 +54,10 @@ struct BWJob {
 
 struct BWSlot {
   int job, k, kind, start, nblk, rb, chain, pad;
+  // image offsets of the phase's stamp descriptor and paint: with them the
+  // block loads its stamp, paint and job record at once, not one after
+  // another
+  int64_t o_stamp, o_paint;
 };
 
 // One wave: the group table gives the slot of every 8 blocks (slots start on
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJ
   if (b >= d.nblk) return;
   const BWJob& J = jobs[d.job];
   SSlot S;
-  S.st = reinterpret_cast<const PStamp*>(base + J.o_st) + d.k;
+  S.st = reinterpret_cast<const PStamp*>(base + d.o_stamp);
   S.canvas = J.canvas;
   S.snap = J.snap;
   S.dirty = J.dirty;
@@ -91,11 +96,12 @@ __global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJ
   S.tiles_x = J.tiles_x;
 #pragma unroll
   for (int c = 0; c < 4; ++c) S.fb[c] = J.fb[c];
-  const float* paint = reinterpret_cast<const float*>(base + J.o_paint) + 6 * (int64_t)d.k;
+  const float* paint = reinterpret_cast<const float*>(base + d.o_paint);
+  const bool stamp_kind = d.kind >= kSStamp && d.kind <= kSLevels;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    S.pk[c] = paint[c];
-    S.ps[c] = paint[3 + c];
+    S.pk[c] = stamp_kind ? paint[c] : 0.0f;
+    S.ps[c] = stamp_kind ? paint[3 + c] : 0.0f;
   }
   S.pickup_rate = J.pickup_rate;
   S.dep_rate = J.dep_rate;
@@ -219,22 +225,37 @@ void parallel_for(int n, int64_t work, F&& f) {
 
 }  // namespace
 
-// Plans a batch as waves (chunk: units of one canvas in flight at once).
+// Units in flight for each job: `chunk` for the job with the most stamps,
+// proportionally fewer (at least 1) for shorter ones.  The batch ends with
+// its longest job, so extra concurrency goes where it shortens that chain;
+// short jobs keep one pickup map and add no map traffic.
+static std::vector<int> job_chunks(const pnt_batch_job* jobs, int n, int chunk) {
+  chunk = std::max(1, std::min(chunk, kMaxChunk));
+  int64_t most = 1;
+  for (int j = 0; j < n; ++j) most = std::max<int64_t>(most, jobs[j].n_stamps);
+  std::vector<int> c((size_t)n, 1);
+  for (int j = 0; j < n; ++j)
+    c[(size_t)j] = std::max(1, (int)std::lround((double)chunk * (double)jobs[j].n_stamps /
+                                                (double)most));
+  return c;
+}
+
+// Plans a batch as waves (chunk: units of the longest job in flight at once).
 // The brushes' snapshots are readied on stream s first (as for one launch).
 // `pool` receives the extra pickup maps; plan_waves_size gives its size.
 int64_t plan_waves_pool(const pnt_batch_job* jobs, int n, int chunk) {
+  const std::vector<int> c = job_chunks(jobs, n, chunk);
   int64_t need = 0;
-  for (int j = 0; j < n; ++j)
-    need += (int64_t)(std::min(chunk, kMaxChunk) - 1) * 7 * jobs[j].brush->mstride;
+  for (int j = 0; j < n; ++j) need += (int64_t)(c[(size_t)j] - 1) * 7 * jobs[j].brush->mstride;
   return need;
 }
 
 BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipStream_t s) {
-  chunk = std::max(1, std::min(chunk, kMaxChunk));
+  const std::vector<int> chunks = job_chunks(jobs, n, chunk);
   std::vector<JobPlan> P((size_t)n);
   int64_t work = 0;
   for (int j = 0; j < n; ++j) work += jobs[j].n_stamps;
-  parallel_for(n, work, [&](int j) { plan_job(jobs[j], chunk, P[(size_t)j]); });
+  parallel_for(n, work, [&](int j) { plan_job(jobs[j], chunks[(size_t)j], P[(size_t)j]); });
   for (int j = 0; j < n; ++j) brush_prepare_snapshot(jobs[j].brush, jobs[j].canvas, s);
 
   // simulate: every wave takes every ready phase of every job
@@ -248,7 +269,7 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
     int start = 0;
     took.clear();
     auto add = [&](int j, int64_t k, int kind, int nblk, int rb, int chain, int tag) {
-      BWSlot d{j, (int)std::max<int64_t>(k, 0), kind, start, std::max(nblk, 1), rb, chain, 0};
+      BWSlot d{j, (int)std::max<int64_t>(k, 0), kind, start, std::max(nblk, 1), rb, chain, 0, 0, 0};
       const int span = (d.nblk + 7) / 8 * 8;
       for (int q = 0; q < span / 8; ++q) grp.push_back((int)(slots.size() - (size_t)s0));
       slots.push_back(d);
@@ -416,6 +437,10 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
       std::memcpy(W.blob.data() + o_paint[j], J.paint.data(), J.paint.size() * sizeof(float));
   }
   W.pool_floats = pool_at;
+  for (BWSlot& d : slots) {
+    d.o_stamp = o_st[(size_t)d.job] + (int64_t)d.k * (int64_t)sizeof(PStamp);
+    d.o_paint = o_paint[(size_t)d.job] + 6 * (int64_t)d.k * (int64_t)sizeof(float);
+  }
   if (!slots.empty()) std::memcpy(W.blob.data() + o_slots, slots.data(), slots.size() * sizeof(BWSlot));
   if (!grp.empty()) std::memcpy(W.blob.data() + o_grp, grp.data(), grp.size() * sizeof(int));
   for (size_t w = 0; w < ws.size(); ++w) {

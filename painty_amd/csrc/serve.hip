@@ -66,7 +66,7 @@ using Clock = std::chrono::steady_clock;
 namespace {
 
 constexpr int kPhases = 160;       // phases per wave
-constexpr int kChainsPerReq = 4;   // pickup maps of one request (concurrent units)
+constexpr int kChainsPerReq = 8;   // pickup maps of one request (concurrent units)
 constexpr int kAhead = 3;          // waves queued ahead of the engine
 constexpr int kT = kWT;            // window tile side
 constexpr int kT2 = kWT2;          // threads per block
