@@ -18,6 +18,7 @@
 // phases of their own.
 #include <algorithm>
 #include <atomic>
+#include <cstddef>
 #include <cstring>
 #include <exception>
 #include <numeric>
@@ -52,6 +53,27 @@ struct BWJob {
   int64_t dry_ms;
 };
 
+// BWJob as the kernel reads it: the same layout, device pointers global.
+struct BWJobG {
+  gptr<float> canvas;
+  gptr<float> snap;
+  gptr<uint8_t> dirty;
+  gptr<float> out[3];
+  gptr<float> pick[kMaxChunk];
+  gptr<const float> fp;
+  int64_t o_st, o_dl, o_off, o_paint;
+  int cstride, mstride, dirty_bytes, rows, cols, hr, size_map, tiles_x, ntiles, use_snap;
+  int fb[4];
+  double pickup_rate, dep_rate, cap;
+  int pad;
+  gptr<int64_t> tmap;
+  int64_t dry_ms;
+};
+static_assert(sizeof(BWJobG) == sizeof(BWJob) && offsetof(BWJobG, fp) == offsetof(BWJob, fp) &&
+                  offsetof(BWJobG, fb) == offsetof(BWJob, fb) &&
+                  offsetof(BWJobG, tmap) == offsetof(BWJob, tmap),
+              "BWJobG mirrors BWJob");
+
 struct BWSlot {
   int job, k, kind, start, nblk, rb, chain, pad;
   // image offsets of the phase's stamp descriptor and paint: with them the
@@ -62,25 +84,25 @@ struct BWSlot {
 
 // One wave: the group table gives the slot of every 8 blocks (slots start on
 // multiples of 8 blocks).
-__global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJob* jobs,
+__global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJobG* jobs,
                                                      const BWSlot* slots, const int* grp) {
   __shared__ int sm[kWT + 2][kWT + 2];
   const BWSlot d = slots[grp[blockIdx.x >> 3]];
   const int b = (int)blockIdx.x - d.start;
   if (b >= d.nblk) return;
-  const BWJob& J = jobs[d.job];
+  const BWJobG& J = jobs[d.job];
   SSlot S;
   S.st = reinterpret_cast<const PStamp*>(base + d.o_stamp);
-  S.canvas = J.canvas;
-  S.snap = J.snap;
-  S.dirty = J.dirty;
-  S.pick = J.pick[d.chain];
-  S.pick_dst = J.pick[0];  // kSCopyPick: this chain's map -> the brush's own
-  S.fp = J.fp;
+  S.canvas = (float*)J.canvas;
+  S.snap = (float*)J.snap;
+  S.dirty = (uint8_t*)J.dirty;
+  S.pick = (float*)J.pick[d.chain];
+  S.pick_dst = (float*)J.pick[0];  // kSCopyPick: this chain's map -> the brush's own
+  S.fp = (const float*)J.fp;
   S.dl = reinterpret_cast<const int2*>(base + J.o_dl);
   S.dl_off = reinterpret_cast<const int*>(base + J.o_off);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) S.out[c] = J.out[c];
+  for (int c = 0; c < 3; ++c) S.out[c] = (float*)J.out[c];
   S.cstride = J.cstride;
   S.mstride = J.mstride;
   S.dirty_bytes = J.dirty_bytes;
@@ -108,7 +130,7 @@ __global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJ
   S.cap = J.cap;
   S.use_snap = J.use_snap;
   S.bg[0] = S.bg[1] = S.bg[2] = 0.0f;
-  S.tmap = J.tmap;
+  S.tmap = (int64_t*)J.tmap;
   S.dry_ms = J.dry_ms;
   run_wave_slot(S, b, sm);
 }
@@ -454,7 +476,7 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
 // Copies the image (pinned `stage` -> `dev`) and launches every wave on s.
 void enqueue_waves(const BWPlan& W, const char* stage, char* dev, hipStream_t s) {
   if (W.total) PNT_HIP_CHECK(hipMemcpyAsync(dev, stage, (size_t)W.total, hipMemcpyHostToDevice, s));
-  const BWJob* jobs = reinterpret_cast<const BWJob*>(dev + W.o_jobs);
+  const BWJobG* jobs = reinterpret_cast<const BWJobG*>(dev + W.o_jobs);
   for (size_t w = 0; w < W.wave_blocks.size(); ++w) {
     k_batch_wave<<<(unsigned)W.wave_blocks[w], kWT2, 0, s>>>(
         dev, jobs, reinterpret_cast<const BWSlot*>(dev + W.wave_slots[w]),

@@ -54,6 +54,14 @@ struct Rgb3 {
   float* p[3];
 };
 
+// A pointer in the global address space.  Kernels that read device-memory
+// records holding pointers view them through a mirror record with these
+// fields: a pointer loaded as global keeps that address space through the
+// code that uses it, so its loads and stores compile to global_* memory
+// operations instead of flat ones (which also wait on LDS traffic).
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
 // Brush state and parameters one imprint needs.
 struct BrushArgs {
   const float* fp;   // footprint, size_map^2

@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstddef>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -91,6 +92,29 @@ struct SReq {
   double pickup_rate, dep_rate, cap;
 };
 
+// SReq as the kernel reads it: the same layout, device pointers global.
+struct SReqG {
+  gptr<const PStamp> st;
+  gptr<float> canvas;
+  gptr<float> snap;
+  gptr<uint8_t> dirty;
+  gptr<float> pick;
+  gptr<const float> fp;
+  gptr<const int2> dl;
+  gptr<const int> dl_off;
+  gptr<const float> paint;
+  gptr<float> out[3];
+  int cstride, mstride, dirty_bytes, rows, cols, hr, size_map, tiles_x, ntiles, use_snap;
+  int fb[4];
+  float bg[3];
+  int pad;
+  double pickup_rate, dep_rate, cap;
+};
+static_assert(sizeof(SReqG) == sizeof(SReq) && offsetof(SReqG, out) == offsetof(SReq, out) &&
+                  offsetof(SReqG, cstride) == offsetof(SReq, cstride) &&
+                  offsetof(SReqG, pickup_rate) == offsetof(SReq, pickup_rate),
+              "SReqG mirrors SReq");
+
 // One phase of a wave (a kernel argument): its request slot, its kind, the
 // stamp it runs and the blocks it takes.
 struct SPhase {
@@ -104,7 +128,7 @@ struct SPhase {
 };
 
 struct SArgs {
-  const SReq* reqs;
+  const SReqG* reqs;
   int nphases;
   int pad;
   SPhase ph[kPhases];
@@ -126,19 +150,19 @@ __global__ __launch_bounds__(kWT2) void k_serve_wave(SArgs A) {
   const SPhase& P = A.ph[lo];
   const int b = (int)blockIdx.x - P.start;
   if (b >= P.nblk) return;
-  const SReq& Q = A.reqs[P.slot];
+  const SReqG& Q = A.reqs[P.slot];
   SSlot S;
-  S.st = Q.st + P.k;
-  S.canvas = Q.canvas;
-  S.snap = Q.snap;
-  S.dirty = Q.dirty;
-  S.pick = Q.pick + (int64_t)P.chain * 7 * Q.mstride;
+  S.st = (const PStamp*)(Q.st + P.k);
+  S.canvas = (float*)Q.canvas;
+  S.snap = (float*)Q.snap;
+  S.dirty = (uint8_t*)Q.dirty;
+  S.pick = (float*)(Q.pick + (int64_t)P.chain * 7 * Q.mstride);
   S.pick_dst = S.pick;
-  S.fp = Q.fp;
-  S.dl = Q.dl;
-  S.dl_off = Q.dl_off;
+  S.fp = (const float*)Q.fp;
+  S.dl = (const int2*)Q.dl;
+  S.dl_off = (const int*)Q.dl_off;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) S.out[c] = Q.out[c];
+  for (int c = 0; c < 3; ++c) S.out[c] = (float*)Q.out[c];
   S.cstride = Q.cstride;
   S.mstride = Q.mstride;
   S.dirty_bytes = Q.dirty_bytes;
@@ -851,7 +875,7 @@ void pnt_server_s::run_engine() {
       active.push_back(q);
     }
     // one wave from the ready phases, oldest request first
-    A.reqs = d_reqs;
+    A.reqs = reinterpret_cast<const SReqG*>(d_reqs);
     A.nphases = 0;
     int start = 0;
     used.clear();
