@@ -114,14 +114,14 @@ def host_description():
     return {"nproc": usable, "cpu_count": os.cpu_count(), "model": model}
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summaries
-    (profiles/*.md lines `traffic_bytes_per_launch[<kernel>]: N`: FETCH_SIZE
-    and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), or
-    None when no summary covers it."""
+def profiled_traffic(kernel, per="launch"):
+    """HBM bytes per launch (or per batch) of `kernel` from the committed
+    counter summaries (profiles/*.md lines `traffic_bytes_per_<per>[<kernel>]:
+    N`: FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), or None when no summary covers it."""
     import glob
     import re
-    pat = re.compile(r"traffic_bytes_per_launch\[(.+?)\]:\s*(\d+)")
+    pat = re.compile(r"traffic_bytes_per_%s\[(.+?)\]:\s*(\d+)" % per)
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         with open(path) as f:
             for line in f:
@@ -545,7 +545,8 @@ def leg_batched(args):
         "speedup_vs_plain": round(plain / wall, 2) if plain else None,
         "roofline": {"bound": "hbm", "kernel": batch_kernel, "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(batch_kernel),
+                     "traffic": profiled_traffic(batch_kernel, "batch"),
+                     "traffic_per": "batch",
                      "algorithmic_bytes_per_batch": int(alg_bytes),
                      "measured": "algorithmic bytes of the batch / stream time of one batch"
                                  + (" (every wave launch)" if waves else " (one launch)")},
@@ -720,9 +721,10 @@ def leg_diff(args):
 
 def bench_copy(args):
     """Calibration: PaintLayer::copyTo of a size x size layer (k_copy_layer,
-    7 FP32 planes read and written).  Gives the achievable copy bandwidth and,
-    under a counter profiler, the FETCH_SIZE / WRITE_SIZE that a kernel of
-    known traffic reports."""
+    six FP32 planes and the FP64 V plane read and written: 32 B/px each
+    way).  Gives the achievable copy bandwidth and, under a counter
+    profiler, the FETCH_SIZE / WRITE_SIZE that a kernel of known traffic
+    reports."""
     from painty_amd import _lib
     from painty_amd import hip as H
     n = args.size * args.size
@@ -739,7 +741,7 @@ def bench_copy(args):
         _lib.call("pnt_layer_copy_to", a.h, b.h)
     ev.record(e1, ctx.stream)
     ms = ev.elapsed_ms(e0, e1) / reps
-    nbytes = 28 * n
+    nbytes = 32 * n
     print(json.dumps({"calibration": "k_copy_layer", "pixels": n, "bytes_read": nbytes,
                       "bytes_written": nbytes, "ms": round(ms, 4),
                       "gb_s": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)

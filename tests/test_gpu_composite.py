@@ -220,3 +220,20 @@ def test_canvas_lifecycle(hip):
     assert other.shape == (5, 6) and other.getV_buffer()[2, 3] == np.float32(0.5)
     layer.clear()
     assert not layer.getK_buffer().any()
+
+
+def test_volume_plane_is_fp64(hip):
+    """V is held in FP64: a float64 V goes up and comes back bit for bit; a
+    float32 V is widened exactly; the FP32 download rounds it."""
+    rows, cols = 37, 53
+    rng = np.random.default_rng(3)
+    V64 = rng.uniform(0.0, 3.0, (rows, cols)) + 1e-12  # not representable in FP32
+    layer = hip.PaintLayer(rows, cols)
+    layer.setBuffers(V=V64)
+    got = layer.getV_buffer()
+    assert got.dtype == np.float64 and np.array_equal(got, V64)
+    V32 = V64.astype(np.float32)
+    layer.setBuffers(V=V32)
+    assert np.array_equal(layer.getV_buffer(), V32.astype(np.float64))
+    layer.setBuffers(V=V64)
+    assert np.array_equal(layer._download("V"), V64.astype(np.float32))
