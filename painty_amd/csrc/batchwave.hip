@@ -84,7 +84,7 @@ struct BWSlot {
 
 // One wave: the group table gives the slot of every 8 blocks (slots start on
 // multiples of 8 blocks).
-__global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJobG* jobs,
+__global__ __launch_bounds__(kWT2, kWaveOccupancy) void k_batch_wave(const char* base, const BWJobG* jobs,
                                                      const BWSlot* slots, const int* grp) {
   __shared__ int sm[kWT + 2][kWT + 2];
   const BWSlot d = slots[grp[blockIdx.x >> 3]];
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJ
   S.kind = d.kind;
   S.start = d.start;
   S.nblk = d.nblk;
-  S.ntiles = d.kind == kSRing ? 0 : J.ntiles;
+  S.ntiles = J.ntiles;
   S.rb = d.rb;
   S.hr = J.hr;
   S.size_map = J.size_map;
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJ
 #pragma unroll
   for (int c = 0; c < 4; ++c) S.fb[c] = J.fb[c];
   const float* paint = reinterpret_cast<const float*>(base + d.o_paint);
-  const bool stamp_kind = d.kind >= kSStamp && d.kind <= kSLevels;
+  const bool stamp_kind = d.kind == kSStamp;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     S.pk[c] = stamp_kind ? paint[c] : 0.0f;
@@ -132,6 +132,8 @@ __global__ __launch_bounds__(kWT2) void k_batch_wave(const char* base, const BWJ
   S.bg[0] = S.bg[1] = S.bg[2] = 0.0f;
   S.tmap = (int64_t*)J.tmap;
   S.dry_ms = J.dry_ms;
+  S.part = 0;  // streaming phases run whole
+  S.nparts = 1;
   run_wave_slot(S, b, sm);
 }
 
@@ -175,13 +177,7 @@ void plan_job(const pnt_batch_job& job, int chunk, JobPlan& P) {
       if (job.stamps[k].flags & PNT_STAMP_DIP) P.ph[i].push_back(Phase{k, kSClear});
       const PStamp& p = P.st[(size_t)k];
       P.rb[(size_t)k] = b->use_snap ? ring_blocks_for(p.box, kWT2) : 0;
-      if (p.flags & (kStDupX | kStDupY)) {
-        if (b->use_snap) P.ph[i].push_back(Phase{k, kSRing});
-        P.ph[i].push_back(Phase{k, kSWindow});
-        if (p.dl_nlev > 0) P.ph[i].push_back(Phase{k, kSLevels});
-      } else {
-        P.ph[i].push_back(Phase{k, kSStamp});
-      }
+      P.ph[i].push_back(Phase{k, kSStamp});
     }
   P.prev_on_chain.assign(P.U.size(), -1);
   std::vector<int> last(kMaxChunk, -1);
@@ -339,12 +335,12 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
               case kSClear:
                 nblk = (int)std::min<int64_t>(64, (7 * b->mstride + 4 * kWT2 - 1) / (4 * kWT2));
                 break;
-              case kSStamp: nblk = ntiles + rb; break;
-              case kSRing: nblk = rb; break;
-              case kSWindow: nblk = ntiles; break;
+              case kSStamp:  // + the shared-pixel levels' block of a dup stamp
+                nblk = ntiles + rb + (dup_levels(J.st[(size_t)f.k]) ? 1 : 0);
+                break;
               default: nblk = 1; break;
             }
-            add(j, f.k, f.kind, nblk, f.kind == kSWindow ? 0 : rb, J.perm[J.U[i].chain], (int)i);
+            add(j, f.k, f.kind, nblk, rb, J.perm[J.U[i].chain], (int)i);
           }
           break;
         case kJDrying:  // its stamp: the drying phase's last (its time point)

@@ -71,48 +71,41 @@ __device__ __forceinline__ StampGeom geom_of(const FArgs& A, const PStamp& st) {
   return g;
 }
 
-// Stamp j: window tiles on blocks [0, win_blocks) (win_blocks is 0 or
-// ntiles), ring scan on the next ring_blocks blocks.
+// Stamp j: window tiles on blocks [0, ntiles), ring scan on the next
+// ring_blocks blocks, and for a stamp on a duplicated canvas column/row 0
+// its shared-pixel visits by level on one more block.  Visits on the window
+// border refresh their own snapshot pixels (the ring scan leaves them).
 template <int kT>
-__global__ __launch_bounds__(kT* kT) void k_stamp_fused(FArgs A, int j, int win_blocks,
-                                                        int ring_blocks) {
+__global__ __launch_bounds__(kT* kT) void k_stamp_fused(FArgs A, int j, int ring_blocks) {
   __shared__ int sm[kT + 2][kT + 2];
   const PStamp& st = A.st[j];
   const StampGeom g = geom_of(A, st);
   const bool dup = (st.flags & (kStDupX | kStDupY)) != 0;
-  const bool pull = A.use_snap && !dup;
+  const bool pull = A.use_snap != 0;
   const DryClock clk = stamp_clock(A.clk, st);
-  if ((int)blockIdx.x < win_blocks) {
-    const VisitTarget T{&A.canvas, &A.snap, A.dirty, A.use_snap, A.pickup_rate, A.dep_rate,
-                        A.cap};
-    window_tile<kT>(T, A.pick.p, A.fp, A.fb, g, st, dup, pull, tile_of_block(blockIdx.x, A.ntiles),
-                    A.tiles_x, sm, A.pk, A.ps, clk);
-  } else {
-    ring_scan<2>(A.canvas, A.snap, A.dirty, A.fp, g, st.box, pull,
-                 ((int)blockIdx.x - win_blocks) * kT * kT + (int)threadIdx.x,
-                 ring_blocks * kT * kT, clk.tmap != nullptr);
-  }
-}
-
-// Stamp j's visits on a duplicated canvas column/row 0 and their map-cell
-// partners, in reference order by dependency level; one workgroup.
-__global__ __launch_bounds__(256) void k_stamp_levels(FArgs A, int j) {
-  const PStamp& st = A.st[j];
-  const StampGeom g = geom_of(A, st);
   const VisitTarget T{&A.canvas, &A.snap, A.dirty, A.use_snap, A.pickup_rate, A.dep_rate, A.cap};
-  level_visits<256>(T, A.pick.p, A.fp, g, st, A.dl, A.dl_off, A.pk, A.ps,
-                    stamp_clock(A.clk, st));
+  const int b = (int)blockIdx.x;
+  if (b < A.ntiles) {
+    window_tile<kT>(T, A.pick.p, A.fp, A.fb, g, st, dup, pull, tile_of_block(b, A.ntiles),
+                    A.tiles_x, sm, A.pk, A.ps, clk);
+  } else if (b < A.ntiles + ring_blocks) {
+    ring_scan<kRingGroups>(A.canvas, A.snap, A.dirty, A.fp, g, st.box, pull,
+                 (b - A.ntiles) * kT * kT + (int)threadIdx.x, ring_blocks * kT * kT,
+                 clk.tmap != nullptr);
+  } else if (dup) {
+    level_visits<kT * kT>(T, A.pick.p, A.fp, g, st, A.dl, A.dl_off, A.pk, A.ps, clk, pull);
+  }
 }
 
 }  // namespace
 
 // 4-pixel groups of box b, and the blocks that scan them (each thread
-// takes two groups per pass).
+// takes kRingGroups groups per pass).
 int ring_blocks_for(const RingBox& b, int threads) {
   if (b.a_x1 < b.a_x0 || b.a_y1 < b.a_y0) return 0;
   const int64_t span = b.a_x1 - b.a_x0 + 1;
-  const int64_t groups = ((span + 3) / 4 + 1) * (int64_t)(b.a_y1 - b.a_y0 + 1);
-  const int64_t blocks = (groups + 2 * threads - 1) / (2 * threads);
+  const int64_t ngroups = ((span + 3) / 4 + 1) * (int64_t)(b.a_y1 - b.a_y0 + 1);
+  const int64_t blocks = (ngroups + kRingGroups * threads - 1) / (kRingGroups * threads);
   return (int)std::min<int64_t>(std::max<int64_t>(blocks, 1), 1024);
 }
 
@@ -266,15 +259,15 @@ int fused_chains(int want) {
 
 // ----------------------------------------------------------------- waves --
 // The concurrent strokes again, on one stream: each launch ("wave") runs
-// the next phase of every unit that is ready -- a stamp (window tiles and
-// ring scan side by side), or the ring / window / shared-pixel-levels steps
-// of a stamp that straddles canvas column/row 0, or the pickup-map clear of
-// a dip or a geometry change -- each on its own range of blocks.  Units of
+// the next phase of every unit that is ready -- a stamp (window tiles, ring
+// scan and, for a stamp that straddles canvas column/row 0, its
+// shared-pixel levels side by side), or the pickup-map clear of a dip or a
+// geometry change -- each on its own range of blocks.  Units of
 // one wave touch disjoint pixels and their own pickup maps, exactly as on
 // concurrent streams, and the stream orders the waves: no events, and no
 // limit from the hardware queues.
 constexpr int kWaveSlots = 8;
-enum : int { kPhClear = 0, kPhStamp = 1, kPhRing = 2, kPhWindow = 3, kPhLevels = 4 };
+enum : int { kPhClear = 0, kPhStamp = 1 };
 
 struct WSlot {
   int j;       // stamp index (not used by kPhClear)
@@ -332,21 +325,20 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_wave(WArgs A) {
   g.rows = A.rows;
   g.cols = A.cols;
   const bool dup = (st.flags & (kStDupX | kStDupY)) != 0;
-  const bool pull = A.use_snap && !dup;
+  const bool pull = A.use_snap != 0;  // border visits refresh their own snapshot pixels
   const VisitTarget T{&A.canvas, &A.snap, A.dirty, A.use_snap, A.pickup_rate, A.dep_rate, A.cap};
   float* pick[7];
 #pragma unroll
   for (int c = 0; c < 7; ++c) pick[c] = S.pick + c * S.mstride;
   const DryClock clk = stamp_clock(A.clk, st);
-  const int win = (S.kind == kPhStamp || S.kind == kPhWindow) ? S.ntiles : 0;
-  if (S.kind == kPhLevels) {  // one block (nblk == 1)
-    level_visits<kT * kT>(T, pick, S.fp, g, st, A.dl, A.dl_off, S.pk, S.ps, clk);
-  } else if (b < win) {
+  if (b < S.ntiles) {
     window_tile<kT>(T, pick, S.fp, S.fb, g, st, dup, pull, tile_of_block(b, S.ntiles), S.tiles_x,
                     sm, S.pk, S.ps, clk);
-  } else if (b < win + S.rb) {
-    ring_scan<2>(A.canvas, A.snap, A.dirty, S.fp, g, st.box, pull, (b - win) * kT * kT +
+  } else if (b < S.ntiles + S.rb) {
+    ring_scan<kRingGroups>(A.canvas, A.snap, A.dirty, S.fp, g, st.box, pull, (b - S.ntiles) * kT * kT +
                  (int)threadIdx.x, S.rb * kT * kT, clk.tmap != nullptr);
+  } else if (dup) {  // the shared-pixel visits, beside the tiles
+    level_visits<kT * kT>(T, pick, S.fp, g, st, A.dl, A.dl_off, S.pk, S.ps, clk, pull);
   }
 }
 
@@ -424,17 +416,10 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
     const PStamp& p = D.host[k];
     const int rb = a.use_snap ? ring_blocks_for(p.box, kT * kT) : 0;
     const int jk = (int)k;
-    if (!(p.flags & (kStDupX | kStDupY))) {
-      if (q == s) ktimer_mark(b, q);
-      k_stamp_fused<kT><<<a.ntiles + rb, kT * kT, 0, q>>>(a, jk, a.ntiles, rb);
-      if (q == s) ktimer_mark(b, q);
-      b->last_launches += 1;
-    } else {  // snapshot refresh first: a duplicated pixel is not refreshed by its visits
-      if (rb) k_stamp_fused<kT><<<rb, kT * kT, 0, q>>>(a, jk, 0, rb);
-      k_stamp_fused<kT><<<a.ntiles, kT * kT, 0, q>>>(a, jk, a.ntiles, 0);
-      if (p.dl_nlev) k_stamp_levels<<<1, 256, 0, q>>>(a, jk);
-      b->last_launches += (rb ? 1 : 0) + 1 + (p.dl_nlev ? 1 : 0);
-    }
+    if (q == s) ktimer_mark(b, q);
+    k_stamp_fused<kT><<<a.ntiles + rb + (dup_levels(p) ? 1 : 0), kT * kT, 0, q>>>(a, jk, rb);
+    if (q == s) ktimer_mark(b, q);
+    b->last_launches += 1;
     PNT_HIP_CHECK(hipGetLastError());
   };
   auto dry = [&](hipStream_t q) {
@@ -462,14 +447,7 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
     for (size_t i = 0; i < U.size(); ++i)
       for (int64_t k = U[i].s0; k < U[i].s1; ++k) {
         if (clear[k] || (st[k].flags & PNT_STAMP_DIP)) ph[i].push_back(Ph{k, kPhClear});
-        const PStamp& p = D.host[k];
-        if (p.flags & (kStDupX | kStDupY)) {
-          if (b->use_snap) ph[i].push_back(Ph{k, kPhRing});
-          ph[i].push_back(Ph{k, kPhWindow});
-          if (p.dl_nlev > 0) ph[i].push_back(Ph{k, kPhLevels});
-        } else {
-          ph[i].push_back(Ph{k, kPhStamp});
-        }
+        ph[i].push_back(Ph{k, kPhStamp});
       }
     // the brush paint of every stamp: that of the last dip up to it
     std::vector<float> paint(6 * (size_t)n);
@@ -545,19 +523,10 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
             S.pk[c] = paint[6 * (size_t)f.k + c];
             S.ps[c] = paint[6 * (size_t)f.k + 3 + c];
           }
-          switch (f.kind) {
-            case kPhClear:
-              S.nblk = (int)std::min<int64_t>(64, (7 * S.mstride + 4 * T2 - 1) / (4 * T2));
-              break;
-            case kPhStamp: S.nblk = S.ntiles + S.rb; break;
-            case kPhRing: S.nblk = S.rb; break;
-            case kPhWindow: S.nblk = S.ntiles; break;
-            default: S.nblk = 1; break;
-          }
-          if (f.kind == kPhRing || f.kind == kPhWindow) {  // one part of a split stamp
-            if (f.kind == kPhRing) S.ntiles = 0;
-            else S.rb = 0;
-          }
+          if (f.kind == kPhClear)
+            S.nblk = (int)std::min<int64_t>(64, (7 * S.mstride + 4 * T2 - 1) / (4 * T2));
+          else  // + the shared-pixel levels' block of a dup stamp
+            S.nblk = S.ntiles + S.rb + (dup_levels(D.host[f.k]) ? 1 : 0);
           start += (std::max(S.nblk, 1) + 7) / 8 * 8;
         }
         ktimer_mark(b, s);
@@ -857,7 +826,7 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_chain(CArgs A) {
     for (int64_t i = (int64_t)off * kT * kT + t; i < A.pick_floats; i += (int64_t)r.x * kT * kT)
       A.pick[i] = 0.0f;
   } else if (role == 1) {
-    ring_scan<2>(A.f.canvas, A.f.snap, A.f.dirty, A.f.fp, g, st.box, pull,
+    ring_scan<kRingGroups>(A.f.canvas, A.f.snap, A.f.dirty, A.f.fp, g, st.box, pull,
                  (off - r.x) * kT * kT + t, r.y * kT * kT);
   } else if (role == 2) {
     const VisitTarget T{&A.f.canvas, &A.f.snap, A.f.dirty, A.f.use_snap, A.f.pickup_rate,
@@ -1169,13 +1138,7 @@ void run_fused_noalloc(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, in
     const PStamp& p = h_st[(size_t)k];
     const int rb = a.use_snap ? ring_blocks_for(p.box, 256) : 0;
     const int jk = (int)k;
-    if (!(p.flags & (kStDupX | kStDupY))) {
-      k_stamp_fused<16><<<a.ntiles + rb, 256, 0, s>>>(a, jk, a.ntiles, rb);
-    } else {
-      if (rb) k_stamp_fused<16><<<rb, 256, 0, s>>>(a, jk, 0, rb);
-      k_stamp_fused<16><<<a.ntiles, 256, 0, s>>>(a, jk, a.ntiles, 0);
-      if (p.dl_nlev) k_stamp_levels<<<1, 256, 0, s>>>(a, jk);
-    }
+    k_stamp_fused<16><<<a.ntiles + rb + (dup_levels(p) ? 1 : 0), 256, 0, s>>>(a, jk, rb);
     PNT_HIP_CHECK(hipGetLastError());
     if (st[k].flags & PNT_STAMP_DRY_AFTER) {
       if (b->use_snap)

@@ -90,6 +90,11 @@ struct StampArgs {
   DryClock clk;
 };
 
+// 4-pixel groups a ring-scan thread takes per pass (ring_blocks_for).  More
+// groups per thread means fewer ring blocks per stamp but longer ones, and
+// a stamp launch lasts as long as its slowest block: two measured fastest.
+constexpr int kRingGroups = 2;
+
 // Snapshot ring of updateSnapshot(canvas, center), FootprintBrush.hxx:286-305.
 struct RingBox {
   int a_x0, a_y0, a_x1, a_y1;  // allowed box (inclusive, clipped to the canvas)

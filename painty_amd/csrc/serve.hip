@@ -40,6 +40,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -124,7 +125,7 @@ struct SPhase {
   int k;      // stamp index (stamp kinds)
   int start;  // first block
   int nblk;
-  int rb;     // ring-scan blocks (stamp and ring kinds)
+  int rb;     // ring-scan blocks (stamp kinds); part | nparts << 16 (streaming kinds)
 };
 
 struct SArgs {
@@ -137,7 +138,7 @@ static_assert(sizeof(SArgs) <= 4000, "wave arguments must fit the kernel argumen
 
 // One wave of the server: its phases arrive as kernel arguments, each
 // request's shared fields from its device descriptor.
-__global__ __launch_bounds__(kWT2) void k_serve_wave(SArgs A) {
+__global__ __launch_bounds__(kWT2, kWaveOccupancy) void k_serve_wave(SArgs A) {
   __shared__ int sm[kWT + 2][kWT + 2];
   int lo = 0, hi = A.nphases - 1;  // the last phase with start <= blockIdx.x
   while (lo < hi) {
@@ -171,14 +172,19 @@ __global__ __launch_bounds__(kWT2) void k_serve_wave(SArgs A) {
   S.kind = P.kind;
   S.start = P.start;
   S.nblk = P.nblk;
-  S.ntiles = P.kind == kSRing ? 0 : Q.ntiles;
-  S.rb = P.rb;
+  S.ntiles = Q.ntiles;
+  // a streaming phase carries its part in rb: part | nparts << 16
+  const bool streaming =
+      P.kind == kSInit || P.kind == kSSnap || P.kind == kSDry || P.kind == kSCompose;
+  S.rb = streaming ? 0 : P.rb;
+  S.part = streaming ? (P.rb & 0xffff) : 0;
+  S.nparts = streaming ? (P.rb >> 16) : 1;
   S.hr = Q.hr;
   S.size_map = Q.size_map;
   S.tiles_x = Q.tiles_x;
 #pragma unroll
   for (int c = 0; c < 4; ++c) S.fb[c] = Q.fb[c];
-  const bool stamp_kind = P.kind >= kSStamp && P.kind <= kSLevels;
+  const bool stamp_kind = P.kind == kSStamp;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     S.pk[c] = stamp_kind ? Q.paint[6 * (int64_t)P.k + c] : 0.0f;
@@ -195,7 +201,12 @@ __global__ __launch_bounds__(kWT2) void k_serve_wave(SArgs A) {
   run_wave_slot(S, b, sm);
 }
 
-int stream_blocks(int64_t items) { return wave_stream_blocks(items); }
+// Blocks of one part of a streaming phase: at most 512, each thread taking
+// several items, so that a part leaves room in its wave.
+int part_blocks(int64_t items) {
+  const int64_t b = (items + kT2 - 1) / kT2;
+  return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 512);
+}
 
 int64_t canvas_floats(int rows, int cols) {
   return (kCanvasStrides + 3) * round_up((int64_t)rows * cols, 64);  // canvas + composite planes
@@ -242,6 +253,7 @@ struct Req {
   Resident* res = nullptr;
   bool fresh = false;  // a new canvas: initialised by the prologue
   int stage = kPrologue;
+  int part = 0;           // next part of the current streaming phase
   size_t g0 = 0, g1 = 0;  // units of the current drying phase
   std::vector<size_t> next;
   std::vector<char> udone;
@@ -295,6 +307,11 @@ struct pnt_server_s {
   int device = 0;
   int64_t budget = 0;  // bytes
   int64_t held = 0;    // bytes of canvas blocks held: resident + pooled + in transit
+  // A streaming phase (canvas initialisation, snapshot copy, dryCanvas,
+  // composite) is cut into parts of about part_bytes of traffic, one part
+  // per wave, so that no wave -- and no other request's next stamp -- waits
+  // on the whole of a large canvas.
+  int64_t part_bytes = 64ll << 20;
   std::multimap<int64_t, float*> pool;  // idle blocks by capacity (floats)
   std::mutex mu;
   std::condition_variable cv_engine, cv_done, cv_plan, cv_copy;
@@ -366,13 +383,7 @@ void pnt_server_s::plan(Req* q) {
         q->ph[i].push_back(Phase{k, kSClear});
       const PStamp& p = q->st[(size_t)k];
       q->rb[(size_t)k] = snap ? ring_blocks_for(p.box, kT2) : 0;
-      if (p.flags & (kStDupX | kStDupY)) {
-        if (snap) q->ph[i].push_back(Phase{k, kSRing});
-        q->ph[i].push_back(Phase{k, kSWindow});
-        if (p.dl_nlev > 0) q->ph[i].push_back(Phase{k, kSLevels});
-      } else {
-        q->ph[i].push_back(Phase{k, kSStamp});
-      }
+      q->ph[i].push_back(Phase{k, kSStamp});
     }
   q->prev_on_chain.assign(q->U.size(), -1);
   {
@@ -713,17 +724,24 @@ bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
     break;
   }
   if (A.nphases == kPhases) return false;
+  // part q->part of a streaming phase of `bytes` traffic over `items` per
+  // part's range; `done` marks its last part
+  auto streaming = [&](int kind, int64_t bytes, int64_t items, int done) {
+    const int np = (int)std::min<int64_t>(std::max<int64_t>((bytes + part_bytes - 1) / part_bytes, 1),
+                                          4096);
+    SPhase& P = base(kind);
+    P.rb = q->part | (np << 16);
+    P.nblk = part_blocks(items / np);
+    push(P, q->part + 1 == np ? done : -4);
+  };
+  const int64_t cs4 = R.stride / 4;  // float4s per plane
   switch (q->stage) {
     case kPrologue: {
-      if (q->fresh) {
-        SPhase& P = base(kSInit);
-        P.nblk = stream_blocks(kCanvasStrides * R.stride / 4);
-        push(P, -1);
-        return true;
-      }
-      SPhase& P = base(kSSnap);
-      P.nblk = stream_blocks(kLayerStrides * R.stride / 4);
-      push(P, -1);
+      if (q->fresh)
+        streaming(kSInit, (int64_t)(kCanvasStrides + (snap ? kLayerStrides : 0)) * R.stride * 4,
+                  cs4, -1);
+      else
+        streaming(kSSnap, (int64_t)2 * kLayerStrides * R.stride * 4, cs4, -1);
       return true;
     }
     case kUnits: {
@@ -741,10 +759,10 @@ bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
           case kSClear:
             P.nblk = (int)std::min<int64_t>(64, (7 * q->mstride + 4 * kT2 - 1) / (4 * kT2));
             break;
-          case kSStamp: P.rb = rb; P.nblk = q->ntiles + rb; break;
-          case kSRing: P.rb = rb; P.nblk = rb; break;
-          case kSWindow: P.rb = 0; P.nblk = q->ntiles; break;
-          default: P.nblk = 1; break;  // kSLevels
+          default:  // kSStamp, + the shared-pixel levels' block of a dup stamp
+            P.rb = rb;
+            P.nblk = q->ntiles + rb + (dup_levels(q->st[(size_t)f.k]) ? 1 : 0);
+            break;
         }
         if (P.nblk == 0) P.nblk = 1;  // an empty ring: the block returns at once
         push(P, (int)i);
@@ -752,18 +770,12 @@ bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
       }
       return any;
     }
-    case kDrying: {
-      SPhase& P = base(kSDry);
-      P.nblk = stream_blocks(n);
-      push(P, -2);
+    case kDrying:
+      streaming(kSDry, 96 * n, n, -2);
       return true;
-    }
-    case kComposite: {
-      SPhase& P = base(kSCompose);
-      P.nblk = stream_blocks(n);
-      push(P, -3);
+    case kComposite:
+      streaming(kSCompose, 56 * n, n, -3);
       return true;
-    }
     default:
       return false;
   }
@@ -776,9 +788,13 @@ void pnt_server_s::after_wave(const std::vector<std::pair<Req*, int>>& used) {
     if (u.second >= 0) {
       const size_t i = (size_t)u.second;
       if (++q->next[i] == q->ph[i].size()) q->udone[i] = 1;
+    } else if (u.second == -4) {  // a part of a streaming phase issued
+      q->part++;
     } else if (u.second == -1) {  // prologue issued
+      q->part = 0;
       q->stage = q->U.empty() ? kComposite : kUnits;
     } else if (u.second == -2) {  // dry issued: the next drying phase
+      q->part = 0;
       q->g0 = q->g1 + 1;
       if (q->g0 >= q->U.size()) {
         q->stage = kComposite;
@@ -788,6 +804,7 @@ void pnt_server_s::after_wave(const std::vector<std::pair<Req*, int>>& used) {
         q->stage = kUnits;
       }
     } else if (u.second == -3) {  // composite issued: the request's last phase
+      q->part = 0;
       q->stage = kIssued;
       PNT_HIP_CHECK(hipEventRecord(slots[(size_t)q->slot].fin, ctx->stream));
     }
@@ -1005,6 +1022,8 @@ int pnt_server_create(int device, int n_slots, int64_t hbm_budget_bytes, pnt_ser
     auto* S = new pnt_server_s;
     S->device = device;
     S->budget = hbm_budget_bytes;
+    if (const char* e = std::getenv("PAINTY_SERVE_PART_MB"))
+      S->part_bytes = std::max<int64_t>(1, std::atoll(e)) << 20;
     try {
       for (pnt_ctx* c : {&S->ctx, &S->qctx, &S->cctx}) {
         const int st = pnt_ctx_create(device, c);

@@ -207,39 +207,49 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles) {
 }
 
 // ----------------------------------------------------------- ring helpers -
+// Does stamp p straddle canvas column/row 0 with shared-pixel visits (which
+// then run by level on a block of their own)?
+__host__ __device__ inline bool dup_levels(const PStamp& p) {
+  return (p.flags & (kStDupX | kStDupY)) && p.dl_nlev > 0;
+}
+
 __device__ __forceinline__ bool on_dup_line(const PStamp& st, int col, int row) {
   return ((st.flags & kStDupX) && (int)(col + st.cx) == 0) ||
          ((st.flags & kStDupY) && (int)(row + st.cy) == 0);
 }
 
-// The window offset k in [-hr, hr] with (int)(k + c) == x, when exactly one
-// exists (no duplicate on this axis); false otherwise.
-__device__ __forceinline__ bool window_offset(int x, double c, int hr, int* k) {
+// Every window offset k in [-hr, hr] with (int)(k + c) == x, ascending: one
+// for most pixels, two on a duplicated canvas column/row 0, none off the
+// window.  Returns how many.
+__device__ __forceinline__ int window_offsets(int x, double c, int hr, int k[2]) {
 #pragma clang fp contract(off)
   const int k0 = x - (int)floor(c);
+  int n = 0;
   for (int d = -1; d <= 1; ++d) {
     const int kk = k0 + d;
-    if (kk >= -hr && kk <= hr && (int)(kk + c) == x) {
-      *k = kk;
-      return true;
-    }
+    if (kk >= -hr && kk <= hr && (int)(kk + c) == x && n < 2) k[n++] = kk;
   }
-  return false;
+  return n;
 }
 
 // Is canvas pixel (x, y), on the ring of this stamp, also visited with a
 // footprint height > 0 by it (and therefore refreshed by that visit)?  With
 // a drying clock every visited pixel counts: its window tile refreshes it
-// before checkDry changes the canvas.
+// before checkDry changes the canvas.  A pixel of a duplicated column/row 0
+// counts when any of its visits does (the first of them refreshes it).
 __device__ __forceinline__ bool ring_pixel_visited(const StampGeom& g, const RingBox& b,
                                                    const float* fp, int x, int y,
                                                    bool any_visit = false) {
   if (x < b.tl_x || x > b.br_x || y < b.tl_y || y > b.br_y) return false;
-  int col, row;
-  if (!window_offset(x, g.cx, g.hr, &col) || !window_offset(y, g.cy, g.hr, &row)) return false;
-  int64_t ic;
-  const int m = stamp_visit(g, col, row, &ic);
-  return m >= 0 && (any_visit || fp[m] > 0.0f);
+  int cs[2], rs[2];
+  const int nc = window_offsets(x, g.cx, g.hr, cs), nr = window_offsets(y, g.cy, g.hr, rs);
+  for (int r = 0; r < nr; ++r)
+    for (int c = 0; c < nc; ++c) {
+      int64_t ic;
+      const int m = stamp_visit(g, cs[c], rs[r], &ic);
+      if (m >= 0 && (any_visit || fp[m] > 0.0f)) return true;
+    }
+  return false;
 }
 
 __device__ __forceinline__ bool on_window_border(const RingBox& b, int x, int y) {
@@ -253,7 +263,8 @@ __device__ __forceinline__ bool on_window_border(const RingBox& b, int x, int y)
 // for the set pixels outside the window interior copies canvas -> snapshot
 // and clears the byte.  With `pull`, window-border pixels that a visit of
 // this stamp refreshes itself are left to that visit.  Each thread takes
-// kG groups per pass with all their loads in flight together (the dirty
+// kG groups per pass (so that a stamp's ring needs few blocks) with all
+// their dirty-word loads in flight together (the dirty
 // strip behind a moving window is contiguous, so a pixel-serial copy would
 // wait on one round trip per pixel).  Plane and dirty buffers are padded
 // past the last pixel to a multiple of 16, so a group never leaves them.
@@ -288,7 +299,7 @@ __device__ __forceinline__ void ring_scan(const Layer7& canvas, const Layer7& sn
       if (interior_row && x0 > b.tl_x && x0 + 3 < b.br_x) continue;  // inside the window interior
       const unsigned d = *reinterpret_cast<const unsigned*>(dirty + bs);
       if (!d) continue;
-      unsigned m = 0u;
+      unsigned m = 0u, border = 0u;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (!((d >> (8 * q)) & 0xffu)) continue;
@@ -296,39 +307,53 @@ __device__ __forceinline__ void ring_scan(const Layer7& canvas, const Layer7& sn
         if (i < lo || i >= lo + span) continue;
         const int col = x0 + q;
         if (interior_row && col > b.tl_x && col < b.br_x) continue;
-        if (pull && ring_pixel_visited(g, b, fp, col, row, clocked)) continue;
         m |= 1u << q;
+        if (pull && on_window_border(b, col, row)) border |= 1u << q;
       }
+      // window-border pixels a visit refreshes itself (few: one loop body)
+#pragma unroll 1
+      for (int q = 0; q < 4; ++q)
+        if (((border >> q) & 1u) && ring_pixel_visited(g, b, fp, x0 + q, row, clocked))
+          m &= ~(1u << q);
       mask[u] = m;
       base[u] = bs;
     }
-    float4 v[kG][6];
-    double2 w[kG][2];  // the four volumes
+    // the copies (few: the strip the last stamps left dirty), two groups at
+    // a time with their loads in flight together
+    constexpr int kC = kG < 2 ? kG : 2;
 #pragma unroll
-    for (int u = 0; u < kG; ++u) {
-      if (!mask[u]) continue;
+    for (int u0 = 0; u0 < kG; u0 += kC) {
+      float4 v[kC][6];
+      double2 w[kC][2];  // the four volumes
 #pragma unroll
-      for (int c = 0; c < 6; ++c) v[u][c] = reinterpret_cast<const float4*>(canvas.ks[c])[base[u] >> 2];
-      w[u][0] = reinterpret_cast<const double2*>(canvas.v)[base[u] >> 1];
-      w[u][1] = reinterpret_cast<const double2*>(canvas.v)[(base[u] >> 1) + 1];
-    }
+      for (int h = 0; h < kC; ++h) {
+        const int u = u0 + h;
+        if (!mask[u]) continue;
 #pragma unroll
-    for (int u = 0; u < kG; ++u) {
-      if (!mask[u]) continue;
-      if (mask[u] == 0xfu) {
+        for (int c = 0; c < 6; ++c) v[h][c] = reinterpret_cast<const float4*>(canvas.ks[c])[base[u] >> 2];
+        w[h][0] = reinterpret_cast<const double2*>(canvas.v)[base[u] >> 1];
+        w[h][1] = reinterpret_cast<const double2*>(canvas.v)[(base[u] >> 1) + 1];
+      }
 #pragma unroll
-        for (int c = 0; c < 6; ++c) reinterpret_cast<float4*>(snap.ks[c])[base[u] >> 2] = v[u][c];
-        reinterpret_cast<double2*>(snap.v)[base[u] >> 1] = w[u][0];
-        reinterpret_cast<double2*>(snap.v)[(base[u] >> 1) + 1] = w[u][1];
-        *reinterpret_cast<unsigned*>(dirty + base[u]) = 0u;
-      } else {
+      for (int h = 0; h < kC; ++h) {
+        const int u = u0 + h;
+        if (!mask[u]) continue;
+        if (mask[u] == 0xfu) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (!((mask[u] >> q) & 1u)) continue;
+          for (int c = 0; c < 6; ++c) reinterpret_cast<float4*>(snap.ks[c])[base[u] >> 2] = v[h][c];
+          reinterpret_cast<double2*>(snap.v)[base[u] >> 1] = w[h][0];
+          reinterpret_cast<double2*>(snap.v)[(base[u] >> 1) + 1] = w[h][1];
+          *reinterpret_cast<unsigned*>(dirty + base[u]) = 0u;
+        } else {
 #pragma unroll
-          for (int c = 0; c < 6; ++c) snap.ks[c][base[u] + q] = reinterpret_cast<const float*>(&v[u][c])[q];
-          snap.v[base[u] + q] = reinterpret_cast<const double*>(&w[u][0])[q];
-          dirty[base[u] + q] = 0;
+          for (int q = 0; q < 4; ++q) {
+            if (!((mask[u] >> q) & 1u)) continue;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+              snap.ks[c][base[u] + q] = reinterpret_cast<const float*>(&v[h][c])[q];
+            snap.v[base[u] + q] = reinterpret_cast<const double*>(&w[h][0])[q];
+            dirty[base[u] + q] = 0;
+          }
         }
       }
     }
@@ -429,6 +454,7 @@ __device__ __forceinline__ void check_only(const VisitTarget& T, const DryClock&
 // before its pickup and deposit (FootprintBrush.hxx:115), by the thread that
 // runs its visit: map cells are then taken without the footprint-box cut,
 // so the pairs are those of the plain kernel.
+
 template <int kT>
 __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* pick,
                                             const float* fp, const FpBox& fb, const StampGeom& g,
@@ -532,17 +558,44 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
   __syncthreads();  // sm is rewritten by the block's next tile
 }
 
+// Is visit (col, row), at canvas pixel ic on the window border of box b,
+// the first visit of that pixel in reference order (rows, then columns)
+// that acts: footprint height > 0, or any visit when `any_visit` (a drying
+// clock)?  Only a pixel of a duplicated column/row 0 has more than one.
+__device__ __forceinline__ bool first_acting_visit(const StampGeom& g, const RingBox& b,
+                                                   const float* fp, int col, int row, int64_t ic,
+                                                   bool any_visit) {
+  const int x = (int)(ic % g.cols), y = (int)(ic / g.cols);
+  if (!on_window_border(b, x, y)) return false;
+  int cs[2], rs[2];
+  const int nc = window_offsets(x, g.cx, g.hr, cs), nr = window_offsets(y, g.cy, g.hr, rs);
+  for (int r = 0; r < nr; ++r)
+    for (int c = 0; c < nc; ++c) {
+      int64_t i2;
+      const int m = stamp_visit(g, cs[c], rs[r], &i2);
+      if (m >= 0 && (any_visit || fp[m] > 0.0f)) return cs[c] == col && rs[r] == row;
+    }
+  return false;
+}
+
 // Visits sharing a canvas pixel on column/row 0 (and their map-cell
 // partners), listed in reference order and grouped by dependency level; the
 // threads of one workgroup, levels separated by barriers.  With a clock,
 // each listed visit first checks its pixel (the second check of a pixel in
 // one imprint finds its time already at the imprint's and changes nothing).
+//
+// With `pull` (the stamp's window tiles and ring scan run in the same
+// launch), a listed visit on the window border refreshes its own snapshot
+// pixel when it is the first visit of that pixel that acts (footprint
+// height > 0, or any visit with a clock) in reference order; the ring scan
+// leaves such pixels alone.
 template <int kThreads>
 __device__ __forceinline__ void level_visits(const VisitTarget& T, float* const* pick,
                                              const float* fp, const StampGeom& g, const PStamp& st,
                                              const int2* dl, const int* dl_off, const float pk[3],
                                              const float ps[3],
-                                             const DryClock& clk = DryClock{}) {
+                                             const DryClock& clk = DryClock{},
+                                             bool pull = false) {
   for (int L = 0; L < st.dl_nlev; ++L) {
     const int e0 = dl_off[st.dl_lev + L], e1 = dl_off[st.dl_lev + L + 1];
     for (int e = e0 + (int)threadIdx.x; e < e1; e += kThreads) {
@@ -550,8 +603,10 @@ __device__ __forceinline__ void level_visits(const VisitTarget& T, float* const*
       int64_t ic;
       const int m = stamp_visit(g, v.x, v.y, &ic);
       if (m < 0) continue;
-      if (clk.tmap) check_only(T, clk, ic, false);
-      if (fp[m] > 0.0f) cached_visit(T, GlobalPick{pick, m}, fp[m], ic, false, pk, ps);
+      const bool pv = pull && first_acting_visit(g, st.box, fp, v.x, v.y, ic, clk.tmap != nullptr);
+      if (clk.tmap) check_only(T, clk, ic, pv);  // with pv, the refresh comes first
+      if (fp[m] > 0.0f)
+        cached_visit(T, GlobalPick{pick, m}, fp[m], ic, pv && !clk.tmap, pk, ps);
     }
     __builtin_amdgcn_s_waitcnt(0);  // this level's stores before the next level's loads
     __syncthreads();

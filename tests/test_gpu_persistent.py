@@ -226,6 +226,40 @@ def test_imprint_strokes_equals_stroke_by_stroke(hip, oracle, chains, sched, mon
     compare_canvas("strokes%d" % chains, b, ocv, pickup=(bb, ob))
 
 
+@pytest.mark.parametrize("sched", ["waves", "streams"])
+@pytest.mark.parametrize("snapshot", [True, False])
+def test_edge_strokes_equal_plain_bitwise(hip, oracle, sched, snapshot, monkeypatch):
+    """Strokes along the top and left edges: most stamps straddle canvas
+    column/row 0 (pixels visited twice, some on the window border), and the
+    fused schedulers run each such stamp in one launch (window tiles, ring
+    scan and the shared-pixel levels side by side).  The result equals the
+    plain per-stamp path, which splits them, to the bit, and the oracle."""
+    wl = W.small(72, 90, 0, 11, 5, pickup=True, noisy=True)
+    st = np.zeros(60, dtype=W.STAMP_DTYPE)
+    t = np.arange(60, dtype=np.float64)
+    st["cx"][:30] = 2.3 + 2.1 * t[:30]          # along row 0, centres above and below it
+    st["cy"][:30] = 0.35 + 3.0 * np.sin(t[:30] / 3.0)
+    # down column 0; with the centre in (10, 11) the window's left border is column 0 itself
+    st["cx"][30:] = 10.45 + 0.5 * np.cos(t[30:] / 4.0)
+    st["cy"][30:] = 1.4 + 1.9 * (t[30:] - 30)
+    st["theta"] = 0.37 * t
+    st["K"], st["S"] = PA
+    st["flags"][0] = st["flags"][30] = W.DIP
+    st["flags"][44] |= W.DRY_AFTER
+    monkeypatch.setenv("PAINTY_FUSED_SCHED", sched)
+    monkeypatch.setenv("PAINTY_FUSED_CHAINS", "4")
+    a, ab = run_gpu(wl, st, use_snapshot=snapshot, mode=PLAIN)
+    b, bb = run_gpu(wl, st, use_snapshot=snapshot, mode=FUSED)
+    _same(a, ab, b, bb)
+    c, cb = _fresh(hip, wl)
+    cb.setUseSnapshotBuffer(snapshot)
+    cb.imprintStrokes([(wl.radii[0], wl.cores[0], st[:30]), (wl.radii[0], wl.cores[0], st[30:])],
+                      c, FUSED)
+    _same(a, ab, c, cb)
+    ocv, ob = run_oracle(wl, st, use_snapshot=snapshot)
+    compare_canvas("edges/%s/snap%d" % (sched, snapshot), c, ocv, pickup=(cb, ob))
+
+
 @pytest.mark.parametrize("mode", [PLAIN, FUSED])
 def test_imprint_strokes_geometry_rules(hip, oracle, mode):
     """Radius changes within 0.5 keep the footprint and the pickup map; a
