@@ -161,6 +161,8 @@ struct JobPlan {
   size_t g0 = 0, g1 = 0;
   std::vector<size_t> next;
   std::vector<char> udone;
+  std::vector<int64_t> sdone;  // stamps of each unit issued
+  StampNeeds needs;
 };
 
 void plan_job(const pnt_batch_job& job, int chunk, JobPlan& P) {
@@ -209,6 +211,8 @@ void plan_job(const pnt_batch_job& job, int chunk, JobPlan& P) {
   }
   P.next.assign(P.U.size(), 0);
   P.udone.assign(P.U.size(), 0);
+  P.sdone.assign(P.U.size(), 0);
+  P.needs = plan_stamp_needs(P.U, P.st.data());
   for (size_t i = 0; i < P.U.size(); ++i)
     if (P.ph[i].empty()) P.udone[i] = 1;
   P.g0 = P.g1 = 0;
@@ -324,9 +328,13 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
           for (size_t i = J.g0; i <= J.g1; ++i) {
             if (J.udone[i]) continue;
             bool ok = J.prev_on_chain[i] < 0 || J.udone[(size_t)J.prev_on_chain[i]];
-            for (int d : J.U[i].deps) ok = ok && J.udone[(size_t)d];
-            if (!ok) continue;
             const Phase& f = J.ph[i][J.next[i]];
+            if (ok && f.kind == kSStamp)  // the stamps it follows have run
+              for (int e = J.needs.off[(size_t)f.k]; e < J.needs.off[(size_t)f.k + 1] && ok; ++e) {
+                const int2 nd = J.needs.need[(size_t)e];
+                ok = J.udone[(size_t)nd.x] || J.sdone[(size_t)nd.x] >= nd.y;
+              }
+            if (!ok) continue;
             const int rb = J.rb[(size_t)f.k];
             const int ntl = ((2 * ((b->size_map - 1) / 2) + 1 + kWT - 1) / kWT);
             const int ntiles = ntl * ntl;
@@ -364,6 +372,7 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
       JobPlan& J = P[(size_t)t.first];
       if (t.second >= 0) {
         const size_t i = (size_t)t.second;
+        if (J.ph[i][J.next[i]].kind == kSStamp) J.sdone[i] = J.ph[i][J.next[i]].k - J.U[i].s0 + 1;
         if (++J.next[i] == J.ph[i].size()) J.udone[i] = 1;
       } else if (t.second == -1) {
         J.g0 = J.g1 + 1;

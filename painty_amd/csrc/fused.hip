@@ -238,6 +238,35 @@ std::vector<Unit> plan_units(const pnt_stamp* st, const PStamp* h, const uint8_t
   return U;
 }
 
+// For every stamp k of a unit, the units it depends on and how many of
+// their leading stamps must have run first: one past the last of that
+// unit's stamps whose allowed box meets stamp k's (none meeting: no entry).
+// A stamp touches nothing outside its allowed box, so a unit may start
+// while the units it follows still run, as long as every pair of stamps
+// whose boxes meet keeps the list's order.
+StampNeeds plan_stamp_needs(const std::vector<Unit>& U, const PStamp* h) {
+  StampNeeds N;
+  const int64_t n = U.empty() ? 0 : U.back().s1;
+  N.off.assign((size_t)n + 1, 0);
+  auto meet = [&](int64_t a, int64_t b) {
+    const RingBox &x = h[a].box, &y = h[b].box;
+    if (x.a_x1 < x.a_x0 || y.a_x1 < y.a_x0) return false;
+    return !(x.a_x1 < y.a_x0 || y.a_x1 < x.a_x0 || x.a_y1 < y.a_y0 || y.a_y1 < x.a_y0);
+  };
+  for (size_t i = 0; i < U.size(); ++i)
+    for (int64_t k = U[i].s0; k < U[i].s1; ++k) {
+      N.off[(size_t)k] = (int)N.need.size();
+      for (int d : U[i].deps) {
+        int64_t last = -1;
+        for (int64_t q = U[(size_t)d].s1 - 1; q >= U[(size_t)d].s0 && last < 0; --q)
+          if (meet(q, k)) last = q;
+        if (last >= 0) N.need.push_back(make_int2(d, (int)(last - U[(size_t)d].s0 + 1)));
+      }
+    }
+  N.off[(size_t)n] = (int)N.need.size();
+  return N;
+}
+
 namespace {
 
 // Concurrent units run as waves on one stream (the default), or on
@@ -477,6 +506,8 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
     W.clk = A0.clk;
     std::vector<size_t> next(U.size(), 0);
     std::vector<char> done(U.size(), 0);
+    std::vector<int64_t> sdone(U.size(), 0);  // stamps of each unit issued
+    const StampNeeds NS = plan_stamp_needs(U, D.host);
     std::vector<int> prev_on_chain(U.size(), -1);
     {
       std::vector<int> last(used, -1);
@@ -495,7 +526,12 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
         for (size_t i = g0; i <= g1 && (int)ready.size() < kWaveSlots; ++i) {
           if (done[i]) continue;
           bool ok = prev_on_chain[i] < 0 || done[(size_t)prev_on_chain[i]];
-          for (int d : U[i].deps) ok = ok && done[(size_t)d];
+          const Ph& f = ph[i][next[i]];
+          if (ok && f.kind == kPhStamp)  // the stamps it follows have run
+            for (int j = NS.off[(size_t)f.k]; j < NS.off[(size_t)f.k + 1] && ok; ++j) {
+              const int2 nd = NS.need[(size_t)j];
+              ok = done[(size_t)nd.x] || sdone[(size_t)nd.x] >= nd.y;
+            }
           if (ok) ready.push_back((int)i);
         }
         if (ready.empty()) break;  // the phase is done
@@ -534,8 +570,11 @@ void run_fused_list(pnt_brush_s* b, pnt_canvas_s* cv, const pnt_stamp* st, int64
         PNT_HIP_CHECK(hipGetLastError());
         ktimer_mark(b, s);
         b->last_launches += 1;
-        for (int i : ready)
+        for (int i : ready) {
+          const Ph& f = ph[(size_t)i][next[(size_t)i]];
+          if (f.kind == kPhStamp) sdone[(size_t)i] = f.k - U[(size_t)i].s0 + 1;
           if (++next[(size_t)i] == ph[(size_t)i].size()) done[(size_t)i] = 1;
+        }
       }
       if (U[g1].dry_after) dry(s);
       g0 = g1 + 1;

@@ -59,6 +59,13 @@ struct Unit {
 // before stamp k.
 std::vector<Unit> plan_units(const pnt_stamp* st, const PStamp* h, const uint8_t* clear,
                              int64_t n, int nchains);
+// Per stamp, the units it depends on and how many of their leading stamps
+// must have run first (need[off[k] .. off[k+1]) = (unit, count)).
+struct StampNeeds {
+  std::vector<int> off;
+  std::vector<int2> need;
+};
+StampNeeds plan_stamp_needs(const std::vector<Unit>& U, const PStamp* h);
 // Blocks of `threads` threads that scan the snapshot ring of box b.
 int ring_blocks_for(const RingBox& b, int threads);
 // Plans stamps [0, n) for a brush of footprint side size_map and radius on a

@@ -257,6 +257,8 @@ struct Req {
   size_t g0 = 0, g1 = 0;  // units of the current drying phase
   std::vector<size_t> next;
   std::vector<char> udone;
+  std::vector<int64_t> sdone;  // stamps of each unit issued
+  StampNeeds needs;
   const PStamp* d_st = nullptr;
   const float* d_paint = nullptr;
   const int2* d_dl = nullptr;
@@ -411,6 +413,8 @@ void pnt_server_s::plan(Req* q) {
                   al((int64_t)(q->paint.size() * sizeof(float))) + al((int64_t)sizeof(SReq));
   q->next.assign(q->U.size(), 0);
   q->udone.assign(q->U.size(), 0);
+  q->sdone.assign(q->U.size(), 0);
+  q->needs = plan_stamp_needs(q->U, q->st.data());
   for (size_t i = 0; i < q->U.size(); ++i)
     if (q->ph[i].empty()) q->udone[i] = 1;
   q->g0 = q->g1 = 0;  // the first drying phase: units [0, g1]
@@ -748,9 +752,13 @@ bool pnt_server_s::collect(Req* q, SArgs& A, int& start,
       for (size_t i = q->g0; i <= q->g1 && A.nphases < kPhases; ++i) {
         if (q->udone[i]) continue;
         bool ok = q->prev_on_chain[i] < 0 || q->udone[(size_t)q->prev_on_chain[i]];
-        for (int d : q->U[i].deps) ok = ok && q->udone[(size_t)d];
-        if (!ok) continue;
         const Phase& f = q->ph[i][q->next[i]];
+        if (ok && f.kind == kSStamp)  // the stamps it follows have run
+          for (int e = q->needs.off[(size_t)f.k]; e < q->needs.off[(size_t)f.k + 1] && ok; ++e) {
+            const int2 nd = q->needs.need[(size_t)e];
+            ok = q->udone[(size_t)nd.x] || q->sdone[(size_t)nd.x] >= nd.y;
+          }
+        if (!ok) continue;
         SPhase& P = base(f.kind);
         P.k = (int)f.k;
         P.chain = (int8_t)q->U[i].chain;
@@ -787,6 +795,7 @@ void pnt_server_s::after_wave(const std::vector<std::pair<Req*, int>>& used) {
     Req* q = u.first;
     if (u.second >= 0) {
       const size_t i = (size_t)u.second;
+      if (q->ph[i][q->next[i]].kind == kSStamp) q->sdone[i] = q->ph[i][q->next[i]].k - q->U[i].s0 + 1;
       if (++q->next[i] == q->ph[i].size()) q->udone[i] = 1;
     } else if (u.second == -4) {  // a part of a streaming phase issued
       q->part++;
