@@ -329,8 +329,10 @@ struct WArgs {
 static_assert(sizeof(WArgs) <= 4000, "wave arguments must fit the kernel argument segment");
 
 
+// 16x16 tiles: five workgroups per CU (at most 102 VGPRs; it needs 102
+// without the bound and spills nothing under it).
 template <int kT>
-__global__ __launch_bounds__(kT* kT) void k_stamp_wave(WArgs A) {
+__global__ __launch_bounds__(kT* kT, kT == 16 ? 5 : 1) void k_stamp_wave(WArgs A) {
   __shared__ int sm[kT + 2][kT + 2];
   int si = 0;
   while (si + 1 < A.nslots && (int)blockIdx.x >= A.slot[si + 1].start) ++si;

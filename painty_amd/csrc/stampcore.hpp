@@ -318,9 +318,10 @@ __device__ __forceinline__ void ring_scan(const Layer7& canvas, const Layer7& sn
       mask[u] = m;
       base[u] = bs;
     }
-    // the copies (few: the strip the last stamps left dirty), two groups at
-    // a time with their loads in flight together
-    constexpr int kC = kG < 2 ? kG : 2;
+    // the copies (few: the strip the last stamps left dirty), one group at a
+    // time: two groups' planes in registers at once set the register peak
+    // of every kernel that runs ring scans
+    constexpr int kC = 1;
 #pragma unroll
     for (int u0 = 0; u0 < kG; u0 += kC) {
       float4 v[kC][6];
@@ -496,8 +497,9 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
     if (!(dup && (on_dup_line(st, col, row) ||
                   (pdir && on_dup_line(st, col + (pdir == 1), row + (pdir == 2)))))) {
       act = true;
-      ic = canvas_index(g, col, row);
-      pv = pull && on_window_border(st.box, (int)(ic % g.cols), (int)(ic / g.cols));
+      const int x = (int)(col + g.cx), y = (int)(row + g.cy);  // canvas_index's pixel
+      ic = (int64_t)y * g.cols + x;
+      pv = pull && on_window_border(st.box, x, y);
       // the footprint height and every value the visits read go out in one
       // batch (a visit with fh <= 0 then uses none of them)
       fh = fp[mm];
@@ -506,8 +508,9 @@ __device__ __forceinline__ void window_tile(const VisitTarget& T, float* const* 
       acc.alias = !T.use_snap;
       load_pixel(T, ic, pv, acc);
       if (pdir) {  // the second visitor of the cell, right or below
-        pic = canvas_index(g, col + (pdir == 1), row + (pdir == 2));
-        ppull = pull && on_window_border(st.box, (int)(pic % g.cols), (int)(pic / g.cols));
+        const int px = (int)(col + (pdir == 1) + g.cx), py = (int)(row + (pdir == 2) + g.cy);
+        pic = (int64_t)py * g.cols + px;
+        ppull = pull && on_window_border(st.box, px, py);
         if constexpr (kPre) {
           ld_px(*T.canvas, pic, pcv, pcvv);
           if (T.use_snap && !ppull) {

@@ -25,8 +25,7 @@ namespace pnt {
 constexpr int kWT = 16;          // window tile side of wave phases
 constexpr int kWT2 = kWT * kWT;  // threads per block
 // Waves per SIMD the wave kernels keep (at most 128 VGPRs): four workgroups
-// of a launch per CU.  With its shared-pixel levels inlined, the kernel
-// would otherwise take 136-138 VGPRs and only three.
+// of a launch per CU.  They need 108-112; five (at most 102) spills.
 constexpr int kWaveOccupancy = 4;
 
 enum : int {
