@@ -582,6 +582,31 @@ void ctx_release(pnt_ctx ctx) {
   if (ctx->kernel_error) (void)hipHostFree(ctx->kernel_error);
   delete ctx;
 }
+
+// A context whose stream has a hardware queue of its own: a stream made with
+// a CU mask (every CU of the device) gets its own queue instead of sharing
+// one of the process's pool (GPU_MAX_HW_QUEUES, 4 by default) with the
+// streams made before it.  For streams whose kernels must not queue behind
+// another stream's (the server's wave lanes).
+pnt_ctx ctx_create_own_queue(int device) {
+  auto* c = new pnt_ctx_s;
+  c->device = device;
+  try {
+    c->use();
+    int cus = 0;
+    PNT_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0u);
+    for (int i = 0; i < cus; ++i) mask[(size_t)i / 32] |= 1u << (i % 32);
+    PNT_HIP_CHECK(hipExtStreamCreateWithCUMask(&c->stream, (uint32_t)mask.size(), mask.data()));
+    PNT_HIP_CHECK(hipHostMalloc((void**)&c->kernel_error, sizeof(unsigned), hipHostMallocMapped));
+    *c->kernel_error = 0;
+  } catch (...) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    throw;
+  }
+  return c;
+}
 }  // namespace pnt
 
 extern "C" {

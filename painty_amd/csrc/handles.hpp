@@ -46,6 +46,7 @@ struct pnt_ctx_s {
 namespace pnt {
 void ctx_retain(pnt_ctx ctx);
 void ctx_release(pnt_ctx ctx);
+pnt_ctx ctx_create_own_queue(int device);  // api.hip
 }  // namespace pnt
 
 struct pnt_canvas_s;

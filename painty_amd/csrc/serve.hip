@@ -15,18 +15,22 @@
 // units -- runs of one stroke -- with the units they depend on; see
 // plan_units in fused.hip).  One engine thread admits planned requests into
 // up to n_slots request slots and drives the device with "waves": each wave
-// is ONE launch of k_serve_wave on the engine's stream, carrying the next
-// phase of every ready unit of every admitted request (up to kPhases phases:
-// a canvas initialisation, a full snapshot copy, a pickup-map clear, a stamp
-// -- window tiles and ring scan side by side -- or the split parts of a
-// stamp that straddles canvas column/row 0, a dryCanvas, or the composite),
-// oldest request first.  Requests that arrive while others run join the
-// next wave: their work overlaps on the device, in the same launches.
-// Phases in one wave touch disjoint memory (different canvases, or units of
-// one canvas with disjoint snapshot boxes and their own pickup maps), and the
-// stream orders the waves, so each request computes exactly the arithmetic
-// of running alone: the same to the bit.  At most kAhead waves are queued
-// ahead of the host, so a new request waits for few.
+// is ONE launch of k_serve_wave, carrying the next phase of every ready unit
+// of every admitted request of its lane (up to kPhases phases: a part of a
+// canvas initialisation or of a full snapshot copy, a pickup-map clear, a
+// stamp -- window tiles, ring scan and any shared-pixel levels side by side
+// -- a part of a dryCanvas, or a part of the composite), oldest request
+// first.  Requests that arrive while others run join the next wave: their
+// work overlaps on the device, in the same launches.  Two lanes, each a
+// stream of waves: the main one, and an express one for a few requests
+// whose stamps form a long dependent chain, so that such a chain advances in
+// small, short waves.  Phases in one wave touch disjoint memory (different
+// canvases, or units of one canvas with disjoint snapshot boxes and their
+// own pickup maps); a request's waves are all on its lane's stream, in
+// order, and one canvas has one request running at a time, so each request
+// computes exactly the arithmetic of running alone: the same to the bit.  At
+// most kAhead waves are queued per lane ahead of the host, so a new request
+// waits for few.
 //
 // Admission: requests on one canvas run in submission order, one at a time;
 // a request whose canvas is not resident is admitted when its canvas fits
@@ -253,6 +257,8 @@ struct Req {
   Resident* res = nullptr;
   bool fresh = false;  // a new canvas: initialised by the prologue
   int stage = kPrologue;
+  int lane = 0;              // 0: the main wave stream, 1: the express one
+  int64_t alone_waves = 0;   // waves the request's stamps take when it runs alone
   int part = 0;           // next part of the current streaming phase
   size_t g0 = 0, g1 = 0;  // units of the current drying phase
   std::vector<size_t> next;
@@ -335,9 +341,20 @@ struct pnt_server_s {
   pnt_ctx ctx = nullptr;     // engine stream
   pnt_ctx qctx = nullptr;    // views of resident canvases (pnt_server_canvas)
   pnt_ctx cctx = nullptr;    // copier stream
-  hipEvent_t wave_ev[kAhead + 1] = {};
   SReq* d_reqs = nullptr;    // one descriptor per request slot (device)
-  int64_t waves = 0;
+  // Two wave streams ("lanes"): the main one carries the ready phases of
+  // most requests; the express one those of up to express_max requests
+  // whose stamps form a long dependent chain (at least express_waves waves
+  // alone).  A long chain advances one stamp per wave it is in; in the
+  // express lane its waves are small and short, instead of each waiting on
+  // everything else in flight.  Each lane keeps at most kAhead waves queued.
+  struct Lane {
+    pnt_ctx ctx = nullptr;
+    hipEvent_t ev[kAhead + 1] = {};
+    int64_t waves = 0;
+  } lanes[2];
+  int express_max = 4, express_active = 0;
+  int64_t express_waves = 200;
   std::thread engine, copier;
   std::vector<std::thread> planners;
 
@@ -359,6 +376,8 @@ struct pnt_server_s {
 };
 
 // ------------------------------------------------------------- planning --
+static int64_t alone_waves(const Req& q);
+
 void pnt_server_s::plan(Req* q) {
   const pnt_request& r = q->r;
   q->size_map = footprint_size_map(r.radius, q->footprint.data(), r.width);
@@ -419,6 +438,46 @@ void pnt_server_s::plan(Req* q) {
     if (q->ph[i].empty()) q->udone[i] = 1;
   q->g0 = q->g1 = 0;  // the first drying phase: units [0, g1]
   while (q->g1 + 1 < q->U.size() && !q->U[q->g1].dry_after) ++q->g1;
+  q->alone_waves = alone_waves(*q);
+}
+
+// Waves the request's units take when it runs alone: every wave takes the
+// next phase of every ready unit, as the engine does, and a dry is one more.
+static int64_t alone_waves(const Req& q) {
+  const size_t nu = q.U.size();
+  std::vector<size_t> next(nu, 0);
+  std::vector<char> done(q.udone);
+  std::vector<int64_t> sdone(nu, 0);
+  std::vector<int> ready;
+  int64_t waves = 0;
+  for (size_t g0 = 0; g0 < nu;) {
+    size_t g1 = g0;
+    while (g1 + 1 < nu && !q.U[g1].dry_after) ++g1;
+    for (;;) {
+      ready.clear();
+      for (size_t i = g0; i <= g1; ++i) {
+        if (done[i]) continue;
+        bool ok = q.prev_on_chain[i] < 0 || done[(size_t)q.prev_on_chain[i]];
+        const Phase& f = q.ph[i][next[i]];
+        if (ok && f.kind == kSStamp)
+          for (int e = q.needs.off[(size_t)f.k]; e < q.needs.off[(size_t)f.k + 1] && ok; ++e) {
+            const int2 nd = q.needs.need[(size_t)e];
+            ok = done[(size_t)nd.x] || sdone[(size_t)nd.x] >= nd.y;
+          }
+        if (ok) ready.push_back((int)i);
+      }
+      if (ready.empty()) break;
+      ++waves;
+      for (int i : ready) {
+        const Phase& f = q.ph[(size_t)i][next[(size_t)i]];
+        if (f.kind == kSStamp) sdone[(size_t)i] = f.k - q.U[(size_t)i].s0 + 1;
+        if (++next[(size_t)i] == q.ph[(size_t)i].size()) done[(size_t)i] = 1;
+      }
+    }
+    if (q.U[g1].dry_after) ++waves;
+    g0 = g1 + 1;
+  }
+  return waves;
 }
 
 void pnt_server_s::planner() {
@@ -585,7 +644,7 @@ std::vector<Req*> pnt_server_s::admit() {
 void pnt_server_s::setup(Req* q) {
   SlotRes& S = slots[(size_t)q->slot];
   Resident& R = *q->res;
-  hipStream_t s = ctx->stream;
+  hipStream_t s = lanes[q->lane].ctx->stream;
   if (!R.block) R.block = dev_alloc(R.cap);
   auto grow_f = [&](float*& p, int64_t& cap, int64_t need) {
     if (need <= cap) return;
@@ -676,7 +735,7 @@ void pnt_server_s::write_descriptor(Req* q, int64_t& o) {
   D.cap = 1.0;  // FootprintBrush.hxx:483
   std::memcpy(S.stage + o, &D, sizeof D);
   PNT_HIP_CHECK(hipMemcpyAsync(d_reqs + q->slot, S.stage + o, sizeof D, hipMemcpyHostToDevice,
-                               ctx->stream));
+                               lanes[q->lane].ctx->stream));
   o += round_up((int64_t)sizeof D, 256);
 }
 
@@ -815,7 +874,7 @@ void pnt_server_s::after_wave(const std::vector<std::pair<Req*, int>>& used) {
     } else if (u.second == -3) {  // composite issued: the request's last phase
       q->part = 0;
       q->stage = kIssued;
-      PNT_HIP_CHECK(hipEventRecord(slots[(size_t)q->slot].fin, ctx->stream));
+      PNT_HIP_CHECK(hipEventRecord(slots[(size_t)q->slot].fin, lanes[q->lane].ctx->stream));
     }
   }
 }
@@ -856,6 +915,7 @@ void pnt_server_s::release_if_closing(int64_t id) {
 // The device finished a request (engine thread): its slot is free; the
 // composite goes to the caller through the copier, or the request is done.
 void pnt_server_s::complete(Req* q) {
+  if (q->lane == 1) --express_active;  // engine-thread state
   std::lock_guard<std::mutex> lk(mu);
   free_slots.push_back(q->slot);
   slots[(size_t)q->slot].req = nullptr;
@@ -872,7 +932,6 @@ void pnt_server_s::complete(Req* q) {
 
 void pnt_server_s::run_engine() {
   (void)hipSetDevice(device);
-  hipStream_t s = ctx->stream;
   SArgs A;
   std::vector<std::pair<Req*, int>> used;
   for (;;) {
@@ -887,8 +946,12 @@ void pnt_server_s::run_engine() {
       }
     }
     for (Req* q : fresh) {
+      // a long dependent chain goes to the express lane while it has room
+      q->lane = lanes[1].ctx && express_active < express_max && q->alone_waves >= express_waves;
+      if (q->lane == 1) ++express_active;
       const int st = guard_call([&] { setup(q); });
       if (st != PNT_OK) {
+        if (q->lane == 1) --express_active;
         const std::string e = pnt_last_error();
         std::lock_guard<std::mutex> lk(mu);
         free_slots.push_back(q->slot);
@@ -900,27 +963,39 @@ void pnt_server_s::run_engine() {
       }
       active.push_back(q);
     }
-    // one wave from the ready phases, oldest request first
-    A.reqs = reinterpret_cast<const SReqG*>(d_reqs);
-    A.nphases = 0;
-    int start = 0;
-    used.clear();
-    for (Req* q : active) {
-      if (A.nphases == kPhases) break;
-      if (q->stage == kIssued) continue;
-      collect(q, A, start, used);
-    }
-    if (A.nphases > 0) {
-      if (waves >= kAhead)  // at most kAhead waves queued ahead of the host
-        PNT_HIP_CHECK(hipEventSynchronize(wave_ev[(waves - kAhead) % (kAhead + 1)]));
+    // one wave per lane from its requests' ready phases, oldest request
+    // first; a lane with kAhead waves queued waits (blocking only when the
+    // express lane is idle, so that it is never held up behind the main one)
+    bool progressed = false;
+    for (int ln = 0; ln < 2; ++ln) {
+      Lane& L = lanes[ln];
+      if (!L.ctx) continue;
+      if (L.waves >= kAhead) {
+        hipEvent_t oldest = L.ev[(L.waves - kAhead) % (kAhead + 1)];
+        if (express_active == 0)
+          PNT_HIP_CHECK(hipEventSynchronize(oldest));
+        else if (hipEventQuery(oldest) != hipSuccess)
+          continue;
+      }
+      A.reqs = reinterpret_cast<const SReqG*>(d_reqs);
+      A.nphases = 0;
+      int start = 0;
+      used.clear();
+      for (Req* q : active) {
+        if (A.nphases == kPhases) break;
+        if (q->lane != ln || q->stage == kIssued) continue;
+        collect(q, A, start, used);
+      }
+      if (A.nphases == 0) continue;
+      hipStream_t s = L.ctx->stream;
       k_serve_wave<<<(unsigned)start, kT2, 0, s>>>(A);
       PNT_HIP_CHECK(hipGetLastError());
-      PNT_HIP_CHECK(hipEventRecord(wave_ev[waves % (kAhead + 1)], s));
-      ++waves;
+      PNT_HIP_CHECK(hipEventRecord(L.ev[L.waves % (kAhead + 1)], s));
+      ++L.waves;
       after_wave(used);
+      progressed = true;
     }
     // requests whose device work is done
-    bool progressed = A.nphases > 0;
     for (size_t i = 0; i < active.size();) {
       Req* q = active[i];
       if (q->stage == kIssued && hipEventQuery(slots[(size_t)q->slot].fin) == hipSuccess) {
@@ -931,9 +1006,18 @@ void pnt_server_s::run_engine() {
       }
       ++i;
     }
-    if (!progressed) {  // everything issued is still running: wait a little
-      std::unique_lock<std::mutex> lk(mu);
-      cv_engine.wait_for(lk, std::chrono::microseconds(50));
+    if (!progressed) {  // everything issued is still running
+      Lane& X = lanes[1];
+      if (express_active == 0) {  // wait a little
+        std::unique_lock<std::mutex> lk(mu);
+        cv_engine.wait_for(lk, std::chrono::microseconds(50));
+      } else if (X.waves >= kAhead) {
+        // the express lane's waves are a few microseconds each: wait for
+        // its oldest, not for a timer, so that its queue never runs dry
+        PNT_HIP_CHECK(hipEventSynchronize(X.ev[(X.waves - kAhead) % (kAhead + 1)]));
+      } else {
+        std::this_thread::yield();
+      }
     }
   }
 }
@@ -1034,24 +1118,36 @@ int pnt_server_create(int device, int n_slots, int64_t hbm_budget_bytes, pnt_ser
     if (const char* e = std::getenv("PAINTY_SERVE_PART_MB"))
       S->part_bytes = std::max<int64_t>(1, std::atoll(e)) << 20;
     try {
+      if (const char* e = std::getenv("PAINTY_SERVE_EXPRESS")) S->express_max = std::max(0, std::atoi(e));
+      if (const char* e = std::getenv("PAINTY_SERVE_EXPRESS_WAVES"))
+        S->express_waves = std::max<int64_t>(1, std::atoll(e));
       for (pnt_ctx* c : {&S->ctx, &S->qctx, &S->cctx}) {
         const int st = pnt_ctx_create(device, c);
         if (st != PNT_OK) throw Error(st, pnt_last_error());
       }
+      // the two lanes run side by side only on hardware queues of their own
+      S->lanes[0].ctx = S->ctx;
+      if (S->express_max > 0) {
+        pnt_ctx_destroy(S->ctx);
+        S->ctx = S->lanes[0].ctx = ctx_create_own_queue(device);
+        S->lanes[1].ctx = ctx_create_own_queue(device);
+      }
+      for (auto& L : S->lanes)
+        if (L.ctx)
+          for (hipEvent_t& e : L.ev) PNT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       S->slots.resize((size_t)n_slots);
       for (int i = n_slots - 1; i >= 0; --i) S->free_slots.push_back(i);
       for (SlotRes& r : S->slots)
         PNT_HIP_CHECK(hipEventCreateWithFlags(&r.fin, hipEventDisableTiming));
-      for (hipEvent_t& e : S->wave_ev)
-        PNT_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       PNT_HIP_CHECK(hipMalloc((void**)&S->d_reqs, (size_t)n_slots * sizeof(SReq)));
     } catch (...) {
       if (S->d_reqs) (void)hipFree(S->d_reqs);
       for (SlotRes& r : S->slots)
         if (r.fin) (void)hipEventDestroy(r.fin);
-      for (hipEvent_t e : S->wave_ev)
-        if (e) (void)hipEventDestroy(e);
-      for (pnt_ctx c : {S->ctx, S->qctx, S->cctx})
+      for (auto& L : S->lanes)
+        for (hipEvent_t e : L.ev)
+          if (e) (void)hipEventDestroy(e);
+      for (pnt_ctx c : {S->ctx, S->qctx, S->cctx, S->lanes[1].ctx})
         if (c) pnt_ctx_destroy(c);
       delete S;
       throw;
@@ -1185,6 +1281,7 @@ int pnt_server_destroy(pnt_server S) {
     S->copier.join();
     (void)hipSetDevice(S->device);
     (void)hipStreamSynchronize(S->ctx->stream);
+    if (S->lanes[1].ctx) (void)hipStreamSynchronize(S->lanes[1].ctx->stream);
     for (auto& kv : S->tickets) delete kv.second;
     for (auto& kv : S->canvases) {
       if (kv.second.view) pnt_canvas_destroy(kv.second.view);
@@ -1200,8 +1297,10 @@ int pnt_server_destroy(pnt_server S) {
       if (r.stage) (void)hipHostFree(r.stage);
       if (r.fin) (void)hipEventDestroy(r.fin);
     }
-    for (hipEvent_t e : S->wave_ev)
-      if (e) (void)hipEventDestroy(e);
+    for (auto& L : S->lanes)
+      for (hipEvent_t e : L.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (S->lanes[1].ctx) pnt_ctx_destroy(S->lanes[1].ctx);
     pnt_ctx_destroy(S->ctx);
     pnt_ctx_destroy(S->qctx);
     pnt_ctx_destroy(S->cctx);
