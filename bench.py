@@ -440,6 +440,34 @@ def leg_single(args, wl):
 
 
 # ------------------------------------------------------------------ batched --
+def batch_tail(H, jobs, lens):
+    """One more run of the batch as waves with a timing event after every
+    wave (pnt_batch_profile): when each job's last wave ends, and how much of
+    the batch runs in waves too small to fill the device."""
+    import torch
+    jf, jl, wb, we = H.batch_profile(jobs)
+    end = we[jl]
+    dt = np.diff(np.concatenate([[0.0], we]))
+    resident = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+    small = wb < resident
+    k = int(np.argmax(end))
+    out = {
+        "waves": int(len(wb)), "profiled_ms": round(float(we[-1]), 3),
+        "job_end_ms": {q: round(float(np.percentile(end, p)), 3)
+                       for q, p in (("p50", 50), ("p90", 90), ("p99", 99), ("max", 100))},
+        "after_p90_ms": round(float(we[-1] - np.percentile(end, 90)), 3),
+        "workgroups_per_wave": {"p50": int(np.median(wb)), "min": int(wb.min()),
+                                "max": int(wb.max())},
+        "waves_below_one_round": int(small.sum()), "ms_in_waves_below_one_round":
+            round(float(dt[small].sum()), 3), "one_round_workgroups": int(resident),
+        "last_job": {"stamps": lens[k][0], "rows": lens[k][1], "cols": lens[k][2],
+                     "footprint_px": lens[k][3], "waves": int(jl[k] - jf[k] + 1)},
+        "measured": "HIP event after every wave launch (pnt_batch_profile), one eager run",
+    }
+    log("bench: batch tail %s" % json.dumps(out))
+    return out
+
+
 def leg_batched(args):
     """Batched configuration: 256 canvases (sides U{256..2048}, 50..500 stamps,
     footprints 32..128 px, pickup on) stamped and composited in one launch
@@ -454,7 +482,7 @@ def leg_batched(args):
     ctx = H.default_context()
     stream = ctx.stream
     ev = HipEvents()
-    jobs, visits, ring, n_px, n_st = [], 0, 0, 0, 0
+    jobs, lens, visits, ring, n_px, n_st = [], [], 0, 0, 0, 0
     t0 = time.perf_counter()
     for wl in bw.jobs:
         cv = H.Canvas(wl.rows, wl.cols)
@@ -471,6 +499,7 @@ def leg_batched(args):
         n_px += wl.rows * wl.cols
         n_st += len(arr)
         jobs.append((cv, br, arr, out))
+        lens.append((len(arr), wl.rows, wl.cols, wl.cores[0].shape[0]))
         wl.wet = wl.dry = wl.R0 = None  # host copies no longer needed
     del bw
     log("bench: uploaded %d canvases, %.1f Mpx, %d stamps in %.1fs" %
@@ -518,6 +547,7 @@ def leg_batched(args):
         log("bench: graph replay %.2f ms/batch wall, %.2f ms/batch stream" % (gw, gg))
         if gw < wall:
             wall, gpu, path = gw, gg, "captured graph replay"
+    tail = batch_tail(H, jobs, lens) if H.batch_get_config()[2] else None
     plain = None
     if args.plain_steps:
         a2, b2 = ev.event(), ev.event()
@@ -543,6 +573,7 @@ def leg_batched(args):
         "tuned": tuned,
         "plain_sequential_ms": round(plain, 2) if plain else None,
         "speedup_vs_plain": round(plain / wall, 2) if plain else None,
+        "tail": tail,
         "roofline": {"bound": "hbm", "kernel": batch_kernel, "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(batch_kernel, "batch"),

@@ -345,6 +345,17 @@ int pnt_batch_get_config(pnt_ctx ctx, pnt_batch_config* cfg);
  * The jobs' state is unchanged on return. */
 int pnt_batch_tune(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs,
                    pnt_batch_config* best, double* best_ms);
+/* Diagnostic, no counterpart in the reference: runs the batch once as waves
+ * (the context's chunk), as pnt_batch_run would, with a timing event after
+ * every wave launch, and reports the schedule.  job_first[k] / job_last[k]
+ * (n_jobs entries each): the first and last wave holding a phase of job k
+ * (-1: none).  wave_blocks[w], wave_end_ms[w] (cap entries each): the
+ * workgroups of wave w and the time from the batch's start (after its image
+ * upload) to the end of wave w.  *n_waves: the wave count; waves beyond cap
+ * run but are not reported.  Any output pointer may be NULL. */
+int pnt_batch_profile(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int cap,
+                      int* job_first, int* job_last, int* wave_blocks, float* wave_end_ms,
+                      int* n_waves);
 /* The batch planned once and captured as a HIP graph (plan copy, counter
  * clear, k_batch -- or, in the waves configuration, the plan copy and every
  * wave); each launch replays it on the jobs' current state -- the

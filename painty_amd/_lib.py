@@ -144,6 +144,9 @@ SIGNATURES = {
     "pnt_batch_set_config": [_vp, C.POINTER(BatchConfig)],
     "pnt_batch_get_config": [_vp, C.POINTER(BatchConfig)],
     "pnt_batch_tune": [_vp, C.POINTER(BatchJob), C.c_int, C.POINTER(BatchConfig), _dp],
+    "pnt_batch_profile": [_vp, C.POINTER(BatchJob), C.c_int, C.c_int, C.POINTER(C.c_int),
+                          C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float),
+                          C.POINTER(C.c_int)],
     "pnt_batch_capture": [_vp, C.POINTER(BatchJob), C.c_int, C.POINTER(_vp)],
     "pnt_batch_graph_launch": [_vp],
     "pnt_batch_graph_destroy": [_vp],

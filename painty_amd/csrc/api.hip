@@ -1733,6 +1733,28 @@ int pnt_batch_tune(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, pnt_batch
   });
 }
 
+int pnt_batch_profile(pnt_ctx ctx, const pnt_batch_job* jobs, int n_jobs, int cap,
+                      int* job_first, int* job_last, int* wave_blocks, float* wave_end_ms,
+                      int* n_waves) {
+  return guard([&] {
+    validate_batch(ctx, jobs, n_jobs);
+    PNT_REQUIRE(n_jobs > 0, "empty batch");
+    PNT_REQUIRE(cap >= 0, "negative cap");
+    BatchWaveProfile P;
+    profile_batch_waves(ctx, jobs, n_jobs, &P);
+    const int nw = (int)P.wave_blocks.size();
+    for (int k = 0; k < n_jobs; ++k) {
+      if (job_first) job_first[k] = P.job_first[(size_t)k];
+      if (job_last) job_last[k] = P.job_last[(size_t)k];
+    }
+    for (int w = 0; w < std::min(nw, cap); ++w) {
+      if (wave_blocks) wave_blocks[w] = P.wave_blocks[(size_t)w];
+      if (wave_end_ms) wave_end_ms[w] = P.wave_end_ms[(size_t)w];
+    }
+    if (n_waves) *n_waves = nw;
+  });
+}
+
 int pnt_batch_set_config(pnt_ctx ctx, const pnt_batch_config* cfg) {
   return guard([&] {
     PNT_REQUIRE(ctx, "NULL context");

@@ -672,8 +672,10 @@ static void grow_stage(char*& p, int64_t& cap, int64_t need) {
   PNT_HIP_CHECK(hipHostMalloc((void**)&p, (size_t)cap));
 }
 
-// The batch as waves (batchwave.hip) on the context's stream.
-static void run_batch_waves(pnt_ctx ctx, const pnt_batch_job* jobs, int n, int chunk) {
+// The batch as waves (batchwave.hip) on the context's stream.  With `prof`,
+// a timing event after every wave, read back before returning.
+static void run_batch_waves(pnt_ctx ctx, const pnt_batch_job* jobs, int n, int chunk,
+                            BatchWaveProfile* prof = nullptr) {
   hipStream_t s = ctx->stream;
   BatchState& B = *batch_state(ctx);
   const int64_t need = plan_waves_pool(jobs, n, chunk);
@@ -689,10 +691,32 @@ static void run_batch_waves(pnt_ctx ctx, const pnt_batch_job* jobs, int n, int c
   grow_stage(B.stage, B.stage_cap, W.total);
   grow_dev(B.dev, B.dev_cap, W.total, s);
   std::memcpy(B.stage, W.blob.data(), W.blob.size());
-  enqueue_waves(W, B.stage, B.dev, s);
+  std::vector<hipEvent_t> ev;
+  if (prof) {
+    ev.assign(W.wave_blocks.size() + 1, nullptr);
+    for (hipEvent_t& e : ev) PNT_HIP_CHECK(hipEventCreate(&e));
+  }
+  enqueue_waves(W, B.stage, B.dev, s, prof ? ev.data() : nullptr);
   PNT_HIP_CHECK(hipEventRecord(B.uploaded, s));
   order_job_streams(ctx, jobs, n);
   finish_jobs(jobs, n);
+  if (prof) {
+    PNT_HIP_CHECK(hipEventSynchronize(ev.back()));
+    prof->wave_end_ms.assign(W.wave_blocks.size(), 0.0f);
+    for (size_t w = 0; w < W.wave_blocks.size(); ++w)
+      PNT_HIP_CHECK(hipEventElapsedTime(&prof->wave_end_ms[w], ev[0], ev[w + 1]));
+    for (hipEvent_t e : ev) PNT_HIP_CHECK(hipEventDestroy(e));
+    prof->wave_blocks = W.wave_blocks;
+    prof->job_first = W.job_first;
+    prof->job_last = W.job_last;
+  }
+}
+
+void profile_batch_waves(pnt_ctx ctx, const pnt_batch_job* jobs, int n, BatchWaveProfile* prof) {
+  if (*(volatile unsigned*)ctx->kernel_error)
+    throw Error(PNT_EKERNEL, "an earlier persistent stamp kernel aborted");
+  const pnt_batch_config C = batch_state(ctx)->cfg;
+  run_batch_waves(ctx, jobs, n, C.waves ? C.chunk : 4, prof);
 }
 
 void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,

@@ -286,6 +286,7 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
   std::vector<int64_t> ws, wg;
   std::vector<int> wb;
   std::vector<std::pair<int, int>> took;  // (job, unit or -1 dry / -2 copy / -3 compose)
+  std::vector<int> jfirst((size_t)n, -1), jlast((size_t)n, -1);
   for (;;) {
     const int64_t s0 = (int64_t)slots.size(), g0 = (int64_t)grp.size();
     int start = 0;
@@ -370,6 +371,8 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
     wb.push_back(start);
     for (const auto& t : took) {
       JobPlan& J = P[(size_t)t.first];
+      if (jfirst[(size_t)t.first] < 0) jfirst[(size_t)t.first] = (int)ws.size() - 1;
+      jlast[(size_t)t.first] = (int)ws.size() - 1;
       if (t.second >= 0) {
         const size_t i = (size_t)t.second;
         if (J.ph[i][J.next[i]].kind == kSStamp) J.sdone[i] = J.ph[i][J.next[i]].k - J.U[i].s0 + 1;
@@ -475,18 +478,23 @@ BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipS
     W.wave_grp.push_back(o_grp + wg[w] * (int64_t)sizeof(int));
   }
   W.wave_blocks = wb;
+  W.job_first = std::move(jfirst);
+  W.job_last = std::move(jlast);
   return W;
 }
 
 // Copies the image (pinned `stage` -> `dev`) and launches every wave on s.
-void enqueue_waves(const BWPlan& W, const char* stage, char* dev, hipStream_t s) {
+void enqueue_waves(const BWPlan& W, const char* stage, char* dev, hipStream_t s,
+                   const hipEvent_t* ev) {
   if (W.total) PNT_HIP_CHECK(hipMemcpyAsync(dev, stage, (size_t)W.total, hipMemcpyHostToDevice, s));
+  if (ev) PNT_HIP_CHECK(hipEventRecord(ev[0], s));
   const BWJobG* jobs = reinterpret_cast<const BWJobG*>(dev + W.o_jobs);
   for (size_t w = 0; w < W.wave_blocks.size(); ++w) {
     k_batch_wave<<<(unsigned)W.wave_blocks[w], kWT2, 0, s>>>(
         dev, jobs, reinterpret_cast<const BWSlot*>(dev + W.wave_slots[w]),
         reinterpret_cast<const int*>(dev + W.wave_grp[w]));
     PNT_HIP_CHECK(hipGetLastError());
+    if (ev) PNT_HIP_CHECK(hipEventRecord(ev[w + 1], s));
   }
 }
 

@@ -102,6 +102,12 @@ void persistent_release(PersistentState*& p);
 // batches (batch.hip).  cfg NULL: the context's configuration.
 void run_batch_one_launch(pnt_ctx ctx, const pnt_batch_job* jobs, int n,
                           const pnt_batch_config* cfg);
+// pnt_batch_profile: the batch run once as waves, timed wave by wave.
+struct BatchWaveProfile {
+  std::vector<float> wave_end_ms;  // from the image upload to the end of each wave
+  std::vector<int> wave_blocks, job_first, job_last;
+};
+void profile_batch_waves(pnt_ctx ctx, const pnt_batch_job* jobs, int n, BatchWaveProfile* prof);
 void tune_batch(pnt_ctx ctx, const pnt_batch_job* jobs, int n, pnt_batch_config* best,
                 double* best_ms);
 void set_batch_config(pnt_ctx ctx, const pnt_batch_config& c);
@@ -122,10 +128,14 @@ struct BWPlan {
   int64_t o_jobs = 0;
   std::vector<int64_t> wave_slots, wave_grp;
   std::vector<int> wave_blocks;
+  std::vector<int> job_first, job_last;  // each job's first and last wave (-1: none)
   int njobs = 0;
 };
 int64_t plan_waves_pool(const pnt_batch_job* jobs, int n, int chunk);
 BWPlan plan_waves(const pnt_batch_job* jobs, int n, int chunk, float* pool, hipStream_t s);
-void enqueue_waves(const BWPlan& W, const char* stage, char* dev, hipStream_t s);
+// ev (optional, wave_blocks.size() + 1 events): ev[0] is recorded after the
+// image copy, ev[w + 1] after wave w.
+void enqueue_waves(const BWPlan& W, const char* stage, char* dev, hipStream_t s,
+                   const hipEvent_t* ev = nullptr);
 
 }  // namespace pnt

@@ -677,6 +677,27 @@ def batch_tune(jobs, ctx: Context | None = None):
     return (c.tile, c.max_group, c.waves, c.chunk), ms.value
 
 
+def batch_profile(jobs, ctx: Context | None = None):
+    """pnt_batch_profile: runs the batch once as waves, timed wave by wave.
+    Returns (job_first, job_last, wave_blocks, wave_end_ms) as numpy arrays:
+    each job's first and last wave, and each wave's workgroups and end time
+    (ms from the batch's start)."""
+    ctx = ctx or default_context()
+    arr, _keep = _batch_jobs(jobs)
+    n = len(jobs)
+    # every wave holds at least one phase of some job: a stamp, a unit's
+    # map clear, a dry, the pickup copy or the composite
+    cap = 3 * sum(int(arr[k].n_stamps) for k in range(n)) + 8 * n + 64
+    jf, jl = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    wb, we = np.zeros(cap, np.int32), np.zeros(cap, np.float32)
+    nw = C.c_int()
+    ip = C.POINTER(C.c_int)
+    call("pnt_batch_profile", ctx.h, arr, n, cap, jf.ctypes.data_as(ip), jl.ctypes.data_as(ip),
+         wb.ctypes.data_as(ip), we.ctypes.data_as(C.POINTER(C.c_float)), C.byref(nw))
+    k = min(nw.value, cap)
+    return jf, jl, wb[:k], we[:k]
+
+
 class BatchGraph(Owned):
     """pnt_batch_capture: the batch captured once as a HIP graph;
     ``launch()`` replays it on the jobs' current state."""

@@ -212,6 +212,33 @@ def test_batch_graph_replays_equal_eager_and_oracle(hip, oracle, restore_config,
         assert_close("graph/job%d.out" % k, q[3].getR0(), ocv.compose())
 
 
+def test_batch_profile_runs_the_batch_and_reports_its_waves(hip, restore_config):
+    """pnt_batch_profile runs the batch as pnt_batch_run does in the waves
+    configuration (the same bits) and reports a schedule that covers it:
+    every job's first wave before its last, the last wave belongs to some
+    job, wave end times rise, and the two long jobs end after the small
+    ones."""
+    hip.batch_set_config(16, 1, 1, 1)
+    jobs = _big_small(90)
+    ref, prof = [], []
+    for wl in jobs:
+        cv, br, out = _setup(hip, wl)
+        ref.append((cv, br, wl.stamps, out))
+        cv, br, out = _setup(hip, wl)
+        prof.append((cv, br, wl.stamps, out))
+    hip.batch_run(ref, mode=0)
+    jf, jl, wb, we = hip.batch_profile(prof)
+    for k, (e, q) in enumerate(zip(ref, prof)):
+        for a, b in zip(_layers(e[0]) + [e[3].getR0()] + list(e[1].getPickupMap()),
+                        _layers(q[0]) + [q[3].getR0()] + list(q[1].getPickupMap())):
+            assert np.array_equal(a, b), "profile/job%d" % k
+    assert len(wb) > 0 and len(we) == len(wb)
+    assert (jf >= 0).all() and (jf <= jl).all()
+    assert jl.max() == len(wb) - 1
+    assert (wb > 0).all() and we[0] > 0 and np.all(np.diff(we) >= 0)
+    assert jl[2:].max() < jl[:2].min()
+
+
 def test_batch_graph_refuses_stale_state(hip):
     wl = W.small(50, 60, 12, 5, 81)
     cv, br, out = _setup(hip, wl)
