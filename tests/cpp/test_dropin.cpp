@@ -40,6 +40,8 @@ int km(const char* in, const char* out) {
   std::ofstream o(out);
   int n = 0;
   f >> n;
+  std::vector<vec3> Ks, Ss, R0s, Rs;
+  std::vector<double> ds;
   for (int i = 0; i < n; ++i) {
     vec3 K, S, R0;
     double d;
@@ -47,7 +49,20 @@ int km(const char* in, const char* out) {
     const vec3 R = painty::ComputeReflectance(K, S, R0, d);
     o.precision(9);
     o << R[0] << " " << R[1] << " " << R[2] << "\n";
+    Ks.push_back(K);
+    Ss.push_back(S);
+    R0s.push_back(R0);
+    ds.push_back(d);
+    Rs.push_back(R);
   }
+  // the many-pixel form: one call, the same bits (NaN where the single form is NaN)
+  const std::vector<vec3> Rb = painty::ComputeReflectance(Ks, Ss, R0s, ds);
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c)
+      if (std::memcmp(&Rb[(size_t)i][c], &Rs[(size_t)i][c], sizeof(double)) != 0) {
+        std::fprintf(stderr, "km: pixel %d channel %d: many-pixel form differs\n", i, c);
+        return 4;
+      }
   return 0;
 }
 
