@@ -187,3 +187,47 @@ def test_release_ends_a_canvas(hip):
     for name, x, y in (("K", K, aK), ("S", S, aS), ("V", V, aV), ("R0", R0, aR0)):
         assert np.array_equal(x, y), name
     srv.close()
+
+
+def test_cancel_drops_a_queued_request(hip):
+    """pnt_server_cancel: a request queued behind a long one on the same
+    canvas is dropped -- its wait raises Cancelled, and the canvas equals the
+    other requests run alone, bit for bit.  A request that has run, or is
+    running, is not cancelled; a ticket already waited on is unknown."""
+    core = W.gaussian_core(33, 8.0)
+    bg = (0.8, 0.85, 0.9)
+    lists = [W.strokes(256, 256, 600, 81, stamps_per_stroke=(40, 80), reach=0.4),
+             W.strokes(256, 256, 20, 82, stamps_per_stroke=(4, 8), reach=0.3),
+             W.strokes(256, 256, 20, 83, stamps_per_stroke=(4, 8), reach=0.3)]
+    reqs = [dict(canvas_id=1, rows=256, cols=256, background=bg, stamps=s, radius=16.0,
+                 footprint=core, use_snapshot=True) for s in lists]
+    srv = SV.Server(n_slots=2, hbm_budget_bytes=1 << 30)
+    ta, tb, tc = (srv.submit(**q) for q in reqs)
+    srv.cancel(tb)  # queued: requests on one canvas run one at a time, in order
+    srv.wait(ta)
+    with pytest.raises(_lib.Cancelled):
+        srv.wait(tb)
+    srv.wait(tc)
+    K, S, V, R0, R = srv.canvas_state(1)
+    aK, aS, aV, aR0, aR = run_alone({1: (256, 256, np.float32(bg))}, [reqs[0], reqs[2]])[1]
+    for name, x, y in (("K", K, aK), ("S", S, aS), ("V", V, aV), ("R0", R0, aR0)):
+        assert np.array_equal(x, y), name
+    assert np.array_equal(R.transpose(1, 2, 0), aR)
+    with pytest.raises(_lib.InvalidArgument):
+        srv.cancel(ta)  # waited on: the ticket is gone
+    # a request that may already be running: either it is cancelled, and its
+    # wait says so, or the cancel is refused and it completes
+    td = srv.submit(**reqs[0])
+    srv.wait(srv.submit(2, 64, 64, lists[1][:5], 4.0, W.gaussian_core(9, 2.25)))
+    try:
+        srv.cancel(td)
+        cancelled = True
+    except _lib.PaintyError as e:
+        assert e.code == _lib.ESTATE
+        cancelled = False
+    if cancelled:
+        with pytest.raises(_lib.Cancelled):
+            srv.wait(td)
+    else:
+        srv.wait(td)
+    srv.close()
