@@ -101,11 +101,11 @@ __global__ __launch_bounds__(kT* kT) void k_stamp_fused(FArgs A, int j, int ring
 
 // 4-pixel groups of box b, and the blocks that scan them (each thread
 // takes kRingGroups groups per pass).
-int ring_blocks_for(const RingBox& b, int threads) {
+int ring_blocks_for(const RingBox& b, int threads, int groups) {
   if (b.a_x1 < b.a_x0 || b.a_y1 < b.a_y0) return 0;
   const int64_t span = b.a_x1 - b.a_x0 + 1;
   const int64_t ngroups = ((span + 3) / 4 + 1) * (int64_t)(b.a_y1 - b.a_y0 + 1);
-  const int64_t blocks = (ngroups + kRingGroups * threads - 1) / (kRingGroups * threads);
+  const int64_t blocks = (ngroups + groups * threads - 1) / (groups * threads);
   return (int)std::min<int64_t>(std::max<int64_t>(blocks, 1), 1024);
 }
 

@@ -67,7 +67,9 @@ struct StampNeeds {
 };
 StampNeeds plan_stamp_needs(const std::vector<Unit>& U, const PStamp* h);
 // Blocks of `threads` threads that scan the snapshot ring of box b.
-int ring_blocks_for(const RingBox& b, int threads);
+// groups: 4-pixel groups per ring-scan thread (the ring_scan<groups> the
+// launch runs).
+int ring_blocks_for(const RingBox& b, int threads, int groups = kRingGroups);
 // Plans stamps [0, n) for a brush of footprint side size_map and radius on a
 // rows x cols canvas (FP64 geometry, ring boxes, shared-pixel levels).
 void plan_stamps_rc(int size_map, double radius, int rows, int cols, const pnt_stamp* st,

@@ -119,6 +119,7 @@ __device__ __forceinline__ void wave_compose_pixel(const Layer7& L, const float*
 }
 
 // Block b (of S.nblk) of phase S; all kWT2 threads of the block call it.
+template <int kG = kRingGroups>
 __device__ __forceinline__ void run_wave_slot(const SSlot& S, int b, int (*sm)[kWT + 2]) {
   const int tid = b * kWT2 + (int)threadIdx.x, nth = S.nblk * kWT2;
   const int64_t cs = S.cstride;
@@ -204,7 +205,7 @@ __device__ __forceinline__ void run_wave_slot(const SSlot& S, int b, int (*sm)[k
     window_tile<kWT>(T, pick, S.fp, fb, g, st, dup, pull, tile_of_block(b, S.ntiles), S.tiles_x,
                      sm, S.pk, S.ps, ck);
   } else if (b < win + S.rb) {
-    ring_scan<kRingGroups>(canvas, snap, S.dirty, S.fp, g, st.box, pull, (b - win) * kWT2 + (int)threadIdx.x,
+    ring_scan<kG>(canvas, snap, S.dirty, S.fp, g, st.box, pull, (b - win) * kWT2 + (int)threadIdx.x,
                  S.rb * kWT2, S.tmap != nullptr);
   } else if (dup) {  // the shared-pixel visits, beside the tiles
     level_visits<kWT2>(T, pick, S.fp, g, st, S.dl, S.dl_off, S.pk, S.ps, ck, pull);
