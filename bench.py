@@ -17,7 +17,9 @@ prints them in ONE JSON line:
   substrate, then wet, 88 B/px), with rooflines.
 * batched: 256 canvases (sides 256..2048, 50..500 stamps, footprints
   32..128 px) stamped and composited in one launch vs the same jobs as plain
-  per-canvas launches.
+  per-canvas launches; `tail`: one more run timed wave by wave
+  (pnt_batch_profile), when the jobs end and how much of the batch runs in
+  waves too small to fill the device.
 * served: Poisson arrivals of stroke + composite requests (canvases
   512..4096, 10..1000 stamps, 30% follow-ups), p50/p99 latency and req/s per
   offered rate, against the plain per-request path; and the same on 2048x2048
