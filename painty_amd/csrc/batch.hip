@@ -691,7 +691,14 @@ static void run_batch_waves(pnt_ctx ctx, const pnt_batch_job* jobs, int n, int c
   grow_stage(B.stage, B.stage_cap, W.total);
   grow_dev(B.dev, B.dev_cap, W.total, s);
   std::memcpy(B.stage, W.blob.data(), W.blob.size());
-  std::vector<hipEvent_t> ev;
+  struct Events {  // destroyed on every way out, a failed call's included
+    std::vector<hipEvent_t> e;
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } events;
+  std::vector<hipEvent_t>& ev = events.e;
   if (prof) {
     ev.assign(W.wave_blocks.size() + 1, nullptr);
     for (hipEvent_t& e : ev) PNT_HIP_CHECK(hipEventCreate(&e));
@@ -705,7 +712,6 @@ static void run_batch_waves(pnt_ctx ctx, const pnt_batch_job* jobs, int n, int c
     prof->wave_end_ms.assign(W.wave_blocks.size(), 0.0f);
     for (size_t w = 0; w < W.wave_blocks.size(); ++w)
       PNT_HIP_CHECK(hipEventElapsedTime(&prof->wave_end_ms[w], ev[0], ev[w + 1]));
-    for (hipEvent_t e : ev) PNT_HIP_CHECK(hipEventDestroy(e));
     prof->wave_blocks = W.wave_blocks;
     prof->job_first = W.job_first;
     prof->job_last = W.job_last;
