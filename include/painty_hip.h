@@ -45,7 +45,6 @@ extern "C" {
 #define PNT_ENOMEM (-3)  /* device allocation failed / over the memory budget */
 #define PNT_ESTATE (-4)  /* call not valid in the object's current state */
 #define PNT_EKERNEL (-5) /* a kernel reported an error (bounded wait expired) */
-#define PNT_ECANCELED (-6) /* the request was cancelled before it ran (pnt_server_cancel) */
 
 #define PNT_AOS 0
 #define PNT_PLANAR 1
@@ -543,13 +542,6 @@ int pnt_server_submit(pnt_server server, const pnt_request* request,
 /* Blocks until the ticket's request has run; its latency (submit to done,
  * ms) and its status. */
 int pnt_server_wait(pnt_server server, int64_t ticket, double* latency_ms);
-/* Cancels a request that has not started: one still queued or being
- * planned, not yet admitted to a slot.  It then runs nothing, and
- * pnt_server_wait on its ticket returns PNT_ECANCELED; later requests on its
- * canvas run as if it had never been submitted.  PNT_ESTATE when the request
- * is already running or done (it then completes as usual), PNT_EINVAL for a
- * ticket that is unknown or already waited on. */
-int pnt_server_cancel(pnt_server server, int64_t ticket);
 /* A resident canvas (no request running on it) and its composite planes. */
 int pnt_server_canvas(pnt_server server, int64_t canvas_id, pnt_canvas* canvas,
                       pnt_dev_rgb* composite);

@@ -12,7 +12,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpainty_hip.so")
 
-OK, EINVAL, EHIP, ENOMEM, ESTATE, EKERNEL, ECANCELED = 0, -1, -2, -3, -4, -5, -6
+OK, EINVAL, EHIP, ENOMEM, ESTATE, EKERNEL = 0, -1, -2, -3, -4, -5
 AOS, PLANAR = 0, 1
 STAMP_DIP, STAMP_DRY_AFTER = 1, 2
 EXEC_PLAIN, EXEC_PERSISTENT, EXEC_FUSED, EXEC_CHAINED = 0, 1, 2, 3
@@ -28,10 +28,6 @@ class PaintyError(RuntimeError):
 
 class InvalidArgument(PaintyError, ValueError):
     """Where the reference throws std::invalid_argument."""
-
-
-class Cancelled(PaintyError):
-    """A server request cancelled before it ran (pnt_server_cancel)."""
 
 
 class PntStamp(C.Structure):
@@ -207,7 +203,6 @@ SIGNATURES = {
     "pnt_server_create": [C.c_int, C.c_int, C.c_int64, C.POINTER(_vp)],
     "pnt_server_submit": [_vp, C.POINTER(PntRequest), C.POINTER(C.c_int64)],
     "pnt_server_wait": [_vp, C.c_int64, C.POINTER(C.c_double)],
-    "pnt_server_cancel": [_vp, C.c_int64],
     "pnt_server_canvas": [_vp, C.c_int64, C.POINTER(_vp), C.POINTER(DevRgb)],
     "pnt_server_release": [_vp, C.c_int64],
     "pnt_server_stats": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -242,8 +237,6 @@ def check(status: int) -> None:
         msg = load().pnt_last_error().decode(errors="replace")
         if status == EINVAL:
             raise InvalidArgument(status, msg)
-        if status == ECANCELED:
-            raise Cancelled(status, msg)
         raise PaintyError(status, msg)
 
 

@@ -237,7 +237,6 @@ struct Req {
   int status = PNT_OK;
   std::string error;
   bool done = false;
-  bool cancelled = false;  // pnt_server_cancel before admission: finished unrun
   // plan (planner thread)
   bool planned = false;
   int size_map = 0, hr = 0, tiles_x = 0, ntiles = 0;
@@ -574,14 +573,6 @@ std::vector<Req*> pnt_server_s::admit() {
     seen.push_back(id);
     if (!q->planned) {
       ++it;
-      continue;
-    }
-    if (q->cancelled) {  // pnt_server_cancel: nothing of it runs
-      it = queue.erase(it);
-      if (--pending[id] == 0) pending.erase(id);
-      finish(q, PNT_ECANCELED, "the request was cancelled");
-      release_if_closing(id);
-      seen.pop_back();  // the canvas' next request may be admitted now
       continue;
     }
     if (q->status != PNT_OK) {  // planning failed
@@ -1223,28 +1214,6 @@ int pnt_server_wait(pnt_server S, int64_t ticket, double* latency_ms) {
     const std::string msg = q->error;
     delete q;
     if (st != PNT_OK) throw Error(st, msg);
-  });
-}
-
-int pnt_server_cancel(pnt_server S, int64_t ticket) {
-  return guard_call([&] {
-    PNT_REQUIRE(S, "NULL server");
-    std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->tickets.find(ticket);
-    PNT_REQUIRE(it != S->tickets.end(), "unknown ticket");
-    Req* q = it->second;
-    if (q->done) throw Error(PNT_ESTATE, "the request has already run");
-    if (std::find(S->queue.begin(), S->queue.end(), q) == S->queue.end())
-      throw Error(PNT_ESTATE, "the request is already running");
-    q->cancelled = true;
-    // not picked up by a planner yet: no plan is needed; one being planned
-    // is finished by the engine once its planner lets go of it
-    auto p = std::find(S->planq.begin(), S->planq.end(), q);
-    if (p != S->planq.end()) {
-      S->planq.erase(p);
-      q->planned = true;
-    }
-    S->cv_engine.notify_all();
   });
 }
 

@@ -60,12 +60,6 @@ class Server(_lib.Owned):
             self._keep.pop(ticket, None)
         return ms.value
 
-    def cancel(self, ticket: int) -> None:
-        """pnt_server_cancel: drops a request that has not started; its wait
-        then raises Cancelled.  Raises PaintyError (PNT_ESTATE) when the
-        request is already running or done."""
-        _lib.call("pnt_server_cancel", self.h, int(ticket))
-
     def canvas_state(self, canvas_id: int):
         """Wet layer K, S, V, R0 and the last composite of a resident canvas
         (host arrays, reference layouts)."""
